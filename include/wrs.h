@@ -1,0 +1,186 @@
+/*
+ * wrs.h -- C ABI of the MI355X (gfx950) weighted-random-sampling library.
+ *
+ * Construct-then-sample, as the reference kistenklaus/merian-wrs exposes it:
+ * weights in -> PSA alias table (or ITS CDF) out -> samples out.  Every entry
+ * point is asynchronous on the given HIP stream, takes plain device pointers
+ * owned by the caller, and returns a wrs_status; nothing throws across the ABI.
+ * A plan owns the device workspace of the builds (look-back records, partition
+ * indices and prefixes, per-build scalars).  One plan must not be used by two
+ * streams at the same time; distinct plans are independent.
+ *
+ * Each entry names the reference interface it replaces (file:line in the
+ * reference repository).
+ */
+#ifndef WRS_H
+#define WRS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* wrs_stream_t; /* == hipStream_t; NULL = default stream */
+
+typedef enum {
+    WRS_OK = 0,
+    WRS_EINVAL = 1,  /* bad argument (null pointer, n == 0, n > plan max, misaligned) */
+    WRS_ENOMEM = 2,  /* device allocation failed */
+    WRS_EHIP = 3,    /* a HIP runtime call or launch failed; see wrs_last_error() */
+    WRS_ENOTSUP = 4, /* no gfx950 device / unsupported configuration */
+} wrs_status;
+
+/* Arithmetic of a build.  The reference GPU kernels are plain FP32 throughout
+ * (src/device/prefix_partition/decoupled/shader.comp:96-193). */
+typedef enum {
+    WRS_PREC_FP32 = 0,        /* FP32 mean, scan, carry, split and pack (reference arithmetic) */
+    WRS_PREC_FP64_CARRY = 1,  /* FP64 total/mean, exact look-back carry, FP64 split/pack, FP32 tile scan */
+    WRS_PREC_COMPENSATED = 2, /* as FP64_CARRY with a compensated (hi+lo) FP32 tile scan: default */
+} wrs_precision;
+
+/* Alias table row, 8 bytes: keep bucket i with probability p, else take a.
+ * Reference: AliasTableEntry{float p; uint a} src/host/types/alias_table.hpp:12-44,
+ * std430 layout src/device/wrs/alias/psa/layout/alias_table.hpp:13-19. */
+typedef struct {
+    float p;
+    uint32_t a;
+} wrs_alias_entry;
+
+/* Split point: first i lights and j heavies, spill = residual of heavy j.
+ * Reference Split{uint i; uint j; float spill} src/host/types/split.hpp:9-40;
+ * the spill is kept in FP64 here. */
+typedef struct {
+    uint32_t i;
+    uint32_t j;
+    double spill;
+} wrs_split;
+
+/* Build configuration.  Reference: PSAConfig (src/device/wrs/alias/psa/PSA.hpp:19-41),
+ * whose splitSize sets the split granularity. */
+typedef struct {
+    uint32_t split_size;     /* buckets per split interval (K = ceil(N/split_size)); 0 -> 32 */
+    wrs_precision precision; /* default WRS_PREC_COMPENSATED */
+} wrs_config;
+
+typedef struct wrs_plan wrs_plan;
+
+/* Creates a plan able to build distributions of up to max_n weights
+ * (max_n < 2^32).  Allocates 12*max_n + O(max_n/4096) bytes of device memory.
+ * Replaces the construction of device::PSA / device::WRS and their
+ * Buffers::allocate (src/device/wrs/alias/psa/PSA.hpp:43-147,
+ * src/device/wrs/WRS.hpp:60-88). */
+wrs_status wrs_plan_create(const wrs_config* cfg, uint64_t max_n, wrs_plan** out);
+wrs_status wrs_plan_destroy(wrs_plan* plan);
+
+/* PSA alias-table build: weights[n] (finite, >= 0, not all zero) -> table[n].
+ * Three passes: FP64 total, heavy/light prefix-partition, fused split+pack.
+ * Replaces device::PSA::run (src/device/wrs/alias/psa/PSA.hpp:170-237) and
+ * device::WRS::build for an AliasTable config (src/device/wrs/WRS.hpp:121-143). */
+wrs_status wrs_psa_build(wrs_plan* plan, const float* weights, uint64_t n,
+                         wrs_alias_entry* table, wrs_stream_t stream);
+
+/* Device pointers for the intermediate results of a build (any may be NULL). */
+typedef struct {
+    double* mean;          /* [1]  mean (pivot and bucket size) */
+    uint32_t* heavy_count; /* [1]  H */
+    uint32_t* indices;     /* [n]  partition indices, heavies at [0,H), light r at n-1-r */
+    float* prefix_hi;      /* [n]  partition prefixes, same layout */
+    float* prefix_lo;      /* [n]  compensation words (WRS_PREC_COMPENSATED only) */
+    wrs_split* splits;     /* [K]  splits[k-1] = S(k), k = 1..K, splits[K-1] = (L, H-1, 0) */
+} wrs_psa_debug;
+
+/* wrs_psa_build that also copies out the intermediates (for parity tests). */
+wrs_status wrs_psa_build_debug(wrs_plan* plan, const float* weights, uint64_t n,
+                               wrs_alias_entry* table, const wrs_psa_debug* dbg,
+                               wrs_stream_t stream);
+
+/* FP64 total and mean of weights[n] (deterministic fixed-order reduction).
+ * Replaces Mean<float>::run / AtomicMean (src/device/mean/Mean.hpp:118-140,
+ * src/device/mean/atomic/shader.comp:69-82). out2 = {total, mean}. */
+wrs_status wrs_mean(wrs_plan* plan, const float* weights, uint64_t n, double* out2,
+                    wrs_stream_t stream);
+
+/* Heavy/light prefix-partition around `pivot` (heavy <=> w > pivot; pass NAN
+ * to use the mean).  indices/prefix_hi[n] in the reference layout (heavies
+ * forward, lights reversed from the tail), prefix_lo[n] optional, heavy_count[1].
+ * Replaces PrefixPartition<float>::run (src/device/prefix_partition/
+ * PrefixPartition.hpp:38-206; kernel decoupled/shader.comp:313-362). */
+wrs_status wrs_prefix_partition(wrs_plan* plan, const float* weights, uint64_t n, double pivot,
+                                uint32_t* indices, float* prefix_hi, float* prefix_lo,
+                                uint32_t* heavy_count, wrs_stream_t stream);
+
+/* Inclusive prefix sum out[i] = in[0] + ... + in[i] (hi word, optional lo word).
+ * Replaces PrefixSum<float>::run / DecoupledPrefixSum (src/device/prefix_sum/
+ * decoupled/DecoupledPrefixSum.hpp:174-197, shader.comp:321-333). */
+wrs_status wrs_scan_inclusive(wrs_plan* plan, const float* in, uint64_t n, float* out_hi,
+                              float* out_lo, wrs_stream_t stream);
+
+/* ITS build: normalised CDF cdf[i] = (w[0]+..+w[i]) / sum(w) (hi, optional lo).
+ * Replaces ITS::build (src/device/wrs/its/ITS.hpp:90-108), which scans into an
+ * un-normalised CMF. */
+wrs_status wrs_its_build(wrs_plan* plan, const float* weights, uint64_t n, float* cdf_hi,
+                         float* cdf_lo, wrs_stream_t stream);
+
+/* Alias sampling: out[t] for t < s, Philox4x32-10 with key = seed and
+ * counter = counter_base + t.  Replaces SampleAliasTable::run
+ * (src/device/wrs/alias/sampling/AliasTableSampling.hpp:75-85) and
+ * WRS::sample (src/device/wrs/WRS.hpp:145-168). */
+wrs_status wrs_alias_sample(const wrs_alias_entry* table, uint64_t n, uint64_t s, uint64_t seed,
+                            uint64_t counter_base, uint32_t* out, wrs_stream_t stream);
+
+/* ITS sampling by lower-bound binary search over the CDF (cdf_lo optional).
+ * Replaces InverseTransformSampling::run (src/device/wrs/its/sampling/
+ * InverseTransformSampling.hpp:123-138). */
+wrs_status wrs_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n, uint64_t s,
+                          uint64_t seed, uint64_t counter_base, uint32_t* out,
+                          wrs_stream_t stream);
+
+/* Philox4x32-10 blocks: out[4t..4t+3] = philox(ctr = (lo(base+t), hi(base+t),
+ * stream_word, 0), key = seed).  Replaces the PRNG module (src/device/prng/
+ * philox/Philox.hpp:47-104), which is a non-standard 2x32 7-round variant. */
+wrs_status wrs_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uint64_t n,
+                      uint32_t* out, wrs_stream_t stream);
+
+/* Seeded synthetic weights on the device (the reference generates benchmark
+ * weights with its PRNG kernel, src/bench/psa_split2.cpp:148-228). */
+typedef enum {
+    WRS_GEN_UNIFORM01 = 0, /* U(0,1), 24-bit */
+    WRS_GEN_LOGNORMAL = 1, /* exp(a + b z), z ~ N(0,1) by Box-Muller */
+    WRS_GEN_ZIPF = 2,      /* rank^-a, ranks a seeded permutation of 1..n */
+    WRS_GEN_SPIKE = 3,     /* 1% of positions U(1e5,1e6), the rest U(1e-3,1) */
+} wrs_generator;
+wrs_status wrs_generate(wrs_generator kind, double a, double b, uint64_t seed, uint64_t n,
+                        float* out, wrs_stream_t stream);
+
+/* ---- batched (segmented) builds of many independent distributions ------
+ * Segment b is weights[offsets[b] .. offsets[b+1]); its table occupies the
+ * same range of `tables` with segment-local indices.  offsets_host and
+ * offsets_dev hold the same nseg+1 values.  No reference counterpart: the
+ * reference builds one distribution per dispatch sequence. */
+typedef struct wrs_batch_plan wrs_batch_plan;
+wrs_status wrs_batch_plan_create(const wrs_config* cfg, const uint64_t* offsets_host,
+                                 uint32_t nseg, wrs_batch_plan** out);
+wrs_status wrs_batch_plan_destroy(wrs_batch_plan* plan);
+wrs_status wrs_psa_build_batched(wrs_batch_plan* plan, const float* weights,
+                                 const uint64_t* offsets_dev, wrs_alias_entry* tables,
+                                 wrs_stream_t stream);
+/* per_segment samples of each segment: out[b*per_segment + t] is a segment-local index,
+ * Philox counter = counter_base + b*per_segment + t. */
+wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_t* offsets_dev,
+                                    uint32_t nseg, uint32_t per_segment, uint64_t seed,
+                                    uint64_t counter_base, uint32_t* out, wrs_stream_t stream);
+
+/* ---- introspection ---- */
+const char* wrs_last_error(void);         /* thread-local message of the last failure */
+const char* wrs_version(void);
+/* Per-phase device times (ms) of the most recent build on this plan when
+ * timing is enabled: [0] reduce, [1] finalize, [2] partition, [3] split+pack. */
+wrs_status wrs_plan_set_timing(wrs_plan* plan, int enable);
+wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WRS_H */

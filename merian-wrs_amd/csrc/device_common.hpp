@@ -1,0 +1,220 @@
+// device_common.hpp -- building blocks shared by the CDNA4 (gfx950) kernels.
+//
+//  * Philox4x32-10 (Random123, Salmon et al. SC'11), counter-based, so every
+//    sample / element draws its own counter and the stream is independent of
+//    the launch shape.
+//  * error-free transformation two_sum and the double-float ("compensated
+//    FP32") combine used by the in-tile scans.
+//  * a 126-bit fixed-point accumulator (two 63-bit limbs) for the inter-tile
+//    look-back carry: integer addition is associative, so the carry bits do
+//    not depend on the order in which the look-back happened to find its
+//    predecessors' aggregates.
+//  * self-validating 64-bit look-back words: every published word starts as
+//    the sentinel ~0 and is written exactly once per launch with a value whose
+//    top bit is clear, so a reader that sees all words of a record non-sentinel
+//    has the final record without any release/acquire fence.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wrsdev {
+
+constexpr int kWave = 64;
+constexpr uint64_t kSentinel = ~0ull;
+constexpr uint64_t kMask63 = (1ull << 63) - 1ull;
+
+// ------------------------------------------------------------- Philox ----
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                            uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// counter = (lo(t), hi(t), c2, c3), key = (lo(seed), hi(seed))
+__device__ __forceinline__ U4 philox_at(uint64_t seed, uint64_t t, uint32_t c2, uint32_t c3) {
+    return philox4x32_10((uint32_t)t, (uint32_t)(t >> 32), c2, c3, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+}
+
+constexpr uint32_t kTagGen = 0x47454E31u;  // weight generators
+constexpr uint32_t kTagFeistel = 0x46535431u;
+
+__device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * 0x1.0p-24f; }
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+    return (double)((((uint64_t)hi << 32) | lo) >> 11) * 0x1.0p-53;
+}
+__device__ __forceinline__ double u53_open0(uint32_t hi, uint32_t lo) {
+    return (double)(((((uint64_t)hi << 32) | lo) >> 11) + 1ull) * 0x1.0p-53;
+}
+
+// ------------------------------------------------- compensated FP32 ----
+// two_sum: s + e == a + b exactly (Knuth).  Built with -ffp-contract and no
+// reassociation, so the operation order below is what executes.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+struct DF {  // unevaluated sum hi + lo
+    float hi, lo;
+};
+
+__device__ __forceinline__ DF df_add(DF a, DF b) {
+    float s, e;
+    two_sum(a.hi, b.hi, s, e);
+    return DF{s, (a.lo + b.lo) + e};
+}
+__device__ __forceinline__ DF df_add1(DF a, float x) {
+    float s, e;
+    two_sum(a.hi, x, s, e);
+    return DF{s, a.lo + e};
+}
+__device__ __forceinline__ double df_value(DF a) { return (double)a.hi + (double)a.lo; }
+
+// ---------------------------------------------------- 126-bit fixed point ----
+struct FX {  // value = hi * 2^63 + lo, lo < 2^63, hi < 2^63
+    uint64_t lo, hi;
+};
+__device__ __forceinline__ FX fx_zero() { return FX{0, 0}; }
+__device__ __forceinline__ FX fx_add(FX a, FX b) {
+    const uint64_t l = a.lo + b.lo;
+    return FX{l & kMask63, a.hi + b.hi + (l >> 63)};
+}
+// floor(d * 2^shift) for finite d >= 0 whose scaled value is < 2^126.
+__device__ __forceinline__ FX fx_from_double(double d, int shift) {
+    if (!(d > 0.0)) return fx_zero();
+    const uint64_t bits = (uint64_t)__double_as_longlong(d);
+    const int e = (int)((bits >> 52) & 0x7ff);
+    uint64_t m = bits & ((1ull << 52) - 1ull);
+    int ex;
+    if (e == 0) {
+        ex = -1074;
+    } else {
+        m |= (1ull << 52);
+        ex = e - 1075;
+    }
+    const int sh = ex + shift;
+    if (sh >= 0) {
+        const unsigned __int128 v = (unsigned __int128)m << sh;
+        return FX{(uint64_t)v & kMask63, (uint64_t)(v >> 63)};
+    }
+    if (sh <= -64) return fx_zero();
+    return FX{(m >> (-sh)) & kMask63, 0};
+}
+__device__ __forceinline__ double fx_to_double(FX v, int shift) {
+    const double r = (double)v.hi * 0x1.0p63 + (double)v.lo;
+    return ldexp(r, -shift);
+}
+
+// ------------------------------------------------------ agent-scope words ----
+__device__ __forceinline__ void st_word(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_word(const uint64_t* p) {
+    return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------ wave ops ----
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+template <class T>
+__device__ __forceinline__ T shfl_up(T v, int d) {
+    return __shfl_up(v, (unsigned)d, kWave);
+}
+template <class T>
+__device__ __forceinline__ T shfl(T v, int src) {
+    return __shfl(v, src, kWave);
+}
+
+// Inclusive wave scan, Hillis-Steele over 64 lanes: lane l receives
+// v_0 (+) ... (+) v_l with the earlier lanes always the left operand, so the
+// result is a fixed function of the lane values.
+__device__ __forceinline__ float wave_incl_f(float v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const float n = shfl_up(v, d);
+        if (l >= d) v = n + v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_u(uint32_t v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t n = shfl_up(v, d);
+        if (l >= d) v = n + v;
+    }
+    return v;
+}
+__device__ __forceinline__ DF wave_incl_df(DF v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        DF n;
+        n.hi = shfl_up(v.hi, d);
+        n.lo = shfl_up(v.lo, d);
+        if (l >= d) v = df_add(n, v);
+    }
+    return v;
+}
+
+// Fixed-order wave sum (butterfly) of a double: identical bits on all lanes.
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) v = v + __shfl_xor(v, d, kWave);
+    return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) v = v + __shfl_xor(v, d, kWave);
+    return v;
+}
+__device__ __forceinline__ FX wave_sum_fx(FX v) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        FX o;
+        o.lo = __shfl_xor(v.lo, d, kWave);
+        o.hi = __shfl_xor(v.hi, d, kWave);
+        v = fx_add(v, o);
+    }
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// Largest float f with (double)f <= m (round toward -inf): for float w,
+// (double)w > m  <=>  w > pivot_rd(m).
+__device__ __forceinline__ float pivot_rd(double m) {
+    float f = (float)m;
+    if ((double)f > m) f = nextafterf(f, -__builtin_huge_valf());
+    return f;
+}
+
+}  // namespace wrsdev

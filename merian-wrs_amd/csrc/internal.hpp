@@ -1,0 +1,79 @@
+// internal.hpp -- device-resident build parameters, tile geometry and the
+// host-side launchers each kernel translation unit exports.  Not part of the
+// public C ABI (include/wrs.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wrsk {
+
+// Arithmetic of a build (the C ABI's wrs_precision values).
+enum Prec : int {
+    kFP32 = 0,       // FP32 everywhere, FP32 look-back carry (the reference GPU's arithmetic)
+    kFP64Carry = 1,  // FP32 in-tile scan, exact fixed-point carry read out in FP64, FP64 mean/split
+    kComp = 2,       // compensated FP32 in-tile scan (hi+lo) + the kFP64Carry carry and split
+};
+
+// Per-build scalars, resident in device memory so that a whole build can be
+// recorded on a stream (or captured in a graph) without host round trips.
+struct Params {
+    double total;          // sum of the weights (FP32 mode: an FP32 value)
+    double mean;           // total / N: pivot and bucket size (FP32 mode: FP32 value)
+    float pivot;           // heavy <=> w > pivot (pivot = mean rounded toward -inf)
+    int32_t shift;         // fixed-point scale: carry words hold floor(x * 2^shift)
+    uint32_t ticket;       // dynamic tile ticket of the look-back kernels
+    uint32_t heavy_count;  // H, written by the partition's last tile
+    double inv_total;      // 1 / total (ITS normalisation)
+    uint64_t reserved[3];
+};
+static_assert(sizeof(Params) == 64, "Params layout");
+
+struct AliasEntry {
+    float p;
+    uint32_t a;
+};
+
+struct SplitRec {  // debug output of the split stage
+    uint32_t i;
+    uint32_t j;
+    double spill;
+};
+
+constexpr int kBlock = 256;                 // threads per look-back block (4 waves)
+constexpr int kItems = 16;                  // consecutive elements per thread
+constexpr int kTile = kBlock * kItems;      // 4096 elements per tile
+constexpr int kStateWords = 16;             // 128-byte look-back record per tile
+constexpr uint32_t kMaxPartials = 2048;     // reduce blocks
+constexpr int kSplitBlock = 256;            // split+pack threads per block (255 intervals)
+
+inline uint64_t tiles_for(uint64_t n) { return (n + kTile - 1) / kTile; }
+inline uint32_t partials_for(uint64_t n) {
+    const uint64_t t = tiles_for(n);
+    return (uint32_t)(t < kMaxPartials ? (t ? t : 1) : kMaxPartials);
+}
+
+// ---- launchers (each returns the launch's hipError_t) ----
+hipError_t launch_reduce(const float* w, uint64_t n, int prec, double* partials, uint32_t G,
+                         uint64_t* states, uint64_t state_words, hipStream_t s);
+hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int prec,
+                           int has_pivot, double pivot, Params* prm, hipStream_t s);
+hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
+                            uint32_t* idx, float* hi, float* lo, hipStream_t s);
+hipError_t launch_scan(const float* w, uint64_t n, int prec, int normalise, Params* prm,
+                       uint64_t* states, float* hi, float* lo, hipStream_t s);
+hipError_t launch_splitpack(const float* w, uint64_t n, uint32_t K, int prec, const Params* prm,
+                            const uint32_t* idx, const float* hi, const float* lo,
+                            AliasEntry* table, SplitRec* splits, hipStream_t s);
+hipError_t launch_alias_sample(const AliasEntry* table, uint64_t n, uint64_t s_count,
+                               uint64_t seed, uint64_t counter_base, uint32_t* out, hipStream_t s);
+hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                             uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                             uint32_t* out, hipStream_t s);
+hipError_t launch_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uint64_t n,
+                         uint32_t* out4, hipStream_t s);
+hipError_t launch_generate(int kind, double a, double b, uint64_t seed, uint64_t n, float* out,
+                           hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t words, uint64_t value, hipStream_t s);
+
+}  // namespace wrsk

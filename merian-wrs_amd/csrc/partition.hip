@@ -1,0 +1,393 @@
+// partition.hip -- single-pass heavy/light prefix-partition with decoupled
+// look-back (pass 2 of a PSA build).
+//
+// Semantics follow the reference kernel src/device/prefix_partition/decoupled/
+// shader.comp:313-362 and its CPU counterpart src/host/reference/partition.hpp:35-53:
+//   heavy <=> w > pivot (strict), stable order;
+//   idx[0..H)        heavy element indices, ascending;
+//   idx[N-1-r]       the r-th light element index (lights stored reversed);
+//   prefix[same pos] inclusive prefix of the heavy (resp. light) stream;
+//   heavy_count = H.
+//
+// MI355X design (one 256-thread block = 4 wave64s per 4096-element tile):
+//   1. dynamic tile ticket (atomicAdd) so that every predecessor of a tile has
+//      been scheduled before it spins on it;
+//   2. coalesced float4 loads, transposed through LDS (row stride 17 words, so
+//      both the transpose writes and the per-thread reads avoid bank
+//      conflicts) so that each thread owns 16 consecutive elements;
+//   3. per-thread sequential (compensated) sums, wave64 shuffle scan, then a
+//      4-entry cross-wave scan in LDS, all in a fixed order;
+//   4. the tile aggregate is published as self-validating 64-bit words in HBM
+//      (126-bit fixed point for the two weight streams, exact integer count);
+//      wave 0 looks back up to 64 predecessors per step (one per lane);
+//   5. outputs are staged in LDS in final order (heavies by rank, lights by
+//      reversed rank) and written as two contiguous coalesced runs.
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace wrsk {
+using namespace wrsdev;
+
+namespace {
+
+constexpr int kPad = kTile + kTile / 16;  // LDS words of the padded transpose buffer
+
+// Look-back record layout (16 words per tile):
+//   [0..4]  aggregate: heavy sum (2 words), light sum (2 words), heavy count
+//   [8..12] inclusive prefix, same fields
+// FP32 mode keeps each sum in one float (word .lo; .hi = 0).
+struct Carry {
+    FX h, l;
+    uint64_t c;
+};
+
+__device__ __forceinline__ Carry carry_zero() { return Carry{fx_zero(), fx_zero(), 0}; }
+
+template <int PREC>
+__device__ __forceinline__ Carry carry_add(Carry a, Carry b) {
+    Carry r;
+    if constexpr (PREC == kFP32) {
+        r.h.lo = (uint64_t)__float_as_uint(__uint_as_float((uint32_t)a.h.lo) +
+                                           __uint_as_float((uint32_t)b.h.lo));
+        r.l.lo = (uint64_t)__float_as_uint(__uint_as_float((uint32_t)a.l.lo) +
+                                           __uint_as_float((uint32_t)b.l.lo));
+        r.h.hi = r.l.hi = 0;
+    } else {
+        r.h = fx_add(a.h, b.h);
+        r.l = fx_add(a.l, b.l);
+    }
+    r.c = a.c + b.c;
+    return r;
+}
+
+template <int PREC>
+__device__ __forceinline__ Carry wave_sum_carry(Carry v) {
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        Carry o;
+        o.h.lo = __shfl_xor(v.h.lo, d, kWave);
+        o.h.hi = __shfl_xor(v.h.hi, d, kWave);
+        o.l.lo = __shfl_xor(v.l.lo, d, kWave);
+        o.l.hi = __shfl_xor(v.l.hi, d, kWave);
+        o.c = __shfl_xor(v.c, d, kWave);
+        v = carry_add<PREC>(v, o);
+    }
+    return v;
+}
+
+__device__ __forceinline__ void publish(uint64_t* rec, const Carry& v) {
+    st_word(rec + 0, v.h.lo);
+    st_word(rec + 1, v.h.hi);
+    st_word(rec + 2, v.l.lo);
+    st_word(rec + 3, v.l.hi);
+    st_word(rec + 4, v.c);
+}
+
+// Returns true if all five words are published.
+__device__ __forceinline__ bool read_rec(const uint64_t* rec, Carry& v) {
+    v.h.lo = ld_word(rec + 0);
+    v.h.hi = ld_word(rec + 1);
+    v.l.lo = ld_word(rec + 2);
+    v.l.hi = ld_word(rec + 3);
+    v.c = ld_word(rec + 4);
+    return ((v.h.lo | v.h.hi | v.l.lo | v.l.hi | v.c) >> 63) == 0;
+}
+
+// In-tile per-thread accumulator: compensated (DF) or plain float.
+template <int PREC>
+struct Acc {
+    DF h, l;
+    uint32_t c;
+};
+
+template <int PREC>
+__device__ __forceinline__ Acc<PREC> acc_add(Acc<PREC> a, Acc<PREC> b) {
+    Acc<PREC> r;
+    if constexpr (PREC == kComp) {
+        r.h = df_add(a.h, b.h);
+        r.l = df_add(a.l, b.l);
+    } else {
+        r.h = DF{a.h.hi + b.h.hi, 0.0f};
+        r.l = DF{a.l.hi + b.l.hi, 0.0f};
+    }
+    r.c = a.c + b.c;
+    return r;
+}
+
+template <int PREC>
+__device__ __forceinline__ Acc<PREC> acc_shfl_up(Acc<PREC> v, int d) {
+    Acc<PREC> r;
+    r.h.hi = shfl_up(v.h.hi, d);
+    r.l.hi = shfl_up(v.l.hi, d);
+    if constexpr (PREC == kComp) {
+        r.h.lo = shfl_up(v.h.lo, d);
+        r.l.lo = shfl_up(v.l.lo, d);
+    } else {
+        r.h.lo = 0.0f;
+        r.l.lo = 0.0f;
+    }
+    r.c = shfl_up(v.c, d);
+    return r;
+}
+
+template <int PREC>
+__device__ __forceinline__ double stream_value(DF v) {
+    if constexpr (PREC == kComp) return df_value(v);
+    else return (double)v.hi;
+}
+
+}  // namespace
+
+template <int PREC, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_partition(const float* __restrict__ w, uint32_t n,
+                                                      Params* __restrict__ prm,
+                                                      uint64_t* __restrict__ states,
+                                                      uint32_t* __restrict__ idx_out,
+                                                      float* __restrict__ hi_out,
+                                                      float* __restrict__ lo_out) {
+    __shared__ float s_buf[kPad];
+    __shared__ uint32_t s_tile;
+    __shared__ Acc<PREC> s_wave[kBlock / kWave];
+    __shared__ double s_carry_h, s_carry_l;
+    __shared__ float s_carry_hf, s_carry_lf;
+    __shared__ uint32_t s_carry_c;
+
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wid = t / kWave;
+    if (t == 0) s_tile = atomicAdd(&prm->ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const float pivot = prm->pivot;
+    const int shift = prm->shift;
+
+    // ---- 2. load + transpose -------------------------------------------
+    if (VEC && m == (uint32_t)kTile) {
+        const float4* w4 = reinterpret_cast<const float4*>(w + base);
+        float4 v4[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v4[j] = __builtin_nontemporal_load(w4 + t + kBlock * j);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e = 4 * (t + kBlock * j);
+            const int q = e + e / 16;
+            s_buf[q + 0] = v4[j].x;
+            s_buf[q + 1] = v4[j].y;
+            s_buf[q + 2] = v4[j].z;
+            s_buf[q + 3] = v4[j].w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * j;
+            s_buf[e + e / 16] = e < m ? w[base + e] : 0.0f;
+        }
+    }
+    __syncthreads();
+    float v[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) v[k] = s_buf[17 * t + k];
+    const uint32_t e0 = (uint32_t)(kItems * t);
+
+    // ---- 3. in-tile scan -----------------------------------------------
+    Acc<PREC> mine{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        if (e0 + k < m) {
+            const bool heavy = v[k] > pivot;
+            if constexpr (PREC == kComp) {
+                if (heavy) mine.h = df_add1(mine.h, v[k]);
+                else mine.l = df_add1(mine.l, v[k]);
+            } else {
+                if (heavy) mine.h.hi += v[k];
+                else mine.l.hi += v[k];
+            }
+            mine.c += heavy ? 1u : 0u;
+        }
+    }
+    Acc<PREC> incl = mine;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const Acc<PREC> o = acc_shfl_up<PREC>(incl, d);
+        if (lane >= d) incl = acc_add<PREC>(o, incl);
+    }
+    Acc<PREC> lane_excl = acc_shfl_up<PREC>(incl, 1);
+    if (lane == 0) lane_excl = Acc<PREC>{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
+    if (lane == kWave - 1) s_wave[wid] = incl;
+    __syncthreads();
+    Acc<PREC> wave_excl{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
+    for (int q = 0; q < wid; ++q) wave_excl = acc_add<PREC>(wave_excl, s_wave[q]);
+    Acc<PREC> tile_agg = s_wave[0];
+#pragma unroll
+    for (int q = 1; q < kBlock / kWave; ++q) tile_agg = acc_add<PREC>(tile_agg, s_wave[q]);
+    const Acc<PREC> thr_excl = acc_add<PREC>(wave_excl, lane_excl);
+
+    // ---- 4. publish + decoupled look-back ------------------------------
+    uint64_t* rec = states + (uint64_t)tile * kStateWords;
+    if (wid == 0) {
+        Carry agg;
+        if constexpr (PREC == kFP32) {
+            agg.h = FX{(uint64_t)__float_as_uint(tile_agg.h.hi), 0};
+            agg.l = FX{(uint64_t)__float_as_uint(tile_agg.l.hi), 0};
+        } else {
+            agg.h = fx_from_double(stream_value<PREC>(tile_agg.h), shift);
+            agg.l = fx_from_double(stream_value<PREC>(tile_agg.l), shift);
+        }
+        agg.c = tile_agg.c;
+        if (tile == 0) {
+            if (lane == 0) publish(rec + 8, agg);
+        } else {
+            if (lane == 0) publish(rec + 0, agg);
+            Carry excl = carry_zero();
+            int64_t look = (int64_t)tile - 1;
+            for (;;) {
+                const int64_t tt = look - lane;
+                int st;  // 0 not ready, 1 aggregate, 2 inclusive prefix
+                Carry pv = carry_zero();
+                if (tt < 0) {
+                    st = 2;
+                } else {
+                    const uint64_t* r = states + (uint64_t)tt * kStateWords;
+                    if (read_rec(r + 8, pv)) st = 2;
+                    else if (read_rec(r + 0, pv)) st = 1;
+                    else {
+                        st = 0;
+                        pv = carry_zero();
+                    }
+                }
+                const uint64_t b_inc = __ballot(st == 2);
+                const uint64_t b_nr = __ballot(st == 0);
+                const int fi = b_inc ? __builtin_ctzll(b_inc) : kWave;
+                const int fn = b_nr ? __builtin_ctzll(b_nr) : kWave;
+                if (fi < fn) {
+                    const Carry c = lane <= fi ? pv : carry_zero();
+                    excl = carry_add<PREC>(excl, wave_sum_carry<PREC>(c));
+                    break;
+                }
+                if (fn > 0) {
+                    const Carry c = lane < fn ? pv : carry_zero();
+                    excl = carry_add<PREC>(excl, wave_sum_carry<PREC>(c));
+                    look -= fn;
+                } else {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            if (lane == 0) {
+                publish(rec + 8, carry_add<PREC>(excl, agg));
+                if constexpr (PREC == kFP32) {
+                    s_carry_hf = __uint_as_float((uint32_t)excl.h.lo);
+                    s_carry_lf = __uint_as_float((uint32_t)excl.l.lo);
+                } else {
+                    s_carry_h = fx_to_double(excl.h, shift);
+                    s_carry_l = fx_to_double(excl.l, shift);
+                }
+                s_carry_c = (uint32_t)excl.c;
+            }
+        }
+        if (tile == 0 && lane == 0) {
+            s_carry_h = 0.0;
+            s_carry_l = 0.0;
+            s_carry_hf = 0.0f;
+            s_carry_lf = 0.0f;
+            s_carry_c = 0;
+        }
+    }
+    __syncthreads();
+
+    // ---- 5. outputs, staged in LDS in final order -----------------------
+    const uint32_t Hx = s_carry_c;           // heavies before this tile
+    const uint32_t Lx = base - Hx;           // lights before this tile
+    const uint32_t hc = tile_agg.c;          // heavies in this tile
+    uint32_t* s_u = reinterpret_cast<uint32_t*>(s_buf);
+
+    uint32_t slot[kItems];
+    {
+        uint32_t hr = thr_excl.c;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const uint32_t e = e0 + k;
+            const bool heavy = v[k] > pivot;
+            slot[k] = heavy ? hr : (m - 1u - (e - hr));
+            hr += heavy ? 1u : 0u;
+        }
+    }
+    // positions of LDS slot s: heavy run then light run
+    auto out_pos = [&](uint32_t s) -> uint32_t { return s < hc ? Hx + s : n - m - Lx + s; };
+
+    // round A: indices
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (e0 + k < m) s_u[slot[k]] = base + e0 + k;
+    __syncthreads();
+    for (uint32_t s = t; s < m; s += kBlock) idx_out[out_pos(s)] = s_u[s];
+    __syncthreads();
+
+    // round B: prefix (hi word); the lo word is kept for round C
+    float lo_keep[kItems];
+    {
+        DF rh = thr_excl.h, rl = thr_excl.l;
+        const double ch = s_carry_h, cl = s_carry_l;
+        const float chf = s_carry_hf, clf = s_carry_lf;
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const bool heavy = v[k] > pivot;
+            float hi, lo = 0.0f;
+            if constexpr (PREC == kComp) {
+                if (heavy) rh = df_add1(rh, v[k]);
+                else rl = df_add1(rl, v[k]);
+                const double P = (heavy ? ch : cl) + df_value(heavy ? rh : rl);
+                hi = (float)P;
+                lo = (float)(P - (double)hi);
+            } else if constexpr (PREC == kFP64Carry) {
+                if (heavy) rh.hi += v[k];
+                else rl.hi += v[k];
+                const double P = (heavy ? ch : cl) + (double)(heavy ? rh.hi : rl.hi);
+                hi = (float)P;
+            } else {
+                if (heavy) rh.hi += v[k];
+                else rl.hi += v[k];
+                hi = (heavy ? chf : clf) + (heavy ? rh.hi : rl.hi);
+            }
+            lo_keep[k] = lo;
+            if (e0 + k < m) s_buf[slot[k]] = hi;
+        }
+    }
+    __syncthreads();
+    for (uint32_t s = t; s < m; s += kBlock) hi_out[out_pos(s)] = s_buf[s];
+    if constexpr (PREC == kComp) {
+        if (lo_out != nullptr) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kItems; ++k)
+                if (e0 + k < m) s_buf[slot[k]] = lo_keep[k];
+            __syncthreads();
+            for (uint32_t s = t; s < m; s += kBlock) lo_out[out_pos(s)] = s_buf[s];
+        }
+    }
+    if (t == 0 && base + m == n) prm->heavy_count = Hx + hc;
+}
+
+hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
+                            uint32_t* idx, float* hi, float* lo, hipStream_t s) {
+    const uint32_t tiles = (uint32_t)tiles_for(n);
+    const bool vec = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
+    const uint32_t nn = (uint32_t)n;
+#define WRS_LAUNCH_PART(P, V) \
+    k_partition<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, hi, lo)
+    if (prec == kComp) {
+        if (vec) WRS_LAUNCH_PART(kComp, true);
+        else WRS_LAUNCH_PART(kComp, false);
+    } else if (prec == kFP64Carry) {
+        if (vec) WRS_LAUNCH_PART(kFP64Carry, true);
+        else WRS_LAUNCH_PART(kFP64Carry, false);
+    } else {
+        if (vec) WRS_LAUNCH_PART(kFP32, true);
+        else WRS_LAUNCH_PART(kFP32, false);
+    }
+#undef WRS_LAUNCH_PART
+    return hipGetLastError();
+}
+
+}  // namespace wrsk

@@ -1,0 +1,162 @@
+// sample.hip -- alias-table sampling, ITS sampling, Philox streams and the
+// seeded synthetic weight generators.
+//
+// Alias sampling replaces src/device/wrs/alias/sampling/shader.comp:80-111
+// (non-cooperative path): sample t uses Philox4x32-10 with
+// counter = (lo(base+t), hi(base+t), 0, 0) and key = seed; the 64-bit word
+// (x1:x0) picks the bucket by a 64x64->128 multiply-high (no modulo bias at
+// any N < 2^32), x2 gives the 24-bit coin u2, and the acceptance rule is the
+// reference CPU's (src/host/reference/sample_alias_table.hpp:15-24):
+// u2 > p -> alias, else the bucket.
+//
+// ITS sampling replaces src/device/wrs/its/sampling/shader.comp:142-177
+// (non-cooperative): u in (0,1] from the 53-bit word (x0:x1), then the
+// reference's lower-bound binary search (smallest i with cdf(i) >= u,
+// shader.comp:131-140) over cdf(i) = hi[i] (+ lo[i] when given), in FP64.
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace wrsk {
+using namespace wrsdev;
+
+constexpr int kSampleBlock = 256;
+
+__global__ __launch_bounds__(kSampleBlock) void k_alias_sample(const uint2* __restrict__ table,
+                                                               uint64_t n, uint64_t s,
+                                                               uint64_t seed, uint64_t base,
+                                                               uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kSampleBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kSampleBlock + threadIdx.x; g < s; g += stride) {
+        const U4 r = philox_at(seed, base + g, 0u, 0u);
+        const uint64_t x = ((uint64_t)r.y << 32) | r.x;
+        const uint64_t ix = __umul64hi(x, n);
+        const float u2 = u24(r.z);
+        const uint2 e = table[ix];
+        out[g] = (u2 > __uint_as_float(e.x)) ? e.y : (uint32_t)ix;
+    }
+}
+
+template <bool LO>
+__global__ __launch_bounds__(kSampleBlock) void k_its_sample(const float* __restrict__ hi,
+                                                             const float* __restrict__ lo,
+                                                             uint64_t n, uint64_t s,
+                                                             uint64_t seed, uint64_t base,
+                                                             uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kSampleBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kSampleBlock + threadIdx.x; g < s; g += stride) {
+        const U4 r = philox_at(seed, base + g, 0u, 0u);
+        const double u = u53_open0(r.x, r.y);
+        uint64_t a = 0, b = n - 1;
+        while (a < b) {
+            const uint64_t mid = (a + b) >> 1;
+            double c = (double)hi[mid];
+            if (LO) c += (double)lo[mid];
+            if (u > c) a = mid + 1;
+            else b = mid;
+        }
+        out[g] = (uint32_t)a;
+    }
+}
+
+__global__ void k_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uint64_t n,
+                         uint4* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += stride) {
+        const U4 r = philox_at(seed, base + g, stream_word, 0u);
+        out[g] = make_uint4(r.x, r.y, r.z, r.w);
+    }
+}
+
+// Seeded bijection of [0,n): 4-round Feistel network over 2*half bits with a
+// Philox round function, restricted to [0,n) by cycle walking.
+__device__ uint64_t feistel_perm(uint64_t seed, uint64_t n, uint64_t i) {
+    unsigned bits = 1;
+    while (bits < 64 && (1ull << bits) < n) ++bits;
+    const unsigned half = (bits + 1) / 2;
+    const uint64_t mask = (1ull << half) - 1ull;
+    uint64_t x = i;
+    do {
+        uint64_t L = x >> half, R = x & mask;
+        for (uint32_t round = 0; round < 4; ++round) {
+            const U4 r = philox_at(seed, ((uint64_t)round << 32) | (uint32_t)R, 0u, kTagFeistel);
+            const uint64_t F = (((uint64_t)r.y << 32) | r.x) & mask;
+            const uint64_t nl = R;
+            R = L ^ F;
+            L = nl;
+        }
+        x = (L << half) | R;
+    } while (x >= n);
+    return x;
+}
+
+// kind: 0 U(0,1); 1 LogNormal(a, b); 2 Zipf exponent a over a seeded rank
+// permutation; 3 spike mixture (1% of positions U(1e5,1e6), rest U(1e-3,1)).
+__global__ void k_generate(int kind, double a, double b, uint64_t seed, uint64_t n,
+                           float* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const double two_pi = 6.283185307179586476925286766559;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float v;
+        if (kind == 0) {
+            const U4 r = philox_at(seed, i, 0u, kTagGen);
+            v = u24(r.x);
+        } else if (kind == 1) {
+            const U4 r = philox_at(seed, i, 0u, kTagGen);
+            const double u1 = u53_open0(r.x, r.y);
+            const double u2 = u53(r.z, r.w);
+            const double z = sqrt(-2.0 * log(u1)) * cos(two_pi * u2);
+            v = (float)exp(a + b * z);
+        } else if (kind == 2) {
+            const uint64_t rank = 1u + feistel_perm(seed, n, i);
+            v = (float)pow((double)rank, -a);
+        } else {
+            const U4 r = philox_at(seed, i, 0u, kTagGen);
+            const double u = u53(r.x, r.y);
+            const bool spike = feistel_perm(seed, n, i) < n / 100u;
+            v = spike ? (float)(1e5 + 9e5 * u) : (float)(1e-3 + (1.0 - 1e-3) * u);
+        }
+        out[i] = v;
+    }
+}
+
+static uint32_t grid_for(uint64_t work, uint32_t block, uint32_t cap) {
+    uint64_t g = (work + block - 1) / block;
+    if (g > cap) g = cap;
+    return (uint32_t)(g ? g : 1);
+}
+
+hipError_t launch_alias_sample(const AliasEntry* table, uint64_t n, uint64_t s_count,
+                               uint64_t seed, uint64_t counter_base, uint32_t* out,
+                               hipStream_t s) {
+    k_alias_sample<<<grid_for(s_count, kSampleBlock, 1u << 16), kSampleBlock, 0, s>>>(
+        reinterpret_cast<const uint2*>(table), n, s_count, seed, counter_base, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                             uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                             uint32_t* out, hipStream_t s) {
+    const uint32_t g = grid_for(s_count, kSampleBlock, 1u << 16);
+    if (cdf_lo)
+        k_its_sample<true><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, s_count, seed,
+                                                      counter_base, out);
+    else
+        k_its_sample<false><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, s_count, seed,
+                                                       counter_base, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uint64_t n,
+                         uint32_t* out4, hipStream_t s) {
+    k_philox<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(seed, stream_word, base, n,
+                                                        reinterpret_cast<uint4*>(out4));
+    return hipGetLastError();
+}
+
+hipError_t launch_generate(int kind, double a, double b, uint64_t seed, uint64_t n, float* out,
+                           hipStream_t s) {
+    k_generate<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(kind, a, b, seed, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace wrsk

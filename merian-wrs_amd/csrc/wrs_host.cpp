@@ -1,0 +1,624 @@
+// wrs_host.cpp -- the C++ operator layer (wrs_ops.hpp) and the C ABI
+// (include/wrs.h, include/wrs_runtime.h) on top of the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../../include/wrs.h"
+#include "../../include/wrs_runtime.h"
+#include "batched.hpp"
+#include "internal.hpp"
+#include "wrs_ops.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace wrs {
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        std::string msg = std::string(what) + ": " + hipGetErrorName(e) + " (" +
+                          hipGetErrorString(e) + ")";
+        throw Error(e == hipErrorOutOfMemory ? WRS_ENOMEM : WRS_EHIP, msg);
+    }
+}
+
+template <class T>
+static T* dev_alloc(uint64_t count) {
+    void* p = nullptr;
+    if (count == 0) return nullptr;
+    check_hip(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_buffer) {
+    Workspace ws;
+    ws.max_n = max_n;
+    try {
+        ws.params = dev_alloc<wrsk::Params>(1);
+        ws.partials = dev_alloc<double>(wrsk::kMaxPartials);
+        ws.state_words = wrsk::tiles_for(max_n) * wrsk::kStateWords;
+        ws.states = dev_alloc<uint64_t>(ws.state_words);
+        if (partition_buffers) {
+            ws.indices = dev_alloc<uint32_t>(max_n);
+            ws.prefix_hi = dev_alloc<float>(max_n);
+            if (lo_buffer) ws.prefix_lo = dev_alloc<float>(max_n);
+        }
+    } catch (...) {
+        ws.release();
+        throw;
+    }
+    return ws;
+}
+
+void Workspace::release() {
+    void* ptrs[] = {params, partials, states, indices, prefix_hi, prefix_lo};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    params = nullptr;
+    partials = nullptr;
+    states = nullptr;
+    indices = nullptr;
+    prefix_hi = nullptr;
+    prefix_lo = nullptr;
+}
+
+static void require(bool cond, const char* msg) {
+    if (!cond) throw Error(WRS_EINVAL, msg);
+}
+
+void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, double pivot) const {
+    require(weights != nullptr, "weights is null");
+    require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
+    const uint32_t G = wrsk::partials_for(N);
+    const uint64_t words = wrsk::tiles_for(N) * wrsk::kStateWords;
+    check_hip(wrsk::launch_reduce(weights, N, (int)m_prec, ws.partials, G, ws.states, words, s),
+              "reduce launch");
+    const bool has_pivot = !std::isnan(pivot);
+    check_hip(wrsk::launch_finalize(ws.partials, G, N, (int)m_prec, has_pivot ? 1 : 0,
+                                    has_pivot ? pivot : 0.0, ws.params, s),
+              "finalize launch");
+}
+
+void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) const {
+    require(b.elements && b.partitionIndices && b.partitionPrefix, "null partition buffer");
+    require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
+    check_hip(wrsk::launch_partition(b.elements, N, (int)m_prec, ws.params, ws.states,
+                                     b.partitionIndices, b.partitionPrefix,
+                                     m_prec == WRS_PREC_COMPENSATED ? b.partitionPrefixLo : nullptr,
+                                     s),
+              "partition launch");
+}
+
+void PrefixSum::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) const {
+    require(b.elements && b.prefix, "null scan buffer");
+    require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
+    Mean(m_prec).run(s, b.elements, N, ws);
+    check_hip(wrsk::launch_scan(b.elements, N, (int)m_prec, m_normalise ? 1 : 0, ws.params,
+                                ws.states, b.prefix,
+                                m_prec == WRS_PREC_COMPENSATED ? b.prefixLo : nullptr, s),
+              "scan launch");
+}
+
+void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* dbg,
+              hipEvent_t* ev) const {
+    require(b.weights && b.aliasTable && b.internals, "null PSA buffer");
+    Workspace& ws = *b.internals;
+    require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
+    require(ws.indices && ws.prefix_hi, "workspace has no partition buffers");
+    const wrs_precision prec = m_cfg.precision;
+    float* lo = prec == WRS_PREC_COMPENSATED ? ws.prefix_lo : nullptr;
+    const uint32_t K = splitCount(N);
+    const uint32_t G = wrsk::partials_for(N);
+    const uint64_t words = wrsk::tiles_for(N) * wrsk::kStateWords;
+    if (ev) check_hip(hipEventRecord(ev[0], s), "event");
+    check_hip(wrsk::launch_reduce(b.weights, N, (int)prec, ws.partials, G, ws.states, words, s),
+              "reduce launch");
+    if (ev) check_hip(hipEventRecord(ev[1], s), "event");
+    check_hip(wrsk::launch_finalize(ws.partials, G, N, (int)prec, 0, 0.0, ws.params, s),
+              "finalize launch");
+    if (ev) check_hip(hipEventRecord(ev[2], s), "event");
+    check_hip(wrsk::launch_partition(b.weights, N, (int)prec, ws.params, ws.states, ws.indices,
+                                     ws.prefix_hi, lo, s),
+              "partition launch");
+    if (ev) check_hip(hipEventRecord(ev[3], s), "event");
+    check_hip(wrsk::launch_splitpack(b.weights, N, K, (int)prec, ws.params, ws.indices,
+                                     ws.prefix_hi, lo,
+                                     reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable),
+                                     dbg ? reinterpret_cast<wrsk::SplitRec*>(dbg->splits) : nullptr,
+                                     s),
+              "split+pack launch");
+    if (ev) check_hip(hipEventRecord(ev[4], s), "event");
+    if (dbg) {
+        if (dbg->mean)
+            check_hip(hipMemcpyAsync(dbg->mean, &ws.params->mean, sizeof(double),
+                                     hipMemcpyDeviceToDevice, s), "copy mean");
+        if (dbg->heavy_count)
+            check_hip(hipMemcpyAsync(dbg->heavy_count, &ws.params->heavy_count, sizeof(uint32_t),
+                                     hipMemcpyDeviceToDevice, s), "copy H");
+        if (dbg->indices)
+            check_hip(hipMemcpyAsync(dbg->indices, ws.indices, N * sizeof(uint32_t),
+                                     hipMemcpyDeviceToDevice, s), "copy indices");
+        if (dbg->prefix_hi)
+            check_hip(hipMemcpyAsync(dbg->prefix_hi, ws.prefix_hi, N * sizeof(float),
+                                     hipMemcpyDeviceToDevice, s), "copy prefix");
+        if (dbg->prefix_lo && lo)
+            check_hip(hipMemcpyAsync(dbg->prefix_lo, lo, N * sizeof(float),
+                                     hipMemcpyDeviceToDevice, s), "copy prefix lo");
+    }
+}
+
+void SampleAliasTable::run(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed,
+                           uint64_t counterBase) const {
+    require(b.aliasTable && b.samples, "null sampling buffer");
+    require(N > 0 && N < (1ull << 32), "N must be in [1, 2^32)");
+    if (S == 0) return;
+    check_hip(wrsk::launch_alias_sample(reinterpret_cast<const wrsk::AliasEntry*>(b.aliasTable), N,
+                                        S, seed, counterBase, b.samples, s),
+              "alias sample launch");
+}
+
+void InverseTransformSampling::run(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S,
+                                   uint64_t seed, uint64_t counterBase) const {
+    require(b.cmf && b.samples, "null sampling buffer");
+    require(N > 0 && N < (1ull << 32), "N must be in [1, 2^32)");
+    if (S == 0) return;
+    check_hip(wrsk::launch_its_sample(b.cmf, b.cmfLo, N, S, seed, counterBase, b.samples, s),
+              "its sample launch");
+}
+
+void AliasTable::build(hipStream_t s, const Buffers& b, uint64_t N) const {
+    m_psa.run(s, PSA::Buffers{b.weights, b.aliasTable, b.internals}, N);
+}
+void AliasTable::sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S,
+                        uint64_t seed) const {
+    m_sampler.run(s, SampleAliasTable::Buffers{b.aliasTable, b.samples}, N, S, seed);
+}
+void ITS::build(hipStream_t s, const Buffers& b, uint64_t N) const {
+    m_scan.run(s, PrefixSum::Buffers{b.weights, b.cdf, b.cdfLo}, N, *b.internals);
+}
+void ITS::sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed) const {
+    m_sampler.run(s, InverseTransformSampling::Buffers{b.cdf, b.cdfLo, b.samples}, N, S, seed);
+}
+
+WRS::WRS(WRSConfig cfg)
+    : m_method(std::holds_alternative<PSAConfig>(cfg)
+                   ? std::variant<AliasTable, ITS>(AliasTable(std::get<PSAConfig>(cfg)))
+                   : std::variant<AliasTable, ITS>(ITS(std::get<wrs_precision>(cfg)))) {}
+
+void WRS::build(hipStream_t s, const Buffers& b, uint64_t N) const {
+    if (auto* a = std::get_if<AliasTable>(&m_method))
+        a->build(s, AliasTable::Buffers{b.weights, b.aliasTable, b.samples, b.internals}, N);
+    else
+        std::get<ITS>(m_method).build(s, ITS::Buffers{b.weights, b.cdf, b.cdfLo, b.samples, b.internals},
+                                      N);
+}
+void WRS::sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed) const {
+    if (auto* a = std::get_if<AliasTable>(&m_method))
+        a->sample(s, AliasTable::Buffers{b.weights, b.aliasTable, b.samples, b.internals}, N, S,
+                  seed);
+    else
+        std::get<ITS>(m_method).sample(
+            s, ITS::Buffers{b.weights, b.cdf, b.cdfLo, b.samples, b.internals}, N, S, seed);
+}
+
+}  // namespace wrs
+
+// ===================================================================== C ABI
+struct wrs_plan {
+    wrs::PSAConfig cfg;
+    wrs::Workspace ws;
+    bool timing = false;
+    hipEvent_t ev[5] = {};
+};
+
+namespace {
+
+template <class F>
+wrs_status guarded(F&& f) {
+    try {
+        f();
+        return WRS_OK;
+    } catch (const wrs::Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "host allocation failed";
+        return WRS_ENOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return WRS_EHIP;
+    }
+}
+
+wrs::PSAConfig config_of(const wrs_config* cfg) {
+    wrs::PSAConfig c;
+    if (cfg) {
+        c.splitSize = cfg->split_size ? cfg->split_size : 32u;
+        c.precision = cfg->precision;
+    }
+    return c;
+}
+
+void require_plan(const wrs_plan* p) {
+    if (!p) throw wrs::Error(WRS_EINVAL, "plan is null");
+}
+
+hipStream_t hs(wrs_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace
+
+extern "C" {
+
+wrs_status wrs_plan_create(const wrs_config* cfg, uint64_t max_n, wrs_plan** out) {
+    return guarded([&] {
+        if (!out) throw wrs::Error(WRS_EINVAL, "out is null");
+        *out = nullptr;
+        if (max_n == 0 || max_n >= (1ull << 32))
+            throw wrs::Error(WRS_EINVAL, "max_n must be in [1, 2^32)");
+        wrs::PSAConfig c = config_of(cfg);
+        if ((int)c.precision < 0 || (int)c.precision > 2)
+            throw wrs::Error(WRS_EINVAL, "unknown precision");
+        int dev_count = 0;
+        if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+            throw wrs::Error(WRS_ENOTSUP, "no HIP device");
+        wrs_plan* p = new wrs_plan;
+        p->cfg = c;
+        try {
+            p->ws = wrs::Workspace::allocate(max_n, true, c.precision == WRS_PREC_COMPENSATED);
+            for (auto& e : p->ev) wrs::check_hip(hipEventCreate(&e), "hipEventCreate");
+        } catch (...) {
+            p->ws.release();
+            for (auto& e : p->ev)
+                if (e) (void)hipEventDestroy(e);
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+wrs_status wrs_plan_destroy(wrs_plan* plan) {
+    return guarded([&] {
+        if (!plan) return;
+        plan->ws.release();
+        for (auto& e : plan->ev)
+            if (e) (void)hipEventDestroy(e);
+        delete plan;
+    });
+}
+
+wrs_status wrs_psa_build(wrs_plan* plan, const float* weights, uint64_t n, wrs_alias_entry* table,
+                         wrs_stream_t stream) {
+    return guarded([&] {
+        require_plan(plan);
+        wrs::PSA(plan->cfg).run(hs(stream), wrs::PSA::Buffers{weights, table, &plan->ws}, n,
+                                nullptr, plan->timing ? plan->ev : nullptr);
+    });
+}
+
+wrs_status wrs_psa_build_debug(wrs_plan* plan, const float* weights, uint64_t n,
+                               wrs_alias_entry* table, const wrs_psa_debug* dbg,
+                               wrs_stream_t stream) {
+    return guarded([&] {
+        require_plan(plan);
+        wrs::PSA(plan->cfg).run(hs(stream), wrs::PSA::Buffers{weights, table, &plan->ws}, n, dbg,
+                                plan->timing ? plan->ev : nullptr);
+    });
+}
+
+wrs_status wrs_mean(wrs_plan* plan, const float* weights, uint64_t n, double* out2,
+                    wrs_stream_t stream) {
+    return guarded([&] {
+        require_plan(plan);
+        if (!out2) throw wrs::Error(WRS_EINVAL, "out is null");
+        wrs::Mean(plan->cfg.precision).run(hs(stream), weights, n, plan->ws);
+        wrs::check_hip(hipMemcpyAsync(out2, plan->ws.params, 2 * sizeof(double),
+                                      hipMemcpyDeviceToDevice, hs(stream)),
+                       "copy mean");
+    });
+}
+
+wrs_status wrs_prefix_partition(wrs_plan* plan, const float* weights, uint64_t n, double pivot,
+                                uint32_t* indices, float* prefix_hi, float* prefix_lo,
+                                uint32_t* heavy_count, wrs_stream_t stream) {
+    return guarded([&] {
+        require_plan(plan);
+        if (!heavy_count) throw wrs::Error(WRS_EINVAL, "heavy_count is null");
+        wrs::Mean(plan->cfg.precision).run(hs(stream), weights, n, plan->ws, pivot);
+        wrs::PrefixPartition(plan->cfg.precision)
+            .run(hs(stream), wrs::PrefixPartition::Buffers{weights, indices, prefix_hi, prefix_lo},
+                 n, plan->ws);
+        wrs::check_hip(hipMemcpyAsync(heavy_count, &plan->ws.params->heavy_count, sizeof(uint32_t),
+                                      hipMemcpyDeviceToDevice, hs(stream)),
+                       "copy H");
+    });
+}
+
+wrs_status wrs_scan_inclusive(wrs_plan* plan, const float* in, uint64_t n, float* out_hi,
+                              float* out_lo, wrs_stream_t stream) {
+    return guarded([&] {
+        require_plan(plan);
+        wrs::PrefixSum(plan->cfg.precision, false)
+            .run(hs(stream), wrs::PrefixSum::Buffers{in, out_hi, out_lo}, n, plan->ws);
+    });
+}
+
+wrs_status wrs_its_build(wrs_plan* plan, const float* weights, uint64_t n, float* cdf_hi,
+                         float* cdf_lo, wrs_stream_t stream) {
+    return guarded([&] {
+        require_plan(plan);
+        wrs::PrefixSum(plan->cfg.precision, true)
+            .run(hs(stream), wrs::PrefixSum::Buffers{weights, cdf_hi, cdf_lo}, n, plan->ws);
+    });
+}
+
+wrs_status wrs_alias_sample(const wrs_alias_entry* table, uint64_t n, uint64_t s, uint64_t seed,
+                            uint64_t counter_base, uint32_t* out, wrs_stream_t stream) {
+    return guarded([&] {
+        wrs::SampleAliasTable().run(hs(stream), wrs::SampleAliasTable::Buffers{table, out}, n, s,
+                                    seed, counter_base);
+    });
+}
+
+wrs_status wrs_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n, uint64_t s,
+                          uint64_t seed, uint64_t counter_base, uint32_t* out,
+                          wrs_stream_t stream) {
+    return guarded([&] {
+        wrs::InverseTransformSampling().run(
+            hs(stream), wrs::InverseTransformSampling::Buffers{cdf_hi, cdf_lo, out}, n, s, seed,
+            counter_base);
+    });
+}
+
+wrs_status wrs_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uint64_t n,
+                      uint32_t* out, wrs_stream_t stream) {
+    return guarded([&] {
+        if (!out && n) throw wrs::Error(WRS_EINVAL, "out is null");
+        if (n == 0) return;
+        wrs::check_hip(wrsk::launch_philox(seed, stream_word, base, n, out, hs(stream)),
+                       "philox launch");
+    });
+}
+
+wrs_status wrs_generate(wrs_generator kind, double a, double b, uint64_t seed, uint64_t n,
+                        float* out, wrs_stream_t stream) {
+    return guarded([&] {
+        if (!out && n) throw wrs::Error(WRS_EINVAL, "out is null");
+        if ((int)kind < 0 || (int)kind > 3) throw wrs::Error(WRS_EINVAL, "unknown generator");
+        if (n == 0) return;
+        wrs::check_hip(wrsk::launch_generate((int)kind, a, b, seed, n, out, hs(stream)),
+                       "generate launch");
+    });
+}
+
+// ------------------------------------------------------------- batched ----
+struct wrs_batch_plan {
+    wrsb::BatchPlan impl;
+};
+
+wrs_status wrs_batch_plan_create(const wrs_config* cfg, const uint64_t* offsets_host,
+                                 uint32_t nseg, wrs_batch_plan** out) {
+    return guarded([&] {
+        if (!out || !offsets_host || nseg == 0)
+            throw wrs::Error(WRS_EINVAL, "bad batch plan arguments");
+        *out = nullptr;
+        wrs_batch_plan* p = new wrs_batch_plan;
+        try {
+            wrs::PSAConfig c = config_of(cfg);
+            p->impl.create(c.splitSize, (int)c.precision, offsets_host, nseg);
+        } catch (...) {
+            p->impl.destroy();
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+wrs_status wrs_batch_plan_destroy(wrs_batch_plan* plan) {
+    return guarded([&] {
+        if (!plan) return;
+        plan->impl.destroy();
+        delete plan;
+    });
+}
+
+wrs_status wrs_psa_build_batched(wrs_batch_plan* plan, const float* weights,
+                                 const uint64_t* offsets_dev, wrs_alias_entry* tables,
+                                 wrs_stream_t stream) {
+    return guarded([&] {
+        if (!plan || !weights || !offsets_dev || !tables)
+            throw wrs::Error(WRS_EINVAL, "null batched build argument");
+        plan->impl.build(weights, offsets_dev, reinterpret_cast<wrsk::AliasEntry*>(tables),
+                         hs(stream));
+    });
+}
+
+wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_t* offsets_dev,
+                                    uint32_t nseg, uint32_t per_segment, uint64_t seed,
+                                    uint64_t counter_base, uint32_t* out, wrs_stream_t stream) {
+    return guarded([&] {
+        if (!tables || !offsets_dev || !out || nseg == 0)
+            throw wrs::Error(WRS_EINVAL, "null batched sample argument");
+        if (per_segment == 0) return;
+        wrs::check_hip(wrsb::launch_sample_batched(reinterpret_cast<const wrsk::AliasEntry*>(tables),
+                                                   offsets_dev, nseg, per_segment, seed,
+                                                   counter_base, out, hs(stream)),
+                       "batched sample launch");
+    });
+}
+
+// ------------------------------------------------------- introspection ----
+const char* wrs_last_error(void) { return g_last_error.c_str(); }
+const char* wrs_version(void) { return "merian-wrs_amd 0.1 (gfx950)"; }
+
+wrs_status wrs_plan_set_timing(wrs_plan* plan, int enable) {
+    return guarded([&] {
+        require_plan(plan);
+        plan->timing = enable != 0;
+    });
+}
+
+wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms4) {
+    return guarded([&] {
+        require_plan(plan);
+        if (!ms4) throw wrs::Error(WRS_EINVAL, "ms4 is null");
+        wrs::check_hip(hipEventSynchronize(plan->ev[4]), "event sync");
+        for (int k = 0; k < 4; ++k)
+            wrs::check_hip(hipEventElapsedTime(&ms4[k], plan->ev[k], plan->ev[k + 1]),
+                           "event elapsed");
+    });
+}
+
+// ------------------------------------------------------------- runtime ----
+wrs_status wrs_rt_device_count(int* count) {
+    return guarded([&] {
+        if (!count) throw wrs::Error(WRS_EINVAL, "count is null");
+        *count = 0;
+        if (hipGetDeviceCount(count) != hipSuccess) *count = 0;
+    });
+}
+
+wrs_status wrs_rt_device_info(char* name, size_t len, char* arch, size_t arch_len, int* cu_count,
+                              uint64_t* hbm_bytes) {
+    return guarded([&] {
+        hipDeviceProp_t prop;
+        wrs::check_hip(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+        if (name && len) {
+            std::strncpy(name, prop.name, len - 1);
+            name[len - 1] = 0;
+        }
+        if (arch && arch_len) {
+            std::strncpy(arch, prop.gcnArchName, arch_len - 1);
+            arch[arch_len - 1] = 0;
+        }
+        if (cu_count) *cu_count = prop.multiProcessorCount;
+        if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    });
+}
+
+wrs_status wrs_rt_malloc(void** ptr, size_t bytes) {
+    return guarded([&] {
+        if (!ptr) throw wrs::Error(WRS_EINVAL, "ptr is null");
+        *ptr = nullptr;
+        wrs::check_hip(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
+    });
+}
+wrs_status wrs_rt_free(void* ptr) {
+    return guarded([&] {
+        if (ptr) wrs::check_hip(hipFree(ptr), "hipFree");
+    });
+}
+wrs_status wrs_rt_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { wrs::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "h2d"); });
+}
+wrs_status wrs_rt_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { wrs::check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "d2h"); });
+}
+wrs_status wrs_rt_memcpy_dtod(void* dst, const void* src, size_t bytes, wrs_stream_t stream) {
+    return guarded([&] {
+        wrs::check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, hs(stream)), "d2d");
+    });
+}
+wrs_status wrs_rt_memset(void* dst, int value, size_t bytes, wrs_stream_t stream) {
+    return guarded([&] { wrs::check_hip(hipMemsetAsync(dst, value, bytes, hs(stream)), "memset"); });
+}
+wrs_status wrs_rt_stream_create(wrs_stream_t* stream) {
+    return guarded([&] {
+        hipStream_t s = nullptr;
+        wrs::check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream create");
+        *stream = reinterpret_cast<wrs_stream_t>(s);
+    });
+}
+wrs_status wrs_rt_stream_destroy(wrs_stream_t stream) {
+    return guarded([&] {
+        if (stream) wrs::check_hip(hipStreamDestroy(hs(stream)), "stream destroy");
+    });
+}
+wrs_status wrs_rt_stream_sync(wrs_stream_t stream) {
+    return guarded([&] { wrs::check_hip(hipStreamSynchronize(hs(stream)), "stream sync"); });
+}
+wrs_status wrs_rt_device_sync(void) {
+    return guarded([&] { wrs::check_hip(hipDeviceSynchronize(), "device sync"); });
+}
+wrs_status wrs_rt_event_create(wrs_event_t* ev) {
+    return guarded([&] {
+        hipEvent_t e = nullptr;
+        wrs::check_hip(hipEventCreate(&e), "event create");
+        *ev = reinterpret_cast<wrs_event_t>(e);
+    });
+}
+wrs_status wrs_rt_event_destroy(wrs_event_t ev) {
+    return guarded([&] {
+        if (ev) wrs::check_hip(hipEventDestroy(reinterpret_cast<hipEvent_t>(ev)), "event destroy");
+    });
+}
+wrs_status wrs_rt_event_record(wrs_event_t ev, wrs_stream_t stream) {
+    return guarded([&] {
+        wrs::check_hip(hipEventRecord(reinterpret_cast<hipEvent_t>(ev), hs(stream)), "event record");
+    });
+}
+wrs_status wrs_rt_event_elapsed(float* ms, wrs_event_t start, wrs_event_t stop) {
+    return guarded([&] {
+        wrs::check_hip(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)), "event sync");
+        wrs::check_hip(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),
+                                           reinterpret_cast<hipEvent_t>(stop)),
+                       "event elapsed");
+    });
+}
+wrs_status wrs_rt_copy_f32(float* dst, const float* src, uint64_t n, wrs_stream_t stream) {
+    return guarded([&] {
+        if (n == 0) return;
+        wrs::check_hip(wrsb::launch_copy_f32(dst, src, n, hs(stream)), "copy launch");
+    });
+}
+
+struct wrs_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+wrs_status wrs_rt_graph_begin(wrs_stream_t stream) {
+    return guarded([&] {
+        if (!stream) throw wrs::Error(WRS_EINVAL, "graph capture needs a non-default stream");
+        wrs::check_hip(hipStreamBeginCapture(hs(stream), hipStreamCaptureModeThreadLocal),
+                       "begin capture");
+    });
+}
+wrs_status wrs_rt_graph_end(wrs_stream_t stream, wrs_graph** out) {
+    return guarded([&] {
+        wrs_graph* g = new wrs_graph;
+        try {
+            wrs::check_hip(hipStreamEndCapture(hs(stream), &g->graph), "end capture");
+            wrs::check_hip(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0),
+                           "graph instantiate");
+        } catch (...) {
+            if (g->graph) (void)hipGraphDestroy(g->graph);
+            delete g;
+            throw;
+        }
+        *out = g;
+    });
+}
+wrs_status wrs_rt_graph_launch(wrs_graph* graph, wrs_stream_t stream) {
+    return guarded([&] {
+        if (!graph) throw wrs::Error(WRS_EINVAL, "graph is null");
+        wrs::check_hip(hipGraphLaunch(graph->exec, hs(stream)), "graph launch");
+    });
+}
+wrs_status wrs_rt_graph_destroy(wrs_graph* graph) {
+    return guarded([&] {
+        if (!graph) return;
+        if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+        if (graph->graph) (void)hipGraphDestroy(graph->graph);
+        delete graph;
+    });
+}
+
+}  // extern "C"

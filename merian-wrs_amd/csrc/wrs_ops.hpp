@@ -1,0 +1,205 @@
+// wrs_ops.hpp -- C++ host layer above the kernels, mirroring the reference's
+// operator classes (same names, argument meaning and build/sample split) with a
+// HIP stream in place of the Vulkan command buffer:
+//
+//   reference (src/device/...)                       here
+//   Mean<float>::run            mean/Mean.hpp        wrs::Mean::run
+//   PrefixPartition<float>::run prefix_partition/    wrs::PrefixPartition::run
+//   PrefixSum<float>::run       prefix_sum/          wrs::PrefixSum::run
+//   PSA::run                    wrs/alias/psa/PSA.hpp wrs::PSA::run
+//   SampleAliasTable::run       wrs/alias/sampling/  wrs::SampleAliasTable::run
+//   InverseTransformSampling::run wrs/its/sampling/  wrs::InverseTransformSampling::run
+//   AliasTable / ITS / WRS      wrs/alias, wrs/its, wrs/WRS.hpp  same names
+//
+// Buffers are plain device pointers owned by the caller; each class's
+// Buffers::allocate sizes the internal workspace (look-back records, partition
+// indices and prefixes) the way the reference's Buffers::allocate does.
+// Failures throw wrs::Error carrying a wrs_status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <variant>
+
+#include "../../include/wrs.h"
+#include "internal.hpp"
+
+namespace wrs {
+
+class Error : public std::runtime_error {
+  public:
+    Error(wrs_status s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+    wrs_status status;
+};
+
+void check_hip(hipError_t e, const char* what);
+
+// Device workspace shared by the build operators.  All sizes derive from max_n.
+struct Workspace {
+    uint64_t max_n = 0;
+    wrsk::Params* params = nullptr;    // [1]
+    double* partials = nullptr;        // [kMaxPartials]
+    uint64_t* states = nullptr;        // [tiles * kStateWords]
+    uint64_t state_words = 0;
+    uint32_t* indices = nullptr;       // [max_n] (PSA only)
+    float* prefix_hi = nullptr;        // [max_n] (PSA only)
+    float* prefix_lo = nullptr;        // [max_n] (PSA only, compensated)
+
+    static Workspace allocate(uint64_t max_n, bool partition_buffers, bool lo_buffer);
+    void release();
+};
+
+class Mean {
+  public:
+    explicit Mean(wrs_precision prec) : m_prec(prec) {}
+    // Pass 1: FP64 total/mean into ws.params, also resets ws.states for the
+    // look-back kernel that follows.  pivot: NaN -> the mean.
+    void run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws,
+             double pivot = __builtin_nan("")) const;
+
+  private:
+    wrs_precision m_prec;
+};
+
+class PrefixPartition {
+  public:
+    struct Buffers {
+        const float* elements;
+        uint32_t* partitionIndices;
+        float* partitionPrefix;    // hi word
+        float* partitionPrefixLo;  // may be null
+    };
+    explicit PrefixPartition(wrs_precision prec) : m_prec(prec) {}
+    // Requires Mean::run on the same stream first (pivot and reset records).
+    void run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) const;
+
+  private:
+    wrs_precision m_prec;
+};
+
+class PrefixSum {
+  public:
+    struct Buffers {
+        const float* elements;
+        float* prefix;    // hi word
+        float* prefixLo;  // may be null
+    };
+    explicit PrefixSum(wrs_precision prec, bool normalise = false)
+        : m_prec(prec), m_normalise(normalise) {}
+    void run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) const;
+
+  private:
+    wrs_precision m_prec;
+    bool m_normalise;
+};
+
+struct PSAConfig {
+    uint32_t splitSize = 32;
+    wrs_precision precision = WRS_PREC_COMPENSATED;
+};
+
+class PSA {
+  public:
+    struct Buffers {
+        const float* weights;
+        wrs_alias_entry* aliasTable;
+        Workspace* internals;
+    };
+    explicit PSA(PSAConfig cfg) : m_cfg(cfg) {}
+    uint32_t splitCount(uint64_t N) const {
+        const uint64_t K = (N + m_cfg.splitSize - 1) / m_cfg.splitSize;
+        return (uint32_t)(K ? K : 1);
+    }
+    // events (optional, 5 entries) bracket the four passes for phase timing.
+    void run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* dbg = nullptr,
+             hipEvent_t* events = nullptr) const;
+    const PSAConfig& config() const { return m_cfg; }
+
+  private:
+    PSAConfig m_cfg;
+};
+
+class SampleAliasTable {
+  public:
+    struct Buffers {
+        const wrs_alias_entry* aliasTable;
+        uint32_t* samples;
+    };
+    void run(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed,
+             uint64_t counterBase = 0) const;
+};
+
+class InverseTransformSampling {
+  public:
+    struct Buffers {
+        const float* cmf;
+        const float* cmfLo;  // may be null
+        uint32_t* samples;
+    };
+    void run(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed,
+             uint64_t counterBase = 0) const;
+};
+
+// AliasTable = PSA build + alias sampling (src/device/wrs/alias/AliasTable.hpp:75-112)
+class AliasTable {
+  public:
+    struct Buffers {
+        const float* weights;
+        wrs_alias_entry* aliasTable;
+        uint32_t* samples;
+        Workspace* internals;
+    };
+    explicit AliasTable(PSAConfig cfg) : m_psa(cfg) {}
+    void build(hipStream_t s, const Buffers& b, uint64_t N) const;
+    void sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed) const;
+
+  private:
+    PSA m_psa;
+    SampleAliasTable m_sampler;
+};
+
+// ITS = normalised scan + binary-search sampling (src/device/wrs/its/ITS.hpp:79-135)
+class ITS {
+  public:
+    struct Buffers {
+        const float* weights;
+        float* cdf;
+        float* cdfLo;  // may be null
+        uint32_t* samples;
+        Workspace* internals;
+    };
+    explicit ITS(wrs_precision prec) : m_prec(prec), m_scan(prec, true) {}
+    void build(hipStream_t s, const Buffers& b, uint64_t N) const;
+    void sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed) const;
+
+  private:
+    wrs_precision m_prec;
+    PrefixSum m_scan;
+    InverseTransformSampling m_sampler;
+};
+
+// WRS facade: variant dispatch as in src/device/wrs/WRS.hpp:91-172.
+using WRSConfig = std::variant<PSAConfig /* alias */, wrs_precision /* ITS */>;
+class WRS {
+  public:
+    struct Buffers {
+        const float* weights;
+        uint32_t* samples;
+        wrs_alias_entry* aliasTable;  // alias method
+        float* cdf;                   // ITS method
+        float* cdfLo;
+        Workspace* internals;
+    };
+    explicit WRS(WRSConfig cfg);
+    void build(hipStream_t s, const Buffers& b, uint64_t N) const;
+    void sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S,
+                uint64_t seed = 12345u) const;
+
+  private:
+    std::variant<AliasTable, ITS> m_method;
+};
+
+}  // namespace wrs
