@@ -155,19 +155,23 @@ wrs_status wrs_generate(wrs_generator kind, double a, double b, uint64_t seed, u
                         float* out, wrs_stream_t stream);
 
 /* ---- batched (segmented) builds of many independent distributions ------
- * Segment b is weights[offsets[b] .. offsets[b+1]); its table occupies the
- * same range of `tables` with segment-local indices.  offsets_host and
- * offsets_dev hold the same nseg+1 values.  No reference counterpart: the
- * reference builds one distribution per dispatch sequence. */
+ * Segment b is weights[offsets[b] .. offsets[b+1]) (offsets[0] == 0, every
+ * segment non-empty and at most 2^23 weights); its table occupies the same
+ * range of `tables` with segment-local indices.  The plan copies the offsets
+ * and derives its work lists from them.  Every segment gets exactly the
+ * arithmetic of wrs_psa_build, so the tables are bit-identical to building
+ * the segments one at a time.  No reference counterpart: the reference builds
+ * one distribution per dispatch sequence. */
 typedef struct wrs_batch_plan wrs_batch_plan;
 wrs_status wrs_batch_plan_create(const wrs_config* cfg, const uint64_t* offsets_host,
                                  uint32_t nseg, wrs_batch_plan** out);
 wrs_status wrs_batch_plan_destroy(wrs_batch_plan* plan);
 wrs_status wrs_psa_build_batched(wrs_batch_plan* plan, const float* weights,
-                                 const uint64_t* offsets_dev, wrs_alias_entry* tables,
-                                 wrs_stream_t stream);
+                                 wrs_alias_entry* tables, wrs_stream_t stream);
 /* per_segment samples of each segment: out[b*per_segment + t] is a segment-local index,
- * Philox counter = counter_base + b*per_segment + t. */
+ * Philox counter = counter_base + b*per_segment + t (the counters a per-segment
+ * wrs_alias_sample with that counter_base would use).  offsets_dev: nseg+1
+ * offsets in device memory. */
 wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_t* offsets_dev,
                                     uint32_t nseg, uint32_t per_segment, uint64_t seed,
                                     uint64_t counter_base, uint32_t* out, wrs_stream_t stream);

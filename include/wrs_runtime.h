@@ -41,6 +41,16 @@ wrs_status wrs_rt_event_elapsed(float* ms, wrs_event_t start, wrs_event_t stop);
 /* copy kernel of known size (bandwidth calibration): dst[i] = src[i], n floats */
 wrs_status wrs_rt_copy_f32(float* dst, const float* src, uint64_t n, wrs_stream_t stream);
 
+/* Baseline of the batched build: each segment built and sampled by its own
+ * wrs_psa_build + wrs_alias_sample launches, issued from a host loop (what a
+ * caller without the segmented path would write).  Same counters as
+ * wrs_alias_sample_batched, so with identical tables the samples are identical. */
+wrs_status wrs_rt_psa_build_sample_each(wrs_plan* plan, const float* weights,
+                                        const uint64_t* offsets_host, uint32_t nseg,
+                                        wrs_alias_entry* tables, uint32_t per_segment,
+                                        uint64_t seed, uint64_t counter_base, uint32_t* samples,
+                                        wrs_stream_t stream);
+
 /* graph capture of everything issued on `stream` between begin and end */
 typedef struct wrs_graph wrs_graph;
 wrs_status wrs_rt_graph_begin(wrs_stream_t stream);

@@ -1,0 +1,111 @@
+// batched.cpp -- BatchPlan: geometry of the segmented build and its launches.
+#include "batched.hpp"
+
+#include "../../include/wrs.h"
+#include "wrs_ops.hpp"
+
+namespace wrsb {
+
+namespace {
+template <class T>
+T* dalloc(uint64_t count) {
+    void* p = nullptr;
+    wrs::check_hip(hipMalloc(&p, (count ? count : 1) * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+template <class T>
+void upload(T* dst, const std::vector<T>& src) {
+    wrs::check_hip(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice),
+                   "upload");
+}
+}  // namespace
+
+void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_host,
+                       uint32_t nseg_) {
+    split_size = split_size_ ? split_size_ : 32u;
+    prec = prec_;
+    nseg = nseg_;
+    if (prec < 0 || prec > 2) throw wrs::Error(WRS_EINVAL, "unknown precision");
+    offsets.assign(offsets_host, offsets_host + nseg + 1);
+    std::vector<uint32_t> h_item_seg, h_item_tile, h_seg_item0(nseg + 1);
+    std::vector<uint64_t> h_seg_pt0(nseg + 1);
+    uint64_t items64 = 0, pts = 0;
+    for (uint32_t g = 0; g < nseg; ++g) {
+        if (offsets[g + 1] <= offsets[g])
+            throw wrs::Error(WRS_EINVAL, "segments must be non-empty and offsets increasing");
+        const uint64_t n = offsets[g + 1] - offsets[g];
+        if (n >= (1ull << 32)) throw wrs::Error(WRS_EINVAL, "segment too large");
+        const uint64_t tiles = wrsk::tiles_for(n);
+        if (tiles > wrsk::kMaxPartials)
+            throw wrs::Error(WRS_EINVAL, "segment larger than kMaxPartials tiles");
+        h_seg_item0[g] = (uint32_t)items64;
+        for (uint64_t t = 0; t < tiles; ++t) {
+            h_item_seg.push_back(g);
+            h_item_tile.push_back((uint32_t)t);
+        }
+        items64 += tiles;
+        h_seg_pt0[g] = pts;
+        pts += (n + split_size - 1) / split_size + 1;
+    }
+    if (items64 >= (1ull << 31)) throw wrs::Error(WRS_EINVAL, "too many tiles");
+    h_seg_item0[nseg] = (uint32_t)items64;
+    h_seg_pt0[nseg] = pts;
+    items = (uint32_t)items64;
+    total_points = pts;
+    total_n = offsets[nseg] - offsets[0];
+    if (offsets[0] != 0) throw wrs::Error(WRS_EINVAL, "offsets[0] must be 0");
+
+    seg_off = dalloc<uint64_t>(nseg + 1);
+    seg_item0 = dalloc<uint32_t>(nseg + 1);
+    seg_pt0 = dalloc<uint64_t>(nseg + 1);
+    item_seg = dalloc<uint32_t>(items);
+    item_tile = dalloc<uint32_t>(items);
+    seg_prm = dalloc<wrsk::Params>(nseg);
+    partials = dalloc<double>(items);
+    states = dalloc<uint64_t>((uint64_t)items * wrsk::kStateWords);
+    ticket = dalloc<uint32_t>(1);
+    idx = dalloc<uint32_t>(total_n);
+    hi = dalloc<float>(total_n);
+    if (prec == WRS_PREC_COMPENSATED) lo = dalloc<float>(total_n);
+    upload(seg_off, offsets);
+    upload(seg_item0, h_seg_item0);
+    upload(seg_pt0, h_seg_pt0);
+    upload(item_seg, h_item_seg);
+    upload(item_tile, h_item_tile);
+}
+
+void BatchPlan::destroy() {
+    void* ptrs[] = {seg_off, seg_item0, seg_pt0, item_seg, item_tile, seg_prm,
+                    partials, states,   ticket,  idx,      hi,        lo};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    seg_off = nullptr;
+    seg_item0 = nullptr;
+    seg_pt0 = nullptr;
+    item_seg = nullptr;
+    item_tile = nullptr;
+    seg_prm = nullptr;
+    partials = nullptr;
+    states = nullptr;
+    ticket = nullptr;
+    idx = nullptr;
+    hi = nullptr;
+    lo = nullptr;
+}
+
+void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) const {
+    wrs::check_hip(wrsk::launch_seg_reduce(w, seg_off, item_seg, item_tile, items, prec, partials,
+                                           states, s),
+                   "segmented reduce launch");
+    wrs::check_hip(wrsk::launch_seg_finalize(seg_off, seg_item0, nseg, prec, partials, seg_prm,
+                                             ticket, s),
+                   "segmented finalize launch");
+    wrs::check_hip(wrsk::launch_seg_partition(w, seg_off, item_seg, item_tile, seg_item0, items,
+                                              prec, seg_prm, ticket, states, idx, hi, lo, s),
+                   "segmented partition launch");
+    wrs::check_hip(wrsk::launch_seg_splitpack(w, seg_off, seg_pt0, nseg, total_points, split_size,
+                                              prec, seg_prm, idx, hi, lo, tables, s),
+                   "segmented split+pack launch");
+}
+
+}  // namespace wrsb

@@ -1,0 +1,46 @@
+// batched.hpp -- host side of the segmented (batched) PSA build: thousands of
+// independent distributions of uneven sizes, laid out back to back (CSR
+// offsets), built by four segmented launches whose work lists (segment, tile)
+// and (segment, split point) are derived once from the offsets when the plan
+// is created.  Each segment is built with exactly the arithmetic and order of a
+// single-distribution build, so the tables are bit-identical to building the
+// segments one by one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace wrsb {
+
+struct BatchPlan {
+    uint32_t split_size = 32;
+    int prec = 2;
+    uint32_t nseg = 0;
+    uint32_t items = 0;          // (segment, tile) work items
+    uint64_t total_n = 0;        // weights over all segments
+    uint64_t total_points = 0;   // split points over all segments (sum of K_g + 1)
+    std::vector<uint64_t> offsets;  // host copy, nseg + 1
+    // device
+    uint64_t* seg_off = nullptr;     // [nseg+1]
+    uint32_t* seg_item0 = nullptr;   // [nseg+1]
+    uint64_t* seg_pt0 = nullptr;     // [nseg+1]
+    uint32_t* item_seg = nullptr;    // [items]
+    uint32_t* item_tile = nullptr;   // [items]
+    wrsk::Params* seg_prm = nullptr; // [nseg]
+    double* partials = nullptr;      // [items]
+    uint64_t* states = nullptr;      // [items * kStateWords]
+    uint32_t* ticket = nullptr;      // [1]
+    uint32_t* idx = nullptr;         // [total_n]
+    float* hi = nullptr;             // [total_n]
+    float* lo = nullptr;             // [total_n] (compensated only)
+
+    void create(uint32_t split_size, int prec, const uint64_t* offsets_host, uint32_t nseg);
+    void destroy();
+    void build(const float* weights, wrsk::AliasEntry* tables, hipStream_t s) const;
+};
+
+}  // namespace wrsb

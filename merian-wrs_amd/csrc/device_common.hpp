@@ -137,6 +137,21 @@ __device__ __forceinline__ uint64_t ld_word(const uint64_t* p) {
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ------------------------------------------------- streaming (NT) access ----
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_nt4(const float4* p) {
+    const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt4(float4* p, float4 v) {
+    f32x4_t x;
+    x.x = v.x;
+    x.y = v.y;
+    x.z = v.z;
+    x.w = v.w;
+    __builtin_nontemporal_store(x, reinterpret_cast<f32x4_t*>(p));
+}
+
 // ------------------------------------------------------------ wave ops ----
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 

@@ -76,4 +76,27 @@ hipError_t launch_generate(int kind, double a, double b, uint64_t seed, uint64_t
                            hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t words, uint64_t value, hipStream_t s);
 
+// segmented (batched) forms: items are (segment, tile) pairs, points are the
+// flattened split points of all segments
+hipError_t launch_seg_reduce(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
+                             const uint32_t* item_tile, uint32_t items, int prec,
+                             double* partials, uint64_t* states, hipStream_t s);
+hipError_t launch_seg_finalize(const uint64_t* seg_off, const uint32_t* seg_item0, uint32_t nseg,
+                               int prec, const double* partials, Params* seg_prm,
+                               uint32_t* ticket, hipStream_t s);
+hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
+                                const uint32_t* item_tile, const uint32_t* seg_item0,
+                                uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
+                                uint64_t* states, uint32_t* idx, float* hi, float* lo,
+                                hipStream_t s);
+hipError_t launch_seg_splitpack(const float* w, const uint64_t* seg_off, const uint64_t* seg_pt0,
+                                uint32_t nseg, uint64_t total_points, uint32_t split_size,
+                                int prec, const Params* seg_prm, const uint32_t* idx,
+                                const float* hi, const float* lo, AliasEntry* table,
+                                hipStream_t s);
+hipError_t launch_seg_sample(const AliasEntry* tables, const uint64_t* seg_off, uint32_t nseg,
+                             uint32_t per_segment, uint64_t seed, uint64_t counter_base,
+                             uint32_t* out, hipStream_t s);
+hipError_t launch_copy_f32(float* dst, const float* src, uint64_t n, hipStream_t s);
+
 }  // namespace wrsk

@@ -138,15 +138,17 @@ __device__ __forceinline__ double stream_value(DF v) {
 
 }  // namespace
 
+// One 4096-element tile of the distribution w[0..n): `tile` is its index,
+// `states` the look-back records of tile 0 onward, prm its build scalars.
+// Used by the single-distribution kernel and by the segmented (batched) one.
 template <int PREC, bool VEC>
-__global__ __launch_bounds__(kBlock) void k_partition(const float* __restrict__ w, uint32_t n,
-                                                      Params* __restrict__ prm,
-                                                      uint64_t* __restrict__ states,
-                                                      uint32_t* __restrict__ idx_out,
-                                                      float* __restrict__ hi_out,
-                                                      float* __restrict__ lo_out) {
+__device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint32_t n,
+                                               uint32_t tile, Params* __restrict__ prm,
+                                               uint64_t* __restrict__ states,
+                                               uint32_t* __restrict__ idx_out,
+                                               float* __restrict__ hi_out,
+                                               float* __restrict__ lo_out) {
     __shared__ float s_buf[kPad];
-    __shared__ uint32_t s_tile;
     __shared__ Acc<PREC> s_wave[kBlock / kWave];
     __shared__ double s_carry_h, s_carry_l;
     __shared__ float s_carry_hf, s_carry_lf;
@@ -155,9 +157,6 @@ __global__ __launch_bounds__(kBlock) void k_partition(const float* __restrict__ 
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
-    if (t == 0) s_tile = atomicAdd(&prm->ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const float pivot = prm->pivot;
@@ -168,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_partition(const float* __restrict__ 
         const float4* w4 = reinterpret_cast<const float4*>(w + base);
         float4 v4[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v4[j] = __builtin_nontemporal_load(w4 + t + kBlock * j);
+        for (int j = 0; j < 4; ++j) v4[j] = ld_nt4(w4 + t + kBlock * j);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int e = 4 * (t + kBlock * j);
@@ -369,6 +368,42 @@ __global__ __launch_bounds__(kBlock) void k_partition(const float* __restrict__ 
     if (t == 0 && base + m == n) prm->heavy_count = Hx + hc;
 }
 
+template <int PREC, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_partition(const float* __restrict__ w, uint32_t n,
+                                                      Params* __restrict__ prm,
+                                                      uint64_t* __restrict__ states,
+                                                      uint32_t* __restrict__ idx_out,
+                                                      float* __restrict__ hi_out,
+                                                      float* __restrict__ lo_out) {
+    __shared__ uint32_t s_tile;
+    if (threadIdx.x == 0) s_tile = atomicAdd(&prm->ticket, 1u);
+    __syncthreads();
+    partition_tile<PREC, VEC>(w, n, s_tile, prm, states, idx_out, hi_out, lo_out);
+}
+
+// Segmented form: one work item per (segment, tile), items of a segment are
+// consecutive and handed out in ticket order, so a tile's predecessors in its
+// segment always hold smaller tickets.  Segment outputs use segment-local
+// indices and occupy the segment's own range of idx/hi/lo.
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_seg_partition(
+    const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
+    const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile,
+    const uint32_t* __restrict__ seg_item0, Params* __restrict__ seg_prm,
+    uint32_t* __restrict__ ticket, uint64_t* __restrict__ states, uint32_t* __restrict__ idx,
+    float* __restrict__ hi, float* __restrict__ lo) {
+    __shared__ uint32_t s_item;
+    if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t item = s_item;
+    const uint32_t sg = item_seg[item];
+    const uint64_t off = seg_off[sg];
+    const uint32_t n = (uint32_t)(seg_off[sg + 1] - off);
+    partition_tile<PREC, false>(w + off, n, item_tile[item], seg_prm + sg,
+                                states + (uint64_t)seg_item0[sg] * kStateWords, idx + off,
+                                hi + off, lo ? lo + off : nullptr);
+}
+
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
                             uint32_t* idx, float* hi, float* lo, hipStream_t s) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
@@ -387,6 +422,22 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
         else WRS_LAUNCH_PART(kFP32, false);
     }
 #undef WRS_LAUNCH_PART
+    return hipGetLastError();
+}
+
+
+hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
+                                const uint32_t* item_tile, const uint32_t* seg_item0,
+                                uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
+                                uint64_t* states, uint32_t* idx, float* hi, float* lo,
+                                hipStream_t s) {
+#define WRS_LAUNCH_SEG(P)                                                                 \
+    k_seg_partition<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_item0, \
+                                                seg_prm, ticket, states, idx, hi, lo)
+    if (prec == kComp) WRS_LAUNCH_SEG(kComp);
+    else if (prec == kFP64Carry) WRS_LAUNCH_SEG(kFP64Carry);
+    else WRS_LAUNCH_SEG(kFP32);
+#undef WRS_LAUNCH_SEG
     return hipGetLastError();
 }
 

@@ -2,83 +2,87 @@
 //
 // Replaces the reference's AtomicMean (src/device/mean/atomic/shader.comp:69-82),
 // whose float atomicAdd makes the pivot change from run to run.  Here every
-// element is accumulated in FP64 (exact conversion of the float), in an order
-// fixed by N alone: block b sums tiles b, b+G, b+2G, ...; thread t of a tile
-// sums its four float4 chunks; waves reduce by a fixed butterfly and the
+// element is accumulated in FP64 (exact conversion of the float) in an order
+// fixed by N alone: block b sums tiles b, b+G, b+2G, ...; within a tile thread t
+// sums the elements 4(t+256j)+q, j = 0..3, q = 0..3 in that order (the same order
+// whether the rows are loaded as float4 or as scalars, so the result does not
+// depend on pointer alignment); waves reduce by a fixed butterfly and the
 // finalize kernel sums the G partials in a fixed tree.  The same launch also
 // resets the look-back records of the kernel that follows it.
+//
+// The segmented (batched) kernels run exactly the same per-tile and finalize
+// code per segment, so a batched build yields the same bits as building each
+// segment on its own.
 #include "device_common.hpp"
 #include "internal.hpp"
 
 namespace wrsk {
 using namespace wrsdev;
 
+namespace {
+
+template <bool FP64>
+using AccT = typename std::conditional<FP64, double, float>::type;
+
+// Adds tile `tile` of w[0..n) into acc in the fixed per-thread order.
 template <bool FP64, bool VEC>
-__global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ w, uint64_t n,
-                                                   double* __restrict__ partials,
-                                                   uint64_t* __restrict__ states,
-                                                   uint64_t state_words) {
-    using Acc = typename std::conditional<FP64, double, float>::type;
-    __shared__ Acc s_wave[kBlock / kWave];
+__device__ __forceinline__ void reduce_tile(const float* __restrict__ w, uint64_t n, uint64_t tile,
+                                            AccT<FP64>& acc) {
+    using Acc = AccT<FP64>;
     const int t = threadIdx.x;
-    const uint32_t G = gridDim.x;
-    const uint64_t tiles = (n + kTile - 1) / kTile;
-    Acc acc = 0;
-    for (uint64_t tile = blockIdx.x; tile < tiles; tile += G) {
-        const uint64_t base = tile * kTile;
-        if (VEC && base + kTile <= n) {
-            const float4* w4 = reinterpret_cast<const float4*>(w + base);
-            float4 v[4];
+    const uint64_t base = tile * kTile;
+    if (VEC && base + kTile <= n) {
+        const float4* w4 = reinterpret_cast<const float4*>(w + base);
+        float4 v[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = w4[t + kBlock * j];
+        for (int j = 0; j < 4; ++j) v[j] = ld_nt4(w4 + t + kBlock * j);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc += (Acc)v[j].x;
-                acc += (Acc)v[j].y;
-                acc += (Acc)v[j].z;
-                acc += (Acc)v[j].w;
-            }
-        } else {
-#pragma unroll 4
-            for (int j = 0; j < kItems; ++j) {
-                const uint64_t e = base + (uint64_t)t + (uint64_t)kBlock * j;
-                if (e < n) acc += (Acc)w[e];
-            }
+        for (int j = 0; j < 4; ++j) {
+            acc += (Acc)v[j].x;
+            acc += (Acc)v[j].y;
+            acc += (Acc)v[j].z;
+            acc += (Acc)v[j].w;
         }
-    }
-    Acc ws;
-    if constexpr (FP64) ws = wave_sum_d(acc);
-    else ws = wave_sum_f(acc);
-    if (lane_id() == 0) s_wave[t / kWave] = ws;
-    // reset the look-back records for the next kernel on the stream
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + t; i < state_words; i += (uint64_t)G * kBlock)
-        states[i] = kSentinel;
-    __syncthreads();
-    if (t == 0) {
-        Acc b = s_wave[0];
-        for (int k = 1; k < kBlock / kWave; ++k) b = b + s_wave[k];
-        partials[blockIdx.x] = (double)b;
+    } else {
+        float v[kItems];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint64_t e = base + 4u * ((uint64_t)t + (uint64_t)kBlock * j) + q;
+                v[4 * j + q] = e < n ? w[e] : 0.0f;
+            }
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) acc += (Acc)v[k];
     }
 }
 
+// Block sum of one value per thread: wave butterflies, then waves in order.
 template <bool FP64>
-__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials,
-                                                     uint32_t G, uint64_t n,
-                                                     int has_pivot, double pivot,
-                                                     Params* __restrict__ prm) {
-    using Acc = typename std::conditional<FP64, double, float>::type;
+__device__ __forceinline__ AccT<FP64> block_sum(AccT<FP64> acc) {
+    using Acc = AccT<FP64>;
     __shared__ Acc s_wave[kBlock / kWave];
-    const int t = threadIdx.x;
-    Acc acc = 0;
-    for (uint32_t i = t; i < G; i += kBlock) acc = acc + (Acc)partials[i];
     Acc ws;
     if constexpr (FP64) ws = wave_sum_d(acc);
     else ws = wave_sum_f(acc);
-    if (lane_id() == 0) s_wave[t / kWave] = ws;
+    if (lane_id() == 0) s_wave[threadIdx.x / kWave] = ws;
     __syncthreads();
+    Acc b = s_wave[0];
+    for (int k = 1; k < kBlock / kWave; ++k) b = b + s_wave[k];
+    return b;
+}
+
+// Sums G partials and derives the build scalars (thread 0 writes *prm).
+template <bool FP64>
+__device__ __forceinline__ void finalize_block(const double* __restrict__ partials, uint32_t G,
+                                               uint64_t n, int has_pivot, double pivot,
+                                               Params* __restrict__ prm) {
+    using Acc = AccT<FP64>;
+    const int t = threadIdx.x;
+    Acc acc = 0;
+    for (uint32_t i = t; i < G; i += kBlock) acc = acc + (Acc)partials[i];
+    const Acc total = block_sum<FP64>(acc);
     if (t == 0) {
-        Acc total = s_wave[0];
-        for (int k = 1; k < kBlock / kWave; ++k) total = total + s_wave[k];
         Params p;
         if constexpr (FP64) {
             p.total = total;
@@ -103,6 +107,66 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
         p.reserved[0] = p.reserved[1] = p.reserved[2] = 0;
         *prm = p;
     }
+}
+
+}  // namespace
+
+template <bool FP64, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ w, uint64_t n,
+                                                   double* __restrict__ partials,
+                                                   uint64_t* __restrict__ states,
+                                                   uint64_t state_words) {
+    const int t = threadIdx.x;
+    const uint32_t G = gridDim.x;
+    const uint64_t tiles = (n + kTile - 1) / kTile;
+    AccT<FP64> acc = 0;
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += G) reduce_tile<FP64, VEC>(w, n, tile, acc);
+    // reset the look-back records for the next kernel on the stream
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + t; i < state_words; i += (uint64_t)G * kBlock)
+        states[i] = kSentinel;
+    const AccT<FP64> b = block_sum<FP64>(acc);
+    if (t == 0) partials[blockIdx.x] = (double)b;
+}
+
+template <bool FP64>
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials,
+                                                     uint32_t G, uint64_t n, int has_pivot,
+                                                     double pivot, Params* __restrict__ prm) {
+    finalize_block<FP64>(partials, G, n, has_pivot, pivot, prm);
+}
+
+// Segmented: one block per (segment, tile) item; every segment has at most
+// kMaxPartials tiles, so the single-distribution kernel would also give each
+// tile its own block -- identical partials.
+template <bool FP64>
+__global__ __launch_bounds__(kBlock) void k_seg_reduce(const float* __restrict__ w,
+                                                       const uint64_t* __restrict__ seg_off,
+                                                       const uint32_t* __restrict__ item_seg,
+                                                       const uint32_t* __restrict__ item_tile,
+                                                       double* __restrict__ partials,
+                                                       uint64_t* __restrict__ states) {
+    const uint32_t item = blockIdx.x;
+    const uint32_t sg = item_seg[item];
+    const uint64_t off = seg_off[sg];
+    const uint64_t n = seg_off[sg + 1] - off;
+    AccT<FP64> acc = 0;
+    reduce_tile<FP64, false>(w + off, n, item_tile[item], acc);
+    if (threadIdx.x < kStateWords) states[(uint64_t)item * kStateWords + threadIdx.x] = kSentinel;
+    const AccT<FP64> b = block_sum<FP64>(acc);
+    if (threadIdx.x == 0) partials[item] = (double)b;
+}
+
+template <bool FP64>
+__global__ __launch_bounds__(kBlock) void k_seg_finalize(const uint64_t* __restrict__ seg_off,
+                                                         const uint32_t* __restrict__ seg_item0,
+                                                         const double* __restrict__ partials,
+                                                         Params* __restrict__ seg_prm,
+                                                         uint32_t* __restrict__ ticket) {
+    const uint32_t sg = blockIdx.x;
+    const uint32_t i0 = seg_item0[sg];
+    const uint32_t G = seg_item0[sg + 1] - i0;
+    finalize_block<FP64>(partials + i0, G, seg_off[sg + 1] - seg_off[sg], 0, 0.0, seg_prm + sg);
+    if (sg == 0 && threadIdx.x == 0) *ticket = 0;
 }
 
 __global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t words, uint64_t value) {
@@ -132,6 +196,26 @@ hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int p
         k_finalize<true><<<1, kBlock, 0, s>>>(partials, G, n, has_pivot, pivot, prm);
     else
         k_finalize<false><<<1, kBlock, 0, s>>>(partials, G, n, has_pivot, pivot, prm);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_reduce(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
+                             const uint32_t* item_tile, uint32_t items, int prec,
+                             double* partials, uint64_t* states, hipStream_t s) {
+    if (prec != kFP32)
+        k_seg_reduce<true><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, partials, states);
+    else
+        k_seg_reduce<false><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, partials, states);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_finalize(const uint64_t* seg_off, const uint32_t* seg_item0, uint32_t nseg,
+                               int prec, const double* partials, Params* seg_prm,
+                               uint32_t* ticket, hipStream_t s) {
+    if (prec != kFP32)
+        k_seg_finalize<true><<<nseg, kBlock, 0, s>>>(seg_off, seg_item0, partials, seg_prm, ticket);
+    else
+        k_seg_finalize<false><<<nseg, kBlock, 0, s>>>(seg_off, seg_item0, partials, seg_prm, ticket);
     return hipGetLastError();
 }
 

@@ -119,6 +119,35 @@ __global__ void k_generate(int kind, double a, double b, uint64_t seed, uint64_t
     }
 }
 
+// Batched alias sampling: sample g belongs to segment g / per_segment and
+// returns a segment-local index; counter = counter_base + g.
+__global__ __launch_bounds__(kSampleBlock) void k_seg_sample(const uint2* __restrict__ tables,
+                                                             const uint64_t* __restrict__ seg_off,
+                                                             uint64_t total, uint32_t per_segment,
+                                                             uint64_t seed, uint64_t base,
+                                                             uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kSampleBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kSampleBlock + threadIdx.x; g < total; g += stride) {
+        const uint64_t sg = g / per_segment;
+        const uint64_t off = seg_off[sg];
+        const uint64_t n = seg_off[sg + 1] - off;
+        const U4 r = philox_at(seed, base + g, 0u, 0u);
+        const uint64_t x = ((uint64_t)r.y << 32) | r.x;
+        const uint64_t ix = __umul64hi(x, n);
+        const float u2 = u24(r.z);
+        const uint2 e = tables[off + ix];
+        out[g] = (u2 > __uint_as_float(e.x)) ? e.y : (uint32_t)ix;
+    }
+}
+
+// Plain copy of known size, the bandwidth calibration kernel.
+__global__ __launch_bounds__(256) void k_copy_f32(float4* __restrict__ dst,
+                                                  const float4* __restrict__ src, uint64_t n4) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n4; i += stride)
+        st_nt4(dst + i, ld_nt4(src + i));
+}
+
 static uint32_t grid_for(uint64_t work, uint32_t block, uint32_t cap) {
     uint64_t g = (work + block - 1) / block;
     if (g > cap) g = cap;
@@ -156,6 +185,24 @@ hipError_t launch_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uin
 hipError_t launch_generate(int kind, double a, double b, uint64_t seed, uint64_t n, float* out,
                            hipStream_t s) {
     k_generate<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(kind, a, b, seed, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_sample(const AliasEntry* tables, const uint64_t* seg_off, uint32_t nseg,
+                             uint32_t per_segment, uint64_t seed, uint64_t counter_base,
+                             uint32_t* out, hipStream_t s) {
+    const uint64_t total = (uint64_t)nseg * per_segment;
+    k_seg_sample<<<grid_for(total, kSampleBlock, 1u << 16), kSampleBlock, 0, s>>>(
+        reinterpret_cast<const uint2*>(tables), seg_off, total, per_segment, seed, counter_base,
+        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_f32(float* dst, const float* src, uint64_t n, hipStream_t s) {
+    // callers pass 16-byte aligned buffers with n a multiple of 4 (hipMalloc'd, bench sizes)
+    const uint64_t n4 = n / 4;
+    k_copy_f32<<<grid_for(n4, 256, 8192), 256, 0, s>>>(reinterpret_cast<float4*>(dst),
+                                                       reinterpret_cast<const float4*>(src), n4);
     return hipGetLastError();
 }
 

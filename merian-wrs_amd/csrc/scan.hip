@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
         const float4* w4 = reinterpret_cast<const float4*>(w + base);
         float4 v4[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v4[j] = __builtin_nontemporal_load(w4 + t + kBlock * j);
+        for (int j = 0; j < 4; ++j) v4[j] = ld_nt4(w4 + t + kBlock * j);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int e = 4 * (t + kBlock * j);
@@ -221,8 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
         for (int j = 0; j < 4; ++j) {
             const int e = 4 * (t + kBlock * j);
             const int q = e + e / 16;
-            __builtin_nontemporal_store(make_float4(s_buf[q], s_buf[q + 1], s_buf[q + 2], s_buf[q + 3]),
-                                        o4 + t + kBlock * j);
+            st_nt4(o4 + t + kBlock * j,
+                   make_float4(s_buf[q], s_buf[q + 1], s_buf[q + 2], s_buf[q + 3]));
         }
     } else {
         for (uint32_t e = t; e < m; e += kBlock) hi_out[base + e] = s_buf[e + e / 16];
@@ -238,9 +238,8 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
             for (int j = 0; j < 4; ++j) {
                 const int e = 4 * (t + kBlock * j);
                 const int q = e + e / 16;
-                __builtin_nontemporal_store(
-                    make_float4(s_buf[q], s_buf[q + 1], s_buf[q + 2], s_buf[q + 3]),
-                    o4 + t + kBlock * j);
+                st_nt4(o4 + t + kBlock * j,
+                       make_float4(s_buf[q], s_buf[q + 1], s_buf[q + 2], s_buf[q + 3]));
             }
         } else {
             for (uint32_t e = t; e < m; e += kBlock) lo_out[base + e] = s_buf[e + e / 16];

@@ -3,11 +3,10 @@
 // Split points follow the CPU reference's splitK convention
 // (src/host/reference/split.hpp:67-86): K = ceil(N / split_size) intervals,
 // S(0) = (0, 0, 0), S(k) = split(ceil(N*k/K)) for 0 < k < K, S(K) = (L, H-1, 0).
-// split() is the reference's binary search (split.hpp:14-57) with its exact
-// probe sequence, so that the same (i, j) is returned when several satisfy
-// sigma(i,j) <= mean*n < sigma(i,j+1); the prefixes are read from the
-// prefix-partition layout (heavy prefix at j-1, light prefix of i lights at
-// N-i) as the GPU split does (split/scalar/shader.comp:43-59,86-88).
+// split() is the lower-bound search of the reference GPU kernels (see
+// split_search); the prefixes are read from the prefix-partition layout (heavy
+// prefix at j-1, light prefix of i lights at N-i) as the GPU split does
+// (split/scalar/shader.comp:43-59,86-88).
 //
 // pack() is the reference sweep pack2 (src/host/reference/pack.hpp:14-83): from
 // S(k) to S(k+1), a light bucket (w_l/mean, current heavy) or a heavy bucket
@@ -60,102 +59,74 @@ struct SplitV {
     typename Real<PREC>::T spill;
 };
 
-// Clamped split for the paths the reference cannot take in exact arithmetic
-// (it asserts or reads out of bounds there).
-template <int PREC>
-__device__ SplitV<PREC> split_guard(const float* hi, const float* lo, uint32_t N, uint32_t H,
-                                    uint32_t L, typename Real<PREC>::T mean, uint32_t n,
-                                    int64_t j, typename Real<PREC>::T target) {
-    using T = typename Real<PREC>::T;
-    int64_t jj = j < 0 ? 0 : j;
-    if (jj > (int64_t)H - 1) jj = (int64_t)H - 1;
-    int64_t ii = (int64_t)n - jj;
-    if (ii > (int64_t)L) ii = L;
-    const T light = ii == 0 ? (T)0 : pre_at<PREC>(hi, lo, N - (uint32_t)ii);
-    T spill = (light + pre_at<PREC>(hi, lo, (uint32_t)jj)) - target;
-    if (!(spill > (T)0)) spill = mean;
-    return SplitV<PREC>{(uint32_t)ii, (uint32_t)jj, spill};
-}
-
+// Split at n buckets: the largest j in [0, min(n, H-1)] with
+// sigma(min(n-j, L), j) <= mean*n -- the lower-bound search of the reference
+// GPU kernels (split/scalar/shader.comp:72-109, splitpack/inline/shader.comp:
+// 81-114), with i clamped to L (the shaders clamp to L-1, which breaks
+// i + j = n when every light is used).  sigma(n-j, j) grows with j, so this j
+// is unique and is exactly the state of the sequential sweep after n buckets.
+// (The reference CPU split, split.hpp:14-57, accepts any j with
+// sigma(i,j) <= mean*n < sigma(i,j+1); that set can contain j-1 as well, which
+// the CPU binary search sometimes returns, giving non-monotone splits and a
+// wrong table.)  spill = sigma(i, j+1) - mean*n, the residual of heavy j.
 template <int PREC>
 __device__ SplitV<PREC> split_search(const float* __restrict__ hi, const float* __restrict__ lo,
                                      uint32_t N, uint32_t H, uint32_t L,
                                      typename Real<PREC>::T mean, uint32_t n) {
     using T = typename Real<PREC>::T;
-    int64_t a = 0;
-    int64_t b = min((int64_t)n, (int64_t)H);
     const T target = mean * (T)n;
-    for (;;) {
+    int64_t a = 0;
+    int64_t b = min((int64_t)n, (int64_t)H - 1);
+    while (a <= b) {
         const int64_t j = (a + b) / 2;
         const int64_t i = min((int64_t)n - j, (int64_t)L);
-        if (a > b) {
-            if (j < 0 || i >= (int64_t)L || j + 1 >= (int64_t)H)
-                return split_guard<PREC>(hi, lo, N, H, L, mean, n, j, target);
-            // reference: lightPrefix[i] + heavyPrefix[j+1]
-            const T s2 = pre_at<PREC>(hi, lo, N - 1u - (uint32_t)i) +
-                         pre_at<PREC>(hi, lo, (uint32_t)j + 1u);
-            return SplitV<PREC>{(uint32_t)i, (uint32_t)j, s2 - target};
-        }
         const T heavy = j == 0 ? (T)0 : pre_at<PREC>(hi, lo, (uint32_t)j - 1u);
         const T light = i == 0 ? (T)0 : pre_at<PREC>(hi, lo, N - (uint32_t)i);
-        const T sigma = light + heavy;
-        if (j == (int64_t)H) return split_guard<PREC>(hi, lo, N, H, L, mean, n, j, target);
-        const T sigma2 = light + pre_at<PREC>(hi, lo, (uint32_t)j);
-        if (sigma <= target) {
-            if (sigma2 > target) return SplitV<PREC>{(uint32_t)i, (uint32_t)j, sigma2 - target};
-            a = j + 1;
-        } else {
-            if (j == 0) return split_guard<PREC>(hi, lo, N, H, L, mean, n, j, target);
-            b = j - 1;
-        }
+        if (light + heavy <= target) a = j + 1;
+        else b = j - 1;
     }
+    const int64_t j = b < 0 ? 0 : b;
+    const int64_t i = min((int64_t)n - j, (int64_t)L);
+    const T light = i == 0 ? (T)0 : pre_at<PREC>(hi, lo, N - (uint32_t)i);
+    T spill = (light + pre_at<PREC>(hi, lo, (uint32_t)j)) - target;
+    if (!(spill > (T)0)) spill = mean;
+    return SplitV<PREC>{(uint32_t)i, (uint32_t)j, spill};
 }
 
 }  // namespace
 
+// S(k) of a distribution with H heavies and K intervals.
 template <int PREC>
-__global__ __launch_bounds__(kSplitBlock) void k_splitpack(
-    const float* __restrict__ w, uint32_t N, uint32_t K, const Params* __restrict__ prm,
-    const uint32_t* __restrict__ idx, const float* __restrict__ hi, const float* __restrict__ lo,
-    AliasEntry* __restrict__ table, SplitRec* __restrict__ splits) {
+__device__ __forceinline__ SplitV<PREC> split_point(const float* __restrict__ hi,
+                                                    const float* __restrict__ lo, uint32_t N,
+                                                    uint32_t K, uint32_t H,
+                                                    typename Real<PREC>::T mean, uint64_t k) {
     using T = typename Real<PREC>::T;
-    __shared__ uint32_t s_i[kSplitBlock], s_j[kSplitBlock];
-    __shared__ T s_sp[kSplitBlock];
-
-    const int t = threadIdx.x;
-    const uint64_t p = (uint64_t)blockIdx.x * (kSplitBlock - 1) + (uint64_t)t;
-    const uint32_t H = prm->heavy_count;
     const uint32_t L = N - H;
-    const T mean = (T)prm->mean;
-    const bool ident = (H == 0 || L == 0);
+    if (k == 0 || H == 0 || L == 0) return SplitV<PREC>{0u, 0u, (T)0};
+    if (k == K) return SplitV<PREC>{L, H - 1u, (T)0};
+    return split_search<PREC>(hi, lo, N, H, L, mean, (uint32_t)ceil_mul_div(N, k, K));
+}
 
-    SplitV<PREC> sp{0u, 0u, (T)0};
-    if (!ident && p <= K) {
-        if (p == K) sp = SplitV<PREC>{L, H - 1u, (T)0};
-        else if (p > 0) sp = split_search<PREC>(hi, lo, N, H, L, mean,
-                                                (uint32_t)ceil_mul_div(N, p, K));
-    }
-    s_i[t] = sp.i;
-    s_j[t] = sp.j;
-    s_sp[t] = sp.spill;
-    __syncthreads();
-    if (t == kSplitBlock - 1 || p >= K) return;
-
-    if (ident) {  // all weights equal the mean up to its rounding: identity table
-        const uint32_t b0 = p == 0 ? 0u : (uint32_t)ceil_mul_div(N, p, K);
-        const uint32_t b1 = (uint32_t)ceil_mul_div(N, p + 1, K);
+// Packs interval k, from S(k) = s0 to S(k+1) = (i1, j1) (pack2, pack.hpp:14-83).
+template <int PREC>
+__device__ __forceinline__ void pack_interval(const float* __restrict__ w, uint32_t N, uint32_t K,
+                                              uint32_t H, typename Real<PREC>::T mean,
+                                              const uint32_t* __restrict__ idx,
+                                              AliasEntry* __restrict__ table, uint64_t k,
+                                              SplitV<PREC> s0, uint32_t i1, uint32_t j1) {
+    using T = typename Real<PREC>::T;
+    if (H == 0 || H == N) {  // all weights equal the mean up to its rounding: identity
+        const uint32_t b0 = k == 0 ? 0u : (uint32_t)ceil_mul_div(N, k, K);
+        const uint32_t b1 = (uint32_t)ceil_mul_div(N, k + 1, K);
         for (uint32_t x = b0; x < b1; ++x) table[x] = AliasEntry{1.0f, x};
         return;
     }
-
-    const uint32_t i0 = sp.i, j0 = sp.j;
-    uint32_t i1 = s_i[t + 1], j1 = s_j[t + 1];
-    if (splits) splits[p] = SplitRec{i1, j1, (double)s_sp[t + 1]};
+    const uint32_t i0 = s0.i, j0 = s0.j;
     if (i1 < i0) i1 = i0;
     if (j1 < j0) j1 = j0;
-
     uint32_t i = i0, j = j0;
-    T r = sp.spill;
+    T r = s0.spill;
     if (r == (T)0) r = (T)w[idx[j]];
     while (i != i1 || j != j1) {
         bool packHeavy;
@@ -180,6 +151,73 @@ __global__ __launch_bounds__(kSplitBlock) void k_splitpack(
     }
 }
 
+template <int PREC>
+__global__ __launch_bounds__(kSplitBlock) void k_splitpack(
+    const float* __restrict__ w, uint32_t N, uint32_t K, const Params* __restrict__ prm,
+    const uint32_t* __restrict__ idx, const float* __restrict__ hi, const float* __restrict__ lo,
+    AliasEntry* __restrict__ table, SplitRec* __restrict__ splits) {
+    using T = typename Real<PREC>::T;
+    __shared__ uint32_t s_i[kSplitBlock], s_j[kSplitBlock];
+    __shared__ T s_sp[kSplitBlock];
+
+    const int t = threadIdx.x;
+    const uint64_t p = (uint64_t)blockIdx.x * (kSplitBlock - 1) + (uint64_t)t;
+    const uint32_t H = prm->heavy_count;
+    const T mean = (T)prm->mean;
+    SplitV<PREC> sp{0u, 0u, (T)0};
+    if (p <= K) sp = split_point<PREC>(hi, lo, N, K, H, mean, p);
+    s_i[t] = sp.i;
+    s_j[t] = sp.j;
+    s_sp[t] = sp.spill;
+    __syncthreads();
+    if (t == kSplitBlock - 1 || p >= K) return;
+    if (splits) splits[p] = SplitRec{s_i[t + 1], s_j[t + 1], (double)s_sp[t + 1]};
+    pack_interval<PREC>(w, N, K, H, mean, idx, table, p, sp, s_i[t + 1], s_j[t + 1]);
+}
+
+// Segmented form over the flattened split points of all segments: segment g
+// owns points seg_pt0[g] .. seg_pt0[g] + K_g (K_g + 1 points, K_g intervals).
+template <int PREC>
+__global__ __launch_bounds__(kSplitBlock) void k_seg_splitpack(
+    const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
+    const uint64_t* __restrict__ seg_pt0, uint32_t nseg, uint64_t total_points,
+    uint32_t split_size, const Params* __restrict__ seg_prm, const uint32_t* __restrict__ idx,
+    const float* __restrict__ hi, const float* __restrict__ lo, AliasEntry* __restrict__ table) {
+    using T = typename Real<PREC>::T;
+    __shared__ uint32_t s_i[kSplitBlock], s_j[kSplitBlock];
+    __shared__ T s_sp[kSplitBlock];
+    const int t = threadIdx.x;
+    const uint64_t p = (uint64_t)blockIdx.x * (kSplitBlock - 1) + (uint64_t)t;
+    SplitV<PREC> sp{0u, 0u, (T)0};
+    uint32_t g = 0;
+    uint64_t k = 0, K = 0, off = 0;
+    uint32_t N = 0, H = 0;
+    T mean = (T)0;
+    if (p < total_points) {
+        uint32_t a = 0, b = nseg;  // last g with seg_pt0[g] <= p
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) >> 1;
+            if (seg_pt0[mid] <= p) a = mid;
+            else b = mid;
+        }
+        g = a;
+        k = p - seg_pt0[g];
+        off = seg_off[g];
+        N = (uint32_t)(seg_off[g + 1] - off);
+        K = (N + split_size - 1) / split_size;
+        H = seg_prm[g].heavy_count;
+        mean = (T)seg_prm[g].mean;
+        sp = split_point<PREC>(hi + off, lo ? lo + off : nullptr, N, (uint32_t)K, H, mean, k);
+    }
+    s_i[t] = sp.i;
+    s_j[t] = sp.j;
+    s_sp[t] = sp.spill;
+    __syncthreads();
+    if (t == kSplitBlock - 1 || p >= total_points || k >= K) return;
+    pack_interval<PREC>(w + off, N, (uint32_t)K, H, mean, idx + off, table + off, k, sp,
+                        s_i[t + 1], s_j[t + 1]);
+}
+
 hipError_t launch_splitpack(const float* w, uint64_t n, uint32_t K, int prec, const Params* prm,
                             const uint32_t* idx, const float* hi, const float* lo,
                             AliasEntry* table, SplitRec* splits, hipStream_t s) {
@@ -192,6 +230,23 @@ hipError_t launch_splitpack(const float* w, uint64_t n, uint32_t K, int prec, co
                                                               splits);
     else
         k_splitpack<kFP32><<<blocks, kSplitBlock, 0, s>>>(w, nn, K, prm, idx, hi, lo, table, splits);
+    return hipGetLastError();
+}
+
+
+hipError_t launch_seg_splitpack(const float* w, const uint64_t* seg_off, const uint64_t* seg_pt0,
+                                uint32_t nseg, uint64_t total_points, uint32_t split_size,
+                                int prec, const Params* seg_prm, const uint32_t* idx,
+                                const float* hi, const float* lo, AliasEntry* table,
+                                hipStream_t s) {
+    const uint32_t blocks = (uint32_t)((total_points + (kSplitBlock - 2)) / (kSplitBlock - 1));
+#define WRS_LAUNCH_SSP(P)                                                                     \
+    k_seg_splitpack<P><<<blocks, kSplitBlock, 0, s>>>(w, seg_off, seg_pt0, nseg, total_points, \
+                                                      split_size, seg_prm, idx, hi, lo, table)
+    if (prec == kComp) WRS_LAUNCH_SSP(kComp);
+    else if (prec == kFP64Carry) WRS_LAUNCH_SSP(kFP64Carry);
+    else WRS_LAUNCH_SSP(kFP32);
+#undef WRS_LAUNCH_SSP
     return hipGetLastError();
 }
 

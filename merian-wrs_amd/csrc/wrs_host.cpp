@@ -430,13 +430,11 @@ wrs_status wrs_batch_plan_destroy(wrs_batch_plan* plan) {
 }
 
 wrs_status wrs_psa_build_batched(wrs_batch_plan* plan, const float* weights,
-                                 const uint64_t* offsets_dev, wrs_alias_entry* tables,
-                                 wrs_stream_t stream) {
+                                 wrs_alias_entry* tables, wrs_stream_t stream) {
     return guarded([&] {
-        if (!plan || !weights || !offsets_dev || !tables)
+        if (!plan || !weights || !tables)
             throw wrs::Error(WRS_EINVAL, "null batched build argument");
-        plan->impl.build(weights, offsets_dev, reinterpret_cast<wrsk::AliasEntry*>(tables),
-                         hs(stream));
+        plan->impl.build(weights, reinterpret_cast<wrsk::AliasEntry*>(tables), hs(stream));
     });
 }
 
@@ -447,7 +445,7 @@ wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_
         if (!tables || !offsets_dev || !out || nseg == 0)
             throw wrs::Error(WRS_EINVAL, "null batched sample argument");
         if (per_segment == 0) return;
-        wrs::check_hip(wrsb::launch_sample_batched(reinterpret_cast<const wrsk::AliasEntry*>(tables),
+        wrs::check_hip(wrsk::launch_seg_sample(reinterpret_cast<const wrsk::AliasEntry*>(tables),
                                                    offsets_dev, nseg, per_segment, seed,
                                                    counter_base, out, hs(stream)),
                        "batched sample launch");
@@ -575,7 +573,30 @@ wrs_status wrs_rt_event_elapsed(float* ms, wrs_event_t start, wrs_event_t stop) 
 wrs_status wrs_rt_copy_f32(float* dst, const float* src, uint64_t n, wrs_stream_t stream) {
     return guarded([&] {
         if (n == 0) return;
-        wrs::check_hip(wrsb::launch_copy_f32(dst, src, n, hs(stream)), "copy launch");
+        wrs::check_hip(wrsk::launch_copy_f32(dst, src, n, hs(stream)), "copy launch");
+    });
+}
+
+wrs_status wrs_rt_psa_build_sample_each(wrs_plan* plan, const float* weights,
+                                        const uint64_t* offsets_host, uint32_t nseg,
+                                        wrs_alias_entry* tables, uint32_t per_segment,
+                                        uint64_t seed, uint64_t counter_base, uint32_t* samples,
+                                        wrs_stream_t stream) {
+    return guarded([&] {
+        require_plan(plan);
+        if (!weights || !offsets_host || !tables) throw wrs::Error(WRS_EINVAL, "null argument");
+        const wrs::PSA psa(plan->cfg);
+        const wrs::SampleAliasTable sampler;
+        for (uint32_t g = 0; g < nseg; ++g) {
+            const uint64_t off = offsets_host[g];
+            const uint64_t n = offsets_host[g + 1] - off;
+            psa.run(hs(stream), wrs::PSA::Buffers{weights + off, tables + off, &plan->ws}, n);
+            if (per_segment && samples)
+                sampler.run(hs(stream),
+                            wrs::SampleAliasTable::Buffers{tables + off,
+                                                           samples + (uint64_t)g * per_segment},
+                            n, per_segment, seed, counter_base + (uint64_t)g * per_segment);
+        }
     });
 }
 

@@ -171,12 +171,11 @@ class ITS {
         uint32_t* samples;
         Workspace* internals;
     };
-    explicit ITS(wrs_precision prec) : m_prec(prec), m_scan(prec, true) {}
+    explicit ITS(wrs_precision prec) : m_scan(prec, true) {}
     void build(hipStream_t s, const Buffers& b, uint64_t N) const;
     void sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed) const;
 
   private:
-    wrs_precision m_prec;
     PrefixSum m_scan;
     InverseTransformSampling m_sampler;
 };

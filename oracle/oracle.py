@@ -43,6 +43,9 @@ class PsaDebug(C.Structure):
     ]
 
 
+SPLIT_REFERENCE_CPU = 0  # src/host/reference/split.hpp:14-57
+SPLIT_LOWER_BOUND = 1    # src/device/wrs/alias/psa/split/scalar/shader.comp:72-109
+
 SPLIT64_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float64)])
 SPLIT32_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float32)])
 
@@ -83,16 +86,20 @@ def lib():
     L.oro_split_f32.restype = Split32
     L.oro_split_f64.argtypes = [f64p, C.c_uint32, f64p, C.c_uint32, C.c_double, C.c_uint32]
     L.oro_split_f64.restype = Split64
+    L.oro_split_lb_f64.argtypes = [f64p, C.c_uint32, f64p, C.c_uint32, C.c_double, C.c_uint32]
+    L.oro_split_lb_f64.restype = Split64
     L.oro_splitK_f64.argtypes = [f64p, C.c_uint32, f64p, C.c_uint32, C.c_double,
-                                 C.c_uint32, C.c_uint32, C.c_void_p]
+                                 C.c_uint32, C.c_uint32, C.c_int, C.c_void_p]
     L.oro_splitK_f32.argtypes = [f32p, C.c_uint32, f32p, C.c_uint32, C.c_float,
-                                 C.c_uint32, C.c_uint32, C.c_void_p]
+                                 C.c_uint32, C.c_uint32, C.c_int, C.c_void_p]
+    L.oro_sweep_states_f64.argtypes = [u32p, C.c_uint32, u32p, C.c_uint32, f64p, C.c_double,
+                                       u32p, u32p]
     L.oro_pack_splits_f64.argtypes = [u32p, C.c_uint32, u32p, C.c_uint32, f64p, C.c_uint32,
                                       C.c_double, C.c_void_p, C.c_uint32, f64p, u32p]
-    L.oro_psa_alias_table_f32.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_int, f32p, u32p,
-                                          C.c_void_p]
-    L.oro_psa_alias_table_f64.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_double,
-                                          f64p, u32p, C.c_void_p]
+    L.oro_psa_alias_table_f32.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_int, f32p,
+                                          u32p, C.c_void_p]
+    L.oro_psa_alias_table_f64.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                                          C.c_double, f64p, u32p, C.c_void_p]
     L.oro_sample_alias.argtypes = [f32p, u32p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                    u32p]
     L.oro_its_sample.argtypes = [f32p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
@@ -217,23 +224,43 @@ def split_f32(hp, lp, mean: float, n: int):
     return int(s.i), int(s.j), float(s.spill)
 
 
-def splitK_f64(hp, lp, mean: float, N: int, K: int) -> np.ndarray:
+def split_lb_f64(hp, lp, mean: float, n: int):
+    hp = np.ascontiguousarray(hp, np.float64)
+    lp = np.ascontiguousarray(lp, np.float64)
+    s = lib().oro_split_lb_f64(hp if hp.size else np.zeros(1), hp.size,
+                               lp if lp.size else np.zeros(1), lp.size, mean, n)
+    return int(s.i), int(s.j), float(s.spill)
+
+
+def splitK_f64(hp, lp, mean: float, N: int, K: int, rule: int = SPLIT_LOWER_BOUND) -> np.ndarray:
     hp = np.ascontiguousarray(hp, np.float64)
     lp = np.ascontiguousarray(lp, np.float64)
     out = np.zeros(K, SPLIT64_DTYPE)
-    lib().oro_splitK_f64(hp, hp.size, lp, lp.size, mean, N, K, out.ctypes.data)
+    lib().oro_splitK_f64(hp, hp.size, lp, lp.size, mean, N, K, rule, out.ctypes.data)
     return out
 
 
-def splitK_f32(hp, lp, mean: float, N: int, K: int) -> np.ndarray:
+def splitK_f32(hp, lp, mean: float, N: int, K: int, rule: int = SPLIT_REFERENCE_CPU) -> np.ndarray:
     hp = np.ascontiguousarray(hp, np.float32)
     lp = np.ascontiguousarray(lp, np.float32)
     out = np.zeros(K, SPLIT32_DTYPE)
-    lib().oro_splitK_f32(hp, hp.size, lp, lp.size, np.float32(mean), N, K, out.ctypes.data)
+    lib().oro_splitK_f32(hp, hp.size, lp, lp.size, np.float32(mean), N, K, rule, out.ctypes.data)
     return out
 
 
-def psa_alias_table_f32(w, K: int, reference_scale: bool = False):
+def sweep_states(w, mean: float):
+    """(i, j) after n buckets of the split-free sequential sweep, n = 0..N-1."""
+    idx, H = stable_partition_indices(w, mean)
+    n = len(w)
+    si = np.zeros(n, np.uint32)
+    sj = np.zeros(n, np.uint32)
+    lib().oro_sweep_states_f64(idx[:H].copy(), H, idx[H:].copy(), n - H,
+                               np.ascontiguousarray(w, np.float64), float(mean), si, sj)
+    return si, sj, H
+
+
+def psa_alias_table_f32(w, K: int, reference_scale: bool = False,
+                        split_rule: int = SPLIT_REFERENCE_CPU):
     """Reference CPU path with T = P = float.  Returns (p float32, a uint32, debug dict)."""
     w = np.ascontiguousarray(w, np.float32)
     n = w.size
@@ -244,15 +271,19 @@ def psa_alias_table_f32(w, K: int, reference_scale: bool = False):
     lp = np.zeros(n, np.float64)
     sp = np.zeros(K, SPLIT64_DTYPE)
     dbg = PsaDebug(0.0, 0, idx.ctypes.data, hp.ctypes.data, lp.ctypes.data, sp.ctypes.data)
-    lib().oro_psa_alias_table_f32(w, n, K, int(reference_scale), p, a, C.byref(dbg))
+    lib().oro_psa_alias_table_f32(w, n, K, int(reference_scale), int(split_rule), p, a,
+                                  C.byref(dbg))
     H = int(dbg.heavy_count)
     return p, a, dict(mean=dbg.mean, H=H, idx_cpu=idx, heavy_prefix=hp[:H],
                       light_prefix=lp[:n - H], splits=sp)
 
 
-def psa_alias_table_f64(w, K: int, reference_scale: bool = False, mean=None):
+def psa_alias_table_f64(w, K: int, reference_scale: bool = False, mean=None,
+                        split_rule: int = SPLIT_LOWER_BOUND):
     """Reference CPU path with T = P = double (tests-of-tests instantiation).
-    `mean` (optional) pins the pivot/bucket size ("given the pivot")."""
+    `mean` (optional) pins the pivot/bucket size ("given the pivot");
+    `split_rule` selects the reference CPU split or the lower-bound split of
+    the reference GPU kernels (the default: it always matches the sweep)."""
     w = np.ascontiguousarray(w, np.float32)
     n = w.size
     p = np.empty(n, np.float64)
@@ -263,7 +294,8 @@ def psa_alias_table_f64(w, K: int, reference_scale: bool = False, mean=None):
     sp = np.zeros(K, SPLIT64_DTYPE)
     dbg = PsaDebug(0.0, 0, idx.ctypes.data, hp.ctypes.data, lp.ctypes.data, sp.ctypes.data)
     m = float("nan") if mean is None else float(mean)
-    lib().oro_psa_alias_table_f64(w, n, K, int(reference_scale), m, p, a, C.byref(dbg))
+    lib().oro_psa_alias_table_f64(w, n, K, int(reference_scale), int(split_rule), m, p, a,
+                                  C.byref(dbg))
     H = int(dbg.heavy_count)
     return p, a, dict(mean=dbg.mean, H=H, idx_cpu=idx, heavy_prefix=hp[:H],
                       light_prefix=lp[:n - H], splits=sp)

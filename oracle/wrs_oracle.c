@@ -137,7 +137,7 @@ void oro_segment_sizes(uint64_t seed, uint32_t b, uint64_t lo, uint64_t hi, uint
     uint32_t r[4];
     const double ratio = (double)hi / (double)lo;
     for (uint32_t s = 0; s < b; ++s) {
-        philox_at(seed, s, 0u, 0u, TAG_SEG, r);
+        philox_at(seed, s, 0u, TAG_SEG, 0u, r);
         const double u = u53(r[0], r[1]);
         uint64_t v = (uint64_t)floor((double)lo * pow(ratio, u));
         sizes[s] = ORO_MIN(ORO_MAX(v, lo), hi);
@@ -278,21 +278,60 @@ uint64_t oro_ceil_mul_div(uint64_t a, uint64_t b, uint64_t c) {
 DEFINE_SPLIT(float, oro_split_f32_t, oro_split_f32)
 DEFINE_SPLIT(double, oro_split_f64_t, oro_split_f64)
 
+/* ======================================================================== */
+/* Lower-bound split of the reference GPU kernels                            */
+/* (src/device/wrs/alias/psa/split/scalar/shader.comp:72-109, same search in */
+/* splitpack/inline/shader.comp:81-114): the largest j with                  */
+/* sigma(min(n-j, L), j) <= mean*n, searched over j in [0, min(n, H-1)].     */
+/* sigma(n-j, j) grows with j, so this j is unique: it is the state of the    */
+/* sequential sweep after n buckets (oro_sweep_states).  Deviation from the   */
+/* shader text: i is clamped to L, not L-1 (the shader's L-1 gives           */
+/* i + j = n - 1 when every light is used).                                  */
+/* ======================================================================== */
+#define DEFINE_SPLIT_LB(T, ST, NAME)                                           \
+    ST NAME(const T* hp, uint32_t H, const T* lp, uint32_t L, T mean,          \
+            uint32_t n) {                                                      \
+        const T target = mean * (T)n;                                          \
+        int64_t a = 0;                                                         \
+        int64_t b = ORO_MIN((int64_t)n, (int64_t)H - 1);                       \
+        while (a <= b) {                                                       \
+            const int64_t j = (a + b) / 2;                                     \
+            const int64_t i = ORO_MIN((int64_t)n - j, (int64_t)L);             \
+            const T heavy = j == 0 ? (T)0 : hp[j - 1];                         \
+            const T light = i == 0 ? (T)0 : lp[i - 1];                         \
+            if (light + heavy <= target) a = j + 1;                            \
+            else b = j - 1;                                                    \
+        }                                                                      \
+        const int64_t j = b < 0 ? 0 : b;                                       \
+        const int64_t i = ORO_MIN((int64_t)n - j, (int64_t)L);                 \
+        const T light = i == 0 ? (T)0 : lp[i - 1];                             \
+        ST s;                                                                  \
+        s.i = (uint32_t)i;                                                     \
+        s.j = (uint32_t)j;                                                     \
+        s.spill = (light + hp[j]) - target;                                    \
+        if (!(s.spill > (T)0)) s.spill = mean;                                 \
+        return s;                                                              \
+    }
+DEFINE_SPLIT_LB(float, oro_split_f32_t, oro_split_lb_f32)
+DEFINE_SPLIT_LB(double, oro_split_f64_t, oro_split_lb_f64)
+
 /* split.hpp:67-86 splitK: K splits, the k-th at n = ceil(N*k/K) for
  * k = 1..K-1, the last one fixed to (L, H-1, 0). */
-#define DEFINE_SPLITK(T, ST, NAME, SPLIT)                                      \
+#define DEFINE_SPLITK(T, ST, NAME, SPLIT, SPLIT_LB)                            \
     void NAME(const T* hp, uint32_t H, const T* lp, uint32_t L, T mean,        \
-              uint32_t N, uint32_t K, ST* splits) {                            \
+              uint32_t N, uint32_t K, int rule, ST* splits) {                  \
         for (uint32_t k = 1; k + 1 <= K; ++k) {                                \
             const uint32_t n = (uint32_t)oro_ceil_mul_div(N, k, K);            \
-            splits[k - 1] = SPLIT(hp, H, lp, L, mean, n);                      \
+            splits[k - 1] = rule == ORO_SPLIT_LOWER_BOUND                      \
+                                ? SPLIT_LB(hp, H, lp, L, mean, n)              \
+                                : SPLIT(hp, H, lp, L, mean, n);                \
         }                                                                      \
         splits[K - 1].i = L;                                                   \
         splits[K - 1].j = H - 1;                                               \
         splits[K - 1].spill = (T)0;                                            \
     }
-DEFINE_SPLITK(float, oro_split_f32_t, oro_splitK_f32, oro_split_f32)
-DEFINE_SPLITK(double, oro_split_f64_t, oro_splitK_f64, oro_split_f64)
+DEFINE_SPLITK(float, oro_split_f32_t, oro_splitK_f32, oro_split_f32, oro_split_lb_f32)
+DEFINE_SPLITK(double, oro_split_f64_t, oro_splitK_f64, oro_split_f64, oro_split_lb_f64)
 
 /* ======================================================================== */
 /* pack.hpp:14-83 pack2 and :179-202 packSplits.                            */
@@ -350,11 +389,36 @@ DEFINE_SPLITK(double, oro_split_f64_t, oro_splitK_f64, oro_split_f64)
 DEFINE_PACK(float, oro_split_f32_t, oro_pack_splits_f32)
 DEFINE_PACK(double, oro_split_f64_t, oro_pack_splits_f64)
 
+/* The sequential sweep of pack2 over one interval (0,0) -> (L, H-1), i.e.
+ * the PSA pack with no splits: st_i[n], st_j[n] = lights and heavies packed
+ * after n buckets (n = 0..N-1).  This is the ground truth a split must hit. */
+void oro_sweep_states_f64(const uint32_t* heavy, uint32_t H, const uint32_t* light, uint32_t L,
+                          const double* w, double mean, uint32_t* st_i, uint32_t* st_j) {
+    uint32_t i = 0, j = 0, n = 0;
+    double r = w[heavy[0]];
+    const uint32_t i1 = L, j1 = H - 1;
+    st_i[0] = 0;
+    st_j[0] = 0;
+    while (i != i1 || j != j1) {
+        int packHeavy;
+        if (j == j1) packHeavy = 0;
+        else if (i == i1) packHeavy = 1;
+        else packHeavy = r <= mean;
+        const double weight = packHeavy ? w[heavy[j + 1]] : w[light[i]];
+        if (packHeavy) ++j;
+        else ++i;
+        r = (r + weight) - mean;
+        ++n;
+        st_i[n] = i;
+        st_j[n] = j;
+    }
+}
+
 /* ======================================================================== */
 /* psa_alias_table.hpp:16-67                                                 */
 /* ======================================================================== */
 void oro_psa_alias_table_f32(const float* w_in, uint32_t n, uint32_t K, int reference_scale,
-                             float* p, uint32_t* a, oro_psa_debug* dbg) {
+                             int split_rule, float* p, uint32_t* a, oro_psa_debug* dbg) {
     float* w = (float*)malloc(sizeof(float) * n);
     for (uint32_t i = 0; i < n; ++i) {
         /* `weights[i] *= 0.0001` with T = float: float * double literal,
@@ -386,7 +450,7 @@ void oro_psa_alias_table_f32(const float* w_in, uint32_t n, uint32_t K, int refe
     oro_prefix_sum_f32(vals, H, pre);
     oro_prefix_sum_f32(vals + H, L, pre + H);
     oro_split_f32_t* splits = (oro_split_f32_t*)malloc(sizeof(oro_split_f32_t) * K);
-    oro_splitK_f32(pre, H, pre + H, L, mean, n, K, splits);
+    oro_splitK_f32(pre, H, pre + H, L, mean, n, K, split_rule, splits);
     if (dbg) {
         if (dbg->heavy_prefix)
             for (uint32_t r = 0; r < H; ++r) dbg->heavy_prefix[r] = pre[r];
@@ -408,7 +472,8 @@ void oro_psa_alias_table_f32(const float* w_in, uint32_t n, uint32_t K, int refe
 }
 
 void oro_psa_alias_table_f64(const float* w_in, uint32_t n, uint32_t K, int reference_scale,
-                             double mean_override, double* p, uint32_t* a, oro_psa_debug* dbg) {
+                             int split_rule, double mean_override, double* p, uint32_t* a,
+                             oro_psa_debug* dbg) {
     double* w = (double*)malloc(sizeof(double) * n);
     for (uint32_t i = 0; i < n; ++i) {
         w[i] = reference_scale ? (double)w_in[i] * 0.0001 : (double)w_in[i];
@@ -443,7 +508,7 @@ void oro_psa_alias_table_f64(const float* w_in, uint32_t n, uint32_t K, int refe
     oro_prefix_sum_f64(vals, H, pre);
     oro_prefix_sum_f64(vals + H, L, pre + H);
     oro_split_f64_t* splits = (oro_split_f64_t*)malloc(sizeof(oro_split_f64_t) * K);
-    oro_splitK_f64(pre, H, pre + H, L, mean, n, K, splits);
+    oro_splitK_f64(pre, H, pre + H, L, mean, n, K, split_rule, splits);
     if (dbg) {
         if (dbg->heavy_prefix) memcpy(dbg->heavy_prefix, pre, sizeof(double) * H);
         if (dbg->light_prefix) memcpy(dbg->light_prefix, pre + H, sizeof(double) * L);

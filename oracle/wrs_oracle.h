@@ -18,6 +18,9 @@
  * Documented deviations from the reference text (each also in DESIGN.md):
  *   - ceilMulDiv uses 64-bit intermediates (src/host/reference/split.hpp:59-65
  *     overflows in uint32 once K > 2^16); value-identical where it does not.
+ *   - two split rules: the reference CPU's (split.hpp:14-57) and the reference
+ *     GPU kernels' lower bound (split/scalar/shader.comp:72-109); only the
+ *     latter always reproduces the sequential sweep (see oro_split_lb_*).
  *   - split(): the reference reads out of bounds / asserts on paths that are
  *     unreachable in exact arithmetic (j == H, j == 0 with sigma > target).
  *     Those paths return a clamped in-bounds split (see oro_split_*).
@@ -70,10 +73,23 @@ typedef struct { uint32_t i; uint32_t j; double spill; } oro_split_f64_t;
 uint64_t oro_ceil_mul_div(uint64_t a, uint64_t b, uint64_t c);
 oro_split_f32_t oro_split_f32(const float* hp, uint32_t H, const float* lp, uint32_t L, float mean, uint32_t n);
 oro_split_f64_t oro_split_f64(const double* hp, uint32_t H, const double* lp, uint32_t L, double mean, uint32_t n);
+/* Lower-bound split of the reference GPU kernels (split/scalar/shader.comp:72-109):
+ * the largest j with sigma(min(n-j,L), j) <= mean*n.  The reference CPU split
+ * accepts any j with sigma(i,j) <= mean*n < sigma(i,j+1); that set can hold a
+ * second j (one heavy fewer) that is not the state of the sequential sweep, and
+ * the CPU binary search sometimes returns it (non-monotone splits, wrong table).
+ * The lower-bound j is always the sweep state. */
+oro_split_f32_t oro_split_lb_f32(const float* hp, uint32_t H, const float* lp, uint32_t L, float mean, uint32_t n);
+oro_split_f64_t oro_split_lb_f64(const double* hp, uint32_t H, const double* lp, uint32_t L, double mean, uint32_t n);
+#define ORO_SPLIT_REFERENCE_CPU 0 /* split.hpp:14-57 */
+#define ORO_SPLIT_LOWER_BOUND 1   /* split/scalar/shader.comp:72-109 */
 void oro_splitK_f32(const float* hp, uint32_t H, const float* lp, uint32_t L, float mean,
-                    uint32_t N, uint32_t K, oro_split_f32_t* splits);
+                    uint32_t N, uint32_t K, int rule, oro_split_f32_t* splits);
 void oro_splitK_f64(const double* hp, uint32_t H, const double* lp, uint32_t L, double mean,
-                    uint32_t N, uint32_t K, oro_split_f64_t* splits);
+                    uint32_t N, uint32_t K, int rule, oro_split_f64_t* splits);
+/* (i, j) after n buckets of the split-free sequential sweep, n = 0..N-1 */
+void oro_sweep_states_f64(const uint32_t* heavy, uint32_t H, const uint32_t* light, uint32_t L,
+                          const double* w, double mean, uint32_t* st_i, uint32_t* st_j);
 
 /* ---- pack (src/host/reference/pack.hpp:14-83, 179-202) ----------------- */
 void oro_pack_splits_f32(const uint32_t* heavy, uint32_t H, const uint32_t* light, uint32_t L,
@@ -95,11 +111,12 @@ typedef struct {
     oro_split_f64_t* splits; /* [K] (may be NULL) */
 } oro_psa_debug;
 void oro_psa_alias_table_f32(const float* w, uint32_t n, uint32_t K, int reference_scale,
-                             float* p, uint32_t* a, oro_psa_debug* dbg);
+                             int split_rule, float* p, uint32_t* a, oro_psa_debug* dbg);
 /* mean_override: if not NaN, used instead of the Kahan mean (to compare a GPU
  * build "given the pivot"). */
 void oro_psa_alias_table_f64(const float* w, uint32_t n, uint32_t K, int reference_scale,
-                             double mean_override, double* p, uint32_t* a, oro_psa_debug* dbg);
+                             int split_rule, double mean_override, double* p, uint32_t* a,
+                             oro_psa_debug* dbg);
 
 /* ---- sampling ----------------------------------------------------------- */
 /* alias: src/host/reference/sample_alias_table.hpp:15-41 acceptance rule
