@@ -17,8 +17,12 @@ def make_batch(wrs, oracle, nseg, hi, seed=3):
     return sizes, off, w
 
 
-@pytest.mark.parametrize("precision", [2, 0])
+@pytest.mark.parametrize("precision", [2, 1, 0])
 def test_batched_equals_per_segment(wrs, oracle, precision):
+    """FP64 modes: tables and samples bit-identical to per-segment builds (the
+    carry is exact, so its bits do not depend on look-back timing).  FP32 mode
+    sums its carry in float in look-back order, as the reference GPU does, so
+    only the partition is order-independent there: compare validity instead."""
     nseg, per = 200, 64
     print("case: batched vs per-segment", nseg, precision, flush=True)
     sizes, off, w = make_batch(wrs, oracle, nseg, 1 << 15)
@@ -35,8 +39,18 @@ def test_batched_equals_per_segment(wrs, oracle, precision):
     se = wrs.empty(nseg * per, np.uint32)
     wrs.build_sample_each(plan, dw, off, te, per, 42, se)
     wrs.synchronize()
-    assert np.array_equal(tb.numpy(), te.numpy())
-    assert np.array_equal(sb.numpy(), se.numpy())
+    if precision != 0:
+        assert np.array_equal(tb.numpy(), te.numpy())
+        assert np.array_equal(sb.numpy(), se.numpy())
+        return
+    a, b = tb.numpy(), te.numpy()
+    for g in range(nseg):
+        seg = slice(int(off[g]), int(off[g + 1]))
+        n = int(off[g + 1] - off[g])
+        for t in (a[seg], b[seg]):
+            assert np.all(t["a"] < n)
+            q = oracle.implied_distribution(t["p"], t["a"])
+            assert oracle.distribution_error(q, w[seg])["max_abs"] < 1e-3 / n
 
 
 def test_batched_against_oracle(wrs, oracle):
