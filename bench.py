@@ -2,7 +2,7 @@
 """Benchmark of the weighted-random-sampling hot path on one MI355X.
 
 One step = one PSA alias-table build (FP64 total -> heavy/light prefix-partition
--> fused split+pack) of N = 2^26 Zipf(1.1) FP32 weights (ranks a seeded
+-> block-boundary splits -> merge pack) of N = 2^26 Zipf(1.1) FP32 weights (ranks a seeded
 permutation of 1..N, seed 2), compensated FP32 with the exact FP64 carry,
 split size 32, inputs resident in HBM.  The value is the build throughput in
 GB/s of algorithmic bytes B_build = 32N + 12K (K = N/32): mean read 4N,
@@ -116,16 +116,20 @@ def main() -> int:
 
     # ---- per-kernel durations (events on the same stream) ------------------
     plan.set_timing(True)
-    acc = np.zeros(4)
+    acc = np.zeros(5)
     for _ in range(args.steps):
         plan.psa_build(w, n, table, stream=st)
         acc += np.array(plan.phase_times())
     plan.set_timing(False)
     phase = acc / args.steps
-    names = ["reduce", "finalize", "partition", "split_pack"]
-    alg = {"reduce": 4 * n, "finalize": 0, "partition": 12 * n, "split_pack": 16 * n + 12 * K}
+    names = ["reduce", "finalize", "partition", "coarse_split", "merge_pack"]
+    # bytes each kernel must move: reduce reads w; the partition reads w and
+    # writes the indices (its FP64 prefix samples are 16N/64 more); the merge
+    # pack reads the indices, gathers the weights and writes the 8-byte table
+    alg = {"reduce": 4 * n, "finalize": 0, "partition": 8 * n + 16 * (n // 64),
+           "coarse_split": 0, "merge_pack": 16 * n}
     log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in zip(names, phase)))
-    dom = max(("reduce", "partition", "split_pack"), key=lambda k: phase[names.index(k)])
+    dom = max(("reduce", "partition", "merge_pack"), key=lambda k: phase[names.index(k)])
     dom_ms = float(phase[names.index(dom)])
     dom_gbs = alg[dom] / (dom_ms * 1e-3) / 1e9
 

@@ -60,7 +60,7 @@ typedef struct {
 /* Build configuration.  Reference: PSAConfig (src/device/wrs/alias/psa/PSA.hpp:19-41),
  * whose splitSize sets the split granularity. */
 typedef struct {
-    uint32_t split_size;     /* buckets per split interval (K = ceil(N/split_size)); 0 -> 32 */
+    uint32_t split_size;     /* split granularity of the debug split output (K = ceil(N/split_size)); 0 -> 32 */
     wrs_precision precision; /* default WRS_PREC_COMPENSATED */
 } wrs_config;
 
@@ -75,7 +75,8 @@ wrs_status wrs_plan_create(const wrs_config* cfg, uint64_t max_n, wrs_plan** out
 wrs_status wrs_plan_destroy(wrs_plan* plan);
 
 /* PSA alias-table build: weights[n] (finite, >= 0, not all zero) -> table[n].
- * Three passes: FP64 total, heavy/light prefix-partition, fused split+pack.
+ * Passes: FP64 total, heavy/light prefix-partition, split + pack as a parallel
+ * merge (exact splits at every 2048 buckets, then an LDS merge per block).
  * Replaces device::PSA::run (src/device/wrs/alias/psa/PSA.hpp:170-237) and
  * device::WRS::build for an AliasTable config (src/device/wrs/WRS.hpp:121-143). */
 wrs_status wrs_psa_build(wrs_plan* plan, const float* weights, uint64_t n,
@@ -180,9 +181,12 @@ wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_
 const char* wrs_last_error(void);         /* thread-local message of the last failure */
 const char* wrs_version(void);
 /* Per-phase device times (ms) of the most recent build on this plan when
- * timing is enabled: [0] reduce, [1] finalize, [2] partition, [3] split+pack. */
+ * timing is enabled (HIP events on the build's stream): [0] reduce,
+ * [1] finalize, [2] prefix-partition, [3] block-boundary splits, [4] merge pack.
+ * Replaces the merian profiler scopes around each dispatch
+ * (src/device/wrs/alias/psa/splitpack/inline/InlineSplitPack.hpp:109-150). */
 wrs_status wrs_plan_set_timing(wrs_plan* plan, int enable);
-wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms4);
+wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms5);
 
 #ifdef __cplusplus
 }

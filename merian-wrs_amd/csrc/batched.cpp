@@ -27,70 +27,91 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_
     nseg = nseg_;
     if (prec < 0 || prec > 2) throw wrs::Error(WRS_EINVAL, "unknown precision");
     offsets.assign(offsets_host, offsets_host + nseg + 1);
-    std::vector<uint32_t> h_item_seg, h_item_tile, h_seg_item0(nseg + 1);
-    std::vector<uint64_t> h_seg_pt0(nseg + 1);
-    uint64_t items64 = 0, pts = 0;
+    if (offsets[0] != 0) throw wrs::Error(WRS_EINVAL, "offsets[0] must be 0");
+    std::vector<uint32_t> h_item_seg, h_item_tile, h_mitem_seg, h_mitem_blk;
+    std::vector<uint32_t> h_seg_item0(nseg + 1), h_seg_b0(nseg + 1);
+    std::vector<uint64_t> h_seg_samp0(nseg + 1);
+    uint64_t items64 = 0, bnd = 0, samp = 0;
     for (uint32_t g = 0; g < nseg; ++g) {
         if (offsets[g + 1] <= offsets[g])
             throw wrs::Error(WRS_EINVAL, "segments must be non-empty and offsets increasing");
         const uint64_t n = offsets[g + 1] - offsets[g];
-        if (n >= (1ull << 32)) throw wrs::Error(WRS_EINVAL, "segment too large");
         const uint64_t tiles = wrsk::tiles_for(n);
         if (tiles > wrsk::kMaxPartials)
-            throw wrs::Error(WRS_EINVAL, "segment larger than kMaxPartials tiles");
+            throw wrs::Error(WRS_EINVAL, "segment larger than 2^23 weights");
         h_seg_item0[g] = (uint32_t)items64;
         for (uint64_t t = 0; t < tiles; ++t) {
             h_item_seg.push_back(g);
             h_item_tile.push_back((uint32_t)t);
         }
         items64 += tiles;
-        h_seg_pt0[g] = pts;
-        pts += (n + split_size - 1) / split_size + 1;
+        const uint64_t blocks = wrsk::merge_blocks_for(n);
+        h_seg_b0[g] = (uint32_t)bnd;
+        for (uint64_t b = 0; b < blocks; ++b) {
+            h_mitem_seg.push_back(g);
+            h_mitem_blk.push_back((uint32_t)b);
+        }
+        bnd += blocks + 1;
+        h_seg_samp0[g] = samp;
+        samp += wrsk::samples_for(n);
     }
-    if (items64 >= (1ull << 31)) throw wrs::Error(WRS_EINVAL, "too many tiles");
+    if (items64 >= (1ull << 31) || bnd >= (1ull << 31)) throw wrs::Error(WRS_EINVAL, "too many tiles");
     h_seg_item0[nseg] = (uint32_t)items64;
-    h_seg_pt0[nseg] = pts;
+    h_seg_b0[nseg] = (uint32_t)bnd;
+    h_seg_samp0[nseg] = samp;
     items = (uint32_t)items64;
-    total_points = pts;
-    total_n = offsets[nseg] - offsets[0];
-    if (offsets[0] != 0) throw wrs::Error(WRS_EINVAL, "offsets[0] must be 0");
+    mitems = (uint32_t)h_mitem_seg.size();
+    nbound = (uint32_t)bnd;
+    total_n = offsets[nseg];
 
     seg_off = dalloc<uint64_t>(nseg + 1);
     seg_item0 = dalloc<uint32_t>(nseg + 1);
-    seg_pt0 = dalloc<uint64_t>(nseg + 1);
+    seg_b0 = dalloc<uint32_t>(nseg + 1);
+    seg_samp0 = dalloc<uint64_t>(nseg + 1);
     item_seg = dalloc<uint32_t>(items);
     item_tile = dalloc<uint32_t>(items);
+    mitem_seg = dalloc<uint32_t>(mitems);
+    mitem_blk = dalloc<uint32_t>(mitems);
     seg_prm = dalloc<wrsk::Params>(nseg);
     partials = dalloc<double>(items);
     states = dalloc<uint64_t>((uint64_t)items * wrsk::kStateWords);
     ticket = dalloc<uint32_t>(1);
     idx = dalloc<uint32_t>(total_n);
-    hi = dalloc<float>(total_n);
-    if (prec == WRS_PREC_COMPENSATED) lo = dalloc<float>(total_n);
+    samp_h = dalloc<double>(samp);
+    samp_l = dalloc<double>(samp);
+    bounds = dalloc<wrsk::Coarse>(nbound);
     upload(seg_off, offsets);
     upload(seg_item0, h_seg_item0);
-    upload(seg_pt0, h_seg_pt0);
+    upload(seg_b0, h_seg_b0);
+    upload(seg_samp0, h_seg_samp0);
     upload(item_seg, h_item_seg);
     upload(item_tile, h_item_tile);
+    upload(mitem_seg, h_mitem_seg);
+    upload(mitem_blk, h_mitem_blk);
 }
 
 void BatchPlan::destroy() {
-    void* ptrs[] = {seg_off, seg_item0, seg_pt0, item_seg, item_tile, seg_prm,
-                    partials, states,   ticket,  idx,      hi,        lo};
+    void* ptrs[] = {seg_off,   seg_item0, seg_b0, seg_samp0, item_seg, item_tile, mitem_seg,
+                    mitem_blk, seg_prm,   partials, states,  ticket,   idx,       samp_h,
+                    samp_l,    bounds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     seg_off = nullptr;
     seg_item0 = nullptr;
-    seg_pt0 = nullptr;
+    seg_b0 = nullptr;
+    seg_samp0 = nullptr;
     item_seg = nullptr;
     item_tile = nullptr;
+    mitem_seg = nullptr;
+    mitem_blk = nullptr;
     seg_prm = nullptr;
     partials = nullptr;
     states = nullptr;
     ticket = nullptr;
     idx = nullptr;
-    hi = nullptr;
-    lo = nullptr;
+    samp_h = nullptr;
+    samp_l = nullptr;
+    bounds = nullptr;
 }
 
 void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) const {
@@ -101,11 +122,13 @@ void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) c
                                              ticket, s),
                    "segmented finalize launch");
     wrs::check_hip(wrsk::launch_seg_partition(w, seg_off, item_seg, item_tile, seg_item0, items,
-                                              prec, seg_prm, ticket, states, idx, hi, lo, s),
+                                              prec, seg_prm, ticket, states, idx, nullptr, nullptr,
+                                              samp_h, samp_l, seg_samp0, s),
                    "segmented partition launch");
-    wrs::check_hip(wrsk::launch_seg_splitpack(w, seg_off, seg_pt0, nseg, total_points, split_size,
-                                              prec, seg_prm, idx, hi, lo, tables, s),
-                   "segmented split+pack launch");
+    wrs::check_hip(wrsk::launch_seg_mergepack(w, idx, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
+                                              nseg, nbound, mitem_seg, mitem_blk, mitems, prec,
+                                              seg_prm, bounds, tables, s),
+                   "segmented merge pack launch");
 }
 
 }  // namespace wrsb

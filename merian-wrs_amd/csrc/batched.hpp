@@ -1,7 +1,7 @@
 // batched.hpp -- host side of the segmented (batched) PSA build: thousands of
 // independent distributions of uneven sizes, laid out back to back (CSR
 // offsets), built by four segmented launches whose work lists (segment, tile)
-// and (segment, split point) are derived once from the offsets when the plan
+// and (segment, merge block) are derived once from the offsets when the plan
 // is created.  Each segment is built with exactly the arithmetic and order of a
 // single-distribution build, so the tables are bit-identical to building the
 // segments one by one.
@@ -20,23 +20,28 @@ struct BatchPlan {
     uint32_t split_size = 32;
     int prec = 2;
     uint32_t nseg = 0;
-    uint32_t items = 0;          // (segment, tile) work items
+    uint32_t items = 0;          // (segment, tile) work items of reduce and partition
+    uint32_t mitems = 0;         // (segment, merge block) work items of the merge pack
+    uint32_t nbound = 0;         // block-boundary splits over all segments
     uint64_t total_n = 0;        // weights over all segments
-    uint64_t total_points = 0;   // split points over all segments (sum of K_g + 1)
     std::vector<uint64_t> offsets;  // host copy, nseg + 1
     // device
     uint64_t* seg_off = nullptr;     // [nseg+1]
     uint32_t* seg_item0 = nullptr;   // [nseg+1]
-    uint64_t* seg_pt0 = nullptr;     // [nseg+1]
+    uint32_t* seg_b0 = nullptr;      // [nseg+1] first boundary of each segment
+    uint64_t* seg_samp0 = nullptr;   // [nseg+1] first prefix sample of each segment
     uint32_t* item_seg = nullptr;    // [items]
     uint32_t* item_tile = nullptr;   // [items]
+    uint32_t* mitem_seg = nullptr;   // [mitems]
+    uint32_t* mitem_blk = nullptr;   // [mitems]
     wrsk::Params* seg_prm = nullptr; // [nseg]
     double* partials = nullptr;      // [items]
     uint64_t* states = nullptr;      // [items * kStateWords]
     uint32_t* ticket = nullptr;      // [1]
     uint32_t* idx = nullptr;         // [total_n]
-    float* hi = nullptr;             // [total_n]
-    float* lo = nullptr;             // [total_n] (compensated only)
+    double* samp_h = nullptr;        // [seg_samp0[nseg]]
+    double* samp_l = nullptr;        // [seg_samp0[nseg]]
+    wrsk::Coarse* bounds = nullptr;  // [nbound]
 
     void create(uint32_t split_size, int prec, const uint64_t* offsets_host, uint32_t nseg);
     void destroy();

@@ -25,7 +25,9 @@ struct Params {
     uint32_t ticket;       // dynamic tile ticket of the look-back kernels
     uint32_t heavy_count;  // H, written by the partition's last tile
     double inv_total;      // 1 / total (ITS normalisation)
-    uint64_t reserved[3];
+    double heavy_sum;      // sum of the heavy weights (written by the partition's last tile)
+    double light_sum;      // sum of the light weights (idem)
+    uint64_t reserved[1];
 };
 static_assert(sizeof(Params) == 64, "Params layout");
 
@@ -33,6 +35,15 @@ struct AliasEntry {
     float p;
     uint32_t a;
 };
+
+// Exact split at a block boundary: i lights and j heavies precede bucket n,
+// lp = LP(i), hp = HP(j) (prefix sums of the first i lights / j heavies).
+struct Coarse {
+    uint32_t i, j;
+    double lp, hp;
+    uint64_t pad;
+};
+static_assert(sizeof(Coarse) == 32, "Coarse layout");
 
 struct SplitRec {  // debug output of the split stage
     uint32_t i;
@@ -45,7 +56,12 @@ constexpr int kItems = 16;                  // consecutive elements per thread
 constexpr int kTile = kBlock * kItems;      // 4096 elements per tile
 constexpr int kStateWords = 16;             // 128-byte look-back record per tile
 constexpr uint32_t kMaxPartials = 2048;     // reduce blocks
-constexpr int kSplitBlock = 256;            // split+pack threads per block (255 intervals)
+constexpr int kSplitBlock = 256;            // debug split kernel block size
+constexpr uint32_t kSample = 64;            // partition prefix sample stride (ranks)
+constexpr int kMergeTile = 2048;            // buckets per merge-pack block (multiple of kSample)
+
+inline uint64_t samples_for(uint64_t n) { return n / kSample + 1; }
+inline uint64_t merge_blocks_for(uint64_t n) { return (n + kMergeTile - 1) / kMergeTile; }
 
 inline uint64_t tiles_for(uint64_t n) { return (n + kTile - 1) / kTile; }
 inline uint32_t partials_for(uint64_t n) {
@@ -59,12 +75,20 @@ hipError_t launch_reduce(const float* w, uint64_t n, int prec, double* partials,
 hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int prec,
                            int has_pivot, double pivot, Params* prm, hipStream_t s);
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
-                            uint32_t* idx, float* hi, float* lo, hipStream_t s);
+                            uint32_t* idx, float* hi, float* lo, double* samp_h, double* samp_l,
+                            hipStream_t s);
+// exact splits at every kMergeTile buckets, then the LDS merge-path pack
+hipError_t launch_coarse(const float* w, uint64_t n, int prec, const Params* prm,
+                         const uint32_t* idx, const double* samp_h, const double* samp_l,
+                         Coarse* bounds, hipStream_t s);
+hipError_t launch_mergepack(const float* w, uint64_t n, int prec, const Params* prm,
+                            const uint32_t* idx, const Coarse* bounds, AliasEntry* table,
+                            hipStream_t s);
+// the K split points S(k), k = 1..K, of the reference's splitK layout (debug output)
+hipError_t launch_splits(uint64_t n, uint32_t K, int prec, const Params* prm, const float* hi,
+                         const float* lo, SplitRec* splits, hipStream_t s);
 hipError_t launch_scan(const float* w, uint64_t n, int prec, int normalise, Params* prm,
                        uint64_t* states, float* hi, float* lo, hipStream_t s);
-hipError_t launch_splitpack(const float* w, uint64_t n, uint32_t K, int prec, const Params* prm,
-                            const uint32_t* idx, const float* hi, const float* lo,
-                            AliasEntry* table, SplitRec* splits, hipStream_t s);
 hipError_t launch_alias_sample(const AliasEntry* table, uint64_t n, uint64_t s_count,
                                uint64_t seed, uint64_t counter_base, uint32_t* out, hipStream_t s);
 hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
@@ -88,11 +112,14 @@ hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const u
                                 const uint32_t* item_tile, const uint32_t* seg_item0,
                                 uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
                                 uint64_t* states, uint32_t* idx, float* hi, float* lo,
+                                double* samp_h, double* samp_l, const uint64_t* seg_samp0,
                                 hipStream_t s);
-hipError_t launch_seg_splitpack(const float* w, const uint64_t* seg_off, const uint64_t* seg_pt0,
-                                uint32_t nseg, uint64_t total_points, uint32_t split_size,
-                                int prec, const Params* seg_prm, const uint32_t* idx,
-                                const float* hi, const float* lo, AliasEntry* table,
+hipError_t launch_seg_mergepack(const float* w, const uint32_t* idx, const double* samp_h,
+                                const double* samp_l, const uint64_t* seg_off,
+                                const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
+                                uint32_t nbound, const uint32_t* mitem_seg,
+                                const uint32_t* mitem_blk, uint32_t mitems, int prec,
+                                const Params* seg_prm, Coarse* bounds, AliasEntry* table,
                                 hipStream_t s);
 hipError_t launch_seg_sample(const AliasEntry* tables, const uint64_t* seg_off, uint32_t nseg,
                              uint32_t per_segment, uint64_t seed, uint64_t counter_base,

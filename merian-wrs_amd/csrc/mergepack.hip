@@ -1,0 +1,470 @@
+// mergepack.hip -- PSA split + pack as a parallel merge (pass 3 of a build).
+//
+// The reference packs each split interval with a serial sweep
+// (src/host/reference/pack.hpp:14-83, pack/scalar/float.comp:60-180): with
+// heavy j current and residual r, the next bucket is heavy j itself if
+// r <= mean (p = r/mean, alias = next heavy), else the next light i
+// (p = w_i/mean, alias = heavy j).  Writing LP(i) for the sum of the first i
+// lights and HP(j) for the sum of the first j heavies, the residual in state
+// (i, j) is r = LP(i) + HP(j+1) - (i+j)*mean, and "r <= mean" is
+//
+//     Y(j+1) <= X(i),   X(i) = i*mean - LP(i),   Y(j) = HP(j) - j*mean,
+//
+// with X non-decreasing (lights <= mean) and Y increasing (heavies > mean).
+// So the sweep is the merge of the lights keyed X(i) with the heavies keyed
+// Y(j+1), heavy first on ties -- and the split of the reference GPU kernels
+// (largest j with sigma(n-j, j) <= mean*n, split/scalar/shader.comp:72-109) is
+// exactly the merge-path diagonal of that merge.  This file therefore builds
+// the table as a two-level merge:
+//
+//  k_coarse   one wave per block boundary n = b*kMergeTile: the exact split
+//             (i, j) with LP(i), HP(j).  A 64-ary wave search over the sparse
+//             FP64 prefix samples the partition wrote (every kSample-th rank;
+//             at j = m*kSample the light rank n - j is also a multiple of
+//             kSample, so both prefixes are samples) brackets j to a window of
+//             kSample heavies; the window is resolved exactly by a wave scan of
+//             the 64 gathered heavy and light weights around it.
+//  k_mergepack one 256-thread block per kMergeTile buckets: the block's light
+//             and heavy index ranges come from its two boundary splits; their
+//             indices and gathered weights are staged in LDS, the keys X and Y
+//             are rebuilt by a compensated block scan on top of LP(i0), HP(j0),
+//             every thread finds its merge-path diagonal by a binary search in
+//             LDS and merges 8 buckets, and the entries are written back in
+//             rank order (coalesced: consecutive light ranks are consecutive
+//             ascending indices).  The last heavy of all gets (1, itself), as
+//             in pack.hpp:78-81.
+//
+// In the FP64 modes keys, residuals and split arithmetic are FP64 (the block
+// scan itself is compensated FP32 in kComp, plain FP32 in kFP64Carry); in FP32
+// mode everything is FP32, as in the reference kernels.
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace wrsk {
+using namespace wrsdev;
+
+namespace {
+
+template <int PREC>
+struct RealT {
+    using T = double;
+};
+template <>
+struct RealT<kFP32> {
+    using T = float;
+};
+
+// prefix of the first j heavies / first i lights where j, i are multiples of
+// kSample (samples) or i == L (the light total)
+template <class T>
+__device__ __forceinline__ T hp_sample(const double* __restrict__ samp_h, uint32_t j) {
+    return j == 0 ? (T)0 : (T)samp_h[j / kSample - 1u];
+}
+template <class T>
+__device__ __forceinline__ T lp_sample(const double* __restrict__ samp_l, uint32_t i, uint32_t L,
+                                       T light_sum) {
+    if (i == 0) return (T)0;
+    if (i == L) return light_sum;
+    return (T)samp_l[i / kSample - 1u];
+}
+
+template <class T>
+__device__ __forceinline__ T wave_excl_scan(T v) {
+    const int l = lane_id();
+    T incl = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const T o = __shfl_up(incl, (unsigned)d, kWave);
+        if (l >= d) incl = o + incl;
+    }
+    const T ex = __shfl_up(incl, 1u, kWave);
+    return l == 0 ? (T)0 : ex;
+}
+
+}  // namespace
+
+// The exact split at n buckets of the distribution (w, idx, samples) of size
+// N with H heavies; executed by one full wave, result valid in every lane.
+template <int PREC>
+__device__ __forceinline__ Coarse coarse_split(const float* __restrict__ w,
+                                               const uint32_t* __restrict__ idx,
+                                               const double* __restrict__ samp_h,
+                                               const double* __restrict__ samp_l, uint32_t N,
+                                               uint32_t H, typename RealT<PREC>::T mean,
+                                               typename RealT<PREC>::T light_sum, uint32_t n) {
+    using T = typename RealT<PREC>::T;
+    const uint32_t L = N - H;
+    const int lane = lane_id();
+    if (n == 0 || H == 0 || L == 0) return Coarse{0u, 0u, 0.0, 0.0, 0};
+    if (n >= N) return Coarse{L, H - 1u, (double)light_sum, 0.0, 0};
+    const T target = mean * (T)n;
+    const uint32_t lo = n > L ? n - L : 0u;
+    const uint32_t hi = min(n, H - 1u);
+    // phase 1: largest multiple m*kSample in [lo, hi] with sigma <= target
+    const uint32_t m_lo = (lo + kSample - 1u) / kSample;
+    const uint32_t m_hi = hi / kSample;
+    int64_t best = -1;
+    if (m_lo <= m_hi) {
+        uint32_t a = m_lo, b = m_hi;
+        for (;;) {
+            const uint32_t R = b - a + 1u;
+            const uint32_t step = (R + kWave - 1u) / kWave;
+            const uint32_t m = a + (uint32_t)lane * step;
+            bool pred = false;
+            if (m <= b) {
+                const uint32_t j = m * kSample;
+                const uint32_t i = min(n - j, L);
+                pred = lp_sample<T>(samp_l, i, L, light_sum) + hp_sample<T>(samp_h, j) <= target;
+            }
+            const uint64_t bal = __ballot(pred);
+            if (bal == 0) break;
+            const int t = 63 - __builtin_clzll(bal);
+            best = (int64_t)a + (int64_t)t * step;
+            if (step == 1u) break;
+            a = (uint32_t)best + 1u;
+            b = min(b, (uint32_t)best + step - 1u);
+            if (a > b) break;
+        }
+    }
+    const uint32_t base = best >= 0 ? (uint32_t)best * kSample : (lo / kSample) * kSample;
+    // phase 2: resolve j in [base, base + 63] with the gathered weights
+    const uint32_t j = base + (uint32_t)lane;
+    const T hw = j < H ? (T)w[idx[j]] : (T)0;
+    const T hp = hp_sample<T>(samp_h, base) + wave_excl_scan<T>(hw);  // HP(base + lane)
+    const uint32_t i_top = n - base;                                   // multiple of kSample
+    const uint32_t anchor = min(i_top, L);
+    const T lp_anchor = lp_sample<T>(samp_l, anchor, L, light_sum);
+    const int64_t lrank = (int64_t)anchor - 1 - lane;                  // lane e: light rank anchor-1-e
+    const T lw = lrank >= 0 ? (T)w[idx[N - 1u - (uint32_t)lrank]] : (T)0;
+    const T lsuf = wave_excl_scan<T>(lw);  // sum of light ranks [anchor - e, anchor)
+    const int64_t i = (int64_t)n - (int64_t)j;
+    // every lane takes part in the shuffle; e is clamped for lanes that do not use it
+    int64_t e64 = (int64_t)anchor - i;  // 0..63 where used
+    e64 = e64 < 0 ? 0 : (e64 > kWave - 1 ? kWave - 1 : e64);
+    const T suf = __shfl(lsuf, (int)e64, kWave);
+    bool pred;
+    T lp = (T)0;
+    if (j > hi) {
+        pred = false;
+    } else if (j < lo || i > (int64_t)anchor) {
+        pred = true;
+    } else {
+        lp = lp_anchor - suf;
+        pred = lp + hp <= target;
+    }
+    const uint64_t bal = __ballot(pred);
+    int t = bal ? 63 - __builtin_clzll(bal) : (int)(lo - base);
+    if ((uint32_t)t + base < lo) t = (int)(lo - base);
+    const T lp_t = __shfl(lp, t, kWave);
+    const T hp_t = __shfl(hp, t, kWave);
+    const uint32_t js = base + (uint32_t)t;
+    return Coarse{n - js, js, (double)lp_t, (double)hp_t, 0};
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ w,
+                                                const uint32_t* __restrict__ idx,
+                                                const double* __restrict__ samp_h,
+                                                const double* __restrict__ samp_l, uint32_t N,
+                                                const Params* __restrict__ prm, uint32_t nbound,
+                                                Coarse* __restrict__ out) {
+    using T = typename RealT<PREC>::T;
+    const uint32_t b = blockIdx.x * 4u + threadIdx.x / kWave;
+    if (b >= nbound) return;  // whole wave exits together
+    const uint32_t n = min(b * (uint32_t)kMergeTile, N);
+    const Coarse c = coarse_split<PREC>(w, idx, samp_h, samp_l, N, prm->heavy_count,
+                                        (T)prm->mean, (T)prm->light_sum, n);
+    if (lane_id() == 0) out[b] = c;
+}
+
+// Segmented form: boundary p of the flattened boundary list belongs to
+// segment g = item_seg-like lookup over seg_b0 (nseg + 1 prefix).
+template <int PREC>
+__global__ __launch_bounds__(256) void k_seg_coarse(
+    const float* __restrict__ w, const uint32_t* __restrict__ idx, const double* __restrict__ samp_h,
+    const double* __restrict__ samp_l, const uint64_t* __restrict__ seg_off,
+    const uint64_t* __restrict__ seg_samp0, const uint32_t* __restrict__ seg_b0, uint32_t nseg,
+    const Params* __restrict__ seg_prm, Coarse* __restrict__ out) {
+    using T = typename RealT<PREC>::T;
+    const uint32_t p = blockIdx.x * 4u + threadIdx.x / kWave;
+    if (p >= seg_b0[nseg]) return;
+    uint32_t a = 0, b = nseg;  // last g with seg_b0[g] <= p
+    while (b - a > 1u) {
+        const uint32_t mid = (a + b) >> 1;
+        if (seg_b0[mid] <= p) a = mid;
+        else b = mid;
+    }
+    const uint32_t g = a;
+    const uint64_t off = seg_off[g];
+    const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
+    const uint32_t n = min((p - seg_b0[g]) * (uint32_t)kMergeTile, N);
+    const uint64_t so = seg_samp0[g];
+    const Coarse c = coarse_split<PREC>(w + off, idx + off, samp_h + so, samp_l + so, N,
+                                        seg_prm[g].heavy_count, (T)seg_prm[g].mean,
+                                        (T)seg_prm[g].light_sum, n);
+    if (lane_id() == 0) out[p] = c;
+}
+
+// One block of kMergeTile buckets [n0, n1) between boundary splits c0 and c1.
+template <int PREC>
+__device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
+                                                const uint32_t* __restrict__ idx, uint32_t N,
+                                                uint32_t H, typename RealT<PREC>::T mean,
+                                                uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
+                                                AliasEntry* __restrict__ table) {
+    using T = typename RealT<PREC>::T;
+    constexpr int kPer = kMergeTile / kBlock;  // buckets (and staged elements) per thread
+    __shared__ T s_key[kMergeTile + 2];
+    __shared__ uint32_t s_idx[kMergeTile + 2];
+    __shared__ float s_w[kMergeTile];
+    __shared__ uint2 s_ent[kMergeTile];
+    __shared__ DF s_wave_l[kBlock / kWave], s_wave_h[kBlock / kWave];
+
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wid = t / kWave;
+    const uint32_t L = N - H;
+    if (H == 0 || L == 0) {  // all weights equal the mean up to its rounding: identity
+        for (uint32_t x = n0 + t; x < n1; x += kBlock) table[x] = AliasEntry{1.0f, x};
+        return;
+    }
+    const bool last = n1 == N;
+    const uint32_t i0 = c0.i, j0 = c0.j;
+    const uint32_t nl = c1.i - i0;  // lights packed here
+    const uint32_t nh = c1.j - j0;  // heavies packed here (heavy c1.j is current at n1)
+    const uint32_t ne = nl + nh;    // == n1 - n0 (- 1 in the last block)
+
+    // ---- stage indices and weights in rank order -----------------------
+    for (uint32_t r = t; r < nl; r += kBlock) {
+        const uint32_t li = idx[N - 1u - (i0 + r)];
+        s_idx[r] = li;
+        s_w[r] = w[li];
+    }
+    for (uint32_t q = t; q <= nh; q += kBlock) {
+        const uint32_t hi = idx[j0 + q];
+        s_idx[nl + q] = hi;
+        if (q < nh) s_w[nl + q] = w[hi];
+    }
+    __syncthreads();
+
+    // ---- keys: light r -> X(i0 + r) (exclusive LP), heavy q -> Y(j0 + q + 1)
+    DF ml{0.0f, 0.0f}, mh{0.0f, 0.0f};
+    const uint32_t e0 = (uint32_t)(kPer * t);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t e = e0 + k;
+        if (e < ne) {
+            const float x = s_w[e];
+            if constexpr (PREC == kComp) {
+                if (e < nl) ml = df_add1(ml, x);
+                else mh = df_add1(mh, x);
+            } else {
+                if (e < nl) ml.hi += x;
+                else mh.hi += x;
+            }
+        }
+    }
+    DF il = ml, ih = mh;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        DF ol, oh;
+        ol.hi = shfl_up(il.hi, d);
+        oh.hi = shfl_up(ih.hi, d);
+        ol.lo = shfl_up(il.lo, d);
+        oh.lo = shfl_up(ih.lo, d);
+        if (lane >= d) {
+            if constexpr (PREC == kComp) {
+                il = df_add(ol, il);
+                ih = df_add(oh, ih);
+            } else {
+                il.hi = ol.hi + il.hi;
+                ih.hi = oh.hi + ih.hi;
+            }
+        }
+    }
+    DF xl, xh;  // exclusive per lane
+    xl.hi = shfl_up(il.hi, 1);
+    xl.lo = shfl_up(il.lo, 1);
+    xh.hi = shfl_up(ih.hi, 1);
+    xh.lo = shfl_up(ih.lo, 1);
+    if (lane == 0) xl = xh = DF{0.0f, 0.0f};
+    if (lane == kWave - 1) {
+        s_wave_l[wid] = il;
+        s_wave_h[wid] = ih;
+    }
+    __syncthreads();
+    DF rl{0.0f, 0.0f}, rh{0.0f, 0.0f}, tot_l{0.0f, 0.0f};
+    for (int q = 0; q < kBlock / kWave; ++q) {
+        if constexpr (PREC == kComp) {
+            if (q < wid) {
+                rl = df_add(rl, s_wave_l[q]);
+                rh = df_add(rh, s_wave_h[q]);
+            }
+            tot_l = df_add(tot_l, s_wave_l[q]);
+        } else {
+            if (q < wid) {
+                rl.hi += s_wave_l[q].hi;
+                rh.hi += s_wave_h[q].hi;
+            }
+            tot_l.hi += s_wave_l[q].hi;
+        }
+    }
+    if constexpr (PREC == kComp) {
+        rl = df_add(rl, xl);
+        rh = df_add(rh, xh);
+    } else {
+        rl.hi += xl.hi;
+        rh.hi += xh.hi;
+    }
+    const T lp0 = (T)c0.lp, hp0 = (T)c0.hp;
+    auto val = [&](DF v) -> T {
+        if constexpr (PREC == kComp) return (T)df_value(v);
+        else return (T)v.hi;
+    };
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t e = e0 + k;
+        if (e < ne) {
+            const float x = s_w[e];
+            if (e < nl) {  // exclusive prefix, then add
+                s_key[e] = (T)(i0 + e) * mean - (lp0 + val(rl));
+                if constexpr (PREC == kComp) rl = df_add1(rl, x);
+                else rl.hi += x;
+            } else {       // inclusive prefix
+                if constexpr (PREC == kComp) rh = df_add1(rh, x);
+                else rh.hi += x;
+                const uint32_t q = e - nl;
+                s_key[nl + 1u + q] = (hp0 + val(rh)) - (T)(j0 + q + 1u) * mean;
+            }
+        }
+    }
+    if (t == 0) s_key[nl] = (T)(i0 + nl) * mean - (lp0 + val(tot_l));  // X(i1)
+    __syncthreads();
+
+    // ---- merge: thread t emits buckets [kPer*t, kPer*t + kPer) ----------
+    const T* A = s_key;            // A[r], r in [0, nl]
+    const T* B = s_key + nl + 1u;  // B[q], q in [0, nh)
+    const uint32_t d0 = e0;
+    if (d0 < ne) {
+        // largest q in [max(0, d0 - nl), min(d0, nh)] with q == 0 or
+        // B[q-1] <= A[d0 - q] (A[nl] counts as +inf here)
+        int64_t qa = d0 > nl ? (int64_t)(d0 - nl) : 0;
+        int64_t qb = min(d0, nh);
+        while (qa < qb) {
+            const int64_t q = (qa + qb + 1) / 2;
+            const uint32_t r = d0 - (uint32_t)q;
+            const bool ok = r >= nl || B[q - 1] <= A[r];
+            if (ok) qa = q;
+            else qb = q - 1;
+        }
+        uint32_t q = (uint32_t)qa;
+        uint32_t r = d0 - q;
+        const uint32_t d1 = min(d0 + (uint32_t)kPer, ne);
+        for (uint32_t d = d0; d < d1; ++d) {
+            const bool take_heavy = q < nh && (r >= nl || B[q] <= A[r]);
+            if (take_heavy) {
+                const T resid = (B[q] - A[r]) + mean;
+                const float p = (float)(resid / mean);
+                s_ent[nl + q] = make_uint2(__float_as_uint(p), s_idx[nl + q + 1u]);
+                ++q;
+            } else {
+                const float p = (float)((T)s_w[r] / mean);
+                s_ent[r] = make_uint2(__float_as_uint(p), s_idx[nl + q]);
+                ++r;
+            }
+        }
+    }
+    __syncthreads();
+    uint2* tab = reinterpret_cast<uint2*>(table);
+    for (uint32_t e = t; e < ne; e += kBlock) tab[s_idx[e]] = s_ent[e];
+    if (last && t == 0) {
+        const uint32_t h = s_idx[nl + nh];  // heavy H-1
+        table[h] = AliasEntry{1.0f, h};
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_mergepack(const float* __restrict__ w,
+                                                      const uint32_t* __restrict__ idx, uint32_t N,
+                                                      const Params* __restrict__ prm,
+                                                      const Coarse* __restrict__ bounds,
+                                                      AliasEntry* __restrict__ table) {
+    using T = typename RealT<PREC>::T;
+    const uint32_t b = blockIdx.x;
+    const uint32_t n0 = b * (uint32_t)kMergeTile;
+    const uint32_t n1 = min(n0 + (uint32_t)kMergeTile, N);
+    mergepack_block<PREC>(w, idx, N, prm->heavy_count, (T)prm->mean, n0, n1, bounds[b],
+                          bounds[b + 1], table);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_seg_mergepack(
+    const float* __restrict__ w, const uint32_t* __restrict__ idx,
+    const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ mitem_seg,
+    const uint32_t* __restrict__ mitem_blk, const uint32_t* __restrict__ seg_b0,
+    const Params* __restrict__ seg_prm, const Coarse* __restrict__ bounds,
+    AliasEntry* __restrict__ table) {
+    using T = typename RealT<PREC>::T;
+    const uint32_t g = mitem_seg[blockIdx.x];
+    const uint32_t blk = mitem_blk[blockIdx.x];
+    const uint64_t off = seg_off[g];
+    const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
+    const uint32_t n0 = blk * (uint32_t)kMergeTile;
+    const uint32_t n1 = min(n0 + (uint32_t)kMergeTile, N);
+    const Coarse* bd = bounds + seg_b0[g] + blk;
+    mergepack_block<PREC>(w + off, idx + off, N, seg_prm[g].heavy_count, (T)seg_prm[g].mean, n0,
+                          n1, bd[0], bd[1], table + off);
+}
+
+hipError_t launch_coarse(const float* w, uint64_t n, int prec, const Params* prm,
+                         const uint32_t* idx, const double* samp_h, const double* samp_l,
+                         Coarse* bounds, hipStream_t s) {
+    const uint32_t N = (uint32_t)n;
+    const uint32_t nbound = (uint32_t)merge_blocks_for(n) + 1u;
+    const uint32_t cgrid = (nbound + 3u) / 4u;
+    if (prec == kComp)
+        k_coarse<kComp><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, N, prm, nbound, bounds);
+    else if (prec == kFP64Carry)
+        k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, N, prm, nbound, bounds);
+    else
+        k_coarse<kFP32><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, N, prm, nbound, bounds);
+    return hipGetLastError();
+}
+
+hipError_t launch_mergepack(const float* w, uint64_t n, int prec, const Params* prm,
+                            const uint32_t* idx, const Coarse* bounds, AliasEntry* table,
+                            hipStream_t s) {
+    const uint32_t N = (uint32_t)n;
+    const uint32_t blocks = (uint32_t)merge_blocks_for(n);
+    if (prec == kComp)
+        k_mergepack<kComp><<<blocks, kBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+    else if (prec == kFP64Carry)
+        k_mergepack<kFP64Carry><<<blocks, kBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+    else
+        k_mergepack<kFP32><<<blocks, kBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_mergepack(const float* w, const uint32_t* idx, const double* samp_h,
+                                const double* samp_l, const uint64_t* seg_off,
+                                const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
+                                uint32_t nbound, const uint32_t* mitem_seg,
+                                const uint32_t* mitem_blk, uint32_t mitems, int prec,
+                                const Params* seg_prm, Coarse* bounds, AliasEntry* table,
+                                hipStream_t s) {
+    const uint32_t cgrid = (nbound + 3u) / 4u;
+#define WRS_LAUNCH_SMP(P)                                                                       \
+    do {                                                                                        \
+        k_seg_coarse<P><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, seg_off, seg_samp0,      \
+                                              seg_b0, nseg, seg_prm, bounds);                   \
+        k_seg_mergepack<P><<<mitems, kBlock, 0, s>>>(w, idx, seg_off, mitem_seg, mitem_blk,     \
+                                                     seg_b0, seg_prm, bounds, table);           \
+    } while (0)
+    if (prec == kComp) WRS_LAUNCH_SMP(kComp);
+    else if (prec == kFP64Carry) WRS_LAUNCH_SMP(kFP64Carry);
+    else WRS_LAUNCH_SMP(kFP32);
+#undef WRS_LAUNCH_SMP
+    return hipGetLastError();
+}
+
+}  // namespace wrsk

@@ -147,10 +147,12 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
                                                uint64_t* __restrict__ states,
                                                uint32_t* __restrict__ idx_out,
                                                float* __restrict__ hi_out,
-                                               float* __restrict__ lo_out) {
+                                               float* __restrict__ lo_out,
+                                               double* __restrict__ samp_h,
+                                               double* __restrict__ samp_l) {
     __shared__ float s_buf[kPad];
     __shared__ Acc<PREC> s_wave[kBlock / kWave];
-    __shared__ double s_carry_h, s_carry_l;
+    __shared__ double s_carry_h, s_carry_l, s_tot_h, s_tot_l;
     __shared__ float s_carry_hf, s_carry_lf;
     __shared__ uint32_t s_carry_c;
 
@@ -274,13 +276,18 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
                 }
             }
             if (lane == 0) {
-                publish(rec + 8, carry_add<PREC>(excl, agg));
+                const Carry inc = carry_add<PREC>(excl, agg);
+                publish(rec + 8, inc);
                 if constexpr (PREC == kFP32) {
                     s_carry_hf = __uint_as_float((uint32_t)excl.h.lo);
                     s_carry_lf = __uint_as_float((uint32_t)excl.l.lo);
+                    s_tot_h = (double)__uint_as_float((uint32_t)inc.h.lo);
+                    s_tot_l = (double)__uint_as_float((uint32_t)inc.l.lo);
                 } else {
                     s_carry_h = fx_to_double(excl.h, shift);
                     s_carry_l = fx_to_double(excl.l, shift);
+                    s_tot_h = fx_to_double(inc.h, shift);
+                    s_tot_l = fx_to_double(inc.l, shift);
                 }
                 s_carry_c = (uint32_t)excl.c;
             }
@@ -291,6 +298,13 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
             s_carry_hf = 0.0f;
             s_carry_lf = 0.0f;
             s_carry_c = 0;
+            if constexpr (PREC == kFP32) {
+                s_tot_h = (double)__uint_as_float((uint32_t)agg.h.lo);
+                s_tot_l = (double)__uint_as_float((uint32_t)agg.l.lo);
+            } else {
+                s_tot_h = fx_to_double(agg.h, shift);
+                s_tot_l = fx_to_double(agg.l, shift);
+            }
         }
     }
     __syncthreads();
@@ -323,49 +337,70 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
     for (uint32_t s = t; s < m; s += kBlock) idx_out[out_pos(s)] = s_u[s];
     __syncthreads();
 
-    // round B: prefix (hi word); the lo word is kept for round C
-    float lo_keep[kItems];
+    // round B: the inclusive prefix of every element (FP64 in the FP64 modes).
+    // Ranks r with (r+1) % kSample == 0 publish their prefix as a sample; the
+    // full hi (and lo) prefix arrays are written only when requested.
+    float hi_keep[kItems], lo_keep[kItems];
     {
         DF rh = thr_excl.h, rl = thr_excl.l;
         const double ch = s_carry_h, cl = s_carry_l;
         const float chf = s_carry_hf, clf = s_carry_lf;
+        uint32_t hr = Hx + thr_excl.c;                    // global heavy rank of the next heavy
+        uint32_t lr = Lx + (e0 - thr_excl.c);             // global light rank of the next light
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const bool heavy = v[k] > pivot;
+            double P;
             float hi, lo = 0.0f;
             if constexpr (PREC == kComp) {
                 if (heavy) rh = df_add1(rh, v[k]);
                 else rl = df_add1(rl, v[k]);
-                const double P = (heavy ? ch : cl) + df_value(heavy ? rh : rl);
+                P = (heavy ? ch : cl) + df_value(heavy ? rh : rl);
                 hi = (float)P;
                 lo = (float)(P - (double)hi);
             } else if constexpr (PREC == kFP64Carry) {
                 if (heavy) rh.hi += v[k];
                 else rl.hi += v[k];
-                const double P = (heavy ? ch : cl) + (double)(heavy ? rh.hi : rl.hi);
+                P = (heavy ? ch : cl) + (double)(heavy ? rh.hi : rl.hi);
                 hi = (float)P;
             } else {
                 if (heavy) rh.hi += v[k];
                 else rl.hi += v[k];
                 hi = (heavy ? chf : clf) + (heavy ? rh.hi : rl.hi);
+                P = (double)hi;
             }
+            hi_keep[k] = hi;
             lo_keep[k] = lo;
-            if (e0 + k < m) s_buf[slot[k]] = hi;
+            if (e0 + k < m && samp_h != nullptr) {
+                const uint32_t r1 = (heavy ? hr : lr) + 1u;  // prefix covers ranks < r1
+                if ((r1 & (kSample - 1)) == 0) (heavy ? samp_h : samp_l)[r1 / kSample - 1u] = P;
+            }
+            hr += heavy ? 1u : 0u;
+            lr += heavy ? 0u : 1u;
         }
     }
-    __syncthreads();
-    for (uint32_t s = t; s < m; s += kBlock) hi_out[out_pos(s)] = s_buf[s];
-    if constexpr (PREC == kComp) {
-        if (lo_out != nullptr) {
-            __syncthreads();
+    if (hi_out != nullptr) {
 #pragma unroll
-            for (int k = 0; k < kItems; ++k)
-                if (e0 + k < m) s_buf[slot[k]] = lo_keep[k];
-            __syncthreads();
-            for (uint32_t s = t; s < m; s += kBlock) lo_out[out_pos(s)] = s_buf[s];
+        for (int k = 0; k < kItems; ++k)
+            if (e0 + k < m) s_buf[slot[k]] = hi_keep[k];
+        __syncthreads();
+        for (uint32_t s = t; s < m; s += kBlock) hi_out[out_pos(s)] = s_buf[s];
+        if constexpr (PREC == kComp) {
+            if (lo_out != nullptr) {
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < kItems; ++k)
+                    if (e0 + k < m) s_buf[slot[k]] = lo_keep[k];
+                __syncthreads();
+                for (uint32_t s = t; s < m; s += kBlock) lo_out[out_pos(s)] = s_buf[s];
+            }
         }
     }
-    if (t == 0 && base + m == n) prm->heavy_count = Hx + hc;
+    if (t == 0 && base + m == n) {
+        prm->heavy_count = Hx + hc;
+        prm->heavy_sum = s_tot_h;
+        prm->light_sum = s_tot_l;
+    }
 }
 
 template <int PREC, bool VEC>
@@ -374,11 +409,13 @@ __global__ __launch_bounds__(kBlock) void k_partition(const float* __restrict__ 
                                                       uint64_t* __restrict__ states,
                                                       uint32_t* __restrict__ idx_out,
                                                       float* __restrict__ hi_out,
-                                                      float* __restrict__ lo_out) {
+                                                      float* __restrict__ lo_out,
+                                                      double* __restrict__ samp_h,
+                                                      double* __restrict__ samp_l) {
     __shared__ uint32_t s_tile;
     if (threadIdx.x == 0) s_tile = atomicAdd(&prm->ticket, 1u);
     __syncthreads();
-    partition_tile<PREC, VEC>(w, n, s_tile, prm, states, idx_out, hi_out, lo_out);
+    partition_tile<PREC, VEC>(w, n, s_tile, prm, states, idx_out, hi_out, lo_out, samp_h, samp_l);
 }
 
 // Segmented form: one work item per (segment, tile), items of a segment are
@@ -391,7 +428,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile,
     const uint32_t* __restrict__ seg_item0, Params* __restrict__ seg_prm,
     uint32_t* __restrict__ ticket, uint64_t* __restrict__ states, uint32_t* __restrict__ idx,
-    float* __restrict__ hi, float* __restrict__ lo) {
+    float* __restrict__ hi, float* __restrict__ lo, double* __restrict__ samp_h,
+    double* __restrict__ samp_l, const uint64_t* __restrict__ seg_samp0) {
     __shared__ uint32_t s_item;
     if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
     __syncthreads();
@@ -399,18 +437,21 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     const uint32_t sg = item_seg[item];
     const uint64_t off = seg_off[sg];
     const uint32_t n = (uint32_t)(seg_off[sg + 1] - off);
+    const uint64_t so = seg_samp0[sg];
     partition_tile<PREC, false>(w + off, n, item_tile[item], seg_prm + sg,
                                 states + (uint64_t)seg_item0[sg] * kStateWords, idx + off,
-                                hi + off, lo ? lo + off : nullptr);
+                                hi ? hi + off : nullptr, lo ? lo + off : nullptr, samp_h + so,
+                                samp_l + so);
 }
 
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
-                            uint32_t* idx, float* hi, float* lo, hipStream_t s) {
+                            uint32_t* idx, float* hi, float* lo, double* samp_h, double* samp_l,
+                            hipStream_t s) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
     const bool vec = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     const uint32_t nn = (uint32_t)n;
 #define WRS_LAUNCH_PART(P, V) \
-    k_partition<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, hi, lo)
+    k_partition<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, hi, lo, samp_h, samp_l)
     if (prec == kComp) {
         if (vec) WRS_LAUNCH_PART(kComp, true);
         else WRS_LAUNCH_PART(kComp, false);
@@ -430,10 +471,12 @@ hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const u
                                 const uint32_t* item_tile, const uint32_t* seg_item0,
                                 uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
                                 uint64_t* states, uint32_t* idx, float* hi, float* lo,
+                                double* samp_h, double* samp_l, const uint64_t* seg_samp0,
                                 hipStream_t s) {
 #define WRS_LAUNCH_SEG(P)                                                                 \
     k_seg_partition<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_item0, \
-                                                seg_prm, ticket, states, idx, hi, lo)
+                                                seg_prm, ticket, states, idx, hi, lo,       \
+                                                samp_h, samp_l, seg_samp0)
     if (prec == kComp) WRS_LAUNCH_SEG(kComp);
     else if (prec == kFP64Carry) WRS_LAUNCH_SEG(kFP64Carry);
     else WRS_LAUNCH_SEG(kFP32);

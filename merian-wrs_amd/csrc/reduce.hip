@@ -104,7 +104,9 @@ __device__ __forceinline__ void finalize_block(const double* __restrict__ partia
         }
         p.ticket = 0;
         p.heavy_count = 0;
-        p.reserved[0] = p.reserved[1] = p.reserved[2] = 0;
+        p.heavy_sum = 0.0;
+        p.light_sum = 0.0;
+        p.reserved[0] = 0;
         *prm = p;
     }
 }

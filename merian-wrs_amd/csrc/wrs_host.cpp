@@ -48,6 +48,9 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
             ws.indices = dev_alloc<uint32_t>(max_n);
             ws.prefix_hi = dev_alloc<float>(max_n);
             if (lo_buffer) ws.prefix_lo = dev_alloc<float>(max_n);
+            ws.samp_h = dev_alloc<double>(wrsk::samples_for(max_n));
+            ws.samp_l = dev_alloc<double>(wrsk::samples_for(max_n));
+            ws.bounds = dev_alloc<wrsk::Coarse>(wrsk::merge_blocks_for(max_n) + 2);
         }
     } catch (...) {
         ws.release();
@@ -57,7 +60,7 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
 }
 
 void Workspace::release() {
-    void* ptrs[] = {params, partials, states, indices, prefix_hi, prefix_lo};
+    void* ptrs[] = {params, partials, states, indices, prefix_hi, prefix_lo, samp_h, samp_l, bounds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     params = nullptr;
@@ -66,6 +69,9 @@ void Workspace::release() {
     indices = nullptr;
     prefix_hi = nullptr;
     prefix_lo = nullptr;
+    samp_h = nullptr;
+    samp_l = nullptr;
+    bounds = nullptr;
 }
 
 static void require(bool cond, const char* msg) {
@@ -91,7 +97,7 @@ void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace
     check_hip(wrsk::launch_partition(b.elements, N, (int)m_prec, ws.params, ws.states,
                                      b.partitionIndices, b.partitionPrefix,
                                      m_prec == WRS_PREC_COMPENSATED ? b.partitionPrefixLo : nullptr,
-                                     s),
+                                     nullptr, nullptr, s),
               "partition launch");
 }
 
@@ -110,10 +116,13 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     require(b.weights && b.aliasTable && b.internals, "null PSA buffer");
     Workspace& ws = *b.internals;
     require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
-    require(ws.indices && ws.prefix_hi, "workspace has no partition buffers");
+    require(ws.indices && ws.samp_h && ws.bounds, "workspace has no partition buffers");
     const wrs_precision prec = m_cfg.precision;
-    float* lo = prec == WRS_PREC_COMPENSATED ? ws.prefix_lo : nullptr;
-    const uint32_t K = splitCount(N);
+    // full prefix arrays only for a debug build (parity tests); the build
+    // itself needs the indices and the sparse prefix samples
+    const bool full_prefix = dbg && (dbg->prefix_hi || dbg->prefix_lo || dbg->splits);
+    float* hi = full_prefix ? ws.prefix_hi : nullptr;
+    float* lo = full_prefix && prec == WRS_PREC_COMPENSATED ? ws.prefix_lo : nullptr;
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::tiles_for(N) * wrsk::kStateWords;
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
@@ -123,18 +132,24 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     check_hip(wrsk::launch_finalize(ws.partials, G, N, (int)prec, 0, 0.0, ws.params, s),
               "finalize launch");
     if (ev) check_hip(hipEventRecord(ev[2], s), "event");
-    check_hip(wrsk::launch_partition(b.weights, N, (int)prec, ws.params, ws.states, ws.indices,
-                                     ws.prefix_hi, lo, s),
+    check_hip(wrsk::launch_partition(b.weights, N, (int)prec, ws.params, ws.states, ws.indices, hi,
+                                     lo, ws.samp_h, ws.samp_l, s),
               "partition launch");
     if (ev) check_hip(hipEventRecord(ev[3], s), "event");
-    check_hip(wrsk::launch_splitpack(b.weights, N, K, (int)prec, ws.params, ws.indices,
-                                     ws.prefix_hi, lo,
-                                     reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable),
-                                     dbg ? reinterpret_cast<wrsk::SplitRec*>(dbg->splits) : nullptr,
-                                     s),
-              "split+pack launch");
+    check_hip(wrsk::launch_coarse(b.weights, N, (int)prec, ws.params, ws.indices, ws.samp_h,
+                                  ws.samp_l, ws.bounds, s),
+              "coarse split launch");
     if (ev) check_hip(hipEventRecord(ev[4], s), "event");
+    check_hip(wrsk::launch_mergepack(b.weights, N, (int)prec, ws.params, ws.indices, ws.bounds,
+                                     reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable), s),
+              "merge pack launch");
+    if (ev) check_hip(hipEventRecord(ev[5], s), "event");
     if (dbg) {
+        if (dbg->splits)
+            check_hip(wrsk::launch_splits(N, splitCount(N), (int)prec, ws.params, ws.prefix_hi,
+                                          prec == WRS_PREC_COMPENSATED ? ws.prefix_lo : nullptr,
+                                          reinterpret_cast<wrsk::SplitRec*>(dbg->splits), s),
+                      "splits launch");
         if (dbg->mean)
             check_hip(hipMemcpyAsync(dbg->mean, &ws.params->mean, sizeof(double),
                                      hipMemcpyDeviceToDevice, s), "copy mean");
@@ -214,7 +229,7 @@ struct wrs_plan {
     wrs::PSAConfig cfg;
     wrs::Workspace ws;
     bool timing = false;
-    hipEvent_t ev[5] = {};
+    hipEvent_t ev[6] = {};  // brackets reduce, finalize, partition, coarse splits, merge pack
 };
 
 namespace {
@@ -463,13 +478,13 @@ wrs_status wrs_plan_set_timing(wrs_plan* plan, int enable) {
     });
 }
 
-wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms4) {
+wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms5) {
     return guarded([&] {
         require_plan(plan);
-        if (!ms4) throw wrs::Error(WRS_EINVAL, "ms4 is null");
-        wrs::check_hip(hipEventSynchronize(plan->ev[4]), "event sync");
-        for (int k = 0; k < 4; ++k)
-            wrs::check_hip(hipEventElapsedTime(&ms4[k], plan->ev[k], plan->ev[k + 1]),
+        if (!ms5) throw wrs::Error(WRS_EINVAL, "ms5 is null");
+        wrs::check_hip(hipEventSynchronize(plan->ev[5]), "event sync");
+        for (int k = 0; k < 5; ++k)
+            wrs::check_hip(hipEventElapsedTime(&ms5[k], plan->ev[k], plan->ev[k + 1]),
                            "event elapsed");
     });
 }

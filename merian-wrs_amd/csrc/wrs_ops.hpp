@@ -45,8 +45,11 @@ struct Workspace {
     uint64_t* states = nullptr;        // [tiles * kStateWords]
     uint64_t state_words = 0;
     uint32_t* indices = nullptr;       // [max_n] (PSA only)
-    float* prefix_hi = nullptr;        // [max_n] (PSA only)
-    float* prefix_lo = nullptr;        // [max_n] (PSA only, compensated)
+    float* prefix_hi = nullptr;        // [max_n] (PSA debug builds only)
+    float* prefix_lo = nullptr;        // [max_n] (PSA debug builds only, compensated)
+    double* samp_h = nullptr;          // [max_n/kSample + 1] heavy prefix samples (PSA only)
+    double* samp_l = nullptr;          // [max_n/kSample + 1] light prefix samples (PSA only)
+    wrsk::Coarse* bounds = nullptr;    // [max_n/kMergeTile + 2] block-boundary splits (PSA only)
 
     static Workspace allocate(uint64_t max_n, bool partition_buffers, bool lo_buffer);
     void release();
@@ -113,7 +116,8 @@ class PSA {
         const uint64_t K = (N + m_cfg.splitSize - 1) / m_cfg.splitSize;
         return (uint32_t)(K ? K : 1);
     }
-    // events (optional, 5 entries) bracket the four passes for phase timing.
+    // events (optional, 6 entries) bracket the passes for phase timing:
+    // reduce, finalize, partition, coarse splits, merge pack.
     void run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* dbg = nullptr,
              hipEvent_t* events = nullptr) const;
     const PSAConfig& config() const { return m_cfg; }

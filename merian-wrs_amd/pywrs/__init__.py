@@ -302,7 +302,8 @@ class Plan:
         check(lib().wrs_plan_set_timing(self.handle, int(bool(enable))))
 
     def phase_times(self) -> list:
-        ms = (C.c_float * 4)()
+        """ms of [reduce, finalize, partition, coarse splits, merge pack]."""
+        ms = (C.c_float * 5)()
         check(lib().wrs_plan_phase_times(self.handle, ms))
         return list(ms)
 
