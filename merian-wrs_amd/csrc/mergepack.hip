@@ -214,11 +214,12 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
                                                 AliasEntry* __restrict__ table) {
     using T = typename RealT<PREC>::T;
     constexpr int kPer = kMergeTile / kBlock;  // buckets (and staged elements) per thread
-    __shared__ T s_key[kMergeTile + 2];
+    // keys during the merge, entries afterwards (8 bytes per slot either way)
+    __shared__ uint2 s_raw[kMergeTile + 2];
     __shared__ uint32_t s_idx[kMergeTile + 2];
     __shared__ float s_w[kMergeTile];
-    __shared__ uint2 s_ent[kMergeTile];
     __shared__ DF s_wave_l[kBlock / kWave], s_wave_h[kBlock / kWave];
+    T* s_key = reinterpret_cast<T*>(s_raw);
 
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
@@ -234,16 +235,30 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
     const uint32_t nh = c1.j - j0;  // heavies packed here (heavy c1.j is current at n1)
     const uint32_t ne = nl + nh;    // == n1 - n0 (- 1 in the last block)
 
-    // ---- stage indices and weights in rank order -----------------------
-    for (uint32_t r = t; r < nl; r += kBlock) {
-        const uint32_t li = idx[N - 1u - (i0 + r)];
-        s_idx[r] = li;
-        s_w[r] = w[li];
-    }
-    for (uint32_t q = t; q <= nh; q += kBlock) {
-        const uint32_t hi = idx[j0 + q];
-        s_idx[nl + q] = hi;
-        if (q < nh) s_w[nl + q] = w[hi];
+    // ---- stage indices and weights in rank order ------------------------
+    // slot e < nl: light i0 + e; slot nl <= e <= ne: heavy j0 + e - nl.  All
+    // index loads are issued before the dependent weight gathers.
+    {
+        uint32_t gi[kPer + 1];
+#pragma unroll
+        for (int k = 0; k <= kPer; ++k) {
+            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * k;
+            gi[k] = 0;
+            if (e < nl) gi[k] = idx[N - 1u - (i0 + e)];
+            else if (e <= ne) gi[k] = idx[j0 + (e - nl)];
+        }
+        float gw[kPer + 1];
+#pragma unroll
+        for (int k = 0; k <= kPer; ++k) {
+            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * k;
+            gw[k] = e < ne ? w[gi[k]] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k <= kPer; ++k) {
+            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * k;
+            if (e <= ne) s_idx[e] = gi[k];
+            if (e < ne) s_w[e] = gw[k];
+        }
     }
     __syncthreads();
 
@@ -345,6 +360,9 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
     const T* A = s_key;            // A[r], r in [0, nl]
     const T* B = s_key + nl + 1u;  // B[q], q in [0, nh)
     const uint32_t d0 = e0;
+    uint32_t slot[kPer];
+    uint2 ent[kPer];
+    int cnt = 0;
     if (d0 < ne) {
         // largest q in [max(0, d0 - nl), min(d0, nh)] with q == 0 or
         // B[q-1] <= A[d0 - q] (A[nl] counts as +inf here)
@@ -360,23 +378,33 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
         uint32_t q = (uint32_t)qa;
         uint32_t r = d0 - q;
         const uint32_t d1 = min(d0 + (uint32_t)kPer, ne);
-        for (uint32_t d = d0; d < d1; ++d) {
-            const bool take_heavy = q < nh && (r >= nl || B[q] <= A[r]);
-            if (take_heavy) {
-                const T resid = (B[q] - A[r]) + mean;
-                const float p = (float)(resid / mean);
-                s_ent[nl + q] = make_uint2(__float_as_uint(p), s_idx[nl + q + 1u]);
-                ++q;
-            } else {
-                const float p = (float)((T)s_w[r] / mean);
-                s_ent[r] = make_uint2(__float_as_uint(p), s_idx[nl + q]);
-                ++r;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            if (d0 + (uint32_t)k < d1) {
+                const bool take_heavy = q < nh && (r >= nl || B[q] <= A[r]);
+                if (take_heavy) {
+                    const T resid = (B[q] - A[r]) + mean;
+                    const float p = (float)(resid / mean);
+                    slot[k] = nl + q;
+                    ent[k] = make_uint2(__float_as_uint(p), s_idx[nl + q + 1u]);
+                    ++q;
+                } else {
+                    const float p = (float)((T)s_w[r] / mean);
+                    slot[k] = r;
+                    ent[k] = make_uint2(__float_as_uint(p), s_idx[nl + q]);
+                    ++r;
+                }
+                cnt = k + 1;
             }
         }
     }
+    __syncthreads();  // keys no longer read: the buffer now takes the entries
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if (k < cnt) s_raw[slot[k]] = ent[k];
     __syncthreads();
     uint2* tab = reinterpret_cast<uint2*>(table);
-    for (uint32_t e = t; e < ne; e += kBlock) tab[s_idx[e]] = s_ent[e];
+    for (uint32_t e = t; e < ne; e += kBlock) tab[s_idx[e]] = s_raw[e];
     if (last && t == 0) {
         const uint32_t h = s_idx[nl + nh];  // heavy H-1
         table[h] = AliasEntry{1.0f, h};

@@ -24,6 +24,7 @@
 //      reversed rank) and written as two contiguous coalesced runs.
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "lookback.hpp"
 
 namespace wrsk {
 using namespace wrsdev;
@@ -32,9 +33,8 @@ namespace {
 
 constexpr int kPad = kTile + kTile / 16;  // LDS words of the padded transpose buffer
 
-// Look-back record layout (16 words per tile):
-//   [0..4]  aggregate: heavy sum (2 words), light sum (2 words), heavy count
-//   [8..12] inclusive prefix, same fields
+// Look-back record payload (lookback.hpp): heavy sum (2 words), light sum
+// (2 words), heavy count; aggregate at words 0..4, inclusive prefix at 8..12.
 // FP32 mode keeps each sum in one float (word .lo; .hi = 0).
 struct Carry {
     FX h, l;
@@ -75,7 +75,7 @@ __device__ __forceinline__ Carry wave_sum_carry(Carry v) {
     return v;
 }
 
-__device__ __forceinline__ void publish(uint64_t* rec, const Carry& v) {
+__device__ __forceinline__ void publish_rec(uint64_t* rec, const Carry& v) {
     st_word(rec + 0, v.h.lo);
     st_word(rec + 1, v.h.hi);
     st_word(rec + 2, v.l.lo);
@@ -92,6 +92,15 @@ __device__ __forceinline__ bool read_rec(const uint64_t* rec, Carry& v) {
     v.c = ld_word(rec + 4);
     return ((v.h.lo | v.h.hi | v.l.lo | v.l.hi | v.c) >> 63) == 0;
 }
+
+template <int PREC>
+struct CarryOps {
+    static __device__ __forceinline__ Carry zero() { return carry_zero(); }
+    static __device__ __forceinline__ Carry add(Carry a, Carry b) { return carry_add<PREC>(a, b); }
+    static __device__ __forceinline__ Carry wave_sum(Carry v) { return wave_sum_carry<PREC>(v); }
+    static __device__ __forceinline__ bool read(const uint64_t* r, Carry& v) { return read_rec(r, v); }
+    static __device__ __forceinline__ void publish(uint64_t* r, const Carry& v) { publish_rec(r, v); }
+};
 
 // In-tile per-thread accumulator: compensated (DF) or plain float.
 template <int PREC>
@@ -238,46 +247,13 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
         }
         agg.c = tile_agg.c;
         if (tile == 0) {
-            if (lane == 0) publish(rec + 8, agg);
+            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, agg);
         } else {
-            if (lane == 0) publish(rec + 0, agg);
-            Carry excl = carry_zero();
-            int64_t look = (int64_t)tile - 1;
-            for (;;) {
-                const int64_t tt = look - lane;
-                int st;  // 0 not ready, 1 aggregate, 2 inclusive prefix
-                Carry pv = carry_zero();
-                if (tt < 0) {
-                    st = 2;
-                } else {
-                    const uint64_t* r = states + (uint64_t)tt * kStateWords;
-                    if (read_rec(r + 8, pv)) st = 2;
-                    else if (read_rec(r + 0, pv)) st = 1;
-                    else {
-                        st = 0;
-                        pv = carry_zero();
-                    }
-                }
-                const uint64_t b_inc = __ballot(st == 2);
-                const uint64_t b_nr = __ballot(st == 0);
-                const int fi = b_inc ? __builtin_ctzll(b_inc) : kWave;
-                const int fn = b_nr ? __builtin_ctzll(b_nr) : kWave;
-                if (fi < fn) {
-                    const Carry c = lane <= fi ? pv : carry_zero();
-                    excl = carry_add<PREC>(excl, wave_sum_carry<PREC>(c));
-                    break;
-                }
-                if (fn > 0) {
-                    const Carry c = lane < fn ? pv : carry_zero();
-                    excl = carry_add<PREC>(excl, wave_sum_carry<PREC>(c));
-                    look -= fn;
-                } else {
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
+            if (lane == 0) publish_aggregate<Carry, CarryOps<PREC>>(rec, agg);
+            const Carry excl = lookback<Carry, CarryOps<PREC>>(states, tile);
             if (lane == 0) {
                 const Carry inc = carry_add<PREC>(excl, agg);
-                publish(rec + 8, inc);
+                publish_inclusive<Carry, CarryOps<PREC>>(rec, inc);
                 if constexpr (PREC == kFP32) {
                     s_carry_hf = __uint_as_float((uint32_t)excl.h.lo);
                     s_carry_lf = __uint_as_float((uint32_t)excl.l.lo);

@@ -15,6 +15,7 @@
 // as coalesced float4 rows.
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "lookback.hpp"
 
 namespace wrsk {
 using namespace wrsdev;
@@ -52,6 +53,14 @@ __device__ __forceinline__ void sc_publish(uint64_t* rec, FX v) {
     st_word(rec + 0, v.lo);
     st_word(rec + 1, v.hi);
 }
+template <int PREC>
+struct ScanOps {
+    static __device__ __forceinline__ FX zero() { return fx_zero(); }
+    static __device__ __forceinline__ FX add(FX a, FX b) { return sc_add<PREC>(a, b); }
+    static __device__ __forceinline__ FX wave_sum(FX v) { return sc_wave_sum<PREC>(v); }
+    static __device__ __forceinline__ bool read(const uint64_t* r, FX& v) { return sc_read(r, v); }
+    static __device__ __forceinline__ void publish(uint64_t* r, const FX& v) { sc_publish(r, v); }
+};
 }  // namespace
 
 template <int PREC, bool VEC>
@@ -147,41 +156,11 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
         else agg = fx_from_double((double)tile_agg.hi, shift);
         FX excl = fx_zero();
         if (tile == 0) {
-            if (lane == 0) sc_publish(rec + 8, agg);
+            if (lane == 0) publish_inclusive<FX, ScanOps<PREC>>(rec, agg);
         } else {
-            if (lane == 0) sc_publish(rec + 0, agg);
-            int64_t look = (int64_t)tile - 1;
-            for (;;) {
-                const int64_t tt = look - lane;
-                int st;
-                FX pv = fx_zero();
-                if (tt < 0) {
-                    st = 2;
-                } else {
-                    const uint64_t* r = states + (uint64_t)tt * kStateWords;
-                    if (sc_read(r + 8, pv)) st = 2;
-                    else if (sc_read(r + 0, pv)) st = 1;
-                    else {
-                        st = 0;
-                        pv = fx_zero();
-                    }
-                }
-                const uint64_t b_inc = __ballot(st == 2);
-                const uint64_t b_nr = __ballot(st == 0);
-                const int fi = b_inc ? __builtin_ctzll(b_inc) : kWave;
-                const int fn = b_nr ? __builtin_ctzll(b_nr) : kWave;
-                if (fi < fn) {
-                    excl = sc_add<PREC>(excl, sc_wave_sum<PREC>(lane <= fi ? pv : fx_zero()));
-                    break;
-                }
-                if (fn > 0) {
-                    excl = sc_add<PREC>(excl, sc_wave_sum<PREC>(lane < fn ? pv : fx_zero()));
-                    look -= fn;
-                } else {
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            if (lane == 0) sc_publish(rec + 8, sc_add<PREC>(excl, agg));
+            if (lane == 0) publish_aggregate<FX, ScanOps<PREC>>(rec, agg);
+            excl = lookback<FX, ScanOps<PREC>>(states, tile);
+            if (lane == 0) publish_inclusive<FX, ScanOps<PREC>>(rec, sc_add<PREC>(excl, agg));
         }
         if (lane == 0) {
             if constexpr (PREC == kFP32) s_carry_f = __uint_as_float((uint32_t)excl.lo);
