@@ -59,6 +59,7 @@ constexpr uint32_t kMaxPartials = 2048;     // reduce blocks
 constexpr int kSplitBlock = 256;            // debug split kernel block size
 constexpr uint32_t kSample = 64;            // partition prefix sample stride (ranks)
 constexpr int kMergeTile = 2048;            // buckets per merge-pack block (multiple of kSample)
+constexpr int kMergeBlock = 512;            // merge-pack threads per block (4 buckets each)
 
 inline uint64_t samples_for(uint64_t n) { return n / kSample + 1; }
 inline uint64_t merge_blocks_for(uint64_t n) { return (n + kMergeTile - 1) / kMergeTile; }

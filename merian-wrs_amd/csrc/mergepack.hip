@@ -24,12 +24,12 @@
 //             kSample, so both prefixes are samples) brackets j to a window of
 //             kSample heavies; the window is resolved exactly by a wave scan of
 //             the 64 gathered heavy and light weights around it.
-//  k_mergepack one 256-thread block per kMergeTile buckets: the block's light
+//  k_mergepack one 512-thread block per kMergeTile buckets: the block's light
 //             and heavy index ranges come from its two boundary splits; their
 //             indices and gathered weights are staged in LDS, the keys X and Y
 //             are rebuilt by a compensated block scan on top of LP(i0), HP(j0),
 //             every thread finds its merge-path diagonal by a binary search in
-//             LDS and merges 8 buckets, and the entries are written back in
+//             LDS and merges 4 buckets, and the entries are written back in
 //             rank order (coalesced: consecutive light ranks are consecutive
 //             ascending indices).  The last heavy of all gets (1, itself), as
 //             in pack.hpp:78-81.
@@ -213,12 +213,12 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
                                                 uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
                                                 AliasEntry* __restrict__ table) {
     using T = typename RealT<PREC>::T;
-    constexpr int kPer = kMergeTile / kBlock;  // buckets (and staged elements) per thread
+    constexpr int kPer = kMergeTile / kMergeBlock;  // buckets (and staged elements) per thread
     // keys during the merge, entries afterwards (8 bytes per slot either way)
     __shared__ uint2 s_raw[kMergeTile + 2];
     __shared__ uint32_t s_idx[kMergeTile + 2];
     __shared__ float s_w[kMergeTile];
-    __shared__ DF s_wave_l[kBlock / kWave], s_wave_h[kBlock / kWave];
+    __shared__ DF s_wave_l[kMergeBlock / kWave], s_wave_h[kMergeBlock / kWave];
     T* s_key = reinterpret_cast<T*>(s_raw);
 
     const int t = threadIdx.x;
@@ -226,7 +226,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
     const int wid = t / kWave;
     const uint32_t L = N - H;
     if (H == 0 || L == 0) {  // all weights equal the mean up to its rounding: identity
-        for (uint32_t x = n0 + t; x < n1; x += kBlock) table[x] = AliasEntry{1.0f, x};
+        for (uint32_t x = n0 + t; x < n1; x += kMergeBlock) table[x] = AliasEntry{1.0f, x};
         return;
     }
     const bool last = n1 == N;
@@ -242,7 +242,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
         uint32_t gi[kPer + 1];
 #pragma unroll
         for (int k = 0; k <= kPer; ++k) {
-            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * k;
+            const uint32_t e = (uint32_t)t + (uint32_t)kMergeBlock * k;
             gi[k] = 0;
             if (e < nl) gi[k] = idx[N - 1u - (i0 + e)];
             else if (e <= ne) gi[k] = idx[j0 + (e - nl)];
@@ -250,12 +250,12 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
         float gw[kPer + 1];
 #pragma unroll
         for (int k = 0; k <= kPer; ++k) {
-            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * k;
+            const uint32_t e = (uint32_t)t + (uint32_t)kMergeBlock * k;
             gw[k] = e < ne ? w[gi[k]] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k <= kPer; ++k) {
-            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * k;
+            const uint32_t e = (uint32_t)t + (uint32_t)kMergeBlock * k;
             if (e <= ne) s_idx[e] = gi[k];
             if (e < ne) s_w[e] = gw[k];
         }
@@ -309,7 +309,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
     }
     __syncthreads();
     DF rl{0.0f, 0.0f}, rh{0.0f, 0.0f}, tot_l{0.0f, 0.0f};
-    for (int q = 0; q < kBlock / kWave; ++q) {
+    for (int q = 0; q < kMergeBlock / kWave; ++q) {
         if constexpr (PREC == kComp) {
             if (q < wid) {
                 rl = df_add(rl, s_wave_l[q]);
@@ -404,7 +404,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
         if (k < cnt) s_raw[slot[k]] = ent[k];
     __syncthreads();
     uint2* tab = reinterpret_cast<uint2*>(table);
-    for (uint32_t e = t; e < ne; e += kBlock) tab[s_idx[e]] = s_raw[e];
+    for (uint32_t e = t; e < ne; e += kMergeBlock) tab[s_idx[e]] = s_raw[e];
     if (last && t == 0) {
         const uint32_t h = s_idx[nl + nh];  // heavy H-1
         table[h] = AliasEntry{1.0f, h};
@@ -412,7 +412,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
 }
 
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_mergepack(const float* __restrict__ w,
+__global__ __launch_bounds__(kMergeBlock) void k_mergepack(const float* __restrict__ w,
                                                       const uint32_t* __restrict__ idx, uint32_t N,
                                                       const Params* __restrict__ prm,
                                                       const Coarse* __restrict__ bounds,
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_mergepack(const float* __restrict__ 
 }
 
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_seg_mergepack(
+__global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
     const float* __restrict__ w, const uint32_t* __restrict__ idx,
     const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ mitem_seg,
     const uint32_t* __restrict__ mitem_blk, const uint32_t* __restrict__ seg_b0,
@@ -465,11 +465,11 @@ hipError_t launch_mergepack(const float* w, uint64_t n, int prec, const Params* 
     const uint32_t N = (uint32_t)n;
     const uint32_t blocks = (uint32_t)merge_blocks_for(n);
     if (prec == kComp)
-        k_mergepack<kComp><<<blocks, kBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+        k_mergepack<kComp><<<blocks, kMergeBlock, 0, s>>>(w, idx, N, prm, bounds, table);
     else if (prec == kFP64Carry)
-        k_mergepack<kFP64Carry><<<blocks, kBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+        k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(w, idx, N, prm, bounds, table);
     else
-        k_mergepack<kFP32><<<blocks, kBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+        k_mergepack<kFP32><<<blocks, kMergeBlock, 0, s>>>(w, idx, N, prm, bounds, table);
     return hipGetLastError();
 }
 
@@ -485,7 +485,7 @@ hipError_t launch_seg_mergepack(const float* w, const uint32_t* idx, const doubl
     do {                                                                                        \
         k_seg_coarse<P><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, seg_off, seg_samp0,      \
                                               seg_b0, nseg, seg_prm, bounds);                   \
-        k_seg_mergepack<P><<<mitems, kBlock, 0, s>>>(w, idx, seg_off, mitem_seg, mitem_blk,     \
+        k_seg_mergepack<P><<<mitems, kMergeBlock, 0, s>>>(w, idx, seg_off, mitem_seg, mitem_blk,     \
                                                      seg_b0, seg_prm, bounds, table);           \
     } while (0)
     if (prec == kComp) WRS_LAUNCH_SMP(kComp);

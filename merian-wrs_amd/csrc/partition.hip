@@ -1,5 +1,4 @@
-// partition.hip -- single-pass heavy/light prefix-partition with decoupled
-// look-back (pass 2 of a PSA build).
+// partition.hip -- heavy/light prefix-partition (pass 2 of a PSA build).
 //
 // Semantics follow the reference kernel src/device/prefix_partition/decoupled/
 // shader.comp:313-362 and its CPU counterpart src/host/reference/partition.hpp:35-53:
@@ -9,19 +8,30 @@
 //   prefix[same pos] inclusive prefix of the heavy (resp. light) stream;
 //   heavy_count = H.
 //
-// MI355X design (one 256-thread block = 4 wave64s per 4096-element tile):
-//   1. dynamic tile ticket (atomicAdd) so that every predecessor of a tile has
-//      been scheduled before it spins on it;
-//   2. coalesced float4 loads, transposed through LDS (row stride 17 words, so
-//      both the transpose writes and the per-thread reads avoid bank
-//      conflicts) so that each thread owns 16 consecutive elements;
-//   3. per-thread sequential (compensated) sums, wave64 shuffle scan, then a
-//      4-entry cross-wave scan in LDS, all in a fixed order;
-//   4. the tile aggregate is published as self-validating 64-bit words in HBM
-//      (126-bit fixed point for the two weight streams, exact integer count);
-//      wave 0 looks back up to 64 predecessors per step (one per lane);
-//   5. outputs are staged in LDS in final order (heavies by rank, lights by
-//      reversed rank) and written as two contiguous coalesced runs.
+// MI355X design.  A tile is 4096 elements handled by one 256-thread block
+// (4 wave64s):
+//   * coalesced float4 loads, transposed through LDS (row stride 17 words, so
+//     both the transpose writes and the per-thread reads avoid bank
+//     conflicts) so that each thread owns 16 consecutive elements;
+//   * per-thread sequential (compensated) sums, wave64 shuffle scan, then a
+//     4-entry cross-wave scan in LDS, all in a fixed order;
+//   * tile carries are 126-bit fixed-point sums (exact, so their bits do not
+//     depend on the order in which tiles are combined) plus an exact count;
+//   * outputs are staged in LDS in final order (heavies by rank, lights by
+//     reversed rank) and written as two contiguous coalesced runs.
+//
+// Single distribution: reduce-then-scan.  k_part_agg writes every tile's
+// aggregate, one block scans the aggregates (k_part_tilescan), and k_part_emit
+// re-reads the tile and writes the outputs.  The 8 XCDs of the chip have
+// separate L2 caches, so a single-pass chained scan pays a device-coherent
+// round trip per look-back step on every tile; one extra streaming read of
+// the weights is cheaper than that at large N.
+//
+// Segmented (batched) form: one pass with decoupled look-back per segment
+// (segments are short, so the chains are short).
+//
+// Both forms compute identical tile aggregates and carries, so a segment built
+// in a batch is bit-identical to the same distribution built alone.
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "lookback.hpp"
@@ -32,10 +42,13 @@ using namespace wrsdev;
 namespace {
 
 constexpr int kPad = kTile + kTile / 16;  // LDS words of the padded transpose buffer
+constexpr int kScanThreads = 1024;        // tile-scan block
 
-// Look-back record payload (lookback.hpp): heavy sum (2 words), light sum
-// (2 words), heavy count; aggregate at words 0..4, inclusive prefix at 8..12.
-// FP32 mode keeps each sum in one float (word .lo; .hi = 0).
+// Tile carry: heavy sum (2 words), light sum (2 words), heavy count.  In the
+// look-back records the aggregate sits at words 0..4 and the inclusive prefix
+// at 8..12; in the reduce-then-scan path the aggregate sits at 0..4 and the
+// exclusive prefix at 8..12.  FP32 mode keeps each sum in one float (word
+// .lo; .hi = 0).
 struct Carry {
     FX h, l;
     uint64_t c;
@@ -58,6 +71,16 @@ __device__ __forceinline__ Carry carry_add(Carry a, Carry b) {
     }
     r.c = a.c + b.c;
     return r;
+}
+
+__device__ __forceinline__ Carry carry_shfl_up(Carry v, int d) {
+    Carry o;
+    o.h.lo = __shfl_up(v.h.lo, (unsigned)d, kWave);
+    o.h.hi = __shfl_up(v.h.hi, (unsigned)d, kWave);
+    o.l.lo = __shfl_up(v.l.lo, (unsigned)d, kWave);
+    o.l.hi = __shfl_up(v.l.hi, (unsigned)d, kWave);
+    o.c = __shfl_up(v.c, (unsigned)d, kWave);
+    return o;
 }
 
 template <int PREC>
@@ -91,6 +114,18 @@ __device__ __forceinline__ bool read_rec(const uint64_t* rec, Carry& v) {
     v.l.hi = ld_word(rec + 3);
     v.c = ld_word(rec + 4);
     return ((v.h.lo | v.h.hi | v.l.lo | v.l.hi | v.c) >> 63) == 0;
+}
+
+// Plain (kernel-ordered) record access for the reduce-then-scan path.
+__device__ __forceinline__ Carry load_carry(const uint64_t* rec) {
+    return Carry{FX{rec[0], rec[1]}, FX{rec[2], rec[3]}, rec[4]};
+}
+__device__ __forceinline__ void store_carry(uint64_t* rec, const Carry& v) {
+    rec[0] = v.h.lo;
+    rec[1] = v.h.hi;
+    rec[2] = v.l.lo;
+    rec[3] = v.l.hi;
+    rec[4] = v.c;
 }
 
 template <int PREC>
@@ -145,35 +180,45 @@ __device__ __forceinline__ double stream_value(DF v) {
     else return (double)v.hi;
 }
 
-}  // namespace
+// Tile aggregate as a carry word set.
+template <int PREC>
+__device__ __forceinline__ Carry to_carry(const Acc<PREC>& a, int shift) {
+    Carry r;
+    if constexpr (PREC == kFP32) {
+        r.h = FX{(uint64_t)__float_as_uint(a.h.hi), 0};
+        r.l = FX{(uint64_t)__float_as_uint(a.l.hi), 0};
+    } else {
+        r.h = fx_from_double(stream_value<PREC>(a.h), shift);
+        r.l = fx_from_double(stream_value<PREC>(a.l), shift);
+    }
+    r.c = a.c;
+    return r;
+}
 
-// One 4096-element tile of the distribution w[0..n): `tile` is its index,
-// `states` the look-back records of tile 0 onward, prm its build scalars.
-// Used by the single-distribution kernel and by the segmented (batched) one.
+template <int PREC>
+__device__ __forceinline__ double carry_value(const FX& v, int shift) {
+    if constexpr (PREC == kFP32) return (double)__uint_as_float((uint32_t)v.lo);
+    else return fx_to_double(v, shift);
+}
+
+// LDS of one tile block.
+template <int PREC>
+struct TileShared {
+    float buf[kPad];
+    Acc<PREC> wave[kBlock / kWave];
+};
+
+// Loads the m (<= kTile) elements of w[base..) so that thread t owns elements
+// 16t..16t+15 (zero beyond m), classifies them against the pivot and scans:
+// thr_excl is the sum over the thread's predecessors in the tile, tile_agg
+// the tile total.  Ends with the block synchronised and s.buf free.
 template <int PREC, bool VEC>
-__device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint32_t n,
-                                               uint32_t tile, Params* __restrict__ prm,
-                                               uint64_t* __restrict__ states,
-                                               uint32_t* __restrict__ idx_out,
-                                               float* __restrict__ hi_out,
-                                               float* __restrict__ lo_out,
-                                               double* __restrict__ samp_h,
-                                               double* __restrict__ samp_l) {
-    __shared__ float s_buf[kPad];
-    __shared__ Acc<PREC> s_wave[kBlock / kWave];
-    __shared__ double s_carry_h, s_carry_l, s_tot_h, s_tot_l;
-    __shared__ float s_carry_hf, s_carry_lf;
-    __shared__ uint32_t s_carry_c;
-
+__device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t base, uint32_t m,
+                                          float pivot, TileShared<PREC>& s, float (&v)[kItems],
+                                          Acc<PREC>& thr_excl, Acc<PREC>& tile_agg) {
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
-    const uint32_t base = tile * (uint32_t)kTile;
-    const uint32_t m = min((uint32_t)kTile, n - base);
-    const float pivot = prm->pivot;
-    const int shift = prm->shift;
-
-    // ---- 2. load + transpose -------------------------------------------
     if (VEC && m == (uint32_t)kTile) {
         const float4* w4 = reinterpret_cast<const float4*>(w + base);
         float4 v4[4];
@@ -183,25 +228,23 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
         for (int j = 0; j < 4; ++j) {
             const int e = 4 * (t + kBlock * j);
             const int q = e + e / 16;
-            s_buf[q + 0] = v4[j].x;
-            s_buf[q + 1] = v4[j].y;
-            s_buf[q + 2] = v4[j].z;
-            s_buf[q + 3] = v4[j].w;
+            s.buf[q + 0] = v4[j].x;
+            s.buf[q + 1] = v4[j].y;
+            s.buf[q + 2] = v4[j].z;
+            s.buf[q + 3] = v4[j].w;
         }
     } else {
 #pragma unroll
         for (int j = 0; j < kItems; ++j) {
             const uint32_t e = (uint32_t)t + (uint32_t)kBlock * j;
-            s_buf[e + e / 16] = e < m ? w[base + e] : 0.0f;
+            s.buf[e + e / 16] = e < m ? w[base + e] : 0.0f;
         }
     }
     __syncthreads();
-    float v[kItems];
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) v[k] = s_buf[17 * t + k];
+    for (int k = 0; k < kItems; ++k) v[k] = s.buf[17 * t + k];
     const uint32_t e0 = (uint32_t)(kItems * t);
 
-    // ---- 3. in-tile scan -----------------------------------------------
     Acc<PREC> mine{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
@@ -225,71 +268,56 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
     }
     Acc<PREC> lane_excl = acc_shfl_up<PREC>(incl, 1);
     if (lane == 0) lane_excl = Acc<PREC>{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
-    if (lane == kWave - 1) s_wave[wid] = incl;
+    if (lane == kWave - 1) s.wave[wid] = incl;
     __syncthreads();
     Acc<PREC> wave_excl{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
-    for (int q = 0; q < wid; ++q) wave_excl = acc_add<PREC>(wave_excl, s_wave[q]);
-    Acc<PREC> tile_agg = s_wave[0];
+    for (int q = 0; q < wid; ++q) wave_excl = acc_add<PREC>(wave_excl, s.wave[q]);
+    tile_agg = s.wave[0];
 #pragma unroll
-    for (int q = 1; q < kBlock / kWave; ++q) tile_agg = acc_add<PREC>(tile_agg, s_wave[q]);
-    const Acc<PREC> thr_excl = acc_add<PREC>(wave_excl, lane_excl);
+    for (int q = 1; q < kBlock / kWave; ++q) tile_agg = acc_add<PREC>(tile_agg, s.wave[q]);
+    thr_excl = acc_add<PREC>(wave_excl, lane_excl);
+}
 
-    // ---- 4. publish + decoupled look-back ------------------------------
-    uint64_t* rec = states + (uint64_t)tile * kStateWords;
-    if (wid == 0) {
-        Carry agg;
-        if constexpr (PREC == kFP32) {
-            agg.h = FX{(uint64_t)__float_as_uint(tile_agg.h.hi), 0};
-            agg.l = FX{(uint64_t)__float_as_uint(tile_agg.l.hi), 0};
-        } else {
-            agg.h = fx_from_double(stream_value<PREC>(tile_agg.h), shift);
-            agg.l = fx_from_double(stream_value<PREC>(tile_agg.l), shift);
-        }
-        agg.c = tile_agg.c;
-        if (tile == 0) {
-            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, agg);
-        } else {
-            if (lane == 0) publish_aggregate<Carry, CarryOps<PREC>>(rec, agg);
-            const Carry excl = lookback<Carry, CarryOps<PREC>>(states, tile);
-            if (lane == 0) {
-                const Carry inc = carry_add<PREC>(excl, agg);
-                publish_inclusive<Carry, CarryOps<PREC>>(rec, inc);
-                if constexpr (PREC == kFP32) {
-                    s_carry_hf = __uint_as_float((uint32_t)excl.h.lo);
-                    s_carry_lf = __uint_as_float((uint32_t)excl.l.lo);
-                    s_tot_h = (double)__uint_as_float((uint32_t)inc.h.lo);
-                    s_tot_l = (double)__uint_as_float((uint32_t)inc.l.lo);
-                } else {
-                    s_carry_h = fx_to_double(excl.h, shift);
-                    s_carry_l = fx_to_double(excl.l, shift);
-                    s_tot_h = fx_to_double(inc.h, shift);
-                    s_tot_l = fx_to_double(inc.l, shift);
-                }
-                s_carry_c = (uint32_t)excl.c;
-            }
-        }
-        if (tile == 0 && lane == 0) {
-            s_carry_h = 0.0;
-            s_carry_l = 0.0;
-            s_carry_hf = 0.0f;
-            s_carry_lf = 0.0f;
-            s_carry_c = 0;
-            if constexpr (PREC == kFP32) {
-                s_tot_h = (double)__uint_as_float((uint32_t)agg.h.lo);
-                s_tot_l = (double)__uint_as_float((uint32_t)agg.l.lo);
-            } else {
-                s_tot_h = fx_to_double(agg.h, shift);
-                s_tot_l = fx_to_double(agg.l, shift);
-            }
-        }
+// Carries of one tile: the sums and count of everything before it.
+struct TileCarry {
+    double h, l;   // FP64 modes
+    float hf, lf;  // FP32 mode
+    uint32_t c;
+};
+
+template <int PREC>
+__device__ __forceinline__ TileCarry tile_carry(const Carry& excl, int shift) {
+    TileCarry r;
+    if constexpr (PREC == kFP32) {
+        r.hf = __uint_as_float((uint32_t)excl.h.lo);
+        r.lf = __uint_as_float((uint32_t)excl.l.lo);
+        r.h = r.l = 0.0;
+    } else {
+        r.h = fx_to_double(excl.h, shift);
+        r.l = fx_to_double(excl.l, shift);
+        r.hf = r.lf = 0.0f;
     }
-    __syncthreads();
+    r.c = (uint32_t)excl.c;
+    return r;
+}
 
-    // ---- 5. outputs, staged in LDS in final order -----------------------
-    const uint32_t Hx = s_carry_c;           // heavies before this tile
-    const uint32_t Lx = base - Hx;           // lights before this tile
-    const uint32_t hc = tile_agg.c;          // heavies in this tile
-    uint32_t* s_u = reinterpret_cast<uint32_t*>(s_buf);
+// Writes the tile's outputs given its carries (block-uniform): the partition
+// indices, the prefix samples at ranks r with (r+1) % kSample == 0, and the
+// full hi (and lo) prefix arrays when requested.  Expects s.buf free (the
+// block synchronised after its last use).
+template <int PREC>
+__device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m, float pivot,
+                                          const float (&v)[kItems], const Acc<PREC>& thr_excl,
+                                          uint32_t hc, const TileCarry& cy, TileShared<PREC>& s,
+                                          uint32_t* __restrict__ idx_out,
+                                          float* __restrict__ hi_out, float* __restrict__ lo_out,
+                                          double* __restrict__ samp_h,
+                                          double* __restrict__ samp_l) {
+    const int t = threadIdx.x;
+    const uint32_t e0 = (uint32_t)(kItems * t);
+    const uint32_t Hx = cy.c;       // heavies before this tile
+    const uint32_t Lx = base - Hx;  // lights before this tile
+    uint32_t* s_u = reinterpret_cast<uint32_t*>(s.buf);
 
     uint32_t slot[kItems];
     {
@@ -302,27 +330,22 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
             hr += heavy ? 1u : 0u;
         }
     }
-    // positions of LDS slot s: heavy run then light run
-    auto out_pos = [&](uint32_t s) -> uint32_t { return s < hc ? Hx + s : n - m - Lx + s; };
+    // positions of LDS slot q: heavy run then light run
+    auto out_pos = [&](uint32_t q) -> uint32_t { return q < hc ? Hx + q : n - m - Lx + q; };
 
     // round A: indices
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
         if (e0 + k < m) s_u[slot[k]] = base + e0 + k;
     __syncthreads();
-    for (uint32_t s = t; s < m; s += kBlock) idx_out[out_pos(s)] = s_u[s];
-    __syncthreads();
+    for (uint32_t q = t; q < m; q += kBlock) idx_out[out_pos(q)] = s_u[q];
 
-    // round B: the inclusive prefix of every element (FP64 in the FP64 modes).
-    // Ranks r with (r+1) % kSample == 0 publish their prefix as a sample; the
-    // full hi (and lo) prefix arrays are written only when requested.
+    // round B: the inclusive prefix of every element (FP64 in the FP64 modes)
     float hi_keep[kItems], lo_keep[kItems];
     {
         DF rh = thr_excl.h, rl = thr_excl.l;
-        const double ch = s_carry_h, cl = s_carry_l;
-        const float chf = s_carry_hf, clf = s_carry_lf;
-        uint32_t hr = Hx + thr_excl.c;                    // global heavy rank of the next heavy
-        uint32_t lr = Lx + (e0 - thr_excl.c);             // global light rank of the next light
+        uint32_t hr = Hx + thr_excl.c;         // global heavy rank of the next heavy
+        uint32_t lr = Lx + (e0 - thr_excl.c);  // global light rank of the next light
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const bool heavy = v[k] > pivot;
@@ -331,18 +354,18 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
             if constexpr (PREC == kComp) {
                 if (heavy) rh = df_add1(rh, v[k]);
                 else rl = df_add1(rl, v[k]);
-                P = (heavy ? ch : cl) + df_value(heavy ? rh : rl);
+                P = (heavy ? cy.h : cy.l) + df_value(heavy ? rh : rl);
                 hi = (float)P;
                 lo = (float)(P - (double)hi);
             } else if constexpr (PREC == kFP64Carry) {
                 if (heavy) rh.hi += v[k];
                 else rl.hi += v[k];
-                P = (heavy ? ch : cl) + (double)(heavy ? rh.hi : rl.hi);
+                P = (heavy ? cy.h : cy.l) + (double)(heavy ? rh.hi : rl.hi);
                 hi = (float)P;
             } else {
                 if (heavy) rh.hi += v[k];
                 else rl.hi += v[k];
-                hi = (heavy ? chf : clf) + (heavy ? rh.hi : rl.hi);
+                hi = (heavy ? cy.hf : cy.lf) + (heavy ? rh.hi : rl.hi);
                 P = (double)hi;
             }
             hi_keep[k] = hi;
@@ -356,48 +379,121 @@ __device__ __forceinline__ void partition_tile(const float* __restrict__ w, uint
         }
     }
     if (hi_out != nullptr) {
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < kItems; ++k)
-            if (e0 + k < m) s_buf[slot[k]] = hi_keep[k];
+            if (e0 + k < m) s.buf[slot[k]] = hi_keep[k];
         __syncthreads();
-        for (uint32_t s = t; s < m; s += kBlock) hi_out[out_pos(s)] = s_buf[s];
+        for (uint32_t q = t; q < m; q += kBlock) hi_out[out_pos(q)] = s.buf[q];
         if constexpr (PREC == kComp) {
             if (lo_out != nullptr) {
                 __syncthreads();
 #pragma unroll
                 for (int k = 0; k < kItems; ++k)
-                    if (e0 + k < m) s_buf[slot[k]] = lo_keep[k];
+                    if (e0 + k < m) s.buf[slot[k]] = lo_keep[k];
                 __syncthreads();
-                for (uint32_t s = t; s < m; s += kBlock) lo_out[out_pos(s)] = s_buf[s];
+                for (uint32_t q = t; q < m; q += kBlock) lo_out[out_pos(q)] = s.buf[q];
             }
         }
     }
-    if (t == 0 && base + m == n) {
-        prm->heavy_count = Hx + hc;
-        prm->heavy_sum = s_tot_h;
-        prm->light_sum = s_tot_l;
+}
+
+}  // namespace
+
+// ---- single distribution: reduce-then-scan ---------------------------------
+
+// Pass 2a: aggregate of every tile, at words 0..4 of its record.
+template <int PREC, bool VEC>
+__global__ __launch_bounds__(kBlock) void k_part_agg(const float* __restrict__ w, uint32_t n,
+                                                     const Params* __restrict__ prm,
+                                                     uint64_t* __restrict__ states) {
+    __shared__ TileShared<PREC> s;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    float v[kItems];
+    Acc<PREC> thr_excl, tile_agg;
+    tile_scan<PREC, VEC>(w, base, m, prm->pivot, s, v, thr_excl, tile_agg);
+    if (threadIdx.x == 0)
+        store_carry(states + (uint64_t)tile * kStateWords, to_carry<PREC>(tile_agg, prm->shift));
+}
+
+// Pass 2b: exclusive scan of the tile aggregates (one block; thread t takes a
+// contiguous run of tiles), exclusive prefixes at words 8..12 of each record,
+// totals into prm.  The FP64 modes add exactly; FP32 mode adds in a fixed order.
+template <int PREC>
+__global__ __launch_bounds__(kScanThreads) void k_part_tilescan(uint64_t* __restrict__ states,
+                                                                uint32_t tiles,
+                                                                Params* __restrict__ prm) {
+    __shared__ Carry s_wave[kScanThreads / kWave];
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wid = t / kWave;
+    const uint32_t per = (tiles + kScanThreads - 1) / kScanThreads;
+    const uint32_t b = min(tiles, (uint32_t)t * per);
+    const uint32_t e = min(tiles, b + per);
+    Carry mine = carry_zero();
+    for (uint32_t i = b; i < e; ++i)
+        mine = carry_add<PREC>(mine, load_carry(states + (uint64_t)i * kStateWords));
+    Carry incl = mine;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const Carry o = carry_shfl_up(incl, d);
+        if (lane >= d) incl = carry_add<PREC>(o, incl);
+    }
+    Carry lane_excl = carry_shfl_up(incl, 1);
+    if (lane == 0) lane_excl = carry_zero();
+    if (lane == kWave - 1) s_wave[wid] = incl;
+    __syncthreads();
+    Carry run = carry_zero();
+    for (int q = 0; q < wid; ++q) run = carry_add<PREC>(run, s_wave[q]);
+    run = carry_add<PREC>(run, lane_excl);
+    for (uint32_t i = b; i < e; ++i) {
+        uint64_t* rec = states + (uint64_t)i * kStateWords;
+        const Carry agg = load_carry(rec);
+        store_carry(rec + 8, run);
+        run = carry_add<PREC>(run, agg);
+    }
+    if (t == kScanThreads - 1) {
+        // the last thread's run now covers every tile
+        const int shift = prm->shift;
+        prm->heavy_count = (uint32_t)run.c;
+        prm->heavy_sum = carry_value<PREC>(run.h, shift);
+        prm->light_sum = carry_value<PREC>(run.l, shift);
     }
 }
 
+// Pass 2c: outputs of every tile from its exclusive prefix.
 template <int PREC, bool VEC>
-__global__ __launch_bounds__(kBlock) void k_partition(const float* __restrict__ w, uint32_t n,
-                                                      Params* __restrict__ prm,
-                                                      uint64_t* __restrict__ states,
+__global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ w, uint32_t n,
+                                                      const Params* __restrict__ prm,
+                                                      const uint64_t* __restrict__ states,
                                                       uint32_t* __restrict__ idx_out,
                                                       float* __restrict__ hi_out,
                                                       float* __restrict__ lo_out,
                                                       double* __restrict__ samp_h,
                                                       double* __restrict__ samp_l) {
-    __shared__ uint32_t s_tile;
-    if (threadIdx.x == 0) s_tile = atomicAdd(&prm->ticket, 1u);
-    __syncthreads();
-    partition_tile<PREC, VEC>(w, n, s_tile, prm, states, idx_out, hi_out, lo_out, samp_h, samp_l);
+    __shared__ TileShared<PREC> s;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const float pivot = prm->pivot;
+    const Carry excl = load_carry(states + (uint64_t)tile * kStateWords + 8);
+    float v[kItems];
+    Acc<PREC> thr_excl, tile_agg;
+    tile_scan<PREC, VEC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
+    const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
+    __syncthreads();  // s.buf is reused by the emit rounds
+    tile_emit<PREC>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, hi_out, lo_out,
+                    samp_h, samp_l);
 }
 
-// Segmented form: one work item per (segment, tile), items of a segment are
-// consecutive and handed out in ticket order, so a tile's predecessors in its
-// segment always hold smaller tickets.  Segment outputs use segment-local
-// indices and occupy the segment's own range of idx/hi/lo.
+// ---- segmented: single pass with decoupled look-back -----------------------
+
+// One work item per (segment, tile), items of a segment are consecutive and
+// handed out in ticket order, so a tile's predecessors in its segment always
+// hold smaller tickets.  Segment outputs use segment-local indices and occupy
+// the segment's own range of idx/hi/lo.
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_seg_partition(
     const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
@@ -406,7 +502,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     uint32_t* __restrict__ ticket, uint64_t* __restrict__ states, uint32_t* __restrict__ idx,
     float* __restrict__ hi, float* __restrict__ lo, double* __restrict__ samp_h,
     double* __restrict__ samp_l, const uint64_t* __restrict__ seg_samp0) {
+    __shared__ TileShared<PREC> s;
     __shared__ uint32_t s_item;
+    __shared__ Carry s_excl;
     if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t item = s_item;
@@ -414,10 +512,44 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     const uint64_t off = seg_off[sg];
     const uint32_t n = (uint32_t)(seg_off[sg + 1] - off);
     const uint64_t so = seg_samp0[sg];
-    partition_tile<PREC, false>(w + off, n, item_tile[item], seg_prm + sg,
-                                states + (uint64_t)seg_item0[sg] * kStateWords, idx + off,
-                                hi ? hi + off : nullptr, lo ? lo + off : nullptr, samp_h + so,
-                                samp_l + so);
+    const uint32_t tile = item_tile[item];
+    Params* prm = seg_prm + sg;
+    uint64_t* seg_states = states + (uint64_t)seg_item0[sg] * kStateWords;
+
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const float pivot = prm->pivot;
+    const int shift = prm->shift;
+    float v[kItems];
+    Acc<PREC> thr_excl, tile_agg;
+    tile_scan<PREC, false>(w + off, base, m, pivot, s, v, thr_excl, tile_agg);
+
+    const int lane = threadIdx.x & (kWave - 1);
+    if (threadIdx.x < kWave) {
+        uint64_t* rec = seg_states + (uint64_t)tile * kStateWords;
+        const Carry agg = to_carry<PREC>(tile_agg, shift);
+        Carry excl = carry_zero();
+        if (tile == 0) {
+            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, agg);
+        } else {
+            if (lane == 0) publish_aggregate<Carry, CarryOps<PREC>>(rec, agg);
+            excl = lookback<Carry, CarryOps<PREC>>(seg_states, tile);
+            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, carry_add<PREC>(excl, agg));
+        }
+        if (lane == 0) {
+            s_excl = excl;
+            if (base + m == n) {
+                const Carry tot = carry_add<PREC>(excl, agg);
+                prm->heavy_count = (uint32_t)tot.c;
+                prm->heavy_sum = carry_value<PREC>(tot.h, shift);
+                prm->light_sum = carry_value<PREC>(tot.l, shift);
+            }
+        }
+    }
+    __syncthreads();
+    const TileCarry cy = tile_carry<PREC>(s_excl, shift);
+    tile_emit<PREC>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off,
+                    hi ? hi + off : nullptr, lo ? lo + off : nullptr, samp_h + so, samp_l + so);
 }
 
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
@@ -426,8 +558,13 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
     const uint32_t tiles = (uint32_t)tiles_for(n);
     const bool vec = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     const uint32_t nn = (uint32_t)n;
-#define WRS_LAUNCH_PART(P, V) \
-    k_partition<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, hi, lo, samp_h, samp_l)
+#define WRS_LAUNCH_PART(P, V)                                                                  \
+    do {                                                                                       \
+        k_part_agg<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states);                         \
+        k_part_tilescan<P><<<1, kScanThreads, 0, s>>>(states, tiles, prm);                     \
+        k_part_emit<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, hi, lo, samp_h,    \
+                                                   samp_l);                                    \
+    } while (0)
     if (prec == kComp) {
         if (vec) WRS_LAUNCH_PART(kComp, true);
         else WRS_LAUNCH_PART(kComp, false);
@@ -441,7 +578,6 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
 #undef WRS_LAUNCH_PART
     return hipGetLastError();
 }
-
 
 hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
                                 const uint32_t* item_tile, const uint32_t* seg_item0,
