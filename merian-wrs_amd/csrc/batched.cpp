@@ -77,6 +77,7 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_
     states = dalloc<uint64_t>((uint64_t)items * wrsk::kStateWords);
     ticket = dalloc<uint32_t>(1);
     idx = dalloc<uint32_t>(total_n);
+    pw = dalloc<float>(total_n);
     samp_h = dalloc<double>(samp);
     samp_l = dalloc<double>(samp);
     bounds = dalloc<wrsk::Coarse>(nbound);
@@ -92,8 +93,8 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_
 
 void BatchPlan::destroy() {
     void* ptrs[] = {seg_off,   seg_item0, seg_b0, seg_samp0, item_seg, item_tile, mitem_seg,
-                    mitem_blk, seg_prm,   partials, states,  ticket,   idx,       samp_h,
-                    samp_l,    bounds};
+                    mitem_blk, seg_prm,   partials, states,  ticket,   idx,       pw,
+                    samp_h,    samp_l,    bounds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     seg_off = nullptr;
@@ -109,6 +110,7 @@ void BatchPlan::destroy() {
     states = nullptr;
     ticket = nullptr;
     idx = nullptr;
+    pw = nullptr;
     samp_h = nullptr;
     samp_l = nullptr;
     bounds = nullptr;
@@ -122,10 +124,10 @@ void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) c
                                              ticket, s),
                    "segmented finalize launch");
     wrs::check_hip(wrsk::launch_seg_partition(w, seg_off, item_seg, item_tile, seg_item0, items,
-                                              prec, seg_prm, ticket, states, idx, nullptr, nullptr,
-                                              samp_h, samp_l, seg_samp0, s),
+                                              prec, seg_prm, ticket, states, idx, pw, nullptr,
+                                              nullptr, samp_h, samp_l, seg_samp0, s),
                    "segmented partition launch");
-    wrs::check_hip(wrsk::launch_seg_mergepack(w, idx, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
+    wrs::check_hip(wrsk::launch_seg_mergepack(pw, idx, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
                                               nseg, nbound, mitem_seg, mitem_blk, mitems, prec,
                                               seg_prm, bounds, tables, s),
                    "segmented merge pack launch");

@@ -39,6 +39,7 @@ struct BatchPlan {
     uint64_t* states = nullptr;      // [items * kStateWords]
     uint32_t* ticket = nullptr;      // [1]
     uint32_t* idx = nullptr;         // [total_n]
+    float* pw = nullptr;             // [total_n] weights in partition order
     double* samp_h = nullptr;        // [seg_samp0[nseg]]
     double* samp_l = nullptr;        // [seg_samp0[nseg]]
     wrsk::Coarse* bounds = nullptr;  // [nbound]

@@ -65,6 +65,14 @@ inline uint64_t samples_for(uint64_t n) { return n / kSample + 1; }
 inline uint64_t merge_blocks_for(uint64_t n) { return (n + kMergeTile - 1) / kMergeTile; }
 
 inline uint64_t tiles_for(uint64_t n) { return (n + kTile - 1) / kTile; }
+// tile aggregates scanned per block by the partition's spine scan
+constexpr uint32_t kSpineTiles = 1024;
+inline uint64_t spine_blocks_for(uint64_t n) { return (tiles_for(n) + kSpineTiles - 1) / kSpineTiles; }
+// state buffer of a single-distribution build: one record per tile, then one
+// per spine block; all reset to the sentinel by the reduce pass
+inline uint64_t state_words_for(uint64_t n) {
+    return (tiles_for(n) + spine_blocks_for(n)) * kStateWords;
+}
 inline uint32_t partials_for(uint64_t n) {
     const uint64_t t = tiles_for(n);
     return (uint32_t)(t < kMaxPartials ? (t ? t : 1) : kMaxPartials);
@@ -75,14 +83,16 @@ hipError_t launch_reduce(const float* w, uint64_t n, int prec, double* partials,
                          uint64_t* states, uint64_t state_words, hipStream_t s);
 hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int prec,
                            int has_pivot, double pivot, Params* prm, hipStream_t s);
+// idx and pw (the weights in partition order) are always written; hi/lo (full
+// prefix) and the prefix samples only when non-null
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
-                            uint32_t* idx, float* hi, float* lo, double* samp_h, double* samp_l,
-                            hipStream_t s);
+                            uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
+                            double* samp_l, hipStream_t s);
 // exact splits at every kMergeTile buckets, then the LDS merge-path pack
-hipError_t launch_coarse(const float* w, uint64_t n, int prec, const Params* prm,
-                         const uint32_t* idx, const double* samp_h, const double* samp_l,
-                         Coarse* bounds, hipStream_t s);
-hipError_t launch_mergepack(const float* w, uint64_t n, int prec, const Params* prm,
+hipError_t launch_coarse(const float* pw, uint64_t n, int prec, const Params* prm,
+                         const double* samp_h, const double* samp_l, Coarse* bounds,
+                         hipStream_t s);
+hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params* prm,
                             const uint32_t* idx, const Coarse* bounds, AliasEntry* table,
                             hipStream_t s);
 // the K split points S(k), k = 1..K, of the reference's splitK layout (debug output)
@@ -112,10 +122,11 @@ hipError_t launch_seg_finalize(const uint64_t* seg_off, const uint32_t* seg_item
 hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
                                 const uint32_t* item_tile, const uint32_t* seg_item0,
                                 uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
-                                uint64_t* states, uint32_t* idx, float* hi, float* lo,
-                                double* samp_h, double* samp_l, const uint64_t* seg_samp0,
+                                uint64_t* states, uint32_t* idx, float* pw, float* hi,
+                                float* lo, double* samp_h, double* samp_l,
+                                const uint64_t* seg_samp0,
                                 hipStream_t s);
-hipError_t launch_seg_mergepack(const float* w, const uint32_t* idx, const double* samp_h,
+hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const double* samp_h,
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
                                 uint32_t nbound, const uint32_t* mitem_seg,

@@ -23,10 +23,13 @@
 //             at j = m*kSample the light rank n - j is also a multiple of
 //             kSample, so both prefixes are samples) brackets j to a window of
 //             kSample heavies; the window is resolved exactly by a wave scan of
-//             the 64 gathered heavy and light weights around it.
+//             the 64 heavy and light weights around it.  Weights are read from
+//             pw, the weights in partition order written by the partition
+//             next to the indices, so every access here is contiguous.
 //  k_mergepack one 512-thread block per kMergeTile buckets: the block's light
 //             and heavy index ranges come from its two boundary splits; their
-//             indices and gathered weights are staged in LDS, the keys X and Y
+//             indices and weights (both contiguous runs of idx and pw, loaded
+//             in one round trip) are staged in LDS, the keys X and Y
 //             are rebuilt by a compensated block scan on top of LP(i0), HP(j0),
 //             every thread finds its merge-path diagonal by a binary search in
 //             LDS and merges 4 buckets, and the entries are written back in
@@ -83,11 +86,10 @@ __device__ __forceinline__ T wave_excl_scan(T v) {
 
 }  // namespace
 
-// The exact split at n buckets of the distribution (w, idx, samples) of size
-// N with H heavies; executed by one full wave, result valid in every lane.
+// The exact split at n buckets of the distribution (pw, samples) of size N
+// with H heavies; executed by one full wave, result valid in every lane.
 template <int PREC>
-__device__ __forceinline__ Coarse coarse_split(const float* __restrict__ w,
-                                               const uint32_t* __restrict__ idx,
+__device__ __forceinline__ Coarse coarse_split(const float* __restrict__ pw,
                                                const double* __restrict__ samp_h,
                                                const double* __restrict__ samp_l, uint32_t N,
                                                uint32_t H, typename RealT<PREC>::T mean,
@@ -129,13 +131,13 @@ __device__ __forceinline__ Coarse coarse_split(const float* __restrict__ w,
     const uint32_t base = best >= 0 ? (uint32_t)best * kSample : (lo / kSample) * kSample;
     // phase 2: resolve j in [base, base + 63] with the gathered weights
     const uint32_t j = base + (uint32_t)lane;
-    const T hw = j < H ? (T)w[idx[j]] : (T)0;
+    const T hw = j < H ? (T)pw[j] : (T)0;
     const T hp = hp_sample<T>(samp_h, base) + wave_excl_scan<T>(hw);  // HP(base + lane)
     const uint32_t i_top = n - base;                                   // multiple of kSample
     const uint32_t anchor = min(i_top, L);
     const T lp_anchor = lp_sample<T>(samp_l, anchor, L, light_sum);
     const int64_t lrank = (int64_t)anchor - 1 - lane;                  // lane e: light rank anchor-1-e
-    const T lw = lrank >= 0 ? (T)w[idx[N - 1u - (uint32_t)lrank]] : (T)0;
+    const T lw = lrank >= 0 ? (T)pw[N - 1u - (uint32_t)lrank] : (T)0;
     const T lsuf = wave_excl_scan<T>(lw);  // sum of light ranks [anchor - e, anchor)
     const int64_t i = (int64_t)n - (int64_t)j;
     // every lane takes part in the shuffle; e is clamped for lanes that do not use it
@@ -162,8 +164,7 @@ __device__ __forceinline__ Coarse coarse_split(const float* __restrict__ w,
 }
 
 template <int PREC>
-__global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ w,
-                                                const uint32_t* __restrict__ idx,
+__global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 const double* __restrict__ samp_h,
                                                 const double* __restrict__ samp_l, uint32_t N,
                                                 const Params* __restrict__ prm, uint32_t nbound,
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ w,
     const uint32_t b = blockIdx.x * 4u + threadIdx.x / kWave;
     if (b >= nbound) return;  // whole wave exits together
     const uint32_t n = min(b * (uint32_t)kMergeTile, N);
-    const Coarse c = coarse_split<PREC>(w, idx, samp_h, samp_l, N, prm->heavy_count,
+    const Coarse c = coarse_split<PREC>(pw, samp_h, samp_l, N, prm->heavy_count,
                                         (T)prm->mean, (T)prm->light_sum, n);
     if (lane_id() == 0) out[b] = c;
 }
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ w,
 // segment g = item_seg-like lookup over seg_b0 (nseg + 1 prefix).
 template <int PREC>
 __global__ __launch_bounds__(256) void k_seg_coarse(
-    const float* __restrict__ w, const uint32_t* __restrict__ idx, const double* __restrict__ samp_h,
+    const float* __restrict__ pw, const double* __restrict__ samp_h,
     const double* __restrict__ samp_l, const uint64_t* __restrict__ seg_off,
     const uint64_t* __restrict__ seg_samp0, const uint32_t* __restrict__ seg_b0, uint32_t nseg,
     const Params* __restrict__ seg_prm, Coarse* __restrict__ out) {
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
     const uint32_t n = min((p - seg_b0[g]) * (uint32_t)kMergeTile, N);
     const uint64_t so = seg_samp0[g];
-    const Coarse c = coarse_split<PREC>(w + off, idx + off, samp_h + so, samp_l + so, N,
+    const Coarse c = coarse_split<PREC>(pw + off, samp_h + so, samp_l + so, N,
                                         seg_prm[g].heavy_count, (T)seg_prm[g].mean,
                                         (T)seg_prm[g].light_sum, n);
     if (lane_id() == 0) out[p] = c;
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
 
 // One block of kMergeTile buckets [n0, n1) between boundary splits c0 and c1.
 template <int PREC>
-__device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
+__device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
                                                 const uint32_t* __restrict__ idx, uint32_t N,
                                                 uint32_t H, typename RealT<PREC>::T mean,
                                                 uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
@@ -236,22 +237,17 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
     const uint32_t ne = nl + nh;    // == n1 - n0 (- 1 in the last block)
 
     // ---- stage indices and weights in rank order ------------------------
-    // slot e < nl: light i0 + e; slot nl <= e <= ne: heavy j0 + e - nl.  All
-    // index loads are issued before the dependent weight gathers.
+    // slot e < nl: light i0 + e; slot nl <= e <= ne: heavy j0 + e - nl.  Index
+    // and weight loads are independent and all issued together.
     {
         uint32_t gi[kPer + 1];
-#pragma unroll
-        for (int k = 0; k <= kPer; ++k) {
-            const uint32_t e = (uint32_t)t + (uint32_t)kMergeBlock * k;
-            gi[k] = 0;
-            if (e < nl) gi[k] = idx[N - 1u - (i0 + e)];
-            else if (e <= ne) gi[k] = idx[j0 + (e - nl)];
-        }
         float gw[kPer + 1];
 #pragma unroll
         for (int k = 0; k <= kPer; ++k) {
             const uint32_t e = (uint32_t)t + (uint32_t)kMergeBlock * k;
-            gw[k] = e < ne ? w[gi[k]] : 0.0f;
+            const uint32_t pos = e < nl ? N - 1u - (i0 + e) : j0 + (e - nl);
+            gi[k] = e <= ne ? idx[pos] : 0u;
+            gw[k] = e < ne ? pw[pos] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k <= kPer; ++k) {
@@ -412,7 +408,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ w,
 }
 
 template <int PREC>
-__global__ __launch_bounds__(kMergeBlock) void k_mergepack(const float* __restrict__ w,
+__global__ __launch_bounds__(kMergeBlock) void k_mergepack(const float* __restrict__ pw,
                                                       const uint32_t* __restrict__ idx, uint32_t N,
                                                       const Params* __restrict__ prm,
                                                       const Coarse* __restrict__ bounds,
@@ -421,13 +417,13 @@ __global__ __launch_bounds__(kMergeBlock) void k_mergepack(const float* __restri
     const uint32_t b = blockIdx.x;
     const uint32_t n0 = b * (uint32_t)kMergeTile;
     const uint32_t n1 = min(n0 + (uint32_t)kMergeTile, N);
-    mergepack_block<PREC>(w, idx, N, prm->heavy_count, (T)prm->mean, n0, n1, bounds[b],
+    mergepack_block<PREC>(pw, idx, N, prm->heavy_count, (T)prm->mean, n0, n1, bounds[b],
                           bounds[b + 1], table);
 }
 
 template <int PREC>
 __global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
-    const float* __restrict__ w, const uint32_t* __restrict__ idx,
+    const float* __restrict__ pw, const uint32_t* __restrict__ idx,
     const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ mitem_seg,
     const uint32_t* __restrict__ mitem_blk, const uint32_t* __restrict__ seg_b0,
     const Params* __restrict__ seg_prm, const Coarse* __restrict__ bounds,
@@ -440,40 +436,40 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
     const uint32_t n0 = blk * (uint32_t)kMergeTile;
     const uint32_t n1 = min(n0 + (uint32_t)kMergeTile, N);
     const Coarse* bd = bounds + seg_b0[g] + blk;
-    mergepack_block<PREC>(w + off, idx + off, N, seg_prm[g].heavy_count, (T)seg_prm[g].mean, n0,
+    mergepack_block<PREC>(pw + off, idx + off, N, seg_prm[g].heavy_count, (T)seg_prm[g].mean, n0,
                           n1, bd[0], bd[1], table + off);
 }
 
-hipError_t launch_coarse(const float* w, uint64_t n, int prec, const Params* prm,
-                         const uint32_t* idx, const double* samp_h, const double* samp_l,
-                         Coarse* bounds, hipStream_t s) {
+hipError_t launch_coarse(const float* pw, uint64_t n, int prec, const Params* prm,
+                         const double* samp_h, const double* samp_l, Coarse* bounds,
+                         hipStream_t s) {
     const uint32_t N = (uint32_t)n;
     const uint32_t nbound = (uint32_t)merge_blocks_for(n) + 1u;
     const uint32_t cgrid = (nbound + 3u) / 4u;
     if (prec == kComp)
-        k_coarse<kComp><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, N, prm, nbound, bounds);
+        k_coarse<kComp><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
     else if (prec == kFP64Carry)
-        k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, N, prm, nbound, bounds);
+        k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
     else
-        k_coarse<kFP32><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, N, prm, nbound, bounds);
+        k_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
     return hipGetLastError();
 }
 
-hipError_t launch_mergepack(const float* w, uint64_t n, int prec, const Params* prm,
+hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params* prm,
                             const uint32_t* idx, const Coarse* bounds, AliasEntry* table,
                             hipStream_t s) {
     const uint32_t N = (uint32_t)n;
     const uint32_t blocks = (uint32_t)merge_blocks_for(n);
     if (prec == kComp)
-        k_mergepack<kComp><<<blocks, kMergeBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+        k_mergepack<kComp><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
     else if (prec == kFP64Carry)
-        k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+        k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
     else
-        k_mergepack<kFP32><<<blocks, kMergeBlock, 0, s>>>(w, idx, N, prm, bounds, table);
+        k_mergepack<kFP32><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
     return hipGetLastError();
 }
 
-hipError_t launch_seg_mergepack(const float* w, const uint32_t* idx, const double* samp_h,
+hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const double* samp_h,
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
                                 uint32_t nbound, const uint32_t* mitem_seg,
@@ -483,9 +479,9 @@ hipError_t launch_seg_mergepack(const float* w, const uint32_t* idx, const doubl
     const uint32_t cgrid = (nbound + 3u) / 4u;
 #define WRS_LAUNCH_SMP(P)                                                                       \
     do {                                                                                        \
-        k_seg_coarse<P><<<cgrid, 256, 0, s>>>(w, idx, samp_h, samp_l, seg_off, seg_samp0,      \
+        k_seg_coarse<P><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0,          \
                                               seg_b0, nseg, seg_prm, bounds);                   \
-        k_seg_mergepack<P><<<mitems, kMergeBlock, 0, s>>>(w, idx, seg_off, mitem_seg, mitem_blk,     \
+        k_seg_mergepack<P><<<mitems, kMergeBlock, 0, s>>>(pw, idx, seg_off, mitem_seg, mitem_blk,    \
                                                      seg_b0, seg_prm, bounds, table);           \
     } while (0)
     if (prec == kComp) WRS_LAUNCH_SMP(kComp);

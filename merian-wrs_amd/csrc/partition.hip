@@ -21,8 +21,8 @@
 //     reversed rank) and written as two contiguous coalesced runs.
 //
 // Single distribution: reduce-then-scan.  k_part_agg writes every tile's
-// aggregate, one block scans the aggregates (k_part_tilescan), and k_part_emit
-// re-reads the tile and writes the outputs.  The 8 XCDs of the chip have
+// aggregate, k_part_tilescan scans the aggregates, and k_part_emit re-reads
+// the tile and writes the outputs.  The 8 XCDs of the chip have
 // separate L2 caches, so a single-pass chained scan pays a device-coherent
 // round trip per look-back step on every tile; one extra streaming read of
 // the weights is cheaper than that at large N.
@@ -42,12 +42,10 @@ using namespace wrsdev;
 namespace {
 
 constexpr int kPad = kTile + kTile / 16;  // LDS words of the padded transpose buffer
-constexpr int kScanThreads = 1024;        // tile-scan block
 
 // Tile carry: heavy sum (2 words), light sum (2 words), heavy count.  In the
 // look-back records the aggregate sits at words 0..4 and the inclusive prefix
-// at 8..12; in the reduce-then-scan path the aggregate sits at 0..4 and the
-// exclusive prefix at 8..12.  FP32 mode keeps each sum in one float (word
+// at 8..12 of the tile's record.  FP32 mode keeps each sum in one float (word
 // .lo; .hi = 0).
 struct Carry {
     FX h, l;
@@ -302,14 +300,16 @@ __device__ __forceinline__ TileCarry tile_carry(const Carry& excl, int shift) {
 }
 
 // Writes the tile's outputs given its carries (block-uniform): the partition
-// indices, the prefix samples at ranks r with (r+1) % kSample == 0, and the
-// full hi (and lo) prefix arrays when requested.  Expects s.buf free (the
+// indices, the weights in partition order (pw_out, when non-null), the prefix
+// samples at ranks r with (r+1) % kSample == 0, and the full hi (and lo)
+// prefix arrays when requested.  Expects s.buf free (the
 // block synchronised after its last use).
 template <int PREC>
 __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m, float pivot,
                                           const float (&v)[kItems], const Acc<PREC>& thr_excl,
                                           uint32_t hc, const TileCarry& cy, TileShared<PREC>& s,
                                           uint32_t* __restrict__ idx_out,
+                                          float* __restrict__ pw_out,
                                           float* __restrict__ hi_out, float* __restrict__ lo_out,
                                           double* __restrict__ samp_h,
                                           double* __restrict__ samp_l) {
@@ -339,6 +339,15 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
         if (e0 + k < m) s_u[slot[k]] = base + e0 + k;
     __syncthreads();
     for (uint32_t q = t; q < m; q += kBlock) idx_out[out_pos(q)] = s_u[q];
+    // round A': weights
+    if (pw_out != nullptr) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; ++k)
+            if (e0 + k < m) s.buf[slot[k]] = v[k];
+        __syncthreads();
+        for (uint32_t q = t; q < m; q += kBlock) pw_out[out_pos(q)] = s.buf[q];
+    }
 
     // round B: the inclusive prefix of every element (FP64 in the FP64 modes)
     float hi_keep[kItems], lo_keep[kItems];
@@ -402,7 +411,14 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
 
 // ---- single distribution: reduce-then-scan ---------------------------------
 
-// Pass 2a: aggregate of every tile, at words 0..4 of its record.
+// Dense record layout of the reduce-then-scan path inside the look-back state
+// buffer (tiles * kStateWords words): the aggregate of tile i at words
+// [5i, 5i+5), its exclusive prefix at [kExclBase * tiles + 5i, ...).
+constexpr uint32_t kCarryWords = 5;
+constexpr uint32_t kExclBase = 8;
+static_assert(kExclBase + kCarryWords <= kStateWords, "carry records exceed the state buffer");
+
+// Pass 2a: aggregate of every tile.
 template <int PREC, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_part_agg(const float* __restrict__ w, uint32_t n,
                                                      const Params* __restrict__ prm,
@@ -415,26 +431,46 @@ __global__ __launch_bounds__(kBlock) void k_part_agg(const float* __restrict__ w
     Acc<PREC> thr_excl, tile_agg;
     tile_scan<PREC, VEC>(w, base, m, prm->pivot, s, v, thr_excl, tile_agg);
     if (threadIdx.x == 0)
-        store_carry(states + (uint64_t)tile * kStateWords, to_carry<PREC>(tile_agg, prm->shift));
+        store_carry(states + (uint64_t)tile * kCarryWords, to_carry<PREC>(tile_agg, prm->shift));
 }
 
-// Pass 2b: exclusive scan of the tile aggregates (one block; thread t takes a
-// contiguous run of tiles), exclusive prefixes at words 8..12 of each record,
-// totals into prm.  The FP64 modes add exactly; FP32 mode adds in a fixed order.
+// Pass 2b: exclusive scan of the tile aggregates.  Block g scans tiles
+// [g*kSpineTiles, (g+1)*kSpineTiles) (thread t takes kSpinePer consecutive
+// tiles, all loaded at once) and chains to its predecessors by decoupled
+// look-back over per-block records; every aggregate already exists, so the
+// chain resolves in about one round trip.  The last block writes the totals
+// into prm.  The FP64 modes add exactly, so the result does not depend on the
+// look-back order.
+constexpr int kSpineThreads = 256;
+constexpr int kSpinePer = 4;
+static_assert(kSpineThreads * kSpinePer == (int)kSpineTiles, "spine block shape");
+
 template <int PREC>
-__global__ __launch_bounds__(kScanThreads) void k_part_tilescan(uint64_t* __restrict__ states,
-                                                                uint32_t tiles,
-                                                                Params* __restrict__ prm) {
-    __shared__ Carry s_wave[kScanThreads / kWave];
+__global__ __launch_bounds__(kSpineThreads) void k_part_tilescan(uint64_t* __restrict__ states,
+                                                                 uint32_t tiles,
+                                                                 Params* __restrict__ prm) {
+    __shared__ Carry s_wave[kSpineThreads / kWave];
+    __shared__ Carry s_excl;
+    __shared__ uint32_t s_blk;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
-    const uint32_t per = (tiles + kScanThreads - 1) / kScanThreads;
-    const uint32_t b = min(tiles, (uint32_t)t * per);
-    const uint32_t e = min(tiles, b + per);
-    Carry mine = carry_zero();
-    for (uint32_t i = b; i < e; ++i)
-        mine = carry_add<PREC>(mine, load_carry(states + (uint64_t)i * kStateWords));
+    if (t == 0) s_blk = atomicAdd(&prm->ticket, 1u);  // predecessors are scheduled first
+    __syncthreads();
+    const uint32_t blk = s_blk;
+    const uint32_t nblk = (tiles + kSpineTiles - 1) / kSpineTiles;
+    const uint64_t* agg = states;
+    uint64_t* excl = states + (uint64_t)kExclBase * tiles;
+    uint64_t* recs = states + (uint64_t)tiles * kStateWords;  // block records after the tiles'
+
+    const uint32_t b = blk * kSpineTiles + (uint32_t)t * kSpinePer;
+    Carry a[kSpinePer];
+#pragma unroll
+    for (int k = 0; k < kSpinePer; ++k)
+        a[k] = b + k < tiles ? load_carry(agg + (uint64_t)(b + k) * kCarryWords) : carry_zero();
+    Carry mine = a[0];
+#pragma unroll
+    for (int k = 1; k < kSpinePer; ++k) mine = carry_add<PREC>(mine, a[k]);
     Carry incl = mine;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
@@ -445,21 +481,38 @@ __global__ __launch_bounds__(kScanThreads) void k_part_tilescan(uint64_t* __rest
     if (lane == 0) lane_excl = carry_zero();
     if (lane == kWave - 1) s_wave[wid] = incl;
     __syncthreads();
-    Carry run = carry_zero();
+    if (wid == 0) {
+        Carry blk_agg = s_wave[0];
+#pragma unroll
+        for (int q = 1; q < kSpineThreads / kWave; ++q) blk_agg = carry_add<PREC>(blk_agg, s_wave[q]);
+        uint64_t* rec = recs + (uint64_t)blk * kStateWords;
+        Carry ex = carry_zero();
+        if (blk == 0) {
+            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, blk_agg);
+        } else {
+            if (lane == 0) publish_aggregate<Carry, CarryOps<PREC>>(rec, blk_agg);
+            ex = lookback<Carry, CarryOps<PREC>>(recs, blk);
+            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, carry_add<PREC>(ex, blk_agg));
+        }
+        if (lane == 0) {
+            s_excl = ex;
+            if (blk == nblk - 1) {
+                const Carry tot = carry_add<PREC>(ex, blk_agg);
+                const int shift = prm->shift;
+                prm->heavy_count = (uint32_t)tot.c;
+                prm->heavy_sum = carry_value<PREC>(tot.h, shift);
+                prm->light_sum = carry_value<PREC>(tot.l, shift);
+            }
+        }
+    }
+    __syncthreads();
+    Carry run = s_excl;
     for (int q = 0; q < wid; ++q) run = carry_add<PREC>(run, s_wave[q]);
     run = carry_add<PREC>(run, lane_excl);
-    for (uint32_t i = b; i < e; ++i) {
-        uint64_t* rec = states + (uint64_t)i * kStateWords;
-        const Carry agg = load_carry(rec);
-        store_carry(rec + 8, run);
-        run = carry_add<PREC>(run, agg);
-    }
-    if (t == kScanThreads - 1) {
-        // the last thread's run now covers every tile
-        const int shift = prm->shift;
-        prm->heavy_count = (uint32_t)run.c;
-        prm->heavy_sum = carry_value<PREC>(run.h, shift);
-        prm->light_sum = carry_value<PREC>(run.l, shift);
+#pragma unroll
+    for (int k = 0; k < kSpinePer; ++k) {
+        if (b + k < tiles) store_carry(excl + (uint64_t)(b + k) * kCarryWords, run);
+        run = carry_add<PREC>(run, a[k]);
     }
 }
 
@@ -469,6 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
                                                       const Params* __restrict__ prm,
                                                       const uint64_t* __restrict__ states,
                                                       uint32_t* __restrict__ idx_out,
+                                                      float* __restrict__ pw_out,
                                                       float* __restrict__ hi_out,
                                                       float* __restrict__ lo_out,
                                                       double* __restrict__ samp_h,
@@ -478,14 +532,16 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const float pivot = prm->pivot;
-    const Carry excl = load_carry(states + (uint64_t)tile * kStateWords + 8);
+    const uint32_t tiles = (n + kTile - 1) / (uint32_t)kTile;
+    const Carry excl =
+        load_carry(states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords);
     float v[kItems];
     Acc<PREC> thr_excl, tile_agg;
     tile_scan<PREC, VEC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
     const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
     __syncthreads();  // s.buf is reused by the emit rounds
-    tile_emit<PREC>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, hi_out, lo_out,
-                    samp_h, samp_l);
+    tile_emit<PREC>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, pw_out, hi_out,
+                    lo_out, samp_h, samp_l);
 }
 
 // ---- segmented: single pass with decoupled look-back -----------------------
@@ -500,7 +556,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile,
     const uint32_t* __restrict__ seg_item0, Params* __restrict__ seg_prm,
     uint32_t* __restrict__ ticket, uint64_t* __restrict__ states, uint32_t* __restrict__ idx,
-    float* __restrict__ hi, float* __restrict__ lo, double* __restrict__ samp_h,
+    float* __restrict__ pw, float* __restrict__ hi, float* __restrict__ lo,
+    double* __restrict__ samp_h,
     double* __restrict__ samp_l, const uint64_t* __restrict__ seg_samp0) {
     __shared__ TileShared<PREC> s;
     __shared__ uint32_t s_item;
@@ -548,22 +605,23 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     }
     __syncthreads();
     const TileCarry cy = tile_carry<PREC>(s_excl, shift);
-    tile_emit<PREC>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off,
+    tile_emit<PREC>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off, pw + off,
                     hi ? hi + off : nullptr, lo ? lo + off : nullptr, samp_h + so, samp_l + so);
 }
 
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
-                            uint32_t* idx, float* hi, float* lo, double* samp_h, double* samp_l,
-                            hipStream_t s) {
+                            uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
+                            double* samp_l, hipStream_t s) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
     const bool vec = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     const uint32_t nn = (uint32_t)n;
 #define WRS_LAUNCH_PART(P, V)                                                                  \
     do {                                                                                       \
         k_part_agg<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states);                         \
-        k_part_tilescan<P><<<1, kScanThreads, 0, s>>>(states, tiles, prm);                     \
-        k_part_emit<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, hi, lo, samp_h,    \
-                                                   samp_l);                                    \
+        k_part_tilescan<P><<<(tiles + kSpineTiles - 1) / kSpineTiles, kSpineThreads, 0, s>>>(  \
+            states, tiles, prm);                                                               \
+        k_part_emit<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, pw, hi, lo,        \
+                                                   samp_h, samp_l);                            \
     } while (0)
     if (prec == kComp) {
         if (vec) WRS_LAUNCH_PART(kComp, true);
@@ -582,12 +640,12 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
 hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
                                 const uint32_t* item_tile, const uint32_t* seg_item0,
                                 uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
-                                uint64_t* states, uint32_t* idx, float* hi, float* lo,
-                                double* samp_h, double* samp_l, const uint64_t* seg_samp0,
-                                hipStream_t s) {
+                                uint64_t* states, uint32_t* idx, float* pw, float* hi,
+                                float* lo, double* samp_h, double* samp_l,
+                                const uint64_t* seg_samp0, hipStream_t s) {
 #define WRS_LAUNCH_SEG(P)                                                                 \
     k_seg_partition<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_item0, \
-                                                seg_prm, ticket, states, idx, hi, lo,       \
+                                                seg_prm, ticket, states, idx, pw, hi, lo,   \
                                                 samp_h, samp_l, seg_samp0)
     if (prec == kComp) WRS_LAUNCH_SEG(kComp);
     else if (prec == kFP64Carry) WRS_LAUNCH_SEG(kFP64Carry);

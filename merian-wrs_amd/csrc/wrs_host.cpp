@@ -42,10 +42,11 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
     try {
         ws.params = dev_alloc<wrsk::Params>(1);
         ws.partials = dev_alloc<double>(wrsk::kMaxPartials);
-        ws.state_words = wrsk::tiles_for(max_n) * wrsk::kStateWords;
+        ws.state_words = wrsk::state_words_for(max_n);
         ws.states = dev_alloc<uint64_t>(ws.state_words);
         if (partition_buffers) {
             ws.indices = dev_alloc<uint32_t>(max_n);
+            ws.pweights = dev_alloc<float>(max_n);
             ws.prefix_hi = dev_alloc<float>(max_n);
             if (lo_buffer) ws.prefix_lo = dev_alloc<float>(max_n);
             ws.samp_h = dev_alloc<double>(wrsk::samples_for(max_n));
@@ -60,13 +61,15 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
 }
 
 void Workspace::release() {
-    void* ptrs[] = {params, partials, states, indices, prefix_hi, prefix_lo, samp_h, samp_l, bounds};
+    void* ptrs[] = {params,    partials,  states, indices, pweights,
+                    prefix_hi, prefix_lo, samp_h, samp_l,  bounds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     params = nullptr;
     partials = nullptr;
     states = nullptr;
     indices = nullptr;
+    pweights = nullptr;
     prefix_hi = nullptr;
     prefix_lo = nullptr;
     samp_h = nullptr;
@@ -82,7 +85,7 @@ void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, d
     require(weights != nullptr, "weights is null");
     require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
     const uint32_t G = wrsk::partials_for(N);
-    const uint64_t words = wrsk::tiles_for(N) * wrsk::kStateWords;
+    const uint64_t words = wrsk::state_words_for(N);
     check_hip(wrsk::launch_reduce(weights, N, (int)m_prec, ws.partials, G, ws.states, words, s),
               "reduce launch");
     const bool has_pivot = !std::isnan(pivot);
@@ -95,7 +98,7 @@ void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace
     require(b.elements && b.partitionIndices && b.partitionPrefix, "null partition buffer");
     require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
     check_hip(wrsk::launch_partition(b.elements, N, (int)m_prec, ws.params, ws.states,
-                                     b.partitionIndices, b.partitionPrefix,
+                                     b.partitionIndices, nullptr, b.partitionPrefix,
                                      m_prec == WRS_PREC_COMPENSATED ? b.partitionPrefixLo : nullptr,
                                      nullptr, nullptr, s),
               "partition launch");
@@ -124,7 +127,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     float* hi = full_prefix ? ws.prefix_hi : nullptr;
     float* lo = full_prefix && prec == WRS_PREC_COMPENSATED ? ws.prefix_lo : nullptr;
     const uint32_t G = wrsk::partials_for(N);
-    const uint64_t words = wrsk::tiles_for(N) * wrsk::kStateWords;
+    const uint64_t words = wrsk::state_words_for(N);
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
     check_hip(wrsk::launch_reduce(b.weights, N, (int)prec, ws.partials, G, ws.states, words, s),
               "reduce launch");
@@ -132,15 +135,15 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     check_hip(wrsk::launch_finalize(ws.partials, G, N, (int)prec, 0, 0.0, ws.params, s),
               "finalize launch");
     if (ev) check_hip(hipEventRecord(ev[2], s), "event");
-    check_hip(wrsk::launch_partition(b.weights, N, (int)prec, ws.params, ws.states, ws.indices, hi,
-                                     lo, ws.samp_h, ws.samp_l, s),
+    check_hip(wrsk::launch_partition(b.weights, N, (int)prec, ws.params, ws.states, ws.indices,
+                                     ws.pweights, hi, lo, ws.samp_h, ws.samp_l, s),
               "partition launch");
     if (ev) check_hip(hipEventRecord(ev[3], s), "event");
-    check_hip(wrsk::launch_coarse(b.weights, N, (int)prec, ws.params, ws.indices, ws.samp_h,
-                                  ws.samp_l, ws.bounds, s),
+    check_hip(wrsk::launch_coarse(ws.pweights, N, (int)prec, ws.params, ws.samp_h, ws.samp_l,
+                                  ws.bounds, s),
               "coarse split launch");
     if (ev) check_hip(hipEventRecord(ev[4], s), "event");
-    check_hip(wrsk::launch_mergepack(b.weights, N, (int)prec, ws.params, ws.indices, ws.bounds,
+    check_hip(wrsk::launch_mergepack(ws.pweights, N, (int)prec, ws.params, ws.indices, ws.bounds,
                                      reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable), s),
               "merge pack launch");
     if (ev) check_hip(hipEventRecord(ev[5], s), "event");

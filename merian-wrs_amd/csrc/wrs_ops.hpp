@@ -42,9 +42,10 @@ struct Workspace {
     uint64_t max_n = 0;
     wrsk::Params* params = nullptr;    // [1]
     double* partials = nullptr;        // [kMaxPartials]
-    uint64_t* states = nullptr;        // [tiles * kStateWords]
+    uint64_t* states = nullptr;        // [state_words_for(max_n)]
     uint64_t state_words = 0;
     uint32_t* indices = nullptr;       // [max_n] (PSA only)
+    float* pweights = nullptr;         // [max_n] weights in partition order (PSA only)
     float* prefix_hi = nullptr;        // [max_n] (PSA debug builds only)
     float* prefix_lo = nullptr;        // [max_n] (PSA debug builds only, compensated)
     double* samp_h = nullptr;          // [max_n/kSample + 1] heavy prefix samples (PSA only)
