@@ -3,7 +3,7 @@
 
 One step = one PSA alias-table build (FP64 total -> heavy/light prefix-partition
 -> block-boundary splits -> merge pack) of N = 2^26 Zipf(1.1) FP32 weights (ranks a seeded
-permutation of 1..N, seed 2), compensated FP32 with the exact FP64 carry,
+permutation of 1..N, seed 2), FP64 tile sums with the exact fixed-point carry,
 split size 32, inputs resident in HBM.  The value is the build throughput in
 GB/s of algorithmic bytes B_build = 32N + 12K (K = N/32): mean read 4N,
 partition read 4N + index 4N + prefix 4N, split 12K, pack index 4N + weight 4N
@@ -251,7 +251,7 @@ def main() -> int:
         "config": {
             "workload": f"PSA alias build, N=2^{args.log2n} Zipf(1.1) FP32 weights over a seeded "
                         f"rank permutation (seed 2), split size {args.split_size}, "
-                        f"{['fp32', 'fp64-carry', 'compensated fp32 + fp64 carry'][args.precision]}",
+                        f"{['fp32', 'fp64-carry', 'fp64'][args.precision]}",
             "n": n, "split_size": args.split_size, "B_build_bytes": B_build,
             "parallelism": "single GPU",
         },

@@ -12,7 +12,7 @@ namespace wrsk {
 enum Prec : int {
     kFP32 = 0,       // FP32 everywhere, FP32 look-back carry (the reference GPU's arithmetic)
     kFP64Carry = 1,  // FP32 in-tile scan, exact fixed-point carry read out in FP64, FP64 mean/split
-    kComp = 2,       // compensated FP32 in-tile scan (hi+lo) + the kFP64Carry carry and split
+    kFP64 = 2,       // FP64 in-tile sums + the kFP64Carry carry and split (the default)
 };
 
 // Per-build scalars, resident in device memory so that a whole build can be

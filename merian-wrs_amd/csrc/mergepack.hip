@@ -30,7 +30,7 @@
 //             and heavy index ranges come from its two boundary splits; their
 //             indices and weights (both contiguous runs of idx and pw, loaded
 //             in one round trip) are staged in LDS, the keys X and Y
-//             are rebuilt by a compensated block scan on top of LP(i0), HP(j0),
+//             are rebuilt by a block scan on top of LP(i0), HP(j0),
 //             every thread finds its merge-path diagonal by a binary search in
 //             LDS and merges 4 buckets, and the entries are written back in
 //             rank order (coalesced: consecutive light ranks are consecutive
@@ -38,8 +38,10 @@
 //             in pack.hpp:78-81.
 //
 // In the FP64 modes keys, residuals and split arithmetic are FP64 (the block
-// scan itself is compensated FP32 in kComp, plain FP32 in kFP64Carry); in FP32
-// mode everything is FP32, as in the reference kernels.
+// scan itself is FP64 in kFP64, FP32 in kFP64Carry); in FP32 mode everything
+// is FP32, as in the reference kernels.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -219,7 +221,9 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     __shared__ uint2 s_raw[kMergeTile + 2];
     __shared__ uint32_t s_idx[kMergeTile + 2];
     __shared__ float s_w[kMergeTile];
-    __shared__ DF s_wave_l[kMergeBlock / kWave], s_wave_h[kMergeBlock / kWave];
+    // block-scan accumulator: FP64 in kFP64, FP32 otherwise
+    using S = typename std::conditional<PREC == kFP64, double, float>::type;
+    __shared__ S s_wave_l[kMergeBlock / kWave], s_wave_h[kMergeBlock / kWave];
     T* s_key = reinterpret_cast<T*>(s_raw);
 
     const int t = threadIdx.x;
@@ -259,100 +263,72 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     __syncthreads();
 
     // ---- keys: light r -> X(i0 + r) (exclusive LP), heavy q -> Y(j0 + q + 1)
-    DF ml{0.0f, 0.0f}, mh{0.0f, 0.0f};
+    S ml = 0, mh = 0;
     const uint32_t e0 = (uint32_t)(kPer * t);
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t e = e0 + k;
         if (e < ne) {
-            const float x = s_w[e];
-            if constexpr (PREC == kComp) {
-                if (e < nl) ml = df_add1(ml, x);
-                else mh = df_add1(mh, x);
-            } else {
-                if (e < nl) ml.hi += x;
-                else mh.hi += x;
-            }
+            const S x = (S)s_w[e];
+            if (e < nl) ml += x;
+            else mh += x;
         }
     }
-    DF il = ml, ih = mh;
+    S il = ml, ih = mh;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
-        DF ol, oh;
-        ol.hi = shfl_up(il.hi, d);
-        oh.hi = shfl_up(ih.hi, d);
-        ol.lo = shfl_up(il.lo, d);
-        oh.lo = shfl_up(ih.lo, d);
+        const S ol = shfl_up(il, d);
+        const S oh = shfl_up(ih, d);
         if (lane >= d) {
-            if constexpr (PREC == kComp) {
-                il = df_add(ol, il);
-                ih = df_add(oh, ih);
-            } else {
-                il.hi = ol.hi + il.hi;
-                ih.hi = oh.hi + ih.hi;
-            }
+            il = ol + il;
+            ih = oh + ih;
         }
     }
-    DF xl, xh;  // exclusive per lane
-    xl.hi = shfl_up(il.hi, 1);
-    xl.lo = shfl_up(il.lo, 1);
-    xh.hi = shfl_up(ih.hi, 1);
-    xh.lo = shfl_up(ih.lo, 1);
-    if (lane == 0) xl = xh = DF{0.0f, 0.0f};
+    S rl = shfl_up(il, 1), rh = shfl_up(ih, 1);  // exclusive per lane
+    if (lane == 0) rl = rh = 0;
     if (lane == kWave - 1) {
         s_wave_l[wid] = il;
         s_wave_h[wid] = ih;
     }
     __syncthreads();
-    DF rl{0.0f, 0.0f}, rh{0.0f, 0.0f}, tot_l{0.0f, 0.0f};
+    S wl = 0, wh = 0, tot_l = 0;
     for (int q = 0; q < kMergeBlock / kWave; ++q) {
-        if constexpr (PREC == kComp) {
-            if (q < wid) {
-                rl = df_add(rl, s_wave_l[q]);
-                rh = df_add(rh, s_wave_h[q]);
-            }
-            tot_l = df_add(tot_l, s_wave_l[q]);
-        } else {
-            if (q < wid) {
-                rl.hi += s_wave_l[q].hi;
-                rh.hi += s_wave_h[q].hi;
-            }
-            tot_l.hi += s_wave_l[q].hi;
+        if (q < wid) {
+            wl += s_wave_l[q];
+            wh += s_wave_h[q];
         }
+        tot_l += s_wave_l[q];
     }
-    if constexpr (PREC == kComp) {
-        rl = df_add(rl, xl);
-        rh = df_add(rh, xh);
-    } else {
-        rl.hi += xl.hi;
-        rh.hi += xh.hi;
-    }
+    rl = wl + rl;
+    rh = wh + rh;
     const T lp0 = (T)c0.lp, hp0 = (T)c0.hp;
-    auto val = [&](DF v) -> T {
-        if constexpr (PREC == kComp) return (T)df_value(v);
-        else return (T)v.hi;
-    };
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t e = e0 + k;
         if (e < ne) {
-            const float x = s_w[e];
+            const S x = (S)s_w[e];
             if (e < nl) {  // exclusive prefix, then add
-                s_key[e] = (T)(i0 + e) * mean - (lp0 + val(rl));
-                if constexpr (PREC == kComp) rl = df_add1(rl, x);
-                else rl.hi += x;
+                s_key[e] = (T)(i0 + e) * mean - (lp0 + (T)rl);
+                rl += x;
             } else {       // inclusive prefix
-                if constexpr (PREC == kComp) rh = df_add1(rh, x);
-                else rh.hi += x;
+                rh += x;
                 const uint32_t q = e - nl;
-                s_key[nl + 1u + q] = (hp0 + val(rh)) - (T)(j0 + q + 1u) * mean;
+                s_key[nl + 1u + q] = (hp0 + (T)rh) - (T)(j0 + q + 1u) * mean;
             }
         }
     }
-    if (t == 0) s_key[nl] = (T)(i0 + nl) * mean - (lp0 + val(tot_l));  // X(i1)
+    if (t == 0) s_key[nl] = (T)(i0 + nl) * mean - (lp0 + (T)tot_l);  // X(i1)
     __syncthreads();
 
     // ---- merge: thread t emits buckets [kPer*t, kPer*t + kPer) ----------
+    // p = residual / mean: FP32 mode divides as the reference does; the FP64
+    // modes multiply by the FP64 reciprocal (within 1.5 ulp of the FP64
+    // quotient before the rounding to float)
+    const T inv_mean = (T)1 / mean;
+    auto ratio = [&](T x) -> float {
+        if constexpr (PREC == kFP32) return (float)(x / mean);
+        else return (float)(x * inv_mean);
+    };
     const T* A = s_key;            // A[r], r in [0, nl]
     const T* B = s_key + nl + 1u;  // B[q], q in [0, nh)
     const uint32_t d0 = e0;
@@ -380,12 +356,12 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
                 const bool take_heavy = q < nh && (r >= nl || B[q] <= A[r]);
                 if (take_heavy) {
                     const T resid = (B[q] - A[r]) + mean;
-                    const float p = (float)(resid / mean);
+                    const float p = ratio(resid);
                     slot[k] = nl + q;
                     ent[k] = make_uint2(__float_as_uint(p), s_idx[nl + q + 1u]);
                     ++q;
                 } else {
-                    const float p = (float)((T)s_w[r] / mean);
+                    const float p = ratio((T)s_w[r]);
                     slot[k] = r;
                     ent[k] = make_uint2(__float_as_uint(p), s_idx[nl + q]);
                     ++r;
@@ -446,8 +422,8 @@ hipError_t launch_coarse(const float* pw, uint64_t n, int prec, const Params* pr
     const uint32_t N = (uint32_t)n;
     const uint32_t nbound = (uint32_t)merge_blocks_for(n) + 1u;
     const uint32_t cgrid = (nbound + 3u) / 4u;
-    if (prec == kComp)
-        k_coarse<kComp><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
+    if (prec == kFP64)
+        k_coarse<kFP64><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
     else if (prec == kFP64Carry)
         k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
     else
@@ -460,8 +436,8 @@ hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params*
                             hipStream_t s) {
     const uint32_t N = (uint32_t)n;
     const uint32_t blocks = (uint32_t)merge_blocks_for(n);
-    if (prec == kComp)
-        k_mergepack<kComp><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
+    if (prec == kFP64)
+        k_mergepack<kFP64><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
     else if (prec == kFP64Carry)
         k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
     else
@@ -484,7 +460,7 @@ hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const doub
         k_seg_mergepack<P><<<mitems, kMergeBlock, 0, s>>>(pw, idx, seg_off, mitem_seg, mitem_blk,    \
                                                      seg_b0, seg_prm, bounds, table);           \
     } while (0)
-    if (prec == kComp) WRS_LAUNCH_SMP(kComp);
+    if (prec == kFP64) WRS_LAUNCH_SMP(kFP64);
     else if (prec == kFP64Carry) WRS_LAUNCH_SMP(kFP64Carry);
     else WRS_LAUNCH_SMP(kFP32);
 #undef WRS_LAUNCH_SMP

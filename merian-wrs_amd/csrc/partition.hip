@@ -13,8 +13,9 @@
 //   * coalesced float4 loads, transposed through LDS (row stride 17 words, so
 //     both the transpose writes and the per-thread reads avoid bank
 //     conflicts) so that each thread owns 16 consecutive elements;
-//   * per-thread sequential (compensated) sums, wave64 shuffle scan, then a
-//     4-entry cross-wave scan in LDS, all in a fixed order;
+//   * per-thread sequential sums (FP64 in the default mode, FP32 otherwise),
+//     wave64 shuffle scan, then a 4-entry cross-wave scan in LDS, all in a
+//     fixed order;
 //   * tile carries are 126-bit fixed-point sums (exact, so their bits do not
 //     depend on the order in which tiles are combined) plus an exact count;
 //   * outputs are staged in LDS in final order (heavies by rank, lights by
@@ -135,47 +136,38 @@ struct CarryOps {
     static __device__ __forceinline__ void publish(uint64_t* r, const Carry& v) { publish_rec(r, v); }
 };
 
-// In-tile per-thread accumulator: compensated (DF) or plain float.
+// In-tile accumulator: FP64 in the default mode (a float weight converts to
+// FP64 exactly, and a 4096-term FP64 sum is within 2^-41 of exact relative to
+// its magnitude), FP32 in the other modes.  On gfx950 an FP64 add issues at
+// the rate of an unpacked FP32 add.
+template <int PREC>
+struct AccReal {
+    using T = float;
+};
+template <>
+struct AccReal<kFP64> {
+    using T = double;
+};
+
 template <int PREC>
 struct Acc {
-    DF h, l;
+    typename AccReal<PREC>::T h, l;
     uint32_t c;
 };
 
 template <int PREC>
+__device__ __forceinline__ Acc<PREC> acc_zero() {
+    return Acc<PREC>{0, 0, 0u};
+}
+
+template <int PREC>
 __device__ __forceinline__ Acc<PREC> acc_add(Acc<PREC> a, Acc<PREC> b) {
-    Acc<PREC> r;
-    if constexpr (PREC == kComp) {
-        r.h = df_add(a.h, b.h);
-        r.l = df_add(a.l, b.l);
-    } else {
-        r.h = DF{a.h.hi + b.h.hi, 0.0f};
-        r.l = DF{a.l.hi + b.l.hi, 0.0f};
-    }
-    r.c = a.c + b.c;
-    return r;
+    return Acc<PREC>{a.h + b.h, a.l + b.l, a.c + b.c};
 }
 
 template <int PREC>
 __device__ __forceinline__ Acc<PREC> acc_shfl_up(Acc<PREC> v, int d) {
-    Acc<PREC> r;
-    r.h.hi = shfl_up(v.h.hi, d);
-    r.l.hi = shfl_up(v.l.hi, d);
-    if constexpr (PREC == kComp) {
-        r.h.lo = shfl_up(v.h.lo, d);
-        r.l.lo = shfl_up(v.l.lo, d);
-    } else {
-        r.h.lo = 0.0f;
-        r.l.lo = 0.0f;
-    }
-    r.c = shfl_up(v.c, d);
-    return r;
-}
-
-template <int PREC>
-__device__ __forceinline__ double stream_value(DF v) {
-    if constexpr (PREC == kComp) return df_value(v);
-    else return (double)v.hi;
+    return Acc<PREC>{shfl_up(v.h, d), shfl_up(v.l, d), shfl_up(v.c, d)};
 }
 
 // Tile aggregate as a carry word set.
@@ -183,11 +175,11 @@ template <int PREC>
 __device__ __forceinline__ Carry to_carry(const Acc<PREC>& a, int shift) {
     Carry r;
     if constexpr (PREC == kFP32) {
-        r.h = FX{(uint64_t)__float_as_uint(a.h.hi), 0};
-        r.l = FX{(uint64_t)__float_as_uint(a.l.hi), 0};
+        r.h = FX{(uint64_t)__float_as_uint(a.h), 0};
+        r.l = FX{(uint64_t)__float_as_uint(a.l), 0};
     } else {
-        r.h = fx_from_double(stream_value<PREC>(a.h), shift);
-        r.l = fx_from_double(stream_value<PREC>(a.l), shift);
+        r.h = fx_from_double((double)a.h, shift);
+        r.l = fx_from_double((double)a.l, shift);
     }
     r.c = a.c;
     return r;
@@ -243,18 +235,14 @@ __device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t 
     for (int k = 0; k < kItems; ++k) v[k] = s.buf[17 * t + k];
     const uint32_t e0 = (uint32_t)(kItems * t);
 
-    Acc<PREC> mine{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
+    using R = typename AccReal<PREC>::T;
+    Acc<PREC> mine = acc_zero<PREC>();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         if (e0 + k < m) {
             const bool heavy = v[k] > pivot;
-            if constexpr (PREC == kComp) {
-                if (heavy) mine.h = df_add1(mine.h, v[k]);
-                else mine.l = df_add1(mine.l, v[k]);
-            } else {
-                if (heavy) mine.h.hi += v[k];
-                else mine.l.hi += v[k];
-            }
+            if (heavy) mine.h += (R)v[k];
+            else mine.l += (R)v[k];
             mine.c += heavy ? 1u : 0u;
         }
     }
@@ -265,10 +253,10 @@ __device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t 
         if (lane >= d) incl = acc_add<PREC>(o, incl);
     }
     Acc<PREC> lane_excl = acc_shfl_up<PREC>(incl, 1);
-    if (lane == 0) lane_excl = Acc<PREC>{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
+    if (lane == 0) lane_excl = acc_zero<PREC>();
     if (lane == kWave - 1) s.wave[wid] = incl;
     __syncthreads();
-    Acc<PREC> wave_excl{DF{0.0f, 0.0f}, DF{0.0f, 0.0f}, 0u};
+    Acc<PREC> wave_excl = acc_zero<PREC>();
     for (int q = 0; q < wid; ++q) wave_excl = acc_add<PREC>(wave_excl, s.wave[q]);
     tile_agg = s.wave[0];
 #pragma unroll
@@ -352,29 +340,26 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
     // round B: the inclusive prefix of every element (FP64 in the FP64 modes)
     float hi_keep[kItems], lo_keep[kItems];
     {
-        DF rh = thr_excl.h, rl = thr_excl.l;
+        using R = typename AccReal<PREC>::T;
+        R rh = thr_excl.h, rl = thr_excl.l;
         uint32_t hr = Hx + thr_excl.c;         // global heavy rank of the next heavy
         uint32_t lr = Lx + (e0 - thr_excl.c);  // global light rank of the next light
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const bool heavy = v[k] > pivot;
+            if (heavy) rh += (R)v[k];
+            else rl += (R)v[k];
             double P;
             float hi, lo = 0.0f;
-            if constexpr (PREC == kComp) {
-                if (heavy) rh = df_add1(rh, v[k]);
-                else rl = df_add1(rl, v[k]);
-                P = (heavy ? cy.h : cy.l) + df_value(heavy ? rh : rl);
+            if constexpr (PREC == kFP64) {
+                P = (heavy ? cy.h : cy.l) + (heavy ? rh : rl);
                 hi = (float)P;
                 lo = (float)(P - (double)hi);
             } else if constexpr (PREC == kFP64Carry) {
-                if (heavy) rh.hi += v[k];
-                else rl.hi += v[k];
-                P = (heavy ? cy.h : cy.l) + (double)(heavy ? rh.hi : rl.hi);
+                P = (heavy ? cy.h : cy.l) + (double)(heavy ? rh : rl);
                 hi = (float)P;
             } else {
-                if (heavy) rh.hi += v[k];
-                else rl.hi += v[k];
-                hi = (heavy ? cy.hf : cy.lf) + (heavy ? rh.hi : rl.hi);
+                hi = (heavy ? cy.hf : cy.lf) + (heavy ? rh : rl);
                 P = (double)hi;
             }
             hi_keep[k] = hi;
@@ -394,7 +379,7 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
             if (e0 + k < m) s.buf[slot[k]] = hi_keep[k];
         __syncthreads();
         for (uint32_t q = t; q < m; q += kBlock) hi_out[out_pos(q)] = s.buf[q];
-        if constexpr (PREC == kComp) {
+        if constexpr (PREC == kFP64) {
             if (lo_out != nullptr) {
                 __syncthreads();
 #pragma unroll
@@ -623,9 +608,9 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
         k_part_emit<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, pw, hi, lo,        \
                                                    samp_h, samp_l);                            \
     } while (0)
-    if (prec == kComp) {
-        if (vec) WRS_LAUNCH_PART(kComp, true);
-        else WRS_LAUNCH_PART(kComp, false);
+    if (prec == kFP64) {
+        if (vec) WRS_LAUNCH_PART(kFP64, true);
+        else WRS_LAUNCH_PART(kFP64, false);
     } else if (prec == kFP64Carry) {
         if (vec) WRS_LAUNCH_PART(kFP64Carry, true);
         else WRS_LAUNCH_PART(kFP64Carry, false);
@@ -647,7 +632,7 @@ hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const u
     k_seg_partition<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_item0, \
                                                 seg_prm, ticket, states, idx, pw, hi, lo,   \
                                                 samp_h, samp_l, seg_samp0)
-    if (prec == kComp) WRS_LAUNCH_SEG(kComp);
+    if (prec == kFP64) WRS_LAUNCH_SEG(kFP64);
     else if (prec == kFP64Carry) WRS_LAUNCH_SEG(kFP64Carry);
     else WRS_LAUNCH_SEG(kFP32);
 #undef WRS_LAUNCH_SEG

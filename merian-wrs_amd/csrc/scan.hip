@@ -9,10 +9,12 @@
 // Same tile machinery as the prefix-partition: 4096-element tiles, dynamic
 // ticket, LDS transpose to 16 consecutive elements per thread, wave64 scan,
 // tail-safe loads and stores (the reference kernel has no tail check,
-// shader.comp:70-92,181-197).  The carry is the 126-bit fixed-point word pair
+// shader.comp:70-92,181-197).  In-tile sums are FP64 in the default mode.  The carry is the 126-bit fixed-point word pair
 // (exact, so its bits do not depend on look-back timing) in the FP64 modes and
 // an FP32 sum in FP32 mode.  The output is staged back through LDS and written
 // as coalesced float4 rows.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "lookback.hpp"
@@ -71,7 +73,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
                                                  float* __restrict__ lo_out) {
     __shared__ float s_buf[kPad];
     __shared__ uint32_t s_tile;
-    __shared__ DF s_wave[kBlock / kWave];
+    // in-tile accumulator: FP64 in the default mode, FP32 otherwise
+    using R = typename std::conditional<PREC == kFP64, double, float>::type;
+    __shared__ R s_wave[kBlock / kWave];
     __shared__ double s_carry;
     __shared__ float s_carry_f;
 
@@ -114,46 +118,32 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
     for (int k = 0; k < kItems; ++k) v[k] = s_buf[17 * t + k];
     const uint32_t e0 = (uint32_t)(kItems * t);
 
-    DF mine{0.0f, 0.0f};
+    R mine = 0;
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        if (e0 + k < m) {
-            if constexpr (PREC == kComp) mine = df_add1(mine, v[k]);
-            else mine.hi += v[k];
-        }
-    }
-    DF incl = mine;
+    for (int k = 0; k < kItems; ++k)
+        if (e0 + k < m) mine += (R)v[k];
+    R incl = mine;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
-        DF o;
-        o.hi = shfl_up(incl.hi, d);
-        o.lo = PREC == kComp ? shfl_up(incl.lo, d) : 0.0f;
-        if (lane >= d) incl = PREC == kComp ? df_add(o, incl) : DF{o.hi + incl.hi, 0.0f};
+        const R o = shfl_up(incl, d);
+        if (lane >= d) incl = o + incl;
     }
-    DF lane_excl;
-    lane_excl.hi = shfl_up(incl.hi, 1);
-    lane_excl.lo = PREC == kComp ? shfl_up(incl.lo, 1) : 0.0f;
-    if (lane == 0) lane_excl = DF{0.0f, 0.0f};
+    R lane_excl = shfl_up(incl, 1);
+    if (lane == 0) lane_excl = 0;
     if (lane == kWave - 1) s_wave[wid] = incl;
     __syncthreads();
-    DF wave_excl{0.0f, 0.0f};
-    for (int q = 0; q < wid; ++q)
-        wave_excl = PREC == kComp ? df_add(wave_excl, s_wave[q])
-                                  : DF{wave_excl.hi + s_wave[q].hi, 0.0f};
-    DF tile_agg = s_wave[0];
+    R wave_excl = 0;
+    for (int q = 0; q < wid; ++q) wave_excl += s_wave[q];
+    R tile_agg = s_wave[0];
 #pragma unroll
-    for (int q = 1; q < kBlock / kWave; ++q)
-        tile_agg = PREC == kComp ? df_add(tile_agg, s_wave[q])
-                                 : DF{tile_agg.hi + s_wave[q].hi, 0.0f};
-    const DF thr_excl = PREC == kComp ? df_add(wave_excl, lane_excl)
-                                      : DF{wave_excl.hi + lane_excl.hi, 0.0f};
+    for (int q = 1; q < kBlock / kWave; ++q) tile_agg += s_wave[q];
+    const R thr_excl = wave_excl + lane_excl;
 
     uint64_t* rec = states + (uint64_t)tile * kStateWords;
     if (wid == 0) {
         FX agg;
-        if constexpr (PREC == kFP32) agg = FX{(uint64_t)__float_as_uint(tile_agg.hi), 0};
-        else if constexpr (PREC == kComp) agg = fx_from_double(df_value(tile_agg), shift);
-        else agg = fx_from_double((double)tile_agg.hi, shift);
+        if constexpr (PREC == kFP32) agg = FX{(uint64_t)__float_as_uint((float)tile_agg), 0};
+        else agg = fx_from_double((double)tile_agg, shift);
         FX excl = fx_zero();
         if (tile == 0) {
             if (lane == 0) publish_inclusive<FX, ScanOps<PREC>>(rec, agg);
@@ -171,23 +161,21 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
 
     float lo_keep[kItems];
     {
-        DF run = thr_excl;
+        R run = thr_excl;
         const double c = s_carry;
         const float cf = s_carry_f;
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             float hi, lo = 0.0f;
-            if constexpr (PREC == kComp) {
-                run = df_add1(run, v[k]);
-                const double P = (c + df_value(run)) * scale;
+            run += (R)v[k];
+            if constexpr (PREC == kFP64) {
+                const double P = (c + run) * scale;
                 hi = (float)P;
                 lo = (float)(P - (double)hi);
             } else if constexpr (PREC == kFP64Carry) {
-                run.hi += v[k];
-                hi = (float)((c + (double)run.hi) * scale);
+                hi = (float)((c + (double)run) * scale);
             } else {
-                run.hi += v[k];
-                hi = (float)((double)(cf + run.hi) * scale);
+                hi = (float)((double)(cf + run) * scale);
             }
             lo_keep[k] = lo;
             s_buf[17 * t + k] = hi;
@@ -206,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
     } else {
         for (uint32_t e = t; e < m; e += kBlock) hi_out[base + e] = s_buf[e + e / 16];
     }
-    if (PREC == kComp && lo_out != nullptr) {
+    if (PREC == kFP64 && lo_out != nullptr) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kItems; ++k) s_buf[17 * t + k] = lo_keep[k];
@@ -234,9 +222,9 @@ hipError_t launch_scan(const float* w, uint64_t n, int prec, int normalise, Para
     const uint32_t nn = (uint32_t)n;
 #define WRS_LAUNCH_SCAN(P, V) \
     k_scan<P, V><<<tiles, kBlock, 0, s>>>(w, nn, normalise, prm, states, hi, lo)
-    if (prec == kComp) {
-        if (vec) WRS_LAUNCH_SCAN(kComp, true);
-        else WRS_LAUNCH_SCAN(kComp, false);
+    if (prec == kFP64) {
+        if (vec) WRS_LAUNCH_SCAN(kFP64, true);
+        else WRS_LAUNCH_SCAN(kFP64, false);
     } else if (prec == kFP64Carry) {
         if (vec) WRS_LAUNCH_SCAN(kFP64Carry, true);
         else WRS_LAUNCH_SCAN(kFP64Carry, false);

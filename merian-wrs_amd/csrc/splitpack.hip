@@ -12,7 +12,7 @@
 // split array; this kernel writes the K split points of a build for the
 // parity tests, reading the full prefix arrays the debug build requests.  In
 // the FP64 modes the target mean*n, the sigmas and the spill are FP64 (the
-// compensated prefix is read as hi + lo); in FP32 mode they are FP32.
+// FP64-mode prefix is read as hi + lo); in FP32 mode they are FP32.
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -35,7 +35,7 @@ __device__ __forceinline__ typename Real<PREC>::T pre_at(const float* __restrict
                                                          const float* __restrict__ lo,
                                                          uint32_t q) {
     if constexpr (PREC == kFP32) return hi[q];
-    else if constexpr (PREC == kComp) return (double)hi[q] + (double)lo[q];
+    else if constexpr (PREC == kFP64) return (double)hi[q] + (double)lo[q];
     else return (double)hi[q];
 }
 
@@ -118,8 +118,8 @@ hipError_t launch_splits(uint64_t n, uint32_t K, int prec, const Params* prm, co
                          const float* lo, SplitRec* splits, hipStream_t s) {
     const uint32_t blocks = (K + kSplitBlock - 1) / kSplitBlock;
     const uint32_t N = (uint32_t)n;
-    if (prec == kComp)
-        k_splits<kComp><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, hi, lo, splits);
+    if (prec == kFP64)
+        k_splits<kFP64><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, hi, lo, splits);
     else if (prec == kFP64Carry)
         k_splits<kFP64Carry><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, hi, lo, splits);
     else

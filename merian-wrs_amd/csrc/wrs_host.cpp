@@ -99,7 +99,7 @@ void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace
     require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
     check_hip(wrsk::launch_partition(b.elements, N, (int)m_prec, ws.params, ws.states,
                                      b.partitionIndices, nullptr, b.partitionPrefix,
-                                     m_prec == WRS_PREC_COMPENSATED ? b.partitionPrefixLo : nullptr,
+                                     m_prec == WRS_PREC_FP64 ? b.partitionPrefixLo : nullptr,
                                      nullptr, nullptr, s),
               "partition launch");
 }
@@ -110,7 +110,7 @@ void PrefixSum::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) 
     Mean(m_prec).run(s, b.elements, N, ws);
     check_hip(wrsk::launch_scan(b.elements, N, (int)m_prec, m_normalise ? 1 : 0, ws.params,
                                 ws.states, b.prefix,
-                                m_prec == WRS_PREC_COMPENSATED ? b.prefixLo : nullptr, s),
+                                m_prec == WRS_PREC_FP64 ? b.prefixLo : nullptr, s),
               "scan launch");
 }
 
@@ -125,7 +125,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     // itself needs the indices and the sparse prefix samples
     const bool full_prefix = dbg && (dbg->prefix_hi || dbg->prefix_lo || dbg->splits);
     float* hi = full_prefix ? ws.prefix_hi : nullptr;
-    float* lo = full_prefix && prec == WRS_PREC_COMPENSATED ? ws.prefix_lo : nullptr;
+    float* lo = full_prefix && prec == WRS_PREC_FP64 ? ws.prefix_lo : nullptr;
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::state_words_for(N);
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
@@ -150,7 +150,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     if (dbg) {
         if (dbg->splits)
             check_hip(wrsk::launch_splits(N, splitCount(N), (int)prec, ws.params, ws.prefix_hi,
-                                          prec == WRS_PREC_COMPENSATED ? ws.prefix_lo : nullptr,
+                                          prec == WRS_PREC_FP64 ? ws.prefix_lo : nullptr,
                                           reinterpret_cast<wrsk::SplitRec*>(dbg->splits), s),
                       "splits launch");
         if (dbg->mean)
@@ -288,7 +288,7 @@ wrs_status wrs_plan_create(const wrs_config* cfg, uint64_t max_n, wrs_plan** out
         wrs_plan* p = new wrs_plan;
         p->cfg = c;
         try {
-            p->ws = wrs::Workspace::allocate(max_n, true, c.precision == WRS_PREC_COMPENSATED);
+            p->ws = wrs::Workspace::allocate(max_n, true, c.precision == WRS_PREC_FP64);
             for (auto& e : p->ev) wrs::check_hip(hipEventCreate(&e), "hipEventCreate");
         } catch (...) {
             p->ws.release();
