@@ -32,6 +32,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "merian-wrs_amd"))
 
 HBM_PEAK_GBS = 8000.0
+PREC_NAMES = ['fp32', 'fp64-carry', 'compensated-fp32', 'fp64-tiles']
 METRIC = "PSA build GB/s (% HBM peak) + samples/s; batched vs per-launch at fixed error"
 
 
@@ -251,7 +252,7 @@ def main() -> int:
         "config": {
             "workload": f"PSA alias build, N=2^{args.log2n} Zipf(1.1) FP32 weights over a seeded "
                         f"rank permutation (seed 2), split size {args.split_size}, "
-                        f"{['fp32', 'fp64-carry', 'fp64'][args.precision]}",
+                        f"{PREC_NAMES[args.precision]}",
             "n": n, "split_size": args.split_size, "B_build_bytes": B_build,
             "parallelism": "single GPU",
         },

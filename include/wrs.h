@@ -37,7 +37,8 @@ typedef enum {
 typedef enum {
     WRS_PREC_FP32 = 0,        /* FP32 mean, scan, carry, split and pack (reference arithmetic) */
     WRS_PREC_FP64_CARRY = 1,  /* FP64 total/mean, exact look-back carry, FP64 split/pack, FP32 tile scan */
-    WRS_PREC_FP64 = 2,        /* as FP64_CARRY with FP64 tile sums; prefixes exported as hi+lo floats: default */
+    WRS_PREC_COMPENSATED = 2, /* as FP64_CARRY with compensated (two-sum hi+lo) FP32 tile sums: default */
+    WRS_PREC_FP64 = 3,        /* as FP64_CARRY with FP64 tile sums */
 } wrs_precision;
 
 /* Alias table row, 8 bytes: keep bucket i with probability p, else take a.
@@ -61,7 +62,7 @@ typedef struct {
  * whose splitSize sets the split granularity. */
 typedef struct {
     uint32_t split_size;     /* split granularity of the debug split output (K = ceil(N/split_size)); 0 -> 32 */
-    wrs_precision precision; /* default WRS_PREC_FP64 */
+    wrs_precision precision; /* default WRS_PREC_COMPENSATED */
 } wrs_config;
 
 typedef struct wrs_plan wrs_plan;
@@ -88,7 +89,7 @@ typedef struct {
     uint32_t* heavy_count; /* [1]  H */
     uint32_t* indices;     /* [n]  partition indices, heavies at [0,H), light r at n-1-r */
     float* prefix_hi;      /* [n]  partition prefixes, same layout */
-    float* prefix_lo;      /* [n]  low words, prefix = hi + lo (WRS_PREC_FP64 only) */
+    float* prefix_lo;      /* [n]  low words, prefix = hi + lo (WRS_PREC_COMPENSATED and WRS_PREC_FP64) */
     wrs_split* splits;     /* [K]  splits[k-1] = S(k), k = 1..K, splits[K-1] = (L, H-1, 0) */
 } wrs_psa_debug;
 

@@ -25,7 +25,7 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_
     split_size = split_size_ ? split_size_ : 32u;
     prec = prec_;
     nseg = nseg_;
-    if (prec < 0 || prec > 2) throw wrs::Error(WRS_EINVAL, "unknown precision");
+    if (prec < wrsk::kFP32 || prec > wrsk::kFP64) throw wrs::Error(WRS_EINVAL, "unknown precision");
     offsets.assign(offsets_host, offsets_host + nseg + 1);
     if (offsets[0] != 0) throw wrs::Error(WRS_EINVAL, "offsets[0] must be 0");
     std::vector<uint32_t> h_item_seg, h_item_tile, h_mitem_seg, h_mitem_blk;
@@ -124,8 +124,8 @@ void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) c
                                              ticket, s),
                    "segmented finalize launch");
     wrs::check_hip(wrsk::launch_seg_partition(w, seg_off, item_seg, item_tile, seg_item0, items,
-                                              prec, seg_prm, ticket, states, idx, pw, nullptr,
-                                              nullptr, samp_h, samp_l, seg_samp0, s),
+                                              prec, seg_prm, ticket, states, idx, pw, samp_h,
+                                              samp_l, seg_samp0, s),
                    "segmented partition launch");
     wrs::check_hip(wrsk::launch_seg_mergepack(pw, idx, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
                                               nseg, nbound, mitem_seg, mitem_blk, mitems, prec,

@@ -3,6 +3,8 @@
 //  * Philox4x32-10 (Random123, Salmon et al. SC'11), counter-based, so every
 //    sample / element draws its own counter and the stream is independent of
 //    the launch shape.
+//  * error-free transformation two_sum and the double-float ("compensated
+//    FP32") combine used by the compensated tile sums.
 //  * a 126-bit fixed-point accumulator (two 63-bit limbs) for the inter-tile
 //    look-back carry: integer addition is associative, so the carry bits do
 //    not depend on the order in which the look-back happened to find its
@@ -65,6 +67,32 @@ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
 __device__ __forceinline__ double u53_open0(uint32_t hi, uint32_t lo) {
     return (double)(((((uint64_t)hi << 32) | lo) >> 11) + 1ull) * 0x1.0p-53;
 }
+
+// ------------------------------------------------- compensated FP32 ----
+// two_sum: s + e == a + b exactly (Knuth, TAOCP vol. 2, 4.2.2).  Built with
+// -ffp-contract=off and no reassociation, so the operation order below is what
+// executes.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+struct DF {  // unevaluated sum hi + lo
+    float hi, lo;
+};
+
+__device__ __forceinline__ DF df_add(DF a, DF b) {
+    float s, e;
+    two_sum(a.hi, b.hi, s, e);
+    return DF{s, (a.lo + b.lo) + e};
+}
+__device__ __forceinline__ DF df_add1(DF a, float x) {
+    float s, e;
+    two_sum(a.hi, x, s, e);
+    return DF{s, a.lo + e};
+}
+__device__ __forceinline__ double df_value(DF a) { return (double)a.hi + (double)a.lo; }
 
 // ---------------------------------------------------- 126-bit fixed point ----
 struct FX {  // value = hi * 2^63 + lo, lo < 2^63, hi < 2^63

@@ -12,8 +12,11 @@ namespace wrsk {
 enum Prec : int {
     kFP32 = 0,       // FP32 everywhere, FP32 look-back carry (the reference GPU's arithmetic)
     kFP64Carry = 1,  // FP32 in-tile scan, exact fixed-point carry read out in FP64, FP64 mean/split
-    kFP64 = 2,       // FP64 in-tile sums + the kFP64Carry carry and split (the default)
+    kComp = 2,       // compensated FP32 in-tile sums (two_sum hi+lo) + the kFP64Carry carry and split
+    kFP64 = 3,       // FP64 in-tile sums + the kFP64Carry carry and split
 };
+// the modes whose prefixes are exported as a hi + lo float pair
+constexpr bool prec_has_lo(int prec) { return prec >= kComp; }
 
 // Per-build scalars, resident in device memory so that a whole build can be
 // recorded on a stream (or captured in a graph) without host round trips.
@@ -122,10 +125,8 @@ hipError_t launch_seg_finalize(const uint64_t* seg_off, const uint32_t* seg_item
 hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
                                 const uint32_t* item_tile, const uint32_t* seg_item0,
                                 uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
-                                uint64_t* states, uint32_t* idx, float* pw, float* hi,
-                                float* lo, double* samp_h, double* samp_l,
-                                const uint64_t* seg_samp0,
-                                hipStream_t s);
+                                uint64_t* states, uint32_t* idx, float* pw, double* samp_h,
+                                double* samp_l, const uint64_t* seg_samp0, hipStream_t s);
 hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const double* samp_h,
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,

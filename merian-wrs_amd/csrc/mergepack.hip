@@ -37,13 +37,14 @@
 //             ascending indices).  The last heavy of all gets (1, itself), as
 //             in pack.hpp:78-81.
 //
-// In the FP64 modes keys, residuals and split arithmetic are FP64 (the block
-// scan itself is FP64 in kFP64, FP32 in kFP64Carry); in FP32 mode everything
-// is FP32, as in the reference kernels.
+// In the FP64-carry modes keys, residuals and split arithmetic are FP64 (the
+// block scan itself runs in the mode's tile arithmetic: FP32, compensated FP32
+// or FP64); in FP32 mode everything is FP32, as in the reference kernels.
 #include <type_traits>
 
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "tilesum.hpp"
 
 namespace wrsk {
 using namespace wrsdev;
@@ -221,8 +222,9 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     __shared__ uint2 s_raw[kMergeTile + 2];
     __shared__ uint32_t s_idx[kMergeTile + 2];
     __shared__ float s_w[kMergeTile];
-    // block-scan accumulator: FP64 in kFP64, FP32 otherwise
-    using S = typename std::conditional<PREC == kFP64, double, float>::type;
+    // block-scan accumulator in the mode's tile arithmetic (tilesum.hpp)
+    using A1 = Acc1<PREC>;
+    using S = typename A1::T;
     __shared__ S s_wave_l[kMergeBlock / kWave], s_wave_h[kMergeBlock / kWave];
     T* s_key = reinterpret_cast<T*>(s_raw);
 
@@ -263,61 +265,61 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     __syncthreads();
 
     // ---- keys: light r -> X(i0 + r) (exclusive LP), heavy q -> Y(j0 + q + 1)
-    S ml = 0, mh = 0;
+    S ml = A1::zero(), mh = A1::zero();
     const uint32_t e0 = (uint32_t)(kPer * t);
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t e = e0 + k;
         if (e < ne) {
-            const S x = (S)s_w[e];
-            if (e < nl) ml += x;
-            else mh += x;
+            const float x = s_w[e];
+            if (e < nl) ml = A1::add1(ml, x);
+            else mh = A1::add1(mh, x);
         }
     }
     S il = ml, ih = mh;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
-        const S ol = shfl_up(il, d);
-        const S oh = shfl_up(ih, d);
+        const S ol = A1::shfl_up(il, d);
+        const S oh = A1::shfl_up(ih, d);
         if (lane >= d) {
-            il = ol + il;
-            ih = oh + ih;
+            il = A1::add(ol, il);
+            ih = A1::add(oh, ih);
         }
     }
-    S rl = shfl_up(il, 1), rh = shfl_up(ih, 1);  // exclusive per lane
-    if (lane == 0) rl = rh = 0;
+    S rl = A1::shfl_up(il, 1), rh = A1::shfl_up(ih, 1);  // exclusive per lane
+    if (lane == 0) rl = rh = A1::zero();
     if (lane == kWave - 1) {
         s_wave_l[wid] = il;
         s_wave_h[wid] = ih;
     }
     __syncthreads();
-    S wl = 0, wh = 0, tot_l = 0;
+    S wl = A1::zero(), wh = A1::zero(), tot_l = A1::zero();
     for (int q = 0; q < kMergeBlock / kWave; ++q) {
         if (q < wid) {
-            wl += s_wave_l[q];
-            wh += s_wave_h[q];
+            wl = A1::add(wl, s_wave_l[q]);
+            wh = A1::add(wh, s_wave_h[q]);
         }
-        tot_l += s_wave_l[q];
+        tot_l = A1::add(tot_l, s_wave_l[q]);
     }
-    rl = wl + rl;
-    rh = wh + rh;
+    rl = A1::add(wl, rl);
+    rh = A1::add(wh, rh);
     const T lp0 = (T)c0.lp, hp0 = (T)c0.hp;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t e = e0 + k;
         if (e < ne) {
-            const S x = (S)s_w[e];
+            const float x = s_w[e];
             if (e < nl) {  // exclusive prefix, then add
-                s_key[e] = (T)(i0 + e) * mean - (lp0 + (T)rl);
-                rl += x;
+                s_key[e] = (T)(i0 + e) * mean - (lp0 + (T)A1::value(rl));
+                rl = A1::add1(rl, x);
             } else {       // inclusive prefix
-                rh += x;
+                rh = A1::add1(rh, x);
                 const uint32_t q = e - nl;
-                s_key[nl + 1u + q] = (hp0 + (T)rh) - (T)(j0 + q + 1u) * mean;
+                s_key[nl + 1u + q] = (hp0 + (T)A1::value(rh)) - (T)(j0 + q + 1u) * mean;
             }
         }
     }
-    if (t == 0) s_key[nl] = (T)(i0 + nl) * mean - (lp0 + (T)tot_l);  // X(i1)
+    if (t == 0) s_key[nl] = (T)(i0 + nl) * mean - (lp0 + (T)A1::value(tot_l));  // X(i1)
     __syncthreads();
 
     // ---- merge: thread t emits buckets [kPer*t, kPer*t + kPer) ----------
@@ -422,12 +424,11 @@ hipError_t launch_coarse(const float* pw, uint64_t n, int prec, const Params* pr
     const uint32_t N = (uint32_t)n;
     const uint32_t nbound = (uint32_t)merge_blocks_for(n) + 1u;
     const uint32_t cgrid = (nbound + 3u) / 4u;
-    if (prec == kFP64)
-        k_coarse<kFP64><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
-    else if (prec == kFP64Carry)
-        k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
-    else
+    // the split arithmetic is FP32 in FP32 mode and FP64 in all others
+    if (prec == kFP32)
         k_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
+    else
+        k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
     return hipGetLastError();
 }
 
@@ -436,12 +437,20 @@ hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params*
                             hipStream_t s) {
     const uint32_t N = (uint32_t)n;
     const uint32_t blocks = (uint32_t)merge_blocks_for(n);
-    if (prec == kFP64)
-        k_mergepack<kFP64><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
-    else if (prec == kFP64Carry)
-        k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
-    else
-        k_mergepack<kFP32><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
+    switch (prec) {
+        case kFP32:
+            k_mergepack<kFP32><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
+            break;
+        case kFP64Carry:
+            k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
+            break;
+        case kComp:
+            k_mergepack<kComp><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
+            break;
+        default:
+            k_mergepack<kFP64><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
+            break;
+    }
     return hipGetLastError();
 }
 
@@ -460,9 +469,12 @@ hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const doub
         k_seg_mergepack<P><<<mitems, kMergeBlock, 0, s>>>(pw, idx, seg_off, mitem_seg, mitem_blk,    \
                                                      seg_b0, seg_prm, bounds, table);           \
     } while (0)
-    if (prec == kFP64) WRS_LAUNCH_SMP(kFP64);
-    else if (prec == kFP64Carry) WRS_LAUNCH_SMP(kFP64Carry);
-    else WRS_LAUNCH_SMP(kFP32);
+    switch (prec) {
+        case kFP32: WRS_LAUNCH_SMP(kFP32); break;
+        case kFP64Carry: WRS_LAUNCH_SMP(kFP64Carry); break;
+        case kComp: WRS_LAUNCH_SMP(kComp); break;
+        default: WRS_LAUNCH_SMP(kFP64); break;
+    }
 #undef WRS_LAUNCH_SMP
     return hipGetLastError();
 }

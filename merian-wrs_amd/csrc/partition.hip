@@ -36,6 +36,7 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "lookback.hpp"
+#include "tilesum.hpp"
 
 namespace wrsk {
 using namespace wrsdev;
@@ -136,38 +137,27 @@ struct CarryOps {
     static __device__ __forceinline__ void publish(uint64_t* r, const Carry& v) { publish_rec(r, v); }
 };
 
-// In-tile accumulator: FP64 in the default mode (a float weight converts to
-// FP64 exactly, and a 4096-term FP64 sum is within 2^-41 of exact relative to
-// its magnitude), FP32 in the other modes.  On gfx950 an FP64 add issues at
-// the rate of an unpacked FP32 add.
-template <int PREC>
-struct AccReal {
-    using T = float;
-};
-template <>
-struct AccReal<kFP64> {
-    using T = double;
-};
-
+// In-tile accumulator: the heavy and light running sums in the mode's
+// arithmetic (tilesum.hpp) and the heavy count.
 template <int PREC>
 struct Acc {
-    typename AccReal<PREC>::T h, l;
+    typename Acc1<PREC>::T h, l;
     uint32_t c;
 };
 
 template <int PREC>
 __device__ __forceinline__ Acc<PREC> acc_zero() {
-    return Acc<PREC>{0, 0, 0u};
+    return Acc<PREC>{Acc1<PREC>::zero(), Acc1<PREC>::zero(), 0u};
 }
 
 template <int PREC>
 __device__ __forceinline__ Acc<PREC> acc_add(Acc<PREC> a, Acc<PREC> b) {
-    return Acc<PREC>{a.h + b.h, a.l + b.l, a.c + b.c};
+    return Acc<PREC>{Acc1<PREC>::add(a.h, b.h), Acc1<PREC>::add(a.l, b.l), a.c + b.c};
 }
 
 template <int PREC>
 __device__ __forceinline__ Acc<PREC> acc_shfl_up(Acc<PREC> v, int d) {
-    return Acc<PREC>{shfl_up(v.h, d), shfl_up(v.l, d), shfl_up(v.c, d)};
+    return Acc<PREC>{Acc1<PREC>::shfl_up(v.h, d), Acc1<PREC>::shfl_up(v.l, d), shfl_up(v.c, d)};
 }
 
 // Tile aggregate as a carry word set.
@@ -178,8 +168,8 @@ __device__ __forceinline__ Carry to_carry(const Acc<PREC>& a, int shift) {
         r.h = FX{(uint64_t)__float_as_uint(a.h), 0};
         r.l = FX{(uint64_t)__float_as_uint(a.l), 0};
     } else {
-        r.h = fx_from_double((double)a.h, shift);
-        r.l = fx_from_double((double)a.l, shift);
+        r.h = fx_from_double(Acc1<PREC>::value(a.h), shift);
+        r.l = fx_from_double(Acc1<PREC>::value(a.l), shift);
     }
     r.c = a.c;
     return r;
@@ -235,14 +225,14 @@ __device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t 
     for (int k = 0; k < kItems; ++k) v[k] = s.buf[17 * t + k];
     const uint32_t e0 = (uint32_t)(kItems * t);
 
-    using R = typename AccReal<PREC>::T;
+    using A1 = Acc1<PREC>;
     Acc<PREC> mine = acc_zero<PREC>();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         if (e0 + k < m) {
             const bool heavy = v[k] > pivot;
-            if (heavy) mine.h += (R)v[k];
-            else mine.l += (R)v[k];
+            if (heavy) mine.h = A1::add1(mine.h, v[k]);
+            else mine.l = A1::add1(mine.l, v[k]);
             mine.c += heavy ? 1u : 0u;
         }
     }
@@ -289,10 +279,11 @@ __device__ __forceinline__ TileCarry tile_carry(const Carry& excl, int shift) {
 
 // Writes the tile's outputs given its carries (block-uniform): the partition
 // indices, the weights in partition order (pw_out, when non-null), the prefix
-// samples at ranks r with (r+1) % kSample == 0, and the full hi (and lo)
-// prefix arrays when requested.  Expects s.buf free (the
-// block synchronised after its last use).
-template <int PREC>
+// samples at ranks r with (r+1) % kSample == 0, and with FULL the hi (and lo)
+// prefix arrays of every element.  Without FULL a prefix is formed only where
+// a sample falls.  Expects s.buf free (the block synchronised after its last
+// use).
+template <int PREC, bool FULL>
 __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m, float pivot,
                                           const float (&v)[kItems], const Acc<PREC>& thr_excl,
                                           uint32_t hc, const TileCarry& cy, TileShared<PREC>& s,
@@ -337,49 +328,45 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
         for (uint32_t q = t; q < m; q += kBlock) pw_out[out_pos(q)] = s.buf[q];
     }
 
-    // round B: the inclusive prefix of every element (FP64 in the FP64 modes)
+    // round B: inclusive prefixes (FP64 in the FP64-carry modes)
+    using A1 = Acc1<PREC>;
+    auto prefix = [&](bool heavy, const typename A1::T& run) -> double {
+        if constexpr (PREC == kFP32) return (double)((heavy ? cy.hf : cy.lf) + run);
+        else return (heavy ? cy.h : cy.l) + A1::value(run);
+    };
     float hi_keep[kItems], lo_keep[kItems];
     {
-        using R = typename AccReal<PREC>::T;
-        R rh = thr_excl.h, rl = thr_excl.l;
+        typename A1::T rh = thr_excl.h, rl = thr_excl.l;
         uint32_t hr = Hx + thr_excl.c;         // global heavy rank of the next heavy
         uint32_t lr = Lx + (e0 - thr_excl.c);  // global light rank of the next light
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const bool heavy = v[k] > pivot;
-            if (heavy) rh += (R)v[k];
-            else rl += (R)v[k];
-            double P;
-            float hi, lo = 0.0f;
-            if constexpr (PREC == kFP64) {
-                P = (heavy ? cy.h : cy.l) + (heavy ? rh : rl);
-                hi = (float)P;
-                lo = (float)(P - (double)hi);
-            } else if constexpr (PREC == kFP64Carry) {
-                P = (heavy ? cy.h : cy.l) + (double)(heavy ? rh : rl);
-                hi = (float)P;
-            } else {
-                hi = (heavy ? cy.hf : cy.lf) + (heavy ? rh : rl);
-                P = (double)hi;
-            }
-            hi_keep[k] = hi;
-            lo_keep[k] = lo;
-            if (e0 + k < m && samp_h != nullptr) {
-                const uint32_t r1 = (heavy ? hr : lr) + 1u;  // prefix covers ranks < r1
-                if ((r1 & (kSample - 1)) == 0) (heavy ? samp_h : samp_l)[r1 / kSample - 1u] = P;
+            if (heavy) rh = A1::add1(rh, v[k]);
+            else rl = A1::add1(rl, v[k]);
+            const uint32_t r1 = (heavy ? hr : lr) + 1u;  // the prefix covers ranks < r1
+            const bool sample = e0 + k < m && samp_h != nullptr && (r1 & (kSample - 1)) == 0;
+            if constexpr (FULL) {
+                const double P = prefix(heavy, heavy ? rh : rl);
+                const float hi = (float)P;
+                hi_keep[k] = hi;
+                lo_keep[k] = prec_has_lo(PREC) ? (float)(P - (double)hi) : 0.0f;
+                if (sample) (heavy ? samp_h : samp_l)[r1 / kSample - 1u] = P;
+            } else if (sample) {
+                (heavy ? samp_h : samp_l)[r1 / kSample - 1u] = prefix(heavy, heavy ? rh : rl);
             }
             hr += heavy ? 1u : 0u;
             lr += heavy ? 0u : 1u;
         }
     }
-    if (hi_out != nullptr) {
+    if (FULL && hi_out != nullptr) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kItems; ++k)
             if (e0 + k < m) s.buf[slot[k]] = hi_keep[k];
         __syncthreads();
         for (uint32_t q = t; q < m; q += kBlock) hi_out[out_pos(q)] = s.buf[q];
-        if constexpr (PREC == kFP64) {
+        if constexpr (prec_has_lo(PREC)) {
             if (lo_out != nullptr) {
                 __syncthreads();
 #pragma unroll
@@ -502,7 +489,7 @@ __global__ __launch_bounds__(kSpineThreads) void k_part_tilescan(uint64_t* __res
 }
 
 // Pass 2c: outputs of every tile from its exclusive prefix.
-template <int PREC, bool VEC>
+template <int PREC, bool VEC, bool FULL>
 __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ w, uint32_t n,
                                                       const Params* __restrict__ prm,
                                                       const uint64_t* __restrict__ states,
@@ -525,8 +512,8 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
     tile_scan<PREC, VEC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
     const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
     __syncthreads();  // s.buf is reused by the emit rounds
-    tile_emit<PREC>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, pw_out, hi_out,
-                    lo_out, samp_h, samp_l);
+    tile_emit<PREC, FULL>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, pw_out,
+                          hi_out, lo_out, samp_h, samp_l);
 }
 
 // ---- segmented: single pass with decoupled look-back -----------------------
@@ -534,16 +521,15 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
 // One work item per (segment, tile), items of a segment are consecutive and
 // handed out in ticket order, so a tile's predecessors in its segment always
 // hold smaller tickets.  Segment outputs use segment-local indices and occupy
-// the segment's own range of idx/hi/lo.
+// the segment's own range of idx/pw.
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_seg_partition(
     const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
     const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile,
     const uint32_t* __restrict__ seg_item0, Params* __restrict__ seg_prm,
     uint32_t* __restrict__ ticket, uint64_t* __restrict__ states, uint32_t* __restrict__ idx,
-    float* __restrict__ pw, float* __restrict__ hi, float* __restrict__ lo,
-    double* __restrict__ samp_h,
-    double* __restrict__ samp_l, const uint64_t* __restrict__ seg_samp0) {
+    float* __restrict__ pw, double* __restrict__ samp_h, double* __restrict__ samp_l,
+    const uint64_t* __restrict__ seg_samp0) {
     __shared__ TileShared<PREC> s;
     __shared__ uint32_t s_item;
     __shared__ Carry s_excl;
@@ -590,51 +576,75 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     }
     __syncthreads();
     const TileCarry cy = tile_carry<PREC>(s_excl, shift);
-    tile_emit<PREC>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off, pw + off,
-                    hi ? hi + off : nullptr, lo ? lo + off : nullptr, samp_h + so, samp_l + so);
+    tile_emit<PREC, false>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off,
+                           pw + off, nullptr, nullptr, samp_h + so, samp_l + so);
 }
+
+namespace {
+
+template <int P, bool V>
+void launch_part_t(const float* w, uint32_t n, Params* prm, uint64_t* states, uint32_t* idx,
+                   float* pw, float* hi, float* lo, double* samp_h, double* samp_l,
+                   hipStream_t s) {
+    const uint32_t tiles = (uint32_t)tiles_for(n);
+    k_part_agg<P, V><<<tiles, kBlock, 0, s>>>(w, n, prm, states);
+    k_part_tilescan<P><<<(tiles + kSpineTiles - 1) / kSpineTiles, kSpineThreads, 0, s>>>(
+        states, tiles, prm);
+    if (hi)
+        k_part_emit<P, V, true><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, hi, lo,
+                                                         samp_h, samp_l);
+    else
+        k_part_emit<P, V, false><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, nullptr,
+                                                          nullptr, samp_h, samp_l);
+}
+
+template <int P>
+void launch_part_p(bool vec, const float* w, uint32_t n, Params* prm, uint64_t* states,
+                   uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
+                   double* samp_l, hipStream_t s) {
+    if (vec) launch_part_t<P, true>(w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
+    else launch_part_t<P, false>(w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
+}
+
+}  // namespace
 
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
                             uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
                             double* samp_l, hipStream_t s) {
-    const uint32_t tiles = (uint32_t)tiles_for(n);
     const bool vec = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     const uint32_t nn = (uint32_t)n;
-#define WRS_LAUNCH_PART(P, V)                                                                  \
-    do {                                                                                       \
-        k_part_agg<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states);                         \
-        k_part_tilescan<P><<<(tiles + kSpineTiles - 1) / kSpineTiles, kSpineThreads, 0, s>>>(  \
-            states, tiles, prm);                                                               \
-        k_part_emit<P, V><<<tiles, kBlock, 0, s>>>(w, nn, prm, states, idx, pw, hi, lo,        \
-                                                   samp_h, samp_l);                            \
-    } while (0)
-    if (prec == kFP64) {
-        if (vec) WRS_LAUNCH_PART(kFP64, true);
-        else WRS_LAUNCH_PART(kFP64, false);
-    } else if (prec == kFP64Carry) {
-        if (vec) WRS_LAUNCH_PART(kFP64Carry, true);
-        else WRS_LAUNCH_PART(kFP64Carry, false);
-    } else {
-        if (vec) WRS_LAUNCH_PART(kFP32, true);
-        else WRS_LAUNCH_PART(kFP32, false);
+    switch (prec) {
+        case kFP32:
+            launch_part_p<kFP32>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
+            break;
+        case kFP64Carry:
+            launch_part_p<kFP64Carry>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
+            break;
+        case kComp:
+            launch_part_p<kComp>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
+            break;
+        default:
+            launch_part_p<kFP64>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
+            break;
     }
-#undef WRS_LAUNCH_PART
     return hipGetLastError();
 }
 
 hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
                                 const uint32_t* item_tile, const uint32_t* seg_item0,
                                 uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
-                                uint64_t* states, uint32_t* idx, float* pw, float* hi,
-                                float* lo, double* samp_h, double* samp_l,
-                                const uint64_t* seg_samp0, hipStream_t s) {
+                                uint64_t* states, uint32_t* idx, float* pw, double* samp_h,
+                                double* samp_l, const uint64_t* seg_samp0, hipStream_t s) {
 #define WRS_LAUNCH_SEG(P)                                                                 \
     k_seg_partition<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_item0, \
-                                                seg_prm, ticket, states, idx, pw, hi, lo,   \
-                                                samp_h, samp_l, seg_samp0)
-    if (prec == kFP64) WRS_LAUNCH_SEG(kFP64);
-    else if (prec == kFP64Carry) WRS_LAUNCH_SEG(kFP64Carry);
-    else WRS_LAUNCH_SEG(kFP32);
+                                                seg_prm, ticket, states, idx, pw, samp_h,   \
+                                                samp_l, seg_samp0)
+    switch (prec) {
+        case kFP32: WRS_LAUNCH_SEG(kFP32); break;
+        case kFP64Carry: WRS_LAUNCH_SEG(kFP64Carry); break;
+        case kComp: WRS_LAUNCH_SEG(kComp); break;
+        default: WRS_LAUNCH_SEG(kFP64); break;
+    }
 #undef WRS_LAUNCH_SEG
     return hipGetLastError();
 }

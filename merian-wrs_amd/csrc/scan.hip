@@ -13,11 +13,10 @@
 // (exact, so its bits do not depend on look-back timing) in the FP64 modes and
 // an FP32 sum in FP32 mode.  The output is staged back through LDS and written
 // as coalesced float4 rows.
-#include <type_traits>
-
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "lookback.hpp"
+#include "tilesum.hpp"
 
 namespace wrsk {
 using namespace wrsdev;
@@ -73,8 +72,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
                                                  float* __restrict__ lo_out) {
     __shared__ float s_buf[kPad];
     __shared__ uint32_t s_tile;
-    // in-tile accumulator: FP64 in the default mode, FP32 otherwise
-    using R = typename std::conditional<PREC == kFP64, double, float>::type;
+    // in-tile accumulator in the mode's tile arithmetic (tilesum.hpp)
+    using A1 = Acc1<PREC>;
+    using R = typename A1::T;
     __shared__ R s_wave[kBlock / kWave];
     __shared__ double s_carry;
     __shared__ float s_carry_f;
@@ -118,32 +118,32 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
     for (int k = 0; k < kItems; ++k) v[k] = s_buf[17 * t + k];
     const uint32_t e0 = (uint32_t)(kItems * t);
 
-    R mine = 0;
+    R mine = A1::zero();
 #pragma unroll
     for (int k = 0; k < kItems; ++k)
-        if (e0 + k < m) mine += (R)v[k];
+        if (e0 + k < m) mine = A1::add1(mine, v[k]);
     R incl = mine;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
-        const R o = shfl_up(incl, d);
-        if (lane >= d) incl = o + incl;
+        const R o = A1::shfl_up(incl, d);
+        if (lane >= d) incl = A1::add(o, incl);
     }
-    R lane_excl = shfl_up(incl, 1);
-    if (lane == 0) lane_excl = 0;
+    R lane_excl = A1::shfl_up(incl, 1);
+    if (lane == 0) lane_excl = A1::zero();
     if (lane == kWave - 1) s_wave[wid] = incl;
     __syncthreads();
-    R wave_excl = 0;
-    for (int q = 0; q < wid; ++q) wave_excl += s_wave[q];
+    R wave_excl = A1::zero();
+    for (int q = 0; q < wid; ++q) wave_excl = A1::add(wave_excl, s_wave[q]);
     R tile_agg = s_wave[0];
 #pragma unroll
-    for (int q = 1; q < kBlock / kWave; ++q) tile_agg += s_wave[q];
-    const R thr_excl = wave_excl + lane_excl;
+    for (int q = 1; q < kBlock / kWave; ++q) tile_agg = A1::add(tile_agg, s_wave[q]);
+    const R thr_excl = A1::add(wave_excl, lane_excl);
 
     uint64_t* rec = states + (uint64_t)tile * kStateWords;
     if (wid == 0) {
         FX agg;
-        if constexpr (PREC == kFP32) agg = FX{(uint64_t)__float_as_uint((float)tile_agg), 0};
-        else agg = fx_from_double((double)tile_agg, shift);
+        if constexpr (PREC == kFP32) agg = FX{(uint64_t)__float_as_uint(tile_agg), 0};
+        else agg = fx_from_double(A1::value(tile_agg), shift);
         FX excl = fx_zero();
         if (tile == 0) {
             if (lane == 0) publish_inclusive<FX, ScanOps<PREC>>(rec, agg);
@@ -167,9 +167,9 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             float hi, lo = 0.0f;
-            run += (R)v[k];
-            if constexpr (PREC == kFP64) {
-                const double P = (c + run) * scale;
+            run = A1::add1(run, v[k]);
+            if constexpr (prec_has_lo(PREC)) {
+                const double P = (c + A1::value(run)) * scale;
                 hi = (float)P;
                 lo = (float)(P - (double)hi);
             } else if constexpr (PREC == kFP64Carry) {
@@ -188,13 +188,12 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
         for (int j = 0; j < 4; ++j) {
             const int e = 4 * (t + kBlock * j);
             const int q = e + e / 16;
-            st_nt4(o4 + t + kBlock * j,
-                   make_float4(s_buf[q], s_buf[q + 1], s_buf[q + 2], s_buf[q + 3]));
+            o4[t + kBlock * j] = make_float4(s_buf[q], s_buf[q + 1], s_buf[q + 2], s_buf[q + 3]);
         }
     } else {
         for (uint32_t e = t; e < m; e += kBlock) hi_out[base + e] = s_buf[e + e / 16];
     }
-    if (PREC == kFP64 && lo_out != nullptr) {
+    if (prec_has_lo(PREC) && lo_out != nullptr) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < kItems; ++k) s_buf[17 * t + k] = lo_keep[k];
@@ -205,8 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
             for (int j = 0; j < 4; ++j) {
                 const int e = 4 * (t + kBlock * j);
                 const int q = e + e / 16;
-                st_nt4(o4 + t + kBlock * j,
-                       make_float4(s_buf[q], s_buf[q + 1], s_buf[q + 2], s_buf[q + 3]));
+                o4[t + kBlock * j] = make_float4(s_buf[q], s_buf[q + 1], s_buf[q + 2], s_buf[q + 3]);
             }
         } else {
             for (uint32_t e = t; e < m; e += kBlock) lo_out[base + e] = s_buf[e + e / 16];
@@ -225,6 +223,9 @@ hipError_t launch_scan(const float* w, uint64_t n, int prec, int normalise, Para
     if (prec == kFP64) {
         if (vec) WRS_LAUNCH_SCAN(kFP64, true);
         else WRS_LAUNCH_SCAN(kFP64, false);
+    } else if (prec == kComp) {
+        if (vec) WRS_LAUNCH_SCAN(kComp, true);
+        else WRS_LAUNCH_SCAN(kComp, false);
     } else if (prec == kFP64Carry) {
         if (vec) WRS_LAUNCH_SCAN(kFP64Carry, true);
         else WRS_LAUNCH_SCAN(kFP64Carry, false);

@@ -35,7 +35,7 @@ __device__ __forceinline__ typename Real<PREC>::T pre_at(const float* __restrict
                                                          const float* __restrict__ lo,
                                                          uint32_t q) {
     if constexpr (PREC == kFP32) return hi[q];
-    else if constexpr (PREC == kFP64) return (double)hi[q] + (double)lo[q];
+    else if constexpr (prec_has_lo(PREC)) return (double)hi[q] + (double)lo[q];
     else return (double)hi[q];
 }
 
@@ -118,8 +118,9 @@ hipError_t launch_splits(uint64_t n, uint32_t K, int prec, const Params* prm, co
                          const float* lo, SplitRec* splits, hipStream_t s) {
     const uint32_t blocks = (K + kSplitBlock - 1) / kSplitBlock;
     const uint32_t N = (uint32_t)n;
-    if (prec == kFP64)
-        k_splits<kFP64><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, hi, lo, splits);
+    // modes with a hi + lo prefix read both words (kFP64 shares kComp's code)
+    if (prec_has_lo(prec))
+        k_splits<kComp><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, hi, lo, splits);
     else if (prec == kFP64Carry)
         k_splits<kFP64Carry><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, hi, lo, splits);
     else

@@ -1,0 +1,52 @@
+// tilesum.hpp -- the running-sum type of the in-tile scans, by build arithmetic.
+//
+//   kFP32, kFP64Carry  FP32 sums (the reference kernels' arithmetic)
+//   kComp              compensated FP32: a double-float hi + lo kept exact by
+//                      two_sum at every add (error ~2^-48 relative instead of
+//                      2^-24 times the term count)
+//   kFP64              FP64 sums: a float weight converts exactly, and on
+//                      gfx950 with packed FP32 math disabled an FP64 add
+//                      issues at the rate of an FP32 add
+//
+// All operations are in a fixed order, so a tile's sums are a function of its
+// weights only.
+#pragma once
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace wrsk {
+
+template <int PREC>
+struct Acc1 {
+    using T = float;
+    static __device__ __forceinline__ T zero() { return 0.0f; }
+    static __device__ __forceinline__ T add1(T a, float x) { return a + x; }
+    static __device__ __forceinline__ T add(T a, T b) { return a + b; }
+    static __device__ __forceinline__ double value(T a) { return (double)a; }
+    static __device__ __forceinline__ T shfl_up(T a, int d) { return wrsdev::shfl_up(a, d); }
+};
+
+template <>
+struct Acc1<kComp> {
+    using T = wrsdev::DF;
+    static __device__ __forceinline__ T zero() { return T{0.0f, 0.0f}; }
+    static __device__ __forceinline__ T add1(T a, float x) { return wrsdev::df_add1(a, x); }
+    static __device__ __forceinline__ T add(T a, T b) { return wrsdev::df_add(a, b); }
+    static __device__ __forceinline__ double value(T a) { return wrsdev::df_value(a); }
+    static __device__ __forceinline__ T shfl_up(T a, int d) {
+        return T{wrsdev::shfl_up(a.hi, d), wrsdev::shfl_up(a.lo, d)};
+    }
+};
+
+template <>
+struct Acc1<kFP64> {
+    using T = double;
+    static __device__ __forceinline__ T zero() { return 0.0; }
+    static __device__ __forceinline__ T add1(T a, float x) { return a + (double)x; }
+    static __device__ __forceinline__ T add(T a, T b) { return a + b; }
+    static __device__ __forceinline__ double value(T a) { return a; }
+    static __device__ __forceinline__ T shfl_up(T a, int d) { return wrsdev::shfl_up(a, d); }
+};
+
+}  // namespace wrsk

@@ -99,7 +99,7 @@ void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace
     require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
     check_hip(wrsk::launch_partition(b.elements, N, (int)m_prec, ws.params, ws.states,
                                      b.partitionIndices, nullptr, b.partitionPrefix,
-                                     m_prec == WRS_PREC_FP64 ? b.partitionPrefixLo : nullptr,
+                                     wrsk::prec_has_lo(m_prec) ? b.partitionPrefixLo : nullptr,
                                      nullptr, nullptr, s),
               "partition launch");
 }
@@ -110,7 +110,7 @@ void PrefixSum::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) 
     Mean(m_prec).run(s, b.elements, N, ws);
     check_hip(wrsk::launch_scan(b.elements, N, (int)m_prec, m_normalise ? 1 : 0, ws.params,
                                 ws.states, b.prefix,
-                                m_prec == WRS_PREC_FP64 ? b.prefixLo : nullptr, s),
+                                wrsk::prec_has_lo(m_prec) ? b.prefixLo : nullptr, s),
               "scan launch");
 }
 
@@ -125,7 +125,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     // itself needs the indices and the sparse prefix samples
     const bool full_prefix = dbg && (dbg->prefix_hi || dbg->prefix_lo || dbg->splits);
     float* hi = full_prefix ? ws.prefix_hi : nullptr;
-    float* lo = full_prefix && prec == WRS_PREC_FP64 ? ws.prefix_lo : nullptr;
+    float* lo = full_prefix && wrsk::prec_has_lo(prec) ? ws.prefix_lo : nullptr;
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::state_words_for(N);
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
@@ -150,7 +150,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     if (dbg) {
         if (dbg->splits)
             check_hip(wrsk::launch_splits(N, splitCount(N), (int)prec, ws.params, ws.prefix_hi,
-                                          prec == WRS_PREC_FP64 ? ws.prefix_lo : nullptr,
+                                          wrsk::prec_has_lo(prec) ? ws.prefix_lo : nullptr,
                                           reinterpret_cast<wrsk::SplitRec*>(dbg->splits), s),
                       "splits launch");
         if (dbg->mean)
@@ -263,6 +263,11 @@ wrs::PSAConfig config_of(const wrs_config* cfg) {
     return c;
 }
 
+void require_precision(wrs_precision p) {
+    if ((int)p < (int)WRS_PREC_FP32 || (int)p > (int)WRS_PREC_FP64)
+        throw wrs::Error(WRS_EINVAL, "unknown precision");
+}
+
 void require_plan(const wrs_plan* p) {
     if (!p) throw wrs::Error(WRS_EINVAL, "plan is null");
 }
@@ -280,15 +285,14 @@ wrs_status wrs_plan_create(const wrs_config* cfg, uint64_t max_n, wrs_plan** out
         if (max_n == 0 || max_n >= (1ull << 32))
             throw wrs::Error(WRS_EINVAL, "max_n must be in [1, 2^32)");
         wrs::PSAConfig c = config_of(cfg);
-        if ((int)c.precision < 0 || (int)c.precision > 2)
-            throw wrs::Error(WRS_EINVAL, "unknown precision");
+        require_precision(c.precision);
         int dev_count = 0;
         if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
             throw wrs::Error(WRS_ENOTSUP, "no HIP device");
         wrs_plan* p = new wrs_plan;
         p->cfg = c;
         try {
-            p->ws = wrs::Workspace::allocate(max_n, true, c.precision == WRS_PREC_FP64);
+            p->ws = wrs::Workspace::allocate(max_n, true, wrsk::prec_has_lo(c.precision));
             for (auto& e : p->ev) wrs::check_hip(hipEventCreate(&e), "hipEventCreate");
         } catch (...) {
             p->ws.release();
@@ -426,9 +430,10 @@ wrs_status wrs_batch_plan_create(const wrs_config* cfg, const uint64_t* offsets_
         if (!out || !offsets_host || nseg == 0)
             throw wrs::Error(WRS_EINVAL, "bad batch plan arguments");
         *out = nullptr;
+        const wrs::PSAConfig c = config_of(cfg);
+        require_precision(c.precision);
         wrs_batch_plan* p = new wrs_batch_plan;
         try {
-            wrs::PSAConfig c = config_of(cfg);
             p->impl.create(c.splitSize, (int)c.precision, offsets_host, nseg);
         } catch (...) {
             p->impl.destroy();

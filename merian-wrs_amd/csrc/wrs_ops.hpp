@@ -47,7 +47,7 @@ struct Workspace {
     uint32_t* indices = nullptr;       // [max_n] (PSA only)
     float* pweights = nullptr;         // [max_n] weights in partition order (PSA only)
     float* prefix_hi = nullptr;        // [max_n] (PSA debug builds only)
-    float* prefix_lo = nullptr;        // [max_n] (PSA debug builds only, WRS_PREC_FP64)
+    float* prefix_lo = nullptr;        // [max_n] (PSA debug builds only, modes with hi + lo)
     double* samp_h = nullptr;          // [max_n/kSample + 1] heavy prefix samples (PSA only)
     double* samp_l = nullptr;          // [max_n/kSample + 1] light prefix samples (PSA only)
     wrsk::Coarse* bounds = nullptr;    // [max_n/kMergeTile + 2] block-boundary splits (PSA only)
@@ -102,7 +102,7 @@ class PrefixSum {
 
 struct PSAConfig {
     uint32_t splitSize = 32;
-    wrs_precision precision = WRS_PREC_FP64;
+    wrs_precision precision = WRS_PREC_COMPENSATED;
 };
 
 class PSA {

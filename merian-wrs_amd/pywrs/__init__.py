@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libwrs_amd.so")
 
 WRS_OK, WRS_EINVAL, WRS_ENOMEM, WRS_EHIP, WRS_ENOTSUP = range(5)
-PREC_FP32, PREC_FP64_CARRY, PREC_FP64 = 0, 1, 2
+PREC_FP32, PREC_FP64_CARRY, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2, 3
 GEN_UNIFORM01, GEN_LOGNORMAL, GEN_ZIPF, GEN_SPIKE = 0, 1, 2, 3
 
 ALIAS_DTYPE = np.dtype([("p", np.float32), ("a", np.uint32)])
@@ -261,7 +261,7 @@ def _p(x):
 class Plan:
     """wrs_plan: build workspace for distributions of up to max_n weights."""
 
-    def __init__(self, max_n: int, split_size: int = 32, precision: int = PREC_FP64):
+    def __init__(self, max_n: int, split_size: int = 32, precision: int = PREC_COMPENSATED):
         self.max_n = int(max_n)
         self.split_size = int(split_size)
         self.precision = int(precision)
@@ -323,7 +323,7 @@ class BatchPlan:
     """wrs_batch_plan: segmented build of many distributions (CSR offsets)."""
 
     def __init__(self, offsets: np.ndarray, split_size: int = 32,
-                 precision: int = PREC_FP64):
+                 precision: int = PREC_COMPENSATED):
         self.offsets = np.ascontiguousarray(offsets, np.uint64)
         self.nseg = self.offsets.size - 1
         cfg = Config(int(split_size), int(precision))
@@ -419,7 +419,7 @@ class Graph:
 
 # -------------------------------------------------------------- conveniences --
 def psa_alias_table(weights: np.ndarray, split_size: int = 32,
-                    precision: int = PREC_FP64) -> np.ndarray:
+                    precision: int = PREC_COMPENSATED) -> np.ndarray:
     """Host convenience: upload, build, download (structured ALIAS_DTYPE array)."""
     w = np.ascontiguousarray(weights, np.float32)
     n = w.size

@@ -17,7 +17,7 @@ def make_batch(wrs, oracle, nseg, hi, seed=3):
     return sizes, off, w
 
 
-@pytest.mark.parametrize("precision", [2, 1, 0])
+@pytest.mark.parametrize("precision", [2, 3, 1, 0])
 def test_batched_equals_per_segment(wrs, oracle, precision):
     """FP64 modes: tables and samples bit-identical to per-segment builds (the
     carry is exact, so its bits do not depend on look-back timing).  FP32 mode
