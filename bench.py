@@ -1,21 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark of the weighted-random-sampling hot path on one MI355X.
 
-One step = one PSA alias-table build (FP64 total -> heavy/light prefix-partition
--> block-boundary splits -> merge pack) of N = 2^26 Zipf(1.1) FP32 weights (ranks a seeded
-permutation of 1..N, seed 2), FP64 tile sums with the exact fixed-point carry,
-split size 32, inputs resident in HBM.  The value is the build throughput in
-GB/s of algorithmic bytes B_build = 32N + 12K (K = N/32): mean read 4N,
-partition read 4N + index 4N + prefix 4N, split 12K, pack index 4N + weight 4N
-+ table 8N.
+One step = one PSA alias-table build of N = 2^26 Zipf(1.1) FP32 weights (ranks
+a seeded Feistel permutation of 1..N, seed 2) with compensated FP32 tile sums,
+the exact fixed-point carry and FP64 mean/split arithmetic, split size 32,
+inputs resident in HBM: FP64 total -> heavy/light prefix-partition (tile
+aggregates, tile scan, emit) -> block-boundary splits -> merge pack.  `value`
+is the build throughput in GB/s of algorithmic bytes B_build = 32N + 12K
+(K = N/32): mean read 4N, partition read 4N + index 4N + prefix 4N (this build
+writes the partition-ordered weights in its place), split 12K, pack read index
+4N + weight 4N + write table 8N.
 
-Also reported on the same line: the fraction of the 8 TB/s HBM peak, the
-dominant kernel's roofline (algorithmic bytes per launch / its average
-duration, from HIP events on the stream the kernels run on), a copy-kernel
-calibration, alias-sampling samples/s (S = 2^26, 12 B/sample), the implied
-distribution's max relative error vs FP64, the batched 4096-distribution build
-+ sample against per-distribution launches, and the reference CPU path (the
-oracle's restatement, single-threaded) on the same weights.
+Also reported on the same line:
+  roofline      the dominant kernel's algorithmic bytes per launch / its average
+                duration, from HIP events around each kernel on the stream the
+                kernels run on (a second pass of K builds); `traffic` is the HBM
+                bytes per launch of that kernel from the rocprofv3 FETCH_SIZE /
+                WRITE_SIZE summary committed under profiles/ (null if absent);
+  precision     the same build in all four arithmetic modes: time and the
+                implied distribution's max abs/rel error vs FP64;
+  sampling      alias sampling, S = 2^26 Philox samples;
+  its           ITS on N = 2^24 LogNormal(0, 2): normalised scan (8N bytes)
+                and S = 2^24 binary-search samples, CDF error vs the FP64 prefix;
+  spike         N = 2^28 spike mixture build and S = 2^30 alias samples;
+  batched       4096 log-uniform-size LogNormal(0, 3) distributions built and
+                sampled (64 each) by one hipGraph vs one launch sequence per
+                distribution;
+  cpu_baseline  the reference CPU path (the oracle's single-threaded C
+                restatement) on the same N = 2^26 weights.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -23,6 +35,7 @@ import argparse
 import json
 import os
 import platform
+import re
 import sys
 import time
 
@@ -32,25 +45,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "merian-wrs_amd"))
 
 HBM_PEAK_GBS = 8000.0
-PREC_NAMES = ['fp32', 'fp64-carry', 'compensated-fp32', 'fp64-tiles']
+PREC_NAMES = ["fp32", "fp64-carry", "compensated-fp32", "fp64-tiles"]
 METRIC = "PSA build GB/s (% HBM peak) + samples/s; batched vs per-launch at fixed error"
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "psa_build_zipf_n2e26_hbm_traffic.md")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def implied_error(table: np.ndarray, w: np.ndarray) -> dict:
-    n = w.size
-    p = np.clip(table["p"].astype(np.float64), 0.0, 1.0)
-    q = np.bincount(np.arange(n), weights=p, minlength=n)
-    q += np.bincount(table["a"].astype(np.int64), weights=1.0 - p, minlength=n)
-    q /= n
-    w64 = w.astype(np.float64)
-    t = w64 / np.sum(np.sort(w64))  # sorted pairwise sum: relative error ~1e-16
-    d = np.abs(q - t)
-    nz = t > 0
-    return dict(max_abs=float(d.max()), max_rel=float((d[nz] / t[nz]).max()))
+class Implied:
+    """Per-element error of an alias table's implied distribution vs FP64."""
+
+    def __init__(self, w: np.ndarray):
+        w64 = w.astype(np.float64)
+        self.t = w64 / np.sum(np.sort(w64))  # sorted summation: relative error ~1e-16
+        self.nz = self.t > 0
+
+    def error(self, table: np.ndarray) -> dict:
+        n = self.t.size
+        p = np.clip(table["p"].astype(np.float64), 0.0, 1.0)
+        q = p.copy()
+        q += np.bincount(table["a"].astype(np.int64), weights=1.0 - p, minlength=n)
+        q /= n
+        d = np.abs(q - self.t)
+        return dict(max_abs=float(d.max()), max_rel=float((d[self.nz] / self.t[self.nz]).max()))
 
 
 def cpu_model() -> str:
@@ -63,6 +82,30 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary."""
+    try:
+        text = open(PMC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
+    if not m:
+        return None
+    row = json.loads(m.group(1)).get("kernels", {}).get(kernel)
+    return None if row is None else int(row["traffic_bytes"])
+
+
+def time_it(fn, st, wrs, reps: int) -> float:
+    """ms per call of fn() enqueued `reps` times back to back on stream st."""
+    e0, e1 = wrs.Event(), wrs.Event()
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    st.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,13 +113,14 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=26)
     ap.add_argument("--split-size", type=int, default=32)
-    ap.add_argument("--precision", type=int, default=2)
+    ap.add_argument("--precision", type=int, default=2, choices=range(4))
     ap.add_argument("--log2s", type=int, default=26)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline build (no precision study, ITS, spike)")
     args = ap.parse_args()
-
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path has no multi-GPU form here); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -99,75 +143,127 @@ def main() -> int:
     st.synchronize()
     B_build = 32 * n + 12 * K
 
-    # ---- warmup + timed region --------------------------------------------
+    # ---- warmup + timed region: K builds back to back ----------------------
     for _ in range(args.warmup):
         plan.psa_build(w, n, table, stream=st)
     st.synchronize()
-    e0, e1 = wrs.Event(), wrs.Event()
-    e0.record(st)
-    for _ in range(args.steps):
-        plan.psa_build(w, n, table, stream=st)
-    e1.record(st)
-    st.synchronize()
-    ms_total = e0.elapsed_ms(e1)
-    ms_step = ms_total / args.steps
+    ms_step = time_it(lambda: plan.psa_build(w, n, table, stream=st), st, wrs, args.steps)
     gbs = B_build / (ms_step * 1e-3) / 1e9
-    log(f"PSA build N=2^{args.log2n}: {ms_step:.4f} ms/step, {gbs:.1f} GB/s "
-        f"({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
+    log(f"PSA build N=2^{args.log2n} ({PREC_NAMES[args.precision]}): {ms_step:.4f} ms/step, "
+        f"{gbs:.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
 
-    # ---- per-kernel durations (events on the same stream) ------------------
+    # ---- per-kernel durations: K more builds, HIP events around each kernel
     plan.set_timing(True)
-    acc = np.zeros(5)
+    acc = np.zeros(len(wrs.Plan.PHASES))
     for _ in range(args.steps):
         plan.psa_build(w, n, table, stream=st)
         acc += np.array(plan.phase_times())
     plan.set_timing(False)
-    phase = acc / args.steps
-    names = ["reduce", "finalize", "partition", "coarse_split", "merge_pack"]
-    # bytes each kernel must move: reduce reads w; the partition reads w and
-    # writes the indices (its FP64 prefix samples are 16N/64 more); the merge
-    # pack reads the indices, gathers the weights and writes the 8-byte table
-    alg = {"reduce": 4 * n, "finalize": 0, "partition": 8 * n + 16 * (n // 64),
-           "coarse_split": 0, "merge_pack": 16 * n}
-    log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in zip(names, phase)))
-    dom = max(("reduce", "partition", "merge_pack"), key=lambda k: phase[names.index(k)])
-    dom_ms = float(phase[names.index(dom)])
+    phase = dict(zip(wrs.Plan.PHASES, (acc / args.steps).tolist()))
+    # bytes each kernel must move (the merge pack reads the index and the
+    # partition-ordered weight of every rank and writes the 8-byte table; the
+    # emit reads w and writes index + weight; aggregate and reduce read w)
+    alg = {"reduce": 4 * n, "finalize": 0, "tile_aggregate": 4 * n, "tile_scan": 0,
+           "partition_emit": 12 * n, "boundary_splits": 0, "merge_pack": 16 * n}
+    kname = {"reduce": "k_reduce", "tile_aggregate": "k_part_agg",
+             "partition_emit": "k_part_emit", "merge_pack": "k_mergepack"}
+    log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phase.items()))
+    dom = max(kname, key=lambda k: phase[k])
+    dom_ms = phase[dom]
     dom_gbs = alg[dom] / (dom_ms * 1e-3) / 1e9
+    traffic = pmc_traffic(kname[dom])
 
     # ---- copy calibration ---------------------------------------------------
     cdst = wrs.empty(n, np.float32)
     wrs.copy_f32(cdst, w, n, stream=st)
-    c0, c1 = wrs.Event(), wrs.Event()
-    c0.record(st)
-    for _ in range(10):
-        wrs.copy_f32(cdst, w, n, stream=st)
-    c1.record(st)
-    st.synchronize()
-    copy_gbs = 2 * 4 * n / (c0.elapsed_ms(c1) / 10 * 1e-3) / 1e9
+    copy_ms = time_it(lambda: wrs.copy_f32(cdst, w, n, stream=st), st, wrs, 10)
+    copy_gbs = 2 * 4 * n / (copy_ms * 1e-3) / 1e9
     del cdst
     log(f"copy kernel: {copy_gbs:.1f} GB/s")
 
     # ---- alias sampling -----------------------------------------------------
     smp = wrs.empty(S, np.uint32)
     wrs.alias_sample(table, n, S, 12345, smp, stream=st)
-    s0, s1 = wrs.Event(), wrs.Event()
-    s0.record(st)
-    reps = 5
-    for r in range(reps):
-        wrs.alias_sample(table, n, S, 12345, smp, counter_base=r * S, stream=st)
-    s1.record(st)
-    st.synchronize()
-    sample_ms = s0.elapsed_ms(s1) / reps
+    ctr = [0]
+
+    def sample_once():
+        ctr[0] += 1
+        wrs.alias_sample(table, n, S, 12345, smp, counter_base=ctr[0] * S, stream=st)
+
+    sample_ms = time_it(sample_once, st, wrs, 5)
     samples_per_s = S / (sample_ms * 1e-3)
     log(f"alias sampling S=2^{args.log2s}: {sample_ms:.4f} ms, {samples_per_s / 1e9:.2f} Gsamples/s")
     del smp
 
-    # ---- accuracy of the benchmarked table ----------------------------------
+    # ---- accuracy, and the precision study on the same weights -------------
     w_host = w.numpy()
+    implied = None if args.no_error else Implied(w_host)
     err = None
-    if not args.no_error:
-        err = implied_error(table.numpy(), w_host)
-        log(f"implied distribution vs FP64: max_rel {err['max_rel']:.3e}")
+    if implied is not None:
+        err = implied.error(table.numpy())
+        log(f"implied distribution vs FP64: max_abs {err['max_abs']:.3e} max_rel {err['max_rel']:.3e}")
+    study = None
+    if not args.no_extras:
+        study = []
+        for prec in range(4):
+            pp = plan if prec == args.precision else wrs.Plan(n, args.split_size, prec)
+            tt = wrs.empty(n, wrs.ALIAS_DTYPE)
+            for _ in range(2):
+                pp.psa_build(w, n, tt, stream=st)
+            ms = time_it(lambda: pp.psa_build(w, n, tt, stream=st), st, wrs, 5)
+            row = dict(mode=PREC_NAMES[prec], ms=round(ms, 4),
+                       gbs=round(B_build / (ms * 1e-3) / 1e9, 1))
+            if implied is not None:
+                e = implied.error(tt.numpy())
+                row.update(max_abs=e["max_abs"], max_rel=e["max_rel"])
+            study.append(row)
+            log(f"precision {row}")
+            del tt
+
+    # ---- ITS: normalised scan + binary-search sampling ----------------------
+    its = None
+    if not args.no_extras:
+        ni = 1 << 24
+        wi = wrs.empty(ni, np.float32)
+        wrs.generate(wrs.GEN_LOGNORMAL, ni, 1, wi, a=0.0, b=2.0, stream=st)
+        iplan = wrs.Plan(ni, args.split_size, args.precision)
+        hi = wrs.empty(ni, np.float32)
+        lo = wrs.empty(ni, np.float32)
+        iplan.its_build(wi, ni, hi, lo, stream=st)
+        scan_ms = time_it(lambda: iplan.its_build(wi, ni, hi, lo, stream=st), st, wrs, 10)
+        outi = wrs.empty(ni, np.uint32)
+        wrs.its_sample(hi, lo, ni, ni, 7, outi, stream=st)
+        its_ms = time_it(lambda: wrs.its_sample(hi, lo, ni, ni, 7, outi, stream=st), st, wrs, 5)
+        st.synchronize()
+        wi64 = wi.numpy().astype(np.float64)
+        exact = np.cumsum(wi64) / np.sum(np.sort(wi64))
+        got = hi.numpy().astype(np.float64) + lo.numpy().astype(np.float64)
+        its = dict(n=ni, scan_ms=round(scan_ms, 4), scan_gbs_8N=round(8 * ni / (scan_ms * 1e-3) / 1e9, 1),
+                   sample_ms=round(its_ms, 4), samples_per_s=round(ni / (its_ms * 1e-3), 1),
+                   cdf_max_abs_err=float(np.abs(got - exact).max()))
+        log(f"its: {its}")
+        del wi, hi, lo, outi, iplan
+
+    # ---- spike mixture at N = 2^28, S = 2^30 --------------------------------
+    spike = None
+    if not args.no_extras:
+        ns, ss = 1 << 28, 1 << 30
+        wsp = wrs.empty(ns, np.float32)
+        wrs.generate(wrs.GEN_SPIKE, ns, 4, wsp, stream=st)
+        splan = wrs.Plan(ns, args.split_size, args.precision)
+        tsp = wrs.empty(ns, wrs.ALIAS_DTYPE)
+        splan.psa_build(wsp, ns, tsp, stream=st)
+        sp_ms = time_it(lambda: splan.psa_build(wsp, ns, tsp, stream=st), st, wrs, 5)
+        osp = wrs.empty(ss, np.uint32)
+        wrs.alias_sample(tsp, ns, ss, 99, osp, stream=st)
+        sps_ms = time_it(lambda: wrs.alias_sample(tsp, ns, ss, 99, osp, stream=st), st, wrs, 3)
+        b_sp = 32 * ns + 12 * splan.split_count(ns)
+        spike = dict(n=ns, build_ms=round(sp_ms, 4), build_gbs=round(b_sp / (sp_ms * 1e-3) / 1e9, 1),
+                     frac_hbm_peak=round(b_sp / (sp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     samples=ss, sample_ms=round(sps_ms, 3),
+                     samples_per_s=round(ss / (sps_ms * 1e-3), 1))
+        log(f"spike: {spike}")
+        del wsp, tsp, osp, splan
 
     # ---- batched 4096-distribution build + sample --------------------------
     batched = None
@@ -188,13 +284,7 @@ def main() -> int:
             wrs.alias_sample_batched(tb, doff, nseg, per, 42, sb, stream=st)
         g.launch()
         st.synchronize()
-        b0, b1 = wrs.Event(), wrs.Event()
-        b0.record(st)
-        for _ in range(10):
-            g.launch()
-        b1.record(st)
-        st.synchronize()
-        batched_ms = b0.elapsed_ms(b1) / 10
+        batched_ms = time_it(g.launch, st, wrs, 10)
         eplan = wrs.Plan(int(sizes.max()), args.split_size, args.precision)
         te = wrs.empty(tot, wrs.ALIAS_DTYPE)
         se = wrs.empty(nseg * per, np.uint32)
@@ -258,17 +348,23 @@ def main() -> int:
         },
         "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
         "lower_bound_12N_gbs": round(12 * n / (ms_step * 1e-3) / 1e9, 2),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1),
+        "roofline": {"bound": "hbm", "kernel": kname[dom], "achieved": round(dom_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
-                     "algorithmic_bytes": alg[dom], "avg_ms": round(dom_ms, 5), "traffic": None},
-        "phases_ms": {k: round(float(v), 5) for k, v in zip(names, phase)},
+                     "algorithmic_bytes": alg[dom], "avg_ms": round(dom_ms, 5),
+                     "traffic": traffic},
+        "phases_ms": {k: round(float(v), 5) for k, v in phase.items()},
         "copy_kernel_gbs": round(copy_gbs, 1),
         "samples_per_s": round(samples_per_s, 1),
-        "sampling": {"S": S, "ms": round(sample_ms, 4), "gbs_12B": round(12 * S / (sample_ms * 1e-3) / 1e9, 1)},
+        "sampling": {"S": S, "ms": round(sample_ms, 4),
+                     "gbs_12B": round(12 * S / (sample_ms * 1e-3) / 1e9, 1)},
+        "max_abs_error_vs_fp64": None if err is None else err["max_abs"],
         "max_rel_error_vs_fp64": None if err is None else err["max_rel"],
+        "precision_study": study,
+        "its": its,
+        "spike": spike,
         "batched": batched,
         "cpu_baseline": cpu,
-        "device": info["name"],
+        "device": info["name"] or info["arch"],
     }
     print(json.dumps(line), flush=True)
     return 0

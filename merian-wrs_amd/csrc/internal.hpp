@@ -30,7 +30,8 @@ struct Params {
     double inv_total;      // 1 / total (ITS normalisation)
     double heavy_sum;      // sum of the heavy weights (written by the partition's last tile)
     double light_sum;      // sum of the light weights (idem)
-    uint64_t reserved[1];
+    uint32_t repair_ticket;  // dynamic block ticket of the boundary repair
+    uint32_t reserved;
 };
 static_assert(sizeof(Params) == 64, "Params layout");
 
@@ -41,10 +42,10 @@ struct AliasEntry {
 
 // Exact split at a block boundary: i lights and j heavies precede bucket n,
 // lp = LP(i), hp = HP(j) (prefix sums of the first i lights / j heavies).
-struct Coarse {
+struct Coarse {  // split (i lights, j heavies) at a merge-block boundary
     uint32_t i, j;
-    double lp, hp;
-    uint64_t pad;
+    double lp, hp;  // LP(i), HP(j)
+    uint64_t pad;   // scratch of the segmented boundary repair
 };
 static_assert(sizeof(Coarse) == 32, "Coarse layout");
 
@@ -71,10 +72,19 @@ inline uint64_t tiles_for(uint64_t n) { return (n + kTile - 1) / kTile; }
 // tile aggregates scanned per block by the partition's spine scan
 constexpr uint32_t kSpineTiles = 1024;
 inline uint64_t spine_blocks_for(uint64_t n) { return (tiles_for(n) + kSpineTiles - 1) / kSpineTiles; }
-// state buffer of a single-distribution build: one record per tile, then one
-// per spine block; all reset to the sentinel by the reduce pass
-inline uint64_t state_words_for(uint64_t n) {
+// boundaries per block of the boundary repair
+constexpr uint32_t kRepairSpan = 1024;
+inline uint64_t repair_blocks_for(uint64_t n) {
+    return (merge_blocks_for(n) + kRepairSpan - 1) / kRepairSpan;
+}
+// state buffer of a single-distribution build: one record per tile, one per
+// spine block, then two per repair block (one per max-scan pass); all reset
+// to the sentinel by the reduce pass
+inline uint64_t repair_records_offset(uint64_t n) {
     return (tiles_for(n) + spine_blocks_for(n)) * kStateWords;
+}
+inline uint64_t state_words_for(uint64_t n) {
+    return repair_records_offset(n) + 2 * repair_blocks_for(n) * kStateWords;
 }
 inline uint32_t partials_for(uint64_t n) {
     const uint64_t t = tiles_for(n);
@@ -86,15 +96,22 @@ hipError_t launch_reduce(const float* w, uint64_t n, int prec, double* partials,
                          uint64_t* states, uint64_t state_words, hipStream_t s);
 hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int prec,
                            int has_pivot, double pivot, Params* prm, hipStream_t s);
-// idx and pw (the weights in partition order) are always written; hi/lo (full
-// prefix) and the prefix samples only when non-null
+// idx is always written; pw (the weights in partition order), hi/lo (full
+// prefix) and the prefix samples only when non-null.  ev_agg / ev_scan, when
+// non-null, are recorded after the tile-aggregate and tile-scan kernels.
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
                             uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
-                            double* samp_l, hipStream_t s);
+                            double* samp_l, hipStream_t s, hipEvent_t ev_agg = nullptr,
+                            hipEvent_t ev_scan = nullptr);
 // exact splits at every kMergeTile buckets, then the LDS merge-path pack
 hipError_t launch_coarse(const float* pw, uint64_t n, int prec, const Params* prm,
                          const double* samp_h, const double* samp_l, Coarse* bounds,
                          hipStream_t s);
+// makes the boundary splits monotone (no-op for exact keys); states are the
+// build's look-back records (state_words_for(n) words, reset by the reduce)
+hipError_t launch_repair_bounds(Coarse* bounds, uint64_t n, Params* prm, uint64_t* states,
+                                const float* pw, const double* samp_h, const double* samp_l,
+                                hipStream_t s);
 hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params* prm,
                             const uint32_t* idx, const Coarse* bounds, AliasEntry* table,
                             hipStream_t s);

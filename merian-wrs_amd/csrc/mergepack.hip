@@ -26,25 +26,29 @@
 //             the 64 heavy and light weights around it.  Weights are read from
 //             pw, the weights in partition order written by the partition
 //             next to the indices, so every access here is contiguous.
+//  k_repair_bounds_lb  makes the splits monotone where rounding broke it.
 //  k_mergepack one 512-thread block per kMergeTile buckets: the block's light
 //             and heavy index ranges come from its two boundary splits; their
 //             indices and weights (both contiguous runs of idx and pw, loaded
 //             in one round trip) are staged in LDS, the keys X and Y
-//             are rebuilt by a block scan on top of LP(i0), HP(j0),
-//             every thread finds its merge-path diagonal by a binary search in
+//             relative to the block start are rebuilt by an exact integer
+//             block scan, every thread finds its merge-path diagonal by a
+//             binary search in
 //             LDS and merges 4 buckets, and the entries are written back in
 //             rank order (coalesced: consecutive light ranks are consecutive
 //             ascending indices).  The last heavy of all gets (1, itself), as
 //             in pack.hpp:78-81.
 //
-// In the FP64-carry modes keys, residuals and split arithmetic are FP64 (the
-// block scan itself runs in the mode's tile arithmetic: FP32, compensated FP32
-// or FP64); in FP32 mode everything is FP32, as in the reference kernels.
+// The split search runs in FP64 in the FP64-carry modes and in FP32 in FP32
+// mode, as the reference kernels do; the merge keys inside a block are exact
+// 64-bit fixed-point sums in every mode (so a block's merge is always a
+// permutation of its slots), and a boundary repair pass keeps the block ranges
+// valid when rounding has made the splits non-monotone.
 #include <type_traits>
 
 #include "device_common.hpp"
 #include "internal.hpp"
-#include "tilesum.hpp"
+#include "lookback.hpp"
 
 namespace wrsk {
 using namespace wrsdev;
@@ -209,24 +213,34 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     if (lane_id() == 0) out[p] = c;
 }
 
+// Quantisation of the merge keys: x -> rint(x * 2^qs) with qs = 61 - ilogb(total),
+// so that every local key and offset stays below 2^62 in magnitude.
+__device__ __forceinline__ int64_t quant(double x, int qs) { return __double2ll_rn(ldexp(x, qs)); }
+
 // One block of kMergeTile buckets [n0, n1) between boundary splits c0 and c1.
+//
+// Local keys are exact 64-bit fixed-point running sums of non-negative
+// increments (w_q = quant(w), mean_q = quant(mean)):
+//     a_r = sum_{r' < r}  (mean_q - w_q(light i0 + r'))      r in [0, nl]
+//     b_q = sum_{q' <= q} (w_q(heavy j0 + q') - mean_q)       q in [0, nh)
+// so A and B are sorted whatever the rounding of the prefix sums, and the
+// merge below is always a permutation of the block's slots.  With the split
+// offset D = X(i0) - Y(j0) = n0*mean - LP(i0) - HP(j0) (FP64), heavy q goes
+// before light r iff Y(j0+q+1) <= X(i0+r), i.e. b_q - quant(D) <= a_r, and its
+// residual is Y(j0+q+1) - X(i0+r) + mean = (b_q - a_r) 2^-qs - D + mean.
 template <int PREC>
 __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
                                                 const uint32_t* __restrict__ idx, uint32_t N,
-                                                uint32_t H, typename RealT<PREC>::T mean,
+                                                uint32_t H, double mean, double total,
                                                 uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
                                                 AliasEntry* __restrict__ table) {
-    using T = typename RealT<PREC>::T;
     constexpr int kPer = kMergeTile / kMergeBlock;  // buckets (and staged elements) per thread
     // keys during the merge, entries afterwards (8 bytes per slot either way)
     __shared__ uint2 s_raw[kMergeTile + 2];
     __shared__ uint32_t s_idx[kMergeTile + 2];
     __shared__ float s_w[kMergeTile];
-    // block-scan accumulator in the mode's tile arithmetic (tilesum.hpp)
-    using A1 = Acc1<PREC>;
-    using S = typename A1::T;
-    __shared__ S s_wave_l[kMergeBlock / kWave], s_wave_h[kMergeBlock / kWave];
-    T* s_key = reinterpret_cast<T*>(s_raw);
+    __shared__ int64_t s_wave_l[kMergeBlock / kWave], s_wave_h[kMergeBlock / kWave];
+    int64_t* s_key = reinterpret_cast<int64_t*>(s_raw);
 
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
@@ -264,75 +278,79 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     }
     __syncthreads();
 
-    // ---- keys: light r -> X(i0 + r) (exclusive LP), heavy q -> Y(j0 + q + 1)
-    S ml = A1::zero(), mh = A1::zero();
+    // ---- keys -----------------------------------------------------------
+    const int qs = 61 - ilogb(total);
+    const int64_t mean_q = quant(mean, qs);
     const uint32_t e0 = (uint32_t)(kPer * t);
+    int64_t inc[kPer];
+    int64_t ml = 0, mh = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t e = e0 + k;
+        inc[k] = 0;
         if (e < ne) {
-            const float x = s_w[e];
-            if (e < nl) ml = A1::add1(ml, x);
-            else mh = A1::add1(mh, x);
+            const int64_t wq = quant((double)s_w[e], qs);
+            inc[k] = e < nl ? mean_q - wq : wq - mean_q;
+            if (e < nl) ml += inc[k];
+            else mh += inc[k];
         }
     }
-    S il = ml, ih = mh;
+    int64_t il = ml, ih = mh;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
-        const S ol = A1::shfl_up(il, d);
-        const S oh = A1::shfl_up(ih, d);
+        const int64_t ol = shfl_up(il, d);
+        const int64_t oh = shfl_up(ih, d);
         if (lane >= d) {
-            il = A1::add(ol, il);
-            ih = A1::add(oh, ih);
+            il += ol;
+            ih += oh;
         }
     }
-    S rl = A1::shfl_up(il, 1), rh = A1::shfl_up(ih, 1);  // exclusive per lane
-    if (lane == 0) rl = rh = A1::zero();
+    int64_t rl = shfl_up(il, 1), rh = shfl_up(ih, 1);  // exclusive per lane
+    if (lane == 0) rl = rh = 0;
     if (lane == kWave - 1) {
         s_wave_l[wid] = il;
         s_wave_h[wid] = ih;
     }
     __syncthreads();
-    S wl = A1::zero(), wh = A1::zero(), tot_l = A1::zero();
+    int64_t tot_l = 0;
     for (int q = 0; q < kMergeBlock / kWave; ++q) {
         if (q < wid) {
-            wl = A1::add(wl, s_wave_l[q]);
-            wh = A1::add(wh, s_wave_h[q]);
+            rl += s_wave_l[q];
+            rh += s_wave_h[q];
         }
-        tot_l = A1::add(tot_l, s_wave_l[q]);
+        tot_l += s_wave_l[q];
     }
-    rl = A1::add(wl, rl);
-    rh = A1::add(wh, rh);
-    const T lp0 = (T)c0.lp, hp0 = (T)c0.hp;
+    // heavy keys carry the split offset, so that "heavy first" is B[q] <= A[r]
+    const double D = (double)n0 * mean - (c0.lp + c0.hp);
+    const int64_t D_q = quant(D, qs);
+    const double dres = ldexp((double)D_q, -qs) - D + mean;  // residual = (B - A) 2^-qs + dres
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t e = e0 + k;
         if (e < ne) {
-            const float x = s_w[e];
-            if (e < nl) {  // exclusive prefix, then add
-                s_key[e] = (T)(i0 + e) * mean - (lp0 + (T)A1::value(rl));
-                rl = A1::add1(rl, x);
-            } else {       // inclusive prefix
-                rh = A1::add1(rh, x);
-                const uint32_t q = e - nl;
-                s_key[nl + 1u + q] = (hp0 + (T)A1::value(rh)) - (T)(j0 + q + 1u) * mean;
+            if (e < nl) {  // exclusive running sum
+                s_key[e] = rl;
+                rl += inc[k];
+            } else {       // inclusive running sum
+                rh += inc[k];
+                s_key[nl + 1u + (e - nl)] = rh - D_q;
             }
         }
     }
-    if (t == 0) s_key[nl] = (T)(i0 + nl) * mean - (lp0 + (T)A1::value(tot_l));  // X(i1)
+    if (t == 0) s_key[nl] = tot_l;  // a at i1
     __syncthreads();
 
     // ---- merge: thread t emits buckets [kPer*t, kPer*t + kPer) ----------
     // p = residual / mean: FP32 mode divides as the reference does; the FP64
     // modes multiply by the FP64 reciprocal (within 1.5 ulp of the FP64
     // quotient before the rounding to float)
-    const T inv_mean = (T)1 / mean;
-    auto ratio = [&](T x) -> float {
-        if constexpr (PREC == kFP32) return (float)(x / mean);
+    const double inv_mean = 1.0 / mean;
+    auto ratio = [&](double x) -> float {
+        if constexpr (PREC == kFP32) return (float)((float)x / (float)mean);
         else return (float)(x * inv_mean);
     };
-    const T* A = s_key;            // A[r], r in [0, nl]
-    const T* B = s_key + nl + 1u;  // B[q], q in [0, nh)
+    const int64_t* A = s_key;            // A[r], r in [0, nl]
+    const int64_t* B = s_key + nl + 1u;  // B[q], q in [0, nh)
     const uint32_t d0 = e0;
     uint32_t slot[kPer];
     uint2 ent[kPer];
@@ -352,21 +370,25 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
         uint32_t q = (uint32_t)qa;
         uint32_t r = d0 - q;
         const uint32_t d1 = min(d0 + (uint32_t)kPer, ne);
+        int64_t av = A[r];  // r <= nl: A[nl] exists
+        int64_t bv = q < nh ? B[q] : 0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             if (d0 + (uint32_t)k < d1) {
-                const bool take_heavy = q < nh && (r >= nl || B[q] <= A[r]);
+                const bool take_heavy = q < nh && (r >= nl || bv <= av);
                 if (take_heavy) {
-                    const T resid = (B[q] - A[r]) + mean;
-                    const float p = ratio(resid);
+                    const double resid = ldexp((double)(bv - av), -qs) + dres;
+                    const float p = fminf(fmaxf(ratio(resid), 0.0f), 1.0f);
                     slot[k] = nl + q;
                     ent[k] = make_uint2(__float_as_uint(p), s_idx[nl + q + 1u]);
                     ++q;
+                    bv = q < nh ? B[q] : 0;
                 } else {
-                    const float p = ratio((T)s_w[r]);
+                    const float p = fminf(ratio((double)s_w[r]), 1.0f);
                     slot[k] = r;
                     ent[k] = make_uint2(__float_as_uint(p), s_idx[nl + q]);
                     ++r;
+                    av = A[r];
                 }
                 cnt = k + 1;
             }
@@ -391,11 +413,10 @@ __global__ __launch_bounds__(kMergeBlock) void k_mergepack(const float* __restri
                                                       const Params* __restrict__ prm,
                                                       const Coarse* __restrict__ bounds,
                                                       AliasEntry* __restrict__ table) {
-    using T = typename RealT<PREC>::T;
     const uint32_t b = blockIdx.x;
     const uint32_t n0 = b * (uint32_t)kMergeTile;
     const uint32_t n1 = min(n0 + (uint32_t)kMergeTile, N);
-    mergepack_block<PREC>(pw, idx, N, prm->heavy_count, (T)prm->mean, n0, n1, bounds[b],
+    mergepack_block<PREC>(pw, idx, N, prm->heavy_count, prm->mean, prm->total, n0, n1, bounds[b],
                           bounds[b + 1], table);
 }
 
@@ -406,7 +427,6 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
     const uint32_t* __restrict__ mitem_blk, const uint32_t* __restrict__ seg_b0,
     const Params* __restrict__ seg_prm, const Coarse* __restrict__ bounds,
     AliasEntry* __restrict__ table) {
-    using T = typename RealT<PREC>::T;
     const uint32_t g = mitem_seg[blockIdx.x];
     const uint32_t blk = mitem_blk[blockIdx.x];
     const uint64_t off = seg_off[g];
@@ -414,8 +434,251 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
     const uint32_t n0 = blk * (uint32_t)kMergeTile;
     const uint32_t n1 = min(n0 + (uint32_t)kMergeTile, N);
     const Coarse* bd = bounds + seg_b0[g] + blk;
-    mergepack_block<PREC>(pw + off, idx + off, N, seg_prm[g].heavy_count, (T)seg_prm[g].mean, n0,
-                          n1, bd[0], bd[1], table + off);
+    mergepack_block<PREC>(pw + off, idx + off, N, seg_prm[g].heavy_count, seg_prm[g].mean,
+                          seg_prm[g].total, n0, n1, bd[0], bd[1], table + off);
+}
+
+// ---- boundary repair ---------------------------------------------------------
+// Splits found independently per boundary are monotone in the boundary when
+// the key sequences X and Y are; rounding in large prefix sums (FP32 mode at
+// large N, or weights within rounding of the mean) can break that.  This pass
+// makes every block range valid.  With m_b the buckets before boundary b:
+//     j'_b  = max_{b' <= b} j_b'              (heavy ranks non-decreasing)
+//     i''_b = max_{b' <= b} (m_b' - j'_b')    (light ranks non-decreasing)
+//     j''_b = m_b - i''_b
+// keeps i'' + j'' = m_b (so no block grows), j'' non-decreasing (i'' - i'
+// grows by at most m_{b+1} - m_b per step), and the ranges of the raw splits
+// (j'' in [max(0, m_b - L), min(m_b, H - 1)]).  Exact keys are monotone
+// already and then nothing changes.  The last boundary (m = N) is fixed.
+constexpr int kRepairThreads = 1024;
+constexpr int kRepairPer = 16;
+
+__device__ __forceinline__ uint32_t wave_incl_max_u(uint32_t v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const uint32_t o = shfl_up(v, d);
+        if (l >= d) v = v < o ? o : v;
+    }
+    return v;
+}
+
+// A repaired boundary gets the prefixes LP(i), HP(j) of its new split: the
+// nearest sample below plus at most kSample - 1 weights (as in k_coarse).
+__device__ __forceinline__ void repair_anchor(Coarse& c, const float* __restrict__ pw,
+                                              const double* __restrict__ samp_h,
+                                              const double* __restrict__ samp_l, uint32_t N,
+                                              uint32_t H, double light_sum) {
+    const uint32_t L = N - H;
+    double lp = 0.0, hp = 0.0;
+    if (c.i == L) {
+        lp = light_sum;
+    } else if (c.i > 0) {
+        const uint32_t m = c.i / kSample;
+        lp = m ? samp_l[m - 1u] : 0.0;
+        for (uint32_t r = m * kSample; r < c.i; ++r) lp += (double)pw[N - 1u - r];
+    }
+    if (c.j > 0) {
+        const uint32_t m = c.j / kSample;
+        hp = m ? samp_h[m - 1u] : 0.0;
+        for (uint32_t q = m * kSample; q < c.j; ++q) hp += (double)pw[q];
+    }
+    c.lp = lp;
+    c.hp = hp;
+}
+
+// Segmented form: one block per segment, boundaries [seg_b0[g], seg_b0[g + 1])
+// (a segment has at most 2^23 / kMergeTile + 1 boundaries).  The raw split j
+// is kept in the record's pad word between the passes.
+__global__ __launch_bounds__(kRepairThreads) void k_repair_bounds(
+    Coarse* __restrict__ bounds, const uint32_t* __restrict__ seg_b0,
+    const uint64_t* __restrict__ seg_off, const uint64_t* __restrict__ seg_samp0,
+    const Params* __restrict__ seg_prm, const float* __restrict__ pw,
+    const double* __restrict__ samp_h, const double* __restrict__ samp_l) {
+    __shared__ uint32_t s_w[kRepairThreads / kWave];
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wid = t / kWave;
+    const uint32_t g = blockIdx.x;
+    const uint32_t b0 = seg_b0[g];
+    const uint32_t b1 = seg_b0[g + 1];
+    const uint32_t M = b1 - b0 - 1u;  // all but the last boundary
+    Coarse* bd = bounds + b0;
+    const uint64_t off = seg_off[g];
+    const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
+    for (int pass = 0; pass < 2; ++pass) {
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < M; c0 += (uint32_t)(kRepairThreads * kRepairPer)) {
+            const uint32_t base = c0 + (uint32_t)t * kRepairPer;
+            uint32_t v[kRepairPer];
+            uint32_t m = 0;
+#pragma unroll
+            for (int k = 0; k < kRepairPer; ++k) {
+                const uint32_t b = base + k;
+                v[k] = 0;
+                if (b < M) v[k] = pass == 0 ? bd[b].j : b * (uint32_t)kMergeTile - bd[b].j;
+                m = m < v[k] ? v[k] : m;
+            }
+            const uint32_t incl = wave_incl_max_u(m);
+            uint32_t run = shfl_up(incl, 1);
+            if (lane == 0) run = 0;
+            if (lane == kWave - 1) s_w[wid] = incl;
+            __syncthreads();
+            uint32_t chunk = carry;
+            for (int q = 0; q < kRepairThreads / kWave; ++q) {
+                if (q < wid) run = run < s_w[q] ? s_w[q] : run;
+                chunk = chunk < s_w[q] ? s_w[q] : chunk;
+            }
+            run = run < carry ? carry : run;
+#pragma unroll
+            for (int k = 0; k < kRepairPer; ++k) {
+                const uint32_t b = base + k;
+                run = run < v[k] ? v[k] : run;
+                if (b < M) {
+                    if (pass == 0) {
+                        bd[b].pad = bd[b].j;
+                        bd[b].j = run;
+                    } else {
+                        Coarse c = bd[b];
+                        c.i = run;
+                        c.j = b * (uint32_t)kMergeTile - run;
+                        if (c.j != (uint32_t)c.pad)
+                            repair_anchor(c, pw + off, samp_h + seg_samp0[g], samp_l + seg_samp0[g],
+                                          N, seg_prm[g].heavy_count, seg_prm[g].light_sum);
+                        bd[b] = c;
+                    }
+                }
+            }
+            carry = chunk;
+            __syncthreads();  // s_w is rewritten by the next chunk; pass 1 reads pass 0's writes
+        }
+    }
+}
+
+// Single-distribution form: block g repairs boundaries
+// [g * kRepairSpan, (g + 1) * kRepairSpan) and chains to its predecessors by
+// decoupled look-back, once per pass (max is exact and order-free, so the
+// result does not depend on the look-back timing).
+constexpr int kRepairLbThreads = 256;
+constexpr int kRepairLbPer = (int)kRepairSpan / kRepairLbThreads;
+
+struct MaxOps {
+    static __device__ __forceinline__ uint32_t zero() { return 0u; }
+    static __device__ __forceinline__ uint32_t add(uint32_t a, uint32_t b) { return a < b ? b : a; }
+    static __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t o = __shfl_xor(v, d, kWave);
+            v = v < o ? o : v;
+        }
+        return v;
+    }
+    static __device__ __forceinline__ bool read(const uint64_t* r, uint32_t& v) {
+        const uint64_t x = ld_word(r);
+        v = (uint32_t)x;
+        return (x >> 63) == 0;
+    }
+    static __device__ __forceinline__ void publish(uint64_t* r, const uint32_t& v) {
+        st_word(r, (uint64_t)v);
+    }
+};
+
+// In-place inclusive max-scan of the block's items (thread t holds items
+// kRepairLbPer*t ..); returns the block maximum.  Ends synchronised.
+__device__ __forceinline__ uint32_t block_incl_max(uint32_t (&v)[kRepairLbPer], uint32_t* s_w) {
+    const int lane = lane_id();
+    const int wid = threadIdx.x / kWave;
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < kRepairLbPer; ++k) v[k] = m = m < v[k] ? v[k] : m;
+    const uint32_t incl = wave_incl_max_u(m);
+    uint32_t run = shfl_up(incl, 1);
+    if (lane == 0) run = 0;
+    if (lane == kWave - 1) s_w[wid] = incl;
+    __syncthreads();
+    uint32_t total = 0;
+    for (int q = 0; q < kRepairLbThreads / kWave; ++q) {
+        if (q < wid) run = run < s_w[q] ? s_w[q] : run;
+        total = total < s_w[q] ? s_w[q] : total;
+    }
+#pragma unroll
+    for (int k = 0; k < kRepairLbPer; ++k) v[k] = v[k] < run ? run : v[k];
+    __syncthreads();
+    return total;
+}
+
+// Exclusive max over the blocks before blk (records `recs`), executed by wave 0;
+// result broadcast through *s_ex.
+__device__ __forceinline__ void repair_lookback(uint64_t* recs, uint32_t blk, uint32_t agg,
+                                                uint32_t* s_ex) {
+    if (threadIdx.x >= (unsigned)kWave) return;
+    const int lane = lane_id();
+    uint64_t* rec = recs + (uint64_t)blk * kStateWords;
+    uint32_t ex = 0;
+    if (blk == 0) {
+        if (lane == 0) publish_inclusive<uint32_t, MaxOps>(rec, agg);
+    } else {
+        if (lane == 0) publish_aggregate<uint32_t, MaxOps>(rec, agg);
+        ex = lookback<uint32_t, MaxOps>(recs, blk);
+        if (lane == 0) publish_inclusive<uint32_t, MaxOps>(rec, MaxOps::add(ex, agg));
+    }
+    if (lane == 0) *s_ex = ex;
+}
+
+__global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
+    Coarse* __restrict__ bounds, uint32_t nbound, uint32_t N, Params* __restrict__ prm,
+    uint64_t* __restrict__ recs, const float* __restrict__ pw, const double* __restrict__ samp_h,
+    const double* __restrict__ samp_l) {
+    __shared__ uint32_t s_w[kRepairLbThreads / kWave];
+    __shared__ uint32_t s_blk, s_ex;
+    if (threadIdx.x == 0) s_blk = atomicAdd(&prm->repair_ticket, 1u);  // predecessors first
+    __syncthreads();
+    const uint32_t blk = s_blk;
+    const uint32_t M = nbound - 1u;  // all but the last boundary
+    const uint32_t nrb = (M + kRepairSpan - 1u) / kRepairSpan;
+    const uint32_t base = blk * kRepairSpan + (uint32_t)threadIdx.x * kRepairLbPer;
+    // pass 0: j' = running max of j
+    uint32_t v[kRepairLbPer], raw_j[kRepairLbPer];
+#pragma unroll
+    for (int k = 0; k < kRepairLbPer; ++k) raw_j[k] = v[k] = base + k < M ? bounds[base + k].j : 0u;
+    uint32_t agg = block_incl_max(v, s_w);
+    repair_lookback(recs, blk, agg, &s_ex);
+    __syncthreads();
+    uint32_t ex = s_ex;
+    // pass 1: i'' = running max of m_b - j'_b
+#pragma unroll
+    for (int k = 0; k < kRepairLbPer; ++k) {
+        const uint32_t jp = v[k] < ex ? ex : v[k];
+        v[k] = base + k < M ? (base + k) * (uint32_t)kMergeTile - jp : 0u;
+    }
+    agg = block_incl_max(v, s_w);
+    repair_lookback(recs + (uint64_t)nrb * kStateWords, blk, agg, &s_ex);
+    __syncthreads();
+    ex = s_ex;
+#pragma unroll
+    for (int k = 0; k < kRepairLbPer; ++k) {
+        const uint32_t b = base + k;
+        if (b < M) {
+            const uint32_t i = v[k] < ex ? ex : v[k];
+            const uint32_t j = b * (uint32_t)kMergeTile - i;
+            if (j != raw_j[k]) {
+                Coarse c = bounds[b];
+                c.i = i;
+                c.j = j;
+                repair_anchor(c, pw, samp_h, samp_l, N, prm->heavy_count, prm->light_sum);
+                bounds[b] = c;
+            }
+        }
+    }
+}
+
+hipError_t launch_repair_bounds(Coarse* bounds, uint64_t n, Params* prm, uint64_t* states,
+                                const float* pw, const double* samp_h, const double* samp_l,
+                                hipStream_t s) {
+    const uint32_t nbound = (uint32_t)merge_blocks_for(n) + 1u;
+    k_repair_bounds_lb<<<(uint32_t)repair_blocks_for(n), kRepairLbThreads, 0, s>>>(
+        bounds, nbound, (uint32_t)n, prm, states + repair_records_offset(n), pw, samp_h, samp_l);
+    return hipGetLastError();
 }
 
 hipError_t launch_coarse(const float* pw, uint64_t n, int prec, const Params* prm,
@@ -437,20 +700,11 @@ hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params*
                             hipStream_t s) {
     const uint32_t N = (uint32_t)n;
     const uint32_t blocks = (uint32_t)merge_blocks_for(n);
-    switch (prec) {
-        case kFP32:
-            k_mergepack<kFP32><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
-            break;
-        case kFP64Carry:
-            k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
-            break;
-        case kComp:
-            k_mergepack<kComp><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
-            break;
-        default:
-            k_mergepack<kFP64><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
-            break;
-    }
+    // the merge keys are exact in every mode; only FP32 mode's p differs
+    if (prec == kFP32)
+        k_mergepack<kFP32><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
+    else
+        k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
     return hipGetLastError();
 }
 
@@ -466,15 +720,13 @@ hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const doub
     do {                                                                                        \
         k_seg_coarse<P><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0,          \
                                               seg_b0, nseg, seg_prm, bounds);                   \
+        k_repair_bounds<<<nseg, kRepairThreads, 0, s>>>(bounds, seg_b0, seg_off, seg_samp0,     \
+                                                        seg_prm, pw, samp_h, samp_l);            \
         k_seg_mergepack<P><<<mitems, kMergeBlock, 0, s>>>(pw, idx, seg_off, mitem_seg, mitem_blk,    \
                                                      seg_b0, seg_prm, bounds, table);           \
     } while (0)
-    switch (prec) {
-        case kFP32: WRS_LAUNCH_SMP(kFP32); break;
-        case kFP64Carry: WRS_LAUNCH_SMP(kFP64Carry); break;
-        case kComp: WRS_LAUNCH_SMP(kComp); break;
-        default: WRS_LAUNCH_SMP(kFP64); break;
-    }
+    if (prec == kFP32) WRS_LAUNCH_SMP(kFP32);
+    else WRS_LAUNCH_SMP(kFP64Carry);
 #undef WRS_LAUNCH_SMP
     return hipGetLastError();
 }

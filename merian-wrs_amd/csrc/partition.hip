@@ -585,11 +585,13 @@ namespace {
 template <int P, bool V>
 void launch_part_t(const float* w, uint32_t n, Params* prm, uint64_t* states, uint32_t* idx,
                    float* pw, float* hi, float* lo, double* samp_h, double* samp_l,
-                   hipStream_t s) {
+                   hipStream_t s, hipEvent_t ev_agg, hipEvent_t ev_scan) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
     k_part_agg<P, V><<<tiles, kBlock, 0, s>>>(w, n, prm, states);
+    if (ev_agg) (void)hipEventRecord(ev_agg, s);
     k_part_tilescan<P><<<(tiles + kSpineTiles - 1) / kSpineTiles, kSpineThreads, 0, s>>>(
         states, tiles, prm);
+    if (ev_scan) (void)hipEventRecord(ev_scan, s);
     if (hi)
         k_part_emit<P, V, true><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, hi, lo,
                                                          samp_h, samp_l);
@@ -601,32 +603,28 @@ void launch_part_t(const float* w, uint32_t n, Params* prm, uint64_t* states, ui
 template <int P>
 void launch_part_p(bool vec, const float* w, uint32_t n, Params* prm, uint64_t* states,
                    uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
-                   double* samp_l, hipStream_t s) {
-    if (vec) launch_part_t<P, true>(w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
-    else launch_part_t<P, false>(w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
+                   double* samp_l, hipStream_t s, hipEvent_t ea, hipEvent_t es) {
+    if (vec) launch_part_t<P, true>(w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ea, es);
+    else launch_part_t<P, false>(w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ea, es);
 }
 
 }  // namespace
 
 hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
                             uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
-                            double* samp_l, hipStream_t s) {
+                            double* samp_l, hipStream_t s, hipEvent_t ev_agg,
+                            hipEvent_t ev_scan) {
     const bool vec = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     const uint32_t nn = (uint32_t)n;
+#define WRS_PART(P) \
+    launch_part_p<P>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ev_agg, ev_scan)
     switch (prec) {
-        case kFP32:
-            launch_part_p<kFP32>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
-            break;
-        case kFP64Carry:
-            launch_part_p<kFP64Carry>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
-            break;
-        case kComp:
-            launch_part_p<kComp>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
-            break;
-        default:
-            launch_part_p<kFP64>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s);
-            break;
+        case kFP32: WRS_PART(kFP32); break;
+        case kFP64Carry: WRS_PART(kFP64Carry); break;
+        case kComp: WRS_PART(kComp); break;
+        default: WRS_PART(kFP64); break;
     }
+#undef WRS_PART
     return hipGetLastError();
 }
 

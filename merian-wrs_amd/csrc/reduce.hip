@@ -106,7 +106,8 @@ __device__ __forceinline__ void finalize_block(const double* __restrict__ partia
         p.heavy_count = 0;
         p.heavy_sum = 0.0;
         p.light_sum = 0.0;
-        p.reserved[0] = 0;
+        p.repair_ticket = 0;
+        p.reserved = 0;
         *prm = p;
     }
 }

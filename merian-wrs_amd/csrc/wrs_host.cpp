@@ -136,17 +136,21 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
               "finalize launch");
     if (ev) check_hip(hipEventRecord(ev[2], s), "event");
     check_hip(wrsk::launch_partition(b.weights, N, (int)prec, ws.params, ws.states, ws.indices,
-                                     ws.pweights, hi, lo, ws.samp_h, ws.samp_l, s),
+                                     ws.pweights, hi, lo, ws.samp_h, ws.samp_l, s,
+                                     ev ? ev[3] : nullptr, ev ? ev[4] : nullptr),
               "partition launch");
-    if (ev) check_hip(hipEventRecord(ev[3], s), "event");
+    if (ev) check_hip(hipEventRecord(ev[5], s), "event");
     check_hip(wrsk::launch_coarse(ws.pweights, N, (int)prec, ws.params, ws.samp_h, ws.samp_l,
                                   ws.bounds, s),
               "coarse split launch");
-    if (ev) check_hip(hipEventRecord(ev[4], s), "event");
+    check_hip(wrsk::launch_repair_bounds(ws.bounds, N, ws.params, ws.states, ws.pweights,
+                                         ws.samp_h, ws.samp_l, s),
+              "boundary repair launch");
+    if (ev) check_hip(hipEventRecord(ev[6], s), "event");
     check_hip(wrsk::launch_mergepack(ws.pweights, N, (int)prec, ws.params, ws.indices, ws.bounds,
                                      reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable), s),
               "merge pack launch");
-    if (ev) check_hip(hipEventRecord(ev[5], s), "event");
+    if (ev) check_hip(hipEventRecord(ev[7], s), "event");
     if (dbg) {
         if (dbg->splits)
             check_hip(wrsk::launch_splits(N, splitCount(N), (int)prec, ws.params, ws.prefix_hi,
@@ -232,7 +236,7 @@ struct wrs_plan {
     wrs::PSAConfig cfg;
     wrs::Workspace ws;
     bool timing = false;
-    hipEvent_t ev[6] = {};  // brackets reduce, finalize, partition, coarse splits, merge pack
+    hipEvent_t ev[WRS_PHASE_COUNT + 1] = {};  // brackets the build's kernels
 };
 
 namespace {
@@ -486,13 +490,13 @@ wrs_status wrs_plan_set_timing(wrs_plan* plan, int enable) {
     });
 }
 
-wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms5) {
+wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms, uint32_t count) {
     return guarded([&] {
         require_plan(plan);
-        if (!ms5) throw wrs::Error(WRS_EINVAL, "ms5 is null");
-        wrs::check_hip(hipEventSynchronize(plan->ev[5]), "event sync");
-        for (int k = 0; k < 5; ++k)
-            wrs::check_hip(hipEventElapsedTime(&ms5[k], plan->ev[k], plan->ev[k + 1]),
+        if (!ms && count) throw wrs::Error(WRS_EINVAL, "ms is null");
+        wrs::check_hip(hipEventSynchronize(plan->ev[WRS_PHASE_COUNT]), "event sync");
+        for (uint32_t k = 0; k < count && k < WRS_PHASE_COUNT; ++k)
+            wrs::check_hip(hipEventElapsedTime(&ms[k], plan->ev[k], plan->ev[k + 1]),
                            "event elapsed");
     });
 }

@@ -84,7 +84,7 @@ def lib():
         "wrs_psa_build_batched": [vp, vp, vp, vp],
         "wrs_alias_sample_batched": [vp, vp, u32, u32, u64, u64, vp, vp],
         "wrs_plan_set_timing": [vp, i32],
-        "wrs_plan_phase_times": [vp, vp],
+        "wrs_plan_phase_times": [vp, vp, u32],
         "wrs_rt_device_count": [C.POINTER(C.c_int)],
         "wrs_rt_device_info": [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t,
                                C.POINTER(C.c_int), C.POINTER(C.c_uint64)],
@@ -301,10 +301,13 @@ class Plan:
     def set_timing(self, enable: bool) -> None:
         check(lib().wrs_plan_set_timing(self.handle, int(bool(enable))))
 
+    PHASES = ("reduce", "finalize", "tile_aggregate", "tile_scan", "partition_emit",
+              "boundary_splits", "merge_pack")
+
     def phase_times(self) -> list:
-        """ms of [reduce, finalize, partition, coarse splits, merge pack]."""
-        ms = (C.c_float * 5)()
-        check(lib().wrs_plan_phase_times(self.handle, ms))
+        """ms of the kernels of the last timed build, in the order of PHASES."""
+        ms = (C.c_float * len(self.PHASES))()
+        check(lib().wrs_plan_phase_times(self.handle, ms, len(self.PHASES)))
         return list(ms)
 
     def close(self) -> None:
