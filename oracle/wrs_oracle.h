@@ -15,7 +15,7 @@
  * the reference's own test instantiation) and *_f64 (T = P = double, the
  * instantiation the reference's tests-of-tests use, src/host/assert/test.cpp:317-346).
  *
- * Documented deviations from the reference text (each also in DESIGN.md):
+ * Documented deviations from the reference text:
  *   - ceilMulDiv uses 64-bit intermediates (src/host/reference/split.hpp:59-65
  *     overflows in uint32 once K > 2^16); value-identical where it does not.
  *   - two split rules: the reference CPU's (split.hpp:14-57) and the reference

@@ -119,7 +119,7 @@ def test_splits_and_table_match_oracle(wrs, oracle, kind, n, split_size):
 
 @pytest.mark.parametrize("kind", ["uniform", "lognormal", "zipf", "spike"])
 def test_implied_distribution_error_compensated(wrs, oracle, kind):
-    """The north-star accuracy bar: max per-element relative error <= 1e-6."""
+    """The accuracy bar of the compensated build: max per-element relative error <= 1e-6."""
     n = 1 << 20
     print_case("distribution error", kind, n)
     w = gen_weights(oracle, kind, n)
