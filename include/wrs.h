@@ -181,13 +181,15 @@ wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_
 /* ---- introspection ---- */
 const char* wrs_last_error(void);         /* thread-local message of the last failure */
 const char* wrs_version(void);
-/* Per-phase device times (ms) of the most recent build on this plan when
+/* Per-phase device times (ms) of the most recent PSA build on this plan when
  * timing is enabled (HIP events on the build's stream): [0] reduce,
- * [1] finalize, [2] prefix-partition, [3] block-boundary splits, [4] merge pack.
- * Replaces the merian profiler scopes around each dispatch
- * (src/device/wrs/alias/psa/splitpack/inline/InlineSplitPack.hpp:109-150). */
+ * [1] finalize, [2] tile aggregates, [3] tile scan, [4] partition emit,
+ * [5] block-boundary splits, [6] merge pack.  Writes min(count,
+ * WRS_PHASE_COUNT) values.  Replaces the merian profiler scopes around each
+ * dispatch (src/device/wrs/alias/psa/splitpack/inline/InlineSplitPack.hpp:109-150). */
+#define WRS_PHASE_COUNT 7
 wrs_status wrs_plan_set_timing(wrs_plan* plan, int enable);
-wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms5);
+wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms, uint32_t count);
 
 #ifdef __cplusplus
 }
