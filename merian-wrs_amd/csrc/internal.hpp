@@ -118,6 +118,13 @@ hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params*
 // the K split points S(k), k = 1..K, of the reference's splitK layout (debug output)
 hipError_t launch_splits(uint64_t n, uint32_t K, int prec, const Params* prm, const float* hi,
                          const float* lo, SplitRec* splits, hipStream_t s);
+// reduce-then-scan form for large n: needs no total beforehand (it writes
+// prm->total, mean and inv_total itself) and no reset pass; states as for a
+// PSA build of n
+hipError_t launch_scan_rts(const float* w, uint64_t n, int prec, int normalise, Params* prm,
+                           uint64_t* states, float* hi, float* lo, hipStream_t s);
+// tiles up to which the single-pass look-back scan is used
+constexpr uint32_t kScanSinglePassTiles = 64;
 hipError_t launch_scan(const float* w, uint64_t n, int prec, int normalise, Params* prm,
                        uint64_t* states, float* hi, float* lo, hipStream_t s);
 hipError_t launch_alias_sample(const AliasEntry* table, uint64_t n, uint64_t s_count,

@@ -107,11 +107,18 @@ void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace
 void PrefixSum::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) const {
     require(b.elements && b.prefix, "null scan buffer");
     require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
-    Mean(m_prec).run(s, b.elements, N, ws);
-    check_hip(wrsk::launch_scan(b.elements, N, (int)m_prec, m_normalise ? 1 : 0, ws.params,
-                                ws.states, b.prefix,
-                                wrsk::prec_has_lo(m_prec) ? b.prefixLo : nullptr, s),
-              "scan launch");
+    float* lo = wrsk::prec_has_lo(m_prec) ? b.prefixLo : nullptr;
+    if (wrsk::tiles_for(N) <= wrsk::kScanSinglePassTiles) {
+        // short look-back chain: mean pass (total, shift, record reset) + one scan pass
+        Mean(m_prec).run(s, b.elements, N, ws);
+        check_hip(wrsk::launch_scan(b.elements, N, (int)m_prec, m_normalise ? 1 : 0, ws.params,
+                                    ws.states, b.prefix, lo, s),
+                  "scan launch");
+    } else {
+        check_hip(wrsk::launch_scan_rts(b.elements, N, (int)m_prec, m_normalise ? 1 : 0,
+                                        ws.params, ws.states, b.prefix, lo, s),
+                  "scan launch");
+    }
 }
 
 void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* dbg,

@@ -144,9 +144,12 @@ def test_its_sampling_statistics(wrs, oracle):
     assert chi["p"] > 1e-5 and chi["zero_mass_hits"] == 0
 
 
+# 64 * 4096 - 5 takes the single-pass look-back scan, the others the
+# reduce-then-scan form (more than 64 tiles; 3 * 2^20 + 7 spans several spine
+# blocks of 1024 tiles)
+@pytest.mark.parametrize("n", [64 * 4096 - 5, (1 << 20) + 9, 3 * (1 << 22) + 7])
 @pytest.mark.parametrize("precision", [0, 1, 2, 3])
-def test_scan_inclusive(wrs, oracle, precision):
-    n = (1 << 20) + 9
+def test_scan_inclusive(wrs, oracle, precision, n):
     case("scan", precision, n)
     w = oracle.gen_uniform01(3, n)
     plan = wrs.Plan(n, precision=precision)
@@ -155,14 +158,23 @@ def test_scan_inclusive(wrs, oracle, precision):
     lo = wrs.empty(n, np.float32)
     plan.scan_inclusive(dw, n, hi, lo if precision >= 2 else None)
     wrs.synchronize()
+    got_hi = hi.numpy()
+    got_lo = lo.numpy() if precision >= 2 else None
     exact = oracle.prefix_sum_f64(w.astype(np.float64))
-    got = hi.numpy().astype(np.float64)
+    got = got_hi.astype(np.float64)
     if precision >= 2:
-        got += lo.numpy().astype(np.float64)
+        got += got_lo.astype(np.float64)
     rel = np.abs(got - exact).max() / exact[-1]
     print(f"  precision {precision}: max rel error {rel:.3e}")
     bound = {0: 1e-4, 1: 1e-7, 2: 1e-12, 3: 1e-14}[precision]
     assert rel < bound
+    if precision >= 1:
+        # exact carries: a second run reproduces every bit
+        plan.scan_inclusive(dw, n, hi, lo if precision >= 2 else None)
+        wrs.synchronize()
+        assert np.array_equal(hi.numpy(), got_hi)
+        if precision >= 2:
+            assert np.array_equal(lo.numpy(), got_lo)
 
 
 def test_scan_all_ones_exact(wrs):
