@@ -123,9 +123,9 @@ void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) c
     wrs::check_hip(wrsk::launch_seg_finalize(seg_off, seg_item0, nseg, prec, partials, seg_prm,
                                              ticket, s),
                    "segmented finalize launch");
-    wrs::check_hip(wrsk::launch_seg_partition(w, seg_off, item_seg, item_tile, seg_item0, items,
-                                              prec, seg_prm, ticket, states, idx, pw, samp_h,
-                                              samp_l, seg_samp0, s),
+    wrs::check_hip(wrsk::launch_seg_partition(w, seg_off, item_seg, item_tile, seg_item0, nseg,
+                                              items, prec, lookback ? 1 : 0, seg_prm, ticket,
+                                              states, idx, pw, samp_h, samp_l, seg_samp0, s),
                    "segmented partition launch");
     wrs::check_hip(wrsk::launch_seg_mergepack(pw, idx, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
                                               nseg, nbound, mitem_seg, mitem_blk, mitems, prec,

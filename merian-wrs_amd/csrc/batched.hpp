@@ -19,6 +19,7 @@ namespace wrsb {
 struct BatchPlan {
     uint32_t split_size = 32;
     int prec = 2;
+    bool lookback = false;       // segmented partition: single pass with look-back, else reduce-then-scan
     uint32_t nseg = 0;
     uint32_t items = 0;          // (segment, tile) work items of reduce and partition
     uint32_t mitems = 0;         // (segment, merge block) work items of the merge pack

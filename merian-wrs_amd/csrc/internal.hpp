@@ -146,11 +146,15 @@ hipError_t launch_seg_reduce(const float* w, const uint64_t* seg_off, const uint
 hipError_t launch_seg_finalize(const uint64_t* seg_off, const uint32_t* seg_item0, uint32_t nseg,
                                int prec, const double* partials, Params* seg_prm,
                                uint32_t* ticket, hipStream_t s);
+// lookback != 0: one pass with decoupled look-back per segment (states reset
+// by the segmented reduce, ticket by the segmented finalize); else
+// reduce-then-scan (aggregate, per-segment spine, emit; no reset needed)
 hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
-                                const uint32_t* item_tile, const uint32_t* seg_item0,
-                                uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
-                                uint64_t* states, uint32_t* idx, float* pw, double* samp_h,
-                                double* samp_l, const uint64_t* seg_samp0, hipStream_t s);
+                                const uint32_t* item_tile, const uint32_t* seg_item0, uint32_t nseg,
+                                uint32_t items, int prec, int lookback, Params* seg_prm,
+                                uint32_t* ticket, uint64_t* states, uint32_t* idx, float* pw,
+                                double* samp_h, double* samp_l, const uint64_t* seg_samp0,
+                                hipStream_t s);
 hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const double* samp_h,
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,

@@ -450,8 +450,11 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
 // grows by at most m_{b+1} - m_b per step), and the ranges of the raw splits
 // (j'' in [max(0, m_b - L), min(m_b, H - 1)]).  Exact keys are monotone
 // already and then nothing changes.  The last boundary (m = N) is fixed.
-constexpr int kRepairThreads = 1024;
-constexpr int kRepairPer = 16;
+// segmented form: one wave per segment (a batch holds thousands of short
+// segments; a segment has at most 2^23 / kMergeTile + 1 boundaries, i.e. at
+// most 17 rounds of kRepairThreads * kRepairPer)
+constexpr int kRepairThreads = 64;
+constexpr int kRepairPer = 4;
 
 __device__ __forceinline__ uint32_t wave_incl_max_u(uint32_t v) {
     const int l = lane_id();
@@ -487,9 +490,9 @@ __device__ __forceinline__ void repair_anchor(Coarse& c, const float* __restrict
     c.hp = hp;
 }
 
-// Segmented form: one block per segment, boundaries [seg_b0[g], seg_b0[g + 1])
-// (a segment has at most 2^23 / kMergeTile + 1 boundaries).  The raw split j
-// is kept in the record's pad word between the passes.
+// Segmented form: one block (kRepairThreads) per segment, boundaries
+// [seg_b0[g], seg_b0[g + 1]).  The raw split j is kept in the record's pad
+// word between the passes.
 __global__ __launch_bounds__(kRepairThreads) void k_repair_bounds(
     Coarse* __restrict__ bounds, const uint32_t* __restrict__ seg_b0,
     const uint64_t* __restrict__ seg_off, const uint64_t* __restrict__ seg_samp0,

@@ -36,6 +36,7 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "lookback.hpp"
+#include "tileload.hpp"
 #include "tilesum.hpp"
 
 namespace wrsk {
@@ -192,34 +193,14 @@ struct TileShared {
 // 16t..16t+15 (zero beyond m), classifies them against the pivot and scans:
 // thr_excl is the sum over the thread's predecessors in the tile, tile_agg
 // the tile total.  Ends with the block synchronised and s.buf free.
-template <int PREC, bool VEC>
+template <int PREC>
 __device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t base, uint32_t m,
                                           float pivot, TileShared<PREC>& s, float (&v)[kItems],
                                           Acc<PREC>& thr_excl, Acc<PREC>& tile_agg) {
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
-    if (VEC && m == (uint32_t)kTile) {
-        const float4* w4 = reinterpret_cast<const float4*>(w + base);
-        float4 v4[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) v4[j] = ld_nt4(w4 + t + kBlock * j);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int e = 4 * (t + kBlock * j);
-            const int q = e + e / 16;
-            s.buf[q + 0] = v4[j].x;
-            s.buf[q + 1] = v4[j].y;
-            s.buf[q + 2] = v4[j].z;
-            s.buf[q + 3] = v4[j].w;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) {
-            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * j;
-            s.buf[e + e / 16] = e < m ? w[base + e] : 0.0f;
-        }
-    }
+    load_tile_lds<true>(w + base, m, s.buf);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) v[k] = s.buf[17 * t + k];
@@ -391,7 +372,7 @@ constexpr uint32_t kExclBase = 8;
 static_assert(kExclBase + kCarryWords <= kStateWords, "carry records exceed the state buffer");
 
 // Pass 2a: aggregate of every tile.
-template <int PREC, bool VEC>
+template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_part_agg(const float* __restrict__ w, uint32_t n,
                                                      const Params* __restrict__ prm,
                                                      uint64_t* __restrict__ states) {
@@ -401,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_part_agg(const float* __restrict__ w
     const uint32_t m = min((uint32_t)kTile, n - base);
     float v[kItems];
     Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC, VEC>(w, base, m, prm->pivot, s, v, thr_excl, tile_agg);
+    tile_scan<PREC>(w, base, m, prm->pivot, s, v, thr_excl, tile_agg);
     if (threadIdx.x == 0)
         store_carry(states + (uint64_t)tile * kCarryWords, to_carry<PREC>(tile_agg, prm->shift));
 }
@@ -489,7 +470,7 @@ __global__ __launch_bounds__(kSpineThreads) void k_part_tilescan(uint64_t* __res
 }
 
 // Pass 2c: outputs of every tile from its exclusive prefix.
-template <int PREC, bool VEC, bool FULL>
+template <int PREC, bool FULL>
 __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ w, uint32_t n,
                                                       const Params* __restrict__ prm,
                                                       const uint64_t* __restrict__ states,
@@ -509,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
         load_carry(states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords);
     float v[kItems];
     Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC, VEC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
+    tile_scan<PREC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
     const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
     __syncthreads();  // s.buf is reused by the emit rounds
     tile_emit<PREC, FULL>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, pw_out,
@@ -550,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     const int shift = prm->shift;
     float v[kItems];
     Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC, false>(w + off, base, m, pivot, s, v, thr_excl, tile_agg);
+    tile_scan<PREC>(w + off, base, m, pivot, s, v, thr_excl, tile_agg);
 
     const int lane = threadIdx.x & (kWave - 1);
     if (threadIdx.x < kWave) {
@@ -580,32 +561,137 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
                            pw + off, nullptr, nullptr, samp_h + so, samp_l + so);
 }
 
+// ---- segmented: reduce-then-scan ------------------------------------------
+// The same three steps as the single-distribution form, per segment: the
+// aggregate of every (segment, tile) item (dense records, item order), an
+// exclusive scan of each segment's aggregates by one wave, then the emit.
+// No block waits on another, which at batch scale beats the single pass above
+// (whose blocks hold their CU during the look-back round trip) by more than
+// the extra read of the weights costs.  The FP64-mode carries are exact, so
+// both forms -- and the single-distribution build -- give the same bits.
+
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_seg_part_agg(
+    const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
+    const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile,
+    const Params* __restrict__ seg_prm, uint64_t* __restrict__ states) {
+    __shared__ TileShared<PREC> s;
+    const uint32_t item = blockIdx.x;
+    const uint32_t sg = item_seg[item];
+    const uint64_t off = seg_off[sg];
+    const uint32_t n = (uint32_t)(seg_off[sg + 1] - off);
+    const uint32_t base = item_tile[item] * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const Params* prm = seg_prm + sg;
+    float v[kItems];
+    Acc<PREC> thr_excl, tile_agg;
+    tile_scan<PREC>(w + off, base, m, prm->pivot, s, v, thr_excl, tile_agg);
+    if (threadIdx.x == 0)
+        store_carry(states + (uint64_t)item * kCarryWords, to_carry<PREC>(tile_agg, prm->shift));
+}
+
+// One wave per segment: exclusive prefixes of the segment's item aggregates,
+// kSegSpinePer consecutive items per lane per round; the segment totals go to
+// its Params.
+constexpr int kSegSpinePer = 4;
+
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_seg_part_spine(const uint32_t* __restrict__ seg_item0,
+                                                           uint32_t nseg, uint32_t items,
+                                                           Params* __restrict__ seg_prm,
+                                                           uint64_t* __restrict__ states) {
+    const uint32_t sg = blockIdx.x * (uint32_t)(kBlock / kWave) + threadIdx.x / kWave;
+    if (sg >= nseg) return;  // whole wave
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t i0 = seg_item0[sg], i1 = seg_item0[sg + 1];
+    const uint64_t* agg = states;
+    uint64_t* excl = states + (uint64_t)kExclBase * items;
+    Carry run = carry_zero();
+    for (uint32_t c0 = i0; c0 < i1; c0 += (uint32_t)(kWave * kSegSpinePer)) {
+        const uint32_t b = c0 + (uint32_t)lane * kSegSpinePer;
+        Carry a[kSegSpinePer];
+#pragma unroll
+        for (int k = 0; k < kSegSpinePer; ++k)
+            a[k] = b + k < i1 ? load_carry(agg + (uint64_t)(b + k) * kCarryWords) : carry_zero();
+        Carry mine = a[0];
+#pragma unroll
+        for (int k = 1; k < kSegSpinePer; ++k) mine = carry_add<PREC>(mine, a[k]);
+        Carry incl = mine;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const Carry o = carry_shfl_up(incl, d);
+            if (lane >= d) incl = carry_add<PREC>(o, incl);
+        }
+        Carry lex = carry_shfl_up(incl, 1);
+        if (lane == 0) lex = carry_zero();
+        Carry r = carry_add<PREC>(run, lex);
+#pragma unroll
+        for (int k = 0; k < kSegSpinePer; ++k) {
+            if (b + k < i1) store_carry(excl + (uint64_t)(b + k) * kCarryWords, r);
+            r = carry_add<PREC>(r, a[k]);
+        }
+        Carry chunk;
+        chunk.h.lo = __shfl(incl.h.lo, kWave - 1, kWave);
+        chunk.h.hi = __shfl(incl.h.hi, kWave - 1, kWave);
+        chunk.l.lo = __shfl(incl.l.lo, kWave - 1, kWave);
+        chunk.l.hi = __shfl(incl.l.hi, kWave - 1, kWave);
+        chunk.c = __shfl(incl.c, kWave - 1, kWave);
+        run = carry_add<PREC>(run, chunk);
+    }
+    if (lane == 0) {
+        Params* prm = seg_prm + sg;
+        const int shift = prm->shift;
+        prm->heavy_count = (uint32_t)run.c;
+        prm->heavy_sum = carry_value<PREC>(run.h, shift);
+        prm->light_sum = carry_value<PREC>(run.l, shift);
+    }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(kBlock) void k_seg_part_emit(
+    const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
+    const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile, uint32_t items,
+    const Params* __restrict__ seg_prm, const uint64_t* __restrict__ states,
+    uint32_t* __restrict__ idx, float* __restrict__ pw, double* __restrict__ samp_h,
+    double* __restrict__ samp_l, const uint64_t* __restrict__ seg_samp0) {
+    __shared__ TileShared<PREC> s;
+    const uint32_t item = blockIdx.x;
+    const uint32_t sg = item_seg[item];
+    const uint64_t off = seg_off[sg];
+    const uint32_t n = (uint32_t)(seg_off[sg + 1] - off);
+    const uint64_t so = seg_samp0[sg];
+    const uint32_t base = item_tile[item] * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const Params* prm = seg_prm + sg;
+    const float pivot = prm->pivot;
+    const Carry excl = load_carry(states + (uint64_t)kExclBase * items + (uint64_t)item * kCarryWords);
+    float v[kItems];
+    Acc<PREC> thr_excl, tile_agg;
+    tile_scan<PREC>(w + off, base, m, pivot, s, v, thr_excl, tile_agg);
+    const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
+    __syncthreads();  // s.buf is reused by the emit rounds
+    tile_emit<PREC, false>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off, pw + off,
+                           nullptr, nullptr, samp_h + so, samp_l + so);
+}
+
 namespace {
 
-template <int P, bool V>
+template <int P>
 void launch_part_t(const float* w, uint32_t n, Params* prm, uint64_t* states, uint32_t* idx,
                    float* pw, float* hi, float* lo, double* samp_h, double* samp_l,
                    hipStream_t s, hipEvent_t ev_agg, hipEvent_t ev_scan) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
-    k_part_agg<P, V><<<tiles, kBlock, 0, s>>>(w, n, prm, states);
+    k_part_agg<P><<<tiles, kBlock, 0, s>>>(w, n, prm, states);
     if (ev_agg) (void)hipEventRecord(ev_agg, s);
     k_part_tilescan<P><<<(tiles + kSpineTiles - 1) / kSpineTiles, kSpineThreads, 0, s>>>(
         states, tiles, prm);
     if (ev_scan) (void)hipEventRecord(ev_scan, s);
     if (hi)
-        k_part_emit<P, V, true><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, hi, lo,
-                                                         samp_h, samp_l);
+        k_part_emit<P, true><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, hi, lo, samp_h,
+                                                      samp_l);
     else
-        k_part_emit<P, V, false><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, nullptr,
-                                                          nullptr, samp_h, samp_l);
-}
-
-template <int P>
-void launch_part_p(bool vec, const float* w, uint32_t n, Params* prm, uint64_t* states,
-                   uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
-                   double* samp_l, hipStream_t s, hipEvent_t ea, hipEvent_t es) {
-    if (vec) launch_part_t<P, true>(w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ea, es);
-    else launch_part_t<P, false>(w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ea, es);
+        k_part_emit<P, false><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, nullptr,
+                                                       nullptr, samp_h, samp_l);
 }
 
 }  // namespace
@@ -614,10 +700,9 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
                             uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
                             double* samp_l, hipStream_t s, hipEvent_t ev_agg,
                             hipEvent_t ev_scan) {
-    const bool vec = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     const uint32_t nn = (uint32_t)n;
 #define WRS_PART(P) \
-    launch_part_p<P>(vec, w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ev_agg, ev_scan)
+    launch_part_t<P>(w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ev_agg, ev_scan)
     switch (prec) {
         case kFP32: WRS_PART(kFP32); break;
         case kFP64Carry: WRS_PART(kFP64Carry); break;
@@ -629,14 +714,26 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
 }
 
 hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
-                                const uint32_t* item_tile, const uint32_t* seg_item0,
-                                uint32_t items, int prec, Params* seg_prm, uint32_t* ticket,
-                                uint64_t* states, uint32_t* idx, float* pw, double* samp_h,
-                                double* samp_l, const uint64_t* seg_samp0, hipStream_t s) {
-#define WRS_LAUNCH_SEG(P)                                                                 \
-    k_seg_partition<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_item0, \
-                                                seg_prm, ticket, states, idx, pw, samp_h,   \
-                                                samp_l, seg_samp0)
+                                const uint32_t* item_tile, const uint32_t* seg_item0, uint32_t nseg,
+                                uint32_t items, int prec, int lookback, Params* seg_prm,
+                                uint32_t* ticket, uint64_t* states, uint32_t* idx, float* pw,
+                                double* samp_h, double* samp_l, const uint64_t* seg_samp0,
+                                hipStream_t s) {
+    const uint32_t spine_blocks = (nseg + kBlock / kWave - 1) / (kBlock / kWave);
+#define WRS_LAUNCH_SEG(P)                                                                        \
+    if (lookback) {                                                                              \
+        k_seg_partition<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_item0,  \
+                                                    seg_prm, ticket, states, idx, pw, samp_h,    \
+                                                    samp_l, seg_samp0);                          \
+    } else {                                                                                     \
+        k_seg_part_agg<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_prm,     \
+                                                   states);                                      \
+        k_seg_part_spine<P><<<spine_blocks, kBlock, 0, s>>>(seg_item0, nseg, items, seg_prm,     \
+                                                            states);                             \
+        k_seg_part_emit<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, items,      \
+                                                    seg_prm, states, idx, pw, samp_h, samp_l,    \
+                                                    seg_samp0);                                  \
+    }
     switch (prec) {
         case kFP32: WRS_LAUNCH_SEG(kFP32); break;
         case kFP64Carry: WRS_LAUNCH_SEG(kFP64Carry); break;

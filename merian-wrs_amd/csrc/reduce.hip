@@ -15,6 +15,7 @@
 // segment on its own.
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "tileload.hpp"
 
 namespace wrsk {
 using namespace wrsdev;
@@ -24,36 +25,38 @@ namespace {
 template <bool FP64>
 using AccT = typename std::conditional<FP64, double, float>::type;
 
-// Adds tile `tile` of w[0..n) into acc in the fixed per-thread order.
-template <bool FP64, bool VEC>
+// Adds tile `tile` of w[0..n) into acc in the fixed per-thread order.  A full
+// tile at a 16-byte aligned address is loaded straight into registers; any
+// other tile (the tail, or a segment of a batch, which may start anywhere) is
+// staged in LDS as rows (tileload.hpp) and read back as the same float4 rows,
+// so the order -- and the sum -- is the same either way.  Block-uniform.
+template <bool FP64>
 __device__ __forceinline__ void reduce_tile(const float* __restrict__ w, uint64_t n, uint64_t tile,
                                             AccT<FP64>& acc) {
     using Acc = AccT<FP64>;
     const int t = threadIdx.x;
     const uint64_t base = tile * kTile;
-    if (VEC && base + kTile <= n) {
-        const float4* w4 = reinterpret_cast<const float4*>(w + base);
-        float4 v[4];
+    const float* p = w + base;
+    float4 v[4];
+    if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0 && base + kTile <= n) {
+        const float4* w4 = reinterpret_cast<const float4*>(p);
 #pragma unroll
         for (int j = 0; j < 4; ++j) v[j] = ld_nt4(w4 + t + kBlock * j);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            acc += (Acc)v[j].x;
-            acc += (Acc)v[j].y;
-            acc += (Acc)v[j].z;
-            acc += (Acc)v[j].w;
-        }
     } else {
-        float v[kItems];
+        __shared__ float4 s_rows[kTile / 4];
+        const uint32_t m = (uint32_t)(n - base < (uint64_t)kTile ? n - base : (uint64_t)kTile);
+        __syncthreads();  // the previous tile of a grid-stride loop has been read
+        load_tile_lds<false>(p, m, reinterpret_cast<float*>(s_rows));
+        __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < 4; ++j) v[j] = s_rows[t + kBlock * j];
+    }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint64_t e = base + 4u * ((uint64_t)t + (uint64_t)kBlock * j) + q;
-                v[4 * j + q] = e < n ? w[e] : 0.0f;
-            }
-#pragma unroll
-        for (int k = 0; k < kItems; ++k) acc += (Acc)v[k];
+    for (int j = 0; j < 4; ++j) {
+        acc += (Acc)v[j].x;
+        acc += (Acc)v[j].y;
+        acc += (Acc)v[j].z;
+        acc += (Acc)v[j].w;
     }
 }
 
@@ -114,7 +117,7 @@ __device__ __forceinline__ void finalize_block(const double* __restrict__ partia
 
 }  // namespace
 
-template <bool FP64, bool VEC>
+template <bool FP64>
 __global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ w, uint64_t n,
                                                    double* __restrict__ partials,
                                                    uint64_t* __restrict__ states,
@@ -123,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ w, 
     const uint32_t G = gridDim.x;
     const uint64_t tiles = (n + kTile - 1) / kTile;
     AccT<FP64> acc = 0;
-    for (uint64_t tile = blockIdx.x; tile < tiles; tile += G) reduce_tile<FP64, VEC>(w, n, tile, acc);
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += G) reduce_tile<FP64>(w, n, tile, acc);
     // reset the look-back records for the next kernel on the stream
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + t; i < state_words; i += (uint64_t)G * kBlock)
         states[i] = kSentinel;
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_reduce(const float* __restrict__
     const uint64_t off = seg_off[sg];
     const uint64_t n = seg_off[sg + 1] - off;
     AccT<FP64> acc = 0;
-    reduce_tile<FP64, false>(w + off, n, item_tile[item], acc);
+    reduce_tile<FP64>(w + off, n, item_tile[item], acc);
     if (threadIdx.x < kStateWords) states[(uint64_t)item * kStateWords + threadIdx.x] = kSentinel;
     const AccT<FP64> b = block_sum<FP64>(acc);
     if (threadIdx.x == 0) partials[item] = (double)b;
@@ -180,16 +183,10 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t words, uint64_t va
 
 hipError_t launch_reduce(const float* w, uint64_t n, int prec, double* partials, uint32_t G,
                          uint64_t* states, uint64_t state_words, hipStream_t s) {
-    const bool vec = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
-    const bool f64 = prec != kFP32;
-    if (f64 && vec)
-        k_reduce<true, true><<<G, kBlock, 0, s>>>(w, n, partials, states, state_words);
-    else if (f64)
-        k_reduce<true, false><<<G, kBlock, 0, s>>>(w, n, partials, states, state_words);
-    else if (vec)
-        k_reduce<false, true><<<G, kBlock, 0, s>>>(w, n, partials, states, state_words);
+    if (prec != kFP32)
+        k_reduce<true><<<G, kBlock, 0, s>>>(w, n, partials, states, state_words);
     else
-        k_reduce<false, false><<<G, kBlock, 0, s>>>(w, n, partials, states, state_words);
+        k_reduce<false><<<G, kBlock, 0, s>>>(w, n, partials, states, state_words);
     return hipGetLastError();
 }
 

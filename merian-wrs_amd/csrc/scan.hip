@@ -19,6 +19,7 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "lookback.hpp"
+#include "tileload.hpp"
 #include "tilesum.hpp"
 
 namespace wrsk {
@@ -75,13 +76,13 @@ struct ScanShared {
 
 // Loads tile [base, base + m) into registers (thread t holds the 16
 // consecutive elements 16t..16t+15, through an LDS transpose of coalesced
-// float4 rows) and scans it: thr_excl is the sum of the tile's elements before
+// float4 rows, tileload.hpp) and scans it: thr_excl is the sum of the tile's elements before
 // thread t's, tile_agg the tile's sum.  Every kernel of this file computes a
 // tile's sums through this function, so the single-pass and the
 // reduce-then-scan forms give identical in-tile sums.
 template <int PREC>
 __device__ __forceinline__ void scan_tile(const float* __restrict__ w, uint32_t base, uint32_t m,
-                                          bool full, ScanShared<PREC>& s, float (&v)[kItems],
+                                          ScanShared<PREC>& s, float (&v)[kItems],
                                           typename Acc1<PREC>::T& thr_excl,
                                           typename Acc1<PREC>::T& tile_agg) {
     using A1 = Acc1<PREC>;
@@ -89,27 +90,7 @@ __device__ __forceinline__ void scan_tile(const float* __restrict__ w, uint32_t 
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
-    if (full) {
-        const float4* w4 = reinterpret_cast<const float4*>(w + base);
-        float4 v4[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) v4[j] = ld_nt4(w4 + t + kBlock * j);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int e = 4 * (t + kBlock * j);
-            const int q = e + e / 16;
-            s.buf[q + 0] = v4[j].x;
-            s.buf[q + 1] = v4[j].y;
-            s.buf[q + 2] = v4[j].z;
-            s.buf[q + 3] = v4[j].w;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < kItems; ++j) {
-            const uint32_t e = (uint32_t)t + (uint32_t)kBlock * j;
-            s.buf[e + e / 16] = e < m ? w[base + e] : 0.0f;
-        }
-    }
+    load_tile_lds<true>(w + base, m, s.buf);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) v[k] = s.buf[17 * t + k];
@@ -238,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
 
     float v[kItems];
     R thr_excl, tile_agg;
-    scan_tile<PREC>(w, base, m, full, s, v, thr_excl, tile_agg);
+    scan_tile<PREC>(w, base, m, s, v, thr_excl, tile_agg);
 
     uint64_t* rec = states + (uint64_t)tile * kStateWords;
     if (wid == 0) {
@@ -391,7 +372,7 @@ struct SpineOps {
 };
 static_assert(kLimbs <= 7, "a look-back record half holds 7 payload words");
 
-template <int PREC, bool VEC>
+template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_scan_tilesum(const float* __restrict__ w, uint32_t n,
                                                          Params* __restrict__ prm,
                                                          uint64_t* __restrict__ states) {
@@ -409,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_tilesum(const float* __restrict
     if (tile == 0 && threadIdx.x == 0) prm->ticket = 0;
     float v[kItems];
     R thr_excl, tile_agg;
-    scan_tile<PREC>(w, base, m, VEC && m == (uint32_t)kTile, s, v, thr_excl, tile_agg);
+    scan_tile<PREC>(w, base, m, s, v, thr_excl, tile_agg);
     if (threadIdx.x == 0) {
         if constexpr (PREC == kFP32) states[tile] = (uint64_t)__float_as_uint(tile_agg);
         else states[tile] = (uint64_t)__double_as_longlong(Acc1<PREC>::value(tile_agg));
@@ -525,7 +506,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_emit(const float* __restrict__ 
     const float cf = PREC == kFP32 ? __uint_as_float((uint32_t)cw) : 0.0f;
     float v[kItems];
     R thr_excl, tile_agg;
-    scan_tile<PREC>(w, base, m, full, s, v, thr_excl, tile_agg);
+    scan_tile<PREC>(w, base, m, s, v, thr_excl, tile_agg);
     scan_tile_store<PREC>(base, m, full, v, thr_excl, c, cf, scale, s, hi_out, lo_out);
 }
 
@@ -535,7 +516,7 @@ void launch_rts_t(const float* w, uint32_t n, int normalise, Params* prm, uint64
                   float* hi, float* lo, hipStream_t s) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
     const uint32_t nblk = (tiles + kSpineTiles - 1) / kSpineTiles;
-    k_scan_tilesum<P, V><<<tiles, kBlock, 0, s>>>(w, n, prm, states);
+    k_scan_tilesum<P><<<tiles, kBlock, 0, s>>>(w, n, prm, states);
     k_scan_spine<P><<<nblk, kSpineThreads, 0, s>>>(states, tiles, n, prm);
     k_scan_emit<P, V><<<tiles, kBlock, 0, s>>>(w, n, normalise, prm, states, hi, lo);
 }
@@ -549,7 +530,8 @@ void launch_rts_p(bool vec, const float* w, uint32_t n, int normalise, Params* p
 
 hipError_t launch_scan_rts(const float* w, uint64_t n, int prec, int normalise, Params* prm,
                            uint64_t* states, float* hi, float* lo, hipStream_t s) {
-    const bool vec = ((reinterpret_cast<uintptr_t>(w) | reinterpret_cast<uintptr_t>(hi) |
+    // VEC: 16-byte aligned outputs (stored as float4 rows); inputs load as rows at any alignment
+    const bool vec = ((reinterpret_cast<uintptr_t>(hi) |
                        (lo ? reinterpret_cast<uintptr_t>(lo) : 0)) & 15u) == 0;
     const uint32_t nn = (uint32_t)n;
     switch (prec) {
@@ -564,7 +546,8 @@ hipError_t launch_scan_rts(const float* w, uint64_t n, int prec, int normalise, 
 hipError_t launch_scan(const float* w, uint64_t n, int prec, int normalise, Params* prm,
                        uint64_t* states, float* hi, float* lo, hipStream_t s) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
-    const bool vec = ((reinterpret_cast<uintptr_t>(w) | reinterpret_cast<uintptr_t>(hi) |
+    // VEC: 16-byte aligned outputs (stored as float4 rows); inputs load as rows at any alignment
+    const bool vec = ((reinterpret_cast<uintptr_t>(hi) |
                        (lo ? reinterpret_cast<uintptr_t>(lo) : 0)) & 15u) == 0;
     const uint32_t nn = (uint32_t)n;
 #define WRS_LAUNCH_SCAN(P, V) \

@@ -173,17 +173,19 @@ def test_special_cases(wrs, oracle):
     assert oracle.distribution_error(q, w)["zero_weight_mass"] == 0.0
 
 
-def test_unaligned_input_gives_same_table(wrs, oracle):
-    """A weights pointer that is not 16-byte aligned takes the scalar load path;
-    the table must be bit-identical."""
-    print_case("unaligned")
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_unaligned_input_gives_same_table(wrs, oracle, shift):
+    """A weights pointer that is not 16-byte aligned is loaded as the aligned
+    16-byte rows around it; the table must be bit-identical."""
+    print_case("unaligned", shift)
     n = 100003
     w = gen_weights(oracle, "lognormal", n)
-    plan = wrs.Plan(n + 1)
-    dw = wrs.to_device(np.concatenate([np.zeros(1, np.float32), w]))
+    plan = wrs.Plan(n + 3)
+    dw = wrs.to_device(np.concatenate([np.full(shift, 7.0, np.float32), w,
+                                       np.full(4, 9.0, np.float32)]))
     t1 = wrs.empty(n, wrs.ALIAS_DTYPE)
     t2 = wrs.empty(n, wrs.ALIAS_DTYPE)
-    plan.psa_build(dw.offset(1), n, t1)
+    plan.psa_build(dw.offset(shift), n, t1)
     dw2 = wrs.to_device(w)
     plan.psa_build(dw2, n, t2)
     wrs.synchronize()

@@ -177,6 +177,27 @@ def test_scan_inclusive(wrs, oracle, precision, n):
             assert np.array_equal(lo.numpy(), got_lo)
 
 
+@pytest.mark.parametrize("n", [64 * 4096 - 5, (1 << 20) + 9])
+@pytest.mark.parametrize("shift", [1, 3])
+def test_scan_unaligned_same_bits(wrs, oracle, n, shift):
+    """Input and outputs off 16-byte alignment: rows are loaded around the
+    input and stored element-wise; the prefixes are bit-identical."""
+    case("scan unaligned", n, shift)
+    w = oracle.gen_uniform01(4, n)
+    plan = wrs.Plan(n + 4)
+    pad = np.full(shift, 5.0, np.float32)
+    dw = wrs.to_device(np.concatenate([pad, w, np.full(4, 5.0, np.float32)]))
+    hi1 = wrs.empty(n + 4, np.float32)
+    lo1 = wrs.empty(n + 4, np.float32)
+    plan.scan_inclusive(dw.offset(shift), n, hi1.offset(shift), lo1.offset(shift))
+    hi2 = wrs.empty(n, np.float32)
+    lo2 = wrs.empty(n, np.float32)
+    plan.scan_inclusive(wrs.to_device(w), n, hi2, lo2)
+    wrs.synchronize()
+    assert np.array_equal(hi1.numpy()[shift:shift + n], hi2.numpy())
+    assert np.array_equal(lo1.numpy()[shift:shift + n], lo2.numpy())
+
+
 def test_scan_all_ones_exact(wrs):
     n = 1 << 24
     plan = wrs.Plan(n)
