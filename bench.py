@@ -120,6 +120,8 @@ def main() -> int:
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline build (no precision study, ITS, spike)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="default launch geometry instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path has no multi-GPU form here); "
@@ -137,7 +139,15 @@ def main() -> int:
     # ---- synthetic input, generated on the device --------------------------
     w = wrs.empty(n, np.float32)
     wrs.generate(wrs.GEN_ZIPF, n, 2, w, a=1.1, stream=st)
-    plan = wrs.Plan(n, args.split_size, args.precision)
+    st.synchronize()
+    # ---- launch geometry: the autotuner times every merge shape on these weights
+    tune, tune_log = None, []
+    if not args.no_tune:
+        tune, tune_log = wrs.autotune_psa(w, n, precision=args.precision,
+                                          split_size=args.split_size, stream=st)
+        log(f"autotune (PSA, N=2^{args.log2n}): {tune}; candidates (ms): "
+            + ", ".join(f"{e['merge_tile']}/{e['merge_per']}: {e['ms']:.4f}" for e in tune_log))
+    plan = wrs.Plan(n, args.split_size, args.precision, tune=tune)
     K = plan.split_count(n)
     table = wrs.empty(n, wrs.ALIAS_DTYPE)
     st.synchronize()
@@ -206,7 +216,7 @@ def main() -> int:
     if not args.no_extras:
         study = []
         for prec in range(4):
-            pp = plan if prec == args.precision else wrs.Plan(n, args.split_size, prec)
+            pp = plan if prec == args.precision else wrs.Plan(n, args.split_size, prec, tune=tune)
             tt = wrs.empty(n, wrs.ALIAS_DTYPE)
             for _ in range(2):
                 pp.psa_build(w, n, tt, stream=st)
@@ -275,7 +285,14 @@ def main() -> int:
         bw = wrs.empty(tot, np.float32)
         wrs.generate(wrs.GEN_LOGNORMAL, tot, 3, bw, a=0.0, b=3.0, stream=st)
         doff = wrs.to_device(off)
-        bp = wrs.BatchPlan(off, args.split_size, args.precision)
+        btune, btune_log = None, []
+        if not args.no_tune:
+            btune, btune_log = wrs.autotune_batched(off, bw, precision=args.precision,
+                                                    split_size=args.split_size, stream=st)
+            log(f"autotune (batched): {btune}; candidates (ms): " + ", ".join(
+                f"{e['seg_partition']}/{e['merge_tile']}/{e['merge_per']}: {e['ms']:.4f}"
+                for e in btune_log))
+        bp = wrs.BatchPlan(off, args.split_size, args.precision, tune=btune)
         tb = wrs.empty(tot, wrs.ALIAS_DTYPE)
         sb = wrs.empty(nseg * per, np.uint32)
         g = wrs.Graph(st)
@@ -285,7 +302,7 @@ def main() -> int:
         g.launch()
         st.synchronize()
         batched_ms = time_it(g.launch, st, wrs, 10)
-        eplan = wrs.Plan(int(sizes.max()), args.split_size, args.precision)
+        eplan = wrs.Plan(int(sizes.max()), args.split_size, args.precision, tune=btune)
         te = wrs.empty(tot, wrs.ALIAS_DTYPE)
         se = wrs.empty(nseg * per, np.uint32)
         wrs.build_sample_each(eplan, bw, off, te, per, 42, se, stream=st)
@@ -301,6 +318,9 @@ def main() -> int:
         batched = dict(distributions=nseg, weights=tot, samples_per_distribution=per,
                        batched_graph_ms=round(batched_ms, 4), per_distribution_ms=round(each_ms, 3),
                        speedup=round(each_ms / batched_ms, 2), identical_tables_and_samples=same,
+                       tuning=btune, tuning_candidates_ms=[
+                           [e["seg_partition"], e["merge_tile"], e["merge_per"], round(e["ms"], 4)]
+                           for e in btune_log],
                        batched_build_gbs=round((32 * tot + 12 * int(np.sum(-(-sizes // args.split_size))))
                                                / (batched_ms * 1e-3) / 1e9, 1))
         log(f"batched: {batched}")
@@ -345,6 +365,9 @@ def main() -> int:
                         f"{PREC_NAMES[args.precision]}",
             "n": n, "split_size": args.split_size, "B_build_bytes": B_build,
             "parallelism": "single GPU",
+            "tuning": tune,
+            "tuning_candidates_ms": [[e["merge_tile"], e["merge_per"], round(e["ms"], 4)]
+                                     for e in tune_log],
         },
         "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
         "lower_bound_12N_gbs": round(12 * n / (ms_step * 1e-3) / 1e9, 2),

@@ -58,11 +58,29 @@ typedef struct {
     double spill;
 } wrs_split;
 
+/* Launch geometry of a build; zero fields take the defaults.  Chosen by the
+ * autotuner below or by hand.  The table of a build can depend on merge_tile
+ * only through rounding of the block-boundary splits (both results are valid
+ * tables at the same error); builds with equal tunings are bit-identical.
+ * Reference: the PSAConfig knobs splitSize and the pack/split variants
+ * (src/device/wrs/alias/psa/PSA.hpp:19-41), and the split sizes its
+ * benchmarks sweep (src/bench/psa_split2.cpp:34-270). */
+typedef struct {
+    uint32_t merge_tile;    /* split granularity: buckets per merge-pack block,
+                               1024, 2048 (default) or 4096 */
+    uint32_t merge_per;     /* buckets per merge-pack thread: 2, 4 (default) or 8
+                               (merge_tile 4096: 4 or 8) */
+    uint32_t seg_partition; /* batched prefix-partition: 0 reduce-then-scan (default),
+                               1 single pass with per-segment decoupled look-back */
+    uint32_t reserved;      /* 0 */
+} wrs_tuning;
+
 /* Build configuration.  Reference: PSAConfig (src/device/wrs/alias/psa/PSA.hpp:19-41),
  * whose splitSize sets the split granularity. */
 typedef struct {
     uint32_t split_size;     /* split granularity of the debug split output (K = ceil(N/split_size)); 0 -> 32 */
     wrs_precision precision; /* default WRS_PREC_COMPENSATED */
+    wrs_tuning tuning;       /* all zero: default geometry */
 } wrs_config;
 
 typedef struct wrs_plan wrs_plan;
@@ -77,7 +95,7 @@ wrs_status wrs_plan_destroy(wrs_plan* plan);
 
 /* PSA alias-table build: weights[n] (finite, >= 0, not all zero) -> table[n].
  * Passes: FP64 total, heavy/light prefix-partition, split + pack as a parallel
- * merge (exact splits at every 2048 buckets, then an LDS merge per block).
+ * merge (exact splits at every merge_tile buckets, then an LDS merge per block).
  * Replaces device::PSA::run (src/device/wrs/alias/psa/PSA.hpp:170-237) and
  * device::WRS::build for an AliasTable config (src/device/wrs/WRS.hpp:121-143). */
 wrs_status wrs_psa_build(wrs_plan* plan, const float* weights, uint64_t n,
@@ -162,8 +180,9 @@ wrs_status wrs_generate(wrs_generator kind, double a, double b, uint64_t seed, u
  * range of `tables` with segment-local indices.  The plan copies the offsets
  * and derives its work lists from them.  Every segment gets exactly the
  * arithmetic of wrs_psa_build, so the tables are bit-identical to building
- * the segments one at a time.  No reference counterpart: the reference builds
- * one distribution per dispatch sequence. */
+ * the segments one at a time with a plan of the same merge_tile.  No
+ * reference counterpart: the reference builds one distribution per dispatch
+ * sequence. */
 typedef struct wrs_batch_plan wrs_batch_plan;
 wrs_status wrs_batch_plan_create(const wrs_config* cfg, const uint64_t* offsets_host,
                                  uint32_t nseg, wrs_batch_plan** out);
@@ -177,6 +196,31 @@ wrs_status wrs_psa_build_batched(wrs_batch_plan* plan, const float* weights,
 wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_t* offsets_dev,
                                     uint32_t nseg, uint32_t per_segment, uint64_t seed,
                                     uint64_t counter_base, uint32_t* out, wrs_stream_t stream);
+
+/* ---- autotuner ----
+ * Times candidate geometries (wrs_tuning) on the caller's own data with HIP
+ * events and returns the fastest: first the batched partition form with the
+ * default merge shape, then every supported merge shape with the better form
+ * (coordinate descent; each candidate the median of 5 builds after one
+ * warm-up).  Results are cached per process under a key of device, precision
+ * and problem shape (n, or the segment count and a histogram of log2 segment
+ * sizes); with the environment variable WRS_TUNE_CACHE naming a file they are
+ * also read from and appended to that file.  Synchronises `stream`;
+ * allocates its own scratch.  log (optional, log_cap entries) receives every
+ * candidate timed, *log_len their number (0 for a cache hit).  No reference
+ * counterpart: the reference fixes its launch shapes at pipeline creation. */
+typedef struct {
+    wrs_tuning tuning;
+    float ms; /* median build time */
+} wrs_tune_entry;
+#define WRS_TUNE_NO_CACHE 1u /* measure even if the cache holds the key (and update it) */
+wrs_status wrs_autotune_psa(const wrs_config* cfg, const float* weights, uint64_t n,
+                            uint32_t flags, wrs_tuning* best, wrs_tune_entry* log,
+                            uint32_t log_cap, uint32_t* log_len, wrs_stream_t stream);
+wrs_status wrs_autotune_batched(const wrs_config* cfg, const uint64_t* offsets_host,
+                                uint32_t nseg, const float* weights, uint32_t flags,
+                                wrs_tuning* best, wrs_tune_entry* log, uint32_t log_cap,
+                                uint32_t* log_len, wrs_stream_t stream);
 
 /* ---- introspection ---- */
 const char* wrs_last_error(void);         /* thread-local message of the last failure */

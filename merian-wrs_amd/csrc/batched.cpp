@@ -20,18 +20,43 @@ void upload(T* dst, const std::vector<T>& src) {
 }
 }  // namespace
 
-void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_host,
-                       uint32_t nseg_) {
+namespace {
+// merge work lists of a merge tile: (segment, block) items and the first
+// boundary of every segment
+void merge_lists(const std::vector<uint64_t>& offsets, uint32_t nseg, uint32_t tile,
+                 std::vector<uint32_t>& mseg, std::vector<uint32_t>& mblk,
+                 std::vector<uint32_t>& b0, uint64_t& nbound) {
+    mseg.clear();
+    mblk.clear();
+    b0.assign(nseg + 1, 0);
+    uint64_t bnd = 0;
+    for (uint32_t g = 0; g < nseg; ++g) {
+        const uint64_t blocks = wrsk::merge_blocks_for(offsets[g + 1] - offsets[g], tile);
+        b0[g] = (uint32_t)bnd;
+        for (uint64_t b = 0; b < blocks; ++b) {
+            mseg.push_back(g);
+            mblk.push_back((uint32_t)b);
+        }
+        bnd += blocks + 1;
+    }
+    b0[nseg] = (uint32_t)bnd;
+    nbound = bnd;
+}
+}  // namespace
+
+void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning_,
+                       const uint64_t* offsets_host, uint32_t nseg_) {
     split_size = split_size_ ? split_size_ : 32u;
     prec = prec_;
     nseg = nseg_;
     if (prec < wrsk::kFP32 || prec > wrsk::kFP64) throw wrs::Error(WRS_EINVAL, "unknown precision");
+    wrs::require_tuning(tuning_);
     offsets.assign(offsets_host, offsets_host + nseg + 1);
     if (offsets[0] != 0) throw wrs::Error(WRS_EINVAL, "offsets[0] must be 0");
-    std::vector<uint32_t> h_item_seg, h_item_tile, h_mitem_seg, h_mitem_blk;
-    std::vector<uint32_t> h_seg_item0(nseg + 1), h_seg_b0(nseg + 1);
+    std::vector<uint32_t> h_item_seg, h_item_tile;
+    std::vector<uint32_t> h_seg_item0(nseg + 1);
     std::vector<uint64_t> h_seg_samp0(nseg + 1);
-    uint64_t items64 = 0, bnd = 0, samp = 0;
+    uint64_t items64 = 0, samp = 0;
     for (uint32_t g = 0; g < nseg; ++g) {
         if (offsets[g + 1] <= offsets[g])
             throw wrs::Error(WRS_EINVAL, "segments must be non-empty and offsets increasing");
@@ -45,23 +70,19 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_
             h_item_tile.push_back((uint32_t)t);
         }
         items64 += tiles;
-        const uint64_t blocks = wrsk::merge_blocks_for(n);
-        h_seg_b0[g] = (uint32_t)bnd;
-        for (uint64_t b = 0; b < blocks; ++b) {
-            h_mitem_seg.push_back(g);
-            h_mitem_blk.push_back((uint32_t)b);
-        }
-        bnd += blocks + 1;
         h_seg_samp0[g] = samp;
         samp += wrsk::samples_for(n);
     }
-    if (items64 >= (1ull << 31) || bnd >= (1ull << 31)) throw wrs::Error(WRS_EINVAL, "too many tiles");
+    // capacity of the merge lists: the smallest merge tile has the most blocks
+    std::vector<uint32_t> ms_, mb_, b0_;
+    uint64_t bnd_cap = 0;
+    merge_lists(offsets, nseg, wrsk::kMinMergeTile, ms_, mb_, b0_, bnd_cap);
+    if (items64 >= (1ull << 31) || bnd_cap >= (1ull << 31)) throw wrs::Error(WRS_EINVAL, "too many tiles");
     h_seg_item0[nseg] = (uint32_t)items64;
-    h_seg_b0[nseg] = (uint32_t)bnd;
     h_seg_samp0[nseg] = samp;
     items = (uint32_t)items64;
-    mitems = (uint32_t)h_mitem_seg.size();
-    nbound = (uint32_t)bnd;
+    mitems_cap = (uint32_t)ms_.size();
+    nbound_cap = (uint32_t)bnd_cap;
     total_n = offsets[nseg];
 
     seg_off = dalloc<uint64_t>(nseg + 1);
@@ -70,8 +91,8 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_
     seg_samp0 = dalloc<uint64_t>(nseg + 1);
     item_seg = dalloc<uint32_t>(items);
     item_tile = dalloc<uint32_t>(items);
-    mitem_seg = dalloc<uint32_t>(mitems);
-    mitem_blk = dalloc<uint32_t>(mitems);
+    mitem_seg = dalloc<uint32_t>(mitems_cap);
+    mitem_blk = dalloc<uint32_t>(mitems_cap);
     seg_prm = dalloc<wrsk::Params>(nseg);
     partials = dalloc<double>(items);
     states = dalloc<uint64_t>((uint64_t)items * wrsk::kStateWords);
@@ -80,15 +101,31 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const uint64_t* offsets_
     pw = dalloc<float>(total_n);
     samp_h = dalloc<double>(samp);
     samp_l = dalloc<double>(samp);
-    bounds = dalloc<wrsk::Coarse>(nbound);
+    bounds = dalloc<wrsk::Coarse>(nbound_cap);
     upload(seg_off, offsets);
     upload(seg_item0, h_seg_item0);
-    upload(seg_b0, h_seg_b0);
     upload(seg_samp0, h_seg_samp0);
     upload(item_seg, h_item_seg);
     upload(item_tile, h_item_tile);
-    upload(mitem_seg, h_mitem_seg);
-    upload(mitem_blk, h_mitem_blk);
+    shape.tile = 0;  // forces the merge lists of the first tuning
+    set_tuning(tuning_);
+}
+
+void BatchPlan::set_tuning(const wrs_tuning& t) {
+    wrs::require_tuning(t);
+    const wrsk::MergeShape ms = wrs::merge_shape_of(t);
+    if (ms.tile != shape.tile) {
+        std::vector<uint32_t> h_mseg, h_mblk, h_b0;
+        uint64_t bnd = 0;
+        merge_lists(offsets, nseg, ms.tile, h_mseg, h_mblk, h_b0, bnd);
+        upload(mitem_seg, h_mseg);
+        upload(mitem_blk, h_mblk);
+        upload(seg_b0, h_b0);
+        mitems = (uint32_t)h_mseg.size();
+        nbound = (uint32_t)bnd;
+    }
+    tuning = t;
+    shape = ms;
 }
 
 void BatchPlan::destroy() {
@@ -124,12 +161,13 @@ void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) c
                                              ticket, s),
                    "segmented finalize launch");
     wrs::check_hip(wrsk::launch_seg_partition(w, seg_off, item_seg, item_tile, seg_item0, nseg,
-                                              items, prec, lookback ? 1 : 0, seg_prm, ticket,
-                                              states, idx, pw, samp_h, samp_l, seg_samp0, s),
+                                              items, prec, (int)tuning.seg_partition, seg_prm,
+                                              ticket, states, idx, pw, samp_h, samp_l, seg_samp0,
+                                              s),
                    "segmented partition launch");
     wrs::check_hip(wrsk::launch_seg_mergepack(pw, idx, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
-                                              nseg, nbound, mitem_seg, mitem_blk, mitems, prec,
-                                              seg_prm, bounds, tables, s),
+                                              nseg, nbound, mitem_seg, mitem_blk, mitems, shape,
+                                              prec, seg_prm, bounds, tables, s),
                    "segmented merge pack launch");
 }
 

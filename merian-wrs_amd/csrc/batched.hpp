@@ -1,10 +1,11 @@
 // batched.hpp -- host side of the segmented (batched) PSA build: thousands of
 // independent distributions of uneven sizes, laid out back to back (CSR
-// offsets), built by four segmented launches whose work lists (segment, tile)
-// and (segment, merge block) are derived once from the offsets when the plan
-// is created.  Each segment is built with exactly the arithmetic and order of a
+// offsets), built by segmented launches whose work lists (segment, tile) and
+// (segment, merge block) are derived from the offsets when the plan is created
+// (the merge lists again when the tuning changes the merge tile).  Each
+// segment is built with exactly the arithmetic and order of a
 // single-distribution build, so the tables are bit-identical to building the
-// segments one by one.
+// segments one by one with the same merge tile.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -12,6 +13,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "../../include/wrs.h"
 #include "internal.hpp"
 
 namespace wrsb {
@@ -19,7 +21,8 @@ namespace wrsb {
 struct BatchPlan {
     uint32_t split_size = 32;
     int prec = 2;
-    bool lookback = false;       // segmented partition: single pass with look-back, else reduce-then-scan
+    wrs_tuning tuning{};         // seg_partition: 1 single pass with look-back, 0 reduce-then-scan
+    wrsk::MergeShape shape;      // merge shape of the tuning
     uint32_t nseg = 0;
     uint32_t items = 0;          // (segment, tile) work items of reduce and partition
     uint32_t mitems = 0;         // (segment, merge block) work items of the merge pack
@@ -43,9 +46,14 @@ struct BatchPlan {
     float* pw = nullptr;             // [total_n] weights in partition order
     double* samp_h = nullptr;        // [seg_samp0[nseg]]
     double* samp_l = nullptr;        // [seg_samp0[nseg]]
-    wrsk::Coarse* bounds = nullptr;  // [nbound]
+    wrsk::Coarse* bounds = nullptr;  // [nbound at the smallest merge tile]
+    uint32_t mitems_cap = 0, nbound_cap = 0;
 
-    void create(uint32_t split_size, int prec, const uint64_t* offsets_host, uint32_t nseg);
+    void create(uint32_t split_size, int prec, const wrs_tuning& tuning,
+                const uint64_t* offsets_host, uint32_t nseg);
+    // changes the launch geometry; rebuilds the merge work lists when the
+    // merge tile changes (synchronous uploads: not while a build is in flight)
+    void set_tuning(const wrs_tuning& tuning);
     void destroy();
     void build(const float* weights, wrsk::AliasEntry* tables, hipStream_t s) const;
 };

@@ -62,11 +62,18 @@ constexpr int kStateWords = 16;             // 128-byte look-back record per til
 constexpr uint32_t kMaxPartials = 2048;     // reduce blocks
 constexpr int kSplitBlock = 256;            // debug split kernel block size
 constexpr uint32_t kSample = 64;            // partition prefix sample stride (ranks)
-constexpr int kMergeTile = 2048;            // buckets per merge-pack block (multiple of kSample)
-constexpr int kMergeBlock = 512;            // merge-pack threads per block (4 buckets each)
+// Merge-pack geometry: `tile` buckets per block (a multiple of kSample),
+// `per` buckets per thread (tile / per threads).  Supported shapes:
+// tile 1024 or 2048 with per 2, 4 or 8; tile 4096 with per 4 or 8.
+struct MergeShape {
+    uint32_t tile = 2048;
+    uint32_t per = 4;
+};
+constexpr uint32_t kMinMergeTile = 1024;
+bool merge_shape_supported(MergeShape ms);
 
 inline uint64_t samples_for(uint64_t n) { return n / kSample + 1; }
-inline uint64_t merge_blocks_for(uint64_t n) { return (n + kMergeTile - 1) / kMergeTile; }
+inline uint64_t merge_blocks_for(uint64_t n, uint32_t tile) { return (n + tile - 1) / tile; }
 
 inline uint64_t tiles_for(uint64_t n) { return (n + kTile - 1) / kTile; }
 // tile aggregates scanned per block by the partition's spine scan
@@ -74,17 +81,17 @@ constexpr uint32_t kSpineTiles = 1024;
 inline uint64_t spine_blocks_for(uint64_t n) { return (tiles_for(n) + kSpineTiles - 1) / kSpineTiles; }
 // boundaries per block of the boundary repair
 constexpr uint32_t kRepairSpan = 1024;
-inline uint64_t repair_blocks_for(uint64_t n) {
-    return (merge_blocks_for(n) + kRepairSpan - 1) / kRepairSpan;
+inline uint64_t repair_blocks_for(uint64_t n, uint32_t tile) {
+    return (merge_blocks_for(n, tile) + kRepairSpan - 1) / kRepairSpan;
 }
 // state buffer of a single-distribution build: one record per tile, one per
-// spine block, then two per repair block (one per max-scan pass); all reset
-// to the sentinel by the reduce pass
+// spine block, then two per repair block (one per max-scan pass, sized for
+// the smallest merge tile); all reset to the sentinel by the reduce pass
 inline uint64_t repair_records_offset(uint64_t n) {
     return (tiles_for(n) + spine_blocks_for(n)) * kStateWords;
 }
 inline uint64_t state_words_for(uint64_t n) {
-    return repair_records_offset(n) + 2 * repair_blocks_for(n) * kStateWords;
+    return repair_records_offset(n) + 2 * repair_blocks_for(n, kMinMergeTile) * kStateWords;
 }
 inline uint32_t partials_for(uint64_t n) {
     const uint64_t t = tiles_for(n);
@@ -104,17 +111,17 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
                             double* samp_l, hipStream_t s, hipEvent_t ev_agg = nullptr,
                             hipEvent_t ev_scan = nullptr);
 // exact splits at every kMergeTile buckets, then the LDS merge-path pack
-hipError_t launch_coarse(const float* pw, uint64_t n, int prec, const Params* prm,
+hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, const Params* prm,
                          const double* samp_h, const double* samp_l, Coarse* bounds,
                          hipStream_t s);
 // makes the boundary splits monotone (no-op for exact keys); states are the
 // build's look-back records (state_words_for(n) words, reset by the reduce)
-hipError_t launch_repair_bounds(Coarse* bounds, uint64_t n, Params* prm, uint64_t* states,
-                                const float* pw, const double* samp_h, const double* samp_l,
-                                hipStream_t s);
-hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params* prm,
-                            const uint32_t* idx, const Coarse* bounds, AliasEntry* table,
-                            hipStream_t s);
+hipError_t launch_repair_bounds(Coarse* bounds, uint64_t n, MergeShape ms, Params* prm,
+                                uint64_t* states, const float* pw, const double* samp_h,
+                                const double* samp_l, hipStream_t s);
+hipError_t launch_mergepack(const float* pw, uint64_t n, MergeShape ms, int prec,
+                            const Params* prm, const uint32_t* idx, const Coarse* bounds,
+                            AliasEntry* table, hipStream_t s);
 // the K split points S(k), k = 1..K, of the reference's splitK layout (debug output)
 hipError_t launch_splits(uint64_t n, uint32_t K, int prec, const Params* prm, const float* hi,
                          const float* lo, SplitRec* splits, hipStream_t s);
@@ -159,8 +166,8 @@ hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const doub
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
                                 uint32_t nbound, const uint32_t* mitem_seg,
-                                const uint32_t* mitem_blk, uint32_t mitems, int prec,
-                                const Params* seg_prm, Coarse* bounds, AliasEntry* table,
+                                const uint32_t* mitem_blk, uint32_t mitems, MergeShape ms,
+                                int prec, const Params* seg_prm, Coarse* bounds, AliasEntry* table,
                                 hipStream_t s);
 hipError_t launch_seg_sample(const AliasEntry* tables, const uint64_t* seg_off, uint32_t nseg,
                              uint32_t per_segment, uint64_t seed, uint64_t counter_base,

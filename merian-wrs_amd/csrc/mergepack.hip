@@ -17,7 +17,8 @@
 // exactly the merge-path diagonal of that merge.  This file therefore builds
 // the table as a two-level merge:
 //
-//  k_coarse   one wave per block boundary n = b*kMergeTile: the exact split
+//  k_coarse   one wave per block boundary n = b*tile (tile = buckets per
+//             merge block, a multiple of kSample): the exact split
 //             (i, j) with LP(i), HP(j).  A 64-ary wave search over the sparse
 //             FP64 prefix samples the partition wrote (every kSample-th rank;
 //             at j = m*kSample the light rank n - j is also a multiple of
@@ -27,7 +28,8 @@
 //             pw, the weights in partition order written by the partition
 //             next to the indices, so every access here is contiguous.
 //  k_repair_bounds_lb  makes the splits monotone where rounding broke it.
-//  k_mergepack one 512-thread block per kMergeTile buckets: the block's light
+//  k_mergepack one block per `tile` buckets (tile / per threads, `per`
+//             buckets each; the shape is a plan parameter): the block's light
 //             and heavy index ranges come from its two boundary splits; their
 //             indices and weights (both contiguous runs of idx and pw, loaded
 //             in one round trip) are staged in LDS, the keys X and Y
@@ -175,11 +177,11 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 const double* __restrict__ samp_h,
                                                 const double* __restrict__ samp_l, uint32_t N,
                                                 const Params* __restrict__ prm, uint32_t nbound,
-                                                Coarse* __restrict__ out) {
+                                                uint32_t tile, Coarse* __restrict__ out) {
     using T = typename RealT<PREC>::T;
     const uint32_t b = blockIdx.x * 4u + threadIdx.x / kWave;
     if (b >= nbound) return;  // whole wave exits together
-    const uint32_t n = min(b * (uint32_t)kMergeTile, N);
+    const uint32_t n = min(b * tile, N);
     const Coarse c = coarse_split<PREC>(pw, samp_h, samp_l, N, prm->heavy_count,
                                         (T)prm->mean, (T)prm->light_sum, n);
     if (lane_id() == 0) out[b] = c;
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     const float* __restrict__ pw, const double* __restrict__ samp_h,
     const double* __restrict__ samp_l, const uint64_t* __restrict__ seg_off,
     const uint64_t* __restrict__ seg_samp0, const uint32_t* __restrict__ seg_b0, uint32_t nseg,
-    const Params* __restrict__ seg_prm, Coarse* __restrict__ out) {
+    const Params* __restrict__ seg_prm, uint32_t tile, Coarse* __restrict__ out) {
     using T = typename RealT<PREC>::T;
     const uint32_t p = blockIdx.x * 4u + threadIdx.x / kWave;
     if (p >= seg_b0[nseg]) return;
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     const uint32_t g = a;
     const uint64_t off = seg_off[g];
     const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
-    const uint32_t n = min((p - seg_b0[g]) * (uint32_t)kMergeTile, N);
+    const uint32_t n = min((p - seg_b0[g]) * tile, N);
     const uint64_t so = seg_samp0[g];
     const Coarse c = coarse_split<PREC>(pw + off, samp_h + so, samp_l + so, N,
                                         seg_prm[g].heavy_count, (T)seg_prm[g].mean,
@@ -217,7 +219,8 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
 // so that every local key and offset stays below 2^62 in magnitude.
 __device__ __forceinline__ int64_t quant(double x, int qs) { return __double2ll_rn(ldexp(x, qs)); }
 
-// One block of kMergeTile buckets [n0, n1) between boundary splits c0 and c1.
+// One block of TILE buckets [n0, n1) between boundary splits c0 and c1, BLOCK
+// threads, TILE / BLOCK buckets per thread.
 //
 // Local keys are exact 64-bit fixed-point running sums of non-negative
 // increments (w_q = quant(w), mean_q = quant(mean)):
@@ -228,18 +231,19 @@ __device__ __forceinline__ int64_t quant(double x, int qs) { return __double2ll_
 // offset D = X(i0) - Y(j0) = n0*mean - LP(i0) - HP(j0) (FP64), heavy q goes
 // before light r iff Y(j0+q+1) <= X(i0+r), i.e. b_q - quant(D) <= a_r, and its
 // residual is Y(j0+q+1) - X(i0+r) + mean = (b_q - a_r) 2^-qs - D + mean.
-template <int PREC>
+template <int PREC, int TILE, int BLOCK>
 __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
                                                 const uint32_t* __restrict__ idx, uint32_t N,
                                                 uint32_t H, double mean, double total,
                                                 uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
                                                 AliasEntry* __restrict__ table) {
-    constexpr int kPer = kMergeTile / kMergeBlock;  // buckets (and staged elements) per thread
+    constexpr int kPer = TILE / BLOCK;  // buckets (and staged elements) per thread
+    static_assert(TILE % BLOCK == 0 && BLOCK % kWave == 0 && TILE % kSample == 0, "merge shape");
     // keys during the merge, entries afterwards (8 bytes per slot either way)
-    __shared__ uint2 s_raw[kMergeTile + 2];
-    __shared__ uint32_t s_idx[kMergeTile + 2];
-    __shared__ float s_w[kMergeTile];
-    __shared__ int64_t s_wave_l[kMergeBlock / kWave], s_wave_h[kMergeBlock / kWave];
+    __shared__ uint2 s_raw[TILE + 2];
+    __shared__ uint32_t s_idx[TILE + 2];
+    __shared__ float s_w[TILE];
+    __shared__ int64_t s_wave_l[BLOCK / kWave], s_wave_h[BLOCK / kWave];
     int64_t* s_key = reinterpret_cast<int64_t*>(s_raw);
 
     const int t = threadIdx.x;
@@ -247,7 +251,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     const int wid = t / kWave;
     const uint32_t L = N - H;
     if (H == 0 || L == 0) {  // all weights equal the mean up to its rounding: identity
-        for (uint32_t x = n0 + t; x < n1; x += kMergeBlock) table[x] = AliasEntry{1.0f, x};
+        for (uint32_t x = n0 + t; x < n1; x += BLOCK) table[x] = AliasEntry{1.0f, x};
         return;
     }
     const bool last = n1 == N;
@@ -264,14 +268,14 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
         float gw[kPer + 1];
 #pragma unroll
         for (int k = 0; k <= kPer; ++k) {
-            const uint32_t e = (uint32_t)t + (uint32_t)kMergeBlock * k;
+            const uint32_t e = (uint32_t)t + (uint32_t)BLOCK * k;
             const uint32_t pos = e < nl ? N - 1u - (i0 + e) : j0 + (e - nl);
             gi[k] = e <= ne ? idx[pos] : 0u;
             gw[k] = e < ne ? pw[pos] : 0.0f;
         }
 #pragma unroll
         for (int k = 0; k <= kPer; ++k) {
-            const uint32_t e = (uint32_t)t + (uint32_t)kMergeBlock * k;
+            const uint32_t e = (uint32_t)t + (uint32_t)BLOCK * k;
             if (e <= ne) s_idx[e] = gi[k];
             if (e < ne) s_w[e] = gw[k];
         }
@@ -313,7 +317,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     }
     __syncthreads();
     int64_t tot_l = 0;
-    for (int q = 0; q < kMergeBlock / kWave; ++q) {
+    for (int q = 0; q < BLOCK / kWave; ++q) {
         if (q < wid) {
             rl += s_wave_l[q];
             rh += s_wave_h[q];
@@ -400,28 +404,28 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
         if (k < cnt) s_raw[slot[k]] = ent[k];
     __syncthreads();
     uint2* tab = reinterpret_cast<uint2*>(table);
-    for (uint32_t e = t; e < ne; e += kMergeBlock) tab[s_idx[e]] = s_raw[e];
+    for (uint32_t e = t; e < ne; e += BLOCK) tab[s_idx[e]] = s_raw[e];
     if (last && t == 0) {
         const uint32_t h = s_idx[nl + nh];  // heavy H-1
         table[h] = AliasEntry{1.0f, h};
     }
 }
 
-template <int PREC>
-__global__ __launch_bounds__(kMergeBlock) void k_mergepack(const float* __restrict__ pw,
-                                                      const uint32_t* __restrict__ idx, uint32_t N,
-                                                      const Params* __restrict__ prm,
-                                                      const Coarse* __restrict__ bounds,
-                                                      AliasEntry* __restrict__ table) {
+template <int PREC, int TILE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_mergepack(const float* __restrict__ pw,
+                                                     const uint32_t* __restrict__ idx, uint32_t N,
+                                                     const Params* __restrict__ prm,
+                                                     const Coarse* __restrict__ bounds,
+                                                     AliasEntry* __restrict__ table) {
     const uint32_t b = blockIdx.x;
-    const uint32_t n0 = b * (uint32_t)kMergeTile;
-    const uint32_t n1 = min(n0 + (uint32_t)kMergeTile, N);
-    mergepack_block<PREC>(pw, idx, N, prm->heavy_count, prm->mean, prm->total, n0, n1, bounds[b],
+    const uint32_t n0 = b * (uint32_t)TILE;
+    const uint32_t n1 = min(n0 + (uint32_t)TILE, N);
+    mergepack_block<PREC, TILE, BLOCK>(pw, idx, N, prm->heavy_count, prm->mean, prm->total, n0, n1, bounds[b],
                           bounds[b + 1], table);
 }
 
-template <int PREC>
-__global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
+template <int PREC, int TILE, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_seg_mergepack(
     const float* __restrict__ pw, const uint32_t* __restrict__ idx,
     const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ mitem_seg,
     const uint32_t* __restrict__ mitem_blk, const uint32_t* __restrict__ seg_b0,
@@ -431,10 +435,10 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
     const uint32_t blk = mitem_blk[blockIdx.x];
     const uint64_t off = seg_off[g];
     const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
-    const uint32_t n0 = blk * (uint32_t)kMergeTile;
-    const uint32_t n1 = min(n0 + (uint32_t)kMergeTile, N);
+    const uint32_t n0 = blk * (uint32_t)TILE;
+    const uint32_t n1 = min(n0 + (uint32_t)TILE, N);
     const Coarse* bd = bounds + seg_b0[g] + blk;
-    mergepack_block<PREC>(pw + off, idx + off, N, seg_prm[g].heavy_count, seg_prm[g].mean,
+    mergepack_block<PREC, TILE, BLOCK>(pw + off, idx + off, N, seg_prm[g].heavy_count, seg_prm[g].mean,
                           seg_prm[g].total, n0, n1, bd[0], bd[1], table + off);
 }
 
@@ -451,8 +455,8 @@ __global__ __launch_bounds__(kMergeBlock) void k_seg_mergepack(
 // (j'' in [max(0, m_b - L), min(m_b, H - 1)]).  Exact keys are monotone
 // already and then nothing changes.  The last boundary (m = N) is fixed.
 // segmented form: one wave per segment (a batch holds thousands of short
-// segments; a segment has at most 2^23 / kMergeTile + 1 boundaries, i.e. at
-// most 17 rounds of kRepairThreads * kRepairPer)
+// segments; a segment has at most 2^23 / kMinMergeTile + 1 boundaries, i.e. at
+// most 33 rounds of kRepairThreads * kRepairPer)
 constexpr int kRepairThreads = 64;
 constexpr int kRepairPer = 4;
 
@@ -497,7 +501,7 @@ __global__ __launch_bounds__(kRepairThreads) void k_repair_bounds(
     Coarse* __restrict__ bounds, const uint32_t* __restrict__ seg_b0,
     const uint64_t* __restrict__ seg_off, const uint64_t* __restrict__ seg_samp0,
     const Params* __restrict__ seg_prm, const float* __restrict__ pw,
-    const double* __restrict__ samp_h, const double* __restrict__ samp_l) {
+    const double* __restrict__ samp_h, const double* __restrict__ samp_l, uint32_t tile) {
     __shared__ uint32_t s_w[kRepairThreads / kWave];
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
@@ -519,7 +523,7 @@ __global__ __launch_bounds__(kRepairThreads) void k_repair_bounds(
             for (int k = 0; k < kRepairPer; ++k) {
                 const uint32_t b = base + k;
                 v[k] = 0;
-                if (b < M) v[k] = pass == 0 ? bd[b].j : b * (uint32_t)kMergeTile - bd[b].j;
+                if (b < M) v[k] = pass == 0 ? bd[b].j : b * tile - bd[b].j;
                 m = m < v[k] ? v[k] : m;
             }
             const uint32_t incl = wave_incl_max_u(m);
@@ -544,7 +548,7 @@ __global__ __launch_bounds__(kRepairThreads) void k_repair_bounds(
                     } else {
                         Coarse c = bd[b];
                         c.i = run;
-                        c.j = b * (uint32_t)kMergeTile - run;
+                        c.j = b * tile - run;
                         if (c.j != (uint32_t)c.pad)
                             repair_anchor(c, pw + off, samp_h + seg_samp0[g], samp_l + seg_samp0[g],
                                           N, seg_prm[g].heavy_count, seg_prm[g].light_sum);
@@ -631,7 +635,7 @@ __device__ __forceinline__ void repair_lookback(uint64_t* recs, uint32_t blk, ui
 __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
     Coarse* __restrict__ bounds, uint32_t nbound, uint32_t N, Params* __restrict__ prm,
     uint64_t* __restrict__ recs, const float* __restrict__ pw, const double* __restrict__ samp_h,
-    const double* __restrict__ samp_l) {
+    const double* __restrict__ samp_l, uint32_t tile) {
     __shared__ uint32_t s_w[kRepairLbThreads / kWave];
     __shared__ uint32_t s_blk, s_ex;
     if (threadIdx.x == 0) s_blk = atomicAdd(&prm->repair_ticket, 1u);  // predecessors first
@@ -652,7 +656,7 @@ __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
 #pragma unroll
     for (int k = 0; k < kRepairLbPer; ++k) {
         const uint32_t jp = v[k] < ex ? ex : v[k];
-        v[k] = base + k < M ? (base + k) * (uint32_t)kMergeTile - jp : 0u;
+        v[k] = base + k < M ? (base + k) * tile - jp : 0u;
     }
     agg = block_incl_max(v, s_w);
     repair_lookback(recs + (uint64_t)nrb * kStateWords, blk, agg, &s_ex);
@@ -663,7 +667,7 @@ __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
         const uint32_t b = base + k;
         if (b < M) {
             const uint32_t i = v[k] < ex ? ex : v[k];
-            const uint32_t j = b * (uint32_t)kMergeTile - i;
+            const uint32_t j = b * tile - i;
             if (j != raw_j[k]) {
                 Coarse c = bounds[b];
                 c.i = i;
@@ -675,62 +679,90 @@ __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
     }
 }
 
-hipError_t launch_repair_bounds(Coarse* bounds, uint64_t n, Params* prm, uint64_t* states,
-                                const float* pw, const double* samp_h, const double* samp_l,
-                                hipStream_t s) {
-    const uint32_t nbound = (uint32_t)merge_blocks_for(n) + 1u;
-    k_repair_bounds_lb<<<(uint32_t)repair_blocks_for(n), kRepairLbThreads, 0, s>>>(
-        bounds, nbound, (uint32_t)n, prm, states + repair_records_offset(n), pw, samp_h, samp_l);
+hipError_t launch_repair_bounds(Coarse* bounds, uint64_t n, MergeShape ms, Params* prm,
+                                uint64_t* states, const float* pw, const double* samp_h,
+                                const double* samp_l, hipStream_t s) {
+    const uint32_t nbound = (uint32_t)merge_blocks_for(n, ms.tile) + 1u;
+    k_repair_bounds_lb<<<(uint32_t)repair_blocks_for(n, ms.tile), kRepairLbThreads, 0, s>>>(
+        bounds, nbound, (uint32_t)n, prm, states + repair_records_offset(n), pw, samp_h, samp_l,
+        ms.tile);
     return hipGetLastError();
 }
 
-hipError_t launch_coarse(const float* pw, uint64_t n, int prec, const Params* prm,
+hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, const Params* prm,
                          const double* samp_h, const double* samp_l, Coarse* bounds,
                          hipStream_t s) {
     const uint32_t N = (uint32_t)n;
-    const uint32_t nbound = (uint32_t)merge_blocks_for(n) + 1u;
+    const uint32_t nbound = (uint32_t)merge_blocks_for(n, ms.tile) + 1u;
     const uint32_t cgrid = (nbound + 3u) / 4u;
     // the split arithmetic is FP32 in FP32 mode and FP64 in all others
     if (prec == kFP32)
-        k_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
+        k_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, ms.tile, bounds);
     else
-        k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, bounds);
+        k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, ms.tile,
+                                                   bounds);
     return hipGetLastError();
 }
 
-hipError_t launch_mergepack(const float* pw, uint64_t n, int prec, const Params* prm,
-                            const uint32_t* idx, const Coarse* bounds, AliasEntry* table,
-                            hipStream_t s) {
+// The merge shapes instantiated: (buckets per block, buckets per thread).
+#define WRS_MERGE_SHAPES(X) \
+    X(1024, 2) X(1024, 4) X(1024, 8) X(2048, 2) X(2048, 4) X(2048, 8) X(4096, 4) X(4096, 8)
+
+bool merge_shape_supported(MergeShape ms) {
+#define WRS_SHAPE_OK(T, P) if (ms.tile == (T) && ms.per == (P)) return true;
+    WRS_MERGE_SHAPES(WRS_SHAPE_OK)
+#undef WRS_SHAPE_OK
+    return false;
+}
+
+hipError_t launch_mergepack(const float* pw, uint64_t n, MergeShape ms, int prec,
+                            const Params* prm, const uint32_t* idx, const Coarse* bounds,
+                            AliasEntry* table, hipStream_t s) {
     const uint32_t N = (uint32_t)n;
-    const uint32_t blocks = (uint32_t)merge_blocks_for(n);
+    const uint32_t blocks = (uint32_t)merge_blocks_for(n, ms.tile);
     // the merge keys are exact in every mode; only FP32 mode's p differs
-    if (prec == kFP32)
-        k_mergepack<kFP32><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
-    else
-        k_mergepack<kFP64Carry><<<blocks, kMergeBlock, 0, s>>>(pw, idx, N, prm, bounds, table);
-    return hipGetLastError();
+#define WRS_MP(T, P)                                                                              \
+    if (ms.tile == (T) && ms.per == (P)) {                                                        \
+        if (prec == kFP32)                                                                        \
+            k_mergepack<kFP32, T, T / P><<<blocks, T / P, 0, s>>>(pw, idx, N, prm, bounds, table); \
+        else                                                                                      \
+            k_mergepack<kFP64Carry, T, T / P><<<blocks, T / P, 0, s>>>(pw, idx, N, prm, bounds,    \
+                                                                       table);                    \
+        return hipGetLastError();                                                                 \
+    }
+    WRS_MERGE_SHAPES(WRS_MP)
+#undef WRS_MP
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const double* samp_h,
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
                                 uint32_t nbound, const uint32_t* mitem_seg,
-                                const uint32_t* mitem_blk, uint32_t mitems, int prec,
-                                const Params* seg_prm, Coarse* bounds, AliasEntry* table,
+                                const uint32_t* mitem_blk, uint32_t mitems, MergeShape ms,
+                                int prec, const Params* seg_prm, Coarse* bounds, AliasEntry* table,
                                 hipStream_t s) {
+    if (!merge_shape_supported(ms)) return hipErrorInvalidValue;
     const uint32_t cgrid = (nbound + 3u) / 4u;
-#define WRS_LAUNCH_SMP(P)                                                                       \
-    do {                                                                                        \
-        k_seg_coarse<P><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0,          \
-                                              seg_b0, nseg, seg_prm, bounds);                   \
-        k_repair_bounds<<<nseg, kRepairThreads, 0, s>>>(bounds, seg_b0, seg_off, seg_samp0,     \
-                                                        seg_prm, pw, samp_h, samp_l);            \
-        k_seg_mergepack<P><<<mitems, kMergeBlock, 0, s>>>(pw, idx, seg_off, mitem_seg, mitem_blk,    \
-                                                     seg_b0, seg_prm, bounds, table);           \
-    } while (0)
-    if (prec == kFP32) WRS_LAUNCH_SMP(kFP32);
-    else WRS_LAUNCH_SMP(kFP64Carry);
-#undef WRS_LAUNCH_SMP
+    if (prec == kFP32)
+        k_seg_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
+                                                  nseg, seg_prm, ms.tile, bounds);
+    else
+        k_seg_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0,
+                                                       seg_b0, nseg, seg_prm, ms.tile, bounds);
+    k_repair_bounds<<<nseg, kRepairThreads, 0, s>>>(bounds, seg_b0, seg_off, seg_samp0, seg_prm,
+                                                    pw, samp_h, samp_l, ms.tile);
+#define WRS_SMP(T, P)                                                                           \
+    if (ms.tile == (T) && ms.per == (P)) {                                                      \
+        if (prec == kFP32)                                                                      \
+            k_seg_mergepack<kFP32, T, T / P><<<mitems, T / P, 0, s>>>(                          \
+                pw, idx, seg_off, mitem_seg, mitem_blk, seg_b0, seg_prm, bounds, table);        \
+        else                                                                                    \
+            k_seg_mergepack<kFP64Carry, T, T / P><<<mitems, T / P, 0, s>>>(                     \
+                pw, idx, seg_off, mitem_seg, mitem_blk, seg_b0, seg_prm, bounds, table);        \
+    }
+    WRS_MERGE_SHAPES(WRS_SMP)
+#undef WRS_SMP
     return hipGetLastError();
 }
 

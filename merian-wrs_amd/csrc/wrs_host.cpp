@@ -20,6 +20,8 @@ thread_local std::string g_last_error;
 
 namespace wrs {
 
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
 void check_hip(hipError_t e, const char* what) {
     if (e != hipSuccess) {
         std::string msg = std::string(what) + ": " + hipGetErrorName(e) + " (" +
@@ -51,7 +53,7 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
             if (lo_buffer) ws.prefix_lo = dev_alloc<float>(max_n);
             ws.samp_h = dev_alloc<double>(wrsk::samples_for(max_n));
             ws.samp_l = dev_alloc<double>(wrsk::samples_for(max_n));
-            ws.bounds = dev_alloc<wrsk::Coarse>(wrsk::merge_blocks_for(max_n) + 2);
+            ws.bounds = dev_alloc<wrsk::Coarse>(wrsk::merge_blocks_for(max_n, wrsk::kMinMergeTile) + 2);
         }
     } catch (...) {
         ws.release();
@@ -79,6 +81,20 @@ void Workspace::release() {
 
 static void require(bool cond, const char* msg) {
     if (!cond) throw Error(WRS_EINVAL, msg);
+}
+
+wrsk::MergeShape merge_shape_of(const wrs_tuning& t) {
+    wrsk::MergeShape ms;
+    if (t.merge_tile) ms.tile = t.merge_tile;
+    if (t.merge_per) ms.per = t.merge_per;
+    return ms;
+}
+
+void require_tuning(const wrs_tuning& t) {
+    require(wrsk::merge_shape_supported(merge_shape_of(t)),
+            "unsupported merge shape (merge_tile 1024/2048 with merge_per 2/4/8, or 4096 with 4/8)");
+    require(t.seg_partition <= 1u, "seg_partition must be 0 or 1");
+    require(t.reserved == 0u, "wrs_tuning.reserved must be 0");
 }
 
 void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, double pivot) const {
@@ -147,15 +163,17 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
                                      ev ? ev[3] : nullptr, ev ? ev[4] : nullptr),
               "partition launch");
     if (ev) check_hip(hipEventRecord(ev[5], s), "event");
-    check_hip(wrsk::launch_coarse(ws.pweights, N, (int)prec, ws.params, ws.samp_h, ws.samp_l,
+    const wrsk::MergeShape ms = merge_shape_of(m_cfg.tuning);
+    check_hip(wrsk::launch_coarse(ws.pweights, N, ms, (int)prec, ws.params, ws.samp_h, ws.samp_l,
                                   ws.bounds, s),
               "coarse split launch");
-    check_hip(wrsk::launch_repair_bounds(ws.bounds, N, ws.params, ws.states, ws.pweights,
+    check_hip(wrsk::launch_repair_bounds(ws.bounds, N, ms, ws.params, ws.states, ws.pweights,
                                          ws.samp_h, ws.samp_l, s),
               "boundary repair launch");
     if (ev) check_hip(hipEventRecord(ev[6], s), "event");
-    check_hip(wrsk::launch_mergepack(ws.pweights, N, (int)prec, ws.params, ws.indices, ws.bounds,
-                                     reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable), s),
+    check_hip(wrsk::launch_mergepack(ws.pweights, N, ms, (int)prec, ws.params, ws.indices,
+                                     ws.bounds, reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable),
+                                     s),
               "merge pack launch");
     if (ev) check_hip(hipEventRecord(ev[7], s), "event");
     if (dbg) {
@@ -270,7 +288,9 @@ wrs::PSAConfig config_of(const wrs_config* cfg) {
     if (cfg) {
         c.splitSize = cfg->split_size ? cfg->split_size : 32u;
         c.precision = cfg->precision;
+        c.tuning = cfg->tuning;
     }
+    wrs::require_tuning(c.tuning);
     return c;
 }
 
@@ -445,7 +465,7 @@ wrs_status wrs_batch_plan_create(const wrs_config* cfg, const uint64_t* offsets_
         require_precision(c.precision);
         wrs_batch_plan* p = new wrs_batch_plan;
         try {
-            p->impl.create(c.splitSize, (int)c.precision, offsets_host, nseg);
+            p->impl.create(c.splitSize, (int)c.precision, c.tuning, offsets_host, nseg);
         } catch (...) {
             p->impl.destroy();
             delete p;

@@ -36,6 +36,8 @@ class Error : public std::runtime_error {
 };
 
 void check_hip(hipError_t e, const char* what);
+// sets the calling thread's wrs_last_error() message
+void set_last_error(const std::string& msg);
 
 // Device workspace shared by the build operators.  All sizes derive from max_n.
 struct Workspace {
@@ -50,7 +52,7 @@ struct Workspace {
     float* prefix_lo = nullptr;        // [max_n] (PSA debug builds only, modes with hi + lo)
     double* samp_h = nullptr;          // [max_n/kSample + 1] heavy prefix samples (PSA only)
     double* samp_l = nullptr;          // [max_n/kSample + 1] light prefix samples (PSA only)
-    wrsk::Coarse* bounds = nullptr;    // [max_n/kMergeTile + 2] block-boundary splits (PSA only)
+    wrsk::Coarse* bounds = nullptr;    // [max_n/kMinMergeTile + 2] block-boundary splits (PSA only)
 
     static Workspace allocate(uint64_t max_n, bool partition_buffers, bool lo_buffer);
     void release();
@@ -103,7 +105,13 @@ class PrefixSum {
 struct PSAConfig {
     uint32_t splitSize = 32;
     wrs_precision precision = WRS_PREC_COMPENSATED;
+    wrs_tuning tuning{};  // launch geometry (zero fields: defaults)
 };
+
+// Merge shape of a tuning (defaults filled in); throws WRS_EINVAL for a shape
+// or partition form that is not supported.
+wrsk::MergeShape merge_shape_of(const wrs_tuning& t);
+void require_tuning(const wrs_tuning& t);
 
 class PSA {
   public:

@@ -29,6 +29,7 @@ EXPORTS = [
     "wrs_prefix_partition", "wrs_scan_inclusive", "wrs_its_build", "wrs_alias_sample",
     "wrs_its_sample", "wrs_philox", "wrs_generate", "wrs_batch_plan_create",
     "wrs_batch_plan_destroy", "wrs_psa_build_batched", "wrs_alias_sample_batched",
+    "wrs_autotune_psa", "wrs_autotune_batched",
     "wrs_last_error", "wrs_version", "wrs_plan_set_timing", "wrs_plan_phase_times",
     "wrs_rt_device_count", "wrs_rt_device_info", "wrs_rt_malloc", "wrs_rt_free",
     "wrs_rt_memcpy_htod", "wrs_rt_memcpy_dtoh", "wrs_rt_memcpy_dtod", "wrs_rt_memset",
@@ -45,8 +46,37 @@ class WrsError(RuntimeError):
         self.status = status
 
 
+class Tuning(C.Structure):
+    """wrs_tuning: launch geometry (zero fields take the defaults)."""
+    _fields_ = [("merge_tile", C.c_uint32), ("merge_per", C.c_uint32),
+                ("seg_partition", C.c_uint32), ("reserved", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return dict(merge_tile=self.merge_tile, merge_per=self.merge_per,
+                    seg_partition=self.seg_partition)
+
+
+def tuning(merge_tile: int = 0, merge_per: int = 0, seg_partition: int = 0) -> Tuning:
+    return Tuning(int(merge_tile), int(merge_per), int(seg_partition), 0)
+
+
 class Config(C.Structure):
-    _fields_ = [("split_size", C.c_uint32), ("precision", C.c_int)]
+    _fields_ = [("split_size", C.c_uint32), ("precision", C.c_int), ("tuning", Tuning)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("tuning", Tuning), ("ms", C.c_float)]
+
+
+TUNE_NO_CACHE = 1
+
+
+def _config(split_size: int, precision: int, tune) -> Config:
+    if tune is None:
+        tune = Tuning()
+    elif isinstance(tune, dict):
+        tune = tuning(**tune)
+    return Config(int(split_size), int(precision), tune)
 
 
 class PsaDebug(C.Structure):
@@ -83,6 +113,10 @@ def lib():
         "wrs_batch_plan_destroy": [vp],
         "wrs_psa_build_batched": [vp, vp, vp, vp],
         "wrs_alias_sample_batched": [vp, vp, u32, u32, u64, u64, vp, vp],
+        "wrs_autotune_psa": [C.POINTER(Config), vp, u64, u32, C.POINTER(Tuning),
+                             C.POINTER(TuneEntry), u32, C.POINTER(u32), vp],
+        "wrs_autotune_batched": [C.POINTER(Config), vp, u32, vp, u32, C.POINTER(Tuning),
+                                 C.POINTER(TuneEntry), u32, C.POINTER(u32), vp],
         "wrs_plan_set_timing": [vp, i32],
         "wrs_plan_phase_times": [vp, vp, u32],
         "wrs_rt_device_count": [C.POINTER(C.c_int)],
@@ -261,11 +295,12 @@ def _p(x):
 class Plan:
     """wrs_plan: build workspace for distributions of up to max_n weights."""
 
-    def __init__(self, max_n: int, split_size: int = 32, precision: int = PREC_COMPENSATED):
+    def __init__(self, max_n: int, split_size: int = 32, precision: int = PREC_COMPENSATED,
+                 tune=None):
         self.max_n = int(max_n)
         self.split_size = int(split_size)
         self.precision = int(precision)
-        cfg = Config(self.split_size, self.precision)
+        cfg = _config(self.split_size, self.precision, tune)
         p = C.c_void_p()
         check(lib().wrs_plan_create(C.byref(cfg), self.max_n, C.byref(p)))
         self.handle = p.value
@@ -326,10 +361,10 @@ class BatchPlan:
     """wrs_batch_plan: segmented build of many distributions (CSR offsets)."""
 
     def __init__(self, offsets: np.ndarray, split_size: int = 32,
-                 precision: int = PREC_COMPENSATED):
+                 precision: int = PREC_COMPENSATED, tune=None):
         self.offsets = np.ascontiguousarray(offsets, np.uint64)
         self.nseg = self.offsets.size - 1
-        cfg = Config(int(split_size), int(precision))
+        cfg = _config(int(split_size), int(precision), tune)
         p = C.c_void_p()
         check(lib().wrs_batch_plan_create(C.byref(cfg), self.offsets.ctypes.data, self.nseg,
                                           C.byref(p)))
@@ -348,6 +383,39 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+# -------------------------------------------------------------- autotuner --
+def _tune_result(best: Tuning, log, count: int) -> tuple:
+    entries = [dict(log[i].tuning.as_dict(), ms=log[i].ms) for i in range(min(count, len(log)))]
+    return best.as_dict(), entries
+
+
+def autotune_psa(weights, n: int, precision: int = PREC_COMPENSATED, split_size: int = 32,
+                 no_cache: bool = False, stream=None) -> tuple:
+    """(best tuning dict, [candidate dicts with 'ms']) for a PSA build of n weights."""
+    cfg = _config(split_size, precision, None)
+    best = Tuning()
+    log = (TuneEntry * 32)()
+    cnt = C.c_uint32(0)
+    check(lib().wrs_autotune_psa(C.byref(cfg), _p(weights), int(n),
+                                 TUNE_NO_CACHE if no_cache else 0, C.byref(best), log, 32,
+                                 C.byref(cnt), _h(stream)))
+    return _tune_result(best, log, cnt.value)
+
+
+def autotune_batched(offsets: np.ndarray, weights, precision: int = PREC_COMPENSATED,
+                     split_size: int = 32, no_cache: bool = False, stream=None) -> tuple:
+    """(best tuning dict, [candidate dicts with 'ms']) for a batched build."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    cfg = _config(split_size, precision, None)
+    best = Tuning()
+    log = (TuneEntry * 32)()
+    cnt = C.c_uint32(0)
+    check(lib().wrs_autotune_batched(C.byref(cfg), offsets.ctypes.data, offsets.size - 1,
+                                     _p(weights), TUNE_NO_CACHE if no_cache else 0,
+                                     C.byref(best), log, 32, C.byref(cnt), _h(stream)))
+    return _tune_result(best, log, cnt.value)
 
 
 # --------------------------------------------------------------- sampling --

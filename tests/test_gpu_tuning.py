@@ -1,0 +1,101 @@
+"""GPU tests of the launch geometry (wrs_tuning) and the autotuner: every
+supported merge shape builds the oracle's table, batched builds with a tuning
+equal per-segment builds with the same tuning, and the tuner measures every
+candidate once and then answers from its cache."""
+import numpy as np
+import pytest
+
+from test_gpu_batched import make_batch
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1024, 2), (1024, 4), (1024, 8), (2048, 2), (2048, 4), (2048, 8), (4096, 4), (4096, 8)]
+
+
+def case(*a):
+    print("case:", *a, flush=True)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("kind", ["uniform", "zipf"])
+def test_merge_shape_matches_oracle(wrs, oracle, kind, shape):
+    n = (1 << 20) + 5
+    case("merge shape", kind, shape, n)
+    w = oracle.gen_uniform01(0, n) if kind == "uniform" else oracle.gen_zipf(2, 1.1, n)
+    plan = wrs.Plan(n, tune=dict(merge_tile=shape[0], merge_per=shape[1]))
+    dw = wrs.to_device(w)
+    table = wrs.empty(n, wrs.ALIAS_DTYPE)
+    table.fill_bytes(0xFF)
+    mean = wrs.empty(1, np.float64)
+    plan.psa_build_debug(dw, n, table, dict(mean=mean))
+    wrs.synchronize()
+    t = table.numpy()
+    p, a, d = oracle.psa_alias_table_f64(w, -(-n // 32), mean=float(mean.numpy()[0]),
+                                         split_rule=oracle.SPLIT_LOWER_BOUND)
+    assert np.array_equal(t["a"], a)
+    np.testing.assert_allclose(t["p"].astype(np.float64), p, rtol=3e-7, atol=1e-7)
+    err = oracle.distribution_error(oracle.implied_distribution(t["p"], t["a"]), w)
+    assert err["max_rel"] <= 1e-6
+
+
+@pytest.mark.parametrize("tune", [dict(seg_partition=1), dict(merge_tile=1024, merge_per=8),
+                                  dict(seg_partition=1, merge_tile=4096, merge_per=4)])
+def test_batched_tuning_equals_per_segment(wrs, oracle, tune):
+    nseg, per = 150, 64
+    case("batched tuning", tune)
+    sizes, off, w = make_batch(wrs, oracle, nseg, 1 << 15, seed=9)
+    total = int(off[-1])
+    dw = wrs.to_device(w)
+    doff = wrs.to_device(off)
+    bp = wrs.BatchPlan(off, tune=tune)
+    tb = wrs.empty(total, wrs.ALIAS_DTYPE)
+    sb = wrs.empty(nseg * per, np.uint32)
+    bp.build(dw, tb)
+    wrs.alias_sample_batched(tb, doff, nseg, per, 42, sb)
+    plan = wrs.Plan(int(sizes.max()), tune=tune)
+    te = wrs.empty(total, wrs.ALIAS_DTYPE)
+    se = wrs.empty(nseg * per, np.uint32)
+    wrs.build_sample_each(plan, dw, off, te, per, 42, se)
+    wrs.synchronize()
+    assert np.array_equal(tb.numpy(), te.numpy())
+    assert np.array_equal(sb.numpy(), se.numpy())
+
+
+def test_autotune_psa_measures_then_caches(wrs, oracle):
+    n = 1 << 20
+    case("autotune psa", n)
+    w = oracle.gen_zipf(2, 1.1, n)
+    dw = wrs.to_device(w)
+    best, log = wrs.autotune_psa(dw, n, no_cache=True)
+    print("  best", best, [(e["merge_tile"], e["merge_per"], round(e["ms"], 4)) for e in log])
+    assert len(log) == len(SHAPES)
+    assert {(e["merge_tile"], e["merge_per"]) for e in log} == set(SHAPES)
+    assert all(e["ms"] > 0 for e in log)
+    assert (best["merge_tile"], best["merge_per"]) in SHAPES
+    assert min(log, key=lambda e: e["ms"])["merge_tile"] == best["merge_tile"]
+    best2, log2 = wrs.autotune_psa(dw, n)
+    assert best2 == best and log2 == []
+    # the tuned plan builds the oracle's table
+    plan = wrs.Plan(n, tune=best)
+    table = wrs.empty(n, wrs.ALIAS_DTYPE)
+    plan.psa_build(dw, n, table)
+    wrs.synchronize()
+    t = table.numpy()
+    err = oracle.distribution_error(oracle.implied_distribution(t["p"], t["a"]), w)
+    assert err["max_rel"] <= 1e-6
+
+
+def test_autotune_batched_measures_then_caches(wrs, oracle):
+    nseg = 200
+    case("autotune batched", nseg)
+    sizes, off, w = make_batch(wrs, oracle, nseg, 1 << 15, seed=10)
+    dw = wrs.to_device(w)
+    best, log = wrs.autotune_batched(off, dw, no_cache=True)
+    print("  best", best, [(e["seg_partition"], e["merge_tile"], e["merge_per"],
+                            round(e["ms"], 4)) for e in log])
+    # partition form at the default shape (2), then the 7 other shapes
+    assert len(log) == 2 + len(SHAPES) - 1
+    assert {e["seg_partition"] for e in log[:2]} == {0, 1}
+    assert (best["merge_tile"], best["merge_per"]) in SHAPES
+    best2, log2 = wrs.autotune_batched(off, dw)
+    assert best2 == best and log2 == []
