@@ -175,8 +175,14 @@ def main() -> int:
     # emit reads w and writes index + weight; aggregate and reduce read w)
     alg = {"reduce": 4 * n, "finalize": 0, "tile_aggregate": 4 * n, "tile_scan": 0,
            "partition_emit": 12 * n, "boundary_splits": 0, "merge_pack": 16 * n}
-    kname = {"reduce": "k_reduce", "tile_aggregate": "k_part_agg",
-             "partition_emit": "k_part_emit", "merge_pack": "k_mergepack"}
+    # kernel instances (template arguments: precision, merge shape) as rocprofv3 names them
+    p, t_ = args.precision, (tune or {})
+    mt = t_.get("merge_tile") or 2048
+    mp = t_.get("merge_per") or 4
+    kname = {"reduce": f"k_reduce<{'false' if p == 0 else 'true'}>",
+             "tile_aggregate": f"k_part_agg<{p}>",
+             "partition_emit": f"k_part_emit<{p}, false>",
+             "merge_pack": f"k_mergepack<{0 if p == 0 else 1}, {mt}, {mt // mp}>"}
     log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phase.items()))
     dom = max(kname, key=lambda k: phase[k])
     dom_ms = phase[dom]

@@ -231,56 +231,80 @@ __device__ __forceinline__ int64_t quant(double x, int qs) { return __double2ll_
 // offset D = X(i0) - Y(j0) = n0*mean - LP(i0) - HP(j0) (FP64), heavy q goes
 // before light r iff Y(j0+q+1) <= X(i0+r), i.e. b_q - quant(D) <= a_r, and its
 // residual is Y(j0+q+1) - X(i0+r) + mean = (b_q - a_r) 2^-qs - D + mean.
-template <int PREC, int TILE, int BLOCK>
-__device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
-                                                const uint32_t* __restrict__ idx, uint32_t N,
-                                                uint32_t H, double mean, double total,
-                                                uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
-                                                AliasEntry* __restrict__ table) {
-    constexpr int kPer = TILE / BLOCK;  // buckets (and staged elements) per thread
-    static_assert(TILE % BLOCK == 0 && BLOCK % kWave == 0 && TILE % kSample == 0, "merge shape");
-    // keys during the merge, entries afterwards (8 bytes per slot either way)
-    __shared__ uint2 s_raw[TILE + 2];
-    __shared__ uint32_t s_idx[TILE + 2];
-    __shared__ float s_w[TILE];
-    __shared__ int64_t s_wave_l[BLOCK / kWave], s_wave_h[BLOCK / kWave];
-    int64_t* s_key = reinterpret_cast<int64_t*>(s_raw);
+// LDS of one merge block.  Slots are XOR-swizzled within aligned groups of
+// four (sw8 for the 8-byte arrays, sw4 for the 4-byte ones): a thread owns
+// kPer consecutive slots, so without it the lanes of a wave hit the same
+// banks at a stride of kPer slots.
+__device__ __forceinline__ uint32_t sw8(uint32_t x) { return x ^ ((x >> 4) & 3u); }
+__device__ __forceinline__ uint32_t sw4(uint32_t x) { return x ^ ((x >> 5) & 3u); }
 
+template <int TILE, int BLOCK>
+struct MergeShared {
+    uint2 raw[TILE + 4];  // keys during the merge, entries afterwards
+    uint32_t idx[TILE + 4];
+    float w[TILE + 4];
+    int64_t wave_l[BLOCK / kWave], wave_h[BLOCK / kWave];
+};
+
+// Indices and weights of one block's slots, in registers between their load
+// and their store to LDS: slot e < nl is light i0 + e, slot nl <= e <= ne is
+// heavy j0 + e - nl (the weight of slot ne is not needed).  Every load is
+// issued (at position 0 for the slots beyond ne), so that the staging has no
+// branches.
+template <int TILE, int BLOCK>
+struct MergeStage {
+    static constexpr int kPer = TILE / BLOCK;
+    uint32_t gi[kPer + 1];
+    float gw[kPer + 1];
+
+    __device__ __forceinline__ void issue(const float* __restrict__ pw,
+                                          const uint32_t* __restrict__ idx, uint32_t N,
+                                          const Coarse& c0, const Coarse& c1) {
+        const uint32_t t = threadIdx.x;
+        const uint32_t nl = c1.i - c0.i, ne = nl + (c1.j - c0.j);
+#pragma unroll
+        for (int k = 0; k <= kPer; ++k) {
+            const uint32_t e = t + (uint32_t)BLOCK * k;
+            uint32_t pos = e < nl ? N - 1u - (c0.i + e) : c0.j + (e - nl);
+            pos = e <= ne ? pos : 0u;
+            gi[k] = idx[pos];
+            gw[k] = pw[pos];
+        }
+    }
+    __device__ __forceinline__ void store(MergeShared<TILE, BLOCK>& s, uint32_t ne) const {
+        const uint32_t t = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k <= kPer; ++k) {
+            const uint32_t e = t + (uint32_t)BLOCK * k;
+            s.idx[sw4(e <= ne ? e : (uint32_t)TILE + 1u)] = gi[k];
+            s.w[sw4(e < ne ? e : (uint32_t)TILE)] = gw[k];
+        }
+    }
+};
+
+// Keys and merge of one block whose slots are staged in s (block
+// synchronised): leaves the entry of slot e at s.raw[sw8(e)], e < ne, and
+// ends synchronised.
+//
+// Local keys are exact 64-bit fixed-point running sums of non-negative
+// increments (w_q = quant(w), mean_q = quant(mean)):
+//     a_r = sum_{r' < r}  (mean_q - w_q(light i0 + r'))      r in [0, nl]
+//     b_q = sum_{q' <= q} (w_q(heavy j0 + q') - mean_q)       q in [0, nh)
+// so A and B are sorted whatever the rounding of the prefix sums, and the
+// merge below is always a permutation of the block's slots.  With the split
+// offset D = X(i0) - Y(j0) = n0*mean - LP(i0) - HP(j0) (FP64), heavy q goes
+// before light r iff Y(j0+q+1) <= X(i0+r), i.e. b_q - quant(D) <= a_r, and its
+// residual is Y(j0+q+1) - X(i0+r) + mean = (b_q - a_r) 2^-qs - D + mean.
+template <int PREC, int TILE, int BLOCK>
+__device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, double mean, double total,
+                                           uint32_t n0, const Coarse& c0, uint32_t nl,
+                                           uint32_t nh) {
+    constexpr int kPer = TILE / BLOCK;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
-    const uint32_t L = N - H;
-    if (H == 0 || L == 0) {  // all weights equal the mean up to its rounding: identity
-        for (uint32_t x = n0 + t; x < n1; x += BLOCK) table[x] = AliasEntry{1.0f, x};
-        return;
-    }
-    const bool last = n1 == N;
-    const uint32_t i0 = c0.i, j0 = c0.j;
-    const uint32_t nl = c1.i - i0;  // lights packed here
-    const uint32_t nh = c1.j - j0;  // heavies packed here (heavy c1.j is current at n1)
-    const uint32_t ne = nl + nh;    // == n1 - n0 (- 1 in the last block)
-
-    // ---- stage indices and weights in rank order ------------------------
-    // slot e < nl: light i0 + e; slot nl <= e <= ne: heavy j0 + e - nl.  Index
-    // and weight loads are independent and all issued together.
-    {
-        uint32_t gi[kPer + 1];
-        float gw[kPer + 1];
-#pragma unroll
-        for (int k = 0; k <= kPer; ++k) {
-            const uint32_t e = (uint32_t)t + (uint32_t)BLOCK * k;
-            const uint32_t pos = e < nl ? N - 1u - (i0 + e) : j0 + (e - nl);
-            gi[k] = e <= ne ? idx[pos] : 0u;
-            gw[k] = e < ne ? pw[pos] : 0.0f;
-        }
-#pragma unroll
-        for (int k = 0; k <= kPer; ++k) {
-            const uint32_t e = (uint32_t)t + (uint32_t)BLOCK * k;
-            if (e <= ne) s_idx[e] = gi[k];
-            if (e < ne) s_w[e] = gw[k];
-        }
-    }
-    __syncthreads();
+    const uint32_t ne = nl + nh;
+    int64_t* s_key = reinterpret_cast<int64_t*>(s.raw);
 
     // ---- keys -----------------------------------------------------------
     const int qs = 61 - ilogb(total);
@@ -293,7 +317,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
         const uint32_t e = e0 + k;
         inc[k] = 0;
         if (e < ne) {
-            const int64_t wq = quant((double)s_w[e], qs);
+            const int64_t wq = quant((double)s.w[sw4(e)], qs);
             inc[k] = e < nl ? mean_q - wq : wq - mean_q;
             if (e < nl) ml += inc[k];
             else mh += inc[k];
@@ -312,17 +336,17 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     int64_t rl = shfl_up(il, 1), rh = shfl_up(ih, 1);  // exclusive per lane
     if (lane == 0) rl = rh = 0;
     if (lane == kWave - 1) {
-        s_wave_l[wid] = il;
-        s_wave_h[wid] = ih;
+        s.wave_l[wid] = il;
+        s.wave_h[wid] = ih;
     }
     __syncthreads();
     int64_t tot_l = 0;
     for (int q = 0; q < BLOCK / kWave; ++q) {
         if (q < wid) {
-            rl += s_wave_l[q];
-            rh += s_wave_h[q];
+            rl += s.wave_l[q];
+            rh += s.wave_h[q];
         }
-        tot_l += s_wave_l[q];
+        tot_l += s.wave_l[q];
     }
     // heavy keys carry the split offset, so that "heavy first" is B[q] <= A[r]
     const double D = (double)n0 * mean - (c0.lp + c0.hp);
@@ -333,15 +357,15 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
         const uint32_t e = e0 + k;
         if (e < ne) {
             if (e < nl) {  // exclusive running sum
-                s_key[e] = rl;
+                s_key[sw8(e)] = rl;
                 rl += inc[k];
-            } else {       // inclusive running sum
+            } else {       // inclusive running sum, stored at slot e + 1
                 rh += inc[k];
-                s_key[nl + 1u + (e - nl)] = rh - D_q;
+                s_key[sw8(e + 1u)] = rh - D_q;
             }
         }
     }
-    if (t == 0) s_key[nl] = tot_l;  // a at i1
+    if (t == 0) s_key[sw8(nl)] = tot_l;  // a at i1
     __syncthreads();
 
     // ---- merge: thread t emits buckets [kPer*t, kPer*t + kPer) ----------
@@ -353,8 +377,8 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
         if constexpr (PREC == kFP32) return (float)((float)x / (float)mean);
         else return (float)(x * inv_mean);
     };
-    const int64_t* A = s_key;            // A[r], r in [0, nl]
-    const int64_t* B = s_key + nl + 1u;  // B[q], q in [0, nh)
+    auto A = [&](uint32_t r) -> int64_t { return s_key[sw8(r)]; };           // r in [0, nl]
+    auto B = [&](uint32_t q) -> int64_t { return s_key[sw8(nl + 1u + q)]; };  // q in [0, nh)
     const uint32_t d0 = e0;
     uint32_t slot[kPer];
     uint2 ent[kPer];
@@ -367,15 +391,15 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
         while (qa < qb) {
             const int64_t q = (qa + qb + 1) / 2;
             const uint32_t r = d0 - (uint32_t)q;
-            const bool ok = r >= nl || B[q - 1] <= A[r];
+            const bool ok = r >= nl || B((uint32_t)q - 1u) <= A(r);
             if (ok) qa = q;
             else qb = q - 1;
         }
         uint32_t q = (uint32_t)qa;
         uint32_t r = d0 - q;
         const uint32_t d1 = min(d0 + (uint32_t)kPer, ne);
-        int64_t av = A[r];  // r <= nl: A[nl] exists
-        int64_t bv = q < nh ? B[q] : 0;
+        int64_t av = A(r);  // r <= nl: A[nl] exists
+        int64_t bv = q < nh ? B(q) : 0;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             if (d0 + (uint32_t)k < d1) {
@@ -384,15 +408,15 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
                     const double resid = ldexp((double)(bv - av), -qs) + dres;
                     const float p = fminf(fmaxf(ratio(resid), 0.0f), 1.0f);
                     slot[k] = nl + q;
-                    ent[k] = make_uint2(__float_as_uint(p), s_idx[nl + q + 1u]);
+                    ent[k] = make_uint2(__float_as_uint(p), s.idx[sw4(nl + q + 1u)]);
                     ++q;
-                    bv = q < nh ? B[q] : 0;
+                    bv = q < nh ? B(q) : 0;
                 } else {
-                    const float p = fminf(ratio((double)s_w[r]), 1.0f);
+                    const float p = fminf(ratio((double)s.w[sw4(r)]), 1.0f);
                     slot[k] = r;
-                    ent[k] = make_uint2(__float_as_uint(p), s_idx[nl + q]);
+                    ent[k] = make_uint2(__float_as_uint(p), s.idx[sw4(nl + q)]);
                     ++r;
-                    av = A[r];
+                    av = A(r);
                 }
                 cnt = k + 1;
             }
@@ -401,12 +425,42 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     __syncthreads();  // keys no longer read: the buffer now takes the entries
 #pragma unroll
     for (int k = 0; k < kPer; ++k)
-        if (k < cnt) s_raw[slot[k]] = ent[k];
+        if (k < cnt) s.raw[sw8(slot[k])] = ent[k];
     __syncthreads();
+}
+
+// One block of TILE buckets [n0, n1) between boundary splits c0 and c1, BLOCK
+// threads, TILE / BLOCK buckets per thread.  Rows are written with
+// consecutive lanes on consecutive slots (consecutive light indices).  The
+// last heavy of all gets (1, itself), as in pack.hpp:78-81.
+template <int PREC, int TILE, int BLOCK>
+__device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
+                                                const uint32_t* __restrict__ idx, uint32_t N,
+                                                uint32_t H, double mean, double total,
+                                                uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
+                                                AliasEntry* __restrict__ table) {
+    static_assert(TILE % BLOCK == 0 && BLOCK % kWave == 0 && TILE % kSample == 0, "merge shape");
+    __shared__ MergeShared<TILE, BLOCK> s;
+    const uint32_t t = threadIdx.x;
+    const uint32_t L = N - H;
+    if (H == 0 || L == 0) {  // all weights equal the mean up to its rounding: identity
+        for (uint32_t x = n0 + t; x < n1; x += BLOCK) table[x] = AliasEntry{1.0f, x};
+        return;
+    }
+    const uint32_t nl = c1.i - c0.i;  // lights packed here
+    const uint32_t nh = c1.j - c0.j;  // heavies packed here (heavy c1.j is current at n1)
+    const uint32_t ne = nl + nh;      // == n1 - n0 (- 1 in the last block)
+    {
+        MergeStage<TILE, BLOCK> st;
+        st.issue(pw, idx, N, c0, c1);
+        st.store(s, ne);
+    }
+    __syncthreads();
+    merge_core<PREC, TILE, BLOCK>(s, mean, total, n0, c0, nl, nh);
     uint2* tab = reinterpret_cast<uint2*>(table);
-    for (uint32_t e = t; e < ne; e += BLOCK) tab[s_idx[e]] = s_raw[e];
-    if (last && t == 0) {
-        const uint32_t h = s_idx[nl + nh];  // heavy H-1
+    for (uint32_t e = t; e < ne; e += BLOCK) tab[s.idx[sw4(e)]] = s.raw[sw8(e)];
+    if (n1 == N && t == 0) {
+        const uint32_t h = s.idx[sw4(ne)];  // heavy H-1
         table[h] = AliasEntry{1.0f, h};
     }
 }
@@ -420,8 +474,8 @@ __global__ __launch_bounds__(BLOCK) void k_mergepack(const float* __restrict__ p
     const uint32_t b = blockIdx.x;
     const uint32_t n0 = b * (uint32_t)TILE;
     const uint32_t n1 = min(n0 + (uint32_t)TILE, N);
-    mergepack_block<PREC, TILE, BLOCK>(pw, idx, N, prm->heavy_count, prm->mean, prm->total, n0, n1, bounds[b],
-                          bounds[b + 1], table);
+    mergepack_block<PREC, TILE, BLOCK>(pw, idx, N, prm->heavy_count, prm->mean, prm->total, n0,
+                                       n1, bounds[b], bounds[b + 1], table);
 }
 
 template <int PREC, int TILE, int BLOCK>
@@ -715,21 +769,26 @@ bool merge_shape_supported(MergeShape ms) {
     return false;
 }
 
+namespace {
+template <int P, int T, int B>
+hipError_t launch_mp(const float* pw, uint32_t N, const Params* prm, const uint32_t* idx,
+                     const Coarse* bounds, AliasEntry* table, hipStream_t s) {
+    k_mergepack<P, T, B><<<(uint32_t)merge_blocks_for(N, T), B, 0, s>>>(pw, idx, N, prm, bounds,
+                                                                        table);
+    return hipGetLastError();
+}
+}  // namespace
+
 hipError_t launch_mergepack(const float* pw, uint64_t n, MergeShape ms, int prec,
                             const Params* prm, const uint32_t* idx, const Coarse* bounds,
                             AliasEntry* table, hipStream_t s) {
     const uint32_t N = (uint32_t)n;
-    const uint32_t blocks = (uint32_t)merge_blocks_for(n, ms.tile);
     // the merge keys are exact in every mode; only FP32 mode's p differs
-#define WRS_MP(T, P)                                                                              \
-    if (ms.tile == (T) && ms.per == (P)) {                                                        \
-        if (prec == kFP32)                                                                        \
-            k_mergepack<kFP32, T, T / P><<<blocks, T / P, 0, s>>>(pw, idx, N, prm, bounds, table); \
-        else                                                                                      \
-            k_mergepack<kFP64Carry, T, T / P><<<blocks, T / P, 0, s>>>(pw, idx, N, prm, bounds,    \
-                                                                       table);                    \
-        return hipGetLastError();                                                                 \
-    }
+#define WRS_MP(T, P)                                                                       \
+    if (ms.tile == (T) && ms.per == (P))                                                   \
+        return prec == kFP32                                                               \
+                   ? launch_mp<kFP32, T, T / P>(pw, N, prm, idx, bounds, table, s)          \
+                   : launch_mp<kFP64Carry, T, T / P>(pw, N, prm, idx, bounds, table, s);
     WRS_MERGE_SHAPES(WRS_MP)
 #undef WRS_MP
     return hipErrorInvalidValue;
