@@ -22,6 +22,8 @@ Also reported on the same line:
   sampling      alias sampling, S = 2^26 Philox samples;
   its           ITS on N = 2^24 LogNormal(0, 2): normalised scan (8N bytes)
                 and S = 2^24 binary-search samples, CDF error vs the FP64 prefix;
+                the cutpoint guide table (N entries) over the same CDF and S = 2^24
+                guided samples (bit-identical to the ITS samples);
   spike         N = 2^28 spike mixture build and S = 2^30 alias samples;
   batched       4096 log-uniform-size LogNormal(0, 3) distributions built and
                 sampled (64 each) by one hipGraph vs one launch sequence per
@@ -250,15 +252,28 @@ def main() -> int:
         outi = wrs.empty(ni, np.uint32)
         wrs.its_sample(hi, lo, ni, ni, 7, outi, stream=st)
         its_ms = time_it(lambda: wrs.its_sample(hi, lo, ni, ni, 7, outi, stream=st), st, wrs, 5)
+        # cutpoint: guide table (m = N entries) over the same CDF, guided samples
+        gd = wrs.empty(ni + 1, np.uint32)
+        wrs.cutpoint_build(hi, lo, ni, gd, ni, stream=st)
+        guide_ms = time_it(lambda: wrs.cutpoint_build(hi, lo, ni, gd, ni, stream=st), st, wrs, 5)
+        outc = wrs.empty(ni, np.uint32)
+        wrs.cutpoint_sample(hi, lo, ni, gd, ni, ni, 7, outc, stream=st)
+        cut_ms = time_it(lambda: wrs.cutpoint_sample(hi, lo, ni, gd, ni, ni, 7, outc, stream=st),
+                         st, wrs, 5)
         st.synchronize()
+        cut_same = bool(np.array_equal(outc.numpy(), outi.numpy()))
         wi64 = wi.numpy().astype(np.float64)
         exact = np.cumsum(wi64) / np.sum(np.sort(wi64))
         got = hi.numpy().astype(np.float64) + lo.numpy().astype(np.float64)
         its = dict(n=ni, scan_ms=round(scan_ms, 4), scan_gbs_8N=round(8 * ni / (scan_ms * 1e-3) / 1e9, 1),
                    sample_ms=round(its_ms, 4), samples_per_s=round(ni / (its_ms * 1e-3), 1),
-                   cdf_max_abs_err=float(np.abs(got - exact).max()))
+                   cdf_max_abs_err=float(np.abs(got - exact).max()),
+                   cutpoint=dict(guide_entries=ni, guide_build_ms=round(guide_ms, 4),
+                                 sample_ms=round(cut_ms, 4),
+                                 samples_per_s=round(ni / (cut_ms * 1e-3), 1),
+                                 identical_to_its=cut_same))
         log(f"its: {its}")
-        del wi, hi, lo, outi, iplan
+        del wi, hi, lo, outi, iplan, gd, outc
 
     # ---- spike mixture at N = 2^28, S = 2^30 --------------------------------
     spike = None

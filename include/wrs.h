@@ -157,6 +157,23 @@ wrs_status wrs_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n, 
                           uint64_t seed, uint64_t counter_base, uint32_t* out,
                           wrs_stream_t stream);
 
+/* Cutpoint method (Chen & Asau's guide table) over a normalised ITS CDF
+ * (wrs_its_build output; cdf_lo optional, as for wrs_its_sample).  guide has
+ * m + 1 entries (1 <= m < 2^32): guide[g] = smallest i with cdf(i) >= g/m
+ * (n - 1 if none), guide[m] = n - 1.  Replaces the guiding-table pass of
+ * Cutpoint::build (src/device/wrs/cutpoint/Cutpoint.hpp:93-127,
+ * guiding_table/shader.comp:37-54; its guidingTableSize is m). */
+wrs_status wrs_cutpoint_build(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                              uint32_t* guide, uint64_t m, wrs_stream_t stream);
+/* Guided ITS sampling: out[t] is bit-identical to wrs_its_sample's with the
+ * same seed and counters (the guide only narrows the lower-bound search to
+ * [guide[g], guide[g+1]], g = floor(u m)).  Replaces Cutpoint::sample
+ * (Cutpoint.hpp:129-146, sampling/shader.comp:118-177), whose second uniform
+ * per guide cell weights every cell by 1/m instead of by its mass. */
+wrs_status wrs_cutpoint_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                               const uint32_t* guide, uint64_t m, uint64_t s, uint64_t seed,
+                               uint64_t counter_base, uint32_t* out, wrs_stream_t stream);
+
 /* Philox4x32-10 blocks: out[4t..4t+3] = philox(ctr = (lo(base+t), hi(base+t),
  * stream_word, 0), key = seed).  Replaces the PRNG module (src/device/prng/
  * philox/Philox.hpp:47-104), which is a non-standard 2x32 7-round variant. */

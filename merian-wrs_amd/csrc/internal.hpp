@@ -139,6 +139,13 @@ hipError_t launch_alias_sample(const AliasEntry* table, uint64_t n, uint64_t s_c
 hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
                              uint64_t s_count, uint64_t seed, uint64_t counter_base,
                              uint32_t* out, hipStream_t s);
+// guide[0..m] of the normalised CDF (m >= 1), and guided ITS samples
+hipError_t launch_cutpoint_build(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                 uint32_t* guide, uint64_t m, hipStream_t s);
+hipError_t launch_cutpoint_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                  const uint32_t* guide, uint64_t m, uint64_t s_count,
+                                  uint64_t seed, uint64_t counter_base, uint32_t* out,
+                                  hipStream_t s);
 hipError_t launch_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uint64_t n,
                          uint32_t* out4, hipStream_t s);
 hipError_t launch_generate(int kind, double a, double b, uint64_t seed, uint64_t n, float* out,

@@ -58,6 +58,106 @@ __global__ __launch_bounds__(kSampleBlock) void k_its_sample(const float* __rest
     }
 }
 
+// ---- cutpoint (guide table) acceleration of ITS sampling ---------------------
+//
+// Chen & Asau's guide table over the normalised CDF c(i) = hi[i] (+ lo[i]):
+// guide[g] = smallest i in [0, n) with c(i) >= t_g, t_g = g / m, for g < m
+// (n - 1 when there is none), and guide[m] = n - 1.  A sample's u lies in
+// [t_g, t_{g+1}) for g = G(u) below, so its lower bound -- the smallest i with
+// c(i) >= u -- lies in [guide[g], guide[g+1]]: the search over that range
+// returns exactly what the full binary search of k_its_sample returns for the
+// same u.  Replaces the reference's Cutpoint method
+// (src/device/wrs/cutpoint/guiding_table/shader.comp:37-54,
+// src/device/wrs/cutpoint/sampling/shader.comp:118-177), whose sampler picks the
+// guide cell with a second uniform instead of from u and so weights every cell
+// by 1/m whatever its mass.
+
+__device__ __forceinline__ double cut_t(uint64_t g, uint64_t m) { return (double)g / (double)m; }
+
+// largest g in [-1, m-1] with t_g <= c (-1: none), by the same comparison as
+// the table's thresholds
+__device__ __forceinline__ int64_t cut_floor(double c, uint64_t m) {
+    if (!(c >= 0.0)) return -1;
+    double x = floor(c * (double)m);
+    int64_t g = x >= (double)m ? (int64_t)m - 1 : (int64_t)x;
+    while (g + 1 <= (int64_t)m - 1 && cut_t((uint64_t)(g + 1), m) <= c) ++g;
+    while (g >= 0 && cut_t((uint64_t)g, m) > c) --g;
+    return g;
+}
+
+template <bool LO>
+__device__ __forceinline__ double cdf_at(const float* __restrict__ hi, const float* __restrict__ lo,
+                                         uint64_t i) {
+    double c = (double)hi[i];
+    if (LO) c += (double)lo[i];
+    return c;
+}
+
+// lower bound of t in [a, b]: smallest i with c(i) >= t, b if none
+template <bool LO>
+__device__ __forceinline__ uint64_t cdf_lower_bound(const float* __restrict__ hi,
+                                                    const float* __restrict__ lo, uint64_t a,
+                                                    uint64_t b, double t) {
+    while (a < b) {
+        const uint64_t mid = (a + b) >> 1;
+        if (t > cdf_at<LO>(hi, lo, mid)) a = mid + 1;
+        else b = mid;
+    }
+    return a;
+}
+
+// The table is built in two passes, one thread per entry, so that the work
+// does not depend on how the mass is spread: the anchors g = 64k (and g = m)
+// by a search of the whole CDF, then every other entry by a search between
+// its two anchors (the entries are non-decreasing in g).
+constexpr uint32_t kCutAnchor = 64;
+
+template <bool LO>
+__global__ __launch_bounds__(256) void k_cutpoint_anchors(const float* __restrict__ hi,
+                                                          const float* __restrict__ lo,
+                                                          uint64_t n, uint32_t* __restrict__ guide,
+                                                          uint64_t m) {
+    const uint64_t na = (m + kCutAnchor - 1) / kCutAnchor;  // anchors below m
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= na; k += stride) {
+        const uint64_t g = k < na ? k * kCutAnchor : m;
+        guide[g] = g == m ? (uint32_t)(n - 1)
+                          : (uint32_t)cdf_lower_bound<LO>(hi, lo, 0, n - 1, cut_t(g, m));
+    }
+}
+
+template <bool LO>
+__global__ __launch_bounds__(256) void k_cutpoint_fill(const float* __restrict__ hi,
+                                                       const float* __restrict__ lo, uint64_t n,
+                                                       uint32_t* __restrict__ guide, uint64_t m) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < m; g += stride) {
+        if (g % kCutAnchor == 0) continue;
+        const uint64_t ga = g - g % kCutAnchor;
+        const uint64_t gb = ga + kCutAnchor < m ? ga + kCutAnchor : m;
+        guide[g] = (uint32_t)cdf_lower_bound<LO>(hi, lo, guide[ga], guide[gb], cut_t(g, m));
+    }
+}
+
+template <bool LO>
+__global__ __launch_bounds__(kSampleBlock) void k_cutpoint_sample(
+    const float* __restrict__ hi, const float* __restrict__ lo, uint64_t n,
+    const uint32_t* __restrict__ guide, uint64_t m, uint64_t s, uint64_t seed, uint64_t base,
+    uint32_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * kSampleBlock;
+    for (uint64_t t = (uint64_t)blockIdx.x * kSampleBlock + threadIdx.x; t < s; t += stride) {
+        const U4 r = philox_at(seed, base + t, 0u, 0u);
+        const double u = u53_open0(r.x, r.y);  // the u of k_its_sample
+        int64_t g = cut_floor(u, m);
+        g = g < 0 ? 0 : g;
+        uint64_t a = guide[g], b = guide[g + 1];
+        // a table built from another CDF must not send the search out of bounds
+        if (b > n - 1) b = n - 1;
+        if (a > b) a = 0;
+        out[t] = (uint32_t)cdf_lower_bound<LO>(hi, lo, a, b, u);
+    }
+}
+
 __global__ void k_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uint64_t n,
                          uint4* __restrict__ out) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -172,6 +272,34 @@ hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t 
     else
         k_its_sample<false><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, s_count, seed,
                                                        counter_base, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cutpoint_build(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                 uint32_t* guide, uint64_t m, hipStream_t s) {
+    const uint32_t ga = grid_for(m / kCutAnchor + 2, 256, 1u << 16);
+    const uint32_t gf = grid_for(m, 256, 1u << 16);
+    if (cdf_lo) {
+        k_cutpoint_anchors<true><<<ga, 256, 0, s>>>(cdf_hi, cdf_lo, n, guide, m);
+        k_cutpoint_fill<true><<<gf, 256, 0, s>>>(cdf_hi, cdf_lo, n, guide, m);
+    } else {
+        k_cutpoint_anchors<false><<<ga, 256, 0, s>>>(cdf_hi, cdf_lo, n, guide, m);
+        k_cutpoint_fill<false><<<gf, 256, 0, s>>>(cdf_hi, cdf_lo, n, guide, m);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cutpoint_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                  const uint32_t* guide, uint64_t m, uint64_t s_count,
+                                  uint64_t seed, uint64_t counter_base, uint32_t* out,
+                                  hipStream_t s) {
+    const uint32_t g = grid_for(s_count, kSampleBlock, 1u << 16);
+    if (cdf_lo)
+        k_cutpoint_sample<true><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, guide, m, s_count,
+                                                           seed, counter_base, out);
+    else
+        k_cutpoint_sample<false><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, guide, m, s_count,
+                                                            seed, counter_base, out);
     return hipGetLastError();
 }
 

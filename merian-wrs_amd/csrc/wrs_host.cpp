@@ -219,6 +219,33 @@ void InverseTransformSampling::run(hipStream_t s, const Buffers& b, uint64_t N, 
               "its sample launch");
 }
 
+static void require_cutpoint(const float* hi, uint64_t N, const uint32_t* guide, uint64_t m) {
+    require(hi && guide, "null cutpoint buffer");
+    require(N > 0 && N < (1ull << 32), "N must be in [1, 2^32)");
+    require(m > 0 && m < (1ull << 32), "guide size must be in [1, 2^32)");
+}
+
+void Cutpoint::build(hipStream_t s, const Buffers& b, uint64_t N) const {
+    require(b.internals != nullptr, "null workspace");
+    m_scan.run(s, PrefixSum::Buffers{b.weights, b.cdf, b.cdfLo}, N, *b.internals);
+    const uint64_t m = guideSize(N);
+    require_cutpoint(b.cdf, N, b.guide, m);
+    const float* lo = wrsk::prec_has_lo(m_cfg.precision) ? b.cdfLo : nullptr;
+    check_hip(wrsk::launch_cutpoint_build(b.cdf, lo, N, b.guide, m, s), "guide launch");
+}
+
+void Cutpoint::sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed,
+                      uint64_t counterBase) const {
+    const uint64_t m = guideSize(N);
+    require_cutpoint(b.cdf, N, b.guide, m);
+    require(b.samples || S == 0, "null sampling buffer");
+    if (S == 0) return;
+    const float* lo = wrsk::prec_has_lo(m_cfg.precision) ? b.cdfLo : nullptr;
+    check_hip(wrsk::launch_cutpoint_sample(b.cdf, lo, N, b.guide, m, S, seed, counterBase,
+                                           b.samples, s),
+              "cutpoint sample launch");
+}
+
 void AliasTable::build(hipStream_t s, const Buffers& b, uint64_t N) const {
     m_psa.run(s, PSA::Buffers{b.weights, b.aliasTable, b.internals}, N);
 }
@@ -426,6 +453,28 @@ wrs_status wrs_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n, 
         wrs::InverseTransformSampling().run(
             hs(stream), wrs::InverseTransformSampling::Buffers{cdf_hi, cdf_lo, out}, n, s, seed,
             counter_base);
+    });
+}
+
+wrs_status wrs_cutpoint_build(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                              uint32_t* guide, uint64_t m, wrs_stream_t stream) {
+    return guarded([&] {
+        wrs::require_cutpoint(cdf_hi, n, guide, m);
+        wrs::check_hip(wrsk::launch_cutpoint_build(cdf_hi, cdf_lo, n, guide, m, hs(stream)),
+                       "guide launch");
+    });
+}
+
+wrs_status wrs_cutpoint_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                               const uint32_t* guide, uint64_t m, uint64_t s, uint64_t seed,
+                               uint64_t counter_base, uint32_t* out, wrs_stream_t stream) {
+    return guarded([&] {
+        wrs::require_cutpoint(cdf_hi, n, guide, m);
+        if (!out && s) throw wrs::Error(WRS_EINVAL, "out is null");
+        if (s == 0) return;
+        wrs::check_hip(wrsk::launch_cutpoint_sample(cdf_hi, cdf_lo, n, guide, m, s, seed,
+                                                    counter_base, out, hs(stream)),
+                       "cutpoint sample launch");
     });
 }
 

@@ -193,6 +193,38 @@ class ITS {
     InverseTransformSampling m_sampler;
 };
 
+// Cutpoint = normalised scan + guide table + guided sampling
+// (src/device/wrs/cutpoint/Cutpoint.hpp:22-150); guidingEntrySize weights per
+// guide entry, m = ceil(N / guidingEntrySize).
+struct CutpointConfig {
+    wrs_precision precision = WRS_PREC_COMPENSATED;
+    uint32_t guidingEntrySize = 1;
+};
+class Cutpoint {
+  public:
+    struct Buffers {
+        const float* weights;
+        float* cdf;
+        float* cdfLo;     // may be null
+        uint32_t* guide;  // [guideSize(N) + 1]
+        uint32_t* samples;
+        Workspace* internals;
+    };
+    explicit Cutpoint(CutpointConfig cfg) : m_cfg(cfg), m_scan(cfg.precision, true) {}
+    uint64_t guideSize(uint64_t N) const {
+        const uint64_t e = m_cfg.guidingEntrySize ? m_cfg.guidingEntrySize : 1u;
+        const uint64_t m = (N + e - 1) / e;
+        return m ? m : 1;
+    }
+    void build(hipStream_t s, const Buffers& b, uint64_t N) const;
+    void sample(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed = 12345u,
+                uint64_t counterBase = 0) const;
+
+  private:
+    CutpointConfig m_cfg;
+    PrefixSum m_scan;
+};
+
 // WRS facade: variant dispatch as in src/device/wrs/WRS.hpp:91-172.
 using WRSConfig = std::variant<PSAConfig /* alias */, wrs_precision /* ITS */>;
 class WRS {

@@ -27,7 +27,8 @@ SPLIT_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float64
 EXPORTS = [
     "wrs_plan_create", "wrs_plan_destroy", "wrs_psa_build", "wrs_psa_build_debug", "wrs_mean",
     "wrs_prefix_partition", "wrs_scan_inclusive", "wrs_its_build", "wrs_alias_sample",
-    "wrs_its_sample", "wrs_philox", "wrs_generate", "wrs_batch_plan_create",
+    "wrs_its_sample", "wrs_cutpoint_build", "wrs_cutpoint_sample", "wrs_philox", "wrs_generate",
+    "wrs_batch_plan_create",
     "wrs_batch_plan_destroy", "wrs_psa_build_batched", "wrs_alias_sample_batched",
     "wrs_autotune_psa", "wrs_autotune_batched",
     "wrs_last_error", "wrs_version", "wrs_plan_set_timing", "wrs_plan_phase_times",
@@ -107,6 +108,8 @@ def lib():
         "wrs_its_build": [vp, vp, u64, vp, vp, vp],
         "wrs_alias_sample": [vp, u64, u64, u64, u64, vp, vp],
         "wrs_its_sample": [vp, vp, u64, u64, u64, u64, vp, vp],
+        "wrs_cutpoint_build": [vp, vp, u64, vp, u64, vp],
+        "wrs_cutpoint_sample": [vp, vp, u64, vp, u64, u64, u64, u64, vp, vp],
         "wrs_philox": [u64, u32, u64, u64, vp, vp],
         "wrs_generate": [i32, f64, f64, u64, u64, vp, vp],
         "wrs_batch_plan_create": [vp, vp, u32, C.POINTER(vp)],
@@ -428,6 +431,17 @@ def its_sample(cdf_hi, cdf_lo, n: int, s: int, seed: int, out, counter_base: int
                stream=None) -> None:
     check(lib().wrs_its_sample(_p(cdf_hi), _p(cdf_lo), n, s, seed, counter_base, _p(out),
                                _h(stream)))
+
+
+def cutpoint_build(cdf_hi, cdf_lo, n: int, guide, m: int, stream=None) -> None:
+    """guide[0..m] over a normalised CDF (guide holds m + 1 uint32)."""
+    check(lib().wrs_cutpoint_build(_p(cdf_hi), _p(cdf_lo), n, _p(guide), m, _h(stream)))
+
+
+def cutpoint_sample(cdf_hi, cdf_lo, n: int, guide, m: int, s: int, seed: int, out,
+                    counter_base: int = 0, stream=None) -> None:
+    check(lib().wrs_cutpoint_sample(_p(cdf_hi), _p(cdf_lo), n, _p(guide), m, s, seed,
+                                    counter_base, _p(out), _h(stream)))
 
 
 def alias_sample_batched(tables, offsets_dev, nseg: int, per_segment: int, seed: int, out,
