@@ -19,12 +19,14 @@ Also reported on the same line:
                 WRITE_SIZE summary committed under profiles/ (null if absent);
   precision     the same build in all four arithmetic modes: time and the
                 implied distribution's max abs/rel error vs FP64;
-  sampling      alias sampling, S = 2^26 Philox samples;
+  sampling      alias sampling, S = 2^26 Philox samples, and their histogram's
+                pooled chi-square and RMSE ratio, computed on the device;
   its           ITS on N = 2^24 LogNormal(0, 2): normalised scan (8N bytes)
                 and S = 2^24 binary-search samples, CDF error vs the FP64 prefix;
                 the cutpoint guide table (N entries) over the same CDF and S = 2^24
                 guided samples (bit-identical to the ITS samples);
-  spike         N = 2^28 spike mixture build and S = 2^30 alias samples;
+  spike         N = 2^28 spike mixture build and S = 2^30 alias samples, with
+                the same on-device statistics of the 2^30 samples;
   batched       4096 log-uniform-size LogNormal(0, 3) distributions built and
                 sampled (64 each) by one hipGraph vs one launch sequence per
                 distribution;
@@ -95,6 +97,17 @@ def pmc_traffic(kernel: str):
         return None
     row = json.loads(m.group(1)).get("kernels", {}).get(kernel)
     return None if row is None else int(row["traffic_bytes"])
+
+
+def sample_statistics(st: dict) -> dict:
+    """On-device histogram statistics -> the reported figures (p-value by scipy)."""
+    from scipy import stats as sps
+
+    p = float(sps.chi2.sf(st["chi2"], st["df"])) if st["df"] > 0 else 1.0
+    return dict(chi2=round(st["chi2"], 2), df=st["df"], chi2_p=p,
+                rmse=st["rmse"], expected_rmse=st["expected_rmse"],
+                rmse_ratio=round(st["rmse"] / st["expected_rmse"], 5) if st["expected_rmse"] > 0 else None,
+                zero_mass_hits=st["zero_mass_hits"])
 
 
 def time_it(fn, st, wrs, reps: int) -> float:
@@ -211,7 +224,12 @@ def main() -> int:
     sample_ms = time_it(sample_once, st, wrs, 5)
     samples_per_s = S / (sample_ms * 1e-3)
     log(f"alias sampling S=2^{args.log2s}: {sample_ms:.4f} ms, {samples_per_s / 1e9:.2f} Gsamples/s")
-    del smp
+    # statistics of the last S samples, on the device (histogram, pooled chi-square, RMSE)
+    hist = wrs.empty(n + 1, np.uint64)
+    wrs.histogram(smp, S, n, hist, stream=st)
+    sstat = sample_statistics(plan.sample_stats(w, n, hist, S, stream=st))
+    log(f"sample statistics: {sstat}")
+    del smp, hist
 
     # ---- accuracy, and the precision study on the same weights -------------
     w_host = w.numpy()
@@ -288,11 +306,15 @@ def main() -> int:
         osp = wrs.empty(ss, np.uint32)
         wrs.alias_sample(tsp, ns, ss, 99, osp, stream=st)
         sps_ms = time_it(lambda: wrs.alias_sample(tsp, ns, ss, 99, osp, stream=st), st, wrs, 3)
+        hsp = wrs.empty(ns + 1, np.uint64)
+        wrs.histogram(osp, ss, ns, hsp, stream=st)
+        spstat = sample_statistics(splan.sample_stats(wsp, ns, hsp, ss, stream=st))
+        del hsp
         b_sp = 32 * ns + 12 * splan.split_count(ns)
         spike = dict(n=ns, build_ms=round(sp_ms, 4), build_gbs=round(b_sp / (sp_ms * 1e-3) / 1e9, 1),
                      frac_hbm_peak=round(b_sp / (sp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      samples=ss, sample_ms=round(sps_ms, 3),
-                     samples_per_s=round(ss / (sps_ms * 1e-3), 1))
+                     samples_per_s=round(ss / (sps_ms * 1e-3), 1), sample_statistics=spstat)
         log(f"spike: {spike}")
         del wsp, tsp, osp, splan
 
@@ -400,7 +422,8 @@ def main() -> int:
         "copy_kernel_gbs": round(copy_gbs, 1),
         "samples_per_s": round(samples_per_s, 1),
         "sampling": {"S": S, "ms": round(sample_ms, 4),
-                     "gbs_12B": round(12 * S / (sample_ms * 1e-3) / 1e9, 1)},
+                     "gbs_12B": round(12 * S / (sample_ms * 1e-3) / 1e9, 1),
+                     "statistics": sstat},
         "max_abs_error_vs_fp64": None if err is None else err["max_abs"],
         "max_rel_error_vs_fp64": None if err is None else err["max_rel"],
         "precision_study": study,

@@ -214,6 +214,29 @@ wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_
                                     uint32_t nseg, uint32_t per_segment, uint64_t seed,
                                     uint64_t counter_base, uint32_t* out, wrs_stream_t stream);
 
+/* ---- sample statistics (on the device) ----
+ * Histogram of s samples over n bins: hist has n + 1 uint64 counters, hist[n]
+ * counting samples >= n; clear != 0 zeroes them first, else the counts
+ * accumulate (several sampling passes).  Replaces the atomic histogram of the
+ * reference's statistics (src/device/statistics/histogram/atomic/shader.comp:1-38,
+ * Histogram.hpp). */
+wrs_status wrs_histogram(const uint32_t* samples, uint64_t s, uint64_t n, uint64_t* hist,
+                         int clear, wrs_stream_t stream);
+/* Statistics of a histogram of s samples against weights[n] (q = w / sum w),
+ * FP64 in a fixed order, written to out6 (device memory, 6 doubles):
+ *   [0] chi-square over pools of consecutive bins that each expect at least
+ *       min_expected samples (the last partial pool merged into the one before;
+ *       min_expected <= 0: one pool per bin, the reference's statistic)
+ *   [1] degrees of freedom (pools - 1)   [2] sum (hist[i]/s - q_i)^2
+ *   [3] its expectation for an exact sampler, sum q_i (1 - q_i) / s
+ *   [4] samples in bins of zero weight   [5] pools
+ * RMSE = sqrt([2] / n), its expectation sqrt([3] / n).  Uses the plan's
+ * normalised scan (n <= max_n).  Replaces ChiSquare / MeanSquaredError
+ * (src/device/statistics/chi_square/reduce/shader.comp:40-73, rmse/mse/shader.comp:52-77),
+ * whose float atomics make the sums order-dependent. */
+wrs_status wrs_sample_stats(wrs_plan* plan, const float* weights, uint64_t n, const uint64_t* hist,
+                            uint64_t s, double min_expected, double* out6, wrs_stream_t stream);
+
 /* ---- autotuner ----
  * Times candidate geometries (wrs_tuning) on the caller's own data with HIP
  * events and returns the fastest: first the batched partition form with the

@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WRS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["reduce.hip", "partition.hip", "scan.hip", "splitpack.hip", "mergepack.hip", "sample.hip",
-           "batched.cpp", "wrs_host.cpp", "tune.cpp"]
+           "stats.hip", "batched.cpp", "wrs_host.cpp", "tune.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so split/pack arithmetic is the
 # sequence of IEEE operations written (as in the CPU oracle).

@@ -151,6 +151,14 @@ hipError_t launch_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uin
 hipError_t launch_generate(int kind, double a, double b, uint64_t seed, uint64_t n, float* out,
                            hipStream_t s);
 hipError_t launch_fill_u64(uint64_t* p, uint64_t words, uint64_t value, hipStream_t s);
+// sample statistics: hist[0..n] (hist[n] counts samples >= n); stats over
+// tiles of 4096 bins with one 16-word record per tile in recs
+hipError_t launch_histogram(const uint32_t* samples, uint64_t s_count, uint64_t n,
+                            uint64_t* hist, int clear, hipStream_t s);
+uint64_t stats_tiles_for(uint64_t n);
+hipError_t launch_stats(const uint64_t* hist, const float* w, const float* cdf_hi,
+                        const float* cdf_lo, uint64_t n, uint64_t s_count, double min_expected,
+                        const Params* prm, uint64_t* recs, double* out, hipStream_t s);
 
 // segmented (batched) forms: items are (segment, tile) pairs, points are the
 // flattened split points of all segments

@@ -478,6 +478,35 @@ wrs_status wrs_cutpoint_sample(const float* cdf_hi, const float* cdf_lo, uint64_
     });
 }
 
+wrs_status wrs_histogram(const uint32_t* samples, uint64_t s, uint64_t n, uint64_t* hist,
+                         int clear, wrs_stream_t stream) {
+    return guarded([&] {
+        if (!hist || (!samples && s)) throw wrs::Error(WRS_EINVAL, "null histogram buffer");
+        if (n == 0 || n >= (1ull << 32)) throw wrs::Error(WRS_EINVAL, "n must be in [1, 2^32)");
+        wrs::check_hip(wrsk::launch_histogram(samples, s, n, hist, clear, hs(stream)),
+                       "histogram launch");
+    });
+}
+
+wrs_status wrs_sample_stats(wrs_plan* plan, const float* weights, uint64_t n, const uint64_t* hist,
+                            uint64_t s, double min_expected, double* out6, wrs_stream_t stream) {
+    return guarded([&] {
+        require_plan(plan);
+        if (!weights || !hist || !out6) throw wrs::Error(WRS_EINVAL, "null statistics buffer");
+        if (n == 0 || n > plan->ws.max_n) throw wrs::Error(WRS_EINVAL, "n must be in [1, max_n]");
+        if (s == 0) throw wrs::Error(WRS_EINVAL, "s must be positive");
+        wrs::Workspace& ws = plan->ws;
+        const bool lo = wrsk::prec_has_lo(plan->cfg.precision) && ws.prefix_lo;
+        wrs::PrefixSum(plan->cfg.precision, true)
+            .run(hs(stream), wrs::PrefixSum::Buffers{weights, ws.prefix_hi, lo ? ws.prefix_lo : nullptr},
+                 n, ws);
+        wrs::check_hip(wrsk::launch_stats(hist, weights, ws.prefix_hi, lo ? ws.prefix_lo : nullptr,
+                                          n, s, min_expected, ws.params, ws.states, out6,
+                                          hs(stream)),
+                       "statistics launch");
+    });
+}
+
 wrs_status wrs_philox(uint64_t seed, uint32_t stream_word, uint64_t base, uint64_t n,
                       uint32_t* out, wrs_stream_t stream) {
     return guarded([&] {

@@ -30,7 +30,7 @@ EXPORTS = [
     "wrs_its_sample", "wrs_cutpoint_build", "wrs_cutpoint_sample", "wrs_philox", "wrs_generate",
     "wrs_batch_plan_create",
     "wrs_batch_plan_destroy", "wrs_psa_build_batched", "wrs_alias_sample_batched",
-    "wrs_autotune_psa", "wrs_autotune_batched",
+    "wrs_autotune_psa", "wrs_autotune_batched", "wrs_histogram", "wrs_sample_stats",
     "wrs_last_error", "wrs_version", "wrs_plan_set_timing", "wrs_plan_phase_times",
     "wrs_rt_device_count", "wrs_rt_device_info", "wrs_rt_malloc", "wrs_rt_free",
     "wrs_rt_memcpy_htod", "wrs_rt_memcpy_dtoh", "wrs_rt_memcpy_dtod", "wrs_rt_memset",
@@ -120,6 +120,8 @@ def lib():
                              C.POINTER(TuneEntry), u32, C.POINTER(u32), vp],
         "wrs_autotune_batched": [C.POINTER(Config), vp, u32, vp, u32, C.POINTER(Tuning),
                                  C.POINTER(TuneEntry), u32, C.POINTER(u32), vp],
+        "wrs_histogram": [vp, u64, u64, vp, i32, vp],
+        "wrs_sample_stats": [vp, vp, u64, vp, u64, f64, vp, vp],
         "wrs_plan_set_timing": [vp, i32],
         "wrs_plan_phase_times": [vp, vp, u32],
         "wrs_rt_device_count": [C.POINTER(C.c_int)],
@@ -336,6 +338,20 @@ class Plan:
         check(lib().wrs_its_build(self.handle, _p(weights), n, _p(cdf_hi), _p(cdf_lo),
                                   _h(stream)))
 
+    def sample_stats(self, weights, n: int, hist, s: int, min_expected: float = 32.0,
+                     stream=None) -> dict:
+        """chi-square (pooled), squared error and zero-mass hits of a histogram
+        (wrs.histogram) of s samples against weights, computed on the device."""
+        out = empty(6, np.float64)
+        check(lib().wrs_sample_stats(self.handle, _p(weights), n, _p(hist), s,
+                                     float(min_expected), _p(out), _h(stream)))
+        synchronize()
+        v = out.numpy()
+        chi2, dof, sse, esse, zh, pools = v.tolist()
+        return dict(chi2=chi2, df=int(dof), sse=sse, expected_sse=esse, zero_mass_hits=int(zh),
+                    pools=int(pools), rmse=float(np.sqrt(sse / n)),
+                    expected_rmse=float(np.sqrt(esse / n)))
+
     def set_timing(self, enable: bool) -> None:
         check(lib().wrs_plan_set_timing(self.handle, int(bool(enable))))
 
@@ -431,6 +447,11 @@ def its_sample(cdf_hi, cdf_lo, n: int, s: int, seed: int, out, counter_base: int
                stream=None) -> None:
     check(lib().wrs_its_sample(_p(cdf_hi), _p(cdf_lo), n, s, seed, counter_base, _p(out),
                                _h(stream)))
+
+
+def histogram(samples, s: int, n: int, hist, clear: bool = True, stream=None) -> None:
+    """hist[0..n] (n + 1 uint64; hist[n] counts samples >= n)."""
+    check(lib().wrs_histogram(_p(samples), s, n, _p(hist), int(bool(clear)), _h(stream)))
 
 
 def cutpoint_build(cdf_hi, cdf_lo, n: int, guide, m: int, stream=None) -> None:
