@@ -24,19 +24,20 @@ namespace {
 // merge work lists of a merge tile: (segment, block) items and the first
 // boundary of every segment
 void merge_lists(const std::vector<uint64_t>& offsets, uint32_t nseg, uint32_t tile,
-                 std::vector<uint32_t>& mseg, std::vector<uint32_t>& mblk,
-                 std::vector<uint32_t>& b0, uint64_t& nbound) {
-    mseg.clear();
-    mblk.clear();
+                 std::vector<wrsk::SegItem>& mitem, std::vector<uint32_t>& b0,
+                 std::vector<uint32_t>& bseg, uint64_t& nbound) {
+    mitem.clear();
+    bseg.clear();
     b0.assign(nseg + 1, 0);
     uint64_t bnd = 0;
     for (uint32_t g = 0; g < nseg; ++g) {
-        const uint64_t blocks = wrsk::merge_blocks_for(offsets[g + 1] - offsets[g], tile);
+        const uint64_t n = offsets[g + 1] - offsets[g];
+        const uint64_t blocks = wrsk::merge_blocks_for(n, tile);
         b0[g] = (uint32_t)bnd;
-        for (uint64_t b = 0; b < blocks; ++b) {
-            mseg.push_back(g);
-            mblk.push_back((uint32_t)b);
-        }
+        for (uint64_t b = 0; b < blocks; ++b)
+            mitem.push_back(wrsk::SegItem{offsets[g], (uint32_t)n, g, (uint32_t)b,
+                                          (uint32_t)(bnd + b)});
+        for (uint64_t b = 0; b <= blocks; ++b) bseg.push_back(g);
         bnd += blocks + 1;
     }
     b0[nseg] = (uint32_t)bnd;
@@ -53,7 +54,7 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
     wrs::require_tuning(tuning_);
     offsets.assign(offsets_host, offsets_host + nseg + 1);
     if (offsets[0] != 0) throw wrs::Error(WRS_EINVAL, "offsets[0] must be 0");
-    std::vector<uint32_t> h_item_seg, h_item_tile;
+    std::vector<wrsk::SegItem> h_items;
     std::vector<uint32_t> h_seg_item0(nseg + 1);
     std::vector<uint64_t> h_seg_samp0(nseg + 1);
     uint64_t items64 = 0, samp = 0;
@@ -66,22 +67,22 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
             throw wrs::Error(WRS_EINVAL, "segment larger than 2^23 weights");
         h_seg_item0[g] = (uint32_t)items64;
         for (uint64_t t = 0; t < tiles; ++t) {
-            h_item_seg.push_back(g);
-            h_item_tile.push_back((uint32_t)t);
+            h_items.push_back(wrsk::SegItem{offsets[g], (uint32_t)n, g, (uint32_t)t, 0u});
         }
         items64 += tiles;
         h_seg_samp0[g] = samp;
         samp += wrsk::samples_for(n);
     }
     // capacity of the merge lists: the smallest merge tile has the most blocks
-    std::vector<uint32_t> ms_, mb_, b0_;
+    std::vector<wrsk::SegItem> mi_;
+    std::vector<uint32_t> b0_, bs_;
     uint64_t bnd_cap = 0;
-    merge_lists(offsets, nseg, wrsk::kMinMergeTile, ms_, mb_, b0_, bnd_cap);
+    merge_lists(offsets, nseg, wrsk::kMinMergeTile, mi_, b0_, bs_, bnd_cap);
     if (items64 >= (1ull << 31) || bnd_cap >= (1ull << 31)) throw wrs::Error(WRS_EINVAL, "too many tiles");
     h_seg_item0[nseg] = (uint32_t)items64;
     h_seg_samp0[nseg] = samp;
     items = (uint32_t)items64;
-    mitems_cap = (uint32_t)ms_.size();
+    mitems_cap = (uint32_t)mi_.size();
     nbound_cap = (uint32_t)bnd_cap;
     total_n = offsets[nseg];
 
@@ -89,10 +90,9 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
     seg_item0 = dalloc<uint32_t>(nseg + 1);
     seg_b0 = dalloc<uint32_t>(nseg + 1);
     seg_samp0 = dalloc<uint64_t>(nseg + 1);
-    item_seg = dalloc<uint32_t>(items);
-    item_tile = dalloc<uint32_t>(items);
-    mitem_seg = dalloc<uint32_t>(mitems_cap);
-    mitem_blk = dalloc<uint32_t>(mitems_cap);
+    item_d = dalloc<wrsk::SegItem>(items);
+    mitem_d = dalloc<wrsk::SegItem>(mitems_cap);
+    bound_seg = dalloc<uint32_t>(nbound_cap);
     seg_prm = dalloc<wrsk::Params>(nseg);
     partials = dalloc<double>(items);
     states = dalloc<uint64_t>((uint64_t)items * wrsk::kStateWords);
@@ -105,8 +105,7 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
     upload(seg_off, offsets);
     upload(seg_item0, h_seg_item0);
     upload(seg_samp0, h_seg_samp0);
-    upload(item_seg, h_item_seg);
-    upload(item_tile, h_item_tile);
+    upload(item_d, h_items);
     shape.tile = 0;  // forces the merge lists of the first tuning
     set_tuning(tuning_);
 }
@@ -115,13 +114,14 @@ void BatchPlan::set_tuning(const wrs_tuning& t) {
     wrs::require_tuning(t);
     const wrsk::MergeShape ms = wrs::merge_shape_of(t);
     if (ms.tile != shape.tile) {
-        std::vector<uint32_t> h_mseg, h_mblk, h_b0;
+        std::vector<wrsk::SegItem> h_mitem;
+        std::vector<uint32_t> h_b0, h_bseg;
         uint64_t bnd = 0;
-        merge_lists(offsets, nseg, ms.tile, h_mseg, h_mblk, h_b0, bnd);
-        upload(mitem_seg, h_mseg);
-        upload(mitem_blk, h_mblk);
+        merge_lists(offsets, nseg, ms.tile, h_mitem, h_b0, h_bseg, bnd);
+        upload(mitem_d, h_mitem);
         upload(seg_b0, h_b0);
-        mitems = (uint32_t)h_mseg.size();
+        upload(bound_seg, h_bseg);
+        mitems = (uint32_t)h_mitem.size();
         nbound = (uint32_t)bnd;
     }
     tuning = t;
@@ -129,19 +129,18 @@ void BatchPlan::set_tuning(const wrs_tuning& t) {
 }
 
 void BatchPlan::destroy() {
-    void* ptrs[] = {seg_off,   seg_item0, seg_b0, seg_samp0, item_seg, item_tile, mitem_seg,
-                    mitem_blk, seg_prm,   partials, states,  ticket,   idx,       pw,
-                    samp_h,    samp_l,    bounds};
+    void* ptrs[] = {seg_off, seg_item0, seg_b0, seg_samp0, item_d,  mitem_d, bound_seg,
+                    seg_prm, partials,  states, ticket,    idx,     pw,      samp_h,
+                    samp_l,  bounds};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     seg_off = nullptr;
     seg_item0 = nullptr;
     seg_b0 = nullptr;
     seg_samp0 = nullptr;
-    item_seg = nullptr;
-    item_tile = nullptr;
-    mitem_seg = nullptr;
-    mitem_blk = nullptr;
+    item_d = nullptr;
+    mitem_d = nullptr;
+    bound_seg = nullptr;
     seg_prm = nullptr;
     partials = nullptr;
     states = nullptr;
@@ -154,19 +153,19 @@ void BatchPlan::destroy() {
 }
 
 void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) const {
-    wrs::check_hip(wrsk::launch_seg_reduce(w, seg_off, item_seg, item_tile, items, prec, partials,
+    wrs::check_hip(wrsk::launch_seg_reduce(w, item_d, items, prec, partials,
                                            states, s),
                    "segmented reduce launch");
     wrs::check_hip(wrsk::launch_seg_finalize(seg_off, seg_item0, nseg, prec, partials, seg_prm,
                                              ticket, s),
                    "segmented finalize launch");
-    wrs::check_hip(wrsk::launch_seg_partition(w, seg_off, item_seg, item_tile, seg_item0, nseg,
+    wrs::check_hip(wrsk::launch_seg_partition(w, item_d, seg_item0, nseg,
                                               items, prec, (int)tuning.seg_partition, seg_prm,
                                               ticket, states, idx, pw, samp_h, samp_l, seg_samp0,
                                               s),
                    "segmented partition launch");
     wrs::check_hip(wrsk::launch_seg_mergepack(pw, idx, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
-                                              nseg, nbound, mitem_seg, mitem_blk, mitems, shape,
+                                              nseg, nbound, bound_seg, mitem_d, mitems, shape,
                                               prec, seg_prm, bounds, tables, s),
                    "segmented merge pack launch");
 }

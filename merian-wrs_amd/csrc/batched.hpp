@@ -34,10 +34,9 @@ struct BatchPlan {
     uint32_t* seg_item0 = nullptr;   // [nseg+1]
     uint32_t* seg_b0 = nullptr;      // [nseg+1] first boundary of each segment
     uint64_t* seg_samp0 = nullptr;   // [nseg+1] first prefix sample of each segment
-    uint32_t* item_seg = nullptr;    // [items]
-    uint32_t* item_tile = nullptr;   // [items]
-    uint32_t* mitem_seg = nullptr;   // [mitems]
-    uint32_t* mitem_blk = nullptr;   // [mitems]
+    wrsk::SegItem* item_d = nullptr;   // [items] (segment, tile) items
+    wrsk::SegItem* mitem_d = nullptr;  // [mitems] (segment, merge block) items
+    uint32_t* bound_seg = nullptr;   // [nbound] segment of each block boundary
     wrsk::Params* seg_prm = nullptr; // [nseg]
     double* partials = nullptr;      // [items]
     uint64_t* states = nullptr;      // [items * kStateWords]

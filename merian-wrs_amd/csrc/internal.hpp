@@ -49,6 +49,18 @@ struct Coarse {  // split (i lights, j heavies) at a merge-block boundary
 };
 static_assert(sizeof(Coarse) == 32, "Coarse layout");
 
+// One work item of a segmented launch: a (segment, tile) or a (segment, merge
+// block) pair, with what its kernel needs before its first data load, so that
+// the data loads wait for one lookup only.
+struct SegItem {
+    uint64_t off;  // first weight of the segment
+    uint32_t n;    // segment length
+    uint32_t seg;  // segment index
+    uint32_t blk;  // tile (or merge block) within the segment
+    uint32_t bnd;  // merge items: index of the block's first boundary
+};
+static_assert(sizeof(SegItem) == 24, "SegItem layout");
+
 struct SplitRec {  // debug output of the split stage
     uint32_t i;
     uint32_t j;
@@ -162,8 +174,7 @@ hipError_t launch_stats(const uint64_t* hist, const float* w, const float* cdf_h
 
 // segmented (batched) forms: items are (segment, tile) pairs, points are the
 // flattened split points of all segments
-hipError_t launch_seg_reduce(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
-                             const uint32_t* item_tile, uint32_t items, int prec,
+hipError_t launch_seg_reduce(const float* w, const SegItem* item_d, uint32_t items, int prec,
                              double* partials, uint64_t* states, hipStream_t s);
 hipError_t launch_seg_finalize(const uint64_t* seg_off, const uint32_t* seg_item0, uint32_t nseg,
                                int prec, const double* partials, Params* seg_prm,
@@ -171,8 +182,8 @@ hipError_t launch_seg_finalize(const uint64_t* seg_off, const uint32_t* seg_item
 // lookback != 0: one pass with decoupled look-back per segment (states reset
 // by the segmented reduce, ticket by the segmented finalize); else
 // reduce-then-scan (aggregate, per-segment spine, emit; no reset needed)
-hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
-                                const uint32_t* item_tile, const uint32_t* seg_item0, uint32_t nseg,
+hipError_t launch_seg_partition(const float* w, const SegItem* item_d, const uint32_t* seg_item0,
+                                uint32_t nseg,
                                 uint32_t items, int prec, int lookback, Params* seg_prm,
                                 uint32_t* ticket, uint64_t* states, uint32_t* idx, float* pw,
                                 double* samp_h, double* samp_l, const uint64_t* seg_samp0,
@@ -180,9 +191,9 @@ hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const u
 hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const double* samp_h,
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
-                                uint32_t nbound, const uint32_t* mitem_seg,
-                                const uint32_t* mitem_blk, uint32_t mitems, MergeShape ms,
-                                int prec, const Params* seg_prm, Coarse* bounds, AliasEntry* table,
+                                uint32_t nbound, const uint32_t* bound_seg,
+                                const SegItem* mitem_d, uint32_t mitems, MergeShape ms, int prec,
+                                const Params* seg_prm, Coarse* bounds, AliasEntry* table,
                                 hipStream_t s);
 hipError_t launch_seg_sample(const AliasEntry* tables, const uint64_t* seg_off, uint32_t nseg,
                              uint32_t per_segment, uint64_t seed, uint64_t counter_base,

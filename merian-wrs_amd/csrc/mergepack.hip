@@ -187,24 +187,19 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
     if (lane_id() == 0) out[b] = c;
 }
 
-// Segmented form: boundary p of the flattened boundary list belongs to
-// segment g = item_seg-like lookup over seg_b0 (nseg + 1 prefix).
+// Segmented form: boundary p of the flattened boundary list belongs to segment
+// bound_seg[p] (a list the batch plan derives from the offsets).
 template <int PREC>
 __global__ __launch_bounds__(256) void k_seg_coarse(
     const float* __restrict__ pw, const double* __restrict__ samp_h,
     const double* __restrict__ samp_l, const uint64_t* __restrict__ seg_off,
-    const uint64_t* __restrict__ seg_samp0, const uint32_t* __restrict__ seg_b0, uint32_t nseg,
-    const Params* __restrict__ seg_prm, uint32_t tile, Coarse* __restrict__ out) {
+    const uint64_t* __restrict__ seg_samp0, const uint32_t* __restrict__ seg_b0,
+    const uint32_t* __restrict__ bound_seg, uint32_t nbound, const Params* __restrict__ seg_prm,
+    uint32_t tile, Coarse* __restrict__ out) {
     using T = typename RealT<PREC>::T;
     const uint32_t p = blockIdx.x * 4u + threadIdx.x / kWave;
-    if (p >= seg_b0[nseg]) return;
-    uint32_t a = 0, b = nseg;  // last g with seg_b0[g] <= p
-    while (b - a > 1u) {
-        const uint32_t mid = (a + b) >> 1;
-        if (seg_b0[mid] <= p) a = mid;
-        else b = mid;
-    }
-    const uint32_t g = a;
+    if (p >= nbound) return;  // whole wave exits together
+    const uint32_t g = bound_seg[p];
     const uint64_t off = seg_off[g];
     const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
     const uint32_t n = min((p - seg_b0[g]) * tile, N);
@@ -479,21 +474,20 @@ __global__ __launch_bounds__(BLOCK) void k_mergepack(const float* __restrict__ p
 }
 
 template <int PREC, int TILE, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_seg_mergepack(
-    const float* __restrict__ pw, const uint32_t* __restrict__ idx,
-    const uint64_t* __restrict__ seg_off, const uint32_t* __restrict__ mitem_seg,
-    const uint32_t* __restrict__ mitem_blk, const uint32_t* __restrict__ seg_b0,
-    const Params* __restrict__ seg_prm, const Coarse* __restrict__ bounds,
-    AliasEntry* __restrict__ table) {
-    const uint32_t g = mitem_seg[blockIdx.x];
-    const uint32_t blk = mitem_blk[blockIdx.x];
-    const uint64_t off = seg_off[g];
-    const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
-    const uint32_t n0 = blk * (uint32_t)TILE;
+__global__ __launch_bounds__(BLOCK) void k_seg_mergepack(const float* __restrict__ pw,
+                                                         const uint32_t* __restrict__ idx,
+                                                         const SegItem* __restrict__ mitem_d,
+                                                         const Params* __restrict__ seg_prm,
+                                                         const Coarse* __restrict__ bounds,
+                                                         AliasEntry* __restrict__ table) {
+    const SegItem it = mitem_d[blockIdx.x];
+    const uint32_t N = it.n;
+    const uint32_t n0 = it.blk * (uint32_t)TILE;
     const uint32_t n1 = min(n0 + (uint32_t)TILE, N);
-    const Coarse* bd = bounds + seg_b0[g] + blk;
-    mergepack_block<PREC, TILE, BLOCK>(pw + off, idx + off, N, seg_prm[g].heavy_count, seg_prm[g].mean,
-                          seg_prm[g].total, n0, n1, bd[0], bd[1], table + off);
+    const Coarse* bd = bounds + it.bnd;
+    const Params* prm = seg_prm + it.seg;
+    mergepack_block<PREC, TILE, BLOCK>(pw + it.off, idx + it.off, N, prm->heavy_count, prm->mean,
+                                       prm->total, n0, n1, bd[0], bd[1], table + it.off);
 }
 
 // ---- boundary repair ---------------------------------------------------------
@@ -797,28 +791,29 @@ hipError_t launch_mergepack(const float* pw, uint64_t n, MergeShape ms, int prec
 hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const double* samp_h,
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
-                                uint32_t nbound, const uint32_t* mitem_seg,
-                                const uint32_t* mitem_blk, uint32_t mitems, MergeShape ms,
-                                int prec, const Params* seg_prm, Coarse* bounds, AliasEntry* table,
+                                uint32_t nbound, const uint32_t* bound_seg,
+                                const SegItem* mitem_d, uint32_t mitems, MergeShape ms, int prec,
+                                const Params* seg_prm, Coarse* bounds, AliasEntry* table,
                                 hipStream_t s) {
     if (!merge_shape_supported(ms)) return hipErrorInvalidValue;
     const uint32_t cgrid = (nbound + 3u) / 4u;
     if (prec == kFP32)
         k_seg_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
-                                                  nseg, seg_prm, ms.tile, bounds);
+                                                  bound_seg, nbound, seg_prm, ms.tile, bounds);
     else
         k_seg_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0,
-                                                       seg_b0, nseg, seg_prm, ms.tile, bounds);
+                                                       seg_b0, bound_seg, nbound, seg_prm, ms.tile,
+                                                       bounds);
     k_repair_bounds<<<nseg, kRepairThreads, 0, s>>>(bounds, seg_b0, seg_off, seg_samp0, seg_prm,
                                                     pw, samp_h, samp_l, ms.tile);
 #define WRS_SMP(T, P)                                                                           \
     if (ms.tile == (T) && ms.per == (P)) {                                                      \
         if (prec == kFP32)                                                                      \
             k_seg_mergepack<kFP32, T, T / P><<<mitems, T / P, 0, s>>>(                          \
-                pw, idx, seg_off, mitem_seg, mitem_blk, seg_b0, seg_prm, bounds, table);        \
+                pw, idx, mitem_d, seg_prm, bounds, table);                                      \
         else                                                                                    \
             k_seg_mergepack<kFP64Carry, T, T / P><<<mitems, T / P, 0, s>>>(                     \
-                pw, idx, seg_off, mitem_seg, mitem_blk, seg_b0, seg_prm, bounds, table);        \
+                pw, idx, mitem_d, seg_prm, bounds, table);                                      \
     }
     WRS_MERGE_SHAPES(WRS_SMP)
 #undef WRS_SMP

@@ -505,8 +505,7 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
 // the segment's own range of idx/pw.
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_seg_partition(
-    const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
-    const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile,
+    const float* __restrict__ w, const SegItem* __restrict__ item_d,
     const uint32_t* __restrict__ seg_item0, Params* __restrict__ seg_prm,
     uint32_t* __restrict__ ticket, uint64_t* __restrict__ states, uint32_t* __restrict__ idx,
     float* __restrict__ pw, double* __restrict__ samp_h, double* __restrict__ samp_l,
@@ -517,11 +516,12 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     if (threadIdx.x == 0) s_item = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t item = s_item;
-    const uint32_t sg = item_seg[item];
-    const uint64_t off = seg_off[sg];
-    const uint32_t n = (uint32_t)(seg_off[sg + 1] - off);
+    const SegItem it = item_d[item];
+    const uint32_t sg = it.seg;
+    const uint64_t off = it.off;
+    const uint32_t n = it.n;
     const uint64_t so = seg_samp0[sg];
-    const uint32_t tile = item_tile[item];
+    const uint32_t tile = it.blk;
     Params* prm = seg_prm + sg;
     uint64_t* seg_states = states + (uint64_t)seg_item0[sg] * kStateWords;
 
@@ -571,18 +571,18 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
 // both forms -- and the single-distribution build -- give the same bits.
 
 template <int PREC>
-__global__ __launch_bounds__(kBlock) void k_seg_part_agg(
-    const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
-    const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile,
-    const Params* __restrict__ seg_prm, uint64_t* __restrict__ states) {
+__global__ __launch_bounds__(kBlock) void k_seg_part_agg(const float* __restrict__ w,
+                                                         const SegItem* __restrict__ item_d,
+                                                         const Params* __restrict__ seg_prm,
+                                                         uint64_t* __restrict__ states) {
     __shared__ TileShared<PREC> s;
     const uint32_t item = blockIdx.x;
-    const uint32_t sg = item_seg[item];
-    const uint64_t off = seg_off[sg];
-    const uint32_t n = (uint32_t)(seg_off[sg + 1] - off);
-    const uint32_t base = item_tile[item] * (uint32_t)kTile;
+    const SegItem it = item_d[item];
+    const uint64_t off = it.off;
+    const uint32_t n = it.n;
+    const uint32_t base = it.blk * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
-    const Params* prm = seg_prm + sg;
+    const Params* prm = seg_prm + it.seg;
     float v[kItems];
     Acc<PREC> thr_excl, tile_agg;
     tile_scan<PREC>(w + off, base, m, prm->pivot, s, v, thr_excl, tile_agg);
@@ -649,18 +649,18 @@ __global__ __launch_bounds__(kBlock) void k_seg_part_spine(const uint32_t* __res
 
 template <int PREC>
 __global__ __launch_bounds__(kBlock) void k_seg_part_emit(
-    const float* __restrict__ w, const uint64_t* __restrict__ seg_off,
-    const uint32_t* __restrict__ item_seg, const uint32_t* __restrict__ item_tile, uint32_t items,
+    const float* __restrict__ w, const SegItem* __restrict__ item_d, uint32_t items,
     const Params* __restrict__ seg_prm, const uint64_t* __restrict__ states,
     uint32_t* __restrict__ idx, float* __restrict__ pw, double* __restrict__ samp_h,
     double* __restrict__ samp_l, const uint64_t* __restrict__ seg_samp0) {
     __shared__ TileShared<PREC> s;
     const uint32_t item = blockIdx.x;
-    const uint32_t sg = item_seg[item];
-    const uint64_t off = seg_off[sg];
-    const uint32_t n = (uint32_t)(seg_off[sg + 1] - off);
+    const SegItem it = item_d[item];
+    const uint32_t sg = it.seg;
+    const uint64_t off = it.off;
+    const uint32_t n = it.n;
     const uint64_t so = seg_samp0[sg];
-    const uint32_t base = item_tile[item] * (uint32_t)kTile;
+    const uint32_t base = it.blk * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const Params* prm = seg_prm + sg;
     const float pivot = prm->pivot;
@@ -713,8 +713,8 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, u
     return hipGetLastError();
 }
 
-hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
-                                const uint32_t* item_tile, const uint32_t* seg_item0, uint32_t nseg,
+hipError_t launch_seg_partition(const float* w, const SegItem* item_d, const uint32_t* seg_item0,
+                                uint32_t nseg,
                                 uint32_t items, int prec, int lookback, Params* seg_prm,
                                 uint32_t* ticket, uint64_t* states, uint32_t* idx, float* pw,
                                 double* samp_h, double* samp_l, const uint64_t* seg_samp0,
@@ -722,15 +722,14 @@ hipError_t launch_seg_partition(const float* w, const uint64_t* seg_off, const u
     const uint32_t spine_blocks = (nseg + kBlock / kWave - 1) / (kBlock / kWave);
 #define WRS_LAUNCH_SEG(P)                                                                        \
     if (lookback) {                                                                              \
-        k_seg_partition<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_item0,  \
+        k_seg_partition<P><<<items, kBlock, 0, s>>>(w, item_d, seg_item0,                        \
                                                     seg_prm, ticket, states, idx, pw, samp_h,    \
                                                     samp_l, seg_samp0);                          \
     } else {                                                                                     \
-        k_seg_part_agg<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, seg_prm,     \
-                                                   states);                                      \
+        k_seg_part_agg<P><<<items, kBlock, 0, s>>>(w, item_d, seg_prm, states);                  \
         k_seg_part_spine<P><<<spine_blocks, kBlock, 0, s>>>(seg_item0, nseg, items, seg_prm,     \
                                                             states);                             \
-        k_seg_part_emit<P><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, items,      \
+        k_seg_part_emit<P><<<items, kBlock, 0, s>>>(w, item_d, items,                            \
                                                     seg_prm, states, idx, pw, samp_h, samp_l,    \
                                                     seg_samp0);                                  \
     }

@@ -146,17 +146,13 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
 // tile its own block -- identical partials.
 template <bool FP64>
 __global__ __launch_bounds__(kBlock) void k_seg_reduce(const float* __restrict__ w,
-                                                       const uint64_t* __restrict__ seg_off,
-                                                       const uint32_t* __restrict__ item_seg,
-                                                       const uint32_t* __restrict__ item_tile,
+                                                       const SegItem* __restrict__ item_d,
                                                        double* __restrict__ partials,
                                                        uint64_t* __restrict__ states) {
     const uint32_t item = blockIdx.x;
-    const uint32_t sg = item_seg[item];
-    const uint64_t off = seg_off[sg];
-    const uint64_t n = seg_off[sg + 1] - off;
+    const SegItem it = item_d[item];
     AccT<FP64> acc = 0;
-    reduce_tile<FP64>(w + off, n, item_tile[item], acc);
+    reduce_tile<FP64>(w + it.off, it.n, it.blk, acc);
     if (threadIdx.x < kStateWords) states[(uint64_t)item * kStateWords + threadIdx.x] = kSentinel;
     const AccT<FP64> b = block_sum<FP64>(acc);
     if (threadIdx.x == 0) partials[item] = (double)b;
@@ -199,13 +195,12 @@ hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int p
     return hipGetLastError();
 }
 
-hipError_t launch_seg_reduce(const float* w, const uint64_t* seg_off, const uint32_t* item_seg,
-                             const uint32_t* item_tile, uint32_t items, int prec,
+hipError_t launch_seg_reduce(const float* w, const SegItem* item_d, uint32_t items, int prec,
                              double* partials, uint64_t* states, hipStream_t s) {
     if (prec != kFP32)
-        k_seg_reduce<true><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, partials, states);
+        k_seg_reduce<true><<<items, kBlock, 0, s>>>(w, item_d, partials, states);
     else
-        k_seg_reduce<false><<<items, kBlock, 0, s>>>(w, seg_off, item_seg, item_tile, partials, states);
+        k_seg_reduce<false><<<items, kBlock, 0, s>>>(w, item_d, partials, states);
     return hipGetLastError();
 }
 
