@@ -333,7 +333,7 @@ def main() -> int:
             btune, btune_log = wrs.autotune_batched(off, bw, precision=args.precision,
                                                     split_size=args.split_size, stream=st)
             log(f"autotune (batched): {btune}; candidates (ms): " + ", ".join(
-                f"{e['seg_partition']}/{e['merge_tile']}/{e['merge_per']}: {e['ms']:.4f}"
+                f"{e['partition']}/{e['merge_tile']}/{e['merge_per']}: {e['ms']:.4f}"
                 for e in btune_log))
         bp = wrs.BatchPlan(off, args.split_size, args.precision, tune=btune)
         tb = wrs.empty(tot, wrs.ALIAS_DTYPE)
@@ -362,7 +362,7 @@ def main() -> int:
                        batched_graph_ms=round(batched_ms, 4), per_distribution_ms=round(each_ms, 3),
                        speedup=round(each_ms / batched_ms, 2), identical_tables_and_samples=same,
                        tuning=btune, tuning_candidates_ms=[
-                           [e["seg_partition"], e["merge_tile"], e["merge_per"], round(e["ms"], 4)]
+                           [e["partition"], e["merge_tile"], e["merge_per"], round(e["ms"], 4)]
                            for e in btune_log],
                        batched_build_gbs=round((32 * tot + 12 * int(np.sum(-(-sizes // args.split_size))))
                                                / (batched_ms * 1e-3) / 1e9, 1))

@@ -70,8 +70,8 @@ typedef struct {
                                1024, 2048 (default) or 4096 */
     uint32_t merge_per;     /* buckets per merge-pack thread: 2, 4 (default) or 8
                                (merge_tile 4096: 4 or 8) */
-    uint32_t seg_partition; /* batched prefix-partition: 0 reduce-then-scan (default),
-                               1 single pass with per-segment decoupled look-back */
+    uint32_t partition;     /* prefix-partition form: 0 reduce-then-scan (default), 1 single
+                               pass with decoupled look-back (per segment in a batch) */
     uint32_t reserved;      /* 0 */
 } wrs_tuning;
 
@@ -239,7 +239,7 @@ wrs_status wrs_sample_stats(wrs_plan* plan, const float* weights, uint64_t n, co
 
 /* ---- autotuner ----
  * Times candidate geometries (wrs_tuning) on the caller's own data with HIP
- * events and returns the fastest: first the batched partition form with the
+ * events and returns the fastest: first the prefix-partition form with the
  * default merge shape, then every supported merge shape with the better form
  * (coordinate descent; each candidate the median of 5 builds after one
  * warm-up).  Results are cached per process under a key of device, precision

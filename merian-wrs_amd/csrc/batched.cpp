@@ -160,7 +160,7 @@ void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) c
                                              ticket, s),
                    "segmented finalize launch");
     wrs::check_hip(wrsk::launch_seg_partition(w, item_d, seg_item0, nseg,
-                                              items, prec, (int)tuning.seg_partition, seg_prm,
+                                              items, prec, (int)tuning.partition, seg_prm,
                                               ticket, states, idx, pw, samp_h, samp_l, seg_samp0,
                                               s),
                    "segmented partition launch");

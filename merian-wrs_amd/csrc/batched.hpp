@@ -21,7 +21,7 @@ namespace wrsb {
 struct BatchPlan {
     uint32_t split_size = 32;
     int prec = 2;
-    wrs_tuning tuning{};         // seg_partition: 1 single pass with look-back, 0 reduce-then-scan
+    wrs_tuning tuning{};         // partition: 1 single pass with look-back, 0 reduce-then-scan
     wrsk::MergeShape shape;      // merge shape of the tuning
     uint32_t nseg = 0;
     uint32_t items = 0;          // (segment, tile) work items of reduce and partition

@@ -118,10 +118,12 @@ hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int p
 // idx is always written; pw (the weights in partition order), hi/lo (full
 // prefix) and the prefix samples only when non-null.  ev_agg / ev_scan, when
 // non-null, are recorded after the tile-aggregate and tile-scan kernels.
-hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
-                            uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
-                            double* samp_l, hipStream_t s, hipEvent_t ev_agg = nullptr,
-                            hipEvent_t ev_scan = nullptr);
+// lookback != 0: the single-pass look-back form (one kernel), else
+// reduce-then-scan (aggregate, tile scan, emit)
+hipError_t launch_partition(const float* w, uint64_t n, int prec, int lookback, Params* prm,
+                            uint64_t* states, uint32_t* idx, float* pw, float* hi, float* lo,
+                            double* samp_h, double* samp_l, hipStream_t s,
+                            hipEvent_t ev_agg = nullptr, hipEvent_t ev_scan = nullptr);
 // exact splits at every kMergeTile buckets, then the LDS merge-path pack
 hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, const Params* prm,
                          const double* samp_h, const double* samp_l, Coarse* bounds,

@@ -497,6 +497,61 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
                           hi_out, lo_out, samp_h, samp_l);
 }
 
+// ---- single distribution: single pass with decoupled look-back --------------
+// The reference's form (prefix_partition/decoupled/shader.comp:96-362): tiles
+// taken in ticket order, each publishes its aggregate, looks back over its
+// predecessors' records for its exclusive prefix and emits.  One kernel
+// instead of three, which pays where the chain of tiles is short; the tuner
+// chooses between the forms.  Records and ticket are reset by the reduce and
+// finalize passes.
+template <int PREC, bool FULL>
+__global__ __launch_bounds__(kBlock) void k_part_lookback(
+    const float* __restrict__ w, uint32_t n, Params* __restrict__ prm,
+    uint64_t* __restrict__ states, uint32_t* __restrict__ idx_out, float* __restrict__ pw_out,
+    float* __restrict__ hi_out, float* __restrict__ lo_out, double* __restrict__ samp_h,
+    double* __restrict__ samp_l) {
+    __shared__ TileShared<PREC> s;
+    __shared__ uint32_t s_tile;
+    __shared__ Carry s_excl;
+    if (threadIdx.x == 0) s_tile = atomicAdd(&prm->ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const float pivot = prm->pivot;
+    const int shift = prm->shift;
+    float v[kItems];
+    Acc<PREC> thr_excl, tile_agg;
+    tile_scan<PREC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
+
+    const int lane = threadIdx.x & (kWave - 1);
+    if (threadIdx.x < kWave) {
+        uint64_t* rec = states + (uint64_t)tile * kStateWords;
+        const Carry agg = to_carry<PREC>(tile_agg, shift);
+        Carry excl = carry_zero();
+        if (tile == 0) {
+            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, agg);
+        } else {
+            if (lane == 0) publish_aggregate<Carry, CarryOps<PREC>>(rec, agg);
+            excl = lookback<Carry, CarryOps<PREC>>(states, tile);
+            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, carry_add<PREC>(excl, agg));
+        }
+        if (lane == 0) {
+            s_excl = excl;
+            if (base + m == n) {
+                const Carry tot = carry_add<PREC>(excl, agg);
+                prm->heavy_count = (uint32_t)tot.c;
+                prm->heavy_sum = carry_value<PREC>(tot.h, shift);
+                prm->light_sum = carry_value<PREC>(tot.l, shift);
+            }
+        }
+    }
+    __syncthreads();
+    const TileCarry cy = tile_carry<PREC>(s_excl, shift);
+    tile_emit<PREC, FULL>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, pw_out,
+                          hi_out, lo_out, samp_h, samp_l);
+}
+
 // ---- segmented: single pass with decoupled look-back -----------------------
 
 // One work item per (segment, tile), items of a segment are consecutive and
@@ -677,10 +732,21 @@ __global__ __launch_bounds__(kBlock) void k_seg_part_emit(
 namespace {
 
 template <int P>
-void launch_part_t(const float* w, uint32_t n, Params* prm, uint64_t* states, uint32_t* idx,
-                   float* pw, float* hi, float* lo, double* samp_h, double* samp_l,
-                   hipStream_t s, hipEvent_t ev_agg, hipEvent_t ev_scan) {
+void launch_part_t(const float* w, uint32_t n, int lookback, Params* prm, uint64_t* states,
+                   uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
+                   double* samp_l, hipStream_t s, hipEvent_t ev_agg, hipEvent_t ev_scan) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
+    if (lookback) {  // one kernel: the aggregate and tile-scan phases are empty
+        if (ev_agg) (void)hipEventRecord(ev_agg, s);
+        if (ev_scan) (void)hipEventRecord(ev_scan, s);
+        if (hi)
+            k_part_lookback<P, true><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, hi, lo,
+                                                             samp_h, samp_l);
+        else
+            k_part_lookback<P, false><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw,
+                                                              nullptr, nullptr, samp_h, samp_l);
+        return;
+    }
     k_part_agg<P><<<tiles, kBlock, 0, s>>>(w, n, prm, states);
     if (ev_agg) (void)hipEventRecord(ev_agg, s);
     k_part_tilescan<P><<<(tiles + kSpineTiles - 1) / kSpineTiles, kSpineThreads, 0, s>>>(
@@ -696,13 +762,14 @@ void launch_part_t(const float* w, uint32_t n, Params* prm, uint64_t* states, ui
 
 }  // namespace
 
-hipError_t launch_partition(const float* w, uint64_t n, int prec, Params* prm, uint64_t* states,
-                            uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
-                            double* samp_l, hipStream_t s, hipEvent_t ev_agg,
+hipError_t launch_partition(const float* w, uint64_t n, int prec, int lookback, Params* prm,
+                            uint64_t* states, uint32_t* idx, float* pw, float* hi, float* lo,
+                            double* samp_h, double* samp_l, hipStream_t s, hipEvent_t ev_agg,
                             hipEvent_t ev_scan) {
     const uint32_t nn = (uint32_t)n;
-#define WRS_PART(P) \
-    launch_part_t<P>(w, nn, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ev_agg, ev_scan)
+#define WRS_PART(P)                                                                            \
+    launch_part_t<P>(w, nn, lookback, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ev_agg, \
+                     ev_scan)
     switch (prec) {
         case kFP32: WRS_PART(kFP32); break;
         case kFP64Carry: WRS_PART(kFP64Carry); break;

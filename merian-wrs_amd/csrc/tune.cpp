@@ -2,8 +2,8 @@
 // build or of a batched build by timing candidates on the caller's data.
 //
 // Search: coordinate descent over the two independent parts of a build.
-//   1. batched only: the prefix-partition form (reduce-then-scan or single
-//      pass with per-segment look-back), at the default merge shape;
+//   1. the prefix-partition form (reduce-then-scan or single pass with
+//      decoupled look-back, per segment in a batch), at the default merge shape;
 //   2. every supported merge shape (buckets per block x buckets per thread)
 //      with the partition form of step 1.
 // A candidate's time is the median of kReps builds (HIP events on the
@@ -13,7 +13,7 @@
 //
 // Results are cached per process under a key made of the device (name, CU
 // count), the precision and the problem shape; with WRS_TUNE_CACHE set they
-// also persist in that file, one "key<TAB>tile per seg_partition" line each.
+// also persist in that file, one "key<TAB>tile per partition" line each.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -63,7 +63,7 @@ void load_file_locked() {
         if (tab == std::string::npos) continue;
         std::istringstream vals(line.substr(tab + 1));
         wrs_tuning t{};
-        if (vals >> t.merge_tile >> t.merge_per >> t.seg_partition) {
+        if (vals >> t.merge_tile >> t.merge_per >> t.partition) {
             try {
                 wrs::require_tuning(t);
             } catch (...) {
@@ -89,7 +89,7 @@ void cache_put(const std::string& key, const wrs_tuning& t) {
     g_cache[key] = t;
     if (const char* path = cache_path()) {
         std::ofstream out(path, std::ios::app);
-        if (out) out << key << '\t' << t.merge_tile << ' ' << t.merge_per << ' ' << t.seg_partition << '\n';
+        if (out) out << key << '\t' << t.merge_tile << ' ' << t.merge_per << ' ' << t.partition << '\n';
     }
 }
 
@@ -150,7 +150,33 @@ wrs_tuning with_shape(wrs_tuning t, const wrsk::MergeShape& ms) {
 
 }  // namespace
 
-// PSA build of one distribution of n weights: the merge shapes.
+// Coordinate descent over a measuring function: the partition form at the
+// default merge shape, then every other merge shape with the better form.
+template <class M>
+wrs_tuning descend(M&& measure) {
+    const wrs_tuning t0 = with_shape(wrs_tuning{}, kShapes[0]);
+    wrs_tuning best = t0;
+    float best_ms = measure(t0);
+    wrs_tuning t1 = t0;
+    t1.partition = 1;
+    const float ms1 = measure(t1);
+    if (ms1 < best_ms) {
+        best_ms = ms1;
+        best = t1;
+    }
+    const wrs_tuning form = best;
+    for (size_t i = 1; i < sizeof(kShapes) / sizeof(kShapes[0]); ++i) {
+        const wrs_tuning t = with_shape(form, kShapes[i]);
+        const float ms = measure(t);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    return best;
+}
+
+// PSA build of one distribution of n weights.
 wrs_tuning tune_psa(const wrs::PSAConfig& base, const float* w, uint64_t n, uint32_t flags,
                     hipStream_t s, Log& log) {
     std::ostringstream key;
@@ -163,19 +189,15 @@ wrs_tuning tune_psa(const wrs::PSAConfig& base, const float* w, uint64_t n, uint
     try {
         wrs::check_hip(hipMalloc(&table, n * sizeof(wrs_alias_entry)), "hipMalloc");
         Events ev;
-        float best_ms = INFINITY;
-        for (const auto& ms : kShapes) {
+        best = descend([&](const wrs_tuning& t) {
             wrs::PSAConfig c = base;
-            c.tuning = with_shape(wrs_tuning{}, ms);
+            c.tuning = t;
             const wrs::PSA psa(c);
-            const float t = time_median([&] { psa.run(s, wrs::PSA::Buffers{w, table, &ws}, n); },
-                                        s, ev);
-            log.add(c.tuning, t);
-            if (t < best_ms) {
-                best_ms = t;
-                best = c.tuning;
-            }
-        }
+            const float ms =
+                time_median([&] { psa.run(s, wrs::PSA::Buffers{w, table, &ws}, n); }, s, ev);
+            log.add(t, ms);
+            return ms;
+        });
     } catch (...) {
         if (table) (void)hipFree(table);
         ws.release();
@@ -212,33 +234,12 @@ wrs_tuning tune_batched(const wrs::PSAConfig& base, const uint64_t* offsets, uin
         bp.create(base.splitSize, (int)base.precision, wrs_tuning{}, offsets, nseg);
         wrs::check_hip(hipMalloc(&tables, bp.total_n * sizeof(wrsk::AliasEntry)), "hipMalloc");
         Events ev;
-        auto measure = [&](const wrs_tuning& t) {
+        best = descend([&](const wrs_tuning& t) {
             bp.set_tuning(t);
             const float ms = time_median([&] { bp.build(w, tables, s); }, s, ev);
             log.add(t, ms);
             return ms;
-        };
-        // step 1: partition form at the default merge shape
-        const wrs_tuning t0 = with_shape(wrs_tuning{}, kShapes[0]);
-        float best_ms = measure(t0);
-        best = t0;
-        wrs_tuning t1 = t0;
-        t1.seg_partition = 1;
-        const float ms1 = measure(t1);
-        if (ms1 < best_ms) {
-            best_ms = ms1;
-            best = t1;
-        }
-        // step 2: merge shapes (the default one is measured already)
-        const wrs_tuning form = best;
-        for (size_t i = 1; i < sizeof(kShapes) / sizeof(kShapes[0]); ++i) {
-            const wrs_tuning t = with_shape(form, kShapes[i]);
-            const float ms = measure(t);
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = t;
-            }
-        }
+        });
     } catch (...) {
         if (tables) (void)hipFree(tables);
         bp.destroy();

@@ -93,7 +93,7 @@ wrsk::MergeShape merge_shape_of(const wrs_tuning& t) {
 void require_tuning(const wrs_tuning& t) {
     require(wrsk::merge_shape_supported(merge_shape_of(t)),
             "unsupported merge shape (merge_tile 1024/2048 with merge_per 2/4/8, or 4096 with 4/8)");
-    require(t.seg_partition <= 1u, "seg_partition must be 0 or 1");
+    require(t.partition <= 1u, "partition must be 0 or 1");
     require(t.reserved == 0u, "wrs_tuning.reserved must be 0");
 }
 
@@ -113,7 +113,7 @@ void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, d
 void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) const {
     require(b.elements && b.partitionIndices && b.partitionPrefix, "null partition buffer");
     require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
-    check_hip(wrsk::launch_partition(b.elements, N, (int)m_prec, ws.params, ws.states,
+    check_hip(wrsk::launch_partition(b.elements, N, (int)m_prec, 0, ws.params, ws.states,
                                      b.partitionIndices, nullptr, b.partitionPrefix,
                                      wrsk::prec_has_lo(m_prec) ? b.partitionPrefixLo : nullptr,
                                      nullptr, nullptr, s),
@@ -158,7 +158,8 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     check_hip(wrsk::launch_finalize(ws.partials, G, N, (int)prec, 0, 0.0, ws.params, s),
               "finalize launch");
     if (ev) check_hip(hipEventRecord(ev[2], s), "event");
-    check_hip(wrsk::launch_partition(b.weights, N, (int)prec, ws.params, ws.states, ws.indices,
+    check_hip(wrsk::launch_partition(b.weights, N, (int)prec, (int)m_cfg.tuning.partition,
+                                     ws.params, ws.states, ws.indices,
                                      ws.pweights, hi, lo, ws.samp_h, ws.samp_l, s,
                                      ev ? ev[3] : nullptr, ev ? ev[4] : nullptr),
               "partition launch");

@@ -38,8 +38,32 @@ def test_merge_shape_matches_oracle(wrs, oracle, kind, shape):
     assert err["max_rel"] <= 1e-6
 
 
-@pytest.mark.parametrize("tune", [dict(seg_partition=1), dict(merge_tile=1024, merge_per=8),
-                                  dict(seg_partition=1, merge_tile=4096, merge_per=4)])
+@pytest.mark.parametrize("n", [1, 4097, (1 << 20) + 5])
+@pytest.mark.parametrize("precision", [1, 2, 3])
+def test_lookback_partition_same_build(wrs, oracle, precision, n):
+    """The single-pass look-back partition gives the bits of the reduce-then-scan
+    form: exact carries, the same tile arithmetic."""
+    case("look-back partition", precision, n)
+    w = oracle.gen_zipf(2, 1.1, n)
+    dw = wrs.to_device(w)
+    out = []
+    for form in (0, 1):
+        plan = wrs.Plan(n, precision=precision, tune=dict(partition=form))
+        table = wrs.empty(n, wrs.ALIAS_DTYPE)
+        dbg = dict(indices=wrs.empty(n, np.uint32), prefix_hi=wrs.empty(n, np.float32),
+                   prefix_lo=wrs.empty(n, np.float32), heavy_count=wrs.empty(1, np.uint32))
+        plan.psa_build_debug(dw, n, table, dbg)
+        wrs.synchronize()
+        out.append((table.numpy(), {k: v.numpy() for k, v in dbg.items()}))
+    assert np.array_equal(out[0][0], out[1][0])
+    for k in out[0][1]:
+        if k == "prefix_lo" and precision < 2:
+            continue
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
+
+
+@pytest.mark.parametrize("tune", [dict(partition=1), dict(merge_tile=1024, merge_per=8),
+                                  dict(partition=1, merge_tile=4096, merge_per=4)])
 def test_batched_tuning_equals_per_segment(wrs, oracle, tune):
     nseg, per = 150, 64
     case("batched tuning", tune)
@@ -67,8 +91,11 @@ def test_autotune_psa_measures_then_caches(wrs, oracle):
     w = oracle.gen_zipf(2, 1.1, n)
     dw = wrs.to_device(w)
     best, log = wrs.autotune_psa(dw, n, no_cache=True)
-    print("  best", best, [(e["merge_tile"], e["merge_per"], round(e["ms"], 4)) for e in log])
-    assert len(log) == len(SHAPES)
+    print("  best", best, [(e["partition"], e["merge_tile"], e["merge_per"], round(e["ms"], 4))
+                           for e in log])
+    # partition form at the default shape (2), then the 7 other shapes
+    assert len(log) == 2 + len(SHAPES) - 1
+    assert {e["partition"] for e in log[:2]} == {0, 1}
     assert {(e["merge_tile"], e["merge_per"]) for e in log} == set(SHAPES)
     assert all(e["ms"] > 0 for e in log)
     assert (best["merge_tile"], best["merge_per"]) in SHAPES
@@ -91,11 +118,11 @@ def test_autotune_batched_measures_then_caches(wrs, oracle):
     sizes, off, w = make_batch(wrs, oracle, nseg, 1 << 15, seed=10)
     dw = wrs.to_device(w)
     best, log = wrs.autotune_batched(off, dw, no_cache=True)
-    print("  best", best, [(e["seg_partition"], e["merge_tile"], e["merge_per"],
+    print("  best", best, [(e["partition"], e["merge_tile"], e["merge_per"],
                             round(e["ms"], 4)) for e in log])
     # partition form at the default shape (2), then the 7 other shapes
     assert len(log) == 2 + len(SHAPES) - 1
-    assert {e["seg_partition"] for e in log[:2]} == {0, 1}
+    assert {e["partition"] for e in log[:2]} == {0, 1}
     assert (best["merge_tile"], best["merge_per"]) in SHAPES
     best2, log2 = wrs.autotune_batched(off, dw)
     assert best2 == best and log2 == []
