@@ -27,6 +27,9 @@ Also reported on the same line:
                 guided samples (bit-identical to the ITS samples);
   spike         N = 2^28 spike mixture build and S = 2^30 alias samples, with
                 the same on-device statistics of the 2^30 samples;
+  sweep         S = 1e7 samples at N = 1e5 .. 1e8 (U(0,1) weights), the shape of
+                the reference's sampling plots: alias build + sample, ITS and
+                cutpoint sampling throughput;
   batched       4096 log-uniform-size LogNormal(0, 3) distributions built and
                 sampled (64 each) by one hipGraph vs one launch sequence per
                 distribution;
@@ -160,8 +163,9 @@ def main() -> int:
     if not args.no_tune:
         tune, tune_log = wrs.autotune_psa(w, n, precision=args.precision,
                                           split_size=args.split_size, stream=st)
-        log(f"autotune (PSA, N=2^{args.log2n}): {tune}; candidates (ms): "
-            + ", ".join(f"{e['merge_tile']}/{e['merge_per']}: {e['ms']:.4f}" for e in tune_log))
+        log(f"autotune (PSA, N=2^{args.log2n}): {tune}; candidates (ms, partition/tile/per): "
+            + ", ".join(f"{e['partition']}/{e['merge_tile']}/{e['merge_per']}: {e['ms']:.4f}"
+                        for e in tune_log))
     plan = wrs.Plan(n, args.split_size, args.precision, tune=tune)
     K = plan.split_count(n)
     table = wrs.empty(n, wrs.ALIAS_DTYPE)
@@ -369,6 +373,45 @@ def main() -> int:
         log(f"batched: {batched}")
         del bw, tb, sb, te, se
 
+    # ---- sampling throughput over N, as the reference plots it -------------
+    # S = 1e7 samples from U(0,1) weights at N = 1e5 .. 1e8: alias build + sample
+    # (S / (t_build + t_sample), its_sampling_throughput_with_construction), ITS
+    # binary search and cutpoint sampling (S / t_sample, its_sampling_throughput)
+    sweep = None
+    if not args.no_extras:
+        sweep = []
+        Ss = 10_000_000
+        osw = wrs.empty(Ss, np.uint32)
+        for nn in (100_000, 1_000_000, 10_000_000, 100_000_000):
+            wsw = wrs.empty(nn, np.float32)
+            wrs.generate(wrs.GEN_UNIFORM01, nn, 0, wsw, stream=st)
+            st.synchronize()
+            tsw_tune = None
+            if not args.no_tune:
+                tsw_tune, _ = wrs.autotune_psa(wsw, nn, precision=args.precision,
+                                               split_size=args.split_size, stream=st)
+            pl = wrs.Plan(nn, args.split_size, args.precision, tune=tsw_tune)
+            tsw = wrs.empty(nn, wrs.ALIAS_DTYPE)
+            pl.psa_build(wsw, nn, tsw, stream=st)
+            b_ms = time_it(lambda: pl.psa_build(wsw, nn, tsw, stream=st), st, wrs, 5)
+            a_ms = time_it(lambda: wrs.alias_sample(tsw, nn, Ss, 5, osw, stream=st), st, wrs, 5)
+            hi_ = wrs.empty(nn, np.float32)
+            lo_ = wrs.empty(nn, np.float32)
+            pl.its_build(wsw, nn, hi_, lo_, stream=st)
+            i_ms = time_it(lambda: wrs.its_sample(hi_, lo_, nn, Ss, 5, osw, stream=st), st, wrs, 3)
+            gd_ = wrs.empty(nn + 1, np.uint32)
+            wrs.cutpoint_build(hi_, lo_, nn, gd_, nn, stream=st)
+            c_ms = time_it(lambda: wrs.cutpoint_sample(hi_, lo_, nn, gd_, nn, Ss, 5, osw, stream=st),
+                           st, wrs, 3)
+            row = dict(n=nn, psa_build_ms=round(b_ms, 4), alias_sample_ms=round(a_ms, 4),
+                       alias_build_plus_sample_per_s=round(Ss / ((b_ms + a_ms) * 1e-3), 1),
+                       its_sample_per_s=round(Ss / (i_ms * 1e-3), 1),
+                       cutpoint_sample_per_s=round(Ss / (c_ms * 1e-3), 1))
+            sweep.append(row)
+            log(f"sweep: {row}")
+            del wsw, tsw, hi_, lo_, gd_, pl
+        del osw
+
     # ---- CPU baseline: the reference CPU path restated (oracle) ------------
     cpu = None
     if not args.no_cpu_baseline:
@@ -409,8 +452,8 @@ def main() -> int:
             "n": n, "split_size": args.split_size, "B_build_bytes": B_build,
             "parallelism": "single GPU",
             "tuning": tune,
-            "tuning_candidates_ms": [[e["merge_tile"], e["merge_per"], round(e["ms"], 4)]
-                                     for e in tune_log],
+            "tuning_candidates_ms": [[e["partition"], e["merge_tile"], e["merge_per"],
+                                      round(e["ms"], 4)] for e in tune_log],
         },
         "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
         "lower_bound_12N_gbs": round(12 * n / (ms_step * 1e-3) / 1e9, 2),
@@ -430,6 +473,7 @@ def main() -> int:
         "its": its,
         "spike": spike,
         "batched": batched,
+        "sampling_sweep_S1e7": sweep,
         "cpu_baseline": cpu,
         "device": info["name"] or info["arch"],
     }
