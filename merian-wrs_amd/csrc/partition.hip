@@ -10,10 +10,11 @@
 //
 // MI355X design.  A tile is 4096 elements handled by one 256-thread block
 // (4 wave64s):
-//   * coalesced float4 loads, transposed through LDS (row stride 17 words, so
-//     both the transpose writes and the per-thread reads avoid bank
-//     conflicts) so that each thread owns 16 consecutive elements;
-//   * per-thread sequential sums (FP64 in the default mode, FP32 otherwise),
+//   * coalesced float4 rows at any input alignment (tileload.hpp), transposed
+//     through LDS (row stride 17 words, so both the transpose writes and the
+//     per-thread reads avoid bank conflicts) so that each thread owns 16
+//     consecutive elements;
+//   * per-thread sequential sums in the mode's tile arithmetic (tilesum.hpp),
 //     wave64 shuffle scan, then a 4-entry cross-wave scan in LDS, all in a
 //     fixed order;
 //   * tile carries are 126-bit fixed-point sums (exact, so their bits do not
@@ -21,18 +22,19 @@
 //   * outputs are staged in LDS in final order (heavies by rank, lights by
 //     reversed rank) and written as two contiguous coalesced runs.
 //
-// Single distribution: reduce-then-scan.  k_part_agg writes every tile's
-// aggregate, k_part_tilescan scans the aggregates, and k_part_emit re-reads
-// the tile and writes the outputs.  The 8 XCDs of the chip have
-// separate L2 caches, so a single-pass chained scan pays a device-coherent
-// round trip per look-back step on every tile; one extra streaming read of
-// the weights is cheaper than that at large N.
+// Two forms, for a single distribution and per segment of a batch (the tuner
+// picks one per problem):
+//   * reduce-then-scan: the aggregate of every tile, a scan of the
+//     aggregates, then an emit pass that re-reads the tile.  The 8 XCDs of
+//     the chip have separate L2 caches, so a single-pass chained scan pays a
+//     device-coherent round trip per look-back step on every tile; one extra
+//     streaming read of the weights is cheaper than that at large N;
+//   * single pass with decoupled look-back (the reference's form): one kernel,
+//     better where the chain of tiles is short.
 //
-// Segmented (batched) form: one pass with decoupled look-back per segment
-// (segments are short, so the chains are short).
-//
-// Both forms compute identical tile aggregates and carries, so a segment built
-// in a batch is bit-identical to the same distribution built alone.
+// All forms compute identical tile aggregates and carries, so they give the
+// same bits, and a segment built in a batch is bit-identical to the same
+// distribution built alone.
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "lookback.hpp"

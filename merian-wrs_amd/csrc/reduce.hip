@@ -5,8 +5,8 @@
 // element is accumulated in FP64 (exact conversion of the float) in an order
 // fixed by N alone: block b sums tiles b, b+G, b+2G, ...; within a tile thread t
 // sums the elements 4(t+256j)+q, j = 0..3, q = 0..3 in that order (the same order
-// whether the rows are loaded as float4 or as scalars, so the result does not
-// depend on pointer alignment); waves reduce by a fixed butterfly and the
+// whether the rows come straight from memory or through LDS, so the result does
+// not depend on pointer alignment); waves reduce by a fixed butterfly and the
 // finalize kernel sums the G partials in a fixed tree.  The same launch also
 // resets the look-back records of the kernel that follows it.
 //
