@@ -163,9 +163,9 @@ def main() -> int:
     if not args.no_tune:
         tune, tune_log = wrs.autotune_psa(w, n, precision=args.precision,
                                           split_size=args.split_size, stream=st)
-        log(f"autotune (PSA, N=2^{args.log2n}): {tune}; candidates (ms, partition/tile/per): "
-            + ", ".join(f"{e['partition']}/{e['merge_tile']}/{e['merge_per']}: {e['ms']:.4f}"
-                        for e in tune_log))
+        log(f"autotune (PSA, N=2^{args.log2n}): {tune}; candidates (ms, partition/direct/tile/per): "
+            + ", ".join(f"{e['partition']}/{e['direct']}/{e['merge_tile']}/{e['merge_per']}: "
+                        f"{e['ms']:.4f}" for e in tune_log))
     plan = wrs.Plan(n, args.split_size, args.precision, tune=tune)
     K = plan.split_count(n)
     table = wrs.empty(n, wrs.ALIAS_DTYPE)
@@ -189,19 +189,36 @@ def main() -> int:
         acc += np.array(plan.phase_times())
     plan.set_timing(False)
     phase = dict(zip(wrs.Plan.PHASES, (acc / args.steps).tolist()))
-    # bytes each kernel must move (the merge pack reads the index and the
-    # partition-ordered weight of every rank and writes the 8-byte table; the
-    # emit reads w and writes index + weight; aggregate and reduce read w)
-    alg = {"reduce": 4 * n, "finalize": 0, "tile_aggregate": 4 * n, "tile_scan": 0,
-           "partition_emit": 12 * n, "boundary_splits": 0, "merge_pack": 16 * n}
-    # kernel instances (template arguments: precision, merge shape) as rocprofv3 names them
+    # heavy count H of these weights (one debug build)
+    hbuf = wrs.empty(1, np.uint32)
+    plan.psa_build_debug(w, n, table, dict(heavy_count=hbuf), stream=st)
+    st.synchronize()
+    H = int(hbuf.numpy()[0])
     p, t_ = args.precision, (tune or {})
+    direct = bool(t_.get("direct"))
+    lookback = bool(t_.get("partition"))
+    # bytes each kernel must move.  Classic form: the emit reads w and writes
+    # index + weight of every rank, the merge pack reads them back and writes
+    # the 8-byte table.  Direct form: the emit reads w and writes the heavies'
+    # index + weight and every 64th light's position, the merge pack reads w,
+    # the heavies' index + weight and writes the table.  Aggregate and reduce
+    # read w.
+    alg = {"reduce": 4 * n, "finalize": 0, "tile_aggregate": 4 * n, "tile_scan": 0,
+           "partition_emit": (4 * n + 8 * H + 4 * ((n - H) // 64)) if direct else 12 * n,
+           "boundary_splits": 0,
+           "merge_pack": (12 * n + 8 * H) if direct else 16 * n}
+    # kernel instances (template arguments: precision, forms, merge shape) as
+    # rocprofv3 names them
     mt = t_.get("merge_tile") or 2048
     mp = t_.get("merge_per") or 4
+    d_ = "true" if direct else "false"
     kname = {"reduce": f"k_reduce<{'false' if p == 0 else 'true'}>",
              "tile_aggregate": f"k_part_agg<{p}>",
-             "partition_emit": f"k_part_emit<{p}, false>",
-             "merge_pack": f"k_mergepack<{0 if p == 0 else 1}, {mt}, {mt // mp}>"}
+             "partition_emit": (f"k_part_lookback<{p}, false, {d_}>" if lookback
+                                else f"k_part_emit<{p}, false, {d_}>"),
+             "merge_pack": f"k_mergepack<{0 if p == 0 else 1}, {mt}, {mt // mp}, {d_}>"}
+    if lookback:  # one kernel: the aggregate pass is empty
+        del kname["tile_aggregate"]
     log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phase.items()))
     dom = max(kname, key=lambda k: phase[k])
     dom_ms = phase[dom]
@@ -452,8 +469,8 @@ def main() -> int:
             "n": n, "split_size": args.split_size, "B_build_bytes": B_build,
             "parallelism": "single GPU",
             "tuning": tune,
-            "tuning_candidates_ms": [[e["partition"], e["merge_tile"], e["merge_per"],
-                                      round(e["ms"], 4)] for e in tune_log],
+            "tuning_candidates_ms": [[e["partition"], e["direct"], e["merge_tile"],
+                                      e["merge_per"], round(e["ms"], 4)] for e in tune_log],
         },
         "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
         "lower_bound_12N_gbs": round(12 * n / (ms_step * 1e-3) / 1e9, 2),

@@ -45,7 +45,8 @@ struct AliasEntry {
 struct Coarse {  // split (i lights, j heavies) at a merge-block boundary
     uint32_t i, j;
     double lp, hp;  // LP(i), HP(j)
-    uint64_t pad;   // scratch of the segmented boundary repair
+    uint32_t xi;    // direct form: element index of light i (N when i == L)
+    uint32_t aux;   // scratch of the segmented boundary repair
 };
 static_assert(sizeof(Coarse) == 32, "Coarse layout");
 
@@ -119,23 +120,36 @@ hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int p
 // prefix) and the prefix samples only when non-null.  ev_agg / ev_scan, when
 // non-null, are recorded after the tile-aggregate and tile-scan kernels.
 // lookback != 0: the single-pass look-back form (one kernel), else
-// reduce-then-scan (aggregate, tile scan, emit)
+// reduce-then-scan (aggregate, tile scan, emit).  lpos non-null (and hi
+// null): the direct form -- idx/pw get only the heavies ([0, H)) and
+// lpos[r / kSample] the element index of every light rank r % kSample == 0.
 hipError_t launch_partition(const float* w, uint64_t n, int prec, int lookback, Params* prm,
                             uint64_t* states, uint32_t* idx, float* pw, float* hi, float* lo,
-                            double* samp_h, double* samp_l, hipStream_t s,
+                            double* samp_h, double* samp_l, uint32_t* lpos, hipStream_t s,
                             hipEvent_t ev_agg = nullptr, hipEvent_t ev_scan = nullptr);
 // exact splits at every kMergeTile buckets, then the LDS merge-path pack
+// The lights of a direct-form build: the partition arrays idx/pw hold the
+// heavies only, lights are read from the weights w and located through
+// lpos[k] = element index of light rank k * kSample.  lpos null: the classic
+// form (lights at idx/pw[N - 1 - rank]).
+struct LightSrc {
+    const float* w = nullptr;
+    const uint32_t* lpos = nullptr;
+};
+inline uint64_t lightpos_for(uint64_t n) { return n / kSample + 1; }
+
 hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, const Params* prm,
                          const double* samp_h, const double* samp_l, Coarse* bounds,
-                         hipStream_t s);
+                         hipStream_t s, const uint32_t* idx = nullptr, LightSrc ls = {});
 // makes the boundary splits monotone (no-op for exact keys); states are the
 // build's look-back records (state_words_for(n) words, reset by the reduce)
 hipError_t launch_repair_bounds(Coarse* bounds, uint64_t n, MergeShape ms, Params* prm,
                                 uint64_t* states, const float* pw, const double* samp_h,
-                                const double* samp_l, hipStream_t s);
+                                const double* samp_l, hipStream_t s,
+                                const uint32_t* idx = nullptr, LightSrc ls = {});
 hipError_t launch_mergepack(const float* pw, uint64_t n, MergeShape ms, int prec,
                             const Params* prm, const uint32_t* idx, const Coarse* bounds,
-                            AliasEntry* table, hipStream_t s);
+                            AliasEntry* table, hipStream_t s, LightSrc ls = {});
 // the K split points S(k), k = 1..K, of the reference's splitK layout (debug output)
 hipError_t launch_splits(uint64_t n, uint32_t K, int prec, const Params* prm, const float* hi,
                          const float* lo, SplitRec* splits, hipStream_t s);

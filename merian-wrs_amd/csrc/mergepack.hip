@@ -93,21 +93,51 @@ __device__ __forceinline__ T wave_excl_scan(T v) {
     return l == 0 ? (T)0 : ex;
 }
 
+// Direct form: element index of the light of rank r < L.  It is r plus the
+// heavies before it; heavy h (element idx[h], heavies in element order) has
+// f(h) = idx[h] - h lights before it, f is non-decreasing, and heavy h comes
+// before light r iff f(h) <= r.  lpos[k] = x(k kSample) brackets the count to
+// [lpos[k] - k kSample, lpos[k+1] - (k+1) kSample] (H after the last
+// sampled light), searched by bisection (no steps where no heavy lies between
+// the two sampled lights).
+__device__ __forceinline__ uint32_t light_pos(const uint32_t* __restrict__ idx,
+                                              const uint32_t* __restrict__ lpos, uint32_t r,
+                                              uint32_t L, uint32_t H) {
+    const uint32_t k = r / kSample;
+    const uint32_t r1 = (k + 1u) * kSample;
+    // heavies before light k kSample and before light (k+1) kSample (at most H)
+    uint32_t a = min(lpos[k] - k * kSample, H);
+    uint32_t b = r1 < L ? min(lpos[k + 1u] - r1, H) : H;
+    while (a < b) {                               // first h in [a, b) with f(h) > r
+        const uint32_t mid = a + (b - a) / 2u;
+        if (idx[mid] - mid <= r) a = mid + 1u;
+        else b = mid;
+    }
+    return r + a;
+}
+
 }  // namespace
 
 // The exact split at n buckets of the distribution (pw, samples) of size N
-// with H heavies; executed by one full wave, result valid in every lane.
-template <int PREC>
+// with H heavies; executed by one full wave, result valid in every lane.  The
+// direct form reads the lights from ls (and the heavy indices idx to locate
+// them) and records the element index of the split's light.
+template <int PREC, bool DIRECT>
 __device__ __forceinline__ Coarse coarse_split(const float* __restrict__ pw,
                                                const double* __restrict__ samp_h,
                                                const double* __restrict__ samp_l, uint32_t N,
                                                uint32_t H, typename RealT<PREC>::T mean,
-                                               typename RealT<PREC>::T light_sum, uint32_t n) {
+                                               typename RealT<PREC>::T light_sum, uint32_t n,
+                                               const uint32_t* __restrict__ idx,
+                                               const LightSrc& ls) {
     using T = typename RealT<PREC>::T;
     const uint32_t L = N - H;
     const int lane = lane_id();
-    if (n == 0 || H == 0 || L == 0) return Coarse{0u, 0u, 0.0, 0.0, 0};
-    if (n >= N) return Coarse{L, H - 1u, (double)light_sum, 0.0, 0};
+    if (n == 0 || H == 0 || L == 0) {
+        const uint32_t x0 = DIRECT && H != 0 && L != 0 ? ls.lpos[0] : 0u;  // light 0
+        return Coarse{0u, 0u, 0.0, 0.0, x0, 0u};
+    }
+    if (n >= N) return Coarse{L, H - 1u, (double)light_sum, 0.0, N, 0u};
     const T target = mean * (T)n;
     const uint32_t lo = n > L ? n - L : 0u;
     const uint32_t hi = min(n, H - 1u);
@@ -146,7 +176,16 @@ __device__ __forceinline__ Coarse coarse_split(const float* __restrict__ pw,
     const uint32_t anchor = min(i_top, L);
     const T lp_anchor = lp_sample<T>(samp_l, anchor, L, light_sum);
     const int64_t lrank = (int64_t)anchor - 1 - lane;                  // lane e: light rank anchor-1-e
-    const T lw = lrank >= 0 ? (T)pw[N - 1u - (uint32_t)lrank] : (T)0;
+    T lw = (T)0;
+    uint32_t lx = 0;  // direct form: element index of light rank lrank
+    if (lrank >= 0) {
+        if constexpr (DIRECT) {
+            lx = light_pos(idx, ls.lpos, (uint32_t)lrank, L, H);
+            lw = (T)ls.w[lx];
+        } else {
+            lw = (T)pw[N - 1u - (uint32_t)lrank];
+        }
+    }
     const T lsuf = wave_excl_scan<T>(lw);  // sum of light ranks [anchor - e, anchor)
     const int64_t i = (int64_t)n - (int64_t)j;
     // every lane takes part in the shuffle; e is clamped for lanes that do not use it
@@ -169,21 +208,31 @@ __device__ __forceinline__ Coarse coarse_split(const float* __restrict__ pw,
     const T lp_t = __shfl(lp, t, kWave);
     const T hp_t = __shfl(hp, t, kWave);
     const uint32_t js = base + (uint32_t)t;
-    return Coarse{n - js, js, (double)lp_t, (double)hp_t, 0};
+    const uint32_t is = n - js;  // in [anchor - 63, anchor]
+    uint32_t xs = 0;
+    if constexpr (DIRECT) {
+        // lane anchor - 1 - is located light is, or it is the anchor itself
+        int64_t sel = (int64_t)anchor - 1 - (int64_t)is;
+        sel = sel < 0 ? 0 : (sel > kWave - 1 ? kWave - 1 : sel);
+        const uint32_t xl = __shfl(lx, (int)sel, kWave);
+        xs = is == anchor ? (anchor < L ? ls.lpos[anchor / kSample] : N) : xl;
+    }
+    return Coarse{is, js, (double)lp_t, (double)hp_t, xs, 0u};
 }
 
-template <int PREC>
+template <int PREC, bool DIRECT>
 __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 const double* __restrict__ samp_h,
                                                 const double* __restrict__ samp_l, uint32_t N,
                                                 const Params* __restrict__ prm, uint32_t nbound,
-                                                uint32_t tile, Coarse* __restrict__ out) {
+                                                uint32_t tile, Coarse* __restrict__ out,
+                                                const uint32_t* __restrict__ idx, LightSrc ls) {
     using T = typename RealT<PREC>::T;
     const uint32_t b = blockIdx.x * 4u + threadIdx.x / kWave;
     if (b >= nbound) return;  // whole wave exits together
     const uint32_t n = min(b * tile, N);
-    const Coarse c = coarse_split<PREC>(pw, samp_h, samp_l, N, prm->heavy_count,
-                                        (T)prm->mean, (T)prm->light_sum, n);
+    const Coarse c = coarse_split<PREC, DIRECT>(pw, samp_h, samp_l, N, prm->heavy_count,
+                                                (T)prm->mean, (T)prm->light_sum, n, idx, ls);
     if (lane_id() == 0) out[b] = c;
 }
 
@@ -204,9 +253,9 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
     const uint32_t n = min((p - seg_b0[g]) * tile, N);
     const uint64_t so = seg_samp0[g];
-    const Coarse c = coarse_split<PREC>(pw + off, samp_h + so, samp_l + so, N,
-                                        seg_prm[g].heavy_count, (T)seg_prm[g].mean,
-                                        (T)seg_prm[g].light_sum, n);
+    const Coarse c = coarse_split<PREC, false>(pw + off, samp_h + so, samp_l + so, N,
+                                               seg_prm[g].heavy_count, (T)seg_prm[g].mean,
+                                               (T)seg_prm[g].light_sum, n, nullptr, LightSrc{});
     if (lane_id() == 0) out[p] = c;
 }
 
@@ -276,6 +325,110 @@ struct MergeStage {
         }
     }
 };
+
+// Direct-form staging of one block (the same slots as MergeStage): heavies
+// j0..j1 from idx/pw; the nl lights i0..i1-1 are the lights (!(w > pivot), the
+// partition's test) of the element range [x0, x1) = [c0.xi, c1.xi), in
+// element order.  A range of at most kRounds * BLOCK elements (every block
+// of a distribution whose heavies do not cluster) is loaded whole, one round
+// of BLOCK consecutive elements after the other, and compacted by a block
+// scan of the per-(round, wave) light counts; a longer range (a run of
+// heavies between two of the block's lights) locates each light by
+// light_pos.  s_cnt holds kRounds * BLOCK / kWave counts.  Ends with the slots
+// written, not synchronised.
+template <int TILE, int BLOCK>
+__device__ __forceinline__ void stage_direct(MergeShared<TILE, BLOCK>& s,
+                                             uint32_t* __restrict__ s_cnt,
+                                             const float* __restrict__ pw,
+                                             const uint32_t* __restrict__ idx,
+                                             const LightSrc& ls, float pivot, uint32_t N,
+                                             uint32_t H, const Coarse& c0, const Coarse& c1) {
+    constexpr int kPer = TILE / BLOCK;
+    constexpr int kRounds = 2 * kPer;
+    constexpr int kWaves = BLOCK / kWave;
+    constexpr int kCounts = kRounds * kWaves;
+    const uint32_t t = threadIdx.x;
+    const int lane = lane_id();
+    const int wid = (int)t / kWave;
+    const uint32_t L = N - H;
+    const uint32_t nl = c1.i - c0.i, nh = c1.j - c0.j;
+    const uint32_t x1 = min(c1.xi, N), x0 = min(c0.xi, x1), span = x1 - x0;
+    const bool dense = span <= (uint32_t)(kRounds * BLOCK);
+    // Rounds past the heavies or past the range are skipped as a whole (the
+    // conditions are block-uniform): a block usually holds a few heavies and
+    // about TILE elements of range.
+    // heavy h in [0, nh] goes to slot nl + h (the weight of heavy nh is not needed)
+    uint32_t gi[kPer + 1];
+    float gw[kPer + 1];
+#pragma unroll
+    for (int k = 0; k <= kPer; ++k) {
+        if ((uint32_t)BLOCK * k <= nh) {
+            const uint32_t h = t + (uint32_t)BLOCK * k;
+            const uint32_t pos = h <= nh ? c0.j + h : 0u;
+            gi[k] = idx[pos];
+            gw[k] = pw[pos];
+        }
+    }
+    float lw[kRounds];
+    if (dense) {
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            if ((uint32_t)BLOCK * k < span) {
+                const uint32_t o = t + (uint32_t)BLOCK * k;  // offset in the range
+                lw[k] = o < span ? ls.w[x0 + o] : 0.0f;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k <= kPer; ++k) {
+        if ((uint32_t)BLOCK * k <= nh) {
+            const uint32_t h = t + (uint32_t)BLOCK * k;
+            s.idx[sw4(h <= nh ? nl + h : (uint32_t)TILE + 1u)] = gi[k];
+            s.w[sw4(h < nh ? nl + h : (uint32_t)TILE)] = gw[k];
+        }
+    }
+    if (dense) {
+        uint64_t bal[kRounds];
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            bal[k] = 0;
+            if ((uint32_t)BLOCK * k < span) {
+                const uint32_t o = t + (uint32_t)BLOCK * k;
+                bal[k] = __ballot(o < span && !(lw[k] > pivot));
+            }
+            if (lane == 0) s_cnt[k * kWaves + wid] = (uint32_t)__popcll(bal[k]);
+        }
+        __syncthreads();
+        if (wid == 0) {  // exclusive scan of the counts in (round, wave) order
+            uint32_t carry = 0;
+            for (int q0 = 0; q0 < kCounts; q0 += kWave) {
+                const int q = q0 + lane;
+                const uint32_t v = q < kCounts ? s_cnt[q] : 0u;
+                const uint32_t incl = wave_incl_u(v);
+                if (q < kCounts) s_cnt[q] = carry + incl - v;
+                carry += shfl(incl, kWave - 1);
+            }
+        }
+        __syncthreads();
+        const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int k = 0; k < kRounds; ++k) {
+            if ((bal[k] >> lane) & 1ull) {
+                const uint32_t e = s_cnt[k * kWaves + wid] + (uint32_t)__popcll(bal[k] & below);
+                if (e < nl) {
+                    s.idx[sw4(e)] = x0 + t + (uint32_t)BLOCK * k;
+                    s.w[sw4(e)] = lw[k];
+                }
+            }
+        }
+    } else {
+        for (uint32_t e = t; e < nl; e += BLOCK) {
+            const uint32_t x = light_pos(idx, ls.lpos, c0.i + e, L, H);
+            s.idx[sw4(e)] = x;
+            s.w[sw4(e)] = ls.w[x];
+        }
+    }
+}
 
 // Keys and merge of one block whose slots are staged in s (block
 // synchronised): leaves the entry of slot e at s.raw[sw8(e)], e < ne, and
@@ -428,12 +581,14 @@ __device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, double m
 // threads, TILE / BLOCK buckets per thread.  Rows are written with
 // consecutive lanes on consecutive slots (consecutive light indices).  The
 // last heavy of all gets (1, itself), as in pack.hpp:78-81.
-template <int PREC, int TILE, int BLOCK>
+template <int PREC, int TILE, int BLOCK, bool DIRECT = false>
 __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
                                                 const uint32_t* __restrict__ idx, uint32_t N,
                                                 uint32_t H, double mean, double total,
                                                 uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
-                                                AliasEntry* __restrict__ table) {
+                                                AliasEntry* __restrict__ table,
+                                                const LightSrc& ls = LightSrc{},
+                                                float pivot = 0.0f) {
     static_assert(TILE % BLOCK == 0 && BLOCK % kWave == 0 && TILE % kSample == 0, "merge shape");
     __shared__ MergeShared<TILE, BLOCK> s;
     const uint32_t t = threadIdx.x;
@@ -445,7 +600,10 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     const uint32_t nl = c1.i - c0.i;  // lights packed here
     const uint32_t nh = c1.j - c0.j;  // heavies packed here (heavy c1.j is current at n1)
     const uint32_t ne = nl + nh;      // == n1 - n0 (- 1 in the last block)
-    {
+    if constexpr (DIRECT) {
+        __shared__ uint32_t s_cnt[2 * TILE / kWave];
+        stage_direct<TILE, BLOCK>(s, s_cnt, pw, idx, ls, pivot, N, H, c0, c1);
+    } else {
         MergeStage<TILE, BLOCK> st;
         st.issue(pw, idx, N, c0, c1);
         st.store(s, ne);
@@ -453,24 +611,29 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     __syncthreads();
     merge_core<PREC, TILE, BLOCK>(s, mean, total, n0, c0, nl, nh);
     uint2* tab = reinterpret_cast<uint2*>(table);
-    for (uint32_t e = t; e < ne; e += BLOCK) tab[s.idx[sw4(e)]] = s.raw[sw8(e)];
+    for (uint32_t e = t; e < ne; e += BLOCK) {
+        const uint32_t x = s.idx[sw4(e)];
+        if (!DIRECT || x < N) tab[x] = s.raw[sw8(e)];  // direct: every slot is < N by construction
+    }
     if (n1 == N && t == 0) {
         const uint32_t h = s.idx[sw4(ne)];  // heavy H-1
         table[h] = AliasEntry{1.0f, h};
     }
 }
 
-template <int PREC, int TILE, int BLOCK>
+template <int PREC, int TILE, int BLOCK, bool DIRECT>
 __global__ __launch_bounds__(BLOCK) void k_mergepack(const float* __restrict__ pw,
                                                      const uint32_t* __restrict__ idx, uint32_t N,
                                                      const Params* __restrict__ prm,
                                                      const Coarse* __restrict__ bounds,
-                                                     AliasEntry* __restrict__ table) {
+                                                     AliasEntry* __restrict__ table,
+                                                     LightSrc ls) {
     const uint32_t b = blockIdx.x;
     const uint32_t n0 = b * (uint32_t)TILE;
     const uint32_t n1 = min(n0 + (uint32_t)TILE, N);
-    mergepack_block<PREC, TILE, BLOCK>(pw, idx, N, prm->heavy_count, prm->mean, prm->total, n0,
-                                       n1, bounds[b], bounds[b + 1], table);
+    mergepack_block<PREC, TILE, BLOCK, DIRECT>(pw, idx, N, prm->heavy_count, prm->mean,
+                                               prm->total, n0, n1, bounds[b], bounds[b + 1],
+                                               table, ls, prm->pivot);
 }
 
 template <int PREC, int TILE, int BLOCK>
@@ -519,11 +682,15 @@ __device__ __forceinline__ uint32_t wave_incl_max_u(uint32_t v) {
 }
 
 // A repaired boundary gets the prefixes LP(i), HP(j) of its new split: the
-// nearest sample below plus at most kSample - 1 weights (as in k_coarse).
+// nearest sample below plus at most kSample - 1 weights (as in k_coarse), and
+// in the direct form the element index of light i.
+template <bool DIRECT>
 __device__ __forceinline__ void repair_anchor(Coarse& c, const float* __restrict__ pw,
                                               const double* __restrict__ samp_h,
                                               const double* __restrict__ samp_l, uint32_t N,
-                                              uint32_t H, double light_sum) {
+                                              uint32_t H, double light_sum,
+                                              const uint32_t* __restrict__ idx,
+                                              const LightSrc& ls) {
     const uint32_t L = N - H;
     double lp = 0.0, hp = 0.0;
     if (c.i == L) {
@@ -531,8 +698,11 @@ __device__ __forceinline__ void repair_anchor(Coarse& c, const float* __restrict
     } else if (c.i > 0) {
         const uint32_t m = c.i / kSample;
         lp = m ? samp_l[m - 1u] : 0.0;
-        for (uint32_t r = m * kSample; r < c.i; ++r) lp += (double)pw[N - 1u - r];
+        for (uint32_t r = m * kSample; r < c.i; ++r)
+            lp += DIRECT ? (double)ls.w[light_pos(idx, ls.lpos, r, L, H)]
+                         : (double)pw[N - 1u - r];
     }
+    if constexpr (DIRECT) c.xi = c.i < L ? light_pos(idx, ls.lpos, c.i, L, H) : N;
     if (c.j > 0) {
         const uint32_t m = c.j / kSample;
         hp = m ? samp_h[m - 1u] : 0.0;
@@ -591,15 +761,17 @@ __global__ __launch_bounds__(kRepairThreads) void k_repair_bounds(
                 run = run < v[k] ? v[k] : run;
                 if (b < M) {
                     if (pass == 0) {
-                        bd[b].pad = bd[b].j;
+                        bd[b].aux = bd[b].j;
                         bd[b].j = run;
                     } else {
                         Coarse c = bd[b];
                         c.i = run;
                         c.j = b * tile - run;
-                        if (c.j != (uint32_t)c.pad)
-                            repair_anchor(c, pw + off, samp_h + seg_samp0[g], samp_l + seg_samp0[g],
-                                          N, seg_prm[g].heavy_count, seg_prm[g].light_sum);
+                        if (c.j != c.aux)
+                            repair_anchor<false>(c, pw + off, samp_h + seg_samp0[g],
+                                                 samp_l + seg_samp0[g], N,
+                                                 seg_prm[g].heavy_count, seg_prm[g].light_sum,
+                                                 nullptr, LightSrc{});
                         bd[b] = c;
                     }
                 }
@@ -680,10 +852,12 @@ __device__ __forceinline__ void repair_lookback(uint64_t* recs, uint32_t blk, ui
     if (lane == 0) *s_ex = ex;
 }
 
+template <bool DIRECT>
 __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
     Coarse* __restrict__ bounds, uint32_t nbound, uint32_t N, Params* __restrict__ prm,
     uint64_t* __restrict__ recs, const float* __restrict__ pw, const double* __restrict__ samp_h,
-    const double* __restrict__ samp_l, uint32_t tile) {
+    const double* __restrict__ samp_l, uint32_t tile, const uint32_t* __restrict__ idx,
+    LightSrc ls) {
     __shared__ uint32_t s_w[kRepairLbThreads / kWave];
     __shared__ uint32_t s_blk, s_ex;
     if (threadIdx.x == 0) s_blk = atomicAdd(&prm->repair_ticket, 1u);  // predecessors first
@@ -720,7 +894,8 @@ __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
                 Coarse c = bounds[b];
                 c.i = i;
                 c.j = j;
-                repair_anchor(c, pw, samp_h, samp_l, N, prm->heavy_count, prm->light_sum);
+                repair_anchor<DIRECT>(c, pw, samp_h, samp_l, N, prm->heavy_count,
+                                      prm->light_sum, idx, ls);
                 bounds[b] = c;
             }
         }
@@ -729,26 +904,38 @@ __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
 
 hipError_t launch_repair_bounds(Coarse* bounds, uint64_t n, MergeShape ms, Params* prm,
                                 uint64_t* states, const float* pw, const double* samp_h,
-                                const double* samp_l, hipStream_t s) {
+                                const double* samp_l, hipStream_t s, const uint32_t* idx,
+                                LightSrc ls) {
     const uint32_t nbound = (uint32_t)merge_blocks_for(n, ms.tile) + 1u;
-    k_repair_bounds_lb<<<(uint32_t)repair_blocks_for(n, ms.tile), kRepairLbThreads, 0, s>>>(
-        bounds, nbound, (uint32_t)n, prm, states + repair_records_offset(n), pw, samp_h, samp_l,
-        ms.tile);
+    const uint32_t grid = (uint32_t)repair_blocks_for(n, ms.tile);
+    uint64_t* recs = states + repair_records_offset(n);
+    if (ls.lpos)
+        k_repair_bounds_lb<true><<<grid, kRepairLbThreads, 0, s>>>(
+            bounds, nbound, (uint32_t)n, prm, recs, pw, samp_h, samp_l, ms.tile, idx, ls);
+    else
+        k_repair_bounds_lb<false><<<grid, kRepairLbThreads, 0, s>>>(
+            bounds, nbound, (uint32_t)n, prm, recs, pw, samp_h, samp_l, ms.tile, idx, ls);
     return hipGetLastError();
 }
 
 hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, const Params* prm,
                          const double* samp_h, const double* samp_l, Coarse* bounds,
-                         hipStream_t s) {
+                         hipStream_t s, const uint32_t* idx, LightSrc ls) {
     const uint32_t N = (uint32_t)n;
     const uint32_t nbound = (uint32_t)merge_blocks_for(n, ms.tile) + 1u;
     const uint32_t cgrid = (nbound + 3u) / 4u;
     // the split arithmetic is FP32 in FP32 mode and FP64 in all others
-    if (prec == kFP32)
-        k_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, ms.tile, bounds);
-    else
-        k_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, ms.tile,
-                                                   bounds);
+#define WRS_COARSE(P, D)                                                                    \
+    k_coarse<P, D><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, ms.tile, bounds, \
+                                         idx, ls)
+    if (prec == kFP32) {
+        if (ls.lpos) WRS_COARSE(kFP32, true);
+        else WRS_COARSE(kFP32, false);
+    } else {
+        if (ls.lpos) WRS_COARSE(kFP64Carry, true);
+        else WRS_COARSE(kFP64Carry, false);
+    }
+#undef WRS_COARSE
     return hipGetLastError();
 }
 
@@ -766,23 +953,26 @@ bool merge_shape_supported(MergeShape ms) {
 namespace {
 template <int P, int T, int B>
 hipError_t launch_mp(const float* pw, uint32_t N, const Params* prm, const uint32_t* idx,
-                     const Coarse* bounds, AliasEntry* table, hipStream_t s) {
-    k_mergepack<P, T, B><<<(uint32_t)merge_blocks_for(N, T), B, 0, s>>>(pw, idx, N, prm, bounds,
-                                                                        table);
+                     const Coarse* bounds, AliasEntry* table, hipStream_t s, LightSrc ls) {
+    const uint32_t grid = (uint32_t)merge_blocks_for(N, T);
+    if (ls.lpos)
+        k_mergepack<P, T, B, true><<<grid, B, 0, s>>>(pw, idx, N, prm, bounds, table, ls);
+    else
+        k_mergepack<P, T, B, false><<<grid, B, 0, s>>>(pw, idx, N, prm, bounds, table, ls);
     return hipGetLastError();
 }
 }  // namespace
 
 hipError_t launch_mergepack(const float* pw, uint64_t n, MergeShape ms, int prec,
                             const Params* prm, const uint32_t* idx, const Coarse* bounds,
-                            AliasEntry* table, hipStream_t s) {
+                            AliasEntry* table, hipStream_t s, LightSrc ls) {
     const uint32_t N = (uint32_t)n;
     // the merge keys are exact in every mode; only FP32 mode's p differs
 #define WRS_MP(T, P)                                                                       \
     if (ms.tile == (T) && ms.per == (P))                                                   \
         return prec == kFP32                                                               \
-                   ? launch_mp<kFP32, T, T / P>(pw, N, prm, idx, bounds, table, s)          \
-                   : launch_mp<kFP64Carry, T, T / P>(pw, N, prm, idx, bounds, table, s);
+                   ? launch_mp<kFP32, T, T / P>(pw, N, prm, idx, bounds, table, s, ls)      \
+                   : launch_mp<kFP64Carry, T, T / P>(pw, N, prm, idx, bounds, table, s, ls);
     WRS_MERGE_SHAPES(WRS_MP)
 #undef WRS_MP
     return hipErrorInvalidValue;

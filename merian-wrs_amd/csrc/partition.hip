@@ -206,18 +206,14 @@ __device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t 
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) v[k] = s.buf[17 * t + k];
-    const uint32_t e0 = (uint32_t)(kItems * t);
 
-    using A1 = Acc1<PREC>;
+    // elements past m are zero: light, and adding +0 leaves a sum's bits unchanged
     Acc<PREC> mine = acc_zero<PREC>();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        if (e0 + k < m) {
-            const bool heavy = v[k] > pivot;
-            if (heavy) mine.h = A1::add1(mine.h, v[k]);
-            else mine.l = A1::add1(mine.l, v[k]);
-            mine.c += heavy ? 1u : 0u;
-        }
+        const bool heavy = v[k] > pivot;
+        add_split<PREC>(mine.h, mine.l, heavy, v[k]);
+        mine.c += heavy ? 1u : 0u;
     }
     Acc<PREC> incl = mine;
 #pragma unroll
@@ -264,9 +260,14 @@ __device__ __forceinline__ TileCarry tile_carry(const Carry& excl, int shift) {
 // indices, the weights in partition order (pw_out, when non-null), the prefix
 // samples at ranks r with (r+1) % kSample == 0, and with FULL the hi (and lo)
 // prefix arrays of every element.  Without FULL a prefix is formed only where
-// a sample falls.  Expects s.buf free (the block synchronised after its last
-// use).
-template <int PREC, bool FULL>
+// a sample falls.  With DIRECT only the heavies' indices and weights are
+// written (their positions [0, H) of idx/pw, straight from the registers: a
+// few percent of the elements, so the LDS transpose that coalesces the full
+// arrays is skipped), and for every light rank r with r % kSample == 0 its
+// element index goes to lpos[r / kSample]: the merge pack then gathers the
+// lights from the weights themselves.  Expects s.buf free (the block
+// synchronised after its last use).
+template <int PREC, bool FULL, bool DIRECT = false>
 __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m, float pivot,
                                           const float (&v)[kItems], const Acc<PREC>& thr_excl,
                                           uint32_t hc, const TileCarry& cy, TileShared<PREC>& s,
@@ -274,7 +275,9 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
                                           float* __restrict__ pw_out,
                                           float* __restrict__ hi_out, float* __restrict__ lo_out,
                                           double* __restrict__ samp_h,
-                                          double* __restrict__ samp_l) {
+                                          double* __restrict__ samp_l,
+                                          uint32_t* __restrict__ lpos = nullptr) {
+    static_assert(!(FULL && DIRECT), "the direct form writes no full prefix arrays");
     const int t = threadIdx.x;
     const uint32_t e0 = (uint32_t)(kItems * t);
     const uint32_t Hx = cy.c;       // heavies before this tile
@@ -282,7 +285,7 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
     uint32_t* s_u = reinterpret_cast<uint32_t*>(s.buf);
 
     uint32_t slot[kItems];
-    {
+    if constexpr (!DIRECT) {
         uint32_t hr = thr_excl.c;
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
@@ -295,20 +298,22 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
     // positions of LDS slot q: heavy run then light run
     auto out_pos = [&](uint32_t q) -> uint32_t { return q < hc ? Hx + q : n - m - Lx + q; };
 
-    // round A: indices
-#pragma unroll
-    for (int k = 0; k < kItems; ++k)
-        if (e0 + k < m) s_u[slot[k]] = base + e0 + k;
-    __syncthreads();
-    for (uint32_t q = t; q < m; q += kBlock) idx_out[out_pos(q)] = s_u[q];
-    // round A': weights
-    if (pw_out != nullptr) {
-        __syncthreads();
+    if constexpr (!DIRECT) {
+        // round A: indices
 #pragma unroll
         for (int k = 0; k < kItems; ++k)
-            if (e0 + k < m) s.buf[slot[k]] = v[k];
+            if (e0 + k < m) s_u[slot[k]] = base + e0 + k;
         __syncthreads();
-        for (uint32_t q = t; q < m; q += kBlock) pw_out[out_pos(q)] = s.buf[q];
+        for (uint32_t q = t; q < m; q += kBlock) idx_out[out_pos(q)] = s_u[q];
+        // round A': weights
+        if (pw_out != nullptr) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kItems; ++k)
+                if (e0 + k < m) s.buf[slot[k]] = v[k];
+            __syncthreads();
+            for (uint32_t q = t; q < m; q += kBlock) pw_out[out_pos(q)] = s.buf[q];
+        }
     }
 
     // round B: inclusive prefixes (FP64 in the FP64-carry modes)
@@ -325,10 +330,17 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const bool heavy = v[k] > pivot;
-            if (heavy) rh = A1::add1(rh, v[k]);
-            else rl = A1::add1(rl, v[k]);
+            add_split<PREC>(rh, rl, heavy, v[k]);
             const uint32_t r1 = (heavy ? hr : lr) + 1u;  // the prefix covers ranks < r1
             const bool sample = e0 + k < m && samp_h != nullptr && (r1 & (kSample - 1)) == 0;
+            if (DIRECT && e0 + k < m) {
+                if (heavy) {
+                    idx_out[hr] = base + e0 + k;
+                    pw_out[hr] = v[k];
+                } else if ((lr & (kSample - 1)) == 0) {
+                    lpos[lr / kSample] = base + e0 + k;
+                }
+            }
             if constexpr (FULL) {
                 const double P = prefix(heavy, heavy ? rh : rl);
                 const float hi = (float)P;
@@ -472,7 +484,7 @@ __global__ __launch_bounds__(kSpineThreads) void k_part_tilescan(uint64_t* __res
 }
 
 // Pass 2c: outputs of every tile from its exclusive prefix.
-template <int PREC, bool FULL>
+template <int PREC, bool FULL, bool DIRECT>
 __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ w, uint32_t n,
                                                       const Params* __restrict__ prm,
                                                       const uint64_t* __restrict__ states,
@@ -481,7 +493,8 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
                                                       float* __restrict__ hi_out,
                                                       float* __restrict__ lo_out,
                                                       double* __restrict__ samp_h,
-                                                      double* __restrict__ samp_l) {
+                                                      double* __restrict__ samp_l,
+                                                      uint32_t* __restrict__ lpos) {
     __shared__ TileShared<PREC> s;
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
@@ -495,8 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
     tile_scan<PREC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
     const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
     __syncthreads();  // s.buf is reused by the emit rounds
-    tile_emit<PREC, FULL>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, pw_out,
-                          hi_out, lo_out, samp_h, samp_l);
+    tile_emit<PREC, FULL, DIRECT>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out,
+                                  pw_out, hi_out, lo_out, samp_h, samp_l, lpos);
 }
 
 // ---- single distribution: single pass with decoupled look-back --------------
@@ -506,12 +519,12 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
 // instead of three, which pays where the chain of tiles is short; the tuner
 // chooses between the forms.  Records and ticket are reset by the reduce and
 // finalize passes.
-template <int PREC, bool FULL>
+template <int PREC, bool FULL, bool DIRECT>
 __global__ __launch_bounds__(kBlock) void k_part_lookback(
     const float* __restrict__ w, uint32_t n, Params* __restrict__ prm,
     uint64_t* __restrict__ states, uint32_t* __restrict__ idx_out, float* __restrict__ pw_out,
     float* __restrict__ hi_out, float* __restrict__ lo_out, double* __restrict__ samp_h,
-    double* __restrict__ samp_l) {
+    double* __restrict__ samp_l, uint32_t* __restrict__ lpos) {
     __shared__ TileShared<PREC> s;
     __shared__ uint32_t s_tile;
     __shared__ Carry s_excl;
@@ -550,8 +563,8 @@ __global__ __launch_bounds__(kBlock) void k_part_lookback(
     }
     __syncthreads();
     const TileCarry cy = tile_carry<PREC>(s_excl, shift);
-    tile_emit<PREC, FULL>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out, pw_out,
-                          hi_out, lo_out, samp_h, samp_l);
+    tile_emit<PREC, FULL, DIRECT>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out,
+                                  pw_out, hi_out, lo_out, samp_h, samp_l, lpos);
 }
 
 // ---- segmented: single pass with decoupled look-back -----------------------
@@ -736,17 +749,21 @@ namespace {
 template <int P>
 void launch_part_t(const float* w, uint32_t n, int lookback, Params* prm, uint64_t* states,
                    uint32_t* idx, float* pw, float* hi, float* lo, double* samp_h,
-                   double* samp_l, hipStream_t s, hipEvent_t ev_agg, hipEvent_t ev_scan) {
+                   double* samp_l, uint32_t* lpos, hipStream_t s, hipEvent_t ev_agg,
+                   hipEvent_t ev_scan) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
     if (lookback) {  // one kernel: the aggregate and tile-scan phases are empty
         if (ev_agg) (void)hipEventRecord(ev_agg, s);
         if (ev_scan) (void)hipEventRecord(ev_scan, s);
         if (hi)
-            k_part_lookback<P, true><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, hi, lo,
-                                                             samp_h, samp_l);
+            k_part_lookback<P, true, false><<<tiles, kBlock, 0, s>>>(
+                w, n, prm, states, idx, pw, hi, lo, samp_h, samp_l, nullptr);
+        else if (lpos)
+            k_part_lookback<P, false, true><<<tiles, kBlock, 0, s>>>(
+                w, n, prm, states, idx, pw, nullptr, nullptr, samp_h, samp_l, lpos);
         else
-            k_part_lookback<P, false><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw,
-                                                              nullptr, nullptr, samp_h, samp_l);
+            k_part_lookback<P, false, false><<<tiles, kBlock, 0, s>>>(
+                w, n, prm, states, idx, pw, nullptr, nullptr, samp_h, samp_l, nullptr);
         return;
     }
     k_part_agg<P><<<tiles, kBlock, 0, s>>>(w, n, prm, states);
@@ -755,23 +772,26 @@ void launch_part_t(const float* w, uint32_t n, int lookback, Params* prm, uint64
         states, tiles, prm);
     if (ev_scan) (void)hipEventRecord(ev_scan, s);
     if (hi)
-        k_part_emit<P, true><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, hi, lo, samp_h,
-                                                      samp_l);
+        k_part_emit<P, true, false><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, hi, lo,
+                                                             samp_h, samp_l, nullptr);
+    else if (lpos)
+        k_part_emit<P, false, true><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, nullptr,
+                                                             nullptr, samp_h, samp_l, lpos);
     else
-        k_part_emit<P, false><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, nullptr,
-                                                       nullptr, samp_h, samp_l);
+        k_part_emit<P, false, false><<<tiles, kBlock, 0, s>>>(w, n, prm, states, idx, pw, nullptr,
+                                                              nullptr, samp_h, samp_l, nullptr);
 }
 
 }  // namespace
 
 hipError_t launch_partition(const float* w, uint64_t n, int prec, int lookback, Params* prm,
                             uint64_t* states, uint32_t* idx, float* pw, float* hi, float* lo,
-                            double* samp_h, double* samp_l, hipStream_t s, hipEvent_t ev_agg,
-                            hipEvent_t ev_scan) {
+                            double* samp_h, double* samp_l, uint32_t* lpos, hipStream_t s,
+                            hipEvent_t ev_agg, hipEvent_t ev_scan) {
     const uint32_t nn = (uint32_t)n;
-#define WRS_PART(P)                                                                            \
-    launch_part_t<P>(w, nn, lookback, prm, states, idx, pw, hi, lo, samp_h, samp_l, s, ev_agg, \
-                     ev_scan)
+#define WRS_PART(P)                                                                      \
+    launch_part_t<P>(w, nn, lookback, prm, states, idx, pw, hi, lo, samp_h, samp_l, lpos, s, \
+                     ev_agg, ev_scan)
     switch (prec) {
         case kFP32: WRS_PART(kFP32); break;
         case kFP64Carry: WRS_PART(kFP64Carry); break;

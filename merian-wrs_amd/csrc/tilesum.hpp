@@ -25,6 +25,7 @@ struct Acc1 {
     static __device__ __forceinline__ T add(T a, T b) { return a + b; }
     static __device__ __forceinline__ double value(T a) { return (double)a; }
     static __device__ __forceinline__ T shfl_up(T a, int d) { return wrsdev::shfl_up(a, d); }
+    static __device__ __forceinline__ T sel(bool c, T a, T b) { return c ? a : b; }
 };
 
 template <>
@@ -37,6 +38,9 @@ struct Acc1<kComp> {
     static __device__ __forceinline__ T shfl_up(T a, int d) {
         return T{wrsdev::shfl_up(a.hi, d), wrsdev::shfl_up(a.lo, d)};
     }
+    static __device__ __forceinline__ T sel(bool c, T a, T b) {
+        return T{c ? a.hi : b.hi, c ? a.lo : b.lo};
+    }
 };
 
 template <>
@@ -47,6 +51,18 @@ struct Acc1<kFP64> {
     static __device__ __forceinline__ T add(T a, T b) { return a + b; }
     static __device__ __forceinline__ double value(T a) { return a; }
     static __device__ __forceinline__ T shfl_up(T a, int d) { return wrsdev::shfl_up(a, d); }
+    static __device__ __forceinline__ T sel(bool c, T a, T b) { return c ? a : b; }
 };
+
+// Adds x to h if heavy, else to l, with one addition and selects (no branch;
+// the same bits as adding to the chosen sum).
+template <int PREC>
+__device__ __forceinline__ void add_split(typename Acc1<PREC>::T& h, typename Acc1<PREC>::T& l,
+                                          bool heavy, float x) {
+    using A1 = Acc1<PREC>;
+    const typename A1::T a = A1::add1(A1::sel(heavy, h, l), x);
+    h = A1::sel(heavy, a, h);
+    l = A1::sel(heavy, l, a);
+}
 
 }  // namespace wrsk

@@ -1,19 +1,20 @@
 // tune.cpp -- the autotuner: picks the launch geometry (wrs_tuning) of a PSA
 // build or of a batched build by timing candidates on the caller's data.
 //
-// Search: coordinate descent over the two independent parts of a build.
+// Search: coordinate descent over the parts of a build.
 //   1. the prefix-partition form (reduce-then-scan or single pass with
 //      decoupled look-back, per segment in a batch), at the default merge shape;
-//   2. every supported merge shape (buckets per block x buckets per thread)
-//      with the partition form of step 1.
+//   2. single builds: the pack form (classic or direct) with the form of step 1;
+//   3. every supported merge shape (buckets per block x buckets per thread)
+//      with the forms of steps 1 and 2.
 // A candidate's time is the median of kReps builds (HIP events on the
 // caller's stream) after one warm-up build.  The partition form only changes
 // the partition kernels and the merge shape only the split/merge kernels, so
-// the two steps do not interact beyond the shared memory bandwidth.
+// the steps interact little beyond the shared memory bandwidth.
 //
 // Results are cached per process under a key made of the device (name, CU
 // count), the precision and the problem shape; with WRS_TUNE_CACHE set they
-// also persist in that file, one "key<TAB>tile per partition" line each.
+// also persist in that file, one "key<TAB>tile per partition direct" line each.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,6 +65,7 @@ void load_file_locked() {
         std::istringstream vals(line.substr(tab + 1));
         wrs_tuning t{};
         if (vals >> t.merge_tile >> t.merge_per >> t.partition) {
+            if (!(vals >> t.direct)) t.direct = 0;
             try {
                 wrs::require_tuning(t);
             } catch (...) {
@@ -89,7 +91,9 @@ void cache_put(const std::string& key, const wrs_tuning& t) {
     g_cache[key] = t;
     if (const char* path = cache_path()) {
         std::ofstream out(path, std::ios::app);
-        if (out) out << key << '\t' << t.merge_tile << ' ' << t.merge_per << ' ' << t.partition << '\n';
+        if (out)
+            out << key << '\t' << t.merge_tile << ' ' << t.merge_per << ' ' << t.partition << ' '
+                << t.direct << '\n';
     }
 }
 
@@ -151,9 +155,10 @@ wrs_tuning with_shape(wrs_tuning t, const wrsk::MergeShape& ms) {
 }  // namespace
 
 // Coordinate descent over a measuring function: the partition form at the
-// default merge shape, then every other merge shape with the better form.
+// default merge shape, the pack form (try_direct), then every other merge
+// shape with the better forms.
 template <class M>
-wrs_tuning descend(M&& measure) {
+wrs_tuning descend(M&& measure, bool try_direct) {
     const wrs_tuning t0 = with_shape(wrs_tuning{}, kShapes[0]);
     wrs_tuning best = t0;
     float best_ms = measure(t0);
@@ -163,6 +168,15 @@ wrs_tuning descend(M&& measure) {
     if (ms1 < best_ms) {
         best_ms = ms1;
         best = t1;
+    }
+    if (try_direct) {
+        wrs_tuning t2 = best;
+        t2.direct = 1;
+        const float ms2 = measure(t2);
+        if (ms2 < best_ms) {
+            best_ms = ms2;
+            best = t2;
+        }
     }
     const wrs_tuning form = best;
     for (size_t i = 1; i < sizeof(kShapes) / sizeof(kShapes[0]); ++i) {
@@ -197,7 +211,7 @@ wrs_tuning tune_psa(const wrs::PSAConfig& base, const float* w, uint64_t n, uint
                 time_median([&] { psa.run(s, wrs::PSA::Buffers{w, table, &ws}, n); }, s, ev);
             log.add(t, ms);
             return ms;
-        });
+        }, true);
     } catch (...) {
         if (table) (void)hipFree(table);
         ws.release();
@@ -239,7 +253,7 @@ wrs_tuning tune_batched(const wrs::PSAConfig& base, const uint64_t* offsets, uin
             const float ms = time_median([&] { bp.build(w, tables, s); }, s, ev);
             log.add(t, ms);
             return ms;
-        });
+        }, false);
     } catch (...) {
         if (tables) (void)hipFree(tables);
         bp.destroy();

@@ -54,6 +54,7 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
             ws.samp_h = dev_alloc<double>(wrsk::samples_for(max_n));
             ws.samp_l = dev_alloc<double>(wrsk::samples_for(max_n));
             ws.bounds = dev_alloc<wrsk::Coarse>(wrsk::merge_blocks_for(max_n, wrsk::kMinMergeTile) + 2);
+            ws.lightpos = dev_alloc<uint32_t>(wrsk::lightpos_for(max_n));
         }
     } catch (...) {
         ws.release();
@@ -64,7 +65,7 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
 
 void Workspace::release() {
     void* ptrs[] = {params,    partials,  states, indices, pweights,
-                    prefix_hi, prefix_lo, samp_h, samp_l,  bounds};
+                    prefix_hi, prefix_lo, samp_h, samp_l,  bounds, lightpos};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     params = nullptr;
@@ -77,6 +78,7 @@ void Workspace::release() {
     samp_h = nullptr;
     samp_l = nullptr;
     bounds = nullptr;
+    lightpos = nullptr;
 }
 
 static void require(bool cond, const char* msg) {
@@ -94,7 +96,7 @@ void require_tuning(const wrs_tuning& t) {
     require(wrsk::merge_shape_supported(merge_shape_of(t)),
             "unsupported merge shape (merge_tile 1024/2048 with merge_per 2/4/8, or 4096 with 4/8)");
     require(t.partition <= 1u, "partition must be 0 or 1");
-    require(t.reserved == 0u, "wrs_tuning.reserved must be 0");
+    require(t.direct <= 1u, "direct must be 0 or 1");
 }
 
 void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, double pivot) const {
@@ -116,7 +118,7 @@ void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace
     check_hip(wrsk::launch_partition(b.elements, N, (int)m_prec, 0, ws.params, ws.states,
                                      b.partitionIndices, nullptr, b.partitionPrefix,
                                      wrsk::prec_has_lo(m_prec) ? b.partitionPrefixLo : nullptr,
-                                     nullptr, nullptr, s),
+                                     nullptr, nullptr, nullptr, s),
               "partition launch");
 }
 
@@ -149,6 +151,10 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     const bool full_prefix = dbg && (dbg->prefix_hi || dbg->prefix_lo || dbg->splits);
     float* hi = full_prefix ? ws.prefix_hi : nullptr;
     float* lo = full_prefix && wrsk::prec_has_lo(prec) ? ws.prefix_lo : nullptr;
+    // the direct form (partition arrays hold the heavies only) for plain builds;
+    // a debug build reads the full partition arrays
+    wrsk::LightSrc ls;
+    if (m_cfg.tuning.direct && !dbg) ls = wrsk::LightSrc{b.weights, ws.lightpos};
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::state_words_for(N);
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
@@ -160,21 +166,22 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     if (ev) check_hip(hipEventRecord(ev[2], s), "event");
     check_hip(wrsk::launch_partition(b.weights, N, (int)prec, (int)m_cfg.tuning.partition,
                                      ws.params, ws.states, ws.indices,
-                                     ws.pweights, hi, lo, ws.samp_h, ws.samp_l, s,
+                                     ws.pweights, hi, lo, ws.samp_h, ws.samp_l,
+                                     ls.lpos ? ws.lightpos : nullptr, s,
                                      ev ? ev[3] : nullptr, ev ? ev[4] : nullptr),
               "partition launch");
     if (ev) check_hip(hipEventRecord(ev[5], s), "event");
     const wrsk::MergeShape ms = merge_shape_of(m_cfg.tuning);
     check_hip(wrsk::launch_coarse(ws.pweights, N, ms, (int)prec, ws.params, ws.samp_h, ws.samp_l,
-                                  ws.bounds, s),
+                                  ws.bounds, s, ws.indices, ls),
               "coarse split launch");
     check_hip(wrsk::launch_repair_bounds(ws.bounds, N, ms, ws.params, ws.states, ws.pweights,
-                                         ws.samp_h, ws.samp_l, s),
+                                         ws.samp_h, ws.samp_l, s, ws.indices, ls),
               "boundary repair launch");
     if (ev) check_hip(hipEventRecord(ev[6], s), "event");
     check_hip(wrsk::launch_mergepack(ws.pweights, N, ms, (int)prec, ws.params, ws.indices,
                                      ws.bounds, reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable),
-                                     s),
+                                     s, ls),
               "merge pack launch");
     if (ev) check_hip(hipEventRecord(ev[7], s), "event");
     if (dbg) {

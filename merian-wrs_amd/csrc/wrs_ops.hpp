@@ -53,6 +53,8 @@ struct Workspace {
     double* samp_h = nullptr;          // [max_n/kSample + 1] heavy prefix samples (PSA only)
     double* samp_l = nullptr;          // [max_n/kSample + 1] light prefix samples (PSA only)
     wrsk::Coarse* bounds = nullptr;    // [max_n/kMinMergeTile + 2] block-boundary splits (PSA only)
+    uint32_t* lightpos = nullptr;      // [max_n/kSample + 1] direct form: element of every
+                                       // kSample-th light (PSA only)
 
     static Workspace allocate(uint64_t max_n, bool partition_buffers, bool lo_buffer);
     void release();

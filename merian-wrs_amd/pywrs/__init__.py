@@ -50,15 +50,16 @@ class WrsError(RuntimeError):
 class Tuning(C.Structure):
     """wrs_tuning: launch geometry (zero fields take the defaults)."""
     _fields_ = [("merge_tile", C.c_uint32), ("merge_per", C.c_uint32),
-                ("partition", C.c_uint32), ("reserved", C.c_uint32)]
+                ("partition", C.c_uint32), ("direct", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return dict(merge_tile=self.merge_tile, merge_per=self.merge_per,
-                    partition=self.partition)
+                    partition=self.partition, direct=self.direct)
 
 
-def tuning(merge_tile: int = 0, merge_per: int = 0, partition: int = 0) -> Tuning:
-    return Tuning(int(merge_tile), int(merge_per), int(partition), 0)
+def tuning(merge_tile: int = 0, merge_per: int = 0, partition: int = 0,
+           direct: int = 0) -> Tuning:
+    return Tuning(int(merge_tile), int(merge_per), int(partition), int(direct))
 
 
 class Config(C.Structure):

@@ -91,11 +91,12 @@ def test_autotune_psa_measures_then_caches(wrs, oracle):
     w = oracle.gen_zipf(2, 1.1, n)
     dw = wrs.to_device(w)
     best, log = wrs.autotune_psa(dw, n, no_cache=True)
-    print("  best", best, [(e["partition"], e["merge_tile"], e["merge_per"], round(e["ms"], 4))
-                           for e in log])
-    # partition form at the default shape (2), then the 7 other shapes
-    assert len(log) == 2 + len(SHAPES) - 1
+    print("  best", best, [(e["partition"], e["direct"], e["merge_tile"], e["merge_per"],
+                            round(e["ms"], 4)) for e in log])
+    # partition form at the default shape (2), the pack form (1), then the 7 other shapes
+    assert len(log) == 2 + 1 + len(SHAPES) - 1
     assert {e["partition"] for e in log[:2]} == {0, 1}
+    assert [e["direct"] for e in log[:3]] == [0, 0, 1]
     assert {(e["merge_tile"], e["merge_per"]) for e in log} == set(SHAPES)
     assert all(e["ms"] > 0 for e in log)
     assert (best["merge_tile"], best["merge_per"]) in SHAPES
@@ -120,9 +121,10 @@ def test_autotune_batched_measures_then_caches(wrs, oracle):
     best, log = wrs.autotune_batched(off, dw, no_cache=True)
     print("  best", best, [(e["partition"], e["merge_tile"], e["merge_per"],
                             round(e["ms"], 4)) for e in log])
-    # partition form at the default shape (2), then the 7 other shapes
+    # partition form at the default shape (2), then the 7 other shapes (no pack form)
     assert len(log) == 2 + len(SHAPES) - 1
     assert {e["partition"] for e in log[:2]} == {0, 1}
+    assert all(e["direct"] == 0 for e in log)
     assert (best["merge_tile"], best["merge_per"]) in SHAPES
     best2, log2 = wrs.autotune_batched(off, dw)
     assert best2 == best and log2 == []
