@@ -8,7 +8,9 @@
 //   3. every supported merge shape (buckets per block x buckets per thread)
 //      with the forms of steps 1 and 2.
 // A candidate's time is the median of kReps builds (HIP events on the
-// caller's stream) after one warm-up build.  The partition form only changes
+// caller's stream) after one warm-up build; kWarm builds of the first
+// candidate run untimed before the search, so that the first candidate is not
+// timed on an idle (down-clocked) device.  The partition form only changes
 // the partition kernels and the merge shape only the split/merge kernels, so
 // the steps interact little beyond the shared memory bandwidth.
 //
@@ -38,6 +40,7 @@ namespace wrst {
 namespace {
 
 constexpr int kReps = 5;
+constexpr int kWarm = 20;
 
 // the merge shapes tried, default first
 const wrsk::MergeShape kShapes[] = {{2048, 4}, {1024, 2}, {1024, 4}, {1024, 8},
@@ -121,8 +124,8 @@ struct Events {
 
 // median time (ms) of kReps runs of fn on stream s, after one warm-up
 template <class F>
-float time_median(F&& fn, hipStream_t s, Events& ev) {
-    fn();
+float time_median(F&& fn, hipStream_t s, Events& ev, int warm = 1) {
+    for (int r = 0; r < warm; ++r) fn();
     wrs::check_hip(hipStreamSynchronize(s), "tune sync");
     std::vector<float> t(kReps);
     for (int r = 0; r < kReps; ++r) {
@@ -161,7 +164,7 @@ template <class M>
 wrs_tuning descend(M&& measure, bool try_direct) {
     const wrs_tuning t0 = with_shape(wrs_tuning{}, kShapes[0]);
     wrs_tuning best = t0;
-    float best_ms = measure(t0);
+    float best_ms = measure(t0, kWarm);
     wrs_tuning t1 = t0;
     t1.partition = 1;
     const float ms1 = measure(t1);
@@ -203,12 +206,12 @@ wrs_tuning tune_psa(const wrs::PSAConfig& base, const float* w, uint64_t n, uint
     try {
         wrs::check_hip(hipMalloc(&table, n * sizeof(wrs_alias_entry)), "hipMalloc");
         Events ev;
-        best = descend([&](const wrs_tuning& t) {
+        best = descend([&](const wrs_tuning& t, int warm = 1) {
             wrs::PSAConfig c = base;
             c.tuning = t;
             const wrs::PSA psa(c);
-            const float ms =
-                time_median([&] { psa.run(s, wrs::PSA::Buffers{w, table, &ws}, n); }, s, ev);
+            const float ms = time_median(
+                [&] { psa.run(s, wrs::PSA::Buffers{w, table, &ws}, n); }, s, ev, warm);
             log.add(t, ms);
             return ms;
         }, true);
@@ -248,9 +251,9 @@ wrs_tuning tune_batched(const wrs::PSAConfig& base, const uint64_t* offsets, uin
         bp.create(base.splitSize, (int)base.precision, wrs_tuning{}, offsets, nseg);
         wrs::check_hip(hipMalloc(&tables, bp.total_n * sizeof(wrsk::AliasEntry)), "hipMalloc");
         Events ev;
-        best = descend([&](const wrs_tuning& t) {
+        best = descend([&](const wrs_tuning& t, int warm = 1) {
             bp.set_tuning(t);
-            const float ms = time_median([&] { bp.build(w, tables, s); }, s, ev);
+            const float ms = time_median([&] { bp.build(w, tables, s); }, s, ev, warm);
             log.add(t, ms);
             return ms;
         }, false);
