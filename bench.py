@@ -5,11 +5,15 @@ One step = one PSA alias-table build of N = 2^26 Zipf(1.1) FP32 weights (ranks
 a seeded Feistel permutation of 1..N, seed 2) with compensated FP32 tile sums,
 the exact fixed-point carry and FP64 mean/split arithmetic, split size 32,
 inputs resident in HBM: FP64 total -> heavy/light prefix-partition (tile
-aggregates, tile scan, emit) -> block-boundary splits -> merge pack.  `value`
-is the build throughput in GB/s of algorithmic bytes B_build = 32N + 12K
-(K = N/32): mean read 4N, partition read 4N + index 4N + prefix 4N (this build
-writes the partition-ordered weights in its place), split 12K, pack read index
-4N + weight 4N + write table 8N.
+aggregates, tile scan, emit) -> block-boundary splits -> merge pack, in the
+launch geometry and pack form the autotuner picks.  `value` is the build
+throughput in GB/s of the reference pipeline's algorithmic bytes
+B_build = 32N + 12K (K = N/32): mean read 4N, partition read 4N + index 4N +
+prefix 4N (this build writes the partition-ordered weights in its place),
+split 12K, pack read index 4N + weight 4N + write table 8N.  The direct pack
+form moves fewer bytes for the same table (the partition writes only the
+heavies, the pack reads the lights from the weights); `value` keeps the fixed
+B_build so that forms and runs compare by time.
 
 Also reported on the same line:
   roofline      the dominant kernel's algorithmic bytes per launch / its average
@@ -265,9 +269,10 @@ def main() -> int:
         for prec in range(4):
             pp = plan if prec == args.precision else wrs.Plan(n, args.split_size, prec, tune=tune)
             tt = wrs.empty(n, wrs.ALIAS_DTYPE)
-            for _ in range(2):
+            # warm-up: the device idled during the host-side error computation
+            for _ in range(20):
                 pp.psa_build(w, n, tt, stream=st)
-            ms = time_it(lambda: pp.psa_build(w, n, tt, stream=st), st, wrs, 5)
+            ms = time_it(lambda: pp.psa_build(w, n, tt, stream=st), st, wrs, 10)
             row = dict(mode=PREC_NAMES[prec], ms=round(ms, 4),
                        gbs=round(B_build / (ms * 1e-3) / 1e9, 1))
             if implied is not None:
@@ -467,6 +472,7 @@ def main() -> int:
                         f"rank permutation (seed 2), split size {args.split_size}, "
                         f"{PREC_NAMES[args.precision]}",
             "n": n, "split_size": args.split_size, "B_build_bytes": B_build,
+            "heavy_count": H,
             "parallelism": "single GPU",
             "tuning": tune,
             "tuning_candidates_ms": [[e["partition"], e["direct"], e["merge_tile"],

@@ -107,20 +107,71 @@ constexpr uint32_t kStatTile = kStatBlock * kStatPer;
 constexpr int kSumWords = 16;  // one Summary per tile in the state buffer (<= 128 bytes)
 static_assert(sizeof(Summary) <= kSumWords * 8, "summary record");
 
+// Combines the block's kStatBlock summaries (s[t] covers the range after
+// s[t-1]) into s[0] by a fixed pairwise tree: level d joins s[t] and s[t + d]
+// for t a multiple of 2d, so the order of the ranges is kept.
+__device__ __forceinline__ void block_combine(Summary* s) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int d = 1; d < kStatBlock; d <<= 1) {
+        if ((t & (2 * d - 1)) == 0) s[t] = combine(s[t], s[t + d]);
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+// Histogram with a per-block LDS cache of bins: a sample's bin is counted in
+// the block's hash slot for that bin when the slot is free or already holds
+// it, else by a global atomic; the slots are added to the histogram when the
+// block ends.  A skewed distribution sends most samples to a few bins, and
+// without the cache every sample of a hot bin is an atomic on one address
+// (serialised in one L2 channel); with it a hot bin costs one global atomic
+// per block.  A block counts at most s / kHistMaxBlocks + 256 kHistPerThread
+// samples, so its 32-bit slot counts cannot overflow below 2^43 samples (a
+// device holds fewer than 2^37 4-byte samples).
+constexpr int kHistSlots = 2048;
+constexpr uint32_t kHistEmpty = 0xffffffffu;
+constexpr uint64_t kHistPerThread = 64;   // samples per thread before the grid grows
+constexpr uint64_t kHistMaxBlocks = 4096;
 
 __global__ __launch_bounds__(256) void k_histogram(const uint32_t* __restrict__ samples,
                                                    uint64_t s, uint64_t n,
                                                    unsigned long long* __restrict__ hist) {
+    __shared__ uint32_t s_key[kHistSlots];
+    __shared__ uint32_t s_cnt[kHistSlots];
+    for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x) {
+        s_key[i] = kHistEmpty;
+        s_cnt[i] = 0;
+    }
+    __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < s; t += stride) {
         const uint64_t x = samples[t];
-        atomicAdd(hist + (x < n ? x : n), 1ull);
+        const uint32_t key = (uint32_t)(x < n ? x : n);  // n: the out-of-range counter
+        bool cached = false;
+        if (key != kHistEmpty) {
+            const uint32_t h = (key * 2654435761u) >> (32 - 11);  // Fibonacci hash, 11 bits
+            uint32_t cur = s_key[h];
+            if (cur == kHistEmpty) {
+                const uint32_t old = atomicCAS(&s_key[h], kHistEmpty, key);
+                cur = old == kHistEmpty ? key : old;
+            }
+            if (cur == key) {
+                atomicAdd(&s_cnt[h], 1u);
+                cached = true;
+            }
+        }
+        if (!cached) atomicAdd(hist + key, 1ull);
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHistSlots; i += blockDim.x)
+        if (s_cnt[i] != 0) atomicAdd(hist + s_key[i], (unsigned long long)s_cnt[i]);
 }
+static_assert(kHistSlots == 1 << 11, "hash width");
 
 // Per tile of kStatTile bins: thread t takes bins 16t..16t+15 in order, the
-// block combines the thread summaries in thread order (one thread, from LDS).
+// block combines the thread summaries in range order (block_combine).
 // q(i) = w[i] * inv_total, cum(i) = S * cdf(i) with cdf from the plan's scan.
 template <bool LO>
 __global__ __launch_bounds__(kStatBlock) void k_stats_tile(
@@ -138,17 +189,30 @@ __global__ __launch_bounds__(kStatBlock) void k_stats_tile(
     const double p_last = floor(S * c_last / min_expected);
     const uint64_t cap = pooled ? (p_last >= 1.0 ? (uint64_t)p_last - 1 : 0) : ~0ull;
 
+    // every load of the thread's bins first (indices clamped to the last bin),
+    // so that the lines of a lane's 16 consecutive bins are fetched once
+    uint64_t ov[kStatPer];
+    float wv[kStatPer], chv[kStatPer], clv[kStatPer];
+#pragma unroll
+    for (int k = 0; k < kStatPer; ++k) {
+        const uint64_t i = b0 + k < n ? b0 + k : n - 1;
+        ov[k] = hist[i];
+        wv[k] = w[i];
+        chv[k] = pooled ? cdf_hi[i] : 0.0f;
+        clv[k] = LO && pooled ? cdf_lo[i] : 0.0f;
+    }
     Summary me = sum_empty();
+#pragma unroll
     for (int k = 0; k < kStatPer; ++k) {
         const uint64_t i = b0 + k;
-        if (i >= n) break;
-        const uint64_t o = hist[i];
-        const float wi = w[i];
+        if (i >= n) continue;
+        const uint64_t o = ov[k];
+        const float wi = wv[k];
         const double q = (double)wi * inv_total;
         uint64_t id = i;
         if (pooled) {
-            double c = (double)cdf_hi[i];
-            if (LO) c += (double)cdf_lo[i];
+            double c = (double)chv[k];
+            if (LO) c += (double)clv[k];
             const double p = floor(S * c / min_expected);
             id = p >= (double)cap ? cap : (uint64_t)(p > 0.0 ? p : 0.0);
         }
@@ -166,15 +230,12 @@ __global__ __launch_bounds__(kStatBlock) void k_stats_tile(
     }
     s_sum[threadIdx.x] = me;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        Summary acc = s_sum[0];
-        for (int t = 1; t < kStatBlock; ++t) acc = combine(acc, s_sum[t]);
-        *reinterpret_cast<Summary*>(recs + tile * kSumWords) = acc;
-    }
+    block_combine(s_sum);
+    if (threadIdx.x == 0) *reinterpret_cast<Summary*>(recs + tile * kSumWords) = s_sum[0];
 }
 
 // One block: thread t combines tiles [t*per, (t+1)*per) in order, then the
-// thread summaries are combined in thread order; out = {chi2, dof, sse,
+// thread summaries are combined in range order (block_combine); out = {chi2, dof, sse,
 // expected sse, zero-mass hits, pools}.
 __global__ __launch_bounds__(kStatBlock) void k_stats_final(const uint64_t* __restrict__ recs,
                                                             uint64_t tiles,
@@ -186,9 +247,9 @@ __global__ __launch_bounds__(kStatBlock) void k_stats_final(const uint64_t* __re
         me = combine(me, *reinterpret_cast<const Summary*>(recs + k * kSumWords));
     s_sum[threadIdx.x] = me;
     __syncthreads();
+    block_combine(s_sum);
     if (threadIdx.x == 0) {
-        Summary acc = s_sum[0];
-        for (int t = 1; t < kStatBlock; ++t) acc = combine(acc, s_sum[t]);
+        const Summary acc = s_sum[0];
         const double chi2 = acc.inner + pool_term(acc.head) + (acc.single ? 0.0 : pool_term(acc.tail));
         const uint64_t pools = acc.empty ? 0 : acc.bounds + 1;
         out[0] = chi2;
@@ -207,8 +268,8 @@ hipError_t launch_histogram(const uint32_t* samples, uint64_t s_count, uint64_t 
         if (e != hipSuccess) return e;
     }
     if (s_count == 0) return hipSuccess;
-    uint64_t g = (s_count + 255) / 256;
-    if (g > (1u << 16)) g = 1u << 16;
+    uint64_t g = (s_count + 256 * kHistPerThread - 1) / (256 * kHistPerThread);
+    if (g > kHistMaxBlocks) g = kHistMaxBlocks;
     k_histogram<<<(uint32_t)g, 256, 0, s>>>(samples, s_count, n,
                                             reinterpret_cast<unsigned long long*>(hist));
     return hipGetLastError();
