@@ -263,6 +263,20 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
 // so that every local key and offset stays below 2^62 in magnitude.
 __device__ __forceinline__ int64_t quant(double x, int qs) { return __double2ll_rn(ldexp(x, qs)); }
 
+// quant of a float weight w >= 0 (+-0 included) without FP64 arithmetic, the
+// same value: w * 2^qs < 2^63 is exact in FP32 (an exponent shift; a result
+// below 2^-126 rounds to 0 either way), rintf rounds it to the nearest even
+// integer as the FP64 path does, and the integer-valued float converts to
+// int64 as its 24-bit significand shifted left (y >= 2^23) or by the exact
+// 32-bit conversion (y < 2^23).
+__device__ __forceinline__ int64_t quant_w(float w, int qs) {
+    const float y = rintf(ldexpf(w, qs));
+    const uint32_t b = __float_as_uint(y) & 0x7fffffffu;
+    const int ex = (int)(b >> 23) - 150;  // y = significand * 2^ex
+    const uint64_t sig = (uint64_t)((b & 0x7fffffu) | 0x800000u);
+    return ex >= 0 ? (int64_t)(sig << ex) : (int64_t)(uint32_t)fabsf(y);
+}
+
 // One block of TILE buckets [n0, n1) between boundary splits c0 and c1, BLOCK
 // threads, TILE / BLOCK buckets per thread.
 //
@@ -465,7 +479,7 @@ __device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, double m
         const uint32_t e = e0 + k;
         inc[k] = 0;
         if (e < ne) {
-            const int64_t wq = quant((double)s.w[sw4(e)], qs);
+            const int64_t wq = quant_w(s.w[sw4(e)], qs);
             inc[k] = e < nl ? mean_q - wq : wq - mean_q;
             if (e < nl) ml += inc[k];
             else mh += inc[k];
@@ -534,16 +548,16 @@ __device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, double m
     if (d0 < ne) {
         // largest q in [max(0, d0 - nl), min(d0, nh)] with q == 0 or
         // B[q-1] <= A[d0 - q] (A[nl] counts as +inf here)
-        int64_t qa = d0 > nl ? (int64_t)(d0 - nl) : 0;
-        int64_t qb = min(d0, nh);
+        uint32_t qa = d0 > nl ? d0 - nl : 0u;
+        uint32_t qb = min(d0, nh);
         while (qa < qb) {
-            const int64_t q = (qa + qb + 1) / 2;
-            const uint32_t r = d0 - (uint32_t)q;
-            const bool ok = r >= nl || B((uint32_t)q - 1u) <= A(r);
+            const uint32_t q = (qa + qb + 1u) / 2u;
+            const uint32_t r = d0 - q;
+            const bool ok = r >= nl || B(q - 1u) <= A(r);
             if (ok) qa = q;
-            else qb = q - 1;
+            else qb = q - 1u;
         }
-        uint32_t q = (uint32_t)qa;
+        uint32_t q = qa;
         uint32_t r = d0 - q;
         const uint32_t d1 = min(d0 + (uint32_t)kPer, ne);
         int64_t av = A(r);  // r <= nl: A[nl] exists
