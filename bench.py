@@ -33,7 +33,8 @@ Also reported on the same line:
                 the same on-device statistics of the 2^30 samples;
   sweep         S = 1e7 samples at N = 1e5 .. 1e8 (U(0,1) weights), the shape of
                 the reference's sampling plots: alias build + sample, ITS and
-                cutpoint sampling throughput;
+                cutpoint sampling throughput, and the windowed samplers
+                (window 128; groups of 64 samples share a window);
   batched       4096 log-uniform-size LogNormal(0, 3) distributions built and
                 sampled (64 each) by one hipGraph vs one launch sequence per
                 distribution;
@@ -425,10 +426,17 @@ def main() -> int:
             wrs.cutpoint_build(hi_, lo_, nn, gd_, nn, stream=st)
             c_ms = time_it(lambda: wrs.cutpoint_sample(hi_, lo_, nn, gd_, nn, Ss, 5, osw, stream=st),
                            st, wrs, 3)
+            # windowed samplers, window 128 (the reference's cooperative size)
+            aw_ms = time_it(lambda: wrs.alias_sample_window(tsw, nn, Ss, 5, 128, osw, stream=st),
+                            st, wrs, 5)
+            iw_ms = time_it(lambda: wrs.its_sample_window(hi_, lo_, nn, Ss, 5, 128, osw, stream=st),
+                            st, wrs, 3)
             row = dict(n=nn, psa_build_ms=round(b_ms, 4), alias_sample_ms=round(a_ms, 4),
                        alias_build_plus_sample_per_s=round(Ss / ((b_ms + a_ms) * 1e-3), 1),
                        its_sample_per_s=round(Ss / (i_ms * 1e-3), 1),
-                       cutpoint_sample_per_s=round(Ss / (c_ms * 1e-3), 1))
+                       cutpoint_sample_per_s=round(Ss / (c_ms * 1e-3), 1),
+                       alias_window128_sample_per_s=round(Ss / (aw_ms * 1e-3), 1),
+                       its_window128_sample_per_s=round(Ss / (iw_ms * 1e-3), 1))
             sweep.append(row)
             log(f"sweep: {row}")
             del wsw, tsw, hi_, lo_, gd_, pl

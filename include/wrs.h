@@ -161,6 +161,32 @@ wrs_status wrs_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n, 
                           uint64_t seed, uint64_t counter_base, uint32_t* out,
                           wrs_stream_t stream);
 
+/* Windowed ("cooperative") sampling, for throughput where the samples need
+ * not be independent of each other.  Samples come in groups of 64 consecutive
+ * counters (group g = counter / 64, with its own Philox stream); all samples
+ * of a group draw from one window chosen per group, so every sample's marginal
+ * distribution is exact but the samples of a group share the window.
+ *  - alias: the window is `window` consecutive buckets (cyclic, min(window, n))
+ *    from a uniform start; each sample takes a uniform bucket in it and the
+ *    coin of wrs_alias_sample.
+ *  - ITS: the group's uniform is narrowed to at most `window` CDF entries by
+ *    64-part searches (parts != 0) or by halving; each sample draws uniformly
+ *    within the window's CDF interval and searches only the window.
+ * Replace the cooperative paths of SampleAliasTable / InverseTransformSampling
+ * (src/device/wrs/alias/sampling/shader.comp:68-90,103-110, configured by
+ * SampleAliasTableConfig::cooperativeSampleSize, AliasTableSampling.hpp:26-33;
+ * src/device/wrs/its/sampling/shader.comp:100-129, configured by
+ * InverseTransformSamplingConfig, InverseTransformSampling.hpp:69-81), whose
+ * alias window is clamped to [0, N-1] (edge buckets over-sampled) and whose ITS
+ * parts can miss the top of the range.  window >= 1. */
+wrs_status wrs_alias_sample_window(const wrs_alias_entry* table, uint64_t n, uint64_t s,
+                                   uint64_t seed, uint64_t counter_base, uint32_t window,
+                                   uint32_t* out, wrs_stream_t stream);
+wrs_status wrs_its_sample_window(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                 uint64_t s, uint64_t seed, uint64_t counter_base,
+                                 uint32_t window, int parts, uint32_t* out,
+                                 wrs_stream_t stream);
+
 /* Cutpoint method (Chen & Asau's guide table) over a normalised ITS CDF
  * (wrs_its_build output; cdf_lo optional, as for wrs_its_sample).  guide has
  * m + 1 entries (1 <= m < 2^32): guide[g] = smallest i with cdf(i) >= g/m

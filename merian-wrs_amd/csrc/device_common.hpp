@@ -59,6 +59,7 @@ __device__ __forceinline__ U4 philox_at(uint64_t seed, uint64_t t, uint32_t c2, 
 
 constexpr uint32_t kTagGen = 0x47454E31u;  // weight generators
 constexpr uint32_t kTagFeistel = 0x46535431u;
+constexpr uint32_t kTagGroup = 0x47525031u;  // windowed samplers: one stream per group
 
 __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * 0x1.0p-24f; }
 __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {

@@ -167,6 +167,14 @@ hipError_t launch_alias_sample(const AliasEntry* table, uint64_t n, uint64_t s_c
 hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
                              uint64_t s_count, uint64_t seed, uint64_t counter_base,
                              uint32_t* out, hipStream_t s);
+// windowed samplers (groups of 64 consecutive counters share a window);
+// s_count >= 1, window >= 1
+hipError_t launch_alias_sample_window(const AliasEntry* table, uint64_t n, uint64_t s_count,
+                                      uint64_t seed, uint64_t counter_base, uint32_t window,
+                                      uint32_t* out, hipStream_t s);
+hipError_t launch_its_sample_window(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                    uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                                    uint64_t window, int parts, uint32_t* out, hipStream_t s);
 // guide[0..m] of the normalised CDF (m >= 1), and guided ITS samples
 hipError_t launch_cutpoint_build(const float* cdf_hi, const float* cdf_lo, uint64_t n,
                                  uint32_t* guide, uint64_t m, hipStream_t s);

@@ -262,6 +262,118 @@ hipError_t launch_alias_sample(const AliasEntry* table, uint64_t n, uint64_t s_c
     return hipGetLastError();
 }
 
+// ---- windowed (cooperative) samplers ----------------------------------------
+//
+// Samples come in groups of kWindowGroup consecutive counters (group
+// g = counter / kWindowGroup, with its own Philox stream: counter (g, kTagGroup)).
+// All samples of a group draw from one window of the table or of the CDF,
+// chosen per group, so that a wave's reads stay in a small region.  Every
+// sample's marginal distribution is exact; the samples of a group are not
+// independent of each other (they share the window).  Replaces the cooperative
+// paths of the reference samplers, with their biases removed:
+// alias/sampling/shader.comp:68-90,103-110 centres a window on one lane's
+// bucket and clamps it to [0, N-1] (so the edge buckets are over-sampled) and
+// that lane reuses the random pair that placed the centre;
+// its/sampling/shader.comp:100-129 partitions the range into 64 parts of
+// floor(range/64) entries, so a u above the last part's end finds no part.
+constexpr uint64_t kWindowGroup = 64;
+
+// Alias: the window holds T' = min(T, n) buckets from c_g = floor(x_g n / 2^64),
+// cyclic, and sample t takes bucket (c_g + o_t) mod n with o_t = floor(x_t T' / 2^32):
+// c_g is uniform and independent of o_t, so every bucket is equally likely.
+// Coin and acceptance as k_alias_sample (x_t, coin: words 0 and 2 of the
+// sample's own stream).
+__global__ __launch_bounds__(kSampleBlock) void k_alias_sample_window(
+    const uint2* __restrict__ table, uint64_t n, uint64_t s, uint64_t seed, uint64_t base,
+    uint32_t T, uint32_t* __restrict__ out) {
+    const uint64_t Tw = (uint64_t)T < n ? (uint64_t)T : n;
+    const uint64_t stride = (uint64_t)gridDim.x * kSampleBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kSampleBlock + threadIdx.x; g < s; g += stride) {
+        const uint64_t c = base + g;
+        const U4 rg = philox_at(seed, c / kWindowGroup, kTagGroup, 0u);
+        const uint64_t start = __umul64hi(((uint64_t)rg.y << 32) | rg.x, n);
+        const U4 r = philox_at(seed, c, 0u, 0u);
+        uint64_t ix = start + (((uint64_t)r.x * Tw) >> 32);  // < 2n
+        ix = ix >= n ? ix - n : ix;
+        const float u2 = u24(r.z);
+        const uint2 e = table[ix];
+        out[g] = (u2 > __uint_as_float(e.x)) ? e.y : (uint32_t)ix;
+    }
+}
+
+// ITS: one wave per group.  u1 (the group's stream) is narrowed to a window of
+// at most W entries holding it -- by 64-part searches (lane l tests the end of
+// part l, parts of ceil(range/64) entries, the first part whose CDF end is
+// >= u1 wins; none: the last part) or by halving -- so that u1 lies in
+// (cdf(x-1), cdf(y)]; each sample then draws u2 uniformly in that interval from
+// its own stream (the u of k_its_sample) and returns the lower bound of u2 in
+// [x, y].  P(window) is the window's mass and u2 is uniform within it, so the
+// marginal is the CDF's distribution.
+template <bool LO>
+__global__ __launch_bounds__(kSampleBlock) void k_its_sample_window(
+    const float* __restrict__ hi, const float* __restrict__ lo, uint64_t n, uint64_t s,
+    uint64_t seed, uint64_t base, uint64_t W, int parts, uint32_t* __restrict__ out) {
+    const int lane = lane_id();
+    const uint64_t g0 = base / kWindowGroup;
+    const uint64_t ngroups = (base + s - 1) / kWindowGroup - g0 + 1;
+    const uint64_t waves = (uint64_t)gridDim.x * (kSampleBlock / kWave);
+    for (uint64_t k = (uint64_t)blockIdx.x * (kSampleBlock / kWave) + threadIdx.x / kWave;
+         k < ngroups; k += waves) {  // wave-uniform
+        const uint64_t g = g0 + k;
+        const U4 rg = philox_at(seed, g, kTagGroup, 0u);
+        const double u1 = u53_open0(rg.x, rg.y);
+        uint64_t x = 0, y = n - 1;
+        while (y - x + 1 > W) {
+            if (parts) {
+                const uint64_t ps = (y - x + 1 + kWave - 1) / kWave;
+                const uint64_t st = x + (uint64_t)lane * ps;
+                const uint64_t en = st + ps - 1 < y ? st + ps - 1 : y;
+                const bool valid = st <= y;
+                const uint64_t hit = __ballot(valid && u1 <= cdf_at<LO>(hi, lo, valid ? en : y));
+                const int f = hit ? __builtin_ctzll(hit) : 63 - __builtin_clzll(__ballot(valid));
+                x = __shfl(st, f, kWave);
+                y = __shfl(en, f, kWave);
+            } else {
+                const uint64_t mid = (x + y) >> 1;
+                if (u1 > cdf_at<LO>(hi, lo, mid)) x = mid + 1;
+                else y = mid;
+            }
+        }
+        const uint64_t c = g * kWindowGroup + (uint64_t)lane;
+        if (c >= base && c - base < s) {
+            const U4 r = philox_at(seed, c, 0u, 0u);
+            const double v = u53_open0(r.x, r.y);
+            const double cl = x > 0 ? cdf_at<LO>(hi, lo, x - 1) : 0.0;
+            const double ch = cdf_at<LO>(hi, lo, y);
+            const double u2 = cl + (ch - cl) * v;
+            out[c - base] = (uint32_t)cdf_lower_bound<LO>(hi, lo, x, y, u2);
+        }
+    }
+}
+
+hipError_t launch_alias_sample_window(const AliasEntry* table, uint64_t n, uint64_t s_count,
+                                      uint64_t seed, uint64_t counter_base, uint32_t window,
+                                      uint32_t* out, hipStream_t s) {
+    k_alias_sample_window<<<grid_for(s_count, kSampleBlock, 1u << 16), kSampleBlock, 0, s>>>(
+        reinterpret_cast<const uint2*>(table), n, s_count, seed, counter_base, window, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_its_sample_window(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                    uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                                    uint64_t window, int parts, uint32_t* out, hipStream_t s) {
+    const uint64_t groups = (counter_base + s_count - 1) / kWindowGroup -
+                            counter_base / kWindowGroup + 1;
+    const uint32_t g = grid_for(groups * kWave, kSampleBlock, 1u << 16);
+    if (cdf_lo)
+        k_its_sample_window<true><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, s_count, seed,
+                                                             counter_base, window, parts, out);
+    else
+        k_its_sample_window<false><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, s_count, seed,
+                                                              counter_base, window, parts, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
                              uint64_t s_count, uint64_t seed, uint64_t counter_base,
                              uint32_t* out, hipStream_t s) {

@@ -213,9 +213,15 @@ void SampleAliasTable::run(hipStream_t s, const Buffers& b, uint64_t N, uint64_t
     require(b.aliasTable && b.samples, "null sampling buffer");
     require(N > 0 && N < (1ull << 32), "N must be in [1, 2^32)");
     if (S == 0) return;
-    check_hip(wrsk::launch_alias_sample(reinterpret_cast<const wrsk::AliasEntry*>(b.aliasTable), N,
-                                        S, seed, counterBase, b.samples, s),
-              "alias sample launch");
+    if (m_cfg.cooperativeSampleSize)
+        check_hip(wrsk::launch_alias_sample_window(
+                      reinterpret_cast<const wrsk::AliasEntry*>(b.aliasTable), N, S, seed,
+                      counterBase, m_cfg.cooperativeSampleSize, b.samples, s),
+                  "windowed alias sample launch");
+    else
+        check_hip(wrsk::launch_alias_sample(reinterpret_cast<const wrsk::AliasEntry*>(b.aliasTable),
+                                            N, S, seed, counterBase, b.samples, s),
+                  "alias sample launch");
 }
 
 void InverseTransformSampling::run(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S,
@@ -223,8 +229,14 @@ void InverseTransformSampling::run(hipStream_t s, const Buffers& b, uint64_t N, 
     require(b.cmf && b.samples, "null sampling buffer");
     require(N > 0 && N < (1ull << 32), "N must be in [1, 2^32)");
     if (S == 0) return;
-    check_hip(wrsk::launch_its_sample(b.cmf, b.cmfLo, N, S, seed, counterBase, b.samples, s),
-              "its sample launch");
+    if (m_cfg.cooperativeSamplingSize)
+        check_hip(wrsk::launch_its_sample_window(b.cmf, b.cmfLo, N, S, seed, counterBase,
+                                                 m_cfg.cooperativeSamplingSize,
+                                                 m_cfg.pArraySearch ? 1 : 0, b.samples, s),
+                  "windowed its sample launch");
+    else
+        check_hip(wrsk::launch_its_sample(b.cmf, b.cmfLo, N, S, seed, counterBase, b.samples, s),
+                  "its sample launch");
 }
 
 static void require_cutpoint(const float* hi, uint64_t N, const uint32_t* guide, uint64_t m) {
@@ -461,6 +473,28 @@ wrs_status wrs_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n, 
         wrs::InverseTransformSampling().run(
             hs(stream), wrs::InverseTransformSampling::Buffers{cdf_hi, cdf_lo, out}, n, s, seed,
             counter_base);
+    });
+}
+
+wrs_status wrs_alias_sample_window(const wrs_alias_entry* table, uint64_t n, uint64_t s,
+                                   uint64_t seed, uint64_t counter_base, uint32_t window,
+                                   uint32_t* out, wrs_stream_t stream) {
+    return guarded([&] {
+        if (window == 0) throw wrs::Error(WRS_EINVAL, "window must be >= 1");
+        wrs::SampleAliasTable(wrs::SampleAliasTableConfig{window})
+            .run(hs(stream), wrs::SampleAliasTable::Buffers{table, out}, n, s, seed, counter_base);
+    });
+}
+
+wrs_status wrs_its_sample_window(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                 uint64_t s, uint64_t seed, uint64_t counter_base,
+                                 uint32_t window, int parts, uint32_t* out,
+                                 wrs_stream_t stream) {
+    return guarded([&] {
+        if (window == 0) throw wrs::Error(WRS_EINVAL, "window must be >= 1");
+        wrs::InverseTransformSampling(wrs::InverseTransformSamplingConfig{window, parts != 0})
+            .run(hs(stream), wrs::InverseTransformSampling::Buffers{cdf_hi, cdf_lo, out}, n, s,
+                 seed, counter_base);
     });
 }
 

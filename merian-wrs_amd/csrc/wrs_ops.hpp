@@ -137,14 +137,36 @@ class PSA {
     PSAConfig m_cfg;
 };
 
+// cooperativeSampleSize 0: independent samples; T > 0: groups of 64
+// consecutive counters draw from one window of T buckets (SampleAliasTableConfig,
+// src/device/wrs/alias/sampling/AliasTableSampling.hpp:26-33)
+struct SampleAliasTableConfig {
+    uint32_t cooperativeSampleSize = 0;
+};
+
 class SampleAliasTable {
   public:
     struct Buffers {
         const wrs_alias_entry* aliasTable;
         uint32_t* samples;
     };
+    explicit SampleAliasTable(SampleAliasTableConfig c = {}) : m_cfg(c) {}
     void run(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed,
              uint64_t counterBase = 0) const;
+
+  private:
+    SampleAliasTableConfig m_cfg;
+};
+
+// cooperativeSamplingSize 0: independent samples (the exact lower-bound search
+// per sample); W > 0: groups of 64 consecutive counters share a window of at
+// most W CDF entries, narrowed by 64-part searches (pArraySearch) or by halving
+// (InverseTransformSamplingConfig, src/device/wrs/its/sampling/
+// InverseTransformSampling.hpp:69-81, whose default is W = 4096 with 64-part
+// search)
+struct InverseTransformSamplingConfig {
+    uint32_t cooperativeSamplingSize = 0;
+    bool pArraySearch = true;
 };
 
 class InverseTransformSampling {
@@ -154,8 +176,12 @@ class InverseTransformSampling {
         const float* cmfLo;  // may be null
         uint32_t* samples;
     };
+    explicit InverseTransformSampling(InverseTransformSamplingConfig c = {}) : m_cfg(c) {}
     void run(hipStream_t s, const Buffers& b, uint64_t N, uint64_t S, uint64_t seed,
              uint64_t counterBase = 0) const;
+
+  private:
+    InverseTransformSamplingConfig m_cfg;
 };
 
 // AliasTable = PSA build + alias sampling (src/device/wrs/alias/AliasTable.hpp:75-112)

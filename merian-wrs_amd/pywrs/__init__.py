@@ -27,7 +27,7 @@ SPLIT_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float64
 EXPORTS = [
     "wrs_plan_create", "wrs_plan_destroy", "wrs_psa_build", "wrs_psa_build_debug", "wrs_mean",
     "wrs_prefix_partition", "wrs_scan_inclusive", "wrs_its_build", "wrs_alias_sample",
-    "wrs_its_sample", "wrs_cutpoint_build", "wrs_cutpoint_sample", "wrs_philox", "wrs_generate",
+    "wrs_its_sample", "wrs_alias_sample_window", "wrs_its_sample_window", "wrs_cutpoint_build", "wrs_cutpoint_sample", "wrs_philox", "wrs_generate",
     "wrs_batch_plan_create",
     "wrs_batch_plan_destroy", "wrs_psa_build_batched", "wrs_alias_sample_batched",
     "wrs_autotune_psa", "wrs_autotune_batched", "wrs_histogram", "wrs_sample_stats",
@@ -109,6 +109,8 @@ def lib():
         "wrs_its_build": [vp, vp, u64, vp, vp, vp],
         "wrs_alias_sample": [vp, u64, u64, u64, u64, vp, vp],
         "wrs_its_sample": [vp, vp, u64, u64, u64, u64, vp, vp],
+        "wrs_alias_sample_window": [vp, u64, u64, u64, u64, u32, vp, vp],
+        "wrs_its_sample_window": [vp, vp, u64, u64, u64, u64, u32, i32, vp, vp],
         "wrs_cutpoint_build": [vp, vp, u64, vp, u64, vp],
         "wrs_cutpoint_sample": [vp, vp, u64, vp, u64, u64, u64, u64, vp, vp],
         "wrs_philox": [u64, u32, u64, u64, vp, vp],
@@ -448,6 +450,22 @@ def its_sample(cdf_hi, cdf_lo, n: int, s: int, seed: int, out, counter_base: int
                stream=None) -> None:
     check(lib().wrs_its_sample(_p(cdf_hi), _p(cdf_lo), n, s, seed, counter_base, _p(out),
                                _h(stream)))
+
+
+def alias_sample_window(table, n: int, s: int, seed: int, window: int, out,
+                        counter_base: int = 0, stream=None) -> None:
+    """Windowed alias sampling: groups of 64 counters share a window of buckets
+    (exact marginals, samples of a group not independent)."""
+    check(lib().wrs_alias_sample_window(_p(table), n, s, seed, counter_base, int(window), _p(out),
+                                        _h(stream)))
+
+
+def its_sample_window(cdf_hi, cdf_lo, n: int, s: int, seed: int, window: int, out,
+                      parts: bool = True, counter_base: int = 0, stream=None) -> None:
+    """Windowed ITS sampling: groups of 64 counters share a window of <= window
+    CDF entries (exact marginals, samples of a group not independent)."""
+    check(lib().wrs_its_sample_window(_p(cdf_hi), _p(cdf_lo), n, s, seed, counter_base,
+                                      int(window), int(bool(parts)), _p(out), _h(stream)))
 
 
 def histogram(samples, s: int, n: int, hist, clear: bool = True, stream=None) -> None:
