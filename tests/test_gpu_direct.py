@@ -138,3 +138,28 @@ def test_direct_matches_oracle_and_reuses_plan(wrs, oracle):
                                              split_rule=oracle.SPLIT_LOWER_BOUND)
         assert np.array_equal(t["a"], a)
         np.testing.assert_allclose(t["p"].astype(np.float64), p, rtol=3e-7, atol=1e-7)
+
+
+def test_direct_tuning_in_debug_and_batched_builds(wrs, oracle):
+    """Debug builds (which read the full partition arrays) and batch plans
+    build in the classic form whatever the tuning says: same tables."""
+    n = 200003
+    case("direct ignored", n)
+    w = oracle.gen_zipf(4, 1.1, n)
+    dw = wrs.to_device(w)
+    ref = build(wrs, dw, n, wrs.PREC_COMPENSATED, dict(direct=0))
+    plan = wrs.Plan(n, tune=dict(direct=1))
+    t = wrs.empty(n, wrs.ALIAS_DTYPE)
+    idx = wrs.empty(n, np.uint32)
+    plan.psa_build_debug(dw, n, t, dict(indices=idx))
+    wrs.synchronize()
+    assert same(t.numpy(), ref)
+    off = np.array([0, 1000, 70000, n], np.uint64)
+    tables = {}
+    for d in (0, 1):
+        bp = wrs.BatchPlan(off, tune=dict(direct=d))
+        tb = wrs.empty(n, wrs.ALIAS_DTYPE)
+        bp.build(dw, tb)
+        wrs.synchronize()
+        tables[d] = tb.numpy()
+    assert same(tables[0], tables[1])
