@@ -301,7 +301,7 @@ struct MergeShared {
     uint2 raw[TILE + 4];  // keys during the merge, entries afterwards
     uint32_t idx[TILE + 4];
     float w[TILE + 4];
-    int64_t wave_l[BLOCK / kWave], wave_h[BLOCK / kWave];
+    int64_t wave_l[BLOCK / kWave];
 };
 
 // Indices and weights of one block's slots, in registers between their load
@@ -466,68 +466,58 @@ __device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, double m
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
     const uint32_t ne = nl + nh;
-    int64_t* s_key = reinterpret_cast<int64_t*>(s.raw);
 
     // ---- keys -----------------------------------------------------------
     const int qs = 61 - ilogb(total);
     const int64_t mean_q = quant(mean, qs);
     const uint32_t e0 = (uint32_t)(kPer * t);
-    int64_t inc[kPer];
-    int64_t ml = 0, mh = 0;
+    // One running sum over all slots, S(e) = sum_{e' < e} inc(e') (lights come
+    // first, so a_r = S(r) and b_q = S(nl + q + 1) - S(nl)).  Every increment is
+    // below 2^62 and S below 2^63 + a few ulps, so S is exact in uint64; the
+    // slot of A[r] (r <= nl) holds S(r), the slot of heavy q's key S(nl + q + 1).
+    uint64_t inc[kPer];
+    uint64_t mine = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t e = e0 + k;
         inc[k] = 0;
         if (e < ne) {
             const int64_t wq = quant_w(s.w[sw4(e)], qs);
-            inc[k] = e < nl ? mean_q - wq : wq - mean_q;
-            if (e < nl) ml += inc[k];
-            else mh += inc[k];
+            inc[k] = (uint64_t)(e < nl ? mean_q - wq : wq - mean_q);
         }
+        mine += inc[k];
     }
-    int64_t il = ml, ih = mh;
+    uint64_t incl = mine;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
-        const int64_t ol = shfl_up(il, d);
-        const int64_t oh = shfl_up(ih, d);
-        if (lane >= d) {
-            il += ol;
-            ih += oh;
-        }
+        const uint64_t o = shfl_up(incl, d);
+        if (lane >= d) incl += o;
     }
-    int64_t rl = shfl_up(il, 1), rh = shfl_up(ih, 1);  // exclusive per lane
-    if (lane == 0) rl = rh = 0;
-    if (lane == kWave - 1) {
-        s.wave_l[wid] = il;
-        s.wave_h[wid] = ih;
-    }
+    uint64_t run = shfl_up(incl, 1);  // exclusive per lane
+    if (lane == 0) run = 0;
+    if (lane == kWave - 1) s.wave_l[wid] = (int64_t)incl;
     __syncthreads();
-    int64_t tot_l = 0;
-    for (int q = 0; q < BLOCK / kWave; ++q) {
-        if (q < wid) {
-            rl += s.wave_l[q];
-            rh += s.wave_h[q];
-        }
-        tot_l += s.wave_l[q];
-    }
+    for (int q = 0; q < BLOCK / kWave; ++q)
+        if (q < wid) run += (uint64_t)s.wave_l[q];
     // heavy keys carry the split offset, so that "heavy first" is B[q] <= A[r]
     const double D = (double)n0 * mean - (c0.lp + c0.hp);
     const int64_t D_q = quant(D, qs);
     const double dres = ldexp((double)D_q, -qs) - D + mean;  // residual = (B - A) 2^-qs + dres
+    uint64_t* s_sum = reinterpret_cast<uint64_t*>(s.raw);
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
+    for (int k = 0; k <= kPer; ++k) {
         const uint32_t e = e0 + k;
-        if (e < ne) {
-            if (e < nl) {  // exclusive running sum
-                s_key[sw8(e)] = rl;
-                rl += inc[k];
-            } else {       // inclusive running sum, stored at slot e + 1
-                rh += inc[k];
-                s_key[sw8(e + 1u)] = rh - D_q;
+        if (e == nl) s_sum[sw8(nl)] = run;  // A[nl] = S(nl) (the next thread may store it too)
+        if (k < kPer && e < ne) {
+            if (e < nl) {  // S(e), before the slot's own increment
+                s_sum[sw8(e)] = run;
+                run += inc[k];
+            } else {       // S(e + 1), stored at slot e + 1
+                run += inc[k];
+                s_sum[sw8(e + 1u)] = run;
             }
         }
     }
-    if (t == 0) s_key[sw8(nl)] = tot_l;  // a at i1
     __syncthreads();
 
     // ---- merge: thread t emits buckets [kPer*t, kPer*t + kPer) ----------
@@ -539,8 +529,11 @@ __device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, double m
         if constexpr (PREC == kFP32) return (float)((float)x / (float)mean);
         else return (float)(x * inv_mean);
     };
-    auto A = [&](uint32_t r) -> int64_t { return s_key[sw8(r)]; };           // r in [0, nl]
-    auto B = [&](uint32_t q) -> int64_t { return s_key[sw8(nl + 1u + q)]; };  // q in [0, nh)
+    const uint64_t s_nl = s_sum[sw8(nl)];
+    auto A = [&](uint32_t r) -> int64_t { return (int64_t)s_sum[sw8(r)]; };  // r in [0, nl]
+    auto B = [&](uint32_t q) -> int64_t {                                    // q in [0, nh)
+        return (int64_t)(s_sum[sw8(nl + 1u + q)] - s_nl) - D_q;
+    };
     const uint32_t d0 = e0;
     uint32_t slot[kPer];
     uint2 ent[kPer];
