@@ -344,11 +344,12 @@ struct MergeStage {
 // j0..j1 from idx/pw; the nl lights i0..i1-1 are the lights (!(w > pivot), the
 // partition's test) of the element range [x0, x1) = [c0.xi, c1.xi), in
 // element order.  A range of at most kRounds * BLOCK elements (every block
-// of a distribution whose heavies do not cluster) is loaded whole, one round
-// of BLOCK consecutive elements after the other, and compacted by a block
-// scan of the per-(round, wave) light counts; a longer range (a run of
+// of a distribution whose heavies do not cluster) is loaded whole: each wave
+// takes a contiguous share of it (a multiple of 64 elements, the waves in
+// order) in rounds of 64 consecutive elements, counts its lights by ballots,
+// and one exclusive scan of the per-wave counts gives every light its slot; a longer range (a run of
 // heavies between two of the block's lights) locates each light by
-// light_pos.  s_cnt holds kRounds * BLOCK / kWave counts.  Ends with the slots
+// light_pos.  s_cnt holds BLOCK / kWave counts.  Ends with the slots
 // written, not synchronised.
 template <int TILE, int BLOCK>
 __device__ __forceinline__ void stage_direct(MergeShared<TILE, BLOCK>& s,
@@ -360,7 +361,6 @@ __device__ __forceinline__ void stage_direct(MergeShared<TILE, BLOCK>& s,
     constexpr int kPer = TILE / BLOCK;
     constexpr int kRounds = 2 * kPer;
     constexpr int kWaves = BLOCK / kWave;
-    constexpr int kCounts = kRounds * kWaves;
     const uint32_t t = threadIdx.x;
     const int lane = lane_id();
     const int wid = (int)t / kWave;
@@ -383,12 +383,16 @@ __device__ __forceinline__ void stage_direct(MergeShared<TILE, BLOCK>& s,
             gw[k] = pw[pos];
         }
     }
+    // wave wid compacts range offsets [wid Q, (wid+1) Q) in rounds of kWave
+    // consecutive elements (Q: a multiple of kWave, at most kRounds rounds)
+    const uint32_t Q = ((span + kWaves - 1) / kWaves + kWave - 1) & ~(uint32_t)(kWave - 1);
+    const uint32_t ow = (uint32_t)wid * Q + (uint32_t)lane;
     float lw[kRounds];
     if (dense) {
 #pragma unroll
         for (int k = 0; k < kRounds; ++k) {
-            if ((uint32_t)BLOCK * k < span) {
-                const uint32_t o = t + (uint32_t)BLOCK * k;  // offset in the range
+            if ((uint32_t)kWave * k < Q) {  // wave-uniform
+                const uint32_t o = ow + (uint32_t)kWave * k;  // offset in the range
                 lw[k] = o < span ? ls.w[x0 + o] : 0.0f;
             }
         }
@@ -403,37 +407,31 @@ __device__ __forceinline__ void stage_direct(MergeShared<TILE, BLOCK>& s,
     }
     if (dense) {
         uint64_t bal[kRounds];
+        uint32_t cnt = 0;
 #pragma unroll
         for (int k = 0; k < kRounds; ++k) {
             bal[k] = 0;
-            if ((uint32_t)BLOCK * k < span) {
-                const uint32_t o = t + (uint32_t)BLOCK * k;
+            if ((uint32_t)kWave * k < Q) {
+                const uint32_t o = ow + (uint32_t)kWave * k;
                 bal[k] = __ballot(o < span && !(lw[k] > pivot));
-            }
-            if (lane == 0) s_cnt[k * kWaves + wid] = (uint32_t)__popcll(bal[k]);
-        }
-        __syncthreads();
-        if (wid == 0) {  // exclusive scan of the counts in (round, wave) order
-            uint32_t carry = 0;
-            for (int q0 = 0; q0 < kCounts; q0 += kWave) {
-                const int q = q0 + lane;
-                const uint32_t v = q < kCounts ? s_cnt[q] : 0u;
-                const uint32_t incl = wave_incl_u(v);
-                if (q < kCounts) s_cnt[q] = carry + incl - v;
-                carry += shfl(incl, kWave - 1);
+                cnt += (uint32_t)__popcll(bal[k]);
             }
         }
+        if (lane == 0) s_cnt[wid] = cnt;
         __syncthreads();
+        uint32_t e_round = 0;  // lights before this round of this wave
+        for (int q = 0; q < kWaves; ++q) e_round += q < wid ? s_cnt[q] : 0u;
         const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
         for (int k = 0; k < kRounds; ++k) {
             if ((bal[k] >> lane) & 1ull) {
-                const uint32_t e = s_cnt[k * kWaves + wid] + (uint32_t)__popcll(bal[k] & below);
+                const uint32_t e = e_round + (uint32_t)__popcll(bal[k] & below);
                 if (e < nl) {
-                    s.idx[sw4(e)] = x0 + t + (uint32_t)BLOCK * k;
+                    s.idx[sw4(e)] = x0 + ow + (uint32_t)kWave * k;
                     s.w[sw4(e)] = lw[k];
                 }
             }
+            e_round += (uint32_t)__popcll(bal[k]);
         }
     } else {
         for (uint32_t e = t; e < nl; e += BLOCK) {
@@ -608,7 +606,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     const uint32_t nh = c1.j - c0.j;  // heavies packed here (heavy c1.j is current at n1)
     const uint32_t ne = nl + nh;      // == n1 - n0 (- 1 in the last block)
     if constexpr (DIRECT) {
-        __shared__ uint32_t s_cnt[2 * TILE / kWave];
+        __shared__ uint32_t s_cnt[BLOCK / kWave];
         stage_direct<TILE, BLOCK>(s, s_cnt, pw, idx, ls, pivot, N, H, c0, c1);
     } else {
         MergeStage<TILE, BLOCK> st;
