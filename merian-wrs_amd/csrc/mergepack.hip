@@ -220,6 +220,171 @@ __device__ __forceinline__ Coarse coarse_split(const float* __restrict__ pw,
     return Coarse{is, js, (double)lp_t, (double)hp_t, xs, 0u};
 }
 
+// ---- the same split, two boundaries per wave ---------------------------------
+// coarse_split is latency-bound (each wave waits on a chain of dependent loads)
+// and a launch has more boundaries than the device holds waves, so k_coarse
+// runs two boundaries per wave: each half-wave of 32 lanes emulates the 64
+// lanes of coarse_split with two registers per lane (virtual lane v = sl in
+// register 0, v = 32 + sl in register 1), with the same candidates, the same
+// additions in the same order and the same selections, hence the same result.
+struct VLane {
+    int sl;  // lane in the half-wave
+    int h;   // half: 0 or 1
+};
+
+// ballot over the 64 virtual lanes of this half
+__device__ __forceinline__ uint64_t vballot(bool p0, bool p1, const VLane& v) {
+    const uint64_t b0 = (__ballot(p0) >> (32 * v.h)) & 0xffffffffull;
+    const uint64_t b1 = (__ballot(p1) >> (32 * v.h)) & 0xffffffffull;
+    return b0 | (b1 << 32);
+}
+
+// the value virtual lane src holds (src in [0, 64), any per lane)
+template <class T>
+__device__ __forceinline__ T vshfl(T x0, T x1, int src, const VLane& v) {
+    const int l = 32 * v.h + (src & 31);
+    const T a = __shfl(x0, l, kWave);
+    const T b = __shfl(x1, l, kWave);
+    return src < 32 ? a : b;
+}
+
+// wave_excl_scan over the 64 virtual lanes: step d adds virtual lane p - d to
+// lane p (left operand), as the 64-lane Hillis-Steele scan does
+template <class T>
+__device__ __forceinline__ void vexcl_scan(T& x0, T& x1, const VLane& v) {
+    T i0 = x0, i1 = x1;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const int l = 32 * v.h + ((v.sl - d) & 31);
+        const T a0 = __shfl(i0, l, kWave);  // register 0 of lane (sl - d) mod 32
+        const T a1 = __shfl(i1, l, kWave);
+        const T n0 = v.sl >= d ? a0 + i0 : i0;          // p = sl
+        const T n1 = (v.sl >= d ? a1 : a0) + i1;        // p = 32 + sl (>= d)
+        i0 = n0;
+        i1 = n1;
+    }
+    i1 = i0 + i1;  // d = 32
+    const int l = 32 * v.h + ((v.sl - 1) & 31);
+    const T b0 = __shfl(i0, l, kWave);
+    const T b1 = __shfl(i1, l, kWave);
+    x0 = v.sl >= 1 ? b0 : (T)0;
+    x1 = v.sl >= 1 ? b1 : b0;  // p = 32: incl(31), register 0 of lane 31
+}
+
+template <int PREC, bool DIRECT>
+__device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw,
+                                                    const double* __restrict__ samp_h,
+                                                    const double* __restrict__ samp_l, uint32_t N,
+                                                    uint32_t H, typename RealT<PREC>::T mean,
+                                                    typename RealT<PREC>::T light_sum, uint32_t n,
+                                                    const uint32_t* __restrict__ idx,
+                                                    const LightSrc& ls) {
+    using T = typename RealT<PREC>::T;
+    const uint32_t L = N - H;
+    const VLane v{lane_id() & 31, lane_id() >> 5};
+    if (n == 0 || H == 0 || L == 0) {
+        const uint32_t x0 = DIRECT && H != 0 && L != 0 ? ls.lpos[0] : 0u;  // light 0
+        return Coarse{0u, 0u, 0.0, 0.0, x0, 0u};
+    }
+    if (n >= N) return Coarse{L, H - 1u, (double)light_sum, 0.0, N, 0u};
+    const T target = mean * (T)n;
+    const uint32_t lo = n > L ? n - L : 0u;
+    const uint32_t hi = min(n, H - 1u);
+    // phase 1: largest multiple m*kSample in [lo, hi] with sigma <= target
+    const uint32_t m_lo = (lo + kSample - 1u) / kSample;
+    const uint32_t m_hi = hi / kSample;
+    int64_t best = -1;
+    if (m_lo <= m_hi) {
+        uint32_t a = m_lo, b = m_hi;
+        for (;;) {
+            const uint32_t R = b - a + 1u;
+            const uint32_t step = (R + kWave - 1u) / kWave;
+            bool pr[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const uint32_t m = a + (uint32_t)(v.sl + 32 * c) * step;
+                pr[c] = false;
+                if (m <= b) {
+                    const uint32_t j = m * kSample;
+                    const uint32_t i = min(n - j, L);
+                    pr[c] = lp_sample<T>(samp_l, i, L, light_sum) + hp_sample<T>(samp_h, j) <=
+                            target;
+                }
+            }
+            const uint64_t bal = vballot(pr[0], pr[1], v);
+            if (bal == 0) break;
+            const int t = 63 - __builtin_clzll(bal);
+            best = (int64_t)a + (int64_t)t * step;
+            if (step == 1u) break;
+            a = (uint32_t)best + 1u;
+            b = min(b, (uint32_t)best + step - 1u);
+            if (a > b) break;
+        }
+    }
+    const uint32_t base = best >= 0 ? (uint32_t)best * kSample : (lo / kSample) * kSample;
+    // phase 2: resolve j in [base, base + 63] with the gathered weights
+    const uint32_t i_top = n - base;  // multiple of kSample
+    const uint32_t anchor = min(i_top, L);
+    const T lp_anchor = lp_sample<T>(samp_l, anchor, L, light_sum);
+    const T hp_base = hp_sample<T>(samp_h, base);
+    T hw[2], lw[2];
+    uint32_t lx[2] = {0u, 0u};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int vl = v.sl + 32 * c;
+        const uint32_t j = base + (uint32_t)vl;
+        hw[c] = j < H ? (T)pw[j] : (T)0;
+        const int64_t lrank = (int64_t)anchor - 1 - vl;  // virtual lane e: light rank anchor-1-e
+        lw[c] = (T)0;
+        if (lrank >= 0) {
+            if constexpr (DIRECT) {
+                lx[c] = light_pos(idx, ls.lpos, (uint32_t)lrank, L, H);
+                lw[c] = (T)ls.w[lx[c]];
+            } else {
+                lw[c] = (T)pw[N - 1u - (uint32_t)lrank];
+            }
+        }
+    }
+    vexcl_scan<T>(hw[0], hw[1], v);  // HP(base + e) - HP(base)
+    vexcl_scan<T>(lw[0], lw[1], v);  // sum of light ranks [anchor - e, anchor)
+    T lp[2], hp[2];
+    bool pr[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int vl = v.sl + 32 * c;
+        const uint32_t j = base + (uint32_t)vl;
+        hp[c] = hp_base + hw[c];
+        const int64_t i = (int64_t)n - (int64_t)j;
+        int64_t e64 = (int64_t)anchor - i;
+        e64 = e64 < 0 ? 0 : (e64 > kWave - 1 ? kWave - 1 : e64);
+        const T suf = vshfl<T>(lw[0], lw[1], (int)e64, v);
+        lp[c] = (T)0;
+        if (j > hi) {
+            pr[c] = false;
+        } else if (j < lo || i > (int64_t)anchor) {
+            pr[c] = true;
+        } else {
+            lp[c] = lp_anchor - suf;
+            pr[c] = lp[c] + hp[c] <= target;
+        }
+    }
+    const uint64_t bal = vballot(pr[0], pr[1], v);
+    int t = bal ? 63 - __builtin_clzll(bal) : (int)(lo - base);
+    if ((uint32_t)t + base < lo) t = (int)(lo - base);
+    const T lp_t = vshfl<T>(lp[0], lp[1], t, v);
+    const T hp_t = vshfl<T>(hp[0], hp[1], t, v);
+    const uint32_t js = base + (uint32_t)t;
+    const uint32_t is = n - js;  // in [anchor - 63, anchor]
+    uint32_t xs = 0;
+    if constexpr (DIRECT) {
+        int64_t sel = (int64_t)anchor - 1 - (int64_t)is;
+        sel = sel < 0 ? 0 : (sel > kWave - 1 ? kWave - 1 : sel);
+        const uint32_t xl = vshfl<uint32_t>(lx[0], lx[1], (int)sel, v);
+        xs = is == anchor ? (anchor < L ? ls.lpos[anchor / kSample] : N) : xl;
+    }
+    return Coarse{is, js, (double)lp_t, (double)hp_t, xs, 0u};
+}
+
 template <int PREC, bool DIRECT>
 __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 const double* __restrict__ samp_h,
@@ -228,12 +393,13 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 uint32_t tile, Coarse* __restrict__ out,
                                                 const uint32_t* __restrict__ idx, LightSrc ls) {
     using T = typename RealT<PREC>::T;
-    const uint32_t b = blockIdx.x * 4u + threadIdx.x / kWave;
-    if (b >= nbound) return;  // whole wave exits together
+    const uint32_t b = blockIdx.x * 8u + threadIdx.x / 32u;  // one boundary per half-wave
+    if (b >= nbound) return;  // the whole half-wave exits together
     const uint32_t n = min(b * tile, N);
-    const Coarse c = coarse_split<PREC, DIRECT>(pw, samp_h, samp_l, N, prm->heavy_count,
-                                                (T)prm->mean, (T)prm->light_sum, n, idx, ls);
-    if (lane_id() == 0) out[b] = c;
+    const Coarse c = coarse_split_half<PREC, DIRECT>(pw, samp_h, samp_l, N, prm->heavy_count,
+                                                     (T)prm->mean, (T)prm->light_sum, n, idx,
+                                                     ls);
+    if ((lane_id() & 31) == 0) out[b] = c;
 }
 
 // Segmented form: boundary p of the flattened boundary list belongs to segment
@@ -928,7 +1094,7 @@ hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, c
                          hipStream_t s, const uint32_t* idx, LightSrc ls) {
     const uint32_t N = (uint32_t)n;
     const uint32_t nbound = (uint32_t)merge_blocks_for(n, ms.tile) + 1u;
-    const uint32_t cgrid = (nbound + 3u) / 4u;
+    const uint32_t cgrid = (nbound + 7u) / 8u;  // 8 boundaries per 256-thread block
     // the split arithmetic is FP32 in FP32 mode and FP64 in all others
 #define WRS_COARSE(P, D)                                                                    \
     k_coarse<P, D><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, ms.tile, bounds, \
