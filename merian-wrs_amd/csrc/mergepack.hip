@@ -142,7 +142,7 @@ __device__ __forceinline__ Coarse coarse_split(const float* __restrict__ pw,
     const uint32_t lo = n > L ? n - L : 0u;
     const uint32_t hi = min(n, H - 1u);
     // phase 1: largest multiple m*kSample in [lo, hi] with sigma <= target
-    const uint32_t m_lo = (lo + kSample - 1u) / kSample;
+    const uint32_t m_lo = (uint32_t)(((uint64_t)lo + kSample - 1u) / kSample);
     const uint32_t m_hi = hi / kSample;
     int64_t best = -1;
     if (m_lo <= m_hi) {
@@ -291,7 +291,7 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
     const uint32_t lo = n > L ? n - L : 0u;
     const uint32_t hi = min(n, H - 1u);
     // phase 1: largest multiple m*kSample in [lo, hi] with sigma <= target
-    const uint32_t m_lo = (lo + kSample - 1u) / kSample;
+    const uint32_t m_lo = (uint32_t)(((uint64_t)lo + kSample - 1u) / kSample);
     const uint32_t m_hi = hi / kSample;
     int64_t best = -1;
     if (m_lo <= m_hi) {
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
     using T = typename RealT<PREC>::T;
     const uint32_t b = blockIdx.x * 8u + threadIdx.x / 32u;  // one boundary per half-wave
     if (b >= nbound) return;  // the whole half-wave exits together
-    const uint32_t n = min(b * tile, N);
+    const uint32_t n = (uint32_t)min((uint64_t)b * tile, (uint64_t)N);  // N < 2^32 <= b * tile
     const Coarse c = coarse_split_half<PREC, DIRECT>(pw, samp_h, samp_l, N, prm->heavy_count,
                                                      (T)prm->mean, (T)prm->light_sum, n, idx,
                                                      ls);
@@ -765,7 +765,7 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     const uint32_t t = threadIdx.x;
     const uint32_t L = N - H;
     if (H == 0 || L == 0) {  // all weights equal the mean up to its rounding: identity
-        for (uint32_t x = n0 + t; x < n1; x += BLOCK) table[x] = AliasEntry{1.0f, x};
+        for (uint32_t e = t; e < n1 - n0; e += BLOCK) table[n0 + e] = AliasEntry{1.0f, n0 + e};
         return;
     }
     const uint32_t nl = c1.i - c0.i;  // lights packed here
@@ -800,8 +800,8 @@ __global__ __launch_bounds__(BLOCK) void k_mergepack(const float* __restrict__ p
                                                      AliasEntry* __restrict__ table,
                                                      LightSrc ls) {
     const uint32_t b = blockIdx.x;
-    const uint32_t n0 = b * (uint32_t)TILE;
-    const uint32_t n1 = min(n0 + (uint32_t)TILE, N);
+    const uint32_t n0 = b * (uint32_t)TILE;  // < N
+    const uint32_t n1 = (uint32_t)min((uint64_t)n0 + TILE, (uint64_t)N);
     mergepack_block<PREC, TILE, BLOCK, DIRECT>(pw, idx, N, prm->heavy_count, prm->mean,
                                                prm->total, n0, n1, bounds[b], bounds[b + 1],
                                                table, ls, prm->pivot);

@@ -500,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const float pivot = prm->pivot;
-    const uint32_t tiles = (n + kTile - 1) / (uint32_t)kTile;
+    const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);  // n + kTile - 1 may exceed 2^32
     const Carry excl =
         load_carry(states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords);
     float v[kItems];

@@ -496,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_emit(const float* __restrict__ 
     __shared__ ScanShared<PREC> s;
     using R = typename Acc1<PREC>::T;
     const uint32_t tile = blockIdx.x;
-    const uint32_t tiles = (n + kTile - 1) / (uint32_t)kTile;
+    const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);  // n + kTile - 1 may exceed 2^32
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const bool full = VEC && m == (uint32_t)kTile;
