@@ -224,6 +224,13 @@ def main() -> int:
              "merge_pack": f"k_mergepack<{0 if p == 0 else 1}, {mt}, {mt // mp}, {d_}>"}
     if lookback:  # one kernel: the aggregate pass is empty
         del kname["tile_aggregate"]
+    # every kernel instance of one build, as rocprofv3 names them
+    fp = "false" if p == 0 else "true"
+    build_kernels = [f"k_reduce<{fp}>", f"k_finalize<{fp}>"]
+    if not lookback:
+        build_kernels += [f"k_part_agg<{p}>", f"k_part_tilescan<{p}>"]
+    build_kernels += [kname["partition_emit"], f"k_coarse<{0 if p == 0 else 1}, {d_}>",
+                      f"k_repair_bounds_lb<{d_}>", kname["merge_pack"]]
     log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phase.items()))
     dom = max(kname, key=lambda k: phase[k])
     dom_ms = phase[dom]
@@ -493,6 +500,7 @@ def main() -> int:
                      "algorithmic_bytes": alg[dom], "avg_ms": round(dom_ms, 5),
                      "traffic": traffic},
         "phases_ms": {k: round(float(v), 5) for k, v in phase.items()},
+        "build_kernels": build_kernels,
         "copy_kernel_gbs": round(copy_gbs, 1),
         "samples_per_s": round(samples_per_s, 1),
         "sampling": {"S": S, "ms": round(sample_ms, 4),

@@ -17,16 +17,17 @@
 // exactly the merge-path diagonal of that merge.  This file therefore builds
 // the table as a two-level merge:
 //
-//  k_coarse   one wave per block boundary n = b*tile (tile = buckets per
-//             merge block, a multiple of kSample): the exact split
-//             (i, j) with LP(i), HP(j).  A 64-ary wave search over the sparse
+//  k_coarse   one half-wave per block boundary n = b*tile (tile = buckets
+//             per merge block, a multiple of kSample): the exact split
+//             (i, j) with LP(i), HP(j).  A 64-ary search over the sparse
 //             FP64 prefix samples the partition wrote (every kSample-th rank;
 //             at j = m*kSample the light rank n - j is also a multiple of
 //             kSample, so both prefixes are samples) brackets j to a window of
-//             kSample heavies; the window is resolved exactly by a wave scan of
+//             kSample heavies; the window is resolved exactly by a scan of
 //             the 64 heavy and light weights around it.  Weights are read from
 //             pw, the weights in partition order written by the partition
-//             next to the indices, so every access here is contiguous.
+//             next to the indices, so every access here is contiguous (the
+//             direct form reads the lights from the weights themselves).
 //  k_repair_bounds_lb  makes the splits monotone where rounding broke it.
 //  k_mergepack one block per `tile` buckets (tile / per threads, `per`
 //             buckets each; the shape is a plan parameter): the block's light
@@ -80,18 +81,6 @@ __device__ __forceinline__ T lp_sample(const double* __restrict__ samp_l, uint32
     return (T)samp_l[i / kSample - 1u];
 }
 
-template <class T>
-__device__ __forceinline__ T wave_excl_scan(T v) {
-    const int l = lane_id();
-    T incl = v;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const T o = __shfl_up(incl, (unsigned)d, kWave);
-        if (l >= d) incl = o + incl;
-    }
-    const T ex = __shfl_up(incl, 1u, kWave);
-    return l == 0 ? (T)0 : ex;
-}
 
 // Direct form: element index of the light of rank r < L.  It is r plus the
 // heavies before it; heavy h (element idx[h], heavies in element order) has
@@ -118,115 +107,13 @@ __device__ __forceinline__ uint32_t light_pos(const uint32_t* __restrict__ idx,
 
 }  // namespace
 
-// The exact split at n buckets of the distribution (pw, samples) of size N
-// with H heavies; executed by one full wave, result valid in every lane.  The
-// direct form reads the lights from ls (and the heavy indices idx to locate
-// them) and records the element index of the split's light.
-template <int PREC, bool DIRECT>
-__device__ __forceinline__ Coarse coarse_split(const float* __restrict__ pw,
-                                               const double* __restrict__ samp_h,
-                                               const double* __restrict__ samp_l, uint32_t N,
-                                               uint32_t H, typename RealT<PREC>::T mean,
-                                               typename RealT<PREC>::T light_sum, uint32_t n,
-                                               const uint32_t* __restrict__ idx,
-                                               const LightSrc& ls) {
-    using T = typename RealT<PREC>::T;
-    const uint32_t L = N - H;
-    const int lane = lane_id();
-    if (n == 0 || H == 0 || L == 0) {
-        const uint32_t x0 = DIRECT && H != 0 && L != 0 ? ls.lpos[0] : 0u;  // light 0
-        return Coarse{0u, 0u, 0.0, 0.0, x0, 0u};
-    }
-    if (n >= N) return Coarse{L, H - 1u, (double)light_sum, 0.0, N, 0u};
-    const T target = mean * (T)n;
-    const uint32_t lo = n > L ? n - L : 0u;
-    const uint32_t hi = min(n, H - 1u);
-    // phase 1: largest multiple m*kSample in [lo, hi] with sigma <= target
-    const uint32_t m_lo = (uint32_t)(((uint64_t)lo + kSample - 1u) / kSample);
-    const uint32_t m_hi = hi / kSample;
-    int64_t best = -1;
-    if (m_lo <= m_hi) {
-        uint32_t a = m_lo, b = m_hi;
-        for (;;) {
-            const uint32_t R = b - a + 1u;
-            const uint32_t step = (R + kWave - 1u) / kWave;
-            const uint32_t m = a + (uint32_t)lane * step;
-            bool pred = false;
-            if (m <= b) {
-                const uint32_t j = m * kSample;
-                const uint32_t i = min(n - j, L);
-                pred = lp_sample<T>(samp_l, i, L, light_sum) + hp_sample<T>(samp_h, j) <= target;
-            }
-            const uint64_t bal = __ballot(pred);
-            if (bal == 0) break;
-            const int t = 63 - __builtin_clzll(bal);
-            best = (int64_t)a + (int64_t)t * step;
-            if (step == 1u) break;
-            a = (uint32_t)best + 1u;
-            b = min(b, (uint32_t)best + step - 1u);
-            if (a > b) break;
-        }
-    }
-    const uint32_t base = best >= 0 ? (uint32_t)best * kSample : (lo / kSample) * kSample;
-    // phase 2: resolve j in [base, base + 63] with the gathered weights
-    const uint32_t j = base + (uint32_t)lane;
-    const T hw = j < H ? (T)pw[j] : (T)0;
-    const T hp = hp_sample<T>(samp_h, base) + wave_excl_scan<T>(hw);  // HP(base + lane)
-    const uint32_t i_top = n - base;                                   // multiple of kSample
-    const uint32_t anchor = min(i_top, L);
-    const T lp_anchor = lp_sample<T>(samp_l, anchor, L, light_sum);
-    const int64_t lrank = (int64_t)anchor - 1 - lane;                  // lane e: light rank anchor-1-e
-    T lw = (T)0;
-    uint32_t lx = 0;  // direct form: element index of light rank lrank
-    if (lrank >= 0) {
-        if constexpr (DIRECT) {
-            lx = light_pos(idx, ls.lpos, (uint32_t)lrank, L, H);
-            lw = (T)ls.w[lx];
-        } else {
-            lw = (T)pw[N - 1u - (uint32_t)lrank];
-        }
-    }
-    const T lsuf = wave_excl_scan<T>(lw);  // sum of light ranks [anchor - e, anchor)
-    const int64_t i = (int64_t)n - (int64_t)j;
-    // every lane takes part in the shuffle; e is clamped for lanes that do not use it
-    int64_t e64 = (int64_t)anchor - i;  // 0..63 where used
-    e64 = e64 < 0 ? 0 : (e64 > kWave - 1 ? kWave - 1 : e64);
-    const T suf = __shfl(lsuf, (int)e64, kWave);
-    bool pred;
-    T lp = (T)0;
-    if (j > hi) {
-        pred = false;
-    } else if (j < lo || i > (int64_t)anchor) {
-        pred = true;
-    } else {
-        lp = lp_anchor - suf;
-        pred = lp + hp <= target;
-    }
-    const uint64_t bal = __ballot(pred);
-    int t = bal ? 63 - __builtin_clzll(bal) : (int)(lo - base);
-    if ((uint32_t)t + base < lo) t = (int)(lo - base);
-    const T lp_t = __shfl(lp, t, kWave);
-    const T hp_t = __shfl(hp, t, kWave);
-    const uint32_t js = base + (uint32_t)t;
-    const uint32_t is = n - js;  // in [anchor - 63, anchor]
-    uint32_t xs = 0;
-    if constexpr (DIRECT) {
-        // lane anchor - 1 - is located light is, or it is the anchor itself
-        int64_t sel = (int64_t)anchor - 1 - (int64_t)is;
-        sel = sel < 0 ? 0 : (sel > kWave - 1 ? kWave - 1 : sel);
-        const uint32_t xl = __shfl(lx, (int)sel, kWave);
-        xs = is == anchor ? (anchor < L ? ls.lpos[anchor / kSample] : N) : xl;
-    }
-    return Coarse{is, js, (double)lp_t, (double)hp_t, xs, 0u};
-}
-
-// ---- the same split, two boundaries per wave ---------------------------------
-// coarse_split is latency-bound (each wave waits on a chain of dependent loads)
-// and a launch has more boundaries than the device holds waves, so k_coarse
-// runs two boundaries per wave: each half-wave of 32 lanes emulates the 64
-// lanes of coarse_split with two registers per lane (virtual lane v = sl in
-// register 0, v = 32 + sl in register 1), with the same candidates, the same
-// additions in the same order and the same selections, hence the same result.
+// ---- the exact split at a block boundary, two boundaries per wave ------------
+// The split search is bound by its chains of scattered loads and a launch has
+// more boundaries than the device holds waves, so a wave runs two boundaries:
+// each half-wave of 32 lanes works as 64 virtual lanes with two registers per
+// lane (virtual lane v = sl in register 0, v = 32 + sl in register 1).  The
+// scans below perform the additions of a 64-lane Hillis-Steele scan in the
+// same order, so the result does not depend on this mapping.
 struct VLane {
     int sl;  // lane in the half-wave
     int h;   // half: 0 or 1
@@ -248,8 +135,8 @@ __device__ __forceinline__ T vshfl(T x0, T x1, int src, const VLane& v) {
     return src < 32 ? a : b;
 }
 
-// wave_excl_scan over the 64 virtual lanes: step d adds virtual lane p - d to
-// lane p (left operand), as the 64-lane Hillis-Steele scan does
+// exclusive scan over the 64 virtual lanes, with the additions of a 64-lane
+// Hillis-Steele scan: step d adds virtual lane p - d to lane p (left operand)
 template <class T>
 __device__ __forceinline__ void vexcl_scan(T& x0, T& x1, const VLane& v) {
     T i0 = x0, i1 = x1;
@@ -271,6 +158,13 @@ __device__ __forceinline__ void vexcl_scan(T& x0, T& x1, const VLane& v) {
     x1 = v.sl >= 1 ? b1 : b0;  // p = 32: incl(31), register 0 of lane 31
 }
 
+// The exact split at n buckets of the distribution (pw, samples) of size N
+// with H heavies; executed by one half-wave, result valid in each of its lanes.
+// Phase 1: a 64-ary search over the FP64 prefix samples (every kSample-th
+// rank) for the largest multiple j of kSample with LP(n - j) + HP(j) <= n mean;
+// phase 2: the 64 heavy and light weights around it resolve the exact j.  The
+// direct form reads the lights from ls (the heavy indices idx locate them) and
+// records the element index of the split's light.
 template <int PREC, bool DIRECT>
 __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw,
                                                     const double* __restrict__ samp_h,
@@ -412,17 +306,18 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     const uint32_t* __restrict__ bound_seg, uint32_t nbound, const Params* __restrict__ seg_prm,
     uint32_t tile, Coarse* __restrict__ out) {
     using T = typename RealT<PREC>::T;
-    const uint32_t p = blockIdx.x * 4u + threadIdx.x / kWave;
-    if (p >= nbound) return;  // whole wave exits together
+    const uint32_t p = blockIdx.x * 8u + threadIdx.x / 32u;  // one boundary per half-wave
+    if (p >= nbound) return;  // the whole half-wave exits together
     const uint32_t g = bound_seg[p];
     const uint64_t off = seg_off[g];
     const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
     const uint32_t n = min((p - seg_b0[g]) * tile, N);
     const uint64_t so = seg_samp0[g];
-    const Coarse c = coarse_split<PREC, false>(pw + off, samp_h + so, samp_l + so, N,
-                                               seg_prm[g].heavy_count, (T)seg_prm[g].mean,
-                                               (T)seg_prm[g].light_sum, n, nullptr, LightSrc{});
-    if (lane_id() == 0) out[p] = c;
+    const Coarse c = coarse_split_half<PREC, false>(pw + off, samp_h + so, samp_l + so, N,
+                                                    seg_prm[g].heavy_count, (T)seg_prm[g].mean,
+                                                    (T)seg_prm[g].light_sum, n, nullptr,
+                                                    LightSrc{});
+    if ((lane_id() & 31) == 0) out[p] = c;
 }
 
 // Quantisation of the merge keys: x -> rint(x * 2^qs) with qs = 61 - ilogb(total),
@@ -1157,7 +1052,7 @@ hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const doub
                                 const Params* seg_prm, Coarse* bounds, AliasEntry* table,
                                 hipStream_t s) {
     if (!merge_shape_supported(ms)) return hipErrorInvalidValue;
-    const uint32_t cgrid = (nbound + 3u) / 4u;
+    const uint32_t cgrid = (nbound + 7u) / 8u;  // 8 boundaries per 256-thread block
     if (prec == kFP32)
         k_seg_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
                                                   bound_seg, nbound, seg_prm, ms.tile, bounds);

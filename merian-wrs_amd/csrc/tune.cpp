@@ -39,7 +39,7 @@ namespace wrst {
 
 namespace {
 
-constexpr int kReps = 5;
+constexpr int kReps = 9;
 constexpr int kWarm = 20;
 
 // the merge shapes tried, default first
