@@ -108,7 +108,7 @@ __device__ __forceinline__ uint32_t light_pos(const uint32_t* __restrict__ idx,
 }  // namespace
 
 // ---- the exact split at a block boundary, two boundaries per wave ------------
-// The split search is bound by its chains of scattered loads and a launch has
+// The split search is bound by its scattered sample loads and a launch has
 // more boundaries than the device holds waves, so a wave runs two boundaries:
 // each half-wave of 32 lanes works as 64 virtual lanes with two registers per
 // lane (virtual lane v = sl in register 0, v = 32 + sl in register 1).  The
@@ -160,8 +160,10 @@ __device__ __forceinline__ void vexcl_scan(T& x0, T& x1, const VLane& v) {
 
 // The exact split at n buckets of the distribution (pw, samples) of size N
 // with H heavies; executed by one half-wave, result valid in each of its lanes.
-// Phase 1: a 64-ary search over the FP64 prefix samples (every kSample-th
-// rank) for the largest multiple j of kSample with LP(n - j) + HP(j) <= n mean;
+// Phase 1: a 32-ary search over the FP64 prefix samples (every kSample-th
+// rank) for the largest multiple j of kSample with LP(n - j) + HP(j) <= n mean
+// (monotone in j, so any search order finds the same j; 32 candidates a round
+// cost fewer scattered sample loads than 64 for one more round at most);
 // phase 2: the 64 heavy and light weights around it resolve the exact j.  The
 // direct form reads the lights from ls (the heavy indices idx locate them) and
 // records the element index of the split's light.
@@ -190,22 +192,17 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
     int64_t best = -1;
     if (m_lo <= m_hi) {
         uint32_t a = m_lo, b = m_hi;
-        for (;;) {
+        for (;;) {  // 32 candidates a round, one per lane of the half-wave
             const uint32_t R = b - a + 1u;
-            const uint32_t step = (R + kWave - 1u) / kWave;
-            bool pr[2];
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const uint32_t m = a + (uint32_t)(v.sl + 32 * c) * step;
-                pr[c] = false;
-                if (m <= b) {
-                    const uint32_t j = m * kSample;
-                    const uint32_t i = min(n - j, L);
-                    pr[c] = lp_sample<T>(samp_l, i, L, light_sum) + hp_sample<T>(samp_h, j) <=
-                            target;
-                }
+            const uint32_t step = (R + 31u) / 32u;
+            const uint32_t m = a + (uint32_t)v.sl * step;
+            bool pr = false;
+            if (m <= b) {
+                const uint32_t j = m * kSample;
+                const uint32_t i = min(n - j, L);
+                pr = lp_sample<T>(samp_l, i, L, light_sum) + hp_sample<T>(samp_h, j) <= target;
             }
-            const uint64_t bal = vballot(pr[0], pr[1], v);
+            const uint64_t bal = vballot(pr, false, v);
             if (bal == 0) break;
             const int t = 63 - __builtin_clzll(bal);
             best = (int64_t)a + (int64_t)t * step;
