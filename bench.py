@@ -34,7 +34,9 @@ Also reported on the same line:
   sweep         S = 1e7 samples at N = 1e5 .. 1e8 (U(0,1) weights), the shape of
                 the reference's sampling plots: alias build + sample, ITS and
                 cutpoint sampling throughput, and the windowed samplers
-                (window 128; groups of 64 samples share a window);
+                (window 128; groups of 64 samples share a window); `cold`:
+                the same with a 512 MiB write before every call (the
+                reference flushes its L2 between runs);
   batched       4096 log-uniform-size LogNormal(0, 3) distributions built and
                 sampled (64 each) by one hipGraph vs one launch sequence per
                 distribution;
@@ -127,6 +129,22 @@ def time_it(fn, st, wrs, reps: int) -> float:
     e1.record(st)
     st.synchronize()
     return e0.elapsed_ms(e1) / reps
+
+
+def time_cold(fn, st, wrs, reps: int, flush) -> float:
+    """median ms of fn() with the caches flushed before every call: a 512 MiB
+    copy (flush = (dst, src, n)) evicts the L2s and the 256 MB MALL, as the
+    reference flushes its L2 between measured runs."""
+    e0, e1 = wrs.Event(), wrs.Event()
+    t = []
+    for _ in range(reps):
+        wrs.copy_f32(flush[0], flush[1], flush[2], stream=st)
+        e0.record(st)
+        fn()
+        e1.record(st)
+        st.synchronize()
+        t.append(e0.elapsed_ms(e1))
+    return float(np.median(t))
 
 
 def main() -> int:
@@ -412,6 +430,8 @@ def main() -> int:
         sweep = []
         Ss = 10_000_000
         osw = wrs.empty(Ss, np.uint32)
+        fn_ = 1 << 27  # 512 MiB of floats
+        flush = (wrs.empty(fn_, np.float32), wrs.empty(fn_, np.float32), fn_)
         for nn in (100_000, 1_000_000, 10_000_000, 100_000_000):
             wsw = wrs.empty(nn, np.float32)
             wrs.generate(wrs.GEN_UNIFORM01, nn, 0, wsw, stream=st)
@@ -438,16 +458,27 @@ def main() -> int:
                             st, wrs, 5)
             iw_ms = time_it(lambda: wrs.its_sample_window(hi_, lo_, nn, Ss, 5, 128, osw, stream=st),
                             st, wrs, 3)
+            # the same with the caches flushed before every call
+            bc = time_cold(lambda: pl.psa_build(wsw, nn, tsw, stream=st), st, wrs, 5, flush)
+            ac = time_cold(lambda: wrs.alias_sample(tsw, nn, Ss, 5, osw, stream=st), st, wrs, 5,
+                           flush)
+            ic = time_cold(lambda: wrs.its_sample(hi_, lo_, nn, Ss, 5, osw, stream=st), st, wrs,
+                           3, flush)
+            cc = time_cold(lambda: wrs.cutpoint_sample(hi_, lo_, nn, gd_, nn, Ss, 5, osw,
+                                                       stream=st), st, wrs, 3, flush)
             row = dict(n=nn, psa_build_ms=round(b_ms, 4), alias_sample_ms=round(a_ms, 4),
                        alias_build_plus_sample_per_s=round(Ss / ((b_ms + a_ms) * 1e-3), 1),
                        its_sample_per_s=round(Ss / (i_ms * 1e-3), 1),
                        cutpoint_sample_per_s=round(Ss / (c_ms * 1e-3), 1),
                        alias_window128_sample_per_s=round(Ss / (aw_ms * 1e-3), 1),
-                       its_window128_sample_per_s=round(Ss / (iw_ms * 1e-3), 1))
+                       its_window128_sample_per_s=round(Ss / (iw_ms * 1e-3), 1),
+                       cold=dict(alias_build_plus_sample_per_s=round(Ss / ((bc + ac) * 1e-3), 1),
+                                 its_sample_per_s=round(Ss / (ic * 1e-3), 1),
+                                 cutpoint_sample_per_s=round(Ss / (cc * 1e-3), 1)))
             sweep.append(row)
             log(f"sweep: {row}")
             del wsw, tsw, hi_, lo_, gd_, pl
-        del osw
+        del osw, flush
 
     # ---- CPU baseline: the reference CPU path restated (oracle) ------------
     cpu = None
