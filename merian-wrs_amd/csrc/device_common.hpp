@@ -154,8 +154,40 @@ __device__ __forceinline__ void st_nt4(float4* p, float4 v) {
     __builtin_nontemporal_store(x, reinterpret_cast<f32x4_t*>(p));
 }
 
+// Quantisation of the merge keys: x -> rint(x * 2^qs) as a 64-bit integer.
+__device__ __forceinline__ int64_t merge_quant(double x, int qs) {
+    return __double2ll_rn(ldexp(x, qs));
+}
+
 // ------------------------------------------------------------ wave ops ----
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// v + (the value of v DPP-moved by CTRL): both halves move, and lanes whose
+// source is outside the row, or whose row is masked off, add 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t add_dpp_u64(uint64_t v) {
+    const uint32_t vl = (uint32_t)v, vh = (uint32_t)(v >> 32);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)vl, CTRL, ROW_MASK, 0xf, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)vh, CTRL, ROW_MASK, 0xf, true);
+    unsigned c0, c1;
+    const uint32_t rl = __builtin_addc(vl, lo, 0u, &c0);
+    const uint32_t rh = __builtin_addc(vh, hi, c0, &c1);
+    return ((uint64_t)rh << 32) | rl;
+}
+
+// Inclusive scan of a 64-bit integer over the 64 lanes (modular addition, so
+// the association does not matter): shifts by 1, 2, 4, 8 lanes within each
+// row of 16, then the total of row 0 (2) added to row 1 (3) by row_bcast:15,
+// then the total of rows 0-1 added to rows 2-3 by row_bcast:31.
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
+    v = add_dpp_u64<0x111, 0xf>(v);  // row_shr:1
+    v = add_dpp_u64<0x112, 0xf>(v);  // row_shr:2
+    v = add_dpp_u64<0x114, 0xf>(v);  // row_shr:4
+    v = add_dpp_u64<0x118, 0xf>(v);  // row_shr:8
+    v = add_dpp_u64<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = add_dpp_u64<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
 
 template <class T>
 __device__ __forceinline__ T shfl_up(T v, int d) {

@@ -31,9 +31,11 @@ struct Params {
     double heavy_sum;      // sum of the heavy weights (written by the partition's last tile)
     double light_sum;      // sum of the light weights (idem)
     uint32_t repair_ticket;  // dynamic block ticket of the boundary repair
-    uint32_t reserved;
+    int32_t merge_qs;        // merge keys: x -> rint(x * 2^merge_qs), 61 - ilogb(total)
+    double inv_mean;         // 1 / mean (FP64 division)
+    int64_t mean_q;          // rint(mean * 2^merge_qs)
 };
-static_assert(sizeof(Params) == 64, "Params layout");
+static_assert(sizeof(Params) == 80, "Params layout");
 
 struct AliasEntry {
     float p;

@@ -35,12 +35,14 @@
 //             indices and weights (both contiguous runs of idx and pw, loaded
 //             in one round trip) are staged in LDS, the keys X and Y
 //             relative to the block start are rebuilt by an exact integer
-//             block scan, every thread finds its merge-path diagonal by a
-//             binary search in
-//             LDS and merges 4 buckets, and the entries are written back in
-//             rank order (coalesced: consecutive light ranks are consecutive
-//             ascending indices).  The last heavy of all gets (1, itself), as
-//             in pack.hpp:78-81.
+//             block scan (a DPP wave scan), and every item's entry follows
+//             from how many items of the other kind precede it in the merge:
+//             a binary search over the other sorted key list in LDS (a light
+//             continues from its predecessor's count, so it usually needs one
+//             comparison), with no walk of the merged order.  The entries are
+//             written back in rank order (coalesced: consecutive light ranks
+//             are consecutive ascending indices).  The last heavy of all gets
+//             (1, itself), as in pack.hpp:78-81.
 //
 // The split search runs in FP64 in the FP64-carry modes and in FP32 in FP32
 // mode, as the reference kernels do; the merge keys inside a block are exact
@@ -317,16 +319,12 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     if ((lane_id() & 31) == 0) out[p] = c;
 }
 
-// Quantisation of the merge keys: x -> rint(x * 2^qs) with qs = 61 - ilogb(total),
-// so that every local key and offset stays below 2^62 in magnitude.
-__device__ __forceinline__ int64_t quant(double x, int qs) { return __double2ll_rn(ldexp(x, qs)); }
-
 // quant of a float weight w >= 0 (+-0 included) without FP64 arithmetic, the
-// same value: w * 2^qs < 2^63 is exact in FP32 (an exponent shift; a result
-// below 2^-126 rounds to 0 either way), rintf rounds it to the nearest even
-// integer as the FP64 path does, and the integer-valued float converts to
-// int64 as its 24-bit significand shifted left (y >= 2^23) or by the exact
-// 32-bit conversion (y < 2^23).
+// value of merge_quant((double)w, qs): w * 2^qs < 2^63 is exact in FP32 (an
+// exponent shift; a result below 2^-126 rounds to 0 either way), rintf rounds
+// it to the nearest even integer as the FP64 path does, and the
+// integer-valued float converts to int64 as its 24-bit significand shifted
+// left (y >= 2^23) or by the exact 32-bit conversion (y < 2^23).
 __device__ __forceinline__ int64_t quant_w(float w, int qs) {
     const float y = rintf(ldexpf(w, qs));
     const uint32_t b = __float_as_uint(y) & 0x7fffffffu;
@@ -335,32 +333,52 @@ __device__ __forceinline__ int64_t quant_w(float w, int qs) {
     return ex >= 0 ? (int64_t)(sig << ex) : (int64_t)(uint32_t)fabsf(y);
 }
 
-// One block of TILE buckets [n0, n1) between boundary splits c0 and c1, BLOCK
-// threads, TILE / BLOCK buckets per thread.
-//
-// Local keys are exact 64-bit fixed-point running sums of non-negative
-// increments (w_q = quant(w), mean_q = quant(mean)):
-//     a_r = sum_{r' < r}  (mean_q - w_q(light i0 + r'))      r in [0, nl]
-//     b_q = sum_{q' <= q} (w_q(heavy j0 + q') - mean_q)       q in [0, nh)
-// so A and B are sorted whatever the rounding of the prefix sums, and the
-// merge below is always a permutation of the block's slots.  With the split
-// offset D = X(i0) - Y(j0) = n0*mean - LP(i0) - HP(j0) (FP64), heavy q goes
-// before light r iff Y(j0+q+1) <= X(i0+r), i.e. b_q - quant(D) <= a_r, and its
-// residual is Y(j0+q+1) - X(i0+r) + mean = (b_q - a_r) 2^-qs - D + mean.
-// LDS of one merge block.  Slots are XOR-swizzled within aligned groups of
-// four (sw8 for the 8-byte arrays, sw4 for the 4-byte ones): a thread owns
-// kPer consecutive slots, so without it the lanes of a wave hit the same
-// banks at a stride of kPer slots.
-__device__ __forceinline__ uint32_t sw8(uint32_t x) { return x ^ ((x >> 4) & 3u); }
-__device__ __forceinline__ uint32_t sw4(uint32_t x) { return x ^ ((x >> 5) & 3u); }
-
+// LDS of one merge block, slot e = the e-th item of the block: lights first
+// (slot e < nl is light i0 + e), then heavies (slot nl + q is heavy j0 + q,
+// q <= nh; the weight of heavy nh is not needed).  A thread owns kPer
+// consecutive slots and reads and writes them as 16-byte vectors, so the
+// arrays are 16-byte aligned and need no bank swizzle.
 template <int TILE, int BLOCK>
 struct MergeShared {
-    uint2 raw[TILE + 4];  // keys during the merge, entries afterwards
-    uint32_t idx[TILE + 4];
-    float w[TILE + 4];
-    int64_t wave_l[BLOCK / kWave];
+    __align__(16) uint64_t key[TILE + 4];  // running key sums S(e); the entries afterwards
+    __align__(16) uint32_t idx[TILE + 4];
+    __align__(16) float w[TILE + 4];
+    uint64_t wave_l[BLOCK / kWave];
+    int64_t dq;   // quant(D), D the block's split offset
+    double dres;  // residual offset (see merge_core)
 };
+
+// kPer consecutive words at p, p aligned to their total size (up to 16 bytes).
+template <int K>
+__device__ __forceinline__ void lds_load(const float* p, float (&v)[K]) {
+    if constexpr (K % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < K; i += 4) {
+            const float4 x = *reinterpret_cast<const float4*>(p + i);
+            v[i] = x.x;
+            v[i + 1] = x.y;
+            v[i + 2] = x.z;
+            v[i + 3] = x.w;
+        }
+    } else {
+        static_assert(K == 2, "merge: per-thread slots");
+        const float2 x = *reinterpret_cast<const float2*>(p);
+        v[0] = x.x;
+        v[1] = x.y;
+    }
+}
+template <int K>
+__device__ __forceinline__ void lds_store(uint64_t* p, const uint64_t (&v)[K]) {
+#pragma unroll
+    for (int i = 0; i < K; i += 2)
+        *reinterpret_cast<ulonglong2*>(p + i) = make_ulonglong2(v[i], v[i + 1]);
+}
+template <int K>
+__device__ __forceinline__ void lds_store(uint2* p, const uint2 (&v)[K]) {
+#pragma unroll
+    for (int i = 0; i < K; i += 2)
+        *reinterpret_cast<uint4*>(p + i) = make_uint4(v[i].x, v[i].y, v[i + 1].x, v[i + 1].y);
+}
 
 // Indices and weights of one block's slots, in registers between their load
 // and their store to LDS: slot e < nl is light i0 + e, slot nl <= e <= ne is
@@ -392,8 +410,8 @@ struct MergeStage {
 #pragma unroll
         for (int k = 0; k <= kPer; ++k) {
             const uint32_t e = t + (uint32_t)BLOCK * k;
-            s.idx[sw4(e <= ne ? e : (uint32_t)TILE + 1u)] = gi[k];
-            s.w[sw4(e < ne ? e : (uint32_t)TILE)] = gw[k];
+            s.idx[e <= ne ? e : (uint32_t)TILE + 1u] = gi[k];
+            s.w[e < ne ? e : (uint32_t)TILE] = gw[k];
         }
     }
 };
@@ -459,8 +477,8 @@ __device__ __forceinline__ void stage_direct(MergeShared<TILE, BLOCK>& s,
     for (int k = 0; k <= kPer; ++k) {
         if ((uint32_t)BLOCK * k <= nh) {
             const uint32_t h = t + (uint32_t)BLOCK * k;
-            s.idx[sw4(h <= nh ? nl + h : (uint32_t)TILE + 1u)] = gi[k];
-            s.w[sw4(h < nh ? nl + h : (uint32_t)TILE)] = gw[k];
+            s.idx[h <= nh ? nl + h : (uint32_t)TILE + 1u] = gi[k];
+            s.w[h < nh ? nl + h : (uint32_t)TILE] = gw[k];
         }
     }
     if (dense) {
@@ -485,8 +503,8 @@ __device__ __forceinline__ void stage_direct(MergeShared<TILE, BLOCK>& s,
             if ((bal[k] >> lane) & 1ull) {
                 const uint32_t e = e_round + (uint32_t)__popcll(bal[k] & below);
                 if (e < nl) {
-                    s.idx[sw4(e)] = x0 + ow + (uint32_t)kWave * k;
-                    s.w[sw4(e)] = lw[k];
+                    s.idx[e] = x0 + ow + (uint32_t)kWave * k;
+                    s.w[e] = lw[k];
                 }
             }
             e_round += (uint32_t)__popcll(bal[k]);
@@ -494,192 +512,204 @@ __device__ __forceinline__ void stage_direct(MergeShared<TILE, BLOCK>& s,
     } else {
         for (uint32_t e = t; e < nl; e += BLOCK) {
             const uint32_t x = light_pos(idx, ls.lpos, c0.i + e, L, H);
-            s.idx[sw4(e)] = x;
-            s.w[sw4(e)] = ls.w[x];
+            s.idx[e] = x;
+            s.w[e] = ls.w[x];
         }
     }
 }
 
-// Keys and merge of one block whose slots are staged in s (block
-// synchronised): leaves the entry of slot e at s.raw[sw8(e)], e < ne, and
-// ends synchronised.
+// The build scalars the merge uses (from Params).
+struct MergeConst {
+    double mean, inv_mean;
+    int64_t mean_q;
+    int qs;
+};
+
+// Keys and entries of one block whose slots are staged in s (block
+// synchronised, s.dq and s.dres set): leaves the entry of slot e at
+// ((uint2*)s.key)[e], e < ne, and ends synchronised.
 //
 // Local keys are exact 64-bit fixed-point running sums of non-negative
 // increments (w_q = quant(w), mean_q = quant(mean)):
 //     a_r = sum_{r' < r}  (mean_q - w_q(light i0 + r'))      r in [0, nl]
 //     b_q = sum_{q' <= q} (w_q(heavy j0 + q') - mean_q)       q in [0, nh)
-// so A and B are sorted whatever the rounding of the prefix sums, and the
-// merge below is always a permutation of the block's slots.  With the split
-// offset D = X(i0) - Y(j0) = n0*mean - LP(i0) - HP(j0) (FP64), heavy q goes
-// before light r iff Y(j0+q+1) <= X(i0+r), i.e. b_q - quant(D) <= a_r, and its
-// residual is Y(j0+q+1) - X(i0+r) + mean = (b_q - a_r) 2^-qs - D + mean.
+// so A and B are sorted whatever the rounding of the prefix sums.  With the
+// split offset D = X(i0) - Y(j0) = n0*mean - LP(i0) - HP(j0) (FP64), heavy q
+// goes before light r iff Y(j0+q+1) <= X(i0+r), i.e. B[q] = b_q - quant(D) <=
+// a_r = A[r]; its residual is Y(j0+q+1) - X(i0+r) + mean = (B[q] - A[r]) 2^-qs
+// + dres with dres = quant(D) 2^-qs - D + mean.  The sweep of the merged
+// order is never walked: an entry depends only on how many items of the other
+// kind precede its item,
+//     light r:  alias = heavy q(r),  q(r) = #{q < nh : B[q] <= A[r]}
+//               (q(r) == nh: heavy j0 + nh, current at the block's end),
+//     heavy q:  residual against light r(q) = #{r < nl : A[r] < B[q]}
+//               (A[nl] = S(nl) when every light precedes it), alias heavy q+1,
+// both found by binary search over the sorted keys (a light continues from
+// its predecessor's q, so it usually needs one comparison).
 template <int PREC, int TILE, int BLOCK>
-__device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, double mean, double total,
-                                           uint32_t n0, const Coarse& c0, uint32_t nl,
-                                           uint32_t nh) {
+__device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, const MergeConst& mc,
+                                           uint32_t nl, uint32_t nh) {
     constexpr int kPer = TILE / BLOCK;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
     const uint32_t ne = nl + nh;
+    const int qs = mc.qs;
 
-    // ---- keys -----------------------------------------------------------
-    const int qs = 61 - ilogb(total);
-    const int64_t mean_q = quant(mean, qs);
+    // ---- keys: one running sum over all slots, S(e) = sum_{e' < e} inc(e')
+    // (lights first: A[r] = S(r), B[q] = S(nl + q + 1) - S(nl) - quant(D)).
+    // Every increment is below 2^62 and S below 2^63 + a few ulps, so S is
+    // exact in uint64.
     const uint32_t e0 = (uint32_t)(kPer * t);
-    // One running sum over all slots, S(e) = sum_{e' < e} inc(e') (lights come
-    // first, so a_r = S(r) and b_q = S(nl + q + 1) - S(nl)).  Every increment is
-    // below 2^62 and S below 2^63 + a few ulps, so S is exact in uint64; the
-    // slot of A[r] (r <= nl) holds S(r), the slot of heavy q's key S(nl + q + 1).
-    uint64_t inc[kPer];
-    uint64_t mine = 0;
+    float w[kPer];
+    lds_load<kPer>(&s.w[e0], w);
+    uint64_t ex[kPer];  // S(e0 + k) once the thread's offset is added
+    uint64_t run = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t e = e0 + k;
-        inc[k] = 0;
+        uint64_t inc = 0;
         if (e < ne) {
-            const int64_t wq = quant_w(s.w[sw4(e)], qs);
-            inc[k] = (uint64_t)(e < nl ? mean_q - wq : wq - mean_q);
+            const uint64_t d = (uint64_t)quant_w(w[k], qs) - (uint64_t)mc.mean_q;
+            inc = e < nl ? (uint64_t)0 - d : d;
         }
-        mine += inc[k];
+        ex[k] = run;
+        run += inc;
     }
-    uint64_t incl = mine;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const uint64_t o = shfl_up(incl, d);
-        if (lane >= d) incl += o;
-    }
-    uint64_t run = shfl_up(incl, 1);  // exclusive per lane
-    if (lane == 0) run = 0;
-    if (lane == kWave - 1) s.wave_l[wid] = (int64_t)incl;
+    const uint64_t incl = wave_incl_scan_u64(run);
+    if (lane == kWave - 1) s.wave_l[wid] = incl;
     __syncthreads();
+    uint64_t off = incl - run;
     for (int q = 0; q < BLOCK / kWave; ++q)
-        if (q < wid) run += (uint64_t)s.wave_l[q];
-    // heavy keys carry the split offset, so that "heavy first" is B[q] <= A[r]
-    const double D = (double)n0 * mean - (c0.lp + c0.hp);
-    const int64_t D_q = quant(D, qs);
-    const double dres = ldexp((double)D_q, -qs) - D + mean;  // residual = (B - A) 2^-qs + dres
-    uint64_t* s_sum = reinterpret_cast<uint64_t*>(s.raw);
+        if (q < wid) off += s.wave_l[q];
 #pragma unroll
-    for (int k = 0; k <= kPer; ++k) {
-        const uint32_t e = e0 + k;
-        if (e == nl) s_sum[sw8(nl)] = run;  // A[nl] = S(nl) (the next thread may store it too)
-        if (k < kPer && e < ne) {
-            if (e < nl) {  // S(e), before the slot's own increment
-                s_sum[sw8(e)] = run;
-                run += inc[k];
-            } else {       // S(e + 1), stored at slot e + 1
-                run += inc[k];
-                s_sum[sw8(e + 1u)] = run;
-            }
-        }
-    }
+    for (int k = 0; k < kPer; ++k) ex[k] += off;
+    lds_store<kPer>(&s.key[e0], ex);
+    if (e0 + (uint32_t)kPer == (uint32_t)TILE) s.key[TILE] = off + run;  // S(TILE), a full block
     __syncthreads();
 
-    // ---- merge: thread t emits buckets [kPer*t, kPer*t + kPer) ----------
+    // ---- entries -------------------------------------------------------
     // p = residual / mean: FP32 mode divides as the reference does; the FP64
     // modes multiply by the FP64 reciprocal (within 1.5 ulp of the FP64
     // quotient before the rounding to float)
-    const double inv_mean = 1.0 / mean;
     auto ratio = [&](double x) -> float {
-        if constexpr (PREC == kFP32) return (float)((float)x / (float)mean);
-        else return (float)(x * inv_mean);
+        if constexpr (PREC == kFP32) return (float)((float)x / (float)mc.mean);
+        else return (float)(x * mc.inv_mean);
     };
-    const uint64_t s_nl = s_sum[sw8(nl)];
-    auto A = [&](uint32_t r) -> int64_t { return (int64_t)s_sum[sw8(r)]; };  // r in [0, nl]
-    auto B = [&](uint32_t q) -> int64_t {                                    // q in [0, nh)
-        return (int64_t)(s_sum[sw8(nl + 1u + q)] - s_nl) - D_q;
-    };
-    const uint32_t d0 = e0;
-    uint32_t slot[kPer];
-    uint2 ent[kPer];
-    int cnt = 0;
-    if (d0 < ne) {
-        // largest q in [max(0, d0 - nl), min(d0, nh)] with q == 0 or
-        // B[q-1] <= A[d0 - q] (A[nl] counts as +inf here)
-        uint32_t qa = d0 > nl ? d0 - nl : 0u;
-        uint32_t qb = min(d0, nh);
-        while (qa < qb) {
-            const uint32_t q = (qa + qb + 1u) / 2u;
-            const uint32_t r = d0 - q;
-            const bool ok = r >= nl || B(q - 1u) <= A(r);
-            if (ok) qa = q;
-            else qb = q - 1u;
+    const uint64_t kb = s.key[nl] + (uint64_t)s.dq;  // B[q] = S(nl + q + 1) - kb
+    auto A = [&](uint32_t r) -> int64_t { return (int64_t)s.key[r]; };  // r in [0, nl]
+    auto B = [&](uint32_t q) -> int64_t { return (int64_t)(s.key[nl + 1u + q] - kb); };
+    constexpr int64_t kInf = INT64_MAX;  // B[nh]: above every A[r], r < nl
+    // first B[q] > a for q in [lo, nh] (nh if none)
+    auto q_above = [&](uint32_t lo, int64_t a) -> uint32_t {
+        uint32_t hi = nh;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (B(mid) <= a) lo = mid + 1u;
+            else hi = mid;
         }
-        uint32_t q = qa;
-        uint32_t r = d0 - q;
-        const uint32_t d1 = min(d0 + (uint32_t)kPer, ne);
-        int64_t av = A(r);  // r <= nl: A[nl] exists
-        int64_t bv = q < nh ? B(q) : 0;
+        return lo;
+    };
+    uint2 ent[kPer];
+    if (e0 < nl) {
+        uint32_t q = q_above(0u, (int64_t)ex[0]);
+        int64_t bq = q < nh ? B(q) : kInf;
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
-            if (d0 + (uint32_t)k < d1) {
-                const bool take_heavy = q < nh && (r >= nl || bv <= av);
-                if (take_heavy) {
-                    const double resid = ldexp((double)(bv - av), -qs) + dres;
-                    const float p = fminf(fmaxf(ratio(resid), 0.0f), 1.0f);
-                    slot[k] = nl + q;
-                    ent[k] = make_uint2(__float_as_uint(p), s.idx[sw4(nl + q + 1u)]);
-                    ++q;
-                    bv = q < nh ? B(q) : 0;
-                } else {
-                    const float p = fminf(ratio((double)s.w[sw4(r)]), 1.0f);
-                    slot[k] = r;
-                    ent[k] = make_uint2(__float_as_uint(p), s.idx[sw4(nl + q)]);
-                    ++r;
-                    av = A(r);
+            if (e0 + (uint32_t)k < nl) {
+                const int64_t a = (int64_t)ex[k];
+                if (k > 0 && bq <= a) {  // heavies between this light and the last
+                    q = q_above(q + 1u, a);
+                    bq = q < nh ? B(q) : kInf;
                 }
-                cnt = k + 1;
+                const float p = fminf(ratio((double)w[k]), 1.0f);
+                ent[k] = make_uint2(__float_as_uint(p), s.idx[nl + q]);
             }
         }
     }
-    __syncthreads();  // keys no longer read: the buffer now takes the entries
+    // heavies: heavy q by thread q mod BLOCK
+    uint2 hent[kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; ++k)
-        if (k < cnt) s.raw[sw8(slot[k])] = ent[k];
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t q = (uint32_t)t + (uint32_t)BLOCK * j;
+        if ((uint32_t)BLOCK * j < nh && q < nh) {
+            const int64_t bv = B(q);
+            uint32_t lo = 0, hi = nl;  // r(q): first A[r] >= bv, r in [0, nl]
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (A(mid) < bv) lo = mid + 1u;
+                else hi = mid;
+            }
+            const double resid = ldexp((double)(bv - A(lo)), -qs) + s.dres;
+            const float p = fminf(fmaxf(ratio(resid), 0.0f), 1.0f);
+            hent[j] = make_uint2(__float_as_uint(p), s.idx[nl + q + 1u]);
+        }
+    }
+    __syncthreads();  // keys no longer read: the buffer now takes the entries
+    uint2* sent = reinterpret_cast<uint2*>(s.key);
+    if (e0 + (uint32_t)kPer <= nl) {
+        lds_store<kPer>(&sent[e0], ent);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (e0 + (uint32_t)k < nl) sent[e0 + k] = ent[k];
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const uint32_t q = (uint32_t)t + (uint32_t)BLOCK * j;
+        if ((uint32_t)BLOCK * j < nh && q < nh) sent[nl + q] = hent[j];
+    }
     __syncthreads();
 }
 
 // One block of TILE buckets [n0, n1) between boundary splits c0 and c1, BLOCK
-// threads, TILE / BLOCK buckets per thread.  Rows are written with
-// consecutive lanes on consecutive slots (consecutive light indices).  The
-// last heavy of all gets (1, itself), as in pack.hpp:78-81.
+// threads, TILE / BLOCK slots per thread.  Rows are written with consecutive
+// lanes on consecutive slots (consecutive light indices).  The last heavy of
+// all gets (1, itself), as in pack.hpp:78-81.
 template <int PREC, int TILE, int BLOCK, bool DIRECT = false>
 __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
                                                 const uint32_t* __restrict__ idx, uint32_t N,
-                                                uint32_t H, double mean, double total,
-                                                uint32_t n0, uint32_t n1, Coarse c0, Coarse c1,
+                                                const Params* __restrict__ prm, uint32_t n0,
+                                                uint32_t n1, Coarse c0, Coarse c1,
                                                 AliasEntry* __restrict__ table,
-                                                const LightSrc& ls = LightSrc{},
-                                                float pivot = 0.0f) {
+                                                const LightSrc& ls = LightSrc{}) {
     static_assert(TILE % BLOCK == 0 && BLOCK % kWave == 0 && TILE % kSample == 0, "merge shape");
     __shared__ MergeShared<TILE, BLOCK> s;
     const uint32_t t = threadIdx.x;
+    const uint32_t H = prm->heavy_count;
     const uint32_t L = N - H;
     if (H == 0 || L == 0) {  // all weights equal the mean up to its rounding: identity
         for (uint32_t e = t; e < n1 - n0; e += BLOCK) table[n0 + e] = AliasEntry{1.0f, n0 + e};
         return;
     }
+    const MergeConst mc{prm->mean, prm->inv_mean, prm->mean_q, prm->merge_qs};
     const uint32_t nl = c1.i - c0.i;  // lights packed here
     const uint32_t nh = c1.j - c0.j;  // heavies packed here (heavy c1.j is current at n1)
     const uint32_t ne = nl + nh;      // == n1 - n0 (- 1 in the last block)
+    if (t == 0) {  // the block's split offset, read after the staging barrier
+        const double D = (double)n0 * mc.mean - (c0.lp + c0.hp);
+        const int64_t dq = merge_quant(D, mc.qs);
+        s.dq = dq;
+        s.dres = ldexp((double)dq, -mc.qs) - D + mc.mean;
+    }
     if constexpr (DIRECT) {
         __shared__ uint32_t s_cnt[BLOCK / kWave];
-        stage_direct<TILE, BLOCK>(s, s_cnt, pw, idx, ls, pivot, N, H, c0, c1);
+        stage_direct<TILE, BLOCK>(s, s_cnt, pw, idx, ls, prm->pivot, N, H, c0, c1);
     } else {
         MergeStage<TILE, BLOCK> st;
         st.issue(pw, idx, N, c0, c1);
         st.store(s, ne);
     }
     __syncthreads();
-    merge_core<PREC, TILE, BLOCK>(s, mean, total, n0, c0, nl, nh);
+    merge_core<PREC, TILE, BLOCK>(s, mc, nl, nh);
     uint2* tab = reinterpret_cast<uint2*>(table);
+    const uint2* sent = reinterpret_cast<const uint2*>(s.key);
     for (uint32_t e = t; e < ne; e += BLOCK) {
-        const uint32_t x = s.idx[sw4(e)];
-        if (!DIRECT || x < N) tab[x] = s.raw[sw8(e)];  // direct: every slot is < N by construction
+        const uint32_t x = s.idx[e];
+        if (!DIRECT || x < N) tab[x] = sent[e];  // direct: every slot is < N by construction
     }
     if (n1 == N && t == 0) {
-        const uint32_t h = s.idx[sw4(ne)];  // heavy H-1
+        const uint32_t h = s.idx[ne];  // heavy H-1
         table[h] = AliasEntry{1.0f, h};
     }
 }
@@ -694,9 +724,8 @@ __global__ __launch_bounds__(BLOCK) void k_mergepack(const float* __restrict__ p
     const uint32_t b = blockIdx.x;
     const uint32_t n0 = b * (uint32_t)TILE;  // < N
     const uint32_t n1 = (uint32_t)min((uint64_t)n0 + TILE, (uint64_t)N);
-    mergepack_block<PREC, TILE, BLOCK, DIRECT>(pw, idx, N, prm->heavy_count, prm->mean,
-                                               prm->total, n0, n1, bounds[b], bounds[b + 1],
-                                               table, ls, prm->pivot);
+    mergepack_block<PREC, TILE, BLOCK, DIRECT>(pw, idx, N, prm, n0, n1, bounds[b], bounds[b + 1],
+                                               table, ls);
 }
 
 template <int PREC, int TILE, int BLOCK>
@@ -711,9 +740,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_mergepack(const float* __restrict
     const uint32_t n0 = it.blk * (uint32_t)TILE;
     const uint32_t n1 = min(n0 + (uint32_t)TILE, N);
     const Coarse* bd = bounds + it.bnd;
-    const Params* prm = seg_prm + it.seg;
-    mergepack_block<PREC, TILE, BLOCK>(pw + it.off, idx + it.off, N, prm->heavy_count, prm->mean,
-                                       prm->total, n0, n1, bd[0], bd[1], table + it.off);
+    mergepack_block<PREC, TILE, BLOCK>(pw + it.off, idx + it.off, N, seg_prm + it.seg, n0, n1,
+                                       bd[0], bd[1], table + it.off);
 }
 
 // ---- boundary repair ---------------------------------------------------------

@@ -110,7 +110,11 @@ __device__ __forceinline__ void finalize_block(const double* __restrict__ partia
         p.heavy_sum = 0.0;
         p.light_sum = 0.0;
         p.repair_ticket = 0;
-        p.reserved = 0;
+        // merge-key scale: every key and offset of a merge block stays below
+        // 2^62 in magnitude
+        p.merge_qs = p.total > 0.0 ? 61 - ilogb(p.total) : 0;
+        p.mean_q = p.total > 0.0 ? merge_quant(p.mean, p.merge_qs) : 0;
+        p.inv_mean = p.mean > 0.0 ? 1.0 / p.mean : 0.0;
         *prm = p;
     }
 }
