@@ -162,6 +162,23 @@ __device__ __forceinline__ int64_t merge_quant(double x, int qs) {
 // ------------------------------------------------------------ wave ops ----
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// One DPP move (update_dpp with old = 0): lanes whose source is outside the
+// row, or whose row is masked off, receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, true);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __uint_as_float(dpp_u32<CTRL, ROW_MASK>(__float_as_uint(v)));
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    const uint32_t lo = dpp_u32<CTRL, ROW_MASK>((uint32_t)v);
+    const uint32_t hi = dpp_u32<CTRL, ROW_MASK>((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // v + (the value of v DPP-moved by CTRL): both halves move, and lanes whose
 // source is outside the row, or whose row is masked off, add 0.
 template <int CTRL, int ROW_MASK>

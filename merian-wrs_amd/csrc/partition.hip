@@ -163,6 +163,16 @@ __device__ __forceinline__ Acc<PREC> acc_shfl_up(Acc<PREC> v, int d) {
     return Acc<PREC>{Acc1<PREC>::shfl_up(v.h, d), Acc1<PREC>::shfl_up(v.l, d), shfl_up(v.c, d)};
 }
 
+// v + (v DPP-moved by CTRL), the moved value the left operand; lanes that
+// receive nothing add zero (an exact no-op on the sums).
+template <int PREC, int CTRL, int RM>
+__device__ __forceinline__ Acc<PREC> acc_add_dpp(const Acc<PREC>& v) {
+    using A1 = Acc1<PREC>;
+    const Acc<PREC> o{A1::template dpp<CTRL, RM>(v.h), A1::template dpp<CTRL, RM>(v.l),
+                      dpp_u32<CTRL, RM>(v.c)};
+    return acc_add<PREC>(o, v);
+}
+
 // Tile aggregate as a carry word set.
 template <int PREC>
 __device__ __forceinline__ Carry to_carry(const Acc<PREC>& a, int shift) {
@@ -196,12 +206,10 @@ struct TileShared {
 // thr_excl is the sum over the thread's predecessors in the tile, tile_agg
 // the tile total.  Ends with the block synchronised and s.buf free.
 template <int PREC>
-__device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t base, uint32_t m,
-                                          float pivot, TileShared<PREC>& s, float (&v)[kItems],
-                                          Acc<PREC>& thr_excl, Acc<PREC>& tile_agg) {
+__device__ __forceinline__ Acc<PREC> thread_sums(const float* __restrict__ w, uint32_t base,
+                                                 uint32_t m, float pivot, TileShared<PREC>& s,
+                                                 float (&v)[kItems]) {
     const int t = threadIdx.x;
-    const int lane = t & (kWave - 1);
-    const int wid = t / kWave;
     load_tile_lds<true>(w + base, m, s.buf);
     __syncthreads();
 #pragma unroll
@@ -215,6 +223,58 @@ __device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t 
         add_split<PREC>(mine.h, mine.l, heavy, v[k]);
         mine.c += heavy ? 1u : 0u;
     }
+    return mine;
+}
+
+// Tile total from the per-wave totals in s.wave (block synchronised), the
+// waves in order; with wave_excl the total of the waves before wave wid (the
+// prefixes of the same chain of additions).
+template <int PREC>
+__device__ __forceinline__ Acc<PREC> combine_waves(const TileShared<PREC>& s, int wid,
+                                                   Acc<PREC>* wave_excl) {
+    Acc<PREC> c = s.wave[0];
+    if (wave_excl) *wave_excl = wid == 0 ? acc_zero<PREC>() : c;
+#pragma unroll
+    for (int q = 1; q < kBlock / kWave; ++q) {
+        c = acc_add<PREC>(c, s.wave[q]);
+        if (wave_excl && wid == q + 1) *wave_excl = c;
+    }
+    return c;
+}
+
+// The tile total alone (the aggregate passes): a wave's total is lane 63 of a
+// 64-lane Hillis-Steele scan, and these DPP row steps perform exactly the
+// additions that lane depends on, in the same order (within rows of 16 by
+// shifts of 1, 2, 4, 8, then row 2's total into row 3 and rows 0-1's into
+// rows 2-3), so the total has the scan's bits.  Ends with the block
+// synchronised and s.buf free.
+template <int PREC>
+__device__ __forceinline__ Acc<PREC> tile_total(const float* __restrict__ w, uint32_t base,
+                                                uint32_t m, float pivot, TileShared<PREC>& s) {
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
+    float v[kItems];
+    Acc<PREC> x = thread_sums<PREC>(w, base, m, pivot, s, v);
+    x = acc_add_dpp<PREC, 0x111, 0xf>(x);  // row_shr:1
+    x = acc_add_dpp<PREC, 0x112, 0xf>(x);  // row_shr:2
+    x = acc_add_dpp<PREC, 0x114, 0xf>(x);  // row_shr:4
+    x = acc_add_dpp<PREC, 0x118, 0xf>(x);  // row_shr:8
+    x = acc_add_dpp<PREC, 0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+    x = acc_add_dpp<PREC, 0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+    if (lane == kWave - 1) s.wave[wid] = x;
+    __syncthreads();
+    return combine_waves<PREC>(s, wid, nullptr);
+}
+
+template <int PREC>
+__device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t base, uint32_t m,
+                                          float pivot, TileShared<PREC>& s, float (&v)[kItems],
+                                          Acc<PREC>& thr_excl, Acc<PREC>& tile_agg) {
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
+    const Acc<PREC> mine = thread_sums<PREC>(w, base, m, pivot, s, v);
     Acc<PREC> incl = mine;
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
@@ -225,11 +285,8 @@ __device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t 
     if (lane == 0) lane_excl = acc_zero<PREC>();
     if (lane == kWave - 1) s.wave[wid] = incl;
     __syncthreads();
-    Acc<PREC> wave_excl = acc_zero<PREC>();
-    for (int q = 0; q < wid; ++q) wave_excl = acc_add<PREC>(wave_excl, s.wave[q]);
-    tile_agg = s.wave[0];
-#pragma unroll
-    for (int q = 1; q < kBlock / kWave; ++q) tile_agg = acc_add<PREC>(tile_agg, s.wave[q]);
+    Acc<PREC> wave_excl;
+    tile_agg = combine_waves<PREC>(s, wid, &wave_excl);
     thr_excl = acc_add<PREC>(wave_excl, lane_excl);
 }
 
@@ -394,9 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_part_agg(const float* __restrict__ w
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
-    float v[kItems];
-    Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC>(w, base, m, prm->pivot, s, v, thr_excl, tile_agg);
+    const Acc<PREC> tile_agg = tile_total<PREC>(w, base, m, prm->pivot, s);
     if (threadIdx.x == 0)
         store_carry(states + (uint64_t)tile * kCarryWords, to_carry<PREC>(tile_agg, prm->shift));
 }
@@ -653,9 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_part_agg(const float* __restrict
     const uint32_t base = it.blk * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const Params* prm = seg_prm + it.seg;
-    float v[kItems];
-    Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC>(w + off, base, m, prm->pivot, s, v, thr_excl, tile_agg);
+    const Acc<PREC> tile_agg = tile_total<PREC>(w + off, base, m, prm->pivot, s);
     if (threadIdx.x == 0)
         store_carry(states + (uint64_t)item * kCarryWords, to_carry<PREC>(tile_agg, prm->shift));
 }

@@ -26,6 +26,8 @@ struct Acc1 {
     static __device__ __forceinline__ double value(T a) { return (double)a; }
     static __device__ __forceinline__ T shfl_up(T a, int d) { return wrsdev::shfl_up(a, d); }
     static __device__ __forceinline__ T sel(bool c, T a, T b) { return c ? a : b; }
+    template <int CTRL, int RM>
+    static __device__ __forceinline__ T dpp(T a) { return wrsdev::dpp_f32<CTRL, RM>(a); }
 };
 
 template <>
@@ -41,6 +43,10 @@ struct Acc1<kComp> {
     static __device__ __forceinline__ T sel(bool c, T a, T b) {
         return T{c ? a.hi : b.hi, c ? a.lo : b.lo};
     }
+    template <int CTRL, int RM>
+    static __device__ __forceinline__ T dpp(T a) {
+        return T{wrsdev::dpp_f32<CTRL, RM>(a.hi), wrsdev::dpp_f32<CTRL, RM>(a.lo)};
+    }
 };
 
 template <>
@@ -52,6 +58,11 @@ struct Acc1<kFP64> {
     static __device__ __forceinline__ double value(T a) { return a; }
     static __device__ __forceinline__ T shfl_up(T a, int d) { return wrsdev::shfl_up(a, d); }
     static __device__ __forceinline__ T sel(bool c, T a, T b) { return c ? a : b; }
+    template <int CTRL, int RM>
+    static __device__ __forceinline__ T dpp(T a) {
+        return __longlong_as_double(
+            (long long)wrsdev::dpp_u64<CTRL, RM>((uint64_t)__double_as_longlong(a)));
+    }
 };
 
 // Adds x to h if heavy, else to l, with one addition and selects (no branch;
