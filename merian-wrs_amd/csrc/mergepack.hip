@@ -169,6 +169,15 @@ __device__ __forceinline__ void vexcl_scan(T& x0, T& x1, const VLane& v) {
 // phase 2: the 64 heavy and light weights around it resolve the exact j.  The
 // direct form reads the lights from ls (the heavy indices idx locate them) and
 // records the element index of the split's light.
+//
+// A bracket [jb_lo, jb_hi] expected to hold the split's j (the split of the
+// previous boundary, and that plus the buckets in between: splits are
+// monotone) narrows phase 1 to the sample multiples in it, one round of
+// nearby loads instead of rounds of scattered ones.  The narrowed answer is
+// used only where it is the full search's (the predicate true at the
+// bracket's low end unless that is the range's, false past its answer inside
+// the bracket unless the bracket ends with the range); otherwise the full
+// search runs, so the split never depends on the bracket.
 template <int PREC, bool DIRECT>
 __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw,
                                                     const double* __restrict__ samp_h,
@@ -176,7 +185,8 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
                                                     uint32_t H, typename RealT<PREC>::T mean,
                                                     typename RealT<PREC>::T light_sum, uint32_t n,
                                                     const uint32_t* __restrict__ idx,
-                                                    const LightSrc& ls) {
+                                                    const LightSrc& ls, uint32_t jb_lo = 0u,
+                                                    uint32_t jb_hi = 0xffffffffu) {
     using T = typename RealT<PREC>::T;
     const uint32_t L = N - H;
     const VLane v{lane_id() & 31, lane_id() >> 5};
@@ -191,10 +201,11 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
     // phase 1: largest multiple m*kSample in [lo, hi] with sigma <= target
     const uint32_t m_lo = (uint32_t)(((uint64_t)lo + kSample - 1u) / kSample);
     const uint32_t m_hi = hi / kSample;
-    int64_t best = -1;
-    if (m_lo <= m_hi) {
-        uint32_t a = m_lo, b = m_hi;
-        for (;;) {  // 32 candidates a round, one per lane of the half-wave
+    // largest m in [a, b] with the predicate true (-1 if none): 32 candidates
+    // a round, one per lane of the half-wave
+    auto search = [&](uint32_t a, uint32_t b) -> int64_t {
+        int64_t best = -1;
+        for (;;) {
             const uint32_t R = b - a + 1u;
             const uint32_t step = (R + 31u) / 32u;
             const uint32_t m = a + (uint32_t)v.sl * step;
@@ -213,6 +224,18 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
             b = min(b, (uint32_t)best + step - 1u);
             if (a > b) break;
         }
+        return best;
+    };
+    int64_t best = -1;
+    if (m_lo <= m_hi) {
+        const uint32_t ba = max(m_lo, jb_lo / kSample);
+        const uint32_t bb = jb_hi == 0xffffffffu ? m_hi : min(m_hi, jb_hi / kSample + 1u);
+        bool done = false;
+        if (ba <= bb && (ba > m_lo || bb < m_hi)) {
+            best = search(ba, bb);
+            done = (best >= 0 || ba == m_lo) && (best < (int64_t)bb || bb == m_hi);
+        }
+        if (!done) best = search(m_lo, m_hi);
     }
     const uint32_t base = best >= 0 ? (uint32_t)best * kSample : (lo / kSample) * kSample;
     // phase 2: resolve j in [base, base + 63] with the gathered weights
@@ -278,6 +301,12 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
     return Coarse{is, js, (double)lp_t, (double)hp_t, xs, 0u};
 }
 
+// A half-wave takes kCoarseRun consecutive boundaries, each after the first
+// searched in the bracket its predecessor's split gives (runs of 4 measured
+// fastest on the 2^26 Zipf build: 2 and 8 were slower, the chain of dependent
+// searches growing with the run).
+constexpr uint32_t kCoarseRun = 4;
+
 template <int PREC, bool DIRECT>
 __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 const double* __restrict__ samp_h,
@@ -286,13 +315,17 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 uint32_t tile, Coarse* __restrict__ out,
                                                 const uint32_t* __restrict__ idx, LightSrc ls) {
     using T = typename RealT<PREC>::T;
-    const uint32_t b = blockIdx.x * 8u + threadIdx.x / 32u;  // one boundary per half-wave
-    if (b >= nbound) return;  // the whole half-wave exits together
-    const uint32_t n = (uint32_t)min((uint64_t)b * tile, (uint64_t)N);  // N < 2^32 <= b * tile
-    const Coarse c = coarse_split_half<PREC, DIRECT>(pw, samp_h, samp_l, N, prm->heavy_count,
-                                                     (T)prm->mean, (T)prm->light_sum, n, idx,
-                                                     ls);
-    if ((lane_id() & 31) == 0) out[b] = c;
+    const uint32_t b0 = (blockIdx.x * 8u + threadIdx.x / 32u) * kCoarseRun;  // this half-wave's
+    uint32_t jb_lo = 0u, jb_hi = 0xffffffffu;
+    for (uint32_t b = b0; b < b0 + kCoarseRun && b < nbound; ++b) {  // uniform per half-wave
+        const uint32_t n = (uint32_t)min((uint64_t)b * tile, (uint64_t)N);  // N < 2^32 <= b * tile
+        const Coarse c = coarse_split_half<PREC, DIRECT>(pw, samp_h, samp_l, N, prm->heavy_count,
+                                                         (T)prm->mean, (T)prm->light_sum, n, idx,
+                                                         ls, jb_lo, jb_hi);
+        if ((lane_id() & 31) == 0) out[b] = c;
+        jb_lo = c.j;
+        jb_hi = c.j + tile;
+    }
 }
 
 // Segmented form: boundary p of the flattened boundary list belongs to segment
@@ -1014,7 +1047,8 @@ hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, c
                          hipStream_t s, const uint32_t* idx, LightSrc ls) {
     const uint32_t N = (uint32_t)n;
     const uint32_t nbound = (uint32_t)merge_blocks_for(n, ms.tile) + 1u;
-    const uint32_t cgrid = (nbound + 7u) / 8u;  // 8 boundaries per 256-thread block
+    const uint32_t runs = (nbound + kCoarseRun - 1u) / kCoarseRun;
+    const uint32_t cgrid = (runs + 7u) / 8u;  // 8 runs of boundaries per 256-thread block
     // the split arithmetic is FP32 in FP32 mode and FP64 in all others
 #define WRS_COARSE(P, D)                                                                    \
     k_coarse<P, D><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, ms.tile, bounds, \
