@@ -72,11 +72,11 @@ typedef struct {
                                (merge_tile 4096: 4 or 8) */
     uint32_t partition;     /* prefix-partition form: 0 reduce-then-scan (default), 1 single
                                pass with decoupled look-back (per segment in a batch) */
-    uint32_t direct;        /* pack form of a single build: 0 (default) the partition writes
-                               every index and weight in partition order, 1 it writes the
-                               heavies and every 64th light's position only and the pack
-                               reads the lights from the weights (12N fewer bytes; same
-                               table).  Batch plans and debug builds use 0. */
+    uint32_t direct;        /* pack form: 0 (default) the partition writes every index and
+                               weight in partition order, 1 it writes the heavies and every
+                               64th light's position only and the pack reads the lights from
+                               the weights (12N fewer bytes; same table).  Debug builds use
+                               0; batch plans build 1 with partition 0 only. */
 } wrs_tuning;
 
 /* Build configuration.  Reference: PSAConfig (src/device/wrs/alias/psa/PSA.hpp:19-41),

@@ -57,7 +57,8 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
     std::vector<wrsk::SegItem> h_items;
     std::vector<uint32_t> h_seg_item0(nseg + 1);
     std::vector<uint64_t> h_seg_samp0(nseg + 1);
-    uint64_t items64 = 0, samp = 0;
+    std::vector<uint64_t> h_seg_lpos0(nseg + 1);
+    uint64_t items64 = 0, samp = 0, lp = 0;
     for (uint32_t g = 0; g < nseg; ++g) {
         if (offsets[g + 1] <= offsets[g])
             throw wrs::Error(WRS_EINVAL, "segments must be non-empty and offsets increasing");
@@ -72,6 +73,8 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
         items64 += tiles;
         h_seg_samp0[g] = samp;
         samp += wrsk::samples_for(n);
+        h_seg_lpos0[g] = lp;
+        lp += wrsk::lightpos_for(n);
     }
     // capacity of the merge lists: the smallest merge tile has the most blocks
     std::vector<wrsk::SegItem> mi_;
@@ -81,6 +84,7 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
     if (items64 >= (1ull << 31) || bnd_cap >= (1ull << 31)) throw wrs::Error(WRS_EINVAL, "too many tiles");
     h_seg_item0[nseg] = (uint32_t)items64;
     h_seg_samp0[nseg] = samp;
+    h_seg_lpos0[nseg] = lp;
     items = (uint32_t)items64;
     mitems_cap = (uint32_t)mi_.size();
     nbound_cap = (uint32_t)bnd_cap;
@@ -102,7 +106,10 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
     samp_h = dalloc<double>(samp);
     samp_l = dalloc<double>(samp);
     bounds = dalloc<wrsk::Coarse>(nbound_cap);
+    lpos = dalloc<uint32_t>(lp);
+    seg_lpos0 = dalloc<uint64_t>(nseg + 1);
     upload(seg_off, offsets);
+    upload(seg_lpos0, h_seg_lpos0);
     upload(seg_item0, h_seg_item0);
     upload(seg_samp0, h_seg_samp0);
     upload(item_d, h_items);
@@ -129,9 +136,9 @@ void BatchPlan::set_tuning(const wrs_tuning& t) {
 }
 
 void BatchPlan::destroy() {
-    void* ptrs[] = {seg_off, seg_item0, seg_b0, seg_samp0, item_d,  mitem_d, bound_seg,
-                    seg_prm, partials,  states, ticket,    idx,     pw,      samp_h,
-                    samp_l,  bounds};
+    void* ptrs[] = {seg_off,  seg_item0, seg_b0, seg_samp0, item_d, mitem_d,
+                    bound_seg, seg_prm,  partials, states,  ticket, idx,
+                    pw,       samp_h,    samp_l, bounds,    lpos,   seg_lpos0};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     seg_off = nullptr;
@@ -150,9 +157,15 @@ void BatchPlan::destroy() {
     samp_h = nullptr;
     samp_l = nullptr;
     bounds = nullptr;
+    lpos = nullptr;
+    seg_lpos0 = nullptr;
 }
 
+// The direct pack form (tuning.direct) runs with the reduce-then-scan
+// partition; with the single-pass partition the classic form is built (the
+// same tables either way).
 void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) const {
+    const bool direct = tuning.direct && tuning.partition == 0;
     wrs::check_hip(wrsk::launch_seg_reduce(w, item_d, items, prec, partials,
                                            states, s),
                    "segmented reduce launch");
@@ -162,11 +175,12 @@ void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) c
     wrs::check_hip(wrsk::launch_seg_partition(w, item_d, seg_item0, nseg,
                                               items, prec, (int)tuning.partition, seg_prm,
                                               ticket, states, idx, pw, samp_h, samp_l, seg_samp0,
-                                              s),
+                                              s, direct ? lpos : nullptr, seg_lpos0),
                    "segmented partition launch");
     wrs::check_hip(wrsk::launch_seg_mergepack(pw, idx, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
                                               nseg, nbound, bound_seg, mitem_d, mitems, shape,
-                                              prec, seg_prm, bounds, tables, s),
+                                              prec, seg_prm, bounds, tables, s, w,
+                                              direct ? lpos : nullptr, seg_lpos0),
                    "segmented merge pack launch");
 }
 

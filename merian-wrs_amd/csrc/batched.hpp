@@ -46,6 +46,8 @@ struct BatchPlan {
     double* samp_h = nullptr;        // [seg_samp0[nseg]]
     double* samp_l = nullptr;        // [seg_samp0[nseg]]
     wrsk::Coarse* bounds = nullptr;  // [nbound at the smallest merge tile]
+    uint32_t* lpos = nullptr;        // [seg_lpos0[nseg]] direct form: light positions
+    uint64_t* seg_lpos0 = nullptr;   // [nseg+1] first light position of each segment
     uint32_t mitems_cap = 0, nbound_cap = 0;
 
     void create(uint32_t split_size, int prec, const wrs_tuning& tuning,

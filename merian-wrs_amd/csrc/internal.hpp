@@ -213,14 +213,19 @@ hipError_t launch_seg_partition(const float* w, const SegItem* item_d, const uin
                                 uint32_t items, int prec, int lookback, Params* seg_prm,
                                 uint32_t* ticket, uint64_t* states, uint32_t* idx, float* pw,
                                 double* samp_h, double* samp_l, const uint64_t* seg_samp0,
-                                hipStream_t s);
+                                hipStream_t s, uint32_t* lpos = nullptr,
+                                const uint64_t* seg_lpos0 = nullptr);
+// lpos non-null: the direct pack form (segment g's light positions at
+// lpos + seg_lpos0[g]; the reduce-then-scan partition only)
 hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const double* samp_h,
                                 const double* samp_l, const uint64_t* seg_off,
                                 const uint64_t* seg_samp0, const uint32_t* seg_b0, uint32_t nseg,
                                 uint32_t nbound, const uint32_t* bound_seg,
                                 const SegItem* mitem_d, uint32_t mitems, MergeShape ms, int prec,
                                 const Params* seg_prm, Coarse* bounds, AliasEntry* table,
-                                hipStream_t s);
+                                hipStream_t s, const float* w = nullptr,
+                                const uint32_t* lpos = nullptr,
+                                const uint64_t* seg_lpos0 = nullptr);
 hipError_t launch_seg_sample(const AliasEntry* tables, const uint64_t* seg_off, uint32_t nseg,
                              uint32_t per_segment, uint64_t seed, uint64_t counter_base,
                              uint32_t* out, hipStream_t s);

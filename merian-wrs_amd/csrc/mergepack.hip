@@ -330,13 +330,15 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
 
 // Segmented form: boundary p of the flattened boundary list belongs to segment
 // bound_seg[p] (a list the batch plan derives from the offsets).
-template <int PREC>
+template <int PREC, bool DIRECT>
 __global__ __launch_bounds__(256) void k_seg_coarse(
     const float* __restrict__ pw, const double* __restrict__ samp_h,
     const double* __restrict__ samp_l, const uint64_t* __restrict__ seg_off,
     const uint64_t* __restrict__ seg_samp0, const uint32_t* __restrict__ seg_b0,
     const uint32_t* __restrict__ bound_seg, uint32_t nbound, const Params* __restrict__ seg_prm,
-    uint32_t tile, Coarse* __restrict__ out) {
+    uint32_t tile, Coarse* __restrict__ out, const uint32_t* __restrict__ idx,
+    const float* __restrict__ w, const uint32_t* __restrict__ lpos,
+    const uint64_t* __restrict__ seg_lpos0) {
     using T = typename RealT<PREC>::T;
     const uint32_t p = blockIdx.x * 8u + threadIdx.x / 32u;  // one boundary per half-wave
     if (p >= nbound) return;  // the whole half-wave exits together
@@ -345,10 +347,11 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     const uint32_t N = (uint32_t)(seg_off[g + 1] - off);
     const uint32_t n = min((p - seg_b0[g]) * tile, N);
     const uint64_t so = seg_samp0[g];
-    const Coarse c = coarse_split_half<PREC, false>(pw + off, samp_h + so, samp_l + so, N,
-                                                    seg_prm[g].heavy_count, (T)seg_prm[g].mean,
-                                                    (T)seg_prm[g].light_sum, n, nullptr,
-                                                    LightSrc{});
+    const LightSrc ls = DIRECT ? LightSrc{w + off, lpos + seg_lpos0[g]} : LightSrc{};
+    const Coarse c = coarse_split_half<PREC, DIRECT>(pw + off, samp_h + so, samp_l + so, N,
+                                                     seg_prm[g].heavy_count, (T)seg_prm[g].mean,
+                                                     (T)seg_prm[g].light_sum, n,
+                                                     DIRECT ? idx + off : nullptr, ls);
     if ((lane_id() & 31) == 0) out[p] = c;
 }
 
@@ -761,20 +764,24 @@ __global__ __launch_bounds__(BLOCK) void k_mergepack(const float* __restrict__ p
                                                table, ls);
 }
 
-template <int PREC, int TILE, int BLOCK>
+template <int PREC, int TILE, int BLOCK, bool DIRECT>
 __global__ __launch_bounds__(BLOCK) void k_seg_mergepack(const float* __restrict__ pw,
                                                          const uint32_t* __restrict__ idx,
                                                          const SegItem* __restrict__ mitem_d,
                                                          const Params* __restrict__ seg_prm,
                                                          const Coarse* __restrict__ bounds,
-                                                         AliasEntry* __restrict__ table) {
+                                                         AliasEntry* __restrict__ table,
+                                                         const float* __restrict__ w,
+                                                         const uint32_t* __restrict__ lpos,
+                                                         const uint64_t* __restrict__ seg_lpos0) {
     const SegItem it = mitem_d[blockIdx.x];
     const uint32_t N = it.n;
     const uint32_t n0 = it.blk * (uint32_t)TILE;
     const uint32_t n1 = min(n0 + (uint32_t)TILE, N);
     const Coarse* bd = bounds + it.bnd;
-    mergepack_block<PREC, TILE, BLOCK>(pw + it.off, idx + it.off, N, seg_prm + it.seg, n0, n1,
-                                       bd[0], bd[1], table + it.off);
+    const LightSrc ls = DIRECT ? LightSrc{w + it.off, lpos + seg_lpos0[it.seg]} : LightSrc{};
+    mergepack_block<PREC, TILE, BLOCK, DIRECT>(pw + it.off, idx + it.off, N, seg_prm + it.seg, n0,
+                                               n1, bd[0], bd[1], table + it.off, ls);
 }
 
 // ---- boundary repair ---------------------------------------------------------
@@ -839,11 +846,14 @@ __device__ __forceinline__ void repair_anchor(Coarse& c, const float* __restrict
 // Segmented form: one block (kRepairThreads) per segment, boundaries
 // [seg_b0[g], seg_b0[g + 1]).  The raw split j is kept in the record's pad
 // word between the passes.
+template <bool DIRECT>
 __global__ __launch_bounds__(kRepairThreads) void k_repair_bounds(
     Coarse* __restrict__ bounds, const uint32_t* __restrict__ seg_b0,
     const uint64_t* __restrict__ seg_off, const uint64_t* __restrict__ seg_samp0,
     const Params* __restrict__ seg_prm, const float* __restrict__ pw,
-    const double* __restrict__ samp_h, const double* __restrict__ samp_l, uint32_t tile) {
+    const double* __restrict__ samp_h, const double* __restrict__ samp_l, uint32_t tile,
+    const uint32_t* __restrict__ idx, const float* __restrict__ w,
+    const uint32_t* __restrict__ lpos, const uint64_t* __restrict__ seg_lpos0) {
     __shared__ uint32_t s_w[kRepairThreads / kWave];
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
@@ -892,10 +902,11 @@ __global__ __launch_bounds__(kRepairThreads) void k_repair_bounds(
                         c.i = run;
                         c.j = b * tile - run;
                         if (c.j != c.aux)
-                            repair_anchor<false>(c, pw + off, samp_h + seg_samp0[g],
-                                                 samp_l + seg_samp0[g], N,
-                                                 seg_prm[g].heavy_count, seg_prm[g].light_sum,
-                                                 nullptr, LightSrc{});
+                            repair_anchor<DIRECT>(
+                                c, pw + off, samp_h + seg_samp0[g], samp_l + seg_samp0[g], N,
+                                seg_prm[g].heavy_count, seg_prm[g].light_sum,
+                                DIRECT ? idx + off : nullptr,
+                                DIRECT ? LightSrc{w + off, lpos + seg_lpos0[g]} : LightSrc{});
                         bd[b] = c;
                     }
                 }
@@ -1109,26 +1120,49 @@ hipError_t launch_seg_mergepack(const float* pw, const uint32_t* idx, const doub
                                 uint32_t nbound, const uint32_t* bound_seg,
                                 const SegItem* mitem_d, uint32_t mitems, MergeShape ms, int prec,
                                 const Params* seg_prm, Coarse* bounds, AliasEntry* table,
-                                hipStream_t s) {
+                                hipStream_t s, const float* w, const uint32_t* lpos,
+                                const uint64_t* seg_lpos0) {
     if (!merge_shape_supported(ms)) return hipErrorInvalidValue;
+    const bool direct = lpos != nullptr;
+    if (direct && (w == nullptr || seg_lpos0 == nullptr)) return hipErrorInvalidValue;
     const uint32_t cgrid = (nbound + 7u) / 8u;  // 8 boundaries per 256-thread block
-    if (prec == kFP32)
-        k_seg_coarse<kFP32><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0, seg_b0,
-                                                  bound_seg, nbound, seg_prm, ms.tile, bounds);
+#define WRS_SEG_COARSE(P, D)                                                                      \
+    k_seg_coarse<P, D><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0, seg_b0,      \
+                                             bound_seg, nbound, seg_prm, ms.tile, bounds, idx, w, \
+                                             lpos, seg_lpos0)
+    if (prec == kFP32) {
+        if (direct) WRS_SEG_COARSE(kFP32, true);
+        else WRS_SEG_COARSE(kFP32, false);
+    } else {
+        if (direct) WRS_SEG_COARSE(kFP64Carry, true);
+        else WRS_SEG_COARSE(kFP64Carry, false);
+    }
+#undef WRS_SEG_COARSE
+    if (direct)
+        k_repair_bounds<true><<<nseg, kRepairThreads, 0, s>>>(bounds, seg_b0, seg_off, seg_samp0,
+                                                              seg_prm, pw, samp_h, samp_l, ms.tile,
+                                                              idx, w, lpos, seg_lpos0);
     else
-        k_seg_coarse<kFP64Carry><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, seg_off, seg_samp0,
-                                                       seg_b0, bound_seg, nbound, seg_prm, ms.tile,
-                                                       bounds);
-    k_repair_bounds<<<nseg, kRepairThreads, 0, s>>>(bounds, seg_b0, seg_off, seg_samp0, seg_prm,
-                                                    pw, samp_h, samp_l, ms.tile);
+        k_repair_bounds<false><<<nseg, kRepairThreads, 0, s>>>(bounds, seg_b0, seg_off, seg_samp0,
+                                                               seg_prm, pw, samp_h, samp_l,
+                                                               ms.tile, idx, w, lpos, seg_lpos0);
 #define WRS_SMP(T, P)                                                                           \
     if (ms.tile == (T) && ms.per == (P)) {                                                      \
-        if (prec == kFP32)                                                                      \
-            k_seg_mergepack<kFP32, T, T / P><<<mitems, T / P, 0, s>>>(                          \
-                pw, idx, mitem_d, seg_prm, bounds, table);                                      \
-        else                                                                                    \
-            k_seg_mergepack<kFP64Carry, T, T / P><<<mitems, T / P, 0, s>>>(                     \
-                pw, idx, mitem_d, seg_prm, bounds, table);                                      \
+        if (prec == kFP32) {                                                                    \
+            if (direct)                                                                         \
+                k_seg_mergepack<kFP32, T, T / P, true><<<mitems, T / P, 0, s>>>(                \
+                    pw, idx, mitem_d, seg_prm, bounds, table, w, lpos, seg_lpos0);              \
+            else                                                                                \
+                k_seg_mergepack<kFP32, T, T / P, false><<<mitems, T / P, 0, s>>>(               \
+                    pw, idx, mitem_d, seg_prm, bounds, table, w, lpos, seg_lpos0);              \
+        } else {                                                                                \
+            if (direct)                                                                         \
+                k_seg_mergepack<kFP64Carry, T, T / P, true><<<mitems, T / P, 0, s>>>(           \
+                    pw, idx, mitem_d, seg_prm, bounds, table, w, lpos, seg_lpos0);              \
+            else                                                                                \
+                k_seg_mergepack<kFP64Carry, T, T / P, false><<<mitems, T / P, 0, s>>>(          \
+                    pw, idx, mitem_d, seg_prm, bounds, table, w, lpos, seg_lpos0);              \
+        }                                                                                       \
     }
     WRS_MERGE_SHAPES(WRS_SMP)
 #undef WRS_SMP

@@ -770,12 +770,15 @@ __global__ __launch_bounds__(kBlock) void k_seg_part_spine(const uint32_t* __res
     }
 }
 
-template <int PREC>
+// DIRECT: the direct pack form (heavies' idx + pw and the segment's light
+// positions at lpos + seg_lpos0[segment], as the single-distribution emit).
+template <int PREC, bool DIRECT>
 __global__ __launch_bounds__(kBlock) void k_seg_part_emit(
     const float* __restrict__ w, const SegItem* __restrict__ item_d, uint32_t items,
     const Params* __restrict__ seg_prm, const uint64_t* __restrict__ states,
     uint32_t* __restrict__ idx, float* __restrict__ pw, double* __restrict__ samp_h,
-    double* __restrict__ samp_l, const uint64_t* __restrict__ seg_samp0) {
+    double* __restrict__ samp_l, const uint64_t* __restrict__ seg_samp0,
+    uint32_t* __restrict__ lpos, const uint64_t* __restrict__ seg_lpos0) {
     __shared__ TileShared<PREC> s;
     const uint32_t item = blockIdx.x;
     const SegItem it = item_d[item];
@@ -793,8 +796,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_part_emit(
     tile_scan<PREC>(w + off, base, m, pivot, s, v, thr_excl, tile_agg);
     const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
     __syncthreads();  // s.buf is reused by the emit rounds
-    tile_emit<PREC, false>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off, pw + off,
-                           nullptr, nullptr, samp_h + so, samp_l + so);
+    tile_emit<PREC, false, DIRECT>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off,
+                                   pw + off, nullptr, nullptr, samp_h + so, samp_l + so,
+                                   DIRECT ? lpos + seg_lpos0[sg] : nullptr);
 }
 
 namespace {
@@ -860,7 +864,8 @@ hipError_t launch_seg_partition(const float* w, const SegItem* item_d, const uin
                                 uint32_t items, int prec, int lookback, Params* seg_prm,
                                 uint32_t* ticket, uint64_t* states, uint32_t* idx, float* pw,
                                 double* samp_h, double* samp_l, const uint64_t* seg_samp0,
-                                hipStream_t s) {
+                                hipStream_t s, uint32_t* lpos, const uint64_t* seg_lpos0) {
+    if (lpos && lookback) return hipErrorInvalidValue;  // direct: reduce-then-scan only
     const uint32_t spine_blocks = (nseg + kBlock / kWave - 1) / (kBlock / kWave);
 #define WRS_LAUNCH_SEG(P)                                                                        \
     if (lookback) {                                                                              \
@@ -871,9 +876,14 @@ hipError_t launch_seg_partition(const float* w, const SegItem* item_d, const uin
         k_seg_part_agg<P><<<items, kBlock, 0, s>>>(w, item_d, seg_prm, states);                  \
         k_seg_part_spine<P><<<spine_blocks, kBlock, 0, s>>>(seg_item0, nseg, items, seg_prm,     \
                                                             states);                             \
-        k_seg_part_emit<P><<<items, kBlock, 0, s>>>(w, item_d, items,                            \
-                                                    seg_prm, states, idx, pw, samp_h, samp_l,    \
-                                                    seg_samp0);                                  \
+        if (lpos)                                                                                \
+            k_seg_part_emit<P, true><<<items, kBlock, 0, s>>>(w, item_d, items, seg_prm, states, \
+                                                              idx, pw, samp_h, samp_l, seg_samp0, \
+                                                              lpos, seg_lpos0);                   \
+        else                                                                                     \
+            k_seg_part_emit<P, false><<<items, kBlock, 0, s>>>(w, item_d, items, seg_prm, states, \
+                                                               idx, pw, samp_h, samp_l,           \
+                                                               seg_samp0, nullptr, nullptr);      \
     }
     switch (prec) {
         case kFP32: WRS_LAUNCH_SEG(kFP32); break;

@@ -4,7 +4,9 @@
 // Search: coordinate descent over the parts of a build.
 //   1. the prefix-partition form (reduce-then-scan or single pass with
 //      decoupled look-back, per segment in a batch), at the default merge shape;
-//   2. single builds: the pack form (classic or direct) with the form of step 1;
+//   2. the pack form (classic or direct) with the form of step 1 (a batch builds
+//      the direct form with the reduce-then-scan partition only, so there the
+//      direct candidate takes that partition form);
 //   3. every supported merge shape (buckets per block x buckets per thread)
 //      with the forms of steps 1 and 2.
 // A candidate's time is the median of kReps builds (HIP events on the
@@ -158,10 +160,10 @@ wrs_tuning with_shape(wrs_tuning t, const wrsk::MergeShape& ms) {
 }  // namespace
 
 // Coordinate descent over a measuring function: the partition form at the
-// default merge shape, the pack form (try_direct), then every other merge
-// shape with the better forms.
+// default merge shape, the pack form, then every other merge shape with the
+// better forms.
 template <class M>
-wrs_tuning descend(M&& measure, bool try_direct) {
+wrs_tuning descend(M&& measure, bool direct_rts_only) {
     const wrs_tuning t0 = with_shape(wrs_tuning{}, kShapes[0]);
     wrs_tuning best = t0;
     float best_ms = measure(t0, kWarm);
@@ -172,9 +174,10 @@ wrs_tuning descend(M&& measure, bool try_direct) {
         best_ms = ms1;
         best = t1;
     }
-    if (try_direct) {
+    {
         wrs_tuning t2 = best;
         t2.direct = 1;
+        if (direct_rts_only) t2.partition = 0;
         const float ms2 = measure(t2);
         if (ms2 < best_ms) {
             best_ms = ms2;
@@ -214,7 +217,7 @@ wrs_tuning tune_psa(const wrs::PSAConfig& base, const float* w, uint64_t n, uint
                 [&] { psa.run(s, wrs::PSA::Buffers{w, table, &ws}, n); }, s, ev, warm);
             log.add(t, ms);
             return ms;
-        }, true);
+        }, false);
     } catch (...) {
         if (table) (void)hipFree(table);
         ws.release();
@@ -226,7 +229,7 @@ wrs_tuning tune_psa(const wrs::PSAConfig& base, const float* w, uint64_t n, uint
     return best;
 }
 
-// Batched build: the partition form, then the merge shapes.
+// Batched build: the partition form, the pack form, then the merge shapes.
 wrs_tuning tune_batched(const wrs::PSAConfig& base, const uint64_t* offsets, uint32_t nseg,
                         const float* w, uint32_t flags, hipStream_t s, Log& log) {
     // problem shape: segment count, total, histogram of floor(log2(size))
@@ -256,7 +259,7 @@ wrs_tuning tune_batched(const wrs::PSAConfig& base, const uint64_t* offsets, uin
             const float ms = time_median([&] { bp.build(w, tables, s); }, s, ev, warm);
             log.add(t, ms);
             return ms;
-        }, false);
+        }, true);
     } catch (...) {
         if (tables) (void)hipFree(tables);
         bp.destroy();

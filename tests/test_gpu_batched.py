@@ -113,3 +113,51 @@ def test_graph_replays_match_eager(wrs, oracle):
         # eager work between replays
         bp.build(dw, te, stream=st)
         st.synchronize()
+
+
+def edge_batch(oracle):
+    """Segments of 1 to 2^18 weights, including layouts of heavies that stress
+    the direct form: none (equal weights), one huge weight, heavies clustered
+    at the end (ascending) or the start (descending), and Zipf."""
+    segs = []
+    sizes = [1, 2, 3, 63, 64, 65, 100, 4095, 4096, 4097, 8191, 70001, 1 << 18]
+    for i, n in enumerate(sizes):
+        segs.append(oracle.gen_zipf(50 + i, 1.1, n))
+    segs.append(np.ones(5000, np.float32))
+    one = oracle.gen_uniform01(71, 5000)
+    one[1234] = np.float32(1e6)
+    segs.append(one)
+    segs.append(np.sort(oracle.gen_lognormal(72, 0.0, 2.0, 3000)))
+    segs.append(np.sort(oracle.gen_lognormal(73, 0.0, 2.0, 4099))[::-1].copy())
+    sizes = np.array([s.size for s in segs], np.uint64)
+    return sizes, np.concatenate(segs).astype(np.float32)
+
+
+@pytest.mark.parametrize("tile,per", [(1024, 4), (2048, 4), (4096, 8), (1024, 2)])
+@pytest.mark.parametrize("precision", [2, 1])
+def test_batched_direct_equals_classic(wrs, oracle, tile, per, precision):
+    """The direct pack form of a batch (each segment's light positions at its
+    own offset of one array): tables bit-identical to the classic form's and to
+    building every segment alone."""
+    print("case: batched direct", tile, per, precision, flush=True)
+    sizes, w = edge_batch(oracle)
+    off = wrs.segment_offsets(sizes)
+    total = int(off[-1])
+    dw = wrs.to_device(w)
+    tables = {}
+    for d in (0, 1):
+        bp = wrs.BatchPlan(off, precision=precision,
+                           tune=dict(merge_tile=tile, merge_per=per, partition=0, direct=d))
+        tb = wrs.empty(total, wrs.ALIAS_DTYPE)
+        tb.fill_bytes(0xFF)
+        bp.build(dw, tb)
+        wrs.synchronize()
+        tables[d] = tb.numpy()
+    assert np.array_equal(tables[0], tables[1])
+    plan = wrs.Plan(int(sizes.max()), precision=precision,
+                    tune=dict(merge_tile=tile, merge_per=per))
+    te = wrs.empty(total, wrs.ALIAS_DTYPE)
+    se = wrs.empty(len(sizes), np.uint32)
+    wrs.build_sample_each(plan, dw, off, te, 1, 42, se)
+    wrs.synchronize()
+    assert np.array_equal(tables[1], te.numpy())

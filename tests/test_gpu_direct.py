@@ -141,8 +141,9 @@ def test_direct_matches_oracle_and_reuses_plan(wrs, oracle):
 
 
 def test_direct_tuning_in_debug_and_batched_builds(wrs, oracle):
-    """Debug builds (which read the full partition arrays) and batch plans
-    build in the classic form whatever the tuning says: same tables."""
+    """Debug builds (which read the full partition arrays) build in the
+    classic form whatever the tuning says, and batch plans in either form:
+    same tables."""
     n = 200003
     case("direct ignored", n)
     w = oracle.gen_zipf(4, 1.1, n)

@@ -119,12 +119,13 @@ def test_autotune_batched_measures_then_caches(wrs, oracle):
     sizes, off, w = make_batch(wrs, oracle, nseg, 1 << 15, seed=10)
     dw = wrs.to_device(w)
     best, log = wrs.autotune_batched(off, dw, no_cache=True)
-    print("  best", best, [(e["partition"], e["merge_tile"], e["merge_per"],
+    print("  best", best, [(e["partition"], e["direct"], e["merge_tile"], e["merge_per"],
                             round(e["ms"], 4)) for e in log])
-    # partition form at the default shape (2), then the 7 other shapes (no pack form)
-    assert len(log) == 2 + len(SHAPES) - 1
+    # partition form at the default shape (2), the pack form (1: direct, with the
+    # reduce-then-scan partition a batch builds it with), then the 7 other shapes
+    assert len(log) == 2 + 1 + len(SHAPES) - 1
     assert {e["partition"] for e in log[:2]} == {0, 1}
-    assert all(e["direct"] == 0 for e in log)
+    assert [e["direct"] for e in log[:3]] == [0, 0, 1] and log[2]["partition"] == 0
     assert (best["merge_tile"], best["merge_per"]) in SHAPES
     best2, log2 = wrs.autotune_batched(off, dw)
     assert best2 == best and log2 == []
