@@ -235,7 +235,7 @@ def main() -> int:
     mt = t_.get("merge_tile") or 2048
     mp = t_.get("merge_per") or 4
     d_ = "true" if direct else "false"
-    kname = {"reduce": f"k_reduce<{'false' if p == 0 else 'true'}>",
+    kname = {"reduce": f"k_reduce_final<{'false' if p == 0 else 'true'}>",
              "tile_aggregate": f"k_part_agg<{p}>",
              "partition_emit": (f"k_part_lookback<{p}, false, {d_}>" if lookback
                                 else f"k_part_emit<{p}, false, {d_}>"),
@@ -244,7 +244,7 @@ def main() -> int:
         del kname["tile_aggregate"]
     # every kernel instance of one build, as rocprofv3 names them
     fp = "false" if p == 0 else "true"
-    build_kernels = [f"k_reduce<{fp}>", f"k_finalize<{fp}>"]
+    build_kernels = [f"k_reduce_final<{fp}>"]  # the reduce's last block finalizes
     if not lookback:
         build_kernels += [f"k_part_agg<{p}>", f"k_part_tilescan<{p}>"]
     build_kernels += [kname["partition_emit"], f"k_coarse<{0 if p == 0 else 1}, {d_}>",

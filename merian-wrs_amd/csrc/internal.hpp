@@ -114,10 +114,12 @@ inline uint32_t partials_for(uint64_t n) {
 }
 
 // ---- launchers (each returns the launch's hipError_t) ----
-hipError_t launch_reduce(const float* w, uint64_t n, int prec, double* partials, uint32_t G,
-                         uint64_t* states, uint64_t state_words, hipStream_t s);
-hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int prec,
-                           int has_pivot, double pivot, Params* prm, hipStream_t s);
+// Total, mean, pivot and the other build scalars into *prm, the look-back
+// records reset; red: kMaxPartials words, all ~0 before the first launch and
+// restored by every launch (wrs::Workspace allocates them so).
+hipError_t launch_reduce_final(const float* w, uint64_t n, int prec, uint64_t* red, uint32_t G,
+                               uint64_t* states, uint64_t state_words, int has_pivot,
+                               double pivot, Params* prm, hipStream_t s);
 // idx is always written; pw (the weights in partition order), hi/lo (full
 // prefix) and the prefix samples only when non-null.  ev_agg / ev_scan, when
 // non-null, are recorded after the tile-aggregate and tile-scan kernels.

@@ -7,8 +7,8 @@
 // sums the elements 4(t+256j)+q, j = 0..3, q = 0..3 in that order (the same order
 // whether the rows come straight from memory or through LDS, so the result does
 // not depend on pointer alignment); waves reduce by a fixed butterfly and the
-// finalize kernel sums the G partials in a fixed tree.  The same launch also
-// resets the look-back records of the kernel that follows it.
+// last block to finish sums the G partials in a fixed tree.  The same launch
+// also resets the look-back records of the kernel that follows it.
 //
 // The segmented (batched) kernels run exactly the same per-tile and finalize
 // code per segment, so a batched build yields the same bits as building each
@@ -75,15 +75,13 @@ __device__ __forceinline__ AccT<FP64> block_sum(AccT<FP64> acc) {
     return b;
 }
 
-// Sums G partials and derives the build scalars (thread 0 writes *prm).
+// Derives the build scalars from this thread's running sum of the partials
+// i = t, t + kBlock, ... (in that order); thread 0 writes *prm.
 template <bool FP64>
-__device__ __forceinline__ void finalize_block(const double* __restrict__ partials, uint32_t G,
-                                               uint64_t n, int has_pivot, double pivot,
-                                               Params* __restrict__ prm) {
+__device__ __forceinline__ void finalize_acc(AccT<FP64> acc, uint64_t n, int has_pivot,
+                                             double pivot, Params* __restrict__ prm) {
     using Acc = AccT<FP64>;
     const int t = threadIdx.x;
-    Acc acc = 0;
-    for (uint32_t i = t; i < G; i += kBlock) acc = acc + (Acc)partials[i];
     const Acc total = block_sum<FP64>(acc);
     if (t == 0) {
         Params p;
@@ -119,13 +117,37 @@ __device__ __forceinline__ void finalize_block(const double* __restrict__ partia
     }
 }
 
+// Sums G partials in a fixed order and derives the build scalars.
+template <bool FP64>
+__device__ __forceinline__ void finalize_block(const double* __restrict__ partials, uint32_t G,
+                                               uint64_t n, int has_pivot, double pivot,
+                                               Params* __restrict__ prm) {
+    AccT<FP64> acc = 0;
+    for (uint32_t i = threadIdx.x; i < G; i += kBlock) acc = acc + (AccT<FP64>)partials[i];
+    finalize_acc<FP64>(acc, n, has_pivot, pivot, prm);
+}
+
+// Polls of a block total word before block 0 goes on without it: every other
+// block stores its word when its tiles are done, so the wait ends once that
+// store is visible; the bound only keeps a kernel from spinning on.
+constexpr uint32_t kPartialSpin = 1u << 20;
+
 }  // namespace
 
+// Block totals, then the build scalars by block 0.  red holds kMaxPartials
+// total words, the sentinel ~0 between launches (a total is a non-negative
+// double or a NaN from a float, never all ones).  Every block stores its total
+// with an agent-coherent store; block 0, after its own tiles, polls the words
+// until none is the sentinel (it waits for no other block to start: the others
+// depend on nothing), sums them in the fixed order of the segmented finalize
+// (same bits), writes *prm and restores the sentinels for the next launch.  No
+// counter: one atomic per block on a single word costs more than the wait.
 template <bool FP64>
-__global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ w, uint64_t n,
-                                                   double* __restrict__ partials,
-                                                   uint64_t* __restrict__ states,
-                                                   uint64_t state_words) {
+__global__ __launch_bounds__(kBlock) void k_reduce_final(const float* __restrict__ w, uint64_t n,
+                                                         uint64_t* __restrict__ red,
+                                                         uint64_t* __restrict__ states,
+                                                         uint64_t state_words, int has_pivot,
+                                                         double pivot, Params* __restrict__ prm) {
     const int t = threadIdx.x;
     const uint32_t G = gridDim.x;
     const uint64_t tiles = (n + kTile - 1) / kTile;
@@ -135,14 +157,37 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const float* __restrict__ w, 
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + t; i < state_words; i += (uint64_t)G * kBlock)
         states[i] = kSentinel;
     const AccT<FP64> b = block_sum<FP64>(acc);
-    if (t == 0) partials[blockIdx.x] = (double)b;
-}
-
-template <bool FP64>
-__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials,
-                                                     uint32_t G, uint64_t n, int has_pivot,
-                                                     double pivot, Params* __restrict__ prm) {
-    finalize_block<FP64>(partials, G, n, has_pivot, pivot, prm);
+    if (t == 0) st_word(red + blockIdx.x, (uint64_t)__double_as_longlong((double)b));
+    if (blockIdx.x != 0) return;
+    __syncthreads();
+    // this thread's words i = t + kBlock k, all loads in flight at once
+    constexpr int kPer = (int)(kMaxPartials / kBlock);
+    uint64_t x[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t i = (uint32_t)t + (uint32_t)kBlock * k;
+        x[k] = i < G ? ld_word(red + i) : 0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t i = (uint32_t)t + (uint32_t)kBlock * k;
+        for (uint32_t spin = 0; i < G && x[k] == kSentinel && spin < kPartialSpin; ++spin) {
+            __builtin_amdgcn_s_sleep(1);
+            x[k] = ld_word(red + i);
+        }
+    }
+    AccT<FP64> sum = 0;  // the order of finalize_block
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if ((uint32_t)t + (uint32_t)kBlock * k < G)
+            sum = sum + (AccT<FP64>)__longlong_as_double((long long)x[k]);
+    finalize_acc<FP64>(sum, n, has_pivot, pivot, prm);
+    // every thread restores only the words it read itself
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t i = (uint32_t)t + (uint32_t)kBlock * k;
+        if (i < G) st_word(red + i, kSentinel);
+    }
 }
 
 // Segmented: one block per (segment, tile) item; every segment has at most
@@ -181,21 +226,16 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t words, uint64_t va
         p[i] = value;
 }
 
-hipError_t launch_reduce(const float* w, uint64_t n, int prec, double* partials, uint32_t G,
-                         uint64_t* states, uint64_t state_words, hipStream_t s) {
+hipError_t launch_reduce_final(const float* w, uint64_t n, int prec, uint64_t* red, uint32_t G,
+                               uint64_t* states, uint64_t state_words, int has_pivot,
+                               double pivot, Params* prm, hipStream_t s) {
+    if (G == 0 || G > kMaxPartials) return hipErrorInvalidValue;
     if (prec != kFP32)
-        k_reduce<true><<<G, kBlock, 0, s>>>(w, n, partials, states, state_words);
+        k_reduce_final<true><<<G, kBlock, 0, s>>>(w, n, red, states, state_words, has_pivot,
+                                                  pivot, prm);
     else
-        k_reduce<false><<<G, kBlock, 0, s>>>(w, n, partials, states, state_words);
-    return hipGetLastError();
-}
-
-hipError_t launch_finalize(const double* partials, uint32_t G, uint64_t n, int prec,
-                           int has_pivot, double pivot, Params* prm, hipStream_t s) {
-    if (prec != kFP32)
-        k_finalize<true><<<1, kBlock, 0, s>>>(partials, G, n, has_pivot, pivot, prm);
-    else
-        k_finalize<false><<<1, kBlock, 0, s>>>(partials, G, n, has_pivot, pivot, prm);
+        k_reduce_final<false><<<G, kBlock, 0, s>>>(w, n, red, states, state_words, has_pivot,
+                                                   pivot, prm);
     return hipGetLastError();
 }
 

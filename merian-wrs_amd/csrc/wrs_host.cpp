@@ -43,7 +43,10 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
     ws.max_n = max_n;
     try {
         ws.params = dev_alloc<wrsk::Params>(1);
-        ws.partials = dev_alloc<double>(wrsk::kMaxPartials);
+        ws.red = dev_alloc<uint64_t>(wrsk::kMaxPartials);
+        // block totals at the sentinel (each launch of the reduce restores it)
+        check_hip(hipMemset(ws.red, 0xff, wrsk::kMaxPartials * sizeof(uint64_t)), "hipMemset");
+        check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
         ws.state_words = wrsk::state_words_for(max_n);
         ws.states = dev_alloc<uint64_t>(ws.state_words);
         if (partition_buffers) {
@@ -64,12 +67,12 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
 }
 
 void Workspace::release() {
-    void* ptrs[] = {params,    partials,  states, indices, pweights,
+    void* ptrs[] = {params,    red,       states, indices, pweights,
                     prefix_hi, prefix_lo, samp_h, samp_l,  bounds, lightpos};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     params = nullptr;
-    partials = nullptr;
+    red = nullptr;
     states = nullptr;
     indices = nullptr;
     pweights = nullptr;
@@ -104,12 +107,10 @@ void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, d
     require(N > 0 && N <= ws.max_n, "N must be in [1, max_n]");
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::state_words_for(N);
-    check_hip(wrsk::launch_reduce(weights, N, (int)m_prec, ws.partials, G, ws.states, words, s),
-              "reduce launch");
     const bool has_pivot = !std::isnan(pivot);
-    check_hip(wrsk::launch_finalize(ws.partials, G, N, (int)m_prec, has_pivot ? 1 : 0,
-                                    has_pivot ? pivot : 0.0, ws.params, s),
-              "finalize launch");
+    check_hip(wrsk::launch_reduce_final(weights, N, (int)m_prec, ws.red, G, ws.states, words,
+                                        has_pivot ? 1 : 0, has_pivot ? pivot : 0.0, ws.params, s),
+              "reduce launch");
 }
 
 void PrefixPartition::run(hipStream_t s, const Buffers& b, uint64_t N, Workspace& ws) const {
@@ -158,11 +159,11 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::state_words_for(N);
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
-    check_hip(wrsk::launch_reduce(b.weights, N, (int)prec, ws.partials, G, ws.states, words, s),
+    check_hip(wrsk::launch_reduce_final(b.weights, N, (int)prec, ws.red, G, ws.states, words, 0,
+                                        0.0, ws.params, s),
               "reduce launch");
+    // the finalize runs inside the reduce (its last block): an empty phase
     if (ev) check_hip(hipEventRecord(ev[1], s), "event");
-    check_hip(wrsk::launch_finalize(ws.partials, G, N, (int)prec, 0, 0.0, ws.params, s),
-              "finalize launch");
     if (ev) check_hip(hipEventRecord(ev[2], s), "event");
     check_hip(wrsk::launch_partition(b.weights, N, (int)prec, (int)m_cfg.tuning.partition,
                                      ws.params, ws.states, ws.indices,

@@ -43,7 +43,7 @@ void set_last_error(const std::string& msg);
 struct Workspace {
     uint64_t max_n = 0;
     wrsk::Params* params = nullptr;    // [1]
-    double* partials = nullptr;        // [kMaxPartials]
+    uint64_t* red = nullptr;           // [kMaxPartials] reduce block totals (~0 between builds)
     uint64_t* states = nullptr;        // [state_words_for(max_n)]
     uint64_t state_words = 0;
     uint32_t* indices = nullptr;       // [max_n] (PSA only)
