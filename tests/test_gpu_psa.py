@@ -242,3 +242,41 @@ def test_table_valid_in_every_mode(wrs, oracle, kind, precision):
     print(f"  max_abs {err['max_abs']:.3e} max_rel {err['max_rel']:.3e}")
     if precision >= 2:
         assert err["max_rel"] <= 1e-6
+
+
+@pytest.mark.parametrize("direct", [0, 1])
+def test_single_build_graph_replays_and_plan_reuse(wrs, oracle, direct):
+    """A captured single build replays to the eager table, with eager builds of
+    other sizes on the same plan between the replays: the reduce restores its
+    own state (the block-total words it polls) at the end of every launch."""
+    n1, n2 = (1 << 21) + 3, 70001
+    print("case: single-build graph", direct, flush=True)
+    w1 = oracle.gen_zipf(81, 1.1, n1)
+    w2 = oracle.gen_lognormal(82, 0.0, 2.0, n2)
+    d1, d2 = wrs.to_device(w1), wrs.to_device(w2)
+    plan = wrs.Plan(n1, tune=dict(direct=direct))
+    st = wrs.Stream()
+    t1 = wrs.empty(n1, wrs.ALIAS_DTYPE)
+    t2 = wrs.empty(n2, wrs.ALIAS_DTYPE)
+    plan.psa_build(d1, n1, t1, stream=st)
+    st.synchronize()
+    eager1 = t1.numpy()
+    plan.psa_build(d2, n2, t2, stream=st)
+    st.synchronize()
+    eager2 = t2.numpy()
+    tg = wrs.empty(n1, wrs.ALIAS_DTYPE)
+    g = wrs.Graph(st)
+    with g:
+        plan.psa_build(d1, n1, tg, stream=st)
+    for rep in range(4):
+        tg.fill_bytes(0xAB)
+        wrs.synchronize()
+        g.launch()
+        st.synchronize()
+        assert np.array_equal(tg.numpy(), eager1), rep
+        t2.fill_bytes(0xCD)
+        wrs.synchronize()
+        plan.psa_build(d2, n2, t2, stream=st)  # another size between the replays
+        st.synchronize()
+        assert np.array_equal(t2.numpy(), eager2), rep
+    assert np.all(eager1["a"] < n1) and np.all(eager2["a"] < n2)
