@@ -650,6 +650,7 @@ __device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, const Me
     if (e0 < nl) {
         uint32_t q = q_above(0u, (int64_t)ex[0]);
         int64_t bq = q < nh ? B(q) : kInf;
+        uint32_t alias = s.idx[nl + q];  // re-read only when q moves
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             if (e0 + (uint32_t)k < nl) {
@@ -657,9 +658,10 @@ __device__ __forceinline__ void merge_core(MergeShared<TILE, BLOCK>& s, const Me
                 if (k > 0 && bq <= a) {  // heavies between this light and the last
                     q = q_above(q + 1u, a);
                     bq = q < nh ? B(q) : kInf;
+                    alias = s.idx[nl + q];
                 }
                 const float p = fminf(ratio((double)w[k]), 1.0f);
-                ent[k] = make_uint2(__float_as_uint(p), s.idx[nl + q]);
+                ent[k] = make_uint2(__float_as_uint(p), alias);
             }
         }
     }
@@ -722,12 +724,6 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
     const uint32_t nl = c1.i - c0.i;  // lights packed here
     const uint32_t nh = c1.j - c0.j;  // heavies packed here (heavy c1.j is current at n1)
     const uint32_t ne = nl + nh;      // == n1 - n0 (- 1 in the last block)
-    if (t == 0) {  // the block's split offset, read after the staging barrier
-        const double D = (double)n0 * mc.mean - (c0.lp + c0.hp);
-        const int64_t dq = merge_quant(D, mc.qs);
-        s.dq = dq;
-        s.dres = ldexp((double)dq, -mc.qs) - D + mc.mean;
-    }
     if constexpr (DIRECT) {
         __shared__ uint32_t s_cnt[BLOCK / kWave];
         stage_direct<TILE, BLOCK>(s, s_cnt, pw, idx, ls, prm->pivot, N, H, c0, c1);
@@ -735,6 +731,12 @@ __device__ __forceinline__ void mergepack_block(const float* __restrict__ pw,
         MergeStage<TILE, BLOCK> st;
         st.issue(pw, idx, N, c0, c1);
         st.store(s, ne);
+    }
+    if (t == 0) {  // the block's split offset (after the staging loads are issued)
+        const double D = (double)n0 * mc.mean - (c0.lp + c0.hp);
+        const int64_t dq = merge_quant(D, mc.qs);
+        s.dq = dq;
+        s.dres = ldexp((double)dq, -mc.qs) - D + mc.mean;
     }
     __syncthreads();
     merge_core<PREC, TILE, BLOCK>(s, mc, nl, nh);
