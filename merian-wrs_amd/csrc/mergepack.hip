@@ -49,6 +49,7 @@
 // 64-bit fixed-point sums in every mode (so a block's merge is always a
 // permutation of its slots), and a boundary repair pass keeps the block ranges
 // valid when rounding has made the splits non-monotone.
+#include <algorithm>
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -301,11 +302,13 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
     return Coarse{is, js, (double)lp_t, (double)hp_t, xs, 0u};
 }
 
-// A half-wave takes kCoarseRun consecutive boundaries, each after the first
-// searched in the bracket its predecessor's split gives (runs of 4 measured
-// fastest on the 2^26 Zipf build: 2 and 8 were slower, the chain of dependent
-// searches growing with the run).
-constexpr uint32_t kCoarseRun = 4;
+// A half-wave takes `run` consecutive boundaries, each after the first
+// searched in the bracket its predecessor's split gives.  The launcher sizes
+// the run so that the grid is about one block per resident slot of the
+// device (8 blocks of 256 threads per CU): on the 2^26 Zipf build (65 537
+// boundaries, 256 CUs) that is 4, measured fastest against 1, 2, 3, 5, 6 and
+// 8 (shorter runs leave a second partial wave of blocks, longer ones idle
+// CUs and lengthen the chain of dependent searches).
 
 template <int PREC, bool DIRECT>
 __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
@@ -313,11 +316,12 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 const double* __restrict__ samp_l, uint32_t N,
                                                 const Params* __restrict__ prm, uint32_t nbound,
                                                 uint32_t tile, Coarse* __restrict__ out,
-                                                const uint32_t* __restrict__ idx, LightSrc ls) {
+                                                const uint32_t* __restrict__ idx, LightSrc ls,
+                                                uint32_t run) {
     using T = typename RealT<PREC>::T;
-    const uint32_t b0 = (blockIdx.x * 8u + threadIdx.x / 32u) * kCoarseRun;  // this half-wave's
+    const uint32_t b0 = (blockIdx.x * 8u + threadIdx.x / 32u) * run;  // this half-wave's first
     uint32_t jb_lo = 0u, jb_hi = 0xffffffffu;
-    for (uint32_t b = b0; b < b0 + kCoarseRun && b < nbound; ++b) {  // uniform per half-wave
+    for (uint32_t b = b0; b < b0 + run && b < nbound; ++b) {  // uniform per half-wave
         const uint32_t n = (uint32_t)min((uint64_t)b * tile, (uint64_t)N);  // N < 2^32 <= b * tile
         const Coarse c = coarse_split_half<PREC, DIRECT>(pw, samp_h, samp_l, N, prm->heavy_count,
                                                          (T)prm->mean, (T)prm->light_sum, n, idx,
@@ -1060,12 +1064,22 @@ hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, c
                          hipStream_t s, const uint32_t* idx, LightSrc ls) {
     const uint32_t N = (uint32_t)n;
     const uint32_t nbound = (uint32_t)merge_blocks_for(n, ms.tile) + 1u;
-    const uint32_t runs = (nbound + kCoarseRun - 1u) / kCoarseRun;
+    static const uint32_t slots = [] {  // resident 256-thread blocks: 8 per CU
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        return 8u * (uint32_t)cus;
+    }();
+    const uint32_t half_waves = 8u * slots;  // 8 half-waves a block
+    const uint32_t run = std::min(8u, std::max(1u, (nbound + half_waves / 2u) / half_waves));
+    const uint32_t runs = (nbound + run - 1u) / run;
     const uint32_t cgrid = (runs + 7u) / 8u;  // 8 runs of boundaries per 256-thread block
     // the split arithmetic is FP32 in FP32 mode and FP64 in all others
 #define WRS_COARSE(P, D)                                                                    \
     k_coarse<P, D><<<cgrid, 256, 0, s>>>(pw, samp_h, samp_l, N, prm, nbound, ms.tile, bounds, \
-                                         idx, ls)
+                                         idx, ls, run)
     if (prec == kFP32) {
         if (ls.lpos) WRS_COARSE(kFP32, true);
         else WRS_COARSE(kFP32, false);
