@@ -145,7 +145,7 @@ void oro_segment_sizes(uint64_t seed, uint32_t b, uint64_t lo, uint64_t hi, uint
 }
 
 /* ======================================================================== */
-/* reduce.hpp:163-174 kahan_reduction                                        */
+/* reduce.hpp:81-92 kahan_reduction                                          */
 /* ======================================================================== */
 #define DEFINE_KAHAN(T, NAME)                                                  \
     T NAME(const T* x, uint64_t n) {                                           \

@@ -53,7 +53,7 @@ void oro_gen_zipf(uint64_t seed, double s, uint64_t n, float* out);
 void oro_gen_spike(uint64_t seed, uint64_t n, float* out);
 void oro_segment_sizes(uint64_t seed, uint32_t b, uint64_t lo, uint64_t hi, uint64_t* sizes);
 
-/* ---- reductions (src/host/reference/reduce.hpp:163-174) ----------------- */
+/* ---- reductions (src/host/reference/reduce.hpp:81-92) ------------------- */
 float oro_kahan_reduce_f32(const float* x, uint64_t n);
 double oro_kahan_reduce_f64(const double* x, uint64_t n);
 
