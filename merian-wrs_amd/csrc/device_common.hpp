@@ -192,6 +192,18 @@ __device__ __forceinline__ uint64_t add_dpp_u64(uint64_t v) {
     return ((uint64_t)rh << 32) | rl;
 }
 
+// Inclusive scan of a 32-bit integer over the 64 lanes, by the DPP steps of
+// wave_incl_scan_u64 below.
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+    v += dpp_u32<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_u32<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_u32<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_u32<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_u32<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_u32<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 // Inclusive scan of a 64-bit integer over the 64 lanes (modular addition, so
 // the association does not matter): shifts by 1, 2, 4, 8 lanes within each
 // row of 16, then the total of row 0 (2) added to row 1 (3) by row_bcast:15,

@@ -199,31 +199,93 @@ template <int PREC>
 struct TileShared {
     float buf[kPad];
     Acc<PREC> wave[kBlock / kWave];
+    uint32_t cnt[kBlock / kWave];
 };
 
 // Loads the m (<= kTile) elements of w[base..) so that thread t owns elements
-// 16t..16t+15 (zero beyond m), classifies them against the pivot and scans:
-// thr_excl is the sum over the thread's predecessors in the tile, tile_agg
-// the tile total.  Ends with the block synchronised and s.buf free.
+// 16t..16t+15 (zero beyond m).  Ends with s.buf still in use (the caller
+// synchronises before reusing it).
 template <int PREC>
-__device__ __forceinline__ Acc<PREC> thread_sums(const float* __restrict__ w, uint32_t base,
-                                                 uint32_t m, float pivot, TileShared<PREC>& s,
-                                                 float (&v)[kItems]) {
+__device__ __forceinline__ void load_thread_elems(const float* __restrict__ w, uint32_t base,
+                                                  uint32_t m, TileShared<PREC>& s,
+                                                  float (&v)[kItems]) {
     const int t = threadIdx.x;
     load_tile_lds<true>(w + base, m, s.buf);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) v[k] = s.buf[17 * t + k];
+}
 
-    // elements past m are zero: light, and adding +0 leaves a sum's bits unchanged
-    Acc<PREC> mine = acc_zero<PREC>();
+// The thread's heavy and light sums of its elements (heavy: bit k of hmask) in
+// element order, in the mode's tile arithmetic (tilesum.hpp), with the running
+// sums captured after element kh (heavy) and kl (light); -1 captures nothing.
+template <int PREC>
+__device__ __forceinline__ Acc<PREC> run_sums(const float (&v)[kItems], uint32_t hmask, int kh,
+                                              int kl, typename Acc1<PREC>::T& hcap,
+                                              typename Acc1<PREC>::T& lcap) {
+    using A1 = Acc1<PREC>;
+    Acc<PREC> r = acc_zero<PREC>();
+    hcap = A1::zero();
+    lcap = A1::zero();
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
-        const bool heavy = v[k] > pivot;
-        add_split<PREC>(mine.h, mine.l, heavy, v[k]);
-        mine.c += heavy ? 1u : 0u;
+        const bool heavy = (hmask >> k) & 1u;
+        add_split<PREC>(r.h, r.l, heavy, v[k]);
+        hcap = A1::sel(k == kh, r.h, hcap);
+        lcap = A1::sel(k == kl, r.l, lcap);
+        // taken here: otherwise the compiler keeps all 16 running sums live
+        // to select one at the end (126 instead of 66 VGPRs in the emit)
+        A1::pin(hcap);
+        A1::pin(lcap);
     }
-    return mine;
+    r.c = (uint32_t)__popc(hmask);
+    return r;
+}
+
+// Heavy / light masks of the thread's elements (only elements < m count).
+__device__ __forceinline__ void classify(const float (&v)[kItems], float pivot, uint32_t e0,
+                                         uint32_t m, uint32_t& hmask, uint32_t& lmask) {
+    uint32_t h = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) h |= (v[k] > pivot ? 1u : 0u) << k;
+    const uint32_t nvalid = e0 >= m ? 0u : min((uint32_t)kItems, m - e0);
+    const uint32_t valid = (1u << nvalid) - 1u;  // nvalid <= 16
+    hmask = h & valid;
+    lmask = ~h & valid;
+}
+
+// Position of the j-th (from 0) set bit of a 16-bit mask holding more than j bits.
+__device__ __forceinline__ int nth_bit(uint32_t mask, uint32_t j) {
+    int pos = 0;
+    uint32_t c = (uint32_t)__popc(mask & 0xffu);
+    if (j >= c) { j -= c; mask >>= 8; pos += 8; }
+    c = (uint32_t)__popc(mask & 0xfu);
+    if (j >= c) { j -= c; mask >>= 4; pos += 4; }
+    c = (uint32_t)__popc(mask & 0x3u);
+    if (j >= c) { j -= c; mask >>= 2; pos += 2; }
+    c = mask & 1u;
+    if (j >= c) pos += 1;
+    return pos;
+}
+
+// Block-wide exclusive heavy count of this thread (c = its own count) and
+// the tile's count.  Synchronises the block.
+template <int PREC>
+__device__ __forceinline__ void count_scan(uint32_t c, TileShared<PREC>& s, uint32_t& excl,
+                                           uint32_t& total) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint32_t incl = wave_incl_scan_u32(c);
+    if (lane == kWave - 1) s.cnt[wid] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / kWave; ++q) {
+        before += q < wid ? s.cnt[q] : 0u;
+        all += s.cnt[q];
+    }
+    excl = before + incl - c;
+    total = all;
 }
 
 // Tile total from the per-wave totals in s.wave (block synchronised), the
@@ -242,11 +304,23 @@ __device__ __forceinline__ Acc<PREC> combine_waves(const TileShared<PREC>& s, in
     return c;
 }
 
-// The tile total alone (the aggregate passes): a wave's total is lane 63 of a
-// 64-lane Hillis-Steele scan, and these DPP row steps perform exactly the
-// additions that lane depends on, in the same order (within rows of 16 by
-// shifts of 1, 2, 4, 8, then row 2's total into row 3 and rows 0-1's into
-// rows 2-3), so the total has the scan's bits.  Ends with the block
+// Inclusive wave scan of the per-thread sums by DPP row steps (within rows of
+// 16 by shifts of 1, 2, 4, 8, then row 2's total into row 3 and rows 0-1's
+// into rows 2-3), the moved value the left operand.  Lane 63 receives the
+// wave total by the same additions, in the same order, as the 64-lane
+// Hillis-Steele scan, so tile totals built on it have the bits of tile_total.
+template <int PREC>
+__device__ __forceinline__ Acc<PREC> wave_scan_dpp(Acc<PREC> x) {
+    x = acc_add_dpp<PREC, 0x111, 0xf>(x);  // row_shr:1
+    x = acc_add_dpp<PREC, 0x112, 0xf>(x);  // row_shr:2
+    x = acc_add_dpp<PREC, 0x114, 0xf>(x);  // row_shr:4
+    x = acc_add_dpp<PREC, 0x118, 0xf>(x);  // row_shr:8
+    x = acc_add_dpp<PREC, 0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+    x = acc_add_dpp<PREC, 0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
+// The tile total alone (the aggregate passes).  Ends with the block
 // synchronised and s.buf free.
 template <int PREC>
 __device__ __forceinline__ Acc<PREC> tile_total(const float* __restrict__ w, uint32_t base,
@@ -255,39 +329,100 @@ __device__ __forceinline__ Acc<PREC> tile_total(const float* __restrict__ w, uin
     const int lane = t & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
     float v[kItems];
-    Acc<PREC> x = thread_sums<PREC>(w, base, m, pivot, s, v);
-    x = acc_add_dpp<PREC, 0x111, 0xf>(x);  // row_shr:1
-    x = acc_add_dpp<PREC, 0x112, 0xf>(x);  // row_shr:2
-    x = acc_add_dpp<PREC, 0x114, 0xf>(x);  // row_shr:4
-    x = acc_add_dpp<PREC, 0x118, 0xf>(x);  // row_shr:8
-    x = acc_add_dpp<PREC, 0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
-    x = acc_add_dpp<PREC, 0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+    load_thread_elems<PREC>(w, base, m, s, v);
+    // the sums of run_sums (elements past m are zero: adding them changes no
+    // bits, whatever their class), the count of the elements below m only
+    const uint32_t e0 = (uint32_t)(kItems * t);
+    Acc<PREC> x = acc_zero<PREC>();
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const bool heavy = v[k] > pivot;
+        add_split<PREC>(x.h, x.l, heavy, v[k]);
+        x.c += heavy && e0 + k < m ? 1u : 0u;
+    }
+    x = wave_scan_dpp<PREC>(x);
     if (lane == kWave - 1) s.wave[wid] = x;
     __syncthreads();
     return combine_waves<PREC>(s, wid, nullptr);
 }
 
+// Per-thread state of a tile pass.
 template <int PREC>
-__device__ __forceinline__ void tile_scan(const float* __restrict__ w, uint32_t base, uint32_t m,
-                                          float pivot, TileShared<PREC>& s, float (&v)[kItems],
-                                          Acc<PREC>& thr_excl, Acc<PREC>& tile_agg) {
+struct TileScan {
+    using T = typename Acc1<PREC>::T;
+    uint32_t hmask, lmask;  // bit k: element e0 + k (< m) is heavy / light
+    uint32_t c_excl;        // heavies of the tile before this thread
+    Acc<PREC> excl;         // sums (and count) of the tile before this thread
+    Acc<PREC> agg;          // the tile's sums and count
+    // sampled ranks: this thread's heavy and light whose inclusive prefix is a
+    // sample (rank + 1 a multiple of kSample) and its light of rank a multiple
+    // of kSample (local element, -1: none), and the ranks themselves
+    int kh, kl, kp;
+    uint32_t rh, rl, rp;
+    T hcap, lcap;           // the thread's running sums through kh / kl
+};
+
+// Ranks of the thread's sampled elements, given the heavies (Hx) and lights
+// (Lx) before the tile.
+template <int PREC>
+__device__ __forceinline__ void sample_ranks(TileScan<PREC>& ts, uint32_t Hx, uint32_t Lx,
+                                             uint32_t e0) {
+    const uint32_t nh = (uint32_t)__popc(ts.hmask), nl = (uint32_t)__popc(ts.lmask);
+    const uint32_t hr0 = Hx + ts.c_excl;         // rank of the thread's first heavy
+    const uint32_t lr0 = Lx + (e0 - ts.c_excl);  // rank of its first light (if any)
+    const uint32_t jh = (kSample - 1u - hr0) & (kSample - 1u);
+    const uint32_t jl = (kSample - 1u - lr0) & (kSample - 1u);
+    const uint32_t jp = (0u - lr0) & (kSample - 1u);
+    ts.kh = jh < nh ? nth_bit(ts.hmask, jh) : -1;
+    ts.kl = jl < nl ? nth_bit(ts.lmask, jl) : -1;
+    ts.kp = jp < nl ? nth_bit(ts.lmask, jp) : -1;
+    ts.rh = hr0 + jh;
+    ts.rl = lr0 + jl;
+    ts.rp = lr0 + jp;
+}
+
+// One tile pass: loads the tile, classifies, scans the heavy counts, then (with
+// `carry_known`, the heavies before the tile in Hx) sums the thread's elements
+// once in the tile arithmetic, capturing the running sums at its sampled ranks;
+// finally the block scan of the sums (ts.excl, ts.agg).  Without
+// `carry_known` (the look-back forms, whose carry follows from the aggregate)
+// nothing is captured: call recapture() once the carry is known.  Ends with
+// the block synchronised and s.buf free.
+template <int PREC>
+__device__ __forceinline__ void tile_pass(const float* __restrict__ w, uint32_t base, uint32_t m,
+                                          float pivot, bool carry_known, uint32_t Hx,
+                                          TileShared<PREC>& s, float (&v)[kItems],
+                                          TileScan<PREC>& ts) {
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
-    const Acc<PREC> mine = thread_sums<PREC>(w, base, m, pivot, s, v);
-    Acc<PREC> incl = mine;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const Acc<PREC> o = acc_shfl_up<PREC>(incl, d);
-        if (lane >= d) incl = acc_add<PREC>(o, incl);
-    }
-    Acc<PREC> lane_excl = acc_shfl_up<PREC>(incl, 1);
-    if (lane == 0) lane_excl = acc_zero<PREC>();
+    const uint32_t e0 = (uint32_t)(kItems * t);
+    load_thread_elems<PREC>(w, base, m, s, v);
+    classify(v, pivot, e0, m, ts.hmask, ts.lmask);
+    uint32_t c_tile;
+    count_scan<PREC>((uint32_t)__popc(ts.hmask), s, ts.c_excl, c_tile);
+    ts.kh = ts.kl = ts.kp = -1;
+    if (carry_known) sample_ranks<PREC>(ts, Hx, base - Hx, e0);
+    const Acc<PREC> mine = run_sums<PREC>(v, ts.hmask, ts.kh, ts.kl, ts.hcap, ts.lcap);
+    const Acc<PREC> incl = wave_scan_dpp<PREC>(mine);
+    using A1 = Acc1<PREC>;
+    Acc<PREC> lane_excl{A1::template dpp<0x138, 0xf>(incl.h), A1::template dpp<0x138, 0xf>(incl.l),
+                        0u};  // wave_shr:1, lane 0 receives zero
     if (lane == kWave - 1) s.wave[wid] = incl;
     __syncthreads();
     Acc<PREC> wave_excl;
-    tile_agg = combine_waves<PREC>(s, wid, &wave_excl);
-    thr_excl = acc_add<PREC>(wave_excl, lane_excl);
+    ts.agg = combine_waves<PREC>(s, wid, &wave_excl);
+    ts.agg.c = c_tile;
+    ts.excl = acc_add<PREC>(wave_excl, lane_excl);
+    ts.excl.c = ts.c_excl;
+}
+
+// The captures of a pass run without a known carry.
+template <int PREC>
+__device__ __forceinline__ void recapture(const float (&v)[kItems], uint32_t Hx, uint32_t base,
+                                          TileScan<PREC>& ts) {
+    sample_ranks<PREC>(ts, Hx, base - Hx, (uint32_t)(kItems * threadIdx.x));
+    (void)run_sums<PREC>(v, ts.hmask, ts.kh, ts.kl, ts.hcap, ts.lcap);
 }
 
 // Carries of one tile: the sums and count of everything before it.
@@ -313,21 +448,39 @@ __device__ __forceinline__ TileCarry tile_carry(const Carry& excl, int shift) {
     return r;
 }
 
+// Inclusive prefix of the heavy (light) stream at an element of this thread
+// whose thread-local running sum is `run`: the tile carry plus (the sum of the
+// thread's predecessors in the tile plus `run`), FP64 in the FP64-carry modes.
+template <int PREC>
+__device__ __forceinline__ double prefix_at(bool heavy, const TileCarry& cy,
+                                            const TileScan<PREC>& ts,
+                                            const typename Acc1<PREC>::T& run) {
+    using A1 = Acc1<PREC>;
+    const typename A1::T part = A1::add(heavy ? ts.excl.h : ts.excl.l, run);
+    if constexpr (PREC == kFP32) return (double)((heavy ? cy.hf : cy.lf) + part);
+    else return (heavy ? cy.h : cy.l) + A1::value(part);
+}
+
+// Tiles of the direct form with at most this many heavies write them straight
+// from the registers (a few store instructions, mostly with few lanes active);
+// denser tiles stage them in LDS and write them as a run of consecutive
+// words: on a 2^26 build, 94 vs 103 us for Zipf(1.1) weights (2.7% heavy) and
+// 267 vs 119 us for U(0, 1) weights (half heavy) with the other choice.
+constexpr uint32_t kRegHeavies = 512;
+
 // Writes the tile's outputs given its carries (block-uniform): the partition
 // indices, the weights in partition order (pw_out, when non-null), the prefix
-// samples at ranks r with (r+1) % kSample == 0, and with FULL the hi (and lo)
-// prefix arrays of every element.  Without FULL a prefix is formed only where
-// a sample falls.  With DIRECT only the heavies' indices and weights are
-// written (their positions [0, H) of idx/pw, straight from the registers: a
-// few percent of the elements, so the LDS transpose that coalesces the full
-// arrays is skipped), and for every light rank r with r % kSample == 0 its
-// element index goes to lpos[r / kSample]: the merge pack then gathers the
-// lights from the weights themselves.  Expects s.buf free (the block
-// synchronised after its last use).
+// samples at ranks r with (r+1) % kSample == 0 (from the pass's captures),
+// and with FULL the hi (and lo) prefix arrays of every element.  With DIRECT
+// only the heavies' indices and weights are written (their positions [0, H)
+// of idx/pw), and for every
+// light rank r with r % kSample == 0 its element index goes to lpos[r /
+// kSample]: the merge pack then gathers the lights from the weights
+// themselves.  Expects s.buf free (the block synchronised after its last use).
 template <int PREC, bool FULL, bool DIRECT = false>
-__device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m, float pivot,
-                                          const float (&v)[kItems], const Acc<PREC>& thr_excl,
-                                          uint32_t hc, const TileCarry& cy, TileShared<PREC>& s,
+__device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
+                                          const float (&v)[kItems], const TileScan<PREC>& ts,
+                                          const TileCarry& cy, TileShared<PREC>& s,
                                           uint32_t* __restrict__ idx_out,
                                           float* __restrict__ pw_out,
                                           float* __restrict__ hi_out, float* __restrict__ lo_out,
@@ -337,95 +490,96 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
     static_assert(!(FULL && DIRECT), "the direct form writes no full prefix arrays");
     const int t = threadIdx.x;
     const uint32_t e0 = (uint32_t)(kItems * t);
+    const uint32_t hc = ts.agg.c;   // heavies of the tile
     const uint32_t Hx = cy.c;       // heavies before this tile
     const uint32_t Lx = base - Hx;  // lights before this tile
     uint32_t* s_u = reinterpret_cast<uint32_t*>(s.buf);
 
+    // prefix samples (at most one heavy and one light per thread)
+    if (samp_h != nullptr) {
+        if (ts.kh >= 0) samp_h[(ts.rh + 1u) / kSample - 1u] = prefix_at<PREC>(true, cy, ts, ts.hcap);
+        if (ts.kl >= 0) samp_l[(ts.rl + 1u) / kSample - 1u] = prefix_at<PREC>(false, cy, ts, ts.lcap);
+    }
+    if constexpr (DIRECT) {
+        if (ts.kp >= 0) lpos[ts.rp / kSample] = base + e0 + (uint32_t)ts.kp;
+        if (hc <= kRegHeavies) {  // few heavies: straight from the registers
+            uint32_t hr = Hx + ts.c_excl;  // global rank of the next heavy
+#pragma unroll
+            for (int k = 0; k < kItems; ++k) {
+                if ((ts.hmask >> k) & 1u) {
+                    idx_out[hr] = base + e0 + k;
+                    pw_out[hr] = v[k];
+                }
+                hr += (ts.hmask >> k) & 1u;
+            }
+            return;
+        }
+    }
+
+    // LDS slot of each element: heavy run (by tile-local heavy rank), then the
+    // light run reversed
     uint32_t slot[kItems];
-    if constexpr (!DIRECT) {
-        uint32_t hr = thr_excl.c;
+    {
+        uint32_t hr = ts.c_excl;
 #pragma unroll
         for (int k = 0; k < kItems; ++k) {
             const uint32_t e = e0 + k;
-            const bool heavy = v[k] > pivot;
+            const bool heavy = (ts.hmask >> k) & 1u;
             slot[k] = heavy ? hr : (m - 1u - (e - hr));
             hr += heavy ? 1u : 0u;
         }
     }
     // positions of LDS slot q: heavy run then light run
     auto out_pos = [&](uint32_t q) -> uint32_t { return q < hc ? Hx + q : n - m - Lx + q; };
-
-    if constexpr (!DIRECT) {
-        // round A: indices
+    // the direct form writes the heavy run only: slots [0, hc)
+    const uint32_t q_end = DIRECT ? hc : m;
+    if (DIRECT && hc == 0) return;
+    // round A: indices
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (e0 + k < m) s_u[slot[k]] = base + e0 + k;
+    __syncthreads();
+    for (uint32_t q = t; q < q_end; q += kBlock) idx_out[out_pos(q)] = s_u[q];
+    // round A': weights
+    if (pw_out != nullptr) {
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < kItems; ++k)
-            if (e0 + k < m) s_u[slot[k]] = base + e0 + k;
+            if (e0 + k < m) s.buf[slot[k]] = v[k];
         __syncthreads();
-        for (uint32_t q = t; q < m; q += kBlock) idx_out[out_pos(q)] = s_u[q];
-        // round A': weights
-        if (pw_out != nullptr) {
+        for (uint32_t q = t; q < q_end; q += kBlock) pw_out[out_pos(q)] = s.buf[q];
+    }
+    if constexpr (FULL) {
+        // round B: the inclusive prefix of every element (the samples above
+        // are the values of the same formula at the sampled elements)
+        float hi_keep[kItems], lo_keep[kItems];
+        using A1 = Acc1<PREC>;
+        typename A1::T rh = A1::zero(), rl = A1::zero();
+#pragma unroll
+        for (int k = 0; k < kItems; ++k) {
+            const bool heavy = (ts.hmask >> k) & 1u;
+            add_split<PREC>(rh, rl, heavy, v[k]);
+            const double P = prefix_at<PREC>(heavy, cy, ts, heavy ? rh : rl);
+            const float hi = (float)P;
+            hi_keep[k] = hi;
+            lo_keep[k] = prec_has_lo(PREC) ? (float)(P - (double)hi) : 0.0f;
+        }
+        if (hi_out != nullptr) {
             __syncthreads();
 #pragma unroll
             for (int k = 0; k < kItems; ++k)
-                if (e0 + k < m) s.buf[slot[k]] = v[k];
+                if (e0 + k < m) s.buf[slot[k]] = hi_keep[k];
             __syncthreads();
-            for (uint32_t q = t; q < m; q += kBlock) pw_out[out_pos(q)] = s.buf[q];
-        }
-    }
-
-    // round B: inclusive prefixes (FP64 in the FP64-carry modes)
-    using A1 = Acc1<PREC>;
-    auto prefix = [&](bool heavy, const typename A1::T& run) -> double {
-        if constexpr (PREC == kFP32) return (double)((heavy ? cy.hf : cy.lf) + run);
-        else return (heavy ? cy.h : cy.l) + A1::value(run);
-    };
-    float hi_keep[kItems], lo_keep[kItems];
-    {
-        typename A1::T rh = thr_excl.h, rl = thr_excl.l;
-        uint32_t hr = Hx + thr_excl.c;         // global heavy rank of the next heavy
-        uint32_t lr = Lx + (e0 - thr_excl.c);  // global light rank of the next light
+            for (uint32_t q = t; q < m; q += kBlock) hi_out[out_pos(q)] = s.buf[q];
+            if constexpr (prec_has_lo(PREC)) {
+                if (lo_out != nullptr) {
+                    __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            const bool heavy = v[k] > pivot;
-            add_split<PREC>(rh, rl, heavy, v[k]);
-            const uint32_t r1 = (heavy ? hr : lr) + 1u;  // the prefix covers ranks < r1
-            const bool sample = e0 + k < m && samp_h != nullptr && (r1 & (kSample - 1)) == 0;
-            if (DIRECT && e0 + k < m) {
-                if (heavy) {
-                    idx_out[hr] = base + e0 + k;
-                    pw_out[hr] = v[k];
-                } else if ((lr & (kSample - 1)) == 0) {
-                    lpos[lr / kSample] = base + e0 + k;
+                    for (int k = 0; k < kItems; ++k)
+                        if (e0 + k < m) s.buf[slot[k]] = lo_keep[k];
+                    __syncthreads();
+                    for (uint32_t q = t; q < m; q += kBlock) lo_out[out_pos(q)] = s.buf[q];
                 }
-            }
-            if constexpr (FULL) {
-                const double P = prefix(heavy, heavy ? rh : rl);
-                const float hi = (float)P;
-                hi_keep[k] = hi;
-                lo_keep[k] = prec_has_lo(PREC) ? (float)(P - (double)hi) : 0.0f;
-                if (sample) (heavy ? samp_h : samp_l)[r1 / kSample - 1u] = P;
-            } else if (sample) {
-                (heavy ? samp_h : samp_l)[r1 / kSample - 1u] = prefix(heavy, heavy ? rh : rl);
-            }
-            hr += heavy ? 1u : 0u;
-            lr += heavy ? 0u : 1u;
-        }
-    }
-    if (FULL && hi_out != nullptr) {
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kItems; ++k)
-            if (e0 + k < m) s.buf[slot[k]] = hi_keep[k];
-        __syncthreads();
-        for (uint32_t q = t; q < m; q += kBlock) hi_out[out_pos(q)] = s.buf[q];
-        if constexpr (prec_has_lo(PREC)) {
-            if (lo_out != nullptr) {
-                __syncthreads();
-#pragma unroll
-                for (int k = 0; k < kItems; ++k)
-                    if (e0 + k < m) s.buf[slot[k]] = lo_keep[k];
-                __syncthreads();
-                for (uint32_t q = t; q < m; q += kBlock) lo_out[out_pos(q)] = s.buf[q];
             }
         }
     }
@@ -558,13 +712,12 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);  // n + kTile - 1 may exceed 2^32
     const Carry excl =
         load_carry(states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords);
-    float v[kItems];
-    Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
     const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
-    __syncthreads();  // s.buf is reused by the emit rounds
-    tile_emit<PREC, FULL, DIRECT>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out,
-                                  pw_out, hi_out, lo_out, samp_h, samp_l, lpos);
+    float v[kItems];
+    TileScan<PREC> ts;
+    tile_pass<PREC>(w, base, m, pivot, true, cy.c, s, v, ts);
+    tile_emit<PREC, FULL, DIRECT>(n, base, m, v, ts, cy, s, idx_out, pw_out, hi_out, lo_out,
+                                  samp_h, samp_l, lpos);
 }
 
 // ---- single distribution: single pass with decoupled look-back --------------
@@ -591,13 +744,13 @@ __global__ __launch_bounds__(kBlock) void k_part_lookback(
     const float pivot = prm->pivot;
     const int shift = prm->shift;
     float v[kItems];
-    Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC>(w, base, m, pivot, s, v, thr_excl, tile_agg);
+    TileScan<PREC> ts;
+    tile_pass<PREC>(w, base, m, pivot, false, 0u, s, v, ts);
 
     const int lane = threadIdx.x & (kWave - 1);
     if (threadIdx.x < kWave) {
         uint64_t* rec = states + (uint64_t)tile * kStateWords;
-        const Carry agg = to_carry<PREC>(tile_agg, shift);
+        const Carry agg = to_carry<PREC>(ts.agg, shift);
         Carry excl = carry_zero();
         if (tile == 0) {
             if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, agg);
@@ -618,8 +771,9 @@ __global__ __launch_bounds__(kBlock) void k_part_lookback(
     }
     __syncthreads();
     const TileCarry cy = tile_carry<PREC>(s_excl, shift);
-    tile_emit<PREC, FULL, DIRECT>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx_out,
-                                  pw_out, hi_out, lo_out, samp_h, samp_l, lpos);
+    recapture<PREC>(v, cy.c, base, ts);
+    tile_emit<PREC, FULL, DIRECT>(n, base, m, v, ts, cy, s, idx_out, pw_out, hi_out, lo_out,
+                                  samp_h, samp_l, lpos);
 }
 
 // ---- segmented: single pass with decoupled look-back -----------------------
@@ -655,13 +809,13 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     const float pivot = prm->pivot;
     const int shift = prm->shift;
     float v[kItems];
-    Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC>(w + off, base, m, pivot, s, v, thr_excl, tile_agg);
+    TileScan<PREC> ts;
+    tile_pass<PREC>(w + off, base, m, pivot, false, 0u, s, v, ts);
 
     const int lane = threadIdx.x & (kWave - 1);
     if (threadIdx.x < kWave) {
         uint64_t* rec = seg_states + (uint64_t)tile * kStateWords;
-        const Carry agg = to_carry<PREC>(tile_agg, shift);
+        const Carry agg = to_carry<PREC>(ts.agg, shift);
         Carry excl = carry_zero();
         if (tile == 0) {
             if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, agg);
@@ -682,8 +836,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
     }
     __syncthreads();
     const TileCarry cy = tile_carry<PREC>(s_excl, shift);
-    tile_emit<PREC, false>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off,
-                           pw + off, nullptr, nullptr, samp_h + so, samp_l + so);
+    recapture<PREC>(v, cy.c, base, ts);
+    tile_emit<PREC, false>(n, base, m, v, ts, cy, s, idx + off, pw + off, nullptr, nullptr,
+                           samp_h + so, samp_l + so);
 }
 
 // ---- segmented: reduce-then-scan ------------------------------------------
@@ -791,13 +946,12 @@ __global__ __launch_bounds__(kBlock) void k_seg_part_emit(
     const Params* prm = seg_prm + sg;
     const float pivot = prm->pivot;
     const Carry excl = load_carry(states + (uint64_t)kExclBase * items + (uint64_t)item * kCarryWords);
-    float v[kItems];
-    Acc<PREC> thr_excl, tile_agg;
-    tile_scan<PREC>(w + off, base, m, pivot, s, v, thr_excl, tile_agg);
     const TileCarry cy = tile_carry<PREC>(excl, prm->shift);
-    __syncthreads();  // s.buf is reused by the emit rounds
-    tile_emit<PREC, false, DIRECT>(n, base, m, pivot, v, thr_excl, tile_agg.c, cy, s, idx + off,
-                                   pw + off, nullptr, nullptr, samp_h + so, samp_l + so,
+    float v[kItems];
+    TileScan<PREC> ts;
+    tile_pass<PREC>(w + off, base, m, pivot, true, cy.c, s, v, ts);
+    tile_emit<PREC, false, DIRECT>(n, base, m, v, ts, cy, s, idx + off, pw + off, nullptr,
+                                   nullptr, samp_h + so, samp_l + so,
                                    DIRECT ? lpos + seg_lpos0[sg] : nullptr);
 }
 

@@ -9,7 +9,9 @@
 //                      issues at the rate of an FP32 add
 //
 // All operations are in a fixed order, so a tile's sums are a function of its
-// weights only.
+// weights only.  pin() is an empty register barrier: it makes a value exist at
+// that point of the program (an optimisation fence for the scheduler, no
+// instruction).
 #pragma once
 
 #include "device_common.hpp"
@@ -28,6 +30,7 @@ struct Acc1 {
     static __device__ __forceinline__ T sel(bool c, T a, T b) { return c ? a : b; }
     template <int CTRL, int RM>
     static __device__ __forceinline__ T dpp(T a) { return wrsdev::dpp_f32<CTRL, RM>(a); }
+    static __device__ __forceinline__ void pin(T& a) { asm volatile("" : "+v"(a)); }
 };
 
 template <>
@@ -47,6 +50,7 @@ struct Acc1<kComp> {
     static __device__ __forceinline__ T dpp(T a) {
         return T{wrsdev::dpp_f32<CTRL, RM>(a.hi), wrsdev::dpp_f32<CTRL, RM>(a.lo)};
     }
+    static __device__ __forceinline__ void pin(T& a) { asm volatile("" : "+v"(a.hi), "+v"(a.lo)); }
 };
 
 template <>
@@ -63,6 +67,7 @@ struct Acc1<kFP64> {
         return __longlong_as_double(
             (long long)wrsdev::dpp_u64<CTRL, RM>((uint64_t)__double_as_longlong(a)));
     }
+    static __device__ __forceinline__ void pin(T& a) { asm volatile("" : "+v"(a)); }
 };
 
 // Adds x to h if heavy, else to l, with one addition and selects (no branch;

@@ -204,7 +204,7 @@ wrs_tuning tune_psa(const wrs::PSAConfig& base, const float* w, uint64_t n, uint
     wrs_tuning best{};
     if (!(flags & WRS_TUNE_NO_CACHE) && cache_get(key.str(), best)) return best;
 
-    wrs::Workspace ws = wrs::Workspace::allocate(n, true, wrsk::prec_has_lo(base.precision));
+    wrs::Workspace ws = wrs::Workspace::allocate(n, true);
     wrs_alias_entry* table = nullptr;
     try {
         wrs::check_hip(hipMalloc(&table, n * sizeof(wrs_alias_entry)), "hipMalloc");

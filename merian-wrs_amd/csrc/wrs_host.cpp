@@ -38,7 +38,7 @@ static T* dev_alloc(uint64_t count) {
     return static_cast<T*>(p);
 }
 
-Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_buffer) {
+Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers) {
     Workspace ws;
     ws.max_n = max_n;
     try {
@@ -52,8 +52,6 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
         if (partition_buffers) {
             ws.indices = dev_alloc<uint32_t>(max_n);
             ws.pweights = dev_alloc<float>(max_n);
-            ws.prefix_hi = dev_alloc<float>(max_n);
-            if (lo_buffer) ws.prefix_lo = dev_alloc<float>(max_n);
             ws.samp_h = dev_alloc<double>(wrsk::samples_for(max_n));
             ws.samp_l = dev_alloc<double>(wrsk::samples_for(max_n));
             ws.bounds = dev_alloc<wrsk::Coarse>(wrsk::merge_blocks_for(max_n, wrsk::kMinMergeTile) + 2);
@@ -64,6 +62,11 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers, bool lo_bu
         throw;
     }
     return ws;
+}
+
+void Workspace::ensure_prefix(bool lo) {
+    if (!prefix_hi) prefix_hi = dev_alloc<float>(max_n);
+    if (lo && !prefix_lo) prefix_lo = dev_alloc<float>(max_n);
 }
 
 void Workspace::release() {
@@ -150,6 +153,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     // full prefix arrays only for a debug build (parity tests); the build
     // itself needs the indices and the sparse prefix samples
     const bool full_prefix = dbg && (dbg->prefix_hi || dbg->prefix_lo || dbg->splits);
+    if (full_prefix) ws.ensure_prefix(wrsk::prec_has_lo(prec));
     float* hi = full_prefix ? ws.prefix_hi : nullptr;
     float* lo = full_prefix && wrsk::prec_has_lo(prec) ? ws.prefix_lo : nullptr;
     // the direct form (partition arrays hold the heavies only) for plain builds;
@@ -371,7 +375,7 @@ wrs_status wrs_plan_create(const wrs_config* cfg, uint64_t max_n, wrs_plan** out
         wrs_plan* p = new wrs_plan;
         p->cfg = c;
         try {
-            p->ws = wrs::Workspace::allocate(max_n, true, wrsk::prec_has_lo(c.precision));
+            p->ws = wrs::Workspace::allocate(max_n, true);
             for (auto& e : p->ev) wrs::check_hip(hipEventCreate(&e), "hipEventCreate");
         } catch (...) {
             p->ws.release();
@@ -539,6 +543,7 @@ wrs_status wrs_sample_stats(wrs_plan* plan, const float* weights, uint64_t n, co
         if (n == 0 || n > plan->ws.max_n) throw wrs::Error(WRS_EINVAL, "n must be in [1, max_n]");
         if (s == 0) throw wrs::Error(WRS_EINVAL, "s must be positive");
         wrs::Workspace& ws = plan->ws;
+        ws.ensure_prefix(wrsk::prec_has_lo(plan->cfg.precision));
         const bool lo = wrsk::prec_has_lo(plan->cfg.precision) && ws.prefix_lo;
         wrs::PrefixSum(plan->cfg.precision, true)
             .run(hs(stream), wrs::PrefixSum::Buffers{weights, ws.prefix_hi, lo ? ws.prefix_lo : nullptr},

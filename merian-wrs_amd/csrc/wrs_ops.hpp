@@ -56,7 +56,10 @@ struct Workspace {
     uint32_t* lightpos = nullptr;      // [max_n/kSample + 1] direct form: element of every
                                        // kSample-th light (PSA only)
 
-    static Workspace allocate(uint64_t max_n, bool partition_buffers, bool lo_buffer);
+    static Workspace allocate(uint64_t max_n, bool partition_buffers);
+    // the full prefix arrays of debug builds and of the sample statistics,
+    // allocated on first use
+    void ensure_prefix(bool lo);
     void release();
 };
 
