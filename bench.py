@@ -218,7 +218,7 @@ def main() -> int:
     st.synchronize()
     H = int(hbuf.numpy()[0])
     p, t_ = args.precision, (tune or {})
-    direct = bool(t_.get("direct"))
+    direct = t_.get("direct", 0) != 2  # WRS_PACK_CLASSIC: 2; the default is direct
     lookback = bool(t_.get("partition"))
     # bytes each kernel must move.  Classic form: the emit reads w and writes
     # index + weight of every rank, the merge pack reads them back and writes

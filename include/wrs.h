@@ -30,6 +30,8 @@ typedef enum {
     WRS_ENOMEM = 2,  /* device allocation failed */
     WRS_EHIP = 3,    /* a HIP runtime call or launch failed; see wrs_last_error() */
     WRS_ENOTSUP = 4, /* no gfx950 device / unsupported configuration */
+    WRS_EWAIT = 5,   /* a bounded device-side wait of a build gave up: its outputs are
+                        invalid (reported by wrs_plan_status / wrs_batch_plan_status) */
 } wrs_status;
 
 /* Arithmetic of a build.  The reference GPU kernels are plain FP32 throughout
@@ -67,17 +69,21 @@ typedef struct {
  * benchmarks sweep (src/bench/psa_split2.cpp:34-270). */
 typedef struct {
     uint32_t merge_tile;    /* split granularity: buckets per merge-pack block,
-                               1024, 2048 (default) or 4096 */
+                               1024 (default), 2048 or 4096 */
     uint32_t merge_per;     /* buckets per merge-pack thread: 2, 4 (default) or 8
                                (merge_tile 4096: 4 or 8) */
     uint32_t partition;     /* prefix-partition form: 0 reduce-then-scan (default), 1 single
                                pass with decoupled look-back (per segment in a batch) */
-    uint32_t direct;        /* pack form: 0 (default) the partition writes every index and
-                               weight in partition order, 1 it writes the heavies and every
-                               64th light's position only and the pack reads the lights from
-                               the weights (12N fewer bytes; same table).  Debug builds use
-                               0; batch plans build 1 with partition 0 only. */
+    uint32_t direct;        /* pack form, WRS_PACK_*: DIRECT (the default) the partition
+                               writes the heavies and every 64th light's position only and
+                               the pack reads the lights from the weights; CLASSIC it writes
+                               every index and weight in partition order (12N more bytes;
+                               same table).  Debug builds use CLASSIC; batch plans build
+                               DIRECT with partition 0 only. */
 } wrs_tuning;
+#define WRS_PACK_DEFAULT 0u /* the direct form */
+#define WRS_PACK_DIRECT 1u
+#define WRS_PACK_CLASSIC 2u
 
 /* Build configuration.  Reference: PSAConfig (src/device/wrs/alias/psa/PSA.hpp:19-41),
  * whose splitSize sets the split granularity. */
@@ -291,6 +297,25 @@ wrs_status wrs_autotune_batched(const wrs_config* cfg, const uint64_t* offsets_h
                                 uint32_t nseg, const float* weights, uint32_t flags,
                                 wrs_tuning* best, wrs_tune_entry* log, uint32_t log_cap,
                                 uint32_t* log_len, wrs_stream_t stream);
+
+/* ---- status of the device-side waits ----
+ * A build waits on the device in two places: the last block of the total
+ * reads the other blocks' totals (stores already issued), and the look-back
+ * scans read their predecessors' records.  Each wait is bounded (about a
+ * second of polling); when one gives up, the build records the failure in the
+ * plan, its remaining kernels return at once (no table or CDF is written) and
+ * the next build starts clean.  The reference bounds its mean look-back the
+ * same way (MAX_IT, src/device/mean/decoupled/float.comp:143,194-197) but
+ * reports nothing.
+ * wrs_plan_status synchronises `stream` and returns WRS_EWAIT if the plan's
+ * most recent build (or scan) failed so, WRS_OK otherwise. */
+wrs_status wrs_plan_status(wrs_plan* plan, wrs_stream_t stream);
+wrs_status wrs_batch_plan_status(wrs_batch_plan* plan, wrs_stream_t stream);
+/* Failure injection for tests of the error path: wait_polls bounds every wait
+ * of the plan's builds (0: the default), hooks (bits) withhold a publication
+ * so that a wait must give up: 1 the first block's total, 2 the first record
+ * of the tile-aggregate scan.  Not for production use. */
+wrs_status wrs_plan_set_test_hooks(wrs_plan* plan, uint32_t wait_polls, uint32_t hooks);
 
 /* ---- introspection ---- */
 const char* wrs_last_error(void);         /* thread-local message of the last failure */

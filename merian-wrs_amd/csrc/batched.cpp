@@ -165,7 +165,7 @@ void BatchPlan::destroy() {
 // partition; with the single-pass partition the classic form is built (the
 // same tables either way).
 void BatchPlan::build(const float* w, wrsk::AliasEntry* tables, hipStream_t s) const {
-    const bool direct = tuning.direct && tuning.partition == 0;
+    const bool direct = tuning.direct != WRS_PACK_CLASSIC && tuning.partition == 0;
     wrs::check_hip(wrsk::launch_seg_reduce(w, item_d, items, prec, partials,
                                            states, s),
                    "segmented reduce launch");

@@ -34,8 +34,24 @@ struct Params {
     int32_t merge_qs;        // merge keys: x -> rint(x * 2^merge_qs), 61 - ilogb(total)
     double inv_mean;         // 1 / mean (FP64 division)
     int64_t mean_q;          // rint(mean * 2^merge_qs)
+    uint32_t error;          // kErr* bits: a bounded wait of this build gave up
+    uint32_t wait_polls;     // polls of a bounded wait before it gives up
+    uint32_t hooks;          // kHook* failure injection (tests)
+    uint32_t pad;
 };
-static_assert(sizeof(Params) == 80, "Params layout");
+static_assert(sizeof(Params) == 96, "Params layout");
+
+// Params::error bits.  A build whose error is not zero has invalid outputs;
+// the kernels after the failing one return at once (no table is written).
+constexpr uint32_t kErrReduceWait = 1u;    // the total's last block waited in vain for a partial
+constexpr uint32_t kErrLookbackWait = 2u;  // a look-back waited in vain for a predecessor
+// default bound of the waits: about a second of polling, far beyond any
+// legitimate wait (a predecessor's record or an issued store)
+constexpr uint32_t kDefaultWaitPolls = 1u << 22;
+// failure injection for tests of the error path (the plan's test hooks)
+constexpr uint32_t kHookSkipPartial = 1u;  // reduce block 0 does not publish its partial
+constexpr uint32_t kHookSkipSpine = 2u;    // the tile-scan spine's block 0 publishes nothing
+
 
 struct AliasEntry {
     float p;
@@ -75,13 +91,19 @@ constexpr int kItems = 16;                  // consecutive elements per thread
 constexpr int kTile = kBlock * kItems;      // 4096 elements per tile
 constexpr int kStateWords = 16;             // 128-byte look-back record per tile
 constexpr uint32_t kMaxPartials = 2048;     // reduce blocks
+// The reduce's words (wrs::Workspace::red): kMaxPartials block totals (the
+// sentinel ~0 between launches), then kRedCounters 32-bit arrival counters
+// (zero between launches).
+constexpr uint32_t kRedGroup = 32;  // blocks per first-level arrival counter
+constexpr uint32_t kRedCounters = kMaxPartials / kRedGroup + 1;
+constexpr uint32_t kRedWords = kMaxPartials + (kRedCounters + 1) / 2;
 constexpr int kSplitBlock = 256;            // debug split kernel block size
 constexpr uint32_t kSample = 64;            // partition prefix sample stride (ranks)
 // Merge-pack geometry: `tile` buckets per block (a multiple of kSample),
 // `per` buckets per thread (tile / per threads).  Supported shapes:
 // tile 1024 or 2048 with per 2, 4 or 8; tile 4096 with per 4 or 8.
 struct MergeShape {
-    uint32_t tile = 2048;
+    uint32_t tile = 1024;
     uint32_t per = 4;
 };
 constexpr uint32_t kMinMergeTile = 1024;
@@ -119,7 +141,8 @@ inline uint32_t partials_for(uint64_t n) {
 // restored by every launch (wrs::Workspace allocates them so).
 hipError_t launch_reduce_final(const float* w, uint64_t n, int prec, uint64_t* red, uint32_t G,
                                uint64_t* states, uint64_t state_words, int has_pivot,
-                               double pivot, Params* prm, hipStream_t s);
+                               double pivot, Params* prm, hipStream_t s,
+                               uint32_t wait_polls = kDefaultWaitPolls, uint32_t hooks = 0);
 // idx is always written; pw (the weights in partition order), hi/lo (full
 // prefix) and the prefix samples only when non-null.  ev_agg / ev_scan, when
 // non-null, are recorded after the tile-aggregate and tile-scan kernels.

@@ -40,13 +40,24 @@ __device__ __forceinline__ void publish_inclusive(uint64_t* rec, const P& inc) {
 }
 
 // Exclusive prefix of tile `tile` (> 0); executed by one whole wave, the
-// result is identical in every lane.
+// result is identical in every lane.  After `polls` consecutive polls without
+// progress it gives up: it sets kErrLookbackWait in *err and returns what it
+// has (the build's outputs are then invalid and its later kernels return at
+// once).  A predecessor publishes within microseconds of starting, so the
+// bound (kDefaultWaitPolls, about a second) is only reached if one never
+// starts or never publishes.
 template <class P, class Ops>
-__device__ __forceinline__ P lookback(const uint64_t* states, uint32_t tile) {
+__device__ __forceinline__ P lookback(const uint64_t* states, uint32_t tile, uint32_t polls,
+                                      uint32_t* err) {
     const int lane = lane_id();
     P excl = Ops::zero();
     int64_t look = (int64_t)tile - 1;
+    uint32_t idle = 0;
     for (;;) {
+        if (idle > polls) {
+            if (lane == 0) atomicOr(err, wrsk::kErrLookbackWait);
+            return excl;
+        }
         const int64_t tt = look - lane;
         const uint64_t stw = tt < 0 ? kStInc : ld_word(states + (uint64_t)tt * wrsk::kStateWords + kStatusWord);
         const int st = stw == kStInc ? 2 : (stw == kStAgg ? 1 : 0);
@@ -58,6 +69,7 @@ __device__ __forceinline__ P lookback(const uint64_t* states, uint32_t tile) {
         const int cnt = finish ? fi + 1 : fn;
         if (cnt == 0) {
             __builtin_amdgcn_s_sleep(2);
+            ++idle;
             continue;
         }
         P pv = Ops::zero();
@@ -66,11 +78,13 @@ __device__ __forceinline__ P lookback(const uint64_t* states, uint32_t tile) {
             ok = Ops::read(states + (uint64_t)tt * wrsk::kStateWords + (st == 2 ? 8 : 0), pv);
         if (!__all(ok)) {  // a payload is not visible yet
             __builtin_amdgcn_s_sleep(1);
+            ++idle;
             continue;
         }
         excl = Ops::add(excl, Ops::wave_sum(lane < cnt ? pv : Ops::zero()));
         if (finish) return excl;
         look -= cnt;
+        idle = 0;
     }
 }
 

@@ -319,6 +319,7 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
                                                 const uint32_t* __restrict__ idx, LightSrc ls,
                                                 uint32_t run) {
     using T = typename RealT<PREC>::T;
+    if (prm->error) return;
     const uint32_t b0 = (blockIdx.x * 8u + threadIdx.x / 32u) * run;  // this half-wave's first
     uint32_t jb_lo = 0u, jb_hi = 0xffffffffu;
     for (uint32_t b = b0; b < b0 + run && b < nbound; ++b) {  // uniform per half-wave
@@ -763,6 +764,7 @@ __global__ __launch_bounds__(BLOCK) void k_mergepack(const float* __restrict__ p
                                                      const Coarse* __restrict__ bounds,
                                                      AliasEntry* __restrict__ table,
                                                      LightSrc ls) {
+    if (prm->error) return;  // an earlier wait of the build gave up: no table
     const uint32_t b = blockIdx.x;
     const uint32_t n0 = b * (uint32_t)TILE;  // < N
     const uint32_t n1 = (uint32_t)min((uint64_t)n0 + TILE, (uint64_t)N);
@@ -978,7 +980,7 @@ __device__ __forceinline__ uint32_t block_incl_max(uint32_t (&v)[kRepairLbPer], 
 // Exclusive max over the blocks before blk (records `recs`), executed by wave 0;
 // result broadcast through *s_ex.
 __device__ __forceinline__ void repair_lookback(uint64_t* recs, uint32_t blk, uint32_t agg,
-                                                uint32_t* s_ex) {
+                                                uint32_t* s_ex, Params* prm) {
     if (threadIdx.x >= (unsigned)kWave) return;
     const int lane = lane_id();
     uint64_t* rec = recs + (uint64_t)blk * kStateWords;
@@ -987,7 +989,7 @@ __device__ __forceinline__ void repair_lookback(uint64_t* recs, uint32_t blk, ui
         if (lane == 0) publish_inclusive<uint32_t, MaxOps>(rec, agg);
     } else {
         if (lane == 0) publish_aggregate<uint32_t, MaxOps>(rec, agg);
-        ex = lookback<uint32_t, MaxOps>(recs, blk);
+        ex = lookback<uint32_t, MaxOps>(recs, blk, prm->wait_polls, &prm->error);
         if (lane == 0) publish_inclusive<uint32_t, MaxOps>(rec, MaxOps::add(ex, agg));
     }
     if (lane == 0) *s_ex = ex;
@@ -1001,6 +1003,7 @@ __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
     LightSrc ls) {
     __shared__ uint32_t s_w[kRepairLbThreads / kWave];
     __shared__ uint32_t s_blk, s_ex;
+    if (prm->error) return;
     if (threadIdx.x == 0) s_blk = atomicAdd(&prm->repair_ticket, 1u);  // predecessors first
     __syncthreads();
     const uint32_t blk = s_blk;
@@ -1012,7 +1015,7 @@ __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
 #pragma unroll
     for (int k = 0; k < kRepairLbPer; ++k) raw_j[k] = v[k] = base + k < M ? bounds[base + k].j : 0u;
     uint32_t agg = block_incl_max(v, s_w);
-    repair_lookback(recs, blk, agg, &s_ex);
+    repair_lookback(recs, blk, agg, &s_ex, prm);
     __syncthreads();
     uint32_t ex = s_ex;
     // pass 1: i'' = running max of m_b - j'_b
@@ -1022,7 +1025,7 @@ __global__ __launch_bounds__(kRepairLbThreads) void k_repair_bounds_lb(
         v[k] = base + k < M ? (base + k) * tile - jp : 0u;
     }
     agg = block_incl_max(v, s_w);
-    repair_lookback(recs + (uint64_t)nrb * kStateWords, blk, agg, &s_ex);
+    repair_lookback(recs + (uint64_t)nrb * kStateWords, blk, agg, &s_ex, prm);
     __syncthreads();
     ex = s_ex;
 #pragma unroll

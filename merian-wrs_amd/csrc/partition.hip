@@ -602,6 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_part_agg(const float* __restrict__ w
                                                      const Params* __restrict__ prm,
                                                      uint64_t* __restrict__ states) {
     __shared__ TileShared<PREC> s;
+    if (prm->error) return;  // an earlier wait of the build gave up
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
@@ -631,6 +632,7 @@ __global__ __launch_bounds__(kSpineThreads) void k_part_tilescan(uint64_t* __res
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
+    if (prm->error) return;
     if (t == 0) s_blk = atomicAdd(&prm->ticket, 1u);  // predecessors are scheduled first
     __syncthreads();
     const uint32_t blk = s_blk;
@@ -664,10 +666,11 @@ __global__ __launch_bounds__(kSpineThreads) void k_part_tilescan(uint64_t* __res
         uint64_t* rec = recs + (uint64_t)blk * kStateWords;
         Carry ex = carry_zero();
         if (blk == 0) {
-            if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, blk_agg);
+            if (lane == 0 && !(prm->hooks & kHookSkipSpine))
+                publish_inclusive<Carry, CarryOps<PREC>>(rec, blk_agg);
         } else {
             if (lane == 0) publish_aggregate<Carry, CarryOps<PREC>>(rec, blk_agg);
-            ex = lookback<Carry, CarryOps<PREC>>(recs, blk);
+            ex = lookback<Carry, CarryOps<PREC>>(recs, blk, prm->wait_polls, &prm->error);
             if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, carry_add<PREC>(ex, blk_agg));
         }
         if (lane == 0) {
@@ -705,6 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_part_emit(const float* __restrict__ 
                                                       double* __restrict__ samp_l,
                                                       uint32_t* __restrict__ lpos) {
     __shared__ TileShared<PREC> s;
+    if (prm->error) return;
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
@@ -736,6 +740,7 @@ __global__ __launch_bounds__(kBlock) void k_part_lookback(
     __shared__ TileShared<PREC> s;
     __shared__ uint32_t s_tile;
     __shared__ Carry s_excl;
+    if (prm->error) return;
     if (threadIdx.x == 0) s_tile = atomicAdd(&prm->ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -756,7 +761,7 @@ __global__ __launch_bounds__(kBlock) void k_part_lookback(
             if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, agg);
         } else {
             if (lane == 0) publish_aggregate<Carry, CarryOps<PREC>>(rec, agg);
-            excl = lookback<Carry, CarryOps<PREC>>(states, tile);
+            excl = lookback<Carry, CarryOps<PREC>>(states, tile, prm->wait_polls, &prm->error);
             if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, carry_add<PREC>(excl, agg));
         }
         if (lane == 0) {
@@ -821,7 +826,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_partition(
             if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, agg);
         } else {
             if (lane == 0) publish_aggregate<Carry, CarryOps<PREC>>(rec, agg);
-            excl = lookback<Carry, CarryOps<PREC>>(seg_states, tile);
+            excl = lookback<Carry, CarryOps<PREC>>(seg_states, tile, prm->wait_polls, &prm->error);
             if (lane == 0) publish_inclusive<Carry, CarryOps<PREC>>(rec, carry_add<PREC>(excl, agg));
         }
         if (lane == 0) {

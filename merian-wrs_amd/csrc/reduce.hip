@@ -79,7 +79,10 @@ __device__ __forceinline__ AccT<FP64> block_sum(AccT<FP64> acc) {
 // i = t, t + kBlock, ... (in that order); thread 0 writes *prm.
 template <bool FP64>
 __device__ __forceinline__ void finalize_acc(AccT<FP64> acc, uint64_t n, int has_pivot,
-                                             double pivot, Params* __restrict__ prm) {
+                                             double pivot, Params* __restrict__ prm,
+                                             uint32_t error = 0u,
+                                             uint32_t wait_polls = kDefaultWaitPolls,
+                                             uint32_t hooks = 0u) {
     using Acc = AccT<FP64>;
     const int t = threadIdx.x;
     const Acc total = block_sum<FP64>(acc);
@@ -113,6 +116,10 @@ __device__ __forceinline__ void finalize_acc(AccT<FP64> acc, uint64_t n, int has
         p.merge_qs = p.total > 0.0 ? 61 - ilogb(p.total) : 0;
         p.mean_q = p.total > 0.0 ? merge_quant(p.mean, p.merge_qs) : 0;
         p.inv_mean = p.mean > 0.0 ? 1.0 / p.mean : 0.0;
+        p.error = error;
+        p.wait_polls = wait_polls;
+        p.hooks = hooks;
+        p.pad = 0;
         *prm = p;
     }
 }
@@ -127,27 +134,30 @@ __device__ __forceinline__ void finalize_block(const double* __restrict__ partia
     finalize_acc<FP64>(acc, n, has_pivot, pivot, prm);
 }
 
-// Polls of a block total word before block 0 goes on without it: every other
-// block stores its word when its tiles are done, so the wait ends once that
-// store is visible; the bound only keeps a kernel from spinning on.
-constexpr uint32_t kPartialSpin = 1u << 20;
-
 }  // namespace
 
-// Block totals, then the build scalars by block 0.  red holds kMaxPartials
-// total words, the sentinel ~0 between launches (a total is a non-negative
-// double or a NaN from a float, never all ones).  Every block stores its total
-// with an agent-coherent store; block 0, after its own tiles, polls the words
-// until none is the sentinel (it waits for no other block to start: the others
-// depend on nothing), sums them in the fixed order of the segmented finalize
-// (same bits), writes *prm and restores the sentinels for the next launch.  No
-// counter: one atomic per block on a single word costs more than the wait.
+// Block totals, then the build scalars by the last block to finish.  red holds
+// kMaxPartials total words, the sentinel ~0 between launches (a total is a
+// non-negative double or a NaN from a float, never all ones), and the arrival
+// counters, zero between launches.  Every block stores its total with an
+// agent-coherent store and then counts itself in its group's counter (groups
+// of kRedGroup blocks); the last of a group counts the group in the top
+// counter, and the last group's last block -- which therefore runs after every
+// other block has issued its total -- reads the totals, sums them in the
+// fixed order of the segmented finalize (same bits), writes *prm, and restores
+// the sentinels and zero counters for the next launch.  No block ever waits
+// for another to make progress: the reader only waits until stores already
+// issued become visible, a wait bounded by `wait_polls` polls per word whose
+// expiry (a machine fault, or a hook of the tests) sets kErrReduceWait.  The
+// split counters keep the same-address atomics to kRedGroup per word.
 template <bool FP64>
 __global__ __launch_bounds__(kBlock) void k_reduce_final(const float* __restrict__ w, uint64_t n,
                                                          uint64_t* __restrict__ red,
                                                          uint64_t* __restrict__ states,
                                                          uint64_t state_words, int has_pivot,
-                                                         double pivot, Params* __restrict__ prm) {
+                                                         double pivot, Params* __restrict__ prm,
+                                                         uint32_t wait_polls, uint32_t hooks) {
+    __shared__ uint32_t s_last;
     const int t = threadIdx.x;
     const uint32_t G = gridDim.x;
     const uint64_t tiles = (n + kTile - 1) / kTile;
@@ -157,9 +167,20 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const float* __restrict
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + t; i < state_words; i += (uint64_t)G * kBlock)
         states[i] = kSentinel;
     const AccT<FP64> b = block_sum<FP64>(acc);
-    if (t == 0) st_word(red + blockIdx.x, (uint64_t)__double_as_longlong((double)b));
-    if (blockIdx.x != 0) return;
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(red + kMaxPartials);
+    if (t == 0) {
+        if (!(blockIdx.x == 0 && (hooks & kHookSkipPartial)))
+            st_word(red + blockIdx.x, (uint64_t)__double_as_longlong((double)b));
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the store is issued before the count
+        const uint32_t g = blockIdx.x / kRedGroup;
+        const uint32_t gsize = min(kRedGroup, G - g * kRedGroup);
+        uint32_t last = 0;
+        if (atomicAdd(&cnt[g], 1u) == gsize - 1u)
+            last = atomicAdd(&cnt[kRedCounters - 1], 1u) == (G + kRedGroup - 1) / kRedGroup - 1u;
+        s_last = last;
+    }
     __syncthreads();
+    if (!s_last) return;
     // this thread's words i = t + kBlock k, all loads in flight at once
     constexpr int kPer = (int)(kMaxPartials / kBlock);
     uint64_t x[kPer];
@@ -168,26 +189,31 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const float* __restrict
         const uint32_t i = (uint32_t)t + (uint32_t)kBlock * k;
         x[k] = i < G ? ld_word(red + i) : 0ull;
     }
+    bool missing = false;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t i = (uint32_t)t + (uint32_t)kBlock * k;
-        for (uint32_t spin = 0; i < G && x[k] == kSentinel && spin < kPartialSpin; ++spin) {
+        for (uint32_t spin = 0; i < G && x[k] == kSentinel && spin < wait_polls; ++spin) {
             __builtin_amdgcn_s_sleep(1);
             x[k] = ld_word(red + i);
         }
+        missing |= i < G && x[k] == kSentinel;
     }
+    const bool failed = __syncthreads_or(missing) != 0;
     AccT<FP64> sum = 0;  // the order of finalize_block
 #pragma unroll
     for (int k = 0; k < kPer; ++k)
         if ((uint32_t)t + (uint32_t)kBlock * k < G)
             sum = sum + (AccT<FP64>)__longlong_as_double((long long)x[k]);
-    finalize_acc<FP64>(sum, n, has_pivot, pivot, prm);
+    finalize_acc<FP64>(sum, n, has_pivot, pivot, prm, failed ? kErrReduceWait : 0u, wait_polls,
+                       hooks);
     // every thread restores only the words it read itself
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t i = (uint32_t)t + (uint32_t)kBlock * k;
         if (i < G) st_word(red + i, kSentinel);
     }
+    if (t < (int)kRedCounters) cnt[t] = 0u;
 }
 
 // Segmented: one block per (segment, tile) item; every segment has at most
@@ -228,14 +254,15 @@ __global__ void k_fill_u64(uint64_t* __restrict__ p, uint64_t words, uint64_t va
 
 hipError_t launch_reduce_final(const float* w, uint64_t n, int prec, uint64_t* red, uint32_t G,
                                uint64_t* states, uint64_t state_words, int has_pivot,
-                               double pivot, Params* prm, hipStream_t s) {
+                               double pivot, Params* prm, hipStream_t s, uint32_t wait_polls,
+                               uint32_t hooks) {
     if (G == 0 || G > kMaxPartials) return hipErrorInvalidValue;
     if (prec != kFP32)
         k_reduce_final<true><<<G, kBlock, 0, s>>>(w, n, red, states, state_words, has_pivot,
-                                                  pivot, prm);
+                                                  pivot, prm, wait_polls, hooks);
     else
         k_reduce_final<false><<<G, kBlock, 0, s>>>(w, n, red, states, state_words, has_pivot,
-                                                   pivot, prm);
+                                                   pivot, prm, wait_polls, hooks);
     return hipGetLastError();
 }
 

@@ -208,6 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int wid = t / kWave;
+    if (prm->error) return;  // the total's wait gave up
     if (t == 0) s_tile = atomicAdd(&prm->ticket, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(const float* __restrict__ w, ui
             if (lane == 0) publish_inclusive<FX, ScanOps<PREC>>(rec, agg);
         } else {
             if (lane == 0) publish_aggregate<FX, ScanOps<PREC>>(rec, agg);
-            excl = lookback<FX, ScanOps<PREC>>(states, tile);
+            excl = lookback<FX, ScanOps<PREC>>(states, tile, prm->wait_polls, &prm->error);
             if (lane == 0) publish_inclusive<FX, ScanOps<PREC>>(rec, sc_add<PREC>(excl, agg));
         }
         if (lane == 0) {
@@ -387,7 +388,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_tilesum(const float* __restrict
     uint64_t* recs = states + (uint64_t)tiles * kStateWords;
     for (uint32_t i = tile * kBlock + threadIdx.x; i < rec_words; i += tiles * kBlock)
         recs[i] = kSentinel;
-    if (tile == 0 && threadIdx.x == 0) prm->ticket = 0;
+    if (tile == 0 && threadIdx.x == 0) {  // a fresh build for the kernels that follow
+        prm->ticket = 0;
+        prm->error = 0;
+        prm->wait_polls = kDefaultWaitPolls;
+        prm->hooks = 0;
+    }
     float v[kItems];
     R thr_excl, tile_agg;
     scan_tile<PREC>(w, base, m, s, v, thr_excl, tile_agg);
@@ -449,7 +455,7 @@ __global__ __launch_bounds__(kSpineThreads) void k_scan_spine(uint64_t* __restri
             if (lane == 0) publish_inclusive<FX5, SpineOps<PREC>>(rec, blk_agg);
         } else {
             if (lane == 0) publish_aggregate<FX5, SpineOps<PREC>>(rec, blk_agg);
-            ex = lookback<FX5, SpineOps<PREC>>(recs, blk);
+            ex = lookback<FX5, SpineOps<PREC>>(recs, blk, prm->wait_polls, &prm->error);
             if (lane == 0) publish_inclusive<FX5, SpineOps<PREC>>(rec, sp_add<PREC>(ex, blk_agg));
         }
         if (lane == 0) {
@@ -495,6 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_emit(const float* __restrict__ 
                                                       float* __restrict__ lo_out) {
     __shared__ ScanShared<PREC> s;
     using R = typename Acc1<PREC>::T;
+    if (prm->error) return;  // the spine's look-back gave up
     const uint32_t tile = blockIdx.x;
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);  // n + kTile - 1 may exceed 2^32
     const uint32_t base = tile * (uint32_t)kTile;

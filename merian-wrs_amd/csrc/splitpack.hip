@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_splits(uint32_t N, uint32_t K,
                                                          SplitRec* __restrict__ splits) {
     using T = typename Real<PREC>::T;
     const uint64_t k = (uint64_t)blockIdx.x * kSplitBlock + threadIdx.x + 1u;
-    if (k > K) return;
+    if (k > K || prm->error) return;
     const SplitV<PREC> sp = split_point<PREC>(hi, lo, N, K, prm->heavy_count, (T)prm->mean, k);
     splits[k - 1] = SplitRec{sp.i, sp.j, (double)sp.spill};
 }

@@ -45,8 +45,8 @@ constexpr int kReps = 9;
 constexpr int kWarm = 20;
 
 // the merge shapes tried, default first
-const wrsk::MergeShape kShapes[] = {{2048, 4}, {1024, 2}, {1024, 4}, {1024, 8},
-                                    {2048, 2}, {2048, 8}, {4096, 4}, {4096, 8}};
+const wrsk::MergeShape kShapes[] = {{1024, 4}, {1024, 2}, {1024, 8}, {2048, 2},
+                                    {2048, 4}, {2048, 8}, {4096, 4}, {4096, 8}};
 
 std::mutex g_mu;
 std::map<std::string, wrs_tuning> g_cache;
@@ -70,7 +70,7 @@ void load_file_locked() {
         std::istringstream vals(line.substr(tab + 1));
         wrs_tuning t{};
         if (vals >> t.merge_tile >> t.merge_per >> t.partition) {
-            if (!(vals >> t.direct)) t.direct = 0;
+            if (!(vals >> t.direct)) continue;
             try {
                 wrs::require_tuning(t);
             } catch (...) {
@@ -108,7 +108,9 @@ std::string device_key() {
     hipDeviceProp_t prop;
     wrs::check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     std::ostringstream k;
-    k << prop.gcnArchName << "/cu" << prop.multiProcessorCount;
+    // "t2": the tuning encoding (WRS_PACK_* pack forms); lines of other
+    // encodings in a cache file do not match
+    k << "t2/" << prop.gcnArchName << "/cu" << prop.multiProcessorCount;
     return k.str();
 }
 
@@ -164,7 +166,8 @@ wrs_tuning with_shape(wrs_tuning t, const wrsk::MergeShape& ms) {
 // better forms.
 template <class M>
 wrs_tuning descend(M&& measure, bool direct_rts_only) {
-    const wrs_tuning t0 = with_shape(wrs_tuning{}, kShapes[0]);
+    wrs_tuning t0 = with_shape(wrs_tuning{}, kShapes[0]);
+    t0.direct = WRS_PACK_CLASSIC;
     wrs_tuning best = t0;
     float best_ms = measure(t0, kWarm);
     wrs_tuning t1 = t0;
@@ -176,7 +179,7 @@ wrs_tuning descend(M&& measure, bool direct_rts_only) {
     }
     {
         wrs_tuning t2 = best;
-        t2.direct = 1;
+        t2.direct = WRS_PACK_DIRECT;
         if (direct_rts_only) t2.partition = 0;
         const float ms2 = measure(t2);
         if (ms2 < best_ms) {

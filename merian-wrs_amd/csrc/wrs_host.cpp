@@ -7,6 +7,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/wrs.h"
 #include "../../include/wrs_runtime.h"
@@ -43,9 +44,13 @@ Workspace Workspace::allocate(uint64_t max_n, bool partition_buffers) {
     ws.max_n = max_n;
     try {
         ws.params = dev_alloc<wrsk::Params>(1);
-        ws.red = dev_alloc<uint64_t>(wrsk::kMaxPartials);
-        // block totals at the sentinel (each launch of the reduce restores it)
+        ws.red = dev_alloc<uint64_t>(wrsk::kRedWords);
+        // block totals at the sentinel, arrival counters at zero (each launch
+        // of the reduce restores both)
         check_hip(hipMemset(ws.red, 0xff, wrsk::kMaxPartials * sizeof(uint64_t)), "hipMemset");
+        check_hip(hipMemset(ws.red + wrsk::kMaxPartials, 0,
+                            (wrsk::kRedWords - wrsk::kMaxPartials) * sizeof(uint64_t)),
+                  "hipMemset");
         check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
         ws.state_words = wrsk::state_words_for(max_n);
         ws.states = dev_alloc<uint64_t>(ws.state_words);
@@ -102,7 +107,7 @@ void require_tuning(const wrs_tuning& t) {
     require(wrsk::merge_shape_supported(merge_shape_of(t)),
             "unsupported merge shape (merge_tile 1024/2048 with merge_per 2/4/8, or 4096 with 4/8)");
     require(t.partition <= 1u, "partition must be 0 or 1");
-    require(t.direct <= 1u, "direct must be 0 or 1");
+    require(t.direct <= WRS_PACK_CLASSIC, "direct must be a WRS_PACK_* value");
 }
 
 void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, double pivot) const {
@@ -112,7 +117,8 @@ void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, d
     const uint64_t words = wrsk::state_words_for(N);
     const bool has_pivot = !std::isnan(pivot);
     check_hip(wrsk::launch_reduce_final(weights, N, (int)m_prec, ws.red, G, ws.states, words,
-                                        has_pivot ? 1 : 0, has_pivot ? pivot : 0.0, ws.params, s),
+                                        has_pivot ? 1 : 0, has_pivot ? pivot : 0.0, ws.params, s,
+                                        ws.wait_polls, ws.hooks),
               "reduce launch");
 }
 
@@ -159,12 +165,12 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     // the direct form (partition arrays hold the heavies only) for plain builds;
     // a debug build reads the full partition arrays
     wrsk::LightSrc ls;
-    if (m_cfg.tuning.direct && !dbg) ls = wrsk::LightSrc{b.weights, ws.lightpos};
+    if (m_cfg.tuning.direct != WRS_PACK_CLASSIC && !dbg) ls = wrsk::LightSrc{b.weights, ws.lightpos};
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::state_words_for(N);
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
     check_hip(wrsk::launch_reduce_final(b.weights, N, (int)prec, ws.red, G, ws.states, words, 0,
-                                        0.0, ws.params, s),
+                                        0.0, ws.params, s, ws.wait_polls, ws.hooks),
               "reduce launch");
     // the finalize runs inside the reduce (its last block): an empty phase
     if (ev) check_hip(hipEventRecord(ev[1], s), "event");
@@ -652,6 +658,55 @@ wrs_status wrs_plan_phase_times(wrs_plan* plan, float* ms, uint32_t count) {
             wrs::check_hip(hipEventElapsedTime(&ms[k], plan->ev[k], plan->ev[k + 1]),
                            "event elapsed");
     });
+}
+
+wrs_status wrs_plan_status(wrs_plan* plan, wrs_stream_t stream) {
+    wrs_status st = WRS_OK;
+    const wrs_status call = guarded([&] {
+        require_plan(plan);
+        wrs::check_hip(hipStreamSynchronize(hs(stream)), "stream sync");
+        uint32_t err = 0;
+        wrs::check_hip(hipMemcpy(&err, &plan->ws.params->error, sizeof(err),
+                                 hipMemcpyDeviceToHost),
+                       "copy error word");
+        if (err) {
+            st = WRS_EWAIT;
+            g_last_error = std::string("a bounded wait of the last build gave up (") +
+                           ((err & wrsk::kErrReduceWait) ? "total" : "") +
+                           ((err & wrsk::kErrReduceWait) && (err & wrsk::kErrLookbackWait) ? ", " : "") +
+                           ((err & wrsk::kErrLookbackWait) ? "look-back" : "") +
+                           "): its outputs are invalid";
+        }
+    });
+    return call != WRS_OK ? call : st;
+}
+
+wrs_status wrs_plan_set_test_hooks(wrs_plan* plan, uint32_t wait_polls, uint32_t hooks) {
+    return guarded([&] {
+        require_plan(plan);
+        plan->ws.wait_polls = wait_polls ? wait_polls : wrsk::kDefaultWaitPolls;
+        plan->ws.hooks = hooks;
+    });
+}
+
+wrs_status wrs_batch_plan_status(wrs_batch_plan* plan, wrs_stream_t stream) {
+    wrs_status st = WRS_OK;
+    const wrs_status call = guarded([&] {
+        if (!plan) throw wrs::Error(WRS_EINVAL, "plan is null");
+        wrs::check_hip(hipStreamSynchronize(hs(stream)), "stream sync");
+        std::vector<wrsk::Params> prm(plan->impl.nseg);
+        wrs::check_hip(hipMemcpy(prm.data(), plan->impl.seg_prm,
+                                 prm.size() * sizeof(wrsk::Params), hipMemcpyDeviceToHost),
+                       "copy segment parameters");
+        for (uint32_t g = 0; g < plan->impl.nseg; ++g)
+            if (prm[g].error) {
+                st = WRS_EWAIT;
+                g_last_error = "a look-back wait of segment " + std::to_string(g) +
+                               " gave up: the batch's outputs are invalid";
+                break;
+            }
+    });
+    return call != WRS_OK ? call : st;
 }
 
 // ------------------------------------------------------------- runtime ----

@@ -43,7 +43,10 @@ void set_last_error(const std::string& msg);
 struct Workspace {
     uint64_t max_n = 0;
     wrsk::Params* params = nullptr;    // [1]
-    uint64_t* red = nullptr;           // [kMaxPartials] reduce block totals (~0 between builds)
+    uint64_t* red = nullptr;           // [kRedWords] reduce block totals (~0 between builds)
+                                       // and arrival counters (0 between builds)
+    uint32_t wait_polls = wrsk::kDefaultWaitPolls;  // bound of the builds' waits
+    uint32_t hooks = 0;                // failure injection of the error-path tests
     uint64_t* states = nullptr;        // [state_words_for(max_n)]
     uint64_t state_words = 0;
     uint32_t* indices = nullptr;       // [max_n] (PSA only)

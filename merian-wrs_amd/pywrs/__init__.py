@@ -16,9 +16,10 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libwrs_amd.so")
 
-WRS_OK, WRS_EINVAL, WRS_ENOMEM, WRS_EHIP, WRS_ENOTSUP = range(5)
+WRS_OK, WRS_EINVAL, WRS_ENOMEM, WRS_EHIP, WRS_ENOTSUP, WRS_EWAIT = range(6)
 PREC_FP32, PREC_FP64_CARRY, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2, 3
 GEN_UNIFORM01, GEN_LOGNORMAL, GEN_ZIPF, GEN_SPIKE = 0, 1, 2, 3
+PACK_DEFAULT, PACK_DIRECT, PACK_CLASSIC = 0, 1, 2  # wrs_tuning.direct
 
 ALIAS_DTYPE = np.dtype([("p", np.float32), ("a", np.uint32)])
 SPLIT_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float64)])
@@ -37,7 +38,8 @@ EXPORTS = [
     "wrs_rt_stream_create", "wrs_rt_stream_destroy", "wrs_rt_stream_sync", "wrs_rt_device_sync",
     "wrs_rt_event_create", "wrs_rt_event_destroy", "wrs_rt_event_record", "wrs_rt_event_elapsed",
     "wrs_rt_copy_f32", "wrs_rt_psa_build_sample_each", "wrs_rt_graph_begin", "wrs_rt_graph_end",
-    "wrs_rt_graph_launch", "wrs_rt_graph_destroy",
+    "wrs_rt_graph_launch", "wrs_rt_graph_destroy", "wrs_plan_status", "wrs_batch_plan_status",
+    "wrs_plan_set_test_hooks",
 ]
 
 
@@ -127,6 +129,9 @@ def lib():
         "wrs_sample_stats": [vp, vp, u64, vp, u64, f64, vp, vp],
         "wrs_plan_set_timing": [vp, i32],
         "wrs_plan_phase_times": [vp, vp, u32],
+        "wrs_plan_status": [vp, vp],
+        "wrs_batch_plan_status": [vp, vp],
+        "wrs_plan_set_test_hooks": [vp, u32, u32],
         "wrs_rt_device_count": [C.POINTER(C.c_int)],
         "wrs_rt_device_info": [C.c_char_p, C.c_size_t, C.c_char_p, C.c_size_t,
                                C.POINTER(C.c_int), C.POINTER(C.c_uint64)],
@@ -367,6 +372,15 @@ class Plan:
         check(lib().wrs_plan_phase_times(self.handle, ms, len(self.PHASES)))
         return list(ms)
 
+    def status(self, stream=None) -> None:
+        """Raises WrsError(WRS_EWAIT) if a device-side wait of the most recent
+        build on this plan gave up (its outputs are then invalid)."""
+        check(lib().wrs_plan_status(self.handle, _h(stream)))
+
+    def set_test_hooks(self, wait_polls: int = 0, hooks: int = 0) -> None:
+        """Failure injection for error-path tests (0, 0: normal builds)."""
+        check(lib().wrs_plan_set_test_hooks(self.handle, int(wait_polls), int(hooks)))
+
     def close(self) -> None:
         if getattr(self, "handle", None):
             lib().wrs_plan_destroy(self.handle)
@@ -394,6 +408,10 @@ class BatchPlan:
 
     def build(self, weights, tables, stream=None) -> None:
         check(lib().wrs_psa_build_batched(self.handle, _p(weights), _p(tables), _h(stream)))
+
+    def status(self, stream=None) -> None:
+        """Raises WrsError(WRS_EWAIT) if a device-side wait of the last batch gave up."""
+        check(lib().wrs_batch_plan_status(self.handle, _h(stream)))
 
     def close(self) -> None:
         if getattr(self, "handle", None):

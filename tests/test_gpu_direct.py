@@ -76,7 +76,7 @@ def test_direct_equals_classic(wrs, oracle, kind, n):
     case("direct vs classic", kind, n)
     w = weights(oracle, kind, n)
     dw = wrs.to_device(w)
-    t0 = build(wrs, dw, n, wrs.PREC_COMPENSATED, dict(direct=0))
+    t0 = build(wrs, dw, n, wrs.PREC_COMPENSATED, dict(direct=2))
     t1 = build(wrs, dw, n, wrs.PREC_COMPENSATED, dict(direct=1))
     assert same(t0, t1)
 
@@ -89,7 +89,7 @@ def test_direct_every_mode(wrs, oracle, kind, precision, partition):
     case("direct modes", kind, precision, partition, n)
     w = weights(oracle, kind, n)
     dw = wrs.to_device(w)
-    t0 = build(wrs, dw, n, precision, dict(partition=partition))
+    t0 = build(wrs, dw, n, precision, dict(partition=partition, direct=2))
     t1 = build(wrs, dw, n, precision, dict(partition=partition, direct=1))
     if precision == wrs.PREC_FP32 and partition == 1:
         # FP32 mode's look-back carries are float sums in look-back order, so
@@ -110,7 +110,7 @@ def test_direct_every_shape(wrs, oracle, kind, shape):
     w = weights(oracle, kind, n)
     dw = wrs.to_device(w)
     tune = dict(merge_tile=shape[0], merge_per=shape[1])
-    t0 = build(wrs, dw, n, wrs.PREC_COMPENSATED, tune)
+    t0 = build(wrs, dw, n, wrs.PREC_COMPENSATED, dict(tune, direct=2))
     t1 = build(wrs, dw, n, wrs.PREC_COMPENSATED, dict(tune, direct=1))
     assert same(t0, t1)
 
@@ -148,7 +148,7 @@ def test_direct_tuning_in_debug_and_batched_builds(wrs, oracle):
     case("direct ignored", n)
     w = oracle.gen_zipf(4, 1.1, n)
     dw = wrs.to_device(w)
-    ref = build(wrs, dw, n, wrs.PREC_COMPENSATED, dict(direct=0))
+    ref = build(wrs, dw, n, wrs.PREC_COMPENSATED, dict(direct=2))
     plan = wrs.Plan(n, tune=dict(direct=1))
     t = wrs.empty(n, wrs.ALIAS_DTYPE)
     idx = wrs.empty(n, np.uint32)
@@ -157,10 +157,10 @@ def test_direct_tuning_in_debug_and_batched_builds(wrs, oracle):
     assert same(t.numpy(), ref)
     off = np.array([0, 1000, 70000, n], np.uint64)
     tables = {}
-    for d in (0, 1):
+    for d in (2, 1):
         bp = wrs.BatchPlan(off, tune=dict(direct=d))
         tb = wrs.empty(n, wrs.ALIAS_DTYPE)
         bp.build(dw, tb)
         wrs.synchronize()
         tables[d] = tb.numpy()
-    assert same(tables[0], tables[1])
+    assert same(tables[2], tables[1])

@@ -96,7 +96,7 @@ def test_autotune_psa_measures_then_caches(wrs, oracle):
     # partition form at the default shape (2), the pack form (1), then the 7 other shapes
     assert len(log) == 2 + 1 + len(SHAPES) - 1
     assert {e["partition"] for e in log[:2]} == {0, 1}
-    assert [e["direct"] for e in log[:3]] == [0, 0, 1]
+    assert [e["direct"] for e in log[:3]] == [2, 2, 1]
     assert {(e["merge_tile"], e["merge_per"]) for e in log} == set(SHAPES)
     assert all(e["ms"] > 0 for e in log)
     assert (best["merge_tile"], best["merge_per"]) in SHAPES
@@ -125,7 +125,7 @@ def test_autotune_batched_measures_then_caches(wrs, oracle):
     # reduce-then-scan partition a batch builds it with), then the 7 other shapes
     assert len(log) == 2 + 1 + len(SHAPES) - 1
     assert {e["partition"] for e in log[:2]} == {0, 1}
-    assert [e["direct"] for e in log[:3]] == [0, 0, 1] and log[2]["partition"] == 0
+    assert [e["direct"] for e in log[:3]] == [2, 2, 1] and log[2]["partition"] == 0
     assert (best["merge_tile"], best["merge_per"]) in SHAPES
     best2, log2 = wrs.autotune_batched(off, dw)
     assert best2 == best and log2 == []
