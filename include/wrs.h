@@ -160,8 +160,15 @@ wrs_status wrs_its_build(wrs_plan* plan, const float* weights, uint64_t n, float
 wrs_status wrs_alias_sample(const wrs_alias_entry* table, uint64_t n, uint64_t s, uint64_t seed,
                             uint64_t counter_base, uint32_t* out, wrs_stream_t stream);
 
-/* ITS sampling by lower-bound binary search over the CDF (cdf_lo optional).
- * Replaces InverseTransformSampling::run (src/device/wrs/its/sampling/
+/* ITS sampling: out[t] = the smallest i with cdf(i) >= u_t (at most n - 1),
+ * cdf(i) = cdf_hi[i] + cdf_lo[i] in FP64 (cdf_lo optional), u_t in (0, 1]
+ * from Philox4x32-10 (key = seed, counter = counter_base + t).  cdf_lo holds
+ * the low words of a split double CDF (hi = (float)c, lo = (float)(c - hi),
+ * so |lo| <= ulp(hi) / 2), as wrs_its_build writes them.  Above 2^16 entries
+ * (and a 16-byte aligned cdf_hi) the search runs through line indexes of the
+ * CDF that the call builds in a buffer of its own per (device, stream),
+ * n / 31 floats, kept for the next call on that stream and grown in stream
+ * order.  Replaces InverseTransformSampling::run (src/device/wrs/its/sampling/
  * InverseTransformSampling.hpp:123-138). */
 wrs_status wrs_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n, uint64_t s,
                           uint64_t seed, uint64_t counter_base, uint32_t* out,

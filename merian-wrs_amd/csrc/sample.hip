@@ -13,6 +13,10 @@
 // (non-cooperative): u in (0,1] from the 53-bit word (x0:x1), then the
 // reference's lower-bound binary search (smallest i with cdf(i) >= u,
 // shader.comp:131-140) over cdf(i) = hi[i] (+ lo[i] when given), in FP64.
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -20,6 +24,9 @@ namespace wrsk {
 using namespace wrsdev;
 
 constexpr int kSampleBlock = 256;
+// CDFs longer than this are searched through line indexes (k_its_sample_lines);
+// shorter ones stay in the caches anyway
+constexpr uint64_t kItsLineIndexMin = 1u << 16;
 
 __global__ __launch_bounds__(kSampleBlock) void k_alias_sample(const uint2* __restrict__ table,
                                                                uint64_t n, uint64_t s,
@@ -58,6 +65,176 @@ __global__ __launch_bounds__(kSampleBlock) void k_its_sample(const float* __rest
     }
 }
 
+// c(i) = hi[i] (+ lo[i]), in FP64
+template <bool LO>
+__device__ __forceinline__ double cdf_at(const float* __restrict__ hi, const float* __restrict__ lo,
+                                         uint64_t i) {
+    double c = (double)hi[i];
+    if (LO) c += (double)lo[i];
+    return c;
+}
+
+// ---- ITS sampling through line indexes -------------------------------------------
+//
+// The binary search above reads a new 128-byte line of hi (and of lo) at each
+// of its lowest ~11 levels: on N = 2^24 that is ~2.8 KB of random lines per
+// sample, and the sampler runs at the rate of those reads.  Here the CDF gets
+// a hierarchy of line indexes: level 1 holds the last hi of every line of 32
+// hi words, level l + 1 the last entry of every line of level l, up to a top
+// level of at most kItsTop entries that each block keeps in LDS.  A sample
+// searches the top level in LDS, then reads one 128-byte line per level below
+// it (as eight 16-byte loads, one round trip) and counts its entries below the
+// key -- the position of the first entry at or above it -- down to one line of
+// hi; lo is read only for the final exact comparisons.  Since |lo| <= ulp(hi)
+// / 2 and hi is non-decreasing, every element whose hi is below u_lo (the
+// float two ulps under u, rounded down) has hi + lo < u, so the first element
+// with hi >= u_lo bounds the answer from below; the answer -- the smallest i
+// with hi[i] + lo[i] >= u, at most n - 1, as in k_its_sample -- follows by
+// exact comparisons from there (a short step forward; a binary search if many
+// elements share that hi).  On N = 2^24: ~2 lines of HBM traffic per sample.
+constexpr uint32_t kLine = 32;      // floats per 128-byte line
+constexpr uint32_t kItsLevels = 6;  // index levels at most (n < 2^32)
+constexpr uint32_t kItsTop = 4096;  // entries of the top level, kept in LDS
+constexpr uint32_t kItsBlocks = 2048;  // sampler blocks (8 per CU of the 256)
+
+struct ItsIndex {
+    const float* lvl[kItsLevels + 1];  // lvl[0] = hi, lvl[l] = index level l
+    uint64_t len[kItsLevels + 1];      // entries of each level
+    uint32_t top;                      // the top level (its length <= kItsTop)
+};
+
+// Index levels 1..top from hi: entry k of level 1 is hi of element 32k + 31
+// (the last element for the last, partial line); thread k of level 1 also
+// writes entry k / 32^(l-1) of level l when it holds that line's last entry.
+__global__ __launch_bounds__(256) void k_its_index(const float* __restrict__ hi, uint64_t n,
+                                                   float* __restrict__ idx, ItsIndex ix) {
+    const uint64_t m1 = ix.len[1];
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    for (uint64_t k = (uint64_t)blockIdx.x * 256u + threadIdx.x; k < m1; k += stride) {
+        const uint64_t i = k * kLine + kLine - 1u;
+        const float v = hi[i < n ? i : n - 1u];
+        uint64_t span = 1;  // level-1 entries per entry of level l
+        for (uint32_t l = 1; l <= ix.top; ++l) {
+            if ((k + 1u) % span != 0 && k + 1u != m1) break;
+            const_cast<float*>(ix.lvl[l])[k / span] = v;
+            span *= kLine;
+        }
+    }
+    (void)idx;
+}
+
+// u > hi[i] + lo[i], reading lo only when u lies within an ulp of hi[i] (a
+// split double word has |lo| <= ulp(hi) / 2): random samples almost never do,
+// so the search touches one line of hi per sample and none of lo.
+template <bool LO>
+__device__ __forceinline__ bool u_above(double u, const float* __restrict__ hi,
+                                        const float* __restrict__ lo, uint64_t i) {
+    const float h = hi[i];
+    if (!LO) return u > (double)h;
+    const double hd = (double)h;
+    const double ulp = (double)nextafterf(fabsf(h), __builtin_huge_valf()) - (double)fabsf(h);
+    if (u > hd + ulp) return true;
+    if (u < hd - ulp) return false;
+    return u > hd + (double)lo[i];
+}
+
+// One level of the descent for the 64 samples of a wave: lane l holds its
+// sample's entry e (a line of the level below) and key; returns the position
+// in the level below of the first entry >= key in that line (entries past
+// `len` count as >= key).  The lines are read cooperatively: in round r the
+// eight lanes of group g read the line of lane 8r + g as eight 16-byte words
+// (one 128-byte request for the line) and sum their counts, so a wave issues
+// one request per line instead of eight.
+template <bool LEAF>
+__device__ __forceinline__ uint64_t descend_wave(const float* __restrict__ lv, uint64_t len,
+                                                 uint64_t e, float key) {
+    const int lane = lane_id();
+    const int grp = lane >> 3, sub = lane & 7;
+    uint32_t mine = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int src = 8 * r + grp;
+        const uint64_t es = shfl(e, src);
+        const float ks = shfl(key, src);
+        const uint64_t p0 = es * kLine + (uint64_t)sub * 4u;
+        uint32_t c;
+        if (p0 + 4u <= len) {
+            const float4 v = *reinterpret_cast<const float4*>(lv + p0);
+            c = (v.x < ks) + (v.y < ks) + (v.z < ks) + (v.w < ks);
+        } else {
+            c = 0;
+            for (uint64_t p = p0; p < len && p < p0 + 4u; ++p) c += lv[p] < ks;
+        }
+        c += __shfl_xor(c, 1, kWave);
+        c += __shfl_xor(c, 2, kWave);
+        c += __shfl_xor(c, 4, kWave);
+        const uint32_t back = shfl(c, ((lane - 8 * r) & 7) * 8);  // group (lane - 8r)'s sum
+        if (grp == r) mine = back;
+    }
+    return e * kLine + mine;
+}
+
+template <bool LO>
+__global__ __launch_bounds__(kSampleBlock) void k_its_sample_lines(
+    const float* __restrict__ hi, const float* __restrict__ lo, ItsIndex ix, uint64_t n,
+    uint64_t s, uint64_t seed, uint64_t base, uint32_t* __restrict__ out) {
+    __shared__ float s_top[kItsTop];
+    const uint32_t top = ix.top;
+    const uint64_t ntop = ix.len[top];
+    for (uint32_t k = threadIdx.x; k < ntop; k += kSampleBlock) s_top[k] = ix.lvl[top][k];
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * kSampleBlock;
+    // the descent is wave-cooperative: every lane runs every round of its
+    // wave (lanes past s compute a dummy sample and store nothing)
+    const uint64_t wave0 = (uint64_t)blockIdx.x * kSampleBlock + (threadIdx.x & ~(kWave - 1));
+    for (uint64_t gw = wave0; gw < s; gw += stride) {
+        const uint64_t g = gw + (uint64_t)lane_id();
+        const U4 r = philox_at(seed, base + (g < s ? g : 0u), 0u, 0u);
+        const double u = u53_open0(r.x, r.y);
+        const float f = __double2float_rd(u);
+        const float key = LO ? nextafterf(nextafterf(f, -__builtin_huge_valf()),
+                                         -__builtin_huge_valf())
+                             : f;
+        // top level: first entry >= key, in LDS
+        uint32_t a = 0, b = (uint32_t)ntop;
+        while (a < b) {
+            const uint32_t mid = (a + b) >> 1;
+            if (s_top[mid] < key) a = mid + 1;
+            else b = mid;
+        }
+        // levels below, down to hi: the whole wave descends together (a
+        // sample past the top level's end follows line 0 and is then clamped)
+        uint64_t e = a < ntop ? a : 0u;  // line of the level below holding the answer
+        for (uint32_t l = top; l > 1; --l)
+            e = descend_wave<false>(ix.lvl[l - 1], ix.len[l - 1], e, key);
+        e = descend_wave<true>(hi, n, e, key);
+        uint64_t i = a < ntop ? (e < n ? e : n - 1u) : n - 1u;
+        // exact: the smallest i' >= i with cdf(i') >= u, at most n - 1
+        int steps = 0;
+        while (i < n - 1u && u_above<LO>(u, hi, lo, i) && steps < 8) {
+            ++i;
+            ++steps;
+        }
+        if (i < n - 1u && u_above<LO>(u, hi, lo, i)) {
+            // a long run of elements whose CDF words share this hi (weights far
+            // below an ulp of the CDF): gallop over it, then bisect
+            uint64_t p = i + 1u, k = 8;
+            while (p + k < n - 1u && u_above<LO>(u, hi, lo, p + k)) {
+                p += k + 1u;
+                k *= 2u;
+            }
+            uint64_t q = p + k < n - 1u ? p + k : n - 1u;
+            while (p < q) {
+                const uint64_t mid = (p + q) >> 1;
+                if (u_above<LO>(u, hi, lo, mid)) p = mid + 1;
+                else q = mid;
+            }
+            i = p;
+        }
+        if (g < s) out[g] = (uint32_t)i;
+    }
+}
+
 // ---- cutpoint (guide table) acceleration of ITS sampling ---------------------
 //
 // Chen & Asau's guide table over the normalised CDF c(i) = hi[i] (+ lo[i]):
@@ -85,13 +262,6 @@ __device__ __forceinline__ int64_t cut_floor(double c, uint64_t m) {
     return g;
 }
 
-template <bool LO>
-__device__ __forceinline__ double cdf_at(const float* __restrict__ hi, const float* __restrict__ lo,
-                                         uint64_t i) {
-    double c = (double)hi[i];
-    if (LO) c += (double)lo[i];
-    return c;
-}
 
 // lower bound of t in [a, b]: smallest i with c(i) >= t, b if none
 template <bool LO>
@@ -374,10 +544,74 @@ hipError_t launch_its_sample_window(const float* cdf_hi, const float* cdf_lo, ui
     return hipGetLastError();
 }
 
+// Device memory of the ITS line indexes: one buffer per (device, stream),
+// kept between calls and grown in stream order (hipFreeAsync of the old one,
+// hipMallocAsync of the new one on the caller's stream) when a longer CDF
+// comes; a call on a stream whose buffer is large enough allocates nothing.
+static hipError_t index_buffer(hipStream_t s, size_t bytes, float** out) {
+    static std::mutex mu;
+    static std::map<std::pair<int, hipStream_t>, std::pair<float*, size_t>> bufs;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lock(mu);
+    auto& b = bufs[{dev, s}];
+    if (b.second < bytes) {
+        if (b.first) {
+            e = hipFreeAsync(b.first, s);
+            if (e != hipSuccess) return e;
+            b = {nullptr, 0};
+        }
+        const size_t want = bytes + bytes / 4;
+        e = hipMallocAsync(reinterpret_cast<void**>(&b.first), want, s);
+        if (e != hipSuccess) return e;
+        b.second = want;
+    }
+    *out = b.first;
+    return hipSuccess;
+}
+
 hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
                              uint64_t s_count, uint64_t seed, uint64_t counter_base,
                              uint32_t* out, hipStream_t s) {
     const uint32_t g = grid_for(s_count, kSampleBlock, 1u << 16);
+    if (n > kItsLineIndexMin && reinterpret_cast<uintptr_t>(cdf_hi) % 16u == 0) {
+        // the line indexes, in stream-ordered memory (graph-capturable); every
+        // level starts on a 16-byte boundary for the line loads
+        ItsIndex ix{};
+        ix.lvl[0] = cdf_hi;
+        ix.len[0] = n;
+        uint64_t words = 0, len = n;
+        uint32_t l = 0;
+        while (len > kItsTop && l < kItsLevels) {
+            len = (len + kLine - 1u) / kLine;
+            ++l;
+            ix.len[l] = len;
+            words += (len + 3u) & ~(uint64_t)3u;
+        }
+        ix.top = l;
+        // (a caller's CDF at a pointer that is not 16-byte aligned takes the
+        // plain search below)
+        float* buf = nullptr;
+        hipError_t e = index_buffer(s, words * sizeof(float), &buf);
+        if (e != hipSuccess) return e;
+        uint64_t off = 0;
+        for (uint32_t k = 1; k <= l; ++k) {
+            ix.lvl[k] = buf + off;
+            off += (ix.len[k] + 3u) & ~(uint64_t)3u;
+        }
+        k_its_index<<<grid_for(ix.len[1], 256, 1u << 14), 256, 0, s>>>(cdf_hi, n, buf, ix);
+        // every block stages the top level in LDS first: a grid of about one
+        // block per resident slot, each looping over its samples
+        const uint32_t gl = std::min(g, kItsBlocks);
+        if (cdf_lo)
+            k_its_sample_lines<true><<<gl, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, ix, n, s_count,
+                                                                 seed, counter_base, out);
+        else
+            k_its_sample_lines<false><<<gl, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, ix, n, s_count,
+                                                                  seed, counter_base, out);
+        return hipGetLastError();
+    }
     if (cdf_lo)
         k_its_sample<true><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, s_count, seed,
                                                       counter_base, out);
