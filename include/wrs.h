@@ -174,6 +174,20 @@ wrs_status wrs_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n, 
                           uint64_t seed, uint64_t counter_base, uint32_t* out,
                           wrs_stream_t stream);
 
+/* wrs_its_sample without allocation: the same samples (bit-identical), with
+ * the line indexes in a static device array of the library (4.2 MB per GPU,
+ * held until the process ends) instead of a buffer allocated per stream, so
+ * the call makes no device allocation of any kind.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Limit:
+ * 1 <= n <= WRS_ITS_NOALLOC_MAX_N (WRS_EINVAL beyond).  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (two streams, two
+ * threads) race; calls in stream order do not. */
+#define WRS_ITS_NOALLOC_MAX_N (1ull << 25)
+wrs_status wrs_its_sample_noalloc(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                  uint64_t s, uint64_t seed, uint64_t counter_base,
+                                  uint32_t* out, wrs_stream_t stream);
+
 /* Windowed ("cooperative") sampling, for throughput where the samples need
  * not be independent of each other.  Samples come in groups of 64 consecutive
  * counters (group g = counter / 64, with its own Philox stream); all samples

@@ -194,6 +194,12 @@ hipError_t launch_alias_sample(const AliasEntry* table, uint64_t n, uint64_t s_c
 hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
                              uint64_t s_count, uint64_t seed, uint64_t counter_base,
                              uint32_t* out, hipStream_t s);
+// the same samples with the line indexes in a static device array (no
+// allocation); n <= kItsNoAllocMaxN
+constexpr uint64_t kItsNoAllocMaxN = 1ull << 25;
+hipError_t launch_its_sample_noalloc(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                     uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                                     uint32_t* out, hipStream_t s);
 // windowed samplers (groups of 64 consecutive counters share a window);
 // s_count >= 1, window >= 1
 hipError_t launch_alias_sample_window(const AliasEntry* table, uint64_t n, uint64_t s_count,

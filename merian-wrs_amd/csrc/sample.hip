@@ -571,47 +571,67 @@ static hipError_t index_buffer(hipStream_t s, size_t bytes, float** out) {
     return hipSuccess;
 }
 
-hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
-                             uint64_t s_count, uint64_t seed, uint64_t counter_base,
-                             uint32_t* out, hipStream_t s) {
-    const uint32_t g = grid_for(s_count, kSampleBlock, 1u << 16);
-    if (n > kItsLineIndexMin && reinterpret_cast<uintptr_t>(cdf_hi) % 16u == 0) {
-        // the line indexes, in stream-ordered memory (graph-capturable); every
-        // level starts on a 16-byte boundary for the line loads
-        ItsIndex ix{};
-        ix.lvl[0] = cdf_hi;
-        ix.len[0] = n;
-        uint64_t words = 0, len = n;
-        uint32_t l = 0;
-        while (len > kItsTop && l < kItsLevels) {
-            len = (len + kLine - 1u) / kLine;
-            ++l;
-            ix.len[l] = len;
-            words += (len + 3u) & ~(uint64_t)3u;
-        }
-        ix.top = l;
-        // (a caller's CDF at a pointer that is not 16-byte aligned takes the
-        // plain search below)
-        float* buf = nullptr;
-        hipError_t e = index_buffer(s, words * sizeof(float), &buf);
-        if (e != hipSuccess) return e;
-        uint64_t off = 0;
-        for (uint32_t k = 1; k <= l; ++k) {
-            ix.lvl[k] = buf + off;
-            off += (ix.len[k] + 3u) & ~(uint64_t)3u;
-        }
-        k_its_index<<<grid_for(ix.len[1], 256, 1u << 14), 256, 0, s>>>(cdf_hi, n, buf, ix);
-        // every block stages the top level in LDS first: a grid of about one
-        // block per resident slot, each looping over its samples
-        const uint32_t gl = std::min(g, kItsBlocks);
-        if (cdf_lo)
-            k_its_sample_lines<true><<<gl, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, ix, n, s_count,
-                                                                 seed, counter_base, out);
-        else
-            k_its_sample_lines<false><<<gl, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, ix, n, s_count,
-                                                                  seed, counter_base, out);
-        return hipGetLastError();
+// Layout of the line indexes of an n-entry CDF in one buffer: level k >= 1
+// holds ceil(len[k-1] / 32) floats, each level starting on a 16-byte boundary
+// for the line loads; returns the floats the buffer needs.
+static uint64_t its_index_layout(const float* cdf_hi, uint64_t n, ItsIndex& ix) {
+    ix = ItsIndex{};
+    ix.lvl[0] = cdf_hi;
+    ix.len[0] = n;
+    uint64_t words = 0, len = n;
+    uint32_t l = 0;
+    while (len > kItsTop && l < kItsLevels) {
+        len = (len + kLine - 1u) / kLine;
+        ++l;
+        ix.len[l] = len;
+        words += (len + 3u) & ~(uint64_t)3u;
     }
+    ix.top = l;
+    return words;
+}
+
+// The same count at compile time (sizes the static index array below).
+constexpr uint64_t its_index_words(uint64_t n) {
+    uint64_t words = 0, len = n;
+    for (uint32_t l = 0; len > kItsTop && l < kItsLevels; ++l) {
+        len = (len + kLine - 1u) / kLine;
+        words += (len + 3u) & ~(uint64_t)3u;
+    }
+    return words;
+}
+
+static bool its_uses_index(const float* cdf_hi, uint64_t n) {
+    // (a caller's CDF at a pointer that is not 16-byte aligned takes the plain
+    // search)
+    return n > kItsLineIndexMin && reinterpret_cast<uintptr_t>(cdf_hi) % 16u == 0;
+}
+
+// Index build + indexed search, the index in buf (its_index_layout(...) floats).
+static hipError_t its_sample_indexed(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                     uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                                     uint32_t* out, float* buf, ItsIndex ix, hipStream_t s) {
+    uint64_t off = 0;
+    for (uint32_t k = 1; k <= ix.top; ++k) {
+        ix.lvl[k] = buf + off;
+        off += (ix.len[k] + 3u) & ~(uint64_t)3u;
+    }
+    k_its_index<<<grid_for(ix.len[1], 256, 1u << 14), 256, 0, s>>>(cdf_hi, n, buf, ix);
+    // every block stages the top level in LDS first: a grid of about one
+    // block per resident slot, each looping over its samples
+    const uint32_t gl = std::min(grid_for(s_count, kSampleBlock, 1u << 16), kItsBlocks);
+    if (cdf_lo)
+        k_its_sample_lines<true><<<gl, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, ix, n, s_count, seed,
+                                                             counter_base, out);
+    else
+        k_its_sample_lines<false><<<gl, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, ix, n, s_count,
+                                                              seed, counter_base, out);
+    return hipGetLastError();
+}
+
+static hipError_t its_sample_plain(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                   uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                                   uint32_t* out, hipStream_t s) {
+    const uint32_t g = grid_for(s_count, kSampleBlock, 1u << 16);
     if (cdf_lo)
         k_its_sample<true><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, s_count, seed,
                                                       counter_base, out);
@@ -619,6 +639,66 @@ hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t 
         k_its_sample<false><<<g, kSampleBlock, 0, s>>>(cdf_hi, cdf_lo, n, s_count, seed,
                                                        counter_base, out);
     return hipGetLastError();
+}
+
+hipError_t launch_its_sample(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                             uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                             uint32_t* out, hipStream_t s) {
+    if (!its_uses_index(cdf_hi, n))
+        return its_sample_plain(cdf_hi, cdf_lo, n, s_count, seed, counter_base, out, s);
+    // the line indexes, in stream-ordered memory of this stream
+    ItsIndex ix;
+    const uint64_t words = its_index_layout(cdf_hi, n, ix);
+    float* buf = nullptr;
+    hipError_t e = index_buffer(s, words * sizeof(float), &buf);
+    if (e != hipSuccess) return e;
+    return its_sample_indexed(cdf_hi, cdf_lo, n, s_count, seed, counter_base, out, buf, ix, s);
+}
+
+// ---- ITS sampling without allocation ----------------------------------------
+// The line indexes live in one static device array, `scratch`, sized for CDFs
+// of up to kItsNoAllocMaxN entries (its_index_words(2^25) floats, 4.2 MB, held
+// by every GPU the library runs on until the process ends).  Host code never
+// passes the array's name to a kernel: its address on the current device comes
+// from hipGetSymbolAddress, once per device, and is kept per device.  Every
+// call on one GPU rewrites that GPU's array before reading it (nothing in it
+// must start at zero), so calls that overlap on one GPU -- on two streams, or
+// from two threads -- race; calls in stream order do not.
+constexpr uint64_t kItsScratchWords = its_index_words(kItsNoAllocMaxN);
+static __device__ __align__(256) float scratch[kItsScratchWords];
+
+constexpr int kMaxDevices = 64;
+
+static hipError_t scratch_ptr(float** out) {
+    static std::mutex mu;
+    static float* ptrs[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptrs[dev] = static_cast<float*>(p);
+    }
+    *out = ptrs[dev];
+    return hipSuccess;
+}
+
+hipError_t launch_its_sample_noalloc(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                     uint64_t s_count, uint64_t seed, uint64_t counter_base,
+                                     uint32_t* out, hipStream_t s) {
+    if (n > kItsNoAllocMaxN) return hipErrorInvalidValue;  // the host checks first
+    if (!its_uses_index(cdf_hi, n))
+        return its_sample_plain(cdf_hi, cdf_lo, n, s_count, seed, counter_base, out, s);
+    ItsIndex ix;
+    if (its_index_layout(cdf_hi, n, ix) > kItsScratchWords) return hipErrorInvalidValue;
+    float* buf = nullptr;
+    hipError_t e = scratch_ptr(&buf);
+    if (e != hipSuccess) return e;
+    return its_sample_indexed(cdf_hi, cdf_lo, n, s_count, seed, counter_base, out, buf, ix, s);
 }
 
 hipError_t launch_cutpoint_build(const float* cdf_hi, const float* cdf_lo, uint64_t n,

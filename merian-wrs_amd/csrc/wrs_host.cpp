@@ -509,6 +509,22 @@ wrs_status wrs_its_sample_window(const float* cdf_hi, const float* cdf_lo, uint6
     });
 }
 
+static_assert(WRS_ITS_NOALLOC_MAX_N == wrsk::kItsNoAllocMaxN, "static ITS index limit");
+
+wrs_status wrs_its_sample_noalloc(const float* cdf_hi, const float* cdf_lo, uint64_t n,
+                                  uint64_t s, uint64_t seed, uint64_t counter_base,
+                                  uint32_t* out, wrs_stream_t stream) {
+    return guarded([&] {
+        wrs::require(cdf_hi && out, "null sampling buffer");
+        wrs::require(n > 0 && n <= WRS_ITS_NOALLOC_MAX_N,
+                     "n must be in [1, WRS_ITS_NOALLOC_MAX_N = 2^25] for the static index");
+        if (s == 0) return;
+        wrs::check_hip(wrsk::launch_its_sample_noalloc(cdf_hi, cdf_lo, n, s, seed, counter_base,
+                                                       out, hs(stream)),
+                       "its sample (static index) launch");
+    });
+}
+
 wrs_status wrs_cutpoint_build(const float* cdf_hi, const float* cdf_lo, uint64_t n,
                               uint32_t* guide, uint64_t m, wrs_stream_t stream) {
     return guarded([&] {

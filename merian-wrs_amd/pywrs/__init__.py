@@ -28,7 +28,7 @@ SPLIT_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float64
 EXPORTS = [
     "wrs_plan_create", "wrs_plan_destroy", "wrs_psa_build", "wrs_psa_build_debug", "wrs_mean",
     "wrs_prefix_partition", "wrs_scan_inclusive", "wrs_its_build", "wrs_alias_sample",
-    "wrs_its_sample", "wrs_alias_sample_window", "wrs_its_sample_window", "wrs_cutpoint_build", "wrs_cutpoint_sample", "wrs_philox", "wrs_generate",
+    "wrs_its_sample", "wrs_its_sample_noalloc", "wrs_alias_sample_window", "wrs_its_sample_window", "wrs_cutpoint_build", "wrs_cutpoint_sample", "wrs_philox", "wrs_generate",
     "wrs_batch_plan_create",
     "wrs_batch_plan_destroy", "wrs_psa_build_batched", "wrs_alias_sample_batched",
     "wrs_autotune_psa", "wrs_autotune_batched", "wrs_histogram", "wrs_sample_stats",
@@ -111,6 +111,7 @@ def lib():
         "wrs_its_build": [vp, vp, u64, vp, vp, vp],
         "wrs_alias_sample": [vp, u64, u64, u64, u64, vp, vp],
         "wrs_its_sample": [vp, vp, u64, u64, u64, u64, vp, vp],
+        "wrs_its_sample_noalloc": [vp, vp, u64, u64, u64, u64, vp, vp],
         "wrs_alias_sample_window": [vp, u64, u64, u64, u64, u32, vp, vp],
         "wrs_its_sample_window": [vp, vp, u64, u64, u64, u64, u32, i32, vp, vp],
         "wrs_cutpoint_build": [vp, vp, u64, vp, u64, vp],
@@ -468,6 +469,18 @@ def its_sample(cdf_hi, cdf_lo, n: int, s: int, seed: int, out, counter_base: int
                stream=None) -> None:
     check(lib().wrs_its_sample(_p(cdf_hi), _p(cdf_lo), n, s, seed, counter_base, _p(out),
                                _h(stream)))
+
+
+ITS_NOALLOC_MAX_N = 1 << 25
+
+
+def its_sample_noalloc(cdf_hi, cdf_lo, n: int, s: int, seed: int, out, counter_base: int = 0,
+                       stream=None) -> None:
+    """its_sample's samples with the search index in the library's static
+    device array (no allocation; n <= ITS_NOALLOC_MAX_N; calls overlapping on
+    one GPU race)."""
+    check(lib().wrs_its_sample_noalloc(_p(cdf_hi), _p(cdf_lo), n, s, seed, counter_base,
+                                       _p(out), _h(stream)))
 
 
 def alias_sample_window(table, n: int, s: int, seed: int, window: int, out,
