@@ -76,19 +76,35 @@ typedef struct {
                                pass with decoupled look-back (per segment in a batch) */
     uint32_t direct;        /* pack form, WRS_PACK_*: DIRECT (the default) the partition
                                writes the heavies and every 64th light's position only and
-                               the pack reads the lights from the weights; CLASSIC it writes
-                               every index and weight in partition order (12N more bytes;
-                               same table).  Debug builds use CLASSIC; batch plans build
-                               DIRECT with partition 0 only. */
+                               the merge pack reads the lights from the weights; CLASSIC it
+                               writes every index and weight in partition order (12N more
+                               bytes; same table).  SERIAL and INLINE are the reference's
+                               own strategies, kept as baselines: K = ceil(N / split_size)
+                               split points and one thread sweeping each split interval
+                               serially -- SERIAL writes the K split records first
+                               (ScalarSplit + ScalarPack, split/scalar/shader.comp:111-133,
+                               pack/scalar/float.comp:60-180), INLINE finds them in the
+                               sweeping kernel (InlineSplitPack, splitpack/inline/
+                               shader.comp:225-270).  Their tables are identical to each
+                               other and differ from the merge forms' by the rounding of
+                               the sweep's running residual; the first build of such a plan
+                               allocates the full prefix arrays (8*max_n bytes, 4*max_n in
+                               WRS_PREC_FP32 and WRS_PREC_FP64_CARRY) and, for SERIAL,
+                               16*K split records.  Debug builds use CLASSIC; batch plans
+                               build DIRECT with partition 0 only (SERIAL and INLINE are
+                               rejected there). */
 } wrs_tuning;
 #define WRS_PACK_DEFAULT 0u /* the direct form */
 #define WRS_PACK_DIRECT 1u
 #define WRS_PACK_CLASSIC 2u
+#define WRS_PACK_SERIAL 3u
+#define WRS_PACK_INLINE 4u
 
 /* Build configuration.  Reference: PSAConfig (src/device/wrs/alias/psa/PSA.hpp:19-41),
  * whose splitSize sets the split granularity. */
 typedef struct {
-    uint32_t split_size;     /* split granularity of the debug split output (K = ceil(N/split_size)); 0 -> 32 */
+    uint32_t split_size;     /* split granularity (K = ceil(N/split_size)) of the debug split
+                                output and of the SERIAL and INLINE pack forms; 0 -> 32 */
     wrs_precision precision; /* default WRS_PREC_COMPENSATED */
     wrs_tuning tuning;       /* all zero: default geometry */
 } wrs_config;
@@ -96,7 +112,11 @@ typedef struct {
 typedef struct wrs_plan wrs_plan;
 
 /* Creates a plan able to build distributions of up to max_n weights
- * (max_n < 2^32).  Allocates 12*max_n + O(max_n/4096) bytes of device memory.
+ * (max_n < 2^32).  Allocates about 8.4*max_n bytes of device memory: the
+ * indices and weights in partition order (8*max_n), the prefix samples, light
+ * positions, boundary splits and look-back records (max_n/3 together).  The
+ * full prefix arrays (4 or 8 bytes per weight) follow on the first debug build,
+ * sample-statistics call or SERIAL/INLINE build that needs them.
  * Replaces the construction of device::PSA / device::WRS and their
  * Buffers::allocate (src/device/wrs/alias/psa/PSA.hpp:43-147,
  * src/device/wrs/WRS.hpp:60-88). */
@@ -296,12 +316,15 @@ wrs_status wrs_sample_stats(wrs_plan* plan, const float* weights, uint64_t n, co
 
 /* ---- autotuner ----
  * Times candidate geometries (wrs_tuning) on the caller's own data with HIP
- * events and returns the fastest: first the prefix-partition form with the
- * default merge shape, then every supported merge shape with the better form
- * (coordinate descent; each candidate the median of 5 builds after one
- * warm-up).  Results are cached per process under a key of device, precision
- * and problem shape (n, or the segment count and a histogram of log2 segment
- * sizes); with the environment variable WRS_TUNE_CACHE naming a file they are
+ * events and returns the fastest, by coordinate descent: the prefix-partition
+ * form at the default merge shape, then the pack form (classic, direct, and
+ * for a single build the reference's SERIAL and INLINE sweeps at the config's
+ * split size), then every supported merge shape with the better forms (none
+ * if a sweep form won).  Each candidate is the median of 9 builds after one
+ * warm-up; 20 untimed builds of the first candidate run before the search.
+ * Results are cached per process under a key of device, precision and problem
+ * shape (n and split size, or the segment count and a histogram of log2
+ * segment sizes); with the environment variable WRS_TUNE_CACHE naming a file they are
  * also read from and appended to that file.  Synchronises `stream`;
  * allocates its own scratch.  log (optional, log_cap entries) receives every
  * candidate timed, *log_len their number (0 for a cache hit).  No reference

@@ -180,6 +180,15 @@ hipError_t launch_mergepack(const float* pw, uint64_t n, MergeShape ms, int prec
 // the K split points S(k), k = 1..K, of the reference's splitK layout (debug output)
 hipError_t launch_splits(uint64_t n, uint32_t K, int prec, const Params* prm, const float* hi,
                          const float* lo, SplitRec* splits, hipStream_t s);
+// the sweep pack forms: serial (split records from launch_splits, then one
+// thread per split interval) and inline (splits and sweeps in one kernel);
+// idx, pw: the classic partition arrays; hi, lo: the full prefix arrays
+hipError_t launch_pack_serial(uint64_t n, uint32_t K, int prec, const Params* prm,
+                              const SplitRec* splits, const uint32_t* idx, const float* pw,
+                              AliasEntry* table, hipStream_t s);
+hipError_t launch_splitpack_inline(uint64_t n, uint32_t K, int prec, const Params* prm,
+                                   const float* hi, const float* lo, const uint32_t* idx,
+                                   const float* pw, AliasEntry* table, hipStream_t s);
 // reduce-then-scan form for large n: needs no total beforehand (it writes
 // prm->total, mean and inv_total itself) and no reset pass; states as for a
 // PSA build of n

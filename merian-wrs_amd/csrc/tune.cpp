@@ -6,9 +6,11 @@
 //      decoupled look-back, per segment in a batch), at the default merge shape;
 //   2. the pack form (classic or direct) with the form of step 1 (a batch builds
 //      the direct form with the reduce-then-scan partition only, so there the
-//      direct candidate takes that partition form);
+//      direct candidate takes that partition form); a single build also times
+//      the reference's serial and inline split + pack at the plan's split size;
 //   3. every supported merge shape (buckets per block x buckets per thread)
-//      with the forms of steps 1 and 2.
+//      with the forms of steps 1 and 2 (none when a sweep form won: it has no
+//      merge shape).
 // A candidate's time is the median of kReps builds (HIP events on the
 // caller's stream) after one warm-up build; kWarm builds of the first
 // candidate run untimed before the search, so that the first candidate is not
@@ -108,9 +110,9 @@ std::string device_key() {
     hipDeviceProp_t prop;
     wrs::check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     std::ostringstream k;
-    // "t2": the tuning encoding (WRS_PACK_* pack forms); lines of other
-    // encodings in a cache file do not match
-    k << "t2/" << prop.gcnArchName << "/cu" << prop.multiProcessorCount;
+    // "t3": the tuning encoding (WRS_PACK_* pack forms, sweep forms included);
+    // lines of other encodings in a cache file do not match
+    k << "t3/" << prop.gcnArchName << "/cu" << prop.multiProcessorCount;
     return k.str();
 }
 
@@ -187,6 +189,19 @@ wrs_tuning descend(M&& measure, bool direct_rts_only) {
             best = t2;
         }
     }
+    if (!direct_rts_only) {  // the reference's sweep forms (single builds only)
+        for (const uint32_t f : {WRS_PACK_SERIAL, WRS_PACK_INLINE}) {
+            wrs_tuning t3 = best;
+            t3.direct = f;
+            const float ms3 = measure(t3);
+            if (ms3 < best_ms) {
+                best_ms = ms3;
+                best = t3;
+            }
+        }
+        // the merge shape does not apply to them
+        if (best.direct == WRS_PACK_SERIAL || best.direct == WRS_PACK_INLINE) return best;
+    }
     const wrs_tuning form = best;
     for (size_t i = 1; i < sizeof(kShapes) / sizeof(kShapes[0]); ++i) {
         const wrs_tuning t = with_shape(form, kShapes[i]);
@@ -203,7 +218,7 @@ wrs_tuning descend(M&& measure, bool direct_rts_only) {
 wrs_tuning tune_psa(const wrs::PSAConfig& base, const float* w, uint64_t n, uint32_t flags,
                     hipStream_t s, Log& log) {
     std::ostringstream key;
-    key << device_key() << "/p" << (int)base.precision << "/psa/n" << n;
+    key << device_key() << "/p" << (int)base.precision << "/psa/n" << n << "/s" << base.splitSize;
     wrs_tuning best{};
     if (!(flags & WRS_TUNE_NO_CACHE) && cache_get(key.str(), best)) return best;
 

@@ -74,9 +74,18 @@ void Workspace::ensure_prefix(bool lo) {
     if (lo && !prefix_lo) prefix_lo = dev_alloc<float>(max_n);
 }
 
+void Workspace::ensure_splits(uint64_t K) {
+    if (split_cap >= K) return;
+    if (splits) (void)hipFree(splits);
+    splits = nullptr;
+    split_cap = 0;
+    splits = dev_alloc<wrsk::SplitRec>(K);
+    split_cap = K;
+}
+
 void Workspace::release() {
-    void* ptrs[] = {params,    red,       states, indices, pweights,
-                    prefix_hi, prefix_lo, samp_h, samp_l,  bounds, lightpos};
+    void* ptrs[] = {params,    red,       states, indices, pweights, prefix_hi,
+                    prefix_lo, samp_h,    samp_l, bounds,  lightpos, splits};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     params = nullptr;
@@ -90,6 +99,8 @@ void Workspace::release() {
     samp_l = nullptr;
     bounds = nullptr;
     lightpos = nullptr;
+    splits = nullptr;
+    split_cap = 0;
 }
 
 static void require(bool cond, const char* msg) {
@@ -107,7 +118,7 @@ void require_tuning(const wrs_tuning& t) {
     require(wrsk::merge_shape_supported(merge_shape_of(t)),
             "unsupported merge shape (merge_tile 1024/2048 with merge_per 2/4/8, or 4096 with 4/8)");
     require(t.partition <= 1u, "partition must be 0 or 1");
-    require(t.direct <= WRS_PACK_CLASSIC, "direct must be a WRS_PACK_* value");
+    require(t.direct <= WRS_PACK_INLINE, "direct must be a WRS_PACK_* value");
 }
 
 void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, double pivot) const {
@@ -158,14 +169,20 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     const wrs_precision prec = m_cfg.precision;
     // full prefix arrays only for a debug build (parity tests); the build
     // itself needs the indices and the sparse prefix samples
-    const bool full_prefix = dbg && (dbg->prefix_hi || dbg->prefix_lo || dbg->splits);
+    // the sweep forms (the reference's serial and inline split + pack) read
+    // the full prefix arrays and the classic partition arrays
+    const uint32_t form = dbg ? WRS_PACK_CLASSIC : m_cfg.tuning.direct;
+    const bool sweep = form == WRS_PACK_SERIAL || form == WRS_PACK_INLINE;
+    const bool full_prefix = sweep || (dbg && (dbg->prefix_hi || dbg->prefix_lo || dbg->splits));
     if (full_prefix) ws.ensure_prefix(wrsk::prec_has_lo(prec));
+    if (form == WRS_PACK_SERIAL) ws.ensure_splits(splitCount(N));
     float* hi = full_prefix ? ws.prefix_hi : nullptr;
     float* lo = full_prefix && wrsk::prec_has_lo(prec) ? ws.prefix_lo : nullptr;
     // the direct form (partition arrays hold the heavies only) for plain builds;
     // a debug build reads the full partition arrays
     wrsk::LightSrc ls;
-    if (m_cfg.tuning.direct != WRS_PACK_CLASSIC && !dbg) ls = wrsk::LightSrc{b.weights, ws.lightpos};
+    if ((form == WRS_PACK_DEFAULT || form == WRS_PACK_DIRECT) && !dbg)
+        ls = wrsk::LightSrc{b.weights, ws.lightpos};
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::state_words_for(N);
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
@@ -182,6 +199,25 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
                                      ev ? ev[3] : nullptr, ev ? ev[4] : nullptr),
               "partition launch");
     if (ev) check_hip(hipEventRecord(ev[5], s), "event");
+    if (sweep) {
+        const uint32_t K = splitCount(N);
+        auto* table = reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable);
+        if (form == WRS_PACK_SERIAL) {
+            check_hip(wrsk::launch_splits(N, K, (int)prec, ws.params, hi, lo, ws.splits, s),
+                      "splits launch");
+            if (ev) check_hip(hipEventRecord(ev[6], s), "event");
+            check_hip(wrsk::launch_pack_serial(N, K, (int)prec, ws.params, ws.splits, ws.indices,
+                                               ws.pweights, table, s),
+                      "serial pack launch");
+        } else {
+            if (ev) check_hip(hipEventRecord(ev[6], s), "event");
+            check_hip(wrsk::launch_splitpack_inline(N, K, (int)prec, ws.params, hi, lo,
+                                                    ws.indices, ws.pweights, table, s),
+                      "inline split pack launch");
+        }
+        if (ev) check_hip(hipEventRecord(ev[7], s), "event");
+        return;
+    }
     const wrsk::MergeShape ms = merge_shape_of(m_cfg.tuning);
     check_hip(wrsk::launch_coarse(ws.pweights, N, ms, (int)prec, ws.params, ws.samp_h, ws.samp_l,
                                   ws.bounds, s, ws.indices, ls),

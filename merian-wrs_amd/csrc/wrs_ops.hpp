@@ -58,11 +58,15 @@ struct Workspace {
     wrsk::Coarse* bounds = nullptr;    // [max_n/kMinMergeTile + 2] block-boundary splits (PSA only)
     uint32_t* lightpos = nullptr;      // [max_n/kSample + 1] direct form: element of every
                                        // kSample-th light (PSA only)
+    wrsk::SplitRec* splits = nullptr;  // [split_cap] split records of the serial pack form
+    uint64_t split_cap = 0;
 
     static Workspace allocate(uint64_t max_n, bool partition_buffers);
     // the full prefix arrays of debug builds and of the sample statistics,
     // allocated on first use
     void ensure_prefix(bool lo);
+    // the split records of the serial pack form, allocated on first use
+    void ensure_splits(uint64_t K);
     void release();
 };
 
