@@ -29,8 +29,9 @@ Also reported on the same line:
                 and S = 2^24 binary-search samples, CDF error vs the FP64 prefix;
                 the cutpoint guide table (N entries) over the same CDF and S = 2^24
                 guided samples (bit-identical to the ITS samples);
-  spike         N = 2^28 spike mixture build and S = 2^30 alias samples, with
-                the same on-device statistics of the 2^30 samples;
+  spike         N = 2^28 spike mixture build (default and tuned geometry) and
+                S = 2^30 alias samples, with the same on-device statistics of the
+                2^30 samples and the implied distribution's error vs FP64;
   sweep         S = 1e7 samples at N = 1e5 .. 1e8 (U(0,1) weights), the shape of
                 the reference's sampling plots: alias build + sample, ITS and
                 cutpoint sampling throughput, and the windowed samplers
@@ -40,10 +41,20 @@ Also reported on the same line:
   batched       4096 log-uniform-size LogNormal(0, 3) distributions built and
                 sampled (64 each) by one hipGraph vs one launch sequence per
                 distribution;
+  cold          the headline build with a 512 MiB write before every build (the
+                reference flushes its L2 between measured runs): median ms and GB/s;
+  c1            N = 2^20 U(0,1) weights (seed 0): build, S = 2^20 alias samples,
+                their chi-square and RMSE, the implied distribution's error vs FP64;
+  scan_2e26     the normalised scan (ITS build) of the headline weights in FP32 and
+                compensated modes, GB/s of 8N bytes;
+  split_pack    the reference's own split + pack strategies on the headline
+                weights, as pack forms: SERIAL (split records + one thread per
+                split interval) at split sizes 2 .. 1024 and INLINE, with the time
+                of their split and pack kernels;
   cpu_baseline  the reference CPU path (the oracle's single-threaded C
-                restatement) on the same N = 2^26 weights.
+                restatement) on the same N = 2^26 weights, best of 5.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--only SECTION,...] [--no-tune]
 """
 import argparse
 import json
@@ -62,6 +73,7 @@ HBM_PEAK_GBS = 8000.0
 PREC_NAMES = ["fp32", "fp64-carry", "compensated-fp32", "fp64-tiles"]
 METRIC = "PSA build GB/s (% HBM peak) + samples/s; batched vs per-launch at fixed error"
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "psa_build_zipf_n2e26_hbm_traffic.md")
+SECTIONS = ("precision", "c1", "scan26", "split_pack", "its", "spike", "batched", "sweep")
 
 
 def log(*a):
@@ -73,7 +85,10 @@ class Implied:
 
     def __init__(self, w: np.ndarray):
         w64 = w.astype(np.float64)
-        self.t = w64 / np.sum(np.sort(w64))  # sorted summation: relative error ~1e-16
+        # sorted summation (relative error ~1e-16); above 2^26 numpy's pairwise
+        # FP64 sum (relative error ~1e-15, far below the errors measured)
+        total = np.sum(np.sort(w64)) if w64.size <= (1 << 26) else np.sum(w64)
+        self.t = w64 / total
         self.nz = self.t > 0
 
     def error(self, table: np.ndarray) -> dict:
@@ -163,7 +178,18 @@ def main() -> int:
                     help="only the headline build (no precision study, ITS, spike)")
     ap.add_argument("--no-tune", action="store_true",
                     help="default launch geometry instead of the autotuner's choice")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated sections to run besides the headline build, of "
+                         + ",".join(SECTIONS))
     args = ap.parse_args()
+    if args.only is not None:
+        sections = {x for x in args.only.split(",") if x}
+        if not sections <= set(SECTIONS):
+            ap.error(f"--only takes sections of {','.join(SECTIONS)}")
+    else:
+        sections = set(SECTIONS) if not args.no_extras else {"batched"}
+        if args.no_batched:
+            sections.discard("batched")
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path has no multi-GPU form here); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -203,6 +229,12 @@ def main() -> int:
     gbs = B_build / (ms_step * 1e-3) / 1e9
     log(f"PSA build N=2^{args.log2n} ({PREC_NAMES[args.precision]}): {ms_step:.4f} ms/step, "
         f"{gbs:.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of peak)")
+    # ---- the same builds with the caches flushed before each one ------------
+    fn_ = 1 << 27  # 512 MiB of floats
+    flush = (wrs.empty(fn_, np.float32), wrs.empty(fn_, np.float32), fn_)
+    cold_ms = time_cold(lambda: plan.psa_build(w, n, table, stream=st), st, wrs, args.steps, flush)
+    cold_gbs = B_build / (cold_ms * 1e-3) / 1e9
+    log(f"PSA build, 512 MiB flush before each: {cold_ms:.4f} ms, {cold_gbs:.1f} GB/s")
 
     # ---- per-kernel durations: K more builds, HIP events around each kernel
     plan.set_timing(True)
@@ -232,7 +264,7 @@ def main() -> int:
            "merge_pack": (12 * n + 8 * H) if direct else 16 * n}
     # kernel instances (template arguments: precision, forms, merge shape) as
     # rocprofv3 names them
-    mt = t_.get("merge_tile") or 2048
+    mt = t_.get("merge_tile") or 1024
     mp = t_.get("merge_per") or 4
     d_ = "true" if direct else "false"
     kname = {"reduce": f"k_reduce_final<{'false' if p == 0 else 'true'}>",
@@ -290,7 +322,7 @@ def main() -> int:
         err = implied.error(table.numpy())
         log(f"implied distribution vs FP64: max_abs {err['max_abs']:.3e} max_rel {err['max_rel']:.3e}")
     study = None
-    if not args.no_extras:
+    if "precision" in sections:
         study = []
         for prec in range(4):
             pp = plan if prec == args.precision else wrs.Plan(n, args.split_size, prec, tune=tune)
@@ -308,9 +340,93 @@ def main() -> int:
             log(f"precision {row}")
             del tt
 
+    # ---- N = 2^20 U(0,1) weights (seed 0): build + S = 2^20 alias samples ----
+    c1 = None
+    if "c1" in sections:
+        n1 = s1 = 1 << 20
+        w1 = wrs.empty(n1, np.float32)
+        wrs.generate(wrs.GEN_UNIFORM01, n1, 0, w1, stream=st)
+        st.synchronize()
+        t1 = None
+        if not args.no_tune:
+            t1, _ = wrs.autotune_psa(w1, n1, precision=args.precision,
+                                     split_size=args.split_size, stream=st)
+        p1 = wrs.Plan(n1, args.split_size, args.precision, tune=t1)
+        tb1 = wrs.empty(n1, wrs.ALIAS_DTYPE)
+        for _ in range(20):
+            p1.psa_build(w1, n1, tb1, stream=st)
+        b1_ms = time_it(lambda: p1.psa_build(w1, n1, tb1, stream=st), st, wrs, 20)
+        b1_cold = time_cold(lambda: p1.psa_build(w1, n1, tb1, stream=st), st, wrs, 9, flush)
+        o1 = wrs.empty(s1, np.uint32)
+        wrs.alias_sample(tb1, n1, s1, 0, o1, stream=st)
+        a1_ms = time_it(lambda: wrs.alias_sample(tb1, n1, s1, 0, o1, stream=st), st, wrs, 20)
+        h1 = wrs.empty(n1 + 1, np.uint64)
+        wrs.histogram(o1, s1, n1, h1, stream=st)
+        st1 = sample_statistics(p1.sample_stats(w1, n1, h1, s1, stream=st))
+        e1 = Implied(w1.numpy()).error(tb1.numpy())
+        bb1 = 32 * n1 + 12 * p1.split_count(n1)
+        c1 = dict(n=n1, samples=s1, tuning=t1, build_ms=round(b1_ms, 4),
+                  build_gbs=round(bb1 / (b1_ms * 1e-3) / 1e9, 1),
+                  cold_build_ms=round(b1_cold, 4), sample_ms=round(a1_ms, 4),
+                  samples_per_s=round(s1 / (a1_ms * 1e-3), 1),
+                  max_abs_error_vs_fp64=e1["max_abs"], max_rel_error_vs_fp64=e1["max_rel"],
+                  sample_statistics=st1)
+        log(f"c1: {c1}")
+        del w1, tb1, o1, h1, p1
+
+    # ---- normalised scan of the headline weights, FP32 and compensated -----
+    scan26 = None
+    if "scan26" in sections:
+        scan26 = []
+        hi26 = wrs.empty(n, np.float32)
+        lo26 = wrs.empty(n, np.float32)
+        w64 = w_host.astype(np.float64)
+        exact26 = np.cumsum(w64) / np.sum(np.sort(w64))
+        del w64
+        for prec in (0, 2):
+            pl = wrs.Plan(n, args.split_size, prec)
+            lo_ = lo26 if prec >= 2 else None
+            pl.its_build(w, n, hi26, lo_, stream=st)
+            ms = time_it(lambda: pl.its_build(w, n, hi26, lo_, stream=st), st, wrs, 10)
+            got = hi26.numpy().astype(np.float64)
+            if lo_ is not None:
+                got += lo26.numpy().astype(np.float64)
+            scan26.append(dict(mode=PREC_NAMES[prec], n=n, ms=round(ms, 4),
+                               gbs_8N=round(8 * n / (ms * 1e-3) / 1e9, 1),
+                               cdf_max_abs_err=float(np.abs(got - exact26).max())))
+            log(f"scan 2^{args.log2n}: {scan26[-1]}")
+            del pl, got
+        del hi26, lo26, exact26
+
+    # ---- the reference's split + pack strategies on the headline weights ---
+    split_pack = None
+    if "split_pack" in sections:
+        split_pack = []
+        tsp_ = wrs.empty(n, wrs.ALIAS_DTYPE)
+        for form, split in ((3, 2), (3, 4), (3, 8), (3, 32), (3, 128), (3, 1024),
+                            (4, 32), (4, 128), (4, 1024)):
+            pl = wrs.Plan(n, split, args.precision, tune=dict(direct=form))
+            pl.psa_build(w, n, tsp_, stream=st)  # allocates its prefix and split arrays
+            ms = time_it(lambda: pl.psa_build(w, n, tsp_, stream=st), st, wrs, 5)
+            pl.set_timing(True)
+            pl.psa_build(w, n, tsp_, stream=st)
+            ph = dict(zip(wrs.Plan.PHASES, pl.phase_times()))
+            pl.set_timing(False)
+            K_ = pl.split_count(n)
+            row = dict(form="serial" if form == 3 else "inline", split_size=split, ms=round(ms, 4),
+                       gbs=round((32 * n + 12 * K_) / (ms * 1e-3) / 1e9, 1),
+                       partition_ms=round(ph["tile_aggregate"] + ph["tile_scan"]
+                                          + ph["partition_emit"], 4),
+                       split_ms=round(ph["boundary_splits"], 4),
+                       pack_ms=round(ph["merge_pack"], 4))
+            split_pack.append(row)
+            log(f"split+pack {row}")
+            del pl
+        del tsp_
+
     # ---- ITS: normalised scan + binary-search sampling ----------------------
     its = None
-    if not args.no_extras:
+    if "its" in sections:
         ni = 1 << 24
         wi = wrs.empty(ni, np.float32)
         wrs.generate(wrs.GEN_LOGNORMAL, ni, 1, wi, a=0.0, b=2.0, stream=st)
@@ -347,7 +463,7 @@ def main() -> int:
 
     # ---- spike mixture at N = 2^28, S = 2^30 --------------------------------
     spike = None
-    if not args.no_extras:
+    if "spike" in sections:
         ns, ss = 1 << 28, 1 << 30
         wsp = wrs.empty(ns, np.float32)
         wrs.generate(wrs.GEN_SPIKE, ns, 4, wsp, stream=st)
@@ -363,16 +479,33 @@ def main() -> int:
         spstat = sample_statistics(splan.sample_stats(wsp, ns, hsp, ss, stream=st))
         del hsp
         b_sp = 32 * ns + 12 * splan.split_count(ns)
+        del osp
+        # the tuned geometry, and the implied distribution's error vs FP64
+        sp_tune, sp_tms = None, None
+        if not args.no_tune:
+            sp_tune, _ = wrs.autotune_psa(wsp, ns, precision=args.precision,
+                                          split_size=args.split_size, stream=st)
+            tplan = wrs.Plan(ns, args.split_size, args.precision, tune=sp_tune)
+            tplan.psa_build(wsp, ns, tsp, stream=st)
+            sp_tms = time_it(lambda: tplan.psa_build(wsp, ns, tsp, stream=st), st, wrs, 5)
+            del tplan
+        esp = None if args.no_error else Implied(wsp.numpy()).error(tsp.numpy())
         spike = dict(n=ns, build_ms=round(sp_ms, 4), build_gbs=round(b_sp / (sp_ms * 1e-3) / 1e9, 1),
                      frac_hbm_peak=round(b_sp / (sp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     tuning=sp_tune,
+                     tuned_build_ms=None if sp_tms is None else round(sp_tms, 4),
+                     tuned_frac_hbm_peak=None if sp_tms is None else
+                     round(b_sp / (sp_tms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     max_abs_error_vs_fp64=None if esp is None else esp["max_abs"],
+                     max_rel_error_vs_fp64=None if esp is None else esp["max_rel"],
                      samples=ss, sample_ms=round(sps_ms, 3),
                      samples_per_s=round(ss / (sps_ms * 1e-3), 1), sample_statistics=spstat)
         log(f"spike: {spike}")
-        del wsp, tsp, osp, splan
+        del wsp, tsp, splan
 
     # ---- batched 4096-distribution build + sample --------------------------
     batched = None
-    if not args.no_batched:
+    if "batched" in sections:
         nseg, per = 4096, 64
         sizes = wrs.segment_sizes(3, nseg)
         off = wrs.segment_offsets(sizes)
@@ -426,12 +559,10 @@ def main() -> int:
     # (S / (t_build + t_sample), its_sampling_throughput_with_construction), ITS
     # binary search and cutpoint sampling (S / t_sample, its_sampling_throughput)
     sweep = None
-    if not args.no_extras:
+    if "sweep" in sections:
         sweep = []
         Ss = 10_000_000
         osw = wrs.empty(Ss, np.uint32)
-        fn_ = 1 << 27  # 512 MiB of floats
-        flush = (wrs.empty(fn_, np.float32), wrs.empty(fn_, np.float32), fn_)
         for nn in (100_000, 1_000_000, 10_000_000, 100_000_000):
             wsw = wrs.empty(nn, np.float32)
             wrs.generate(wrs.GEN_UNIFORM01, nn, 0, wsw, stream=st)
@@ -478,7 +609,7 @@ def main() -> int:
             sweep.append(row)
             log(f"sweep: {row}")
             del wsw, tsw, hi_, lo_, gd_, pl
-        del osw, flush
+        del osw
 
     # ---- CPU baseline: the reference CPU path restated (oracle) ------------
     cpu = None
@@ -488,7 +619,7 @@ def main() -> int:
 
         O.psa_alias_table_f32(w_host[: 1 << 20], (1 << 20) // args.split_size, True)
         times = []
-        for _ in range(2):
+        for _ in range(5):
             t0 = time.perf_counter()
             O.psa_alias_table_f32(w_host, K, True)
             times.append(time.perf_counter() - t0)
@@ -496,7 +627,7 @@ def main() -> int:
         cpu = dict(value=round(B_build / t_cpu / 1e9, 3), unit="GB/s", cores=1, kind="port",
                    sample=f"full workload: PSA build of the same N=2^{args.log2n} Zipf weights by "
                           f"the restated reference CPU path (float, 1e-4 scale, reference split), "
-                          f"best of 2 after a warm-up, {t_cpu * 1e3:.0f} ms; host {cpu_model()}, "
+                          f"best of 5 after a warm-up, {t_cpu * 1e3:.0f} ms; host {cpu_model()}, "
                           f"{os.cpu_count()} logical CPUs visible")
         log(f"cpu baseline: {cpu}")
 
@@ -525,6 +656,9 @@ def main() -> int:
                                       e["merge_per"], round(e["ms"], 4)] for e in tune_log],
         },
         "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+        "cold": {"ms_per_step": round(cold_ms, 5), "gbs": round(cold_gbs, 2),
+                 "frac_hbm_peak": round(cold_gbs / HBM_PEAK_GBS, 4),
+                 "flush": "512 MiB copy before every build, median of the steps"},
         "lower_bound_12N_gbs": round(12 * n / (ms_step * 1e-3) / 1e9, 2),
         "roofline": {"bound": "hbm", "kernel": kname[dom], "achieved": round(dom_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
@@ -540,6 +674,9 @@ def main() -> int:
         "max_abs_error_vs_fp64": None if err is None else err["max_abs"],
         "max_rel_error_vs_fp64": None if err is None else err["max_rel"],
         "precision_study": study,
+        "c1": c1,
+        "scan_2e26": scan26,
+        "split_pack_baselines": split_pack,
         "its": its,
         "spike": spike,
         "batched": batched,
