@@ -358,7 +358,9 @@ wrs_status wrs_batch_plan_status(wrs_batch_plan* plan, wrs_stream_t stream);
 /* Failure injection for tests of the error path: wait_polls bounds every wait
  * of the plan's builds (0: the default), hooks (bits) withhold a publication
  * so that a wait must give up: 1 the first block's total, 2 the first record
- * of the tile-aggregate scan.  Not for production use. */
+ * of the tile-aggregate scan, 4 the first block's total published under the
+ * previous build's tag (as a late store of that build would land), which the
+ * reader must not take.  Not for production use. */
 wrs_status wrs_plan_set_test_hooks(wrs_plan* plan, uint32_t wait_polls, uint32_t hooks);
 
 /* ---- introspection ---- */

@@ -37,7 +37,7 @@ struct Params {
     uint32_t error;          // kErr* bits: a bounded wait of this build gave up
     uint32_t wait_polls;     // polls of a bounded wait before it gives up
     uint32_t hooks;          // kHook* failure injection (tests)
-    uint32_t pad;
+    uint32_t epoch;          // builds finalized with these Params (the reduce's word tags)
 };
 static_assert(sizeof(Params) == 96, "Params layout");
 
@@ -51,6 +51,8 @@ constexpr uint32_t kDefaultWaitPolls = 1u << 22;
 // failure injection for tests of the error path (the plan's test hooks)
 constexpr uint32_t kHookSkipPartial = 1u;  // reduce block 0 does not publish its partial
 constexpr uint32_t kHookSkipSpine = 2u;    // the tile-scan spine's block 0 publishes nothing
+constexpr uint32_t kHookStaleTotal = 4u;   // reduce block 0 publishes its total under the
+                                           // previous build's tag (a late store of that build)
 
 
 struct AliasEntry {

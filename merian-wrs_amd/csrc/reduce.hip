@@ -119,7 +119,7 @@ __device__ __forceinline__ void finalize_acc(AccT<FP64> acc, uint64_t n, int has
         p.error = error;
         p.wait_polls = wait_polls;
         p.hooks = hooks;
-        p.pad = 0;
+        p.epoch = prm->epoch + 1u;  // the next build's tag
         *prm = p;
     }
 }
@@ -139,7 +139,12 @@ __device__ __forceinline__ void finalize_block(const double* __restrict__ partia
 // Block totals, then the build scalars by the last block to finish.  red holds
 // kMaxPartials total words, the sentinel ~0 between launches (a total is a
 // non-negative double or a NaN from a float, never all ones), and the arrival
-// counters, zero between launches.  Every block stores its total with an
+// counters, zero between launches.  A total word carries the parity of the
+// build's epoch (Params::epoch, which every build's finalize advances) in its
+// sign bit, which a non-negative total leaves free: a store of an earlier build
+// that lands late -- after that build's reader gave up and restored the
+// sentinel -- has the other parity, and the next reader keeps waiting for its
+// own build's word instead of summing the stale one.  Every block stores its total with an
 // agent-coherent store and then counts itself in its group's counter (groups
 // of kRedGroup blocks); the last of a group counts the group in the top
 // counter, and the last group's last block -- which therefore runs after every
@@ -168,9 +173,14 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const float* __restrict
         states[i] = kSentinel;
     const AccT<FP64> b = block_sum<FP64>(acc);
     uint32_t* cnt = reinterpret_cast<uint32_t*>(red + kMaxPartials);
+    const uint64_t tag = (uint64_t)(prm->epoch & 1u) << 63;  // read before any block finalizes
+    constexpr uint64_t kSign = 1ull << 63;
     if (t == 0) {
-        if (!(blockIdx.x == 0 && (hooks & kHookSkipPartial)))
-            st_word(red + blockIdx.x, (uint64_t)__double_as_longlong((double)b));
+        const uint64_t bits = ((uint64_t)__double_as_longlong((double)b) & ~kSign) | tag;
+        if (blockIdx.x == 0 && (hooks & kHookStaleTotal))
+            st_word(red, bits ^ kSign);  // the previous build's tag
+        else if (!(blockIdx.x == 0 && (hooks & kHookSkipPartial)))
+            st_word(red + blockIdx.x, bits);
         __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the store is issued before the count
         const uint32_t g = blockIdx.x / kRedGroup;
         const uint32_t gsize = min(kRedGroup, G - g * kRedGroup);
@@ -189,22 +199,24 @@ __global__ __launch_bounds__(kBlock) void k_reduce_final(const float* __restrict
         const uint32_t i = (uint32_t)t + (uint32_t)kBlock * k;
         x[k] = i < G ? ld_word(red + i) : 0ull;
     }
+    // a word of this build: not the sentinel, and this build's tag
+    auto mine = [&](uint64_t v) { return v != kSentinel && (v & kSign) == tag; };
     bool missing = false;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const uint32_t i = (uint32_t)t + (uint32_t)kBlock * k;
-        for (uint32_t spin = 0; i < G && x[k] == kSentinel && spin < wait_polls; ++spin) {
+        for (uint32_t spin = 0; i < G && !mine(x[k]) && spin < wait_polls; ++spin) {
             __builtin_amdgcn_s_sleep(1);
             x[k] = ld_word(red + i);
         }
-        missing |= i < G && x[k] == kSentinel;
+        missing |= i < G && !mine(x[k]);
     }
     const bool failed = __syncthreads_or(missing) != 0;
     AccT<FP64> sum = 0;  // the order of finalize_block
 #pragma unroll
     for (int k = 0; k < kPer; ++k)
         if ((uint32_t)t + (uint32_t)kBlock * k < G)
-            sum = sum + (AccT<FP64>)__longlong_as_double((long long)x[k]);
+            sum = sum + (AccT<FP64>)__longlong_as_double((long long)(x[k] & ~kSign));
     finalize_acc<FP64>(sum, n, has_pivot, pivot, prm, failed ? kErrReduceWait : 0u, wait_polls,
                        hooks);
     // every thread restores only the words it read itself

@@ -3,7 +3,9 @@
 A build waits on the device in two places: the last block of the total reads
 the other blocks' totals, and the look-back scans read their predecessors'
 records.  Each wait is bounded; the plan's test hooks withhold one publication
-and shrink the bound so that a wait must give up.  Then the plan reports
+(or publish a block total under the previous build's tag, as a late store of
+that build would land: the reader must not sum it) and shrink the bound so
+that a wait must give up.  Then the plan reports
 WRS_EWAIT, no table row is written (the kernels after the failing wait return
 at once), every kernel finishes (the stream synchronises), and the next
 build on the same plan is correct again.
@@ -13,7 +15,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-HOOK_SKIP_PARTIAL, HOOK_SKIP_SPINE = 1, 2
+HOOK_SKIP_PARTIAL, HOOK_SKIP_SPINE, HOOK_STALE_TOTAL = 1, 2, 4
 
 
 def case(*a):
@@ -21,7 +23,8 @@ def case(*a):
 
 
 @pytest.mark.parametrize("hook,n", [(HOOK_SKIP_PARTIAL, (1 << 20) + 7),
-                                    (HOOK_SKIP_SPINE, (1 << 23) + 5)])
+                                    (HOOK_SKIP_SPINE, (1 << 23) + 5),
+                                    (HOOK_STALE_TOTAL, (1 << 22) + 9)])
 @pytest.mark.parametrize("direct", [1, 2])
 def test_forced_wait_failure_is_reported_and_writes_nothing(wrs, oracle, hook, n, direct):
     case("forced wait failure", hook, n, direct)
