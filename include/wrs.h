@@ -131,6 +131,12 @@ wrs_status wrs_plan_destroy(wrs_plan* plan);
 wrs_status wrs_psa_build(wrs_plan* plan, const float* weights, uint64_t n,
                          wrs_alias_entry* table, wrs_stream_t stream);
 
+/* Changes the launch geometry of an existing plan (e.g. to the autotuner's
+ * choice) without reallocating: the workspace does not depend on it.  No
+ * reference counterpart (the reference fixes its configuration when it
+ * creates a pipeline). */
+wrs_status wrs_plan_set_tuning(wrs_plan* plan, const wrs_tuning* tuning);
+
 /* Device pointers for the intermediate results of a build (any may be NULL). */
 typedef struct {
     double* mean;          /* [1]  mean (pivot and bucket size) */

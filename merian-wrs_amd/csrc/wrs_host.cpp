@@ -694,6 +694,15 @@ wrs_status wrs_alias_sample_batched(const wrs_alias_entry* tables, const uint64_
 const char* wrs_last_error(void) { return g_last_error.c_str(); }
 const char* wrs_version(void) { return "merian-wrs_amd 0.1 (gfx950)"; }
 
+wrs_status wrs_plan_set_tuning(wrs_plan* plan, const wrs_tuning* tuning) {
+    return guarded([&] {
+        require_plan(plan);
+        if (!tuning) throw wrs::Error(WRS_EINVAL, "tuning is null");
+        wrs::require_tuning(*tuning);
+        plan->cfg.tuning = *tuning;
+    });
+}
+
 wrs_status wrs_plan_set_timing(wrs_plan* plan, int enable) {
     return guarded([&] {
         require_plan(plan);

@@ -32,7 +32,7 @@ EXPORTS = [
     "wrs_batch_plan_create",
     "wrs_batch_plan_destroy", "wrs_psa_build_batched", "wrs_alias_sample_batched",
     "wrs_autotune_psa", "wrs_autotune_batched", "wrs_histogram", "wrs_sample_stats",
-    "wrs_last_error", "wrs_version", "wrs_plan_set_timing", "wrs_plan_phase_times",
+    "wrs_last_error", "wrs_version", "wrs_plan_set_timing", "wrs_plan_set_tuning", "wrs_plan_phase_times",
     "wrs_rt_device_count", "wrs_rt_device_info", "wrs_rt_malloc", "wrs_rt_free",
     "wrs_rt_memcpy_htod", "wrs_rt_memcpy_dtoh", "wrs_rt_memcpy_dtod", "wrs_rt_memset",
     "wrs_rt_stream_create", "wrs_rt_stream_destroy", "wrs_rt_stream_sync", "wrs_rt_device_sync",
@@ -129,6 +129,7 @@ def lib():
         "wrs_histogram": [vp, u64, u64, vp, i32, vp],
         "wrs_sample_stats": [vp, vp, u64, vp, u64, f64, vp, vp],
         "wrs_plan_set_timing": [vp, i32],
+        "wrs_plan_set_tuning": [vp, vp],
         "wrs_plan_phase_times": [vp, vp, u32],
         "wrs_plan_status": [vp, vp],
         "wrs_batch_plan_status": [vp, vp],
@@ -363,6 +364,11 @@ class Plan:
 
     def set_timing(self, enable: bool) -> None:
         check(lib().wrs_plan_set_timing(self.handle, int(bool(enable))))
+
+    def set_tuning(self, tune) -> None:
+        """Launch geometry of later builds (a dict as `tune` of the constructor)."""
+        t = tuning(**tune) if isinstance(tune, dict) else (tune if tune is not None else Tuning())
+        check(lib().wrs_plan_set_tuning(self.handle, C.byref(t)))
 
     PHASES = ("reduce", "finalize", "tile_aggregate", "tile_scan", "partition_emit",
               "boundary_splits", "merge_pack")

@@ -132,3 +132,26 @@ def test_autotune_batched_measures_then_caches(wrs, oracle):
     assert (best["merge_tile"], best["merge_per"]) in SHAPES
     best2, log2 = wrs.autotune_batched(off, dw)
     assert best2 == best and log2 == []
+
+
+def test_set_tuning_on_an_existing_plan(wrs, oracle):
+    """A plan switched to another geometry builds what a plan created with it
+    builds (bit for bit), and rejects an unsupported one."""
+    n = (1 << 20) + 5
+    case("set tuning", n)
+    w = oracle.gen_zipf(2, 1.1, n)
+    dw = wrs.to_device(w)
+    plan = wrs.Plan(n)
+    for tune in (dict(merge_tile=2048, merge_per=4, direct=1), dict(direct=2, partition=1),
+                 dict(direct=4), dict()):
+        plan.set_tuning(tune)
+        t0 = wrs.empty(n, wrs.ALIAS_DTYPE)
+        plan.psa_build(dw, n, t0)
+        fresh = wrs.Plan(n, tune=tune)
+        t1 = wrs.empty(n, wrs.ALIAS_DTYPE)
+        fresh.psa_build(dw, n, t1)
+        wrs.synchronize()
+        assert np.array_equal(t0.numpy().view(np.uint64), t1.numpy().view(np.uint64)), tune
+    with pytest.raises(wrs.WrsError) as e:
+        plan.set_tuning(dict(merge_tile=3000))
+    assert e.value.status == wrs.WRS_EINVAL
