@@ -326,8 +326,10 @@ wrs_status wrs_sample_stats(wrs_plan* plan, const float* weights, uint64_t n, co
  * form at the default merge shape, then the pack form (classic, direct, and
  * for a single build the reference's SERIAL and INLINE sweeps at the config's
  * split size), then every supported merge shape with the better forms (none
- * if a sweep form won).  Each candidate is the median of 9 builds after one
- * warm-up; 20 untimed builds of the first candidate run before the search.
+ * if a sweep form won), then the two fastest candidates again, each after 20
+ * builds of its own, keeping the faster.  Each candidate is the median of 9
+ * builds after one warm-up; 20 untimed builds of the first candidate run
+ * before the search.
  * Results are cached per process under a key of device, precision and problem
  * shape (n and split size, or the segment count and a histogram of log2
  * segment sizes); with the environment variable WRS_TUNE_CACHE naming a file they are

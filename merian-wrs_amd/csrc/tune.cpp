@@ -10,7 +10,9 @@
 //      the reference's serial and inline split + pack at the plan's split size;
 //   3. every supported merge shape (buckets per block x buckets per thread)
 //      with the forms of steps 1 and 2 (none when a sweep form won: it has no
-//      merge shape).
+//      merge shape);
+//   4. the two fastest candidates again, each after kWarm builds of its own,
+//      and the faster of those two.
 // A candidate's time is the median of kReps builds (HIP events on the
 // caller's stream) after one warm-up build; kWarm builds of the first
 // candidate run untimed before the search, so that the first candidate is not
@@ -167,7 +169,14 @@ wrs_tuning with_shape(wrs_tuning t, const wrsk::MergeShape& ms) {
 // default merge shape, the pack form, then every other merge shape with the
 // better forms.
 template <class M>
-wrs_tuning descend(M&& measure, bool direct_rts_only) {
+wrs_tuning descend(M&& measure_raw, bool direct_rts_only) {
+    // every candidate timed, for the confirmation round
+    std::vector<std::pair<float, wrs_tuning>> seen;
+    auto measure = [&](const wrs_tuning& t, int warm = 1) {
+        const float ms = measure_raw(t, warm);
+        seen.emplace_back(ms, t);
+        return ms;
+    };
     wrs_tuning t0 = with_shape(wrs_tuning{}, kShapes[0]);
     t0.direct = WRS_PACK_CLASSIC;
     wrs_tuning best = t0;
@@ -189,6 +198,7 @@ wrs_tuning descend(M&& measure, bool direct_rts_only) {
             best = t2;
         }
     }
+    bool sweep = false;
     if (!direct_rts_only) {  // the reference's sweep forms (single builds only)
         for (const uint32_t f : {WRS_PACK_SERIAL, WRS_PACK_INLINE}) {
             wrs_tuning t3 = best;
@@ -200,16 +210,29 @@ wrs_tuning descend(M&& measure, bool direct_rts_only) {
             }
         }
         // the merge shape does not apply to them
-        if (best.direct == WRS_PACK_SERIAL || best.direct == WRS_PACK_INLINE) return best;
+        sweep = best.direct == WRS_PACK_SERIAL || best.direct == WRS_PACK_INLINE;
     }
-    const wrs_tuning form = best;
-    for (size_t i = 1; i < sizeof(kShapes) / sizeof(kShapes[0]); ++i) {
-        const wrs_tuning t = with_shape(form, kShapes[i]);
-        const float ms = measure(t);
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = t;
+    if (!sweep) {
+        const wrs_tuning form = best;
+        for (size_t i = 1; i < sizeof(kShapes) / sizeof(kShapes[0]); ++i) {
+            const wrs_tuning t = with_shape(form, kShapes[i]);
+            const float ms = measure(t);
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = t;
+            }
         }
+    }
+    // confirmation: the two fastest candidates again, each after kWarm builds
+    // of its own (a candidate timed right after a slower one can read slow:
+    // the first builds after other work run below the steady rate)
+    std::stable_sort(seen.begin(), seen.end(),
+                     [](const auto& x, const auto& y) { return x.first < y.first; });
+    if (seen.size() >= 2) {
+        const wrs_tuning a = seen[0].second, b = seen[1].second;
+        const float ma = measure_raw(a, kWarm);
+        const float mb = measure_raw(b, kWarm);
+        best = mb < ma ? b : a;
     }
     return best;
 }
