@@ -54,6 +54,10 @@ Also reported on the same line:
   cpu_baseline  the reference CPU path (the oracle's single-threaded C
                 restatement) on the same N = 2^26 weights, best of 5.
 
+Every figure beside the headline is timed after at least 10 ms of untimed
+calls of the same operation (the first calls after other work run below the
+steady rate); the headline takes exactly W warm-up builds and K timed ones.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--only SECTION,...] [--no-tune]
 """
 import argparse
@@ -144,6 +148,18 @@ def time_it(fn, st, wrs, reps: int) -> float:
     e1.record(st)
     st.synchronize()
     return e0.elapsed_ms(e1) / reps
+
+
+def time_warm(fn, st, wrs, reps: int, warm_ms: float = 10.0) -> float:
+    """time_it after at least warm_ms of untimed calls of fn (at least two):
+    the first calls after other work run below the steady rate."""
+    t0 = time.perf_counter()
+    calls = 0
+    while calls < 2 or (time.perf_counter() - t0) * 1e3 < warm_ms:
+        fn()
+        st.synchronize()
+        calls += 1
+    return time_it(fn, st, wrs, reps)
 
 
 def time_cold(fn, st, wrs, reps: int, flush) -> float:
@@ -293,7 +309,7 @@ def main() -> int:
     # ---- copy calibration ---------------------------------------------------
     cdst = wrs.empty(n, np.float32)
     wrs.copy_f32(cdst, w, n, stream=st)
-    copy_ms = time_it(lambda: wrs.copy_f32(cdst, w, n, stream=st), st, wrs, 10)
+    copy_ms = time_warm(lambda: wrs.copy_f32(cdst, w, n, stream=st), st, wrs, 10)
     copy_gbs = 2 * 4 * n / (copy_ms * 1e-3) / 1e9
     del cdst
     log(f"copy kernel: {copy_gbs:.1f} GB/s")
@@ -307,7 +323,7 @@ def main() -> int:
         ctr[0] += 1
         wrs.alias_sample(table, n, S, 12345, smp, counter_base=ctr[0] * S, stream=st)
 
-    sample_ms = time_it(sample_once, st, wrs, 5)
+    sample_ms = time_warm(sample_once, st, wrs, 5)
     samples_per_s = S / (sample_ms * 1e-3)
     log(f"alias sampling S=2^{args.log2s}: {sample_ms:.4f} ms, {samples_per_s / 1e9:.2f} Gsamples/s")
     # statistics of the last S samples, on the device (histogram, pooled chi-square, RMSE)
@@ -333,7 +349,7 @@ def main() -> int:
             # warm-up: the device idled during the host-side error computation
             for _ in range(20):
                 pp.psa_build(w, n, tt, stream=st)
-            ms = time_it(lambda: pp.psa_build(w, n, tt, stream=st), st, wrs, 10)
+            ms = time_warm(lambda: pp.psa_build(w, n, tt, stream=st), st, wrs, 10)
             row = dict(mode=PREC_NAMES[prec], ms=round(ms, 4),
                        gbs=round(B_build / (ms * 1e-3) / 1e9, 1))
             if implied is not None:
@@ -358,11 +374,11 @@ def main() -> int:
         tb1 = wrs.empty(n1, wrs.ALIAS_DTYPE)
         for _ in range(20):
             p1.psa_build(w1, n1, tb1, stream=st)
-        b1_ms = time_it(lambda: p1.psa_build(w1, n1, tb1, stream=st), st, wrs, 20)
+        b1_ms = time_warm(lambda: p1.psa_build(w1, n1, tb1, stream=st), st, wrs, 20)
         b1_cold = time_cold(lambda: p1.psa_build(w1, n1, tb1, stream=st), st, wrs, 9, flush)
         o1 = wrs.empty(s1, np.uint32)
         wrs.alias_sample(tb1, n1, s1, 0, o1, stream=st)
-        a1_ms = time_it(lambda: wrs.alias_sample(tb1, n1, s1, 0, o1, stream=st), st, wrs, 20)
+        a1_ms = time_warm(lambda: wrs.alias_sample(tb1, n1, s1, 0, o1, stream=st), st, wrs, 20)
         h1 = wrs.empty(n1 + 1, np.uint64)
         wrs.histogram(o1, s1, n1, h1, stream=st)
         st1 = sample_statistics(p1.sample_stats(w1, n1, h1, s1, stream=st))
@@ -390,7 +406,7 @@ def main() -> int:
             pl = wrs.Plan(n, args.split_size, prec)
             lo_ = lo26 if prec >= 2 else None
             pl.its_build(w, n, hi26, lo_, stream=st)
-            ms = time_it(lambda: pl.its_build(w, n, hi26, lo_, stream=st), st, wrs, 10)
+            ms = time_warm(lambda: pl.its_build(w, n, hi26, lo_, stream=st), st, wrs, 10)
             got = hi26.numpy().astype(np.float64)
             if lo_ is not None:
                 got += lo26.numpy().astype(np.float64)
@@ -410,7 +426,7 @@ def main() -> int:
                             (4, 32), (4, 128), (4, 1024)):
             pl = wrs.Plan(n, split, args.precision, tune=dict(direct=form))
             pl.psa_build(w, n, tsp_, stream=st)  # allocates its prefix and split arrays
-            ms = time_it(lambda: pl.psa_build(w, n, tsp_, stream=st), st, wrs, 5)
+            ms = time_warm(lambda: pl.psa_build(w, n, tsp_, stream=st), st, wrs, 5)
             pl.set_timing(True)
             pl.psa_build(w, n, tsp_, stream=st)
             ph = dict(zip(wrs.Plan.PHASES, pl.phase_times()))
@@ -437,17 +453,17 @@ def main() -> int:
         hi = wrs.empty(ni, np.float32)
         lo = wrs.empty(ni, np.float32)
         iplan.its_build(wi, ni, hi, lo, stream=st)
-        scan_ms = time_it(lambda: iplan.its_build(wi, ni, hi, lo, stream=st), st, wrs, 10)
+        scan_ms = time_warm(lambda: iplan.its_build(wi, ni, hi, lo, stream=st), st, wrs, 10)
         outi = wrs.empty(ni, np.uint32)
         wrs.its_sample(hi, lo, ni, ni, 7, outi, stream=st)
-        its_ms = time_it(lambda: wrs.its_sample(hi, lo, ni, ni, 7, outi, stream=st), st, wrs, 5)
+        its_ms = time_warm(lambda: wrs.its_sample(hi, lo, ni, ni, 7, outi, stream=st), st, wrs, 5)
         # cutpoint: guide table (m = N entries) over the same CDF, guided samples
         gd = wrs.empty(ni + 1, np.uint32)
         wrs.cutpoint_build(hi, lo, ni, gd, ni, stream=st)
-        guide_ms = time_it(lambda: wrs.cutpoint_build(hi, lo, ni, gd, ni, stream=st), st, wrs, 5)
+        guide_ms = time_warm(lambda: wrs.cutpoint_build(hi, lo, ni, gd, ni, stream=st), st, wrs, 5)
         outc = wrs.empty(ni, np.uint32)
         wrs.cutpoint_sample(hi, lo, ni, gd, ni, ni, 7, outc, stream=st)
-        cut_ms = time_it(lambda: wrs.cutpoint_sample(hi, lo, ni, gd, ni, ni, 7, outc, stream=st),
+        cut_ms = time_warm(lambda: wrs.cutpoint_sample(hi, lo, ni, gd, ni, ni, 7, outc, stream=st),
                          st, wrs, 5)
         st.synchronize()
         cut_same = bool(np.array_equal(outc.numpy(), outi.numpy()))
@@ -473,10 +489,10 @@ def main() -> int:
         splan = wrs.Plan(ns, args.split_size, args.precision)
         tsp = wrs.empty(ns, wrs.ALIAS_DTYPE)
         splan.psa_build(wsp, ns, tsp, stream=st)
-        sp_ms = time_it(lambda: splan.psa_build(wsp, ns, tsp, stream=st), st, wrs, 5)
+        sp_ms = time_warm(lambda: splan.psa_build(wsp, ns, tsp, stream=st), st, wrs, 5)
         osp = wrs.empty(ss, np.uint32)
         wrs.alias_sample(tsp, ns, ss, 99, osp, stream=st)
-        sps_ms = time_it(lambda: wrs.alias_sample(tsp, ns, ss, 99, osp, stream=st), st, wrs, 3)
+        sps_ms = time_warm(lambda: wrs.alias_sample(tsp, ns, ss, 99, osp, stream=st), st, wrs, 3)
         hsp = wrs.empty(ns + 1, np.uint64)
         wrs.histogram(osp, ss, ns, hsp, stream=st)
         spstat = sample_statistics(splan.sample_stats(wsp, ns, hsp, ss, stream=st))
@@ -490,7 +506,7 @@ def main() -> int:
                                           split_size=args.split_size, stream=st)
             tplan = wrs.Plan(ns, args.split_size, args.precision, tune=sp_tune)
             tplan.psa_build(wsp, ns, tsp, stream=st)
-            sp_tms = time_it(lambda: tplan.psa_build(wsp, ns, tsp, stream=st), st, wrs, 5)
+            sp_tms = time_warm(lambda: tplan.psa_build(wsp, ns, tsp, stream=st), st, wrs, 5)
             del tplan
         esp = None if args.no_error else Implied(wsp.numpy()).error(tsp.numpy())
         spike = dict(n=ns, build_ms=round(sp_ms, 4), build_gbs=round(b_sp / (sp_ms * 1e-3) / 1e9, 1),
@@ -577,20 +593,20 @@ def main() -> int:
             pl = wrs.Plan(nn, args.split_size, args.precision, tune=tsw_tune)
             tsw = wrs.empty(nn, wrs.ALIAS_DTYPE)
             pl.psa_build(wsw, nn, tsw, stream=st)
-            b_ms = time_it(lambda: pl.psa_build(wsw, nn, tsw, stream=st), st, wrs, 5)
-            a_ms = time_it(lambda: wrs.alias_sample(tsw, nn, Ss, 5, osw, stream=st), st, wrs, 5)
+            b_ms = time_warm(lambda: pl.psa_build(wsw, nn, tsw, stream=st), st, wrs, 5)
+            a_ms = time_warm(lambda: wrs.alias_sample(tsw, nn, Ss, 5, osw, stream=st), st, wrs, 5)
             hi_ = wrs.empty(nn, np.float32)
             lo_ = wrs.empty(nn, np.float32)
             pl.its_build(wsw, nn, hi_, lo_, stream=st)
-            i_ms = time_it(lambda: wrs.its_sample(hi_, lo_, nn, Ss, 5, osw, stream=st), st, wrs, 3)
+            i_ms = time_warm(lambda: wrs.its_sample(hi_, lo_, nn, Ss, 5, osw, stream=st), st, wrs, 3)
             gd_ = wrs.empty(nn + 1, np.uint32)
             wrs.cutpoint_build(hi_, lo_, nn, gd_, nn, stream=st)
-            c_ms = time_it(lambda: wrs.cutpoint_sample(hi_, lo_, nn, gd_, nn, Ss, 5, osw, stream=st),
+            c_ms = time_warm(lambda: wrs.cutpoint_sample(hi_, lo_, nn, gd_, nn, Ss, 5, osw, stream=st),
                            st, wrs, 3)
             # windowed samplers, window 128 (the reference's cooperative size)
-            aw_ms = time_it(lambda: wrs.alias_sample_window(tsw, nn, Ss, 5, 128, osw, stream=st),
+            aw_ms = time_warm(lambda: wrs.alias_sample_window(tsw, nn, Ss, 5, 128, osw, stream=st),
                             st, wrs, 5)
-            iw_ms = time_it(lambda: wrs.its_sample_window(hi_, lo_, nn, Ss, 5, 128, osw, stream=st),
+            iw_ms = time_warm(lambda: wrs.its_sample_window(hi_, lo_, nn, Ss, 5, 128, osw, stream=st),
                             st, wrs, 3)
             # the same with the caches flushed before every call
             bc = time_cold(lambda: pl.psa_build(wsw, nn, tsw, stream=st), st, wrs, 5, flush)
