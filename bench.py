@@ -49,8 +49,9 @@ Also reported on the same line:
                 compensated modes, GB/s of 8N bytes;
   split_pack    the reference's own split + pack strategies on the headline
                 weights, as pack forms: SERIAL (split records + one thread per
-                split interval) at split sizes 2 .. 1024 and INLINE, with the time
-                of their split and pack kernels;
+                split interval) and SUBGROUP (the same records, 16 lanes per
+                interval) at split sizes 2 .. 1024, and INLINE, with the time of
+                their split and pack kernels;
   cpu_baseline  the reference CPU path (the oracle's single-threaded C
                 restatement) on the same N = 2^26 weights, best of 5.
 
@@ -423,6 +424,7 @@ def main() -> int:
         split_pack = []
         tsp_ = wrs.empty(n, wrs.ALIAS_DTYPE)
         for form, split in ((3, 2), (3, 4), (3, 8), (3, 32), (3, 128), (3, 1024),
+                            (5, 2), (5, 8), (5, 32), (5, 128), (5, 1024),
                             (4, 32), (4, 128), (4, 1024)):
             pl = wrs.Plan(n, split, args.precision, tune=dict(direct=form))
             pl.psa_build(w, n, tsp_, stream=st)  # allocates its prefix and split arrays
@@ -432,7 +434,8 @@ def main() -> int:
             ph = dict(zip(wrs.Plan.PHASES, pl.phase_times()))
             pl.set_timing(False)
             K_ = pl.split_count(n)
-            row = dict(form="serial" if form == 3 else "inline", split_size=split, ms=round(ms, 4),
+            row = dict(form={3: "serial", 4: "inline", 5: "subgroup"}[form], split_size=split,
+                       ms=round(ms, 4),
                        gbs=round((32 * n + 12 * K_) / (ms * 1e-3) / 1e9, 1),
                        partition_ms=round(ph["tile_aggregate"] + ph["tile_scan"]
                                           + ph["partition_emit"], 4),

@@ -78,33 +78,41 @@ typedef struct {
                                writes the heavies and every 64th light's position only and
                                the merge pack reads the lights from the weights; CLASSIC it
                                writes every index and weight in partition order (12N more
-                               bytes; same table).  SERIAL and INLINE are the reference's
+                               bytes; same table).  SERIAL, SUBGROUP and INLINE are the reference's
                                own strategies, kept as baselines: K = ceil(N / split_size)
-                               split points and one thread sweeping each split interval
-                               serially -- SERIAL writes the K split records first
-                               (ScalarSplit + ScalarPack, split/scalar/shader.comp:111-133,
-                               pack/scalar/float.comp:60-180), INLINE finds them in the
-                               sweeping kernel (InlineSplitPack, splitpack/inline/
-                               shader.comp:225-270).  Their tables are identical to each
-                               other and differ from the merge forms' by the rounding of
-                               the sweep's running residual; the first build of such a plan
-                               allocates the full prefix arrays (8*max_n bytes, 4*max_n in
-                               WRS_PREC_FP32 and WRS_PREC_FP64_CARRY) and, for SERIAL,
-                               16*K split records.  Debug builds use CLASSIC; batch plans
-                               build DIRECT with partition 0 only (SERIAL and INLINE are
-                               rejected there). */
+                               split points and a serial sweep of each split interval --
+                               SERIAL writes the K split records first and sweeps each
+                               interval with one thread (ScalarSplit + ScalarPack,
+                               split/scalar/shader.comp:111-133, pack/scalar/float.comp:
+                               60-180), SUBGROUP sweeps the same records with 16 lanes per
+                               interval that load its heavies and lights a chunk at a time
+                               (ScalarSplit + SubgroupPack, pack/subgroup/shader.comp:
+                               68-246), INLINE finds the split points in the sweeping
+                               kernel (InlineSplitPack, splitpack/inline/shader.comp:
+                               225-270).  SERIAL and INLINE give identical tables;
+                               SUBGROUP's residual steps as r + (w - mean), as
+                               SubgroupPack's does, where the others step (r + w) - mean;
+                               all differ from the merge forms' tables by the rounding of
+                               the sweep's running residual.  The first build of such a
+                               plan allocates the full prefix arrays (8*max_n bytes,
+                               4*max_n in WRS_PREC_FP32 and WRS_PREC_FP64_CARRY) and, for
+                               SERIAL and SUBGROUP, 16*K split records.  Debug builds use
+                               CLASSIC; batch plans build DIRECT with partition 0 only
+                               (the sweep forms are rejected there). */
 } wrs_tuning;
 #define WRS_PACK_DEFAULT 0u /* the direct form */
 #define WRS_PACK_DIRECT 1u
 #define WRS_PACK_CLASSIC 2u
 #define WRS_PACK_SERIAL 3u
 #define WRS_PACK_INLINE 4u
+#define WRS_PACK_SUBGROUP 5u
 
 /* Build configuration.  Reference: PSAConfig (src/device/wrs/alias/psa/PSA.hpp:19-41),
  * whose splitSize sets the split granularity. */
 typedef struct {
     uint32_t split_size;     /* split granularity (K = ceil(N/split_size)) of the debug split
-                                output and of the SERIAL and INLINE pack forms; 0 -> 32 */
+                                output and of the SERIAL, SUBGROUP and INLINE pack forms;
+                                0 -> 32 */
     wrs_precision precision; /* default WRS_PREC_COMPENSATED */
     wrs_tuning tuning;       /* all zero: default geometry */
 } wrs_config;
@@ -324,8 +332,8 @@ wrs_status wrs_sample_stats(wrs_plan* plan, const float* weights, uint64_t n, co
  * Times candidate geometries (wrs_tuning) on the caller's own data with HIP
  * events and returns the fastest, by coordinate descent: the prefix-partition
  * form at the default merge shape, then the pack form (classic, direct, and
- * for a single build the reference's SERIAL and INLINE sweeps at the config's
- * split size), then every supported merge shape with the better forms (none
+ * for a single build the reference's SERIAL, SUBGROUP and INLINE sweeps at the
+ * config's split size), then every supported merge shape with the better forms (none
  * if a sweep form won), then the two fastest candidates again, each after 20
  * builds of its own, keeping the faster.  Each candidate is the median of 9
  * builds after one warm-up; 20 untimed builds of the first candidate run

@@ -100,6 +100,7 @@ constexpr uint32_t kRedGroup = 32;  // blocks per first-level arrival counter
 constexpr uint32_t kRedCounters = kMaxPartials / kRedGroup + 1;
 constexpr uint32_t kRedWords = kMaxPartials + (kRedCounters + 1) / 2;
 constexpr int kSplitBlock = 256;            // debug split kernel block size
+constexpr int kPackLanes = 16;              // lanes sharing one interval in the subgroup pack
 constexpr uint32_t kSample = 64;            // partition prefix sample stride (ranks)
 // Merge-pack geometry: `tile` buckets per block (a multiple of kSample),
 // `per` buckets per thread (tile / per threads).  Supported shapes:
@@ -188,6 +189,10 @@ hipError_t launch_splits(uint64_t n, uint32_t K, int prec, const Params* prm, co
 hipError_t launch_pack_serial(uint64_t n, uint32_t K, int prec, const Params* prm,
                               const SplitRec* splits, const uint32_t* idx, const float* pw,
                               AliasEntry* table, hipStream_t s);
+// subgroup form: the serial form's split records, kPackLanes lanes per interval
+hipError_t launch_pack_subgroup(uint64_t n, uint32_t K, int prec, const Params* prm,
+                                const SplitRec* splits, const uint32_t* idx, const float* pw,
+                                AliasEntry* table, hipStream_t s);
 hipError_t launch_splitpack_inline(uint64_t n, uint32_t K, int prec, const Params* prm,
                                    const float* hi, const float* lo, const uint32_t* idx,
                                    const float* pw, AliasEntry* table, hipStream_t s);

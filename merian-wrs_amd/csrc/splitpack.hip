@@ -18,15 +18,20 @@
 //    ScalarPack (split/scalar/shader.comp:111-133, pack/scalar/float.comp:
 //    60-180): k_splits writes the K split records, then one thread per split
 //    interval sweeps it (k_pack_serial);
+//  - the subgroup form (WRS_PACK_SUBGROUP): ScalarSplit + the reference's
+//    SubgroupPack (pack/subgroup/shader.comp:68-246): the same split records,
+//    each interval swept by a group of lanes that load its heavies and
+//    lights a chunk at a time (k_pack_subgroup);
 //  - the inline form (WRS_PACK_INLINE): the reference's InlineSplitPack
 //    (splitpack/inline/shader.comp:225-270), splits and sweeps in one kernel
 //    with no split array: a block finds 257 consecutive split points, keeps
 //    them in LDS, and its 256 threads sweep the 256 intervals between them.
-// Both sweep forms run the same sweep over the same split points, so their
-// tables are identical; they differ from the merge pack's only by the rounding
-// of the sweep's running residual (the merge computes every residual from
-// exact fixed-point prefix sums).  They are baselines: each thread walks
-// split_size buckets one dependent step at a time.
+// The serial and inline forms run the same sweep over the same split points,
+// so their tables are identical; the subgroup form's residual steps in
+// SubgroupPack's rounding order.  All differ from the merge pack's tables only
+// by the rounding of the sweep's running residual (the merge computes every
+// residual from exact fixed-point prefix sums).  They are baselines: each
+// interval is walked split_size buckets one dependent step at a time.
 #include "device_common.hpp"
 #include "internal.hpp"
 
@@ -210,6 +215,112 @@ __global__ __launch_bounds__(kSplitBlock) void k_pack_serial(
     sweep_interval<PREC>(a.i, b.i, a.j, b.j, (T)a.spill, (T)prm->mean, N, H, idx, pw, table);
 }
 
+// Subgroup form, pack pass: the reference's SubgroupPack (pack/subgroup/
+// shader.comp:68-246) over the K records k_splits wrote.  kPackLanes lanes
+// share interval k: each holds one of the next kPackLanes heavies and one of
+// the next kPackLanes lights (index and weight, loaded a chunk at a time with
+// consecutive lanes on consecutive partition slots), all of them step the
+// interval's sweep in lockstep and read the element a step takes from the
+// lane that holds it, and the lane holding an element keeps its row until the
+// chunk is used up, when the chunk's rows are stored together.  A heavy
+// completed by the first heavy of a new chunk is stored at once by lane 0.
+// The sweep is sweep_interval's with SubgroupPack's two differences: the
+// residual steps as r = r + (w - mean) (shader.comp:240), and an interval
+// after the first starts from its spill as it is (shader.comp:118-125; the
+// lower-bound spill is never zero, so this matches the scalar pack's start).
+// The shader's `hasHeavy && w <= mean` (shader.comp:147) would take a light
+// after the interval's last one; as in the scalar pack a heavy is taken.
+template <int PREC>
+__global__ __launch_bounds__(kSplitBlock) void k_pack_subgroup(
+    uint32_t N, uint32_t K, const Params* __restrict__ prm, const SplitRec* __restrict__ splits,
+    const uint32_t* __restrict__ idx, const float* __restrict__ pw,
+    AliasEntry* __restrict__ table) {
+    using T = typename Real<PREC>::T;
+    constexpr int G = kPackLanes;
+    static_assert(kSplitBlock % G == 0 && kWave % G == 0, "lane groups inside a wave");
+    if (prm->error) return;
+    const uint32_t rl = threadIdx.x % G;
+    const uint64_t k = ((uint64_t)blockIdx.x * kSplitBlock + threadIdx.x) / G;
+    if (k >= K) return;  // whole groups leave together
+    const uint32_t H = prm->heavy_count;
+    if (H == 0 || H == N) {
+        if (rl == 0) identity_rows(N, K, k, table);
+        return;
+    }
+    const uint32_t L = N - H;
+    const T mean = (T)prm->mean;
+    const SplitRec a = k == 0 ? SplitRec{0u, 0u, 0.0} : splits[k - 1];
+    const SplitRec b = splits[k];
+    const uint32_t i0 = a.i, i1 = max(b.i, a.i);
+    const uint32_t j0 = a.j, j1 = max(b.j, a.j);
+    // chunks: heavies at slots jb + rl, lights ib + rl (slot N-1-(ib+rl))
+    uint32_t jb = j0, ib = i0;
+    uint32_t hx = 0, lx = 0;
+    float hw = 0.0f, lw = 0.0f;
+    if (jb + rl < H) {
+        hx = idx[jb + rl];
+        hw = pw[jb + rl];
+    }
+    if (ib + rl < L) {
+        lx = idx[N - 1u - (ib + rl)];
+        lw = pw[N - 1u - (ib + rl)];
+    }
+    uint32_t h = __shfl(hx, 0, G);
+    T r = k == 0 ? (T)__shfl(hw, 0, G) : (T)a.spill;
+    AliasEntry he{0.0f, 0u}, le{0.0f, 0u};
+    bool hv = false, lv = false;  // this lane's rows not yet stored
+    uint32_t i = i0, j = j0;
+    while (i != i1 || j != j1) {
+        const bool heavy = j == j1 ? false : (i == i1 ? true : r <= mean);
+        float w;
+        if (heavy) {
+            const uint32_t nx = j + 1u;  // < H: j1 <= H-1
+            if (nx - jb == (uint32_t)G) {
+                if (hv) table[hx] = he;
+                hv = false;
+                jb = nx;
+                if (jb + rl < H) {
+                    hx = idx[jb + rl];
+                    hw = pw[jb + rl];
+                }
+                const uint32_t x = __shfl(hx, 0, G);
+                w = __shfl(hw, 0, G);
+                if (rl == 0) table[h] = AliasEntry{sweep_ratio<PREC>(r, mean), x};
+                h = x;
+            } else {
+                const uint32_t x = __shfl(hx, (int)(nx - jb), G);
+                w = __shfl(hw, (int)(nx - jb), G);
+                if (rl == j - jb) {
+                    he = AliasEntry{sweep_ratio<PREC>(r, mean), x};
+                    hv = true;
+                }
+                h = x;
+            }
+            j = nx;
+        } else {
+            if (i - ib == (uint32_t)G) {
+                if (lv) table[lx] = le;
+                lv = false;
+                ib = i;  // < i1 <= L
+                if (ib + rl < L) {
+                    lx = idx[N - 1u - (ib + rl)];
+                    lw = pw[N - 1u - (ib + rl)];
+                }
+            }
+            w = __shfl(lw, (int)(i - ib), G);
+            if (rl == i - ib) {
+                le = AliasEntry{sweep_ratio<PREC>((T)lw, mean), h};
+                lv = true;
+            }
+            ++i;
+        }
+        r = r + ((T)w - mean);
+    }
+    if (hv) table[hx] = he;
+    if (lv) table[lx] = le;
+    if (j1 == H - 1u && rl == 0) table[h] = AliasEntry{1.0f, h};
+}
+
 // Inline form: block b finds S(k) for k = 256 b .. 256 b + 256 (thread t the
 // t-th, thread 0 also the last) into LDS, then thread t sweeps interval
 // 256 b + t.  The reference's InlineSplitPack ends the last interval at
@@ -272,6 +383,19 @@ hipError_t launch_pack_serial(uint64_t n, uint32_t K, int prec, const Params* pr
         k_pack_serial<kFP32><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, splits, idx, pw, table);
     else
         k_pack_serial<kComp><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, splits, idx, pw, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_subgroup(uint64_t n, uint32_t K, int prec, const Params* prm,
+                                const SplitRec* splits, const uint32_t* idx, const float* pw,
+                                AliasEntry* table, hipStream_t s) {
+    const uint64_t lanes = (uint64_t)K * kPackLanes;
+    const uint32_t blocks = (uint32_t)((lanes + kSplitBlock - 1) / kSplitBlock);
+    const uint32_t N = (uint32_t)n;
+    if (prec == kFP32)
+        k_pack_subgroup<kFP32><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, splits, idx, pw, table);
+    else
+        k_pack_subgroup<kComp><<<blocks, kSplitBlock, 0, s>>>(N, K, prm, splits, idx, pw, table);
     return hipGetLastError();
 }
 

@@ -114,7 +114,7 @@ std::string device_key() {
     std::ostringstream k;
     // "t3": the tuning encoding (WRS_PACK_* pack forms, sweep forms included);
     // lines of other encodings in a cache file do not match
-    k << "t3/" << prop.gcnArchName << "/cu" << prop.multiProcessorCount;
+    k << "t4/" << prop.gcnArchName << "/cu" << prop.multiProcessorCount;
     return k.str();
 }
 
@@ -200,7 +200,7 @@ wrs_tuning descend(M&& measure_raw, bool direct_rts_only) {
     }
     bool sweep = false;
     if (!direct_rts_only) {  // the reference's sweep forms (single builds only)
-        for (const uint32_t f : {WRS_PACK_SERIAL, WRS_PACK_INLINE}) {
+        for (const uint32_t f : {WRS_PACK_SERIAL, WRS_PACK_INLINE, WRS_PACK_SUBGROUP}) {
             wrs_tuning t3 = best;
             t3.direct = f;
             const float ms3 = measure(t3);
@@ -210,7 +210,7 @@ wrs_tuning descend(M&& measure_raw, bool direct_rts_only) {
             }
         }
         // the merge shape does not apply to them
-        sweep = best.direct == WRS_PACK_SERIAL || best.direct == WRS_PACK_INLINE;
+        sweep = best.direct >= WRS_PACK_SERIAL;
     }
     if (!sweep) {
         const wrs_tuning form = best;

@@ -118,7 +118,7 @@ void require_tuning(const wrs_tuning& t) {
     require(wrsk::merge_shape_supported(merge_shape_of(t)),
             "unsupported merge shape (merge_tile 1024/2048 with merge_per 2/4/8, or 4096 with 4/8)");
     require(t.partition <= 1u, "partition must be 0 or 1");
-    require(t.direct <= WRS_PACK_INLINE, "direct must be a WRS_PACK_* value");
+    require(t.direct <= WRS_PACK_SUBGROUP, "direct must be a WRS_PACK_* value");
 }
 
 void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, double pivot) const {
@@ -169,13 +169,14 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     const wrs_precision prec = m_cfg.precision;
     // full prefix arrays only for a debug build (parity tests); the build
     // itself needs the indices and the sparse prefix samples
-    // the sweep forms (the reference's serial and inline split + pack) read
-    // the full prefix arrays and the classic partition arrays
+    // the sweep forms (the reference's serial, subgroup and inline split +
+    // pack) read the full prefix arrays and the classic partition arrays
     const uint32_t form = dbg ? WRS_PACK_CLASSIC : m_cfg.tuning.direct;
-    const bool sweep = form == WRS_PACK_SERIAL || form == WRS_PACK_INLINE;
+    const bool records = form == WRS_PACK_SERIAL || form == WRS_PACK_SUBGROUP;
+    const bool sweep = records || form == WRS_PACK_INLINE;
     const bool full_prefix = sweep || (dbg && (dbg->prefix_hi || dbg->prefix_lo || dbg->splits));
     if (full_prefix) ws.ensure_prefix(wrsk::prec_has_lo(prec));
-    if (form == WRS_PACK_SERIAL) ws.ensure_splits(splitCount(N));
+    if (records) ws.ensure_splits(splitCount(N));
     float* hi = full_prefix ? ws.prefix_hi : nullptr;
     float* lo = full_prefix && wrsk::prec_has_lo(prec) ? ws.prefix_lo : nullptr;
     // the direct form (partition arrays hold the heavies only) for plain builds;
@@ -202,13 +203,18 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     if (sweep) {
         const uint32_t K = splitCount(N);
         auto* table = reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable);
-        if (form == WRS_PACK_SERIAL) {
+        if (records) {
             check_hip(wrsk::launch_splits(N, K, (int)prec, ws.params, hi, lo, ws.splits, s),
                       "splits launch");
             if (ev) check_hip(hipEventRecord(ev[6], s), "event");
-            check_hip(wrsk::launch_pack_serial(N, K, (int)prec, ws.params, ws.splits, ws.indices,
-                                               ws.pweights, table, s),
-                      "serial pack launch");
+            if (form == WRS_PACK_SERIAL)
+                check_hip(wrsk::launch_pack_serial(N, K, (int)prec, ws.params, ws.splits,
+                                                   ws.indices, ws.pweights, table, s),
+                          "serial pack launch");
+            else
+                check_hip(wrsk::launch_pack_subgroup(N, K, (int)prec, ws.params, ws.splits,
+                                                     ws.indices, ws.pweights, table, s),
+                          "subgroup pack launch");
         } else {
             if (ev) check_hip(hipEventRecord(ev[6], s), "event");
             check_hip(wrsk::launch_splitpack_inline(N, K, (int)prec, ws.params, hi, lo,

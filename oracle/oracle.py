@@ -46,6 +46,9 @@ class PsaDebug(C.Structure):
 SPLIT_REFERENCE_CPU = 0  # src/host/reference/split.hpp:14-57
 SPLIT_LOWER_BOUND = 1    # src/device/wrs/alias/psa/split/scalar/shader.comp:72-109
 
+PACK_SCALAR = 0   # pack.hpp:14-83 pack2 (ScalarPack, pack/scalar/float.comp:60-158)
+PACK_GROUPED = 1  # SubgroupPack, pack/subgroup/shader.comp:68-246
+
 SPLIT64_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float64)])
 SPLIT32_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float32)])
 
@@ -96,6 +99,10 @@ def lib():
                                        u32p, u32p]
     L.oro_pack_splits_f64.argtypes = [u32p, C.c_uint32, u32p, C.c_uint32, f64p, C.c_uint32,
                                       C.c_double, C.c_void_p, C.c_uint32, f64p, u32p]
+    L.oro_pack_grouped_f64.argtypes = [u32p, C.c_uint32, u32p, C.c_uint32, f64p, C.c_uint32,
+                                       C.c_double, C.c_void_p, C.c_uint32, f64p, u32p]
+    L.oro_pack_grouped_f32.argtypes = [u32p, C.c_uint32, u32p, C.c_uint32, f32p, C.c_uint32,
+                                       C.c_float, C.c_void_p, C.c_uint32, f32p, u32p]
     L.oro_psa_alias_table_f32.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_int, f32p,
                                           u32p, C.c_void_p]
     L.oro_psa_alias_table_f64.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
@@ -260,8 +267,9 @@ def sweep_states(w, mean: float):
 
 
 def psa_alias_table_f32(w, K: int, reference_scale: bool = False,
-                        split_rule: int = SPLIT_REFERENCE_CPU):
-    """Reference CPU path with T = P = float.  Returns (p float32, a uint32, debug dict)."""
+                        split_rule: int = SPLIT_REFERENCE_CPU, pack: int = PACK_SCALAR):
+    """Reference CPU path with T = P = float.  Returns (p float32, a uint32, debug dict).
+    `pack` = PACK_GROUPED re-packs the same split records with SubgroupPack's sweep."""
     w = np.ascontiguousarray(w, np.float32)
     n = w.size
     p = np.empty(n, np.float32)
@@ -274,16 +282,26 @@ def psa_alias_table_f32(w, K: int, reference_scale: bool = False,
     lib().oro_psa_alias_table_f32(w, n, K, int(reference_scale), int(split_rule), p, a,
                                   C.byref(dbg))
     H = int(dbg.heavy_count)
+    if pack == PACK_GROUPED and 0 < H < n:
+        # the weights the reference path packs (`weights[i] *= 0.0001` with
+        # T = float is evaluated in double and rounded back to float)
+        ws = (w.astype(np.float64) * 0.0001).astype(np.float32) if reference_scale else w
+        sp32 = np.zeros(K, SPLIT32_DTYPE)
+        sp32["i"], sp32["j"], sp32["spill"] = sp["i"], sp["j"], sp["spill"].astype(np.float32)
+        lib().oro_pack_grouped_f32(idx[:H].copy(), H, idx[H:].copy(), n - H, ws, n,
+                                   np.float32(dbg.mean), sp32.ctypes.data, K, p, a)
     return p, a, dict(mean=dbg.mean, H=H, idx_cpu=idx, heavy_prefix=hp[:H],
                       light_prefix=lp[:n - H], splits=sp)
 
 
 def psa_alias_table_f64(w, K: int, reference_scale: bool = False, mean=None,
-                        split_rule: int = SPLIT_LOWER_BOUND):
+                        split_rule: int = SPLIT_LOWER_BOUND, pack: int = PACK_SCALAR):
     """Reference CPU path with T = P = double (tests-of-tests instantiation).
     `mean` (optional) pins the pivot/bucket size ("given the pivot");
     `split_rule` selects the reference CPU split or the lower-bound split of
-    the reference GPU kernels (the default: it always matches the sweep)."""
+    the reference GPU kernels (the default: it always matches the sweep);
+    `pack` = PACK_GROUPED re-packs the same split records with SubgroupPack's
+    sweep."""
     w = np.ascontiguousarray(w, np.float32)
     n = w.size
     p = np.empty(n, np.float64)
@@ -297,6 +315,10 @@ def psa_alias_table_f64(w, K: int, reference_scale: bool = False, mean=None,
     lib().oro_psa_alias_table_f64(w, n, K, int(reference_scale), int(split_rule), m, p, a,
                                   C.byref(dbg))
     H = int(dbg.heavy_count)
+    if pack == PACK_GROUPED and 0 < H < n:
+        w64 = w.astype(np.float64) * 0.0001 if reference_scale else w.astype(np.float64)
+        lib().oro_pack_grouped_f64(idx[:H].copy(), H, idx[H:].copy(), n - H, w64, n,
+                                   float(dbg.mean), sp.ctypes.data, K, p, a)
     return p, a, dict(mean=dbg.mean, H=H, idx_cpu=idx, heavy_prefix=hp[:H],
                       light_prefix=lp[:n - H], splits=sp)
 

@@ -389,6 +389,64 @@ DEFINE_SPLITK(double, oro_split_f64_t, oro_splitK_f64, oro_split_f64, oro_split_
 DEFINE_PACK(float, oro_split_f32_t, oro_pack_splits_f32)
 DEFINE_PACK(double, oro_split_f64_t, oro_pack_splits_f64)
 
+/* ======================================================================== */
+/* SubgroupPack (src/device/wrs/alias/psa/pack/subgroup/shader.comp:68-246)  */
+/* over the same K split records.  The shader steps one interval with a      */
+/* group of lanes that hold the next heavies and lights in registers; read   */
+/* as a sequential sweep it is pack2 with two differences:                   */
+/*   - the residual steps as r = r + (weight - mean) (shader.comp:240), not  */
+/*     (r + weight) - mean (pack.hpp:80);                                    */
+/*   - interval 0 starts from heavy 0's weight and every other interval from */
+/*     its split's spill as it is (shader.comp:86-94, 118-125), where pack2  */
+/*     replaces a zero spill by the current heavy's weight (pack.hpp:32-37). */
+/* Deviation: with the interval's lights used up and r > mean, the shader's  */
+/* `hasHeavy && w <= mean` (shader.comp:147) takes one more light, from      */
+/* registers that hold no light of the interval; here, as in pack2, the      */
+/* heavy is taken.  Interval ends are clamped as in pack2.                   */
+/* ======================================================================== */
+#define DEFINE_PACK_GROUPED(T, ST, NAME)                                       \
+    void NAME(const uint32_t* heavy, uint32_t H, const uint32_t* light,        \
+              uint32_t L, const T* w, uint32_t N, T mean, const ST* splits,    \
+              uint32_t K, T* p, uint32_t* a) {                                 \
+        (void)L;                                                               \
+        (void)N;                                                               \
+        for (uint32_t k = 0; k < K; ++k) {                                     \
+            const uint32_t i0 = k ? splits[k - 1].i : 0u;                      \
+            const uint32_t j0 = k ? splits[k - 1].j : 0u;                      \
+            uint32_t i1 = splits[k].i, j1 = splits[k].j;                       \
+            if (i1 < i0) i1 = i0;                                              \
+            if (j1 < j0) j1 = j0;                                              \
+            uint32_t i = i0, j = j0;                                           \
+            T r = k ? splits[k - 1].spill : w[heavy[j0]];                      \
+            while (i != i1 || j != j1) {                                       \
+                int packHeavy;                                                 \
+                if (j == j1) packHeavy = 0;                                    \
+                else if (i == i1) packHeavy = 1;                               \
+                else packHeavy = r <= mean;                                    \
+                const uint32_t h = heavy[j];                                   \
+                const uint32_t widx = packHeavy ? heavy[j + 1] : light[i];     \
+                const T weight = w[widx];                                      \
+                if (packHeavy) {                                               \
+                    p[h] = r / mean;                                           \
+                    a[h] = widx;                                               \
+                    j += 1;                                                    \
+                } else {                                                       \
+                    p[widx] = weight / mean;                                   \
+                    a[widx] = h;                                               \
+                    i += 1;                                                    \
+                }                                                              \
+                r = r + (weight - mean);                                       \
+            }                                                                  \
+            if (j1 == H - 1) {                                                 \
+                const uint32_t h = heavy[j];                                   \
+                p[h] = (T)1;                                                   \
+                a[h] = h;                                                      \
+            }                                                                  \
+        }                                                                      \
+    }
+DEFINE_PACK_GROUPED(float, oro_split_f32_t, oro_pack_grouped_f32)
+DEFINE_PACK_GROUPED(double, oro_split_f64_t, oro_pack_grouped_f64)
+
 /* The sequential sweep of pack2 over one interval (0,0) -> (L, H-1), i.e.
  * the PSA pack with no splits: st_i[n], st_j[n] = lights and heavies packed
  * after n buckets (n = 0..N-1).  This is the ground truth a split must hit. */

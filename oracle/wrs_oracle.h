@@ -98,6 +98,15 @@ void oro_pack_splits_f32(const uint32_t* heavy, uint32_t H, const uint32_t* ligh
 void oro_pack_splits_f64(const uint32_t* heavy, uint32_t H, const uint32_t* light, uint32_t L,
                          const double* w, uint32_t N, double mean,
                          const oro_split_f64_t* splits, uint32_t K, double* p, uint32_t* a);
+/* SubgroupPack (src/device/wrs/alias/psa/pack/subgroup/shader.comp:68-246) as a
+ * sequential sweep: pack2 with r = r + (weight - mean) and the spill taken as
+ * it is after the first interval (see the definition for the deviation). */
+void oro_pack_grouped_f32(const uint32_t* heavy, uint32_t H, const uint32_t* light, uint32_t L,
+                          const float* w, uint32_t N, float mean,
+                          const oro_split_f32_t* splits, uint32_t K, float* p, uint32_t* a);
+void oro_pack_grouped_f64(const uint32_t* heavy, uint32_t H, const uint32_t* light, uint32_t L,
+                          const double* w, uint32_t N, double mean,
+                          const oro_split_f64_t* splits, uint32_t K, double* p, uint32_t* a);
 
 /* ---- full PSA build (src/host/reference/psa_alias_table.hpp:16-67) -----
  * reference_scale != 0 multiplies the weights by 1e-4 first, as the reference

@@ -94,12 +94,12 @@ def test_autotune_psa_measures_then_caches(wrs, oracle):
     print("  best", best, [(e["partition"], e["direct"], e["merge_tile"], e["merge_per"],
                             round(e["ms"], 4)) for e in log])
     # partition form at the default shape (2), the pack forms (direct, then the
-    # serial and inline sweeps), then the 7 other shapes (at 2^20 a sweep form
-    # is far slower than the merge, so the shape step runs), then the two
-    # fastest again and the faster of those
-    assert len(log) == 2 + 3 + len(SHAPES) - 1 + 2
+    # serial, inline and subgroup sweeps), then the 7 other shapes (at 2^20 a
+    # sweep form is far slower than the merge, so the shape step runs), then
+    # the two fastest again and the faster of those
+    assert len(log) == 2 + 4 + len(SHAPES) - 1 + 2
     assert {e["partition"] for e in log[:2]} == {0, 1}
-    assert [e["direct"] for e in log[:5]] == [2, 2, 1, 3, 4]
+    assert [e["direct"] for e in log[:6]] == [2, 2, 1, 3, 4, 5]
     assert {(e["merge_tile"], e["merge_per"]) for e in log} == set(SHAPES)
     assert all(e["ms"] > 0 for e in log)
     assert best["direct"] in (1, 2)
@@ -148,7 +148,7 @@ def test_set_tuning_on_an_existing_plan(wrs, oracle):
     dw = wrs.to_device(w)
     plan = wrs.Plan(n)
     for tune in (dict(merge_tile=2048, merge_per=4, direct=1), dict(direct=2, partition=1),
-                 dict(direct=4), dict()):
+                 dict(direct=4), dict(direct=5), dict()):
         plan.set_tuning(tune)
         t0 = wrs.empty(n, wrs.ALIAS_DTYPE)
         plan.psa_build(dw, n, t0)

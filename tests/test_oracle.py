@@ -215,9 +215,9 @@ def test_reference_cpu_split_can_break_the_table(oracle):
 
 
 # ----------------------------------------------------------------- PSA --
-def _assert_is_alias_table(w, p, a, margin):
-    """Restates assert_is_alias_table (src/host/assert/is_alias_table.hpp:103-195):
-    implied mass p_i + sum_{a_j=i}(1-p_j), scaled by total/N, within margin of w_i."""
+def _alias_table_error(w, p, a):
+    """max_i |implied mass_i - w_i|, the implied mass p_i + sum_{a_j=i}(1-p_j)
+    scaled by total/N (src/host/assert/is_alias_table.hpp:103-195)."""
     n = len(w)
     assert np.all(a < n), "alias out of range"
     sampled = np.zeros(n)
@@ -225,7 +225,13 @@ def _assert_is_alias_table(w, p, a, margin):
     np.add.at(sampled, a.astype(np.int64), 1.0 - p.astype(np.float64))
     total = math.fsum(np.asarray(w, np.float64))
     sampled *= total / n
-    return np.abs(sampled - np.asarray(w, np.float64)).max() <= margin
+    return float(np.abs(sampled - np.asarray(w, np.float64)).max())
+
+
+def _assert_is_alias_table(w, p, a, margin):
+    """Restates assert_is_alias_table (src/host/assert/is_alias_table.hpp:103-195):
+    implied mass p_i + sum_{a_j=i}(1-p_j), scaled by total/N, within margin of w_i."""
+    return _alias_table_error(w, p, a) <= margin
 
 
 @pytest.mark.parametrize("gen,n", [("uniform", 1 << 14), ("lognormal", 1 << 14), ("zipf", 1 << 14),
@@ -315,6 +321,111 @@ def test_hand_case_closed_form(oracle):
     assert np.allclose(p[1:7], 2.0 / 3.0, rtol=0, atol=1e-15)
     q = oracle.implied_distribution(p, a)
     assert np.allclose(q, w.astype(np.float64) / 12.0, rtol=1e-15, atol=0)
+
+
+# ------------------------------------------------- SubgroupPack (grouped) --
+def _subgroup_shader_sweep(d, w, n, K, f32):
+    """pack/subgroup/shader.comp:68-246 read literally, with one lane per
+    subproblem (its broadcasts become plain reads): the heavy/light counters
+    (:85-112), the start (:114-126), `hasHeavy && w <= mean` (:147), the
+    residual step `w += weight - mean` (:240) and the last heavy (:244-246).
+    Pure Python floats (double) or numpy float32 scalars.  Returns None where
+    the shader would take a light its interval does not hold."""
+    num = np.float32 if f32 else float
+    mean = num(d["mean"])
+    H = d["H"]
+    heavy = d["idx_cpu"][:H].tolist()
+    light = d["idx_cpu"][H:].tolist()
+    wv = [num(x) for x in np.asarray(w, np.float32 if f32 else np.float64)]
+    sp = [(int(s["i"]), int(s["j"]), num(s["spill"])) for s in d["splits"]]
+    p = [None] * n
+    a = [None] * n
+    for k in range(K):
+        i0, j0, spill = (0, 0, num(0)) if k == 0 else sp[k - 1]
+        i1, j1, _ = sp[k]
+        j1 += 1
+        rem_h, rem_l = j1 - j0, i1 - i0
+        nj, ni = j0, i0
+        h = heavy[nj]
+        r = wv[h] if k == 0 else spill
+        nj += 1
+        rem_h -= 1
+        while rem_l > 0 or rem_h > 0:
+            take_heavy = rem_h > 0 and r <= mean
+            if not take_heavy and rem_l <= 0:
+                return None
+            if take_heavy:
+                x = heavy[nj]
+                nj += 1
+                rem_h -= 1
+                p[h], a[h] = r / mean, x
+                h = x
+            else:
+                x = light[ni]
+                ni += 1
+                rem_l -= 1
+                p[x], a[x] = wv[x] / mean, h
+            r += wv[x] - mean
+        if j1 == H:
+            p[h], a[h] = num(1), h
+    return p, a
+
+
+@pytest.mark.parametrize("f32", [False, True])
+@pytest.mark.parametrize("gen,n,K", [("uniform", 2000, 63), ("zipf", 1500, 47),
+                                     ("lognormal", 999, 500), ("spike", 1200, 5)])
+def test_grouped_pack_matches_shader_reading(oracle, gen, n, K, f32):
+    """The C restatement of SubgroupPack against a literal reading of the
+    shader's loop (counters instead of clamped interval ends): same bits."""
+    w = {"uniform": lambda: oracle.gen_uniform01(5, n),
+         "lognormal": lambda: oracle.gen_lognormal(6, 0.0, 2.0, n),
+         "zipf": lambda: oracle.gen_zipf(7, 1.1, n),
+         "spike": lambda: oracle.gen_spike(8, n)}[gen]()
+    fn = oracle.psa_alias_table_f32 if f32 else oracle.psa_alias_table_f64
+    kw = dict(split_rule=oracle.SPLIT_LOWER_BOUND)
+    p, a, d = fn(w, K, pack=oracle.PACK_GROUPED, **kw)
+    lit = _subgroup_shader_sweep(d, w, n, K, f32)
+    assert lit is not None
+    assert a.tolist() == lit[1]
+    assert p.tolist() == [float(x) for x in lit[0]]
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_grouped_pack_equals_pack2_when_exact(oracle, f32):
+    """Integer weights over a power-of-two N: every residual is a multiple of
+    2^-12 below 2^5, exact in either rounding order, so SubgroupPack's sweep
+    and pack2 must give the same table bit for bit (at every split size)."""
+    n = 4096
+    w = (np.floor(oracle.gen_uniform01(9, n) * 16) + 1).astype(np.float32)
+    fn = oracle.psa_alias_table_f32 if f32 else oracle.psa_alias_table_f64
+    for K in (1, 7, 128, 2048):
+        ps, as_, _ = fn(w, K, split_rule=oracle.SPLIT_LOWER_BOUND)
+        pg, ag, _ = fn(w, K, split_rule=oracle.SPLIT_LOWER_BOUND, pack=oracle.PACK_GROUPED)
+        assert np.array_equal(as_, ag) and np.array_equal(ps, pg), K
+
+
+@pytest.mark.parametrize("gen", ["uniform", "lognormal", "zipf"])
+def test_grouped_pack_is_an_alias_table(oracle, gen):
+    """On inexact weights the two roundings differ (so the grouped sweep is
+    not pack2 under another name), the grouped table is still an exact alias
+    table of w in FP64, and in FP32 its implied masses are as close to w as
+    pack2's (the reference's margin of 1e-2 holds for both on uniform
+    weights; lognormal weights up to 4e3 take both just past it)."""
+    n = 1 << 14
+    w = {"uniform": lambda: oracle.gen_uniform01(10, n),
+         "lognormal": lambda: oracle.gen_lognormal(11, 0.0, 2.0, n),
+         "zipf": lambda: oracle.gen_zipf(12, 1.1, n)}[gen]()
+    ps, as_, _ = oracle.psa_alias_table_f32(w, n // 32, split_rule=oracle.SPLIT_LOWER_BOUND)
+    pg, ag, _ = oracle.psa_alias_table_f32(w, n // 32, split_rule=oracle.SPLIT_LOWER_BOUND,
+                                           pack=oracle.PACK_GROUPED)
+    assert not np.array_equal(ps, pg)
+    e_s, e_g = (_alias_table_error(w, p_, a_) for p_, a_ in ((ps, as_), (pg, ag)))
+    assert e_g <= 1.5 * e_s + 1e-9, (e_g, e_s)
+    if gen == "uniform":
+        assert _assert_is_alias_table(w, pg, ag, 1e-2)
+    p, a, _ = oracle.psa_alias_table_f64(w, n // 32, pack=oracle.PACK_GROUPED)
+    err = oracle.distribution_error(oracle.implied_distribution(p, a), w)
+    assert err["max_rel"] < 1e-9, err
 
 
 # ------------------------------------------------------------ sampling --
