@@ -4,9 +4,11 @@
 //   kComp              compensated FP32: a double-float hi + lo kept exact by
 //                      two_sum at every add (error ~2^-48 relative instead of
 //                      2^-24 times the term count)
-//   kFP64              FP64 sums: a float weight converts exactly, and on
-//                      gfx950 with packed FP32 math disabled an FP64 add
-//                      issues at the rate of an FP32 add
+//   kFP64              FP64 sums: a float weight converts exactly; an FP64
+//                      add holds its SIMD twice as long as an FP32 add on
+//                      gfx950 (4 cycles a wave64 instruction against 2), but
+//                      one of them replaces the seven FP32 operations of a
+//                      compensated add
 //
 // All operations are in a fixed order, so a tile's sums are a function of its
 // weights only.  pin() is an empty register barrier: it makes a value exist at
