@@ -171,6 +171,73 @@ __device__ __forceinline__ void vexcl_scan(T& x0, T& x1, const VLane& v) {
 // direct form reads the lights from ls (the heavy indices idx locate them) and
 // records the element index of the split's light.
 //
+// Direct form, phase 2 of a split: the 64 lights of ranks [anchor - kSample,
+// anchor), anchor a multiple of kSample with kSample <= anchor < L, are the
+// lights of the element range [lpos[anchor/kSample - 1], lpos[anchor/kSample])
+// -- exactly kSample lights, with the heavies that lie between them.  When the
+// range holds at most kGatherSpan elements the half-wave loads it whole, one
+// round of contiguous loads where light_pos would take a dependent bisection
+// per light, finds its lights by the partition's test (!(w > pivot)) and hands
+// light anchor-1-e to virtual lane e through LDS.  Returns false (nothing
+// written) when the case does not apply; the caller then locates each light
+// by light_pos, which gives the same lights.  Uniform per half-wave.
+constexpr uint32_t kGatherSpan = 8u * 32u;  // up to eight loads per lane of a half-wave
+
+template <class T>
+__device__ __forceinline__ bool gather_lights(const LightSrc& ls, float pivot, uint32_t anchor,
+                                              uint32_t L, const VLane& v, uint32_t (&lx)[2],
+                                              T (&lw)[2]) {
+    __shared__ uint32_t s_x[256 / 32][kSample];
+    __shared__ float s_w[256 / 32][kSample];
+    static_assert(kSample == kWave, "a virtual lane per light of a sample chunk");
+    if (anchor < kSample || anchor >= L) return false;
+    const uint32_t k = anchor / kSample;
+    const uint32_t x0 = ls.lpos[k - 1u], x1 = ls.lpos[k];
+    const uint32_t span = x1 - x0;  // kSample lights and the heavies between them
+    if (span > kGatherSpan) return false;
+    const uint32_t hv = threadIdx.x / 32u;  // this half-wave's LDS rows (blocks of 256)
+    uint32_t before = 0;  // lights at smaller offsets
+    const uint32_t below = (1u << v.sl) - 1u;
+    // the previous split of this half-wave has read its rows
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // passes of kChunk consecutive elements (one for most ranges), each as
+    // rounds of 32 loaded together
+    constexpr int kChunk = 4 * 32;
+    for (uint32_t o0 = 0; o0 < span; o0 += kChunk) {  // uniform per half-wave
+        float x[kChunk / 32];
+#pragma unroll
+        for (int c = 0; c < kChunk / 32; ++c) {
+            const uint32_t o = o0 + (uint32_t)v.sl + 32u * (uint32_t)c;
+            x[c] = o < span ? ls.w[x0 + o] : 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < kChunk / 32; ++c) {
+            const uint32_t o = o0 + (uint32_t)v.sl + 32u * (uint32_t)c;
+            const bool light = o < span && !(x[c] > pivot);
+            const uint32_t m = (uint32_t)(__ballot(light) >> (32 * v.h));
+            const uint32_t r = before + (uint32_t)__popc(m & below);
+            if (light && r < kSample) {
+                s_x[hv][r] = x0 + o;
+                s_w[hv][r] = x[c];
+            }
+            before += (uint32_t)__popc(m);
+        }
+    }
+    // the rows of this half-wave are written and read by its own lanes only
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const uint32_t e = (uint32_t)v.sl + 32u * (uint32_t)c;  // light anchor - 1 - e
+        lx[c] = s_x[hv][kSample - 1u - e];
+        lw[c] = (T)s_w[hv][kSample - 1u - e];
+    }
+    return true;
+}
+
 // A bracket [jb_lo, jb_hi] expected to hold the split's j (the split of the
 // previous boundary, and that plus the buckets in between: splits are
 // monotone) narrows phase 1 to the sample multiples in it, one round of
@@ -186,7 +253,8 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
                                                     uint32_t H, typename RealT<PREC>::T mean,
                                                     typename RealT<PREC>::T light_sum, uint32_t n,
                                                     const uint32_t* __restrict__ idx,
-                                                    const LightSrc& ls, uint32_t jb_lo = 0u,
+                                                    const LightSrc& ls, float pivot,
+                                                    uint32_t jb_lo = 0u,
                                                     uint32_t jb_hi = 0xffffffffu) {
     using T = typename RealT<PREC>::T;
     const uint32_t L = N - H;
@@ -246,11 +314,14 @@ __device__ __forceinline__ Coarse coarse_split_half(const float* __restrict__ pw
     const T hp_base = hp_sample<T>(samp_h, base);
     T hw[2], lw[2];
     uint32_t lx[2] = {0u, 0u};
+    bool gathered = false;
+    if constexpr (DIRECT) gathered = gather_lights<T>(ls, pivot, anchor, L, v, lx, lw);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const int vl = v.sl + 32 * c;
         const uint32_t j = base + (uint32_t)vl;
         hw[c] = j < H ? (T)pw[j] : (T)0;
+        if (gathered) continue;
         const int64_t lrank = (int64_t)anchor - 1 - vl;  // virtual lane e: light rank anchor-1-e
         lw[c] = (T)0;
         if (lrank >= 0) {
@@ -326,7 +397,7 @@ __global__ __launch_bounds__(256) void k_coarse(const float* __restrict__ pw,
         const uint32_t n = (uint32_t)min((uint64_t)b * tile, (uint64_t)N);  // N < 2^32 <= b * tile
         const Coarse c = coarse_split_half<PREC, DIRECT>(pw, samp_h, samp_l, N, prm->heavy_count,
                                                          (T)prm->mean, (T)prm->light_sum, n, idx,
-                                                         ls, jb_lo, jb_hi);
+                                                         ls, prm->pivot, jb_lo, jb_hi);
         if ((lane_id() & 31) == 0) out[b] = c;
         jb_lo = c.j;
         jb_hi = c.j + tile;
@@ -356,7 +427,8 @@ __global__ __launch_bounds__(256) void k_seg_coarse(
     const Coarse c = coarse_split_half<PREC, DIRECT>(pw + off, samp_h + so, samp_l + so, N,
                                                      seg_prm[g].heavy_count, (T)seg_prm[g].mean,
                                                      (T)seg_prm[g].light_sum, n,
-                                                     DIRECT ? idx + off : nullptr, ls);
+                                                     DIRECT ? idx + off : nullptr, ls,
+                                                     seg_prm[g].pivot);
     if ((lane_id() & 31) == 0) out[p] = c;
 }
 
