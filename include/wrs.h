@@ -98,7 +98,19 @@ typedef struct {
                                4*max_n in WRS_PREC_FP32 and WRS_PREC_FP64_CARRY) and, for
                                SERIAL and SUBGROUP, 16*K split records.  Debug builds use
                                CLASSIC; batch plans build DIRECT with partition 0 only
-                               (the sweep forms are rejected there). */
+                               (the sweep forms and STREAM are rejected there).
+                               STREAM evaluates the same sweep with exact integer keys
+                               (the mean and every weight times 2^(53 - ilogb(mean)))
+                               and writes the table tile by tile in element order: an
+                               aggregate pass, the heavies' keys, one pass over the
+                               weights for the heavies' rows and one for the table; no
+                               split search and no merge blocks.  Its alias indices are
+                               those of the exact sweep (the merge forms' agree except
+                               where FP64 rounding of their block offsets decides a
+                               tie); its heavy probabilities round the exact residual
+                               once.  The first build of such a plan allocates 28 bytes
+                               per weight (keys, indices and rows of up to max_n - 1
+                               heavies); partition is ignored. */
 } wrs_tuning;
 #define WRS_PACK_DEFAULT 0u /* the direct form */
 #define WRS_PACK_DIRECT 1u
@@ -106,6 +118,7 @@ typedef struct {
 #define WRS_PACK_SERIAL 3u
 #define WRS_PACK_INLINE 4u
 #define WRS_PACK_SUBGROUP 5u
+#define WRS_PACK_STREAM 6u
 
 /* Build configuration.  Reference: PSAConfig (src/device/wrs/alias/psa/PSA.hpp:19-41),
  * whose splitSize sets the split granularity. */

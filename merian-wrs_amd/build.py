@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libwrs_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("WRS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["reduce.hip", "partition.hip", "scan.hip", "splitpack.hip", "mergepack.hip", "sample.hip",
+SOURCES = ["reduce.hip", "partition.hip", "scan.hip", "splitpack.hip", "mergepack.hip", "streampack.hip", "sample.hip",
            "stats.hip", "batched.cpp", "wrs_host.cpp", "tune.cpp"]
 
 # -ffp-contract=off: no FMA contraction, so split/pack arithmetic is the

@@ -120,7 +120,7 @@ void BatchPlan::create(uint32_t split_size_, int prec_, const wrs_tuning& tuning
 void BatchPlan::set_tuning(const wrs_tuning& t) {
     wrs::require_tuning(t);
     if (t.direct == WRS_PACK_SERIAL || t.direct == WRS_PACK_INLINE ||
-        t.direct == WRS_PACK_SUBGROUP)
+        t.direct == WRS_PACK_SUBGROUP || t.direct == WRS_PACK_STREAM)
         throw wrs::Error(WRS_EINVAL, "batch plans build the merge pack forms only");
     const wrsk::MergeShape ms = wrs::merge_shape_of(t);
     if (ms.tile != shape.tile) {

@@ -38,8 +38,12 @@ struct Params {
     uint32_t wait_polls;     // polls of a bounded wait before it gives up
     uint32_t hooks;          // kHook* failure injection (tests)
     uint32_t epoch;          // builds finalized with these Params (the reduce's word tags)
+    int32_t key_k;           // exact keys of the streamed pack: x -> x * 2^key_k,
+                             // 53 - ilogb(mean), so that the mean is an integer
+    uint32_t key_pad;
+    uint64_t mean_k;         // mean * 2^key_k (exact, in [2^53, 2^54))
 };
-static_assert(sizeof(Params) == 96, "Params layout");
+static_assert(sizeof(Params) == 112, "Params layout");
 
 // Params::error bits.  A build whose error is not zero has invalid outputs;
 // the kernels after the failing one return at once (no table is written).
@@ -130,6 +134,13 @@ inline uint64_t repair_blocks_for(uint64_t n, uint32_t tile) {
 inline uint64_t repair_records_offset(uint64_t n) {
     return (tiles_for(n) + spine_blocks_for(n)) * kStateWords;
 }
+// Dense record layout of the reduce-then-scan partition inside the state
+// buffer (tiles * kStateWords words): the aggregate of tile i at words
+// [5i, 5i+5), its exclusive prefix at [kExclBase * tiles + 5i, ...); each as
+// heavy sum (two 63-bit limbs), light sum (two limbs), heavy count.
+constexpr uint32_t kCarryWords = 5;
+constexpr uint32_t kExclBase = 8;
+static_assert(kExclBase + kCarryWords <= kStateWords, "carry records exceed the state buffer");
 inline uint64_t state_words_for(uint64_t n) {
     return repair_records_offset(n) + 2 * repair_blocks_for(n, kMinMergeTile) * kStateWords;
 }
@@ -167,6 +178,32 @@ struct LightSrc {
     const uint32_t* lpos = nullptr;
 };
 inline uint64_t lightpos_for(uint64_t n) { return n / kSample + 1; }
+
+// ---- the streamed pack form (streampack.hip) ----
+// exclusive scan of the exact key-sum aggregates that k_sp_agg wrote
+hipError_t launch_key_tilescan(uint64_t* states, uint64_t n, Params* prm, hipStream_t s);
+// Buffers of a streamed build of n weights: slot_idx / slot_w hold tile t's
+// heavies at [t * kTile, t * kTile + c_t) (n entries each); hkey, hidx, hrow,
+// cross one entry per heavy (at most n); qa tiles_for(n) + 1 entries.
+struct StreamBufs {
+    uint32_t* slot_idx = nullptr;
+    float* slot_w = nullptr;
+    void* hkey = nullptr;      // 16-byte keys
+    uint32_t* hidx = nullptr;
+    AliasEntry* hrow = nullptr;
+    uint16_t* cross = nullptr; // tile offset of the light a heavy's key crosses at
+    uint32_t* qa = nullptr;
+};
+// the aggregate pass (after the reduce), then -- after launch_key_tilescan --
+// the heavy keys, the tiles' alias ranges, the heavies' rows and the table;
+// ev_keys / ev_rows (may be null) recorded after the heavy keys and ranges
+// and after the rows
+hipError_t launch_stream_agg(const float* w, uint64_t n, const Params* prm, uint64_t* states,
+                             const StreamBufs& b, hipStream_t s);
+hipError_t launch_stream_pack(const float* w, uint64_t n, int prec, const Params* prm,
+                              const uint64_t* states, const StreamBufs& b, AliasEntry* table,
+                              hipStream_t s, hipEvent_t ev_keys = nullptr,
+                              hipEvent_t ev_rows = nullptr);
 
 hipError_t launch_coarse(const float* pw, uint64_t n, MergeShape ms, int prec, const Params* prm,
                          const double* samp_h, const double* samp_l, Coarse* bounds,

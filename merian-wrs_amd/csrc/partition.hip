@@ -589,12 +589,6 @@ __device__ __forceinline__ void tile_emit(uint32_t n, uint32_t base, uint32_t m,
 
 // ---- single distribution: reduce-then-scan ---------------------------------
 
-// Dense record layout of the reduce-then-scan path inside the look-back state
-// buffer (tiles * kStateWords words): the aggregate of tile i at words
-// [5i, 5i+5), its exclusive prefix at [kExclBase * tiles + 5i, ...).
-constexpr uint32_t kCarryWords = 5;
-constexpr uint32_t kExclBase = 8;
-static_assert(kExclBase + kCarryWords <= kStateWords, "carry records exceed the state buffer");
 
 // Pass 2a: aggregate of every tile.
 template <int PREC>
@@ -622,7 +616,9 @@ constexpr int kSpineThreads = 256;
 constexpr int kSpinePer = 4;
 static_assert(kSpineThreads * kSpinePer == (int)kSpineTiles, "spine block shape");
 
-template <int PREC>
+// KEYS: the aggregates are the streamed pack's exact key sums (scale
+// 2^key_k) instead of the partition's carries (scale 2^shift).
+template <int PREC, bool KEYS = false>
 __global__ __launch_bounds__(kSpineThreads) void k_part_tilescan(uint64_t* __restrict__ states,
                                                                  uint32_t tiles,
                                                                  Params* __restrict__ prm) {
@@ -677,7 +673,7 @@ __global__ __launch_bounds__(kSpineThreads) void k_part_tilescan(uint64_t* __res
             s_excl = ex;
             if (blk == nblk - 1) {
                 const Carry tot = carry_add<PREC>(ex, blk_agg);
-                const int shift = prm->shift;
+                const int shift = KEYS ? prm->key_k : prm->shift;
                 prm->heavy_count = (uint32_t)tot.c;
                 prm->heavy_sum = carry_value<PREC>(tot.h, shift);
                 prm->light_sum = carry_value<PREC>(tot.l, shift);
@@ -1015,6 +1011,13 @@ hipError_t launch_partition(const float* w, uint64_t n, int prec, int lookback, 
         default: WRS_PART(kFP64); break;
     }
 #undef WRS_PART
+    return hipGetLastError();
+}
+
+hipError_t launch_key_tilescan(uint64_t* states, uint64_t n, Params* prm, hipStream_t s) {
+    const uint32_t tiles = (uint32_t)tiles_for(n);
+    k_part_tilescan<kFP64Carry, true><<<(tiles + kSpineTiles - 1) / kSpineTiles, kSpineThreads, 0,
+                                          s>>>(states, tiles, prm);
     return hipGetLastError();
 }
 

@@ -116,6 +116,11 @@ __device__ __forceinline__ void finalize_acc(AccT<FP64> acc, uint64_t n, int has
         p.merge_qs = p.total > 0.0 ? 61 - ilogb(p.total) : 0;
         p.mean_q = p.total > 0.0 ? merge_quant(p.mean, p.merge_qs) : 0;
         p.inv_mean = p.mean > 0.0 ? 1.0 / p.mean : 0.0;
+        // exact keys of the streamed pack: the mean (53 significant bits)
+        // times 2^key_k is an integer below 2^54
+        p.key_k = p.mean > 0.0 ? 53 - ilogb(p.mean) : 0;
+        p.key_pad = 0;
+        p.mean_k = p.mean > 0.0 ? (uint64_t)ldexp(p.mean, p.key_k) : 0;
         p.error = error;
         p.wait_polls = wait_polls;
         p.hooks = hooks;

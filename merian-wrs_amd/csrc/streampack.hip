@@ -1,0 +1,794 @@
+// streampack.hip -- PSA split + pack streamed in element order (the pack form
+// WRS_PACK_STREAM).
+//
+// The reference packs each split interval with a serial sweep
+// (src/host/reference/pack.hpp:14-83, pack/scalar/float.comp:60-180): with
+// heavy j current and residual r, bucket i + j goes to heavy j if r <= mean
+// (p = r/mean, alias = heavy j+1), else to light i (p = w_i/mean, alias =
+// heavy j).  With LP(i), HP(j) the sums of the first i lights / j heavies,
+// "r <= mean" is Y(j+1) <= X(i) for
+//
+//     X(i) = i mean - LP(i)   (non-decreasing),   Y(j) = HP(j) - j mean   (increasing),
+//
+// so the sweep is the merge of the lights keyed X with the heavies keyed
+// Y(j+1), heavy first on ties, and every row of the table follows from a count
+// in that merge:
+//
+//     light i:   alias heavy q(i),  q(i) = #{q < H-1 : Y(q+1) <= X(i)}
+//     heavy q:   (Y(q+1) - X(r) + mean) / mean,  alias heavy q+1,
+//                r = r(q) = #{i < L : X(i) < Y(q+1)}
+//     heavy H-1: (1, itself)  (pack.hpp:78-81)
+//
+// This file evaluates the keys EXACTLY: with K = 53 - ilogb(mean) the FP64
+// mean times 2^K is an integer below 2^54, a light weight times 2^K is an
+// integer below 2^54 (rounded to the nearest integer below mean 2^-30, where
+// the float's own bits run out), and every X and Y is an integer sum below
+// 2^86.  Exact integer keys are consistent between kernels and tiles whatever
+// the order of the additions, so the table can be produced in element order,
+// each tile of 4096 weights on its own:
+//
+//   k_sp_agg      per tile: heavy count, exact light and heavy key sums (the
+//                 tile records the partition's tile scan chains), the tile's
+//                 heavies (index, weight) into its slot range
+//   (tile scan)   k_part_tilescan<.., true> (partition.hip): the exclusive
+//                 prefix of every tile's record
+//   k_sp_heavy    per tile: Y(q+1) and the element index of each of its
+//                 heavies q, in heavy order (hkey, hidx)
+//   k_sp_qrange   per tile t: qa[t] = #{q < H-1 : Y(q+1) <= X(r0(t))}, r0(t)
+//                 the lights before the tile -- the alias range of its lights
+//                 is [qa[t], qa[t+1]], and heavies q in [qa[t], qa[t+1]) have
+//                 their crossing r(q) among its lights
+//   k_sp_rows     per tile: the rows of the heavies whose crossing lies among
+//                 its lights, into hrow[q] (tiles without any skip their read)
+//   k_sp_table    per tile: every row of the tile -- lights from their keys,
+//                 heavies from hrow -- written in element order, as whole
+//                 128-byte lines
+//
+// The light p is w/mean as in the merge pack; a heavy's p is its exact
+// residual rounded once to FP64 and divided by the mean (FP64 modes) or in
+// FP32 (FP32 mode), clamped to [0, 1].
+#include "device_common.hpp"
+#include "internal.hpp"
+#include "tileload.hpp"
+
+namespace wrsk {
+using namespace wrsdev;
+
+namespace {
+
+using u128 = unsigned __int128;
+
+constexpr int kWaves = kBlock / kWave;       // waves of the heavy-range search blocks
+constexpr int kSpBlock = 512;                // threads per tile of the streaming passes
+constexpr int kSpWaves = kSpBlock / kWave;
+constexpr int kGroups = kTile / 4 / kSpBlock;  // float4 groups per thread (2)
+constexpr uint32_t kRowStride = 4u * kSpBlock; // elements per row of groups (2048)
+constexpr uint32_t kKeyCache = 1024;         // heavy keys / indices staged in LDS per tile
+constexpr uint32_t kRowCache = 512;          // heavy rows staged in LDS per tile (table pass)
+constexpr uint32_t kSlotStage = 1024;        // heavy slots staged in LDS per round (aggregate pass)
+
+// Tile layout of the streaming passes: 512 threads per tile, thread t holds
+// the float4 groups t + 512 j (j < 2), i.e. elements 2048 j + 4 t + c (c < 4)
+// -- the loads and the row stores are coalesced as they are, the scans run
+// per row of groups, and two groups per thread keep the registers low enough
+// for 8 waves per SIMD.
+
+// rint(w 2^K) for a float w >= 0 with w 2^K < 2^64: the scaling is exact in
+// FP32, rintf rounds below 2^23 (where the float's bits run out), and the
+// integer-valued float converts as its significand shifted left (y >= 2^23)
+// or by the exact 32-bit conversion (y < 2^23).
+__device__ __forceinline__ uint64_t qkey_round(float w, int K) {
+    const float y = rintf(ldexpf(w, K));
+    const uint32_t b = __float_as_uint(y) & 0x7fffffffu;
+    const int ex = (int)(b >> 23) - 150;
+    const uint64_t sig = (uint64_t)((b & 0x7fffffu) | 0x800000u);
+    return ex >= 0 ? sig << (ex & 63) : (uint64_t)(uint32_t)y;
+}
+// The same value as qkey_round for w 2^K < 2^64: a normal float whose scaled
+// value is an integer (exponent field e with e - 150 + K >= 0) is its
+// significand shifted left; the others (zero, subnormal, small weights) take
+// the rounding route, behind a wave-uniform test so that a wave without such
+// a weight skips it.
+__device__ __forceinline__ bool qkey_exact(float w, int K) {
+    const int e = (int)(__float_as_uint(w) >> 23);
+    return e != 0 && e - 150 + K >= 0;
+}
+__device__ __forceinline__ uint64_t qkey(float w, int K, bool wave_slow) {
+    const uint32_t b = __float_as_uint(w);
+    const int e = (int)(b >> 23);
+    const uint64_t sig = (uint64_t)((b & 0x7fffffu) | 0x800000u);
+    uint64_t q = sig << ((e - 150 + K) & 63);
+    if (wave_slow) {
+        if (!qkey_exact(w, K)) q = qkey_round(w, K);
+    }
+    return q;
+}
+
+__device__ __forceinline__ FX fx_of(u128 v) {
+    return FX{(uint64_t)v & kMask63, (uint64_t)(v >> 63)};
+}
+__device__ __forceinline__ u128 u128_of(uint64_t lo63, uint64_t hi) {
+    return ((u128)hi << 63) | (u128)lo63;
+}
+// nearest double of v
+__device__ __forceinline__ double u128_to_double(u128 v) {
+    const uint64_t hi = (uint64_t)(v >> 64), lo = (uint64_t)v;
+    if (hi == 0) return (double)lo;
+    const int sh = 64 - __clzll((long long)hi);  // 1..64: v >> sh fits 64 bits
+    const u128 t = v >> sh;
+    const uint64_t sticky = (v & (((u128)1 << sh) - 1)) != 0 ? 1u : 0u;
+    return ldexp((double)((uint64_t)t | sticky), sh);
+}
+
+template <bool F32>
+__device__ __forceinline__ float ratio(double x, const Params* prm) {
+    if constexpr (F32) return (float)((float)x / (float)prm->mean);
+    else return (float)(x * prm->inv_mean);
+}
+
+// The 4 float4 groups of this thread (zero past m; scalar loads where w is
+// not 16-byte aligned or a group straddles m).
+__device__ __forceinline__ void nat_load(const float* __restrict__ w, uint32_t base, uint32_t m,
+                                         bool aligned, float (&x)[4 * kGroups]) {
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kGroups; ++j) {
+        const uint32_t e0 = kRowStride * j + 4u * t;
+        if (aligned && e0 + 3u < m) {
+            const float4 r = ld_nt4(reinterpret_cast<const float4*>(w + base + e0));
+            x[4 * j] = r.x;
+            x[4 * j + 1] = r.y;
+            x[4 * j + 2] = r.z;
+            x[4 * j + 3] = r.w;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) x[4 * j + c] = e0 + c < m ? w[base + e0 + c] : 0.0f;
+        }
+    }
+}
+
+// Per-row wave scans of the thread's per-group light counts and key sums:
+// lane 63 of each wave leaves the wave totals in s_c / s_s[kSpWaves * j +
+// wid]; excl_c / excl_s are the thread's exclusive prefixes within its wave.
+struct NatScan {
+    uint32_t excl_c[kGroups];
+    uint64_t excl_s[kGroups];
+};
+__device__ __forceinline__ void nat_scan(const uint32_t (&lc)[kGroups],
+                                         const uint64_t (&ls)[kGroups], uint32_t* s_c,
+                                         uint64_t* s_s, NatScan& o) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int j = 0; j < kGroups; ++j) {
+        const uint32_t ci = wave_incl_scan_u32(lc[j]);
+        const uint64_t si = wave_incl_scan_u64(ls[j]);  // a wave's row: < 64 * 4 * 2^54
+        if (lane == kWave - 1) {
+            s_c[kSpWaves * j + wid] = ci;
+            s_s[kSpWaves * j + wid] = si;
+        }
+        o.excl_c[j] = ci - lc[j];
+        o.excl_s[j] = si - ls[j];
+    }
+}
+// Exclusive prefixes of the (row, wave) totals in element order (row-major),
+// by the first kSpWaves * kGroups lanes of the block; the caller
+// synchronises before and after.
+constexpr int kSegs = kSpWaves * kGroups;
+__device__ __forceinline__ void nat_prefix(const uint32_t* s_c, const uint64_t* s_s,
+                                           uint32_t* s_pc, u128* s_ps) {
+    const int i = threadIdx.x;
+    if (i < kSegs) {
+        uint32_t c = 0;
+        u128 sm = 0;
+        for (int k = 0; k < i; ++k) {
+            c += s_c[k];
+            sm += (u128)s_s[k];
+        }
+        s_pc[i] = c;
+        s_ps[i] = sm;
+    }
+}
+__device__ __forceinline__ void nat_prefix_c(const uint32_t* s_c, uint32_t* s_pc) {
+    const int i = threadIdx.x;
+    if (i < kSegs) {
+        uint32_t c = 0;
+        for (int k = 0; k < i; ++k) c += s_c[k];
+        s_pc[i] = c;
+    }
+}
+__device__ __forceinline__ bool le128(u128 a, u128 b) {
+    const uint64_t ah = (uint64_t)(a >> 64), bh = (uint64_t)(b >> 64);
+    return ah < bh || (ah == bh && (uint64_t)a <= (uint64_t)b);
+}
+
+// X(r0): the key before the first light of tile t (t == tiles: X(L)).
+__device__ __forceinline__ u128 tile_key(const uint64_t* __restrict__ states, uint32_t n,
+                                         uint32_t tiles, uint32_t t, uint64_t mk) {
+    const uint64_t* ex;
+    u128 lp;
+    uint32_t h0;
+    if (t < tiles) {
+        ex = states + (uint64_t)kExclBase * tiles + (uint64_t)t * kCarryWords;
+        lp = u128_of(ex[2], ex[3]);
+        h0 = (uint32_t)ex[4];
+    } else {
+        ex = states + (uint64_t)kExclBase * tiles + (uint64_t)(tiles - 1u) * kCarryWords;
+        const uint64_t* ag = states + (uint64_t)(tiles - 1u) * kCarryWords;
+        lp = u128_of(ex[2], ex[3]) + u128_of(ag[2], ag[3]);
+        h0 = (uint32_t)(ex[4] + ag[4]);
+    }
+    const uint64_t r0 = min((uint64_t)t * kTile, (uint64_t)n) - h0;  // lights before the tile
+    return (u128)r0 * (u128)mk - lp;
+}
+
+// Heavy keys and indices of one tile's range [q0, q0 + kc), read from LDS
+// (ranges that fit the tile's cache) or from memory (the others).  Two types
+// rather than one with a test per access: a select between an LDS and a
+// global pointer would make every access a flat load that waits for both.
+struct LdsKeys {
+    const u128* k;
+    const uint32_t* i;
+    __device__ __forceinline__ u128 key(uint32_t a) const { return k[a]; }
+    __device__ __forceinline__ uint32_t idx(uint32_t a) const { return i[a]; }
+};
+struct MemKeys {
+    const u128* k;
+    const uint32_t* i;
+    __device__ __forceinline__ u128 key(uint32_t a) const { return k[a]; }
+    __device__ __forceinline__ uint32_t idx(uint32_t a) const { return i[a]; }
+};
+// #{a < kc : key(a) <= x}
+template <class KV>
+__device__ __forceinline__ uint32_t count_le(const KV& kv, u128 x, uint32_t kc) {
+    uint32_t a = 0, b = kc;
+    while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        if (le128(kv.key(mid), x)) a = mid + 1u;
+        else b = mid;
+    }
+    return a;
+}
+
+}  // namespace
+
+// ---- pass 2a: tile records and heavy slots ---------------------------------
+__global__ __launch_bounds__(kSpBlock) void k_sp_agg(const float* __restrict__ w, uint32_t n,
+                                                   const Params* __restrict__ prm,
+                                                   uint64_t* __restrict__ states,
+                                                   uint32_t* __restrict__ slot_idx,
+                                                   float* __restrict__ slot_w) {
+    __shared__ uint32_t s_c[kSpWaves * kGroups];
+    __shared__ uint64_t s_l[kSpWaves], s_hl[kSpWaves], s_hh[kSpWaves];
+    __shared__ uint32_t s_si[kSlotStage];
+    __shared__ float s_sw[kSlotStage];
+    if (prm->error) return;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wid = t / kWave;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
+    float x[4 * kGroups];
+    nat_load(w, base, m, aligned, x);
+    const float pivot = prm->pivot;
+    const int K = prm->key_k;
+    // heavy keys are multiples of 2^30 (w >= mean): summed as key / 2^30 (< 2^57)
+    bool slow = false;
+#pragma unroll
+    for (int k = 0; k < 4 * kGroups; ++k)
+        slow |= !qkey_exact(x[k], x[k] > pivot ? K - 30 : K);
+    slow = __any(slow);
+    uint32_t hc[kGroups];
+    uint32_t hm = 0;
+    uint64_t ls = 0, hs = 0;
+#pragma unroll
+    for (int j = 0; j < kGroups; ++j) {
+        const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+        hc[j] = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float v = x[4 * j + c];
+            const bool heavy = v > pivot && e0 + c < m;
+            const uint64_t q = qkey(v, heavy ? K - 30 : K, slow);
+            ls += heavy ? 0ull : q;  // weights past m are zero
+            hs += heavy ? q : 0ull;
+            hm |= (heavy ? 1u : 0u) << (4 * j + c);
+            hc[j] += heavy ? 1u : 0u;
+        }
+    }
+    uint32_t ex_c[kGroups];
+#pragma unroll
+    for (int j = 0; j < kGroups; ++j) {
+        const uint32_t ci = wave_incl_scan_u32(hc[j]);
+        if (lane == kWave - 1) s_c[kSpWaves * j + wid] = ci;
+        ex_c[j] = ci - hc[j];
+    }
+    const uint64_t lt = wave_incl_scan_u64(ls);  // < 64 * 16 * 2^54
+    const uint64_t hl = wave_incl_scan_u64(hs & 0xffffffffull);
+    const uint64_t hh = wave_incl_scan_u64(hs >> 32);
+    if (lane == kWave - 1) {
+        s_l[wid] = lt;
+        s_hl[wid] = hl;
+        s_hh[wid] = hh;
+    }
+    __syncthreads();
+    // tile-local slot of the thread's first heavy in each group
+    uint32_t gs[kGroups];
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < kGroups; ++j) {
+#pragma unroll
+        for (int q = 0; q < kSpWaves; ++q) {
+            if (q == wid) gs[j] = run + ex_c[j];
+            run += s_c[kSpWaves * j + q];
+        }
+    }
+    // the heavies in element order into their slots, through LDS in rounds of
+    // kSlotStage so that the stores are contiguous runs
+    for (uint32_t r0 = 0; r0 < run; r0 += kSlotStage) {  // block-uniform
+#pragma unroll
+        for (int j = 0; j < kGroups; ++j) {
+            uint32_t slot = gs[j];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((hm >> (4 * j + c)) & 1u) {
+                    if (slot - r0 < kSlotStage) {
+                        s_si[slot - r0] = base + kRowStride * j + 4u * (uint32_t)t + (uint32_t)c;
+                        s_sw[slot - r0] = x[4 * j + c];
+                    }
+                    ++slot;
+                }
+            }
+        }
+        __syncthreads();
+        const uint32_t nr = min(kSlotStage, run - r0);
+        for (uint32_t i = (uint32_t)t; i < nr; i += kSpBlock) {
+            slot_idx[base + r0 + i] = s_si[i];
+            slot_w[base + r0 + i] = s_sw[i];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        u128 L = 0, Hl = 0, Hh = 0;
+#pragma unroll
+        for (int q = 0; q < kSpWaves; ++q) {
+            L += (u128)s_l[q];
+            Hl += (u128)s_hl[q];
+            Hh += (u128)s_hh[q];
+        }
+        const u128 Hs = (Hl + (Hh << 32)) << 30;
+        const FX h = fx_of(Hs), l = fx_of(L);
+        uint64_t* rec = states + (uint64_t)tile * kCarryWords;
+        rec[0] = h.lo;
+        rec[1] = h.hi;
+        rec[2] = l.lo;
+        rec[3] = l.hi;
+        rec[4] = run;
+    }
+}
+
+// ---- pass 3a: the heavies' keys in heavy order (one wave per tile) ----------
+__global__ __launch_bounds__(kWave) void k_sp_heavy(uint32_t n, const Params* __restrict__ prm,
+                                                    const uint64_t* __restrict__ states,
+                                                    const uint32_t* __restrict__ slot_idx,
+                                                    const float* __restrict__ slot_w,
+                                                    u128* __restrict__ hkey,
+                                                    uint32_t* __restrict__ hidx) {
+    if (prm->error) return;
+    const uint32_t H = prm->heavy_count;
+    if (H == 0 || H == n) return;  // identity table
+    const int lane = lane_id();
+    const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint64_t* ag = states + (uint64_t)tile * kCarryWords;
+    const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
+    const uint32_t ct = (uint32_t)ag[4];
+    if (ct == 0) return;
+    const uint32_t h0 = (uint32_t)ex[4];
+    const uint64_t mk = prm->mean_k;
+    const int K = prm->key_k;
+    u128 run = u128_of(ex[0], ex[1]);  // HP(h0)
+    for (uint32_t r = 0; r < ct; r += kWave) {  // wave-uniform
+        const uint32_t k = r + (uint32_t)lane;
+        const bool act = k < ct;
+        const float v = act ? slot_w[base + k] : 0.0f;
+        const uint32_t id = act ? slot_idx[base + k] : 0u;
+        // a heavy key is a multiple of 2^30 (w >= mean): key / 2^30 < 2^57, as k_sp_agg sums it
+        const u128 q = act ? (u128)qkey_round(v, K - 30) << 30 : (u128)0;  // three limbs
+        const uint64_t s0 = wave_incl_scan_u64((uint64_t)(uint32_t)q);
+        const uint64_t s1 = wave_incl_scan_u64((uint64_t)(uint32_t)(q >> 32));
+        const uint64_t s2 = wave_incl_scan_u64((uint64_t)(q >> 64));
+        const u128 inc = (u128)s0 + ((u128)s1 << 32) + ((u128)s2 << 64);
+        if (act) {
+            const uint32_t qi = h0 + k;
+            hkey[qi] = run + inc - (u128)(qi + 1u) * (u128)mk;  // Y(qi + 1)
+            hidx[qi] = id;
+        }
+        const uint64_t t0 = __shfl(s0, kWave - 1, kWave), t1 = __shfl(s1, kWave - 1, kWave),
+                       t2 = __shfl(s2, kWave - 1, kWave);
+        run += (u128)t0 + ((u128)t1 << 32) + ((u128)t2 << 64);
+    }
+}
+
+// ---- pass 3b: the alias range of every tile ----------------------------------
+// One wave per tile: qa[t] = first q in [0, H-1) with Y(q+1) > X(r0(t)) (H-1
+// if none) by a 64-ary search over hkey; qa[tiles] = H-1.
+__global__ __launch_bounds__(kBlock) void k_sp_qrange(uint32_t n, const Params* __restrict__ prm,
+                                                      const uint64_t* __restrict__ states,
+                                                      const u128* __restrict__ hkey,
+                                                      uint32_t* __restrict__ qa) {
+    if (prm->error) return;
+    const int lane = lane_id();
+    const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+    const uint32_t t = blockIdx.x * (uint32_t)kWaves + threadIdx.x / kWave;
+    if (t > tiles) return;  // the whole wave
+    const uint32_t H = prm->heavy_count;
+    if (H == 0 || H == n) {
+        if (lane == 0) qa[t] = 0;
+        return;
+    }
+    const uint32_t hq = H - 1u;
+    if (t == tiles) {
+        if (lane == 0) qa[t] = hq;
+        return;
+    }
+    const u128 x0 = tile_key(states, n, tiles, t, prm->mean_k);
+    uint32_t lo = 0, hi = hq;  // the answer lies in [lo, hi]
+    while (lo < hi) {          // wave-uniform
+        const uint32_t R = hi - lo;
+        const uint32_t step = (R + (uint32_t)kWave - 1u) / (uint32_t)kWave;
+        const uint32_t p = lo + (uint32_t)lane * step;
+        const bool in = (uint64_t)lane * step < R;
+        const bool pr = in && hkey[p] > x0;
+        const uint64_t b = __ballot(pr);
+        const uint64_t bi = __ballot(in);
+        if (b) {
+            const int f = __builtin_ctzll(b);
+            const uint32_t pf = lo + (uint32_t)f * step;
+            lo = f == 0 ? lo : lo + (uint32_t)(f - 1) * step + 1u;
+            hi = pf;
+        } else {
+            const int last = 63 - __builtin_clzll(bi);
+            lo = lo + (uint32_t)last * step + 1u;
+        }
+    }
+    if (lane == 0) qa[t] = lo;
+}
+
+// ---- pass 3c: rows of the heavies that cross among a tile's lights -----------
+// Each heavy q the tile owns gets its row (p, heavy q+1) in hrow[q] and the
+// tile offset of the light it crosses at in cross[q] (0xffff past the last
+// light): the table pass counts those crossings instead of comparing keys.
+template <bool F32>
+__global__ __launch_bounds__(kSpBlock) void k_sp_rows(const float* __restrict__ w, uint32_t n,
+                                                      const Params* __restrict__ prm,
+                                                      const uint64_t* __restrict__ states,
+                                                      const u128* __restrict__ hkey,
+                                                      const uint32_t* __restrict__ hidx,
+                                                      const uint32_t* __restrict__ qa,
+                                                      AliasEntry* __restrict__ hrow,
+                                                      uint16_t* __restrict__ cross) {
+    __shared__ u128 s_key[kKeyCache];
+    __shared__ float s_p[kKeyCache];
+    __shared__ uint16_t s_e[kKeyCache];
+    __shared__ uint32_t s_c[kSegs], s_pc[kSegs];
+    __shared__ uint64_t s_s[kSegs];
+    __shared__ u128 s_ps[kSegs];
+    if (prm->error) return;
+    const uint32_t H = prm->heavy_count;
+    if (H == 0 || H == n) return;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t q0 = qa[tile], q1 = qa[tile + 1u];
+    if (q0 >= q1) return;  // no crossing among this tile's lights: its weights are not read
+    const int t = threadIdx.x;
+    const int wid = t / kWave;
+    const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const int K = prm->key_k;
+    const uint64_t mk = prm->mean_k;
+    const float pivot = prm->pivot;
+    const u128 x0 = tile_key(states, n, tiles, tile, mk);
+    const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
+    float x[4 * kGroups];
+    nat_load(w, base, m, aligned, x);
+    // the keys that cross among the tile's lights (all of them its own)
+    const uint32_t kc = q1 - q0;
+    const bool cached = kc <= kKeyCache;
+    u128 sk[kKeyCache / kSpBlock];
+    if (cached) {
+#pragma unroll
+        for (int r = 0; r < (int)(kKeyCache / kSpBlock); ++r) {
+            const uint32_t i = (uint32_t)t + kSpBlock * r;
+            if (i < kc) sk[r] = hkey[q0 + i];
+        }
+    }
+    bool slow = false;
+    uint32_t lc[kGroups];
+    uint64_t ls[kGroups];
+    uint32_t lm = 0;
+#pragma unroll
+    for (int k = 0; k < 4 * kGroups; ++k) slow |= !(x[k] > pivot) && !qkey_exact(x[k], K);
+    slow = __any(slow);
+#pragma unroll
+    for (int j = 0; j < kGroups; ++j) {
+        const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+        lc[j] = 0;
+        ls[j] = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool light = !(x[4 * j + c] > pivot) && e0 + c < m;
+            lm |= (light ? 1u : 0u) << (4 * j + c);
+            lc[j] += light ? 1u : 0u;
+            ls[j] += light ? qkey(x[4 * j + c], K, slow) : 0ull;
+        }
+    }
+    NatScan ns;
+    nat_scan(lc, ls, s_c, s_s, ns);
+    if (cached) {
+#pragma unroll
+        for (int r = 0; r < (int)(kKeyCache / kSpBlock); ++r) {
+            const uint32_t i = (uint32_t)t + kSpBlock * r;
+            if (i < kc) s_key[i] = sk[r];
+        }
+    }
+    __syncthreads();
+    nat_prefix(s_c, s_s, s_pc, s_ps);
+    __syncthreads();
+    // the walk over the thread's lights; emit(a, p, e) records heavy q0 + a
+    auto walk = [&](const auto& kv, const auto& emit) {
+#pragma unroll
+        for (int j = 0; j < kGroups; ++j) {
+            if (((lm >> (4 * j)) & 15u) == 0) continue;
+            const uint32_t bc = s_pc[kSpWaves * j + wid] + ns.excl_c[j];
+            const u128 bs = s_ps[kSpWaves * j + wid] + (u128)ns.excl_s[j];
+            u128 xk = x0 + (u128)bc * mk - bs;  // before the group's first light
+            uint32_t a = count_le(kv, xk, kc);  // keys at or below it: earlier groups'
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((lm >> (4 * j + c)) & 1u) {
+                    const u128 xn = xk + (u128)(mk - qkey(x[4 * j + c], K, slow));
+                    // heavies whose key lies in (xk, xn]: their crossing is this light
+                    while (a < kc) {
+                        const u128 y = kv.key(a);
+                        if (!le128(y, xn)) break;
+                        const double r = ldexp(u128_to_double(y + (u128)mk - xn), -K);
+                        emit(a, fminf(fmaxf(ratio<F32>(r, prm), 0.0f), 1.0f),
+                             kRowStride * j + 4u * (uint32_t)t + (uint32_t)c);
+                        ++a;
+                    }
+                    xk = xn;
+                }
+            }
+        }
+        if (tile + 1u == tiles) {
+            // heavies past the last light (Y(q+1) > X(L), q < H-1): r(q) = L
+            const u128 xl = tile_key(states, n, tiles, tiles, mk);
+            for (uint32_t a = (uint32_t)t; a < kc; a += kSpBlock) {
+                const u128 y = kv.key(a);
+                if (le128(y, xl)) continue;
+                const double r = ldexp(u128_to_double(y + (u128)mk - xl), -K);
+                emit(a, fminf(fmaxf(ratio<F32>(r, prm), 0.0f), 1.0f), 0xffffu);
+            }
+        }
+    };
+    if (cached) {
+        // rows staged in LDS, written as contiguous runs
+        walk(LdsKeys{s_key, nullptr}, [&](uint32_t a, float p, uint32_t e) {
+            s_p[a] = p;
+            s_e[a] = (uint16_t)e;
+        });
+        __syncthreads();
+        for (uint32_t i = (uint32_t)t; i < kc; i += kSpBlock) {
+            hrow[q0 + i] = AliasEntry{s_p[i], hidx[q0 + 1u + i]};
+            cross[q0 + i] = s_e[i];
+        }
+    } else {
+        walk(MemKeys{hkey + q0, nullptr}, [&](uint32_t a, float p, uint32_t e) {
+            hrow[q0 + a] = AliasEntry{p, hidx[q0 + 1u + a]};
+            cross[q0 + a] = (uint16_t)e;
+        });
+    }
+}
+
+// ---- pass 4: the table, in element order -------------------------------------
+// A light's alias is heavy q0 + (the crossings at lights before it in the
+// tile); a heavy's row comes from hrow.  No keys are compared here.
+template <bool F32>
+__global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__ w, uint32_t n,
+                                                       const Params* __restrict__ prm,
+                                                       const uint64_t* __restrict__ states,
+                                                       const uint32_t* __restrict__ hidx,
+                                                       const uint16_t* __restrict__ cross,
+                                                       const uint32_t* __restrict__ qa,
+                                                       const AliasEntry* __restrict__ hrow,
+                                                       AliasEntry* __restrict__ table) {
+    __shared__ uint32_t s_cnt[kTile];           // crossings at each element of the tile
+    __shared__ uint32_t s_idx[kKeyCache + 1];   // hidx[q0 .. q0 + kc]
+    __shared__ AliasEntry s_row[kRowCache];     // hrow[h0 ..]
+    __shared__ uint32_t s_tc[kSegs], s_th[kSegs], s_pc[kSegs], s_ph[kSegs];
+    if (prm->error) return;
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int wid = t / kWave;
+    const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)kTile;
+    const uint32_t m = min((uint32_t)kTile, n - base);
+    const uint32_t H = prm->heavy_count;
+    const bool tab16 = (reinterpret_cast<uintptr_t>(table) & 15u) == 0;
+    uint2 row[4];
+    // rows of group j: two 16-byte stores where the table allows, else per row
+    auto store = [&](int j) {
+        const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+        if (tab16 && e0 + 3u < m) {
+            uint4* o = reinterpret_cast<uint4*>(table + base + e0);
+            o[0] = make_uint4(row[0].x, row[0].y, row[1].x, row[1].y);
+            o[1] = make_uint4(row[2].x, row[2].y, row[3].x, row[3].y);
+        } else {
+            uint2* o = reinterpret_cast<uint2*>(table + base + e0);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (e0 + c < m) o[c] = row[c];
+        }
+    };
+    if (H == 0 || H == n) {  // all weights equal the mean up to its rounding: identity
+#pragma unroll
+        for (int j = 0; j < kGroups; ++j) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                row[c] = make_uint2(__float_as_uint(1.0f), base + kRowStride * j + 4u * t + c);
+            store(j);
+        }
+        return;
+    }
+    const float pivot = prm->pivot;
+    const uint32_t q0 = qa[tile], q1 = qa[tile + 1u];
+    const uint64_t* ag = states + (uint64_t)tile * kCarryWords;
+    const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
+    const uint32_t h0 = (uint32_t)ex[4];
+    const uint32_t ct = (uint32_t)ag[4];
+    const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
+    float x[4 * kGroups];
+    nat_load(w, base, m, aligned, x);
+    const uint32_t kc = q1 - q0;
+    const bool cached = kc <= kKeyCache && ct <= kRowCache;
+    constexpr int kPer = (int)(kKeyCache / kSpBlock);
+    uint32_t cr[kPer], si[kPer];
+    AliasEntry sr = AliasEntry{0.0f, 0u};
+    if (cached) {
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t i = (uint32_t)t + kSpBlock * r;
+            cr[r] = i < kc ? cross[q0 + i] : 0xffffu;
+            si[r] = i < kc ? hidx[q0 + i] : 0u;
+        }
+        if ((uint32_t)t < ct) sr = hrow[h0 + t];
+    }
+    const uint32_t si_last = t == 0 ? hidx[q1] : 0u;  // alias of the tile's last lights
+    // crossing counts: zero, then one LDS atomic per crossing
+    for (uint32_t i = (uint32_t)t; i < (uint32_t)kTile / 4u; i += kSpBlock)
+        reinterpret_cast<uint4*>(s_cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    if (cached) {
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t i = (uint32_t)t + kSpBlock * r;
+            if (cr[r] < (uint32_t)kTile) atomicAdd(&s_cnt[cr[r]], 1u);
+            if (i < kc) s_idx[i] = si[r];
+        }
+        if ((uint32_t)t < ct) s_row[t] = sr;
+        if (t == 0) s_idx[kc] = si_last;
+    } else {
+        for (uint32_t i = (uint32_t)t; i < kc; i += kSpBlock) {
+            const uint32_t e = cross[q0 + i];
+            if (e < (uint32_t)kTile) atomicAdd(&s_cnt[e], 1u);
+        }
+    }
+    __syncthreads();
+    // heavy and crossing counts of the thread's groups, scanned per row
+    uint32_t hm = 0, lm = 0;
+    uint32_t cnt[4 * kGroups];
+    uint32_t ex_h[kGroups], ex_c[kGroups];
+#pragma unroll
+    for (int j = 0; j < kGroups; ++j) {
+        const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+        const uint4 cq = *reinterpret_cast<const uint4*>(&s_cnt[e0]);
+        cnt[4 * j] = cq.x;
+        cnt[4 * j + 1] = cq.y;
+        cnt[4 * j + 2] = cq.z;
+        cnt[4 * j + 3] = cq.w;
+        uint32_t hc = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool valid = e0 + c < m;
+            const bool heavy = x[4 * j + c] > pivot;
+            hm |= (heavy && valid ? 1u : 0u) << (4 * j + c);
+            lm |= (!heavy && valid ? 1u : 0u) << (4 * j + c);
+            hc += heavy && valid ? 1u : 0u;
+        }
+        const uint32_t cc = cq.x + cq.y + cq.z + cq.w;
+        const uint32_t hi = wave_incl_scan_u32(hc);
+        const uint32_t ci = wave_incl_scan_u32(cc);
+        if (lane == kWave - 1) {
+            s_th[kSpWaves * j + wid] = hi;
+            s_tc[kSpWaves * j + wid] = ci;
+        }
+        ex_h[j] = hi - hc;
+        ex_c[j] = ci - cc;
+    }
+    __syncthreads();
+    nat_prefix_c(s_th, s_ph);
+    nat_prefix_c(s_tc, s_pc);
+    __syncthreads();
+    auto rows = [&](const auto& kv, const auto* hr_src) {
+#pragma unroll
+        for (int j = 0; j < kGroups; ++j) {
+            const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+            uint32_t hr = s_ph[kSpWaves * j + wid] + ex_h[j];  // heavies of the tile before
+            uint32_t cx = s_pc[kSpWaves * j + wid] + ex_c[j];  // crossings before the group
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float v = x[4 * j + c];
+                if ((hm >> (4 * j + c)) & 1u) {
+                    if (h0 + hr + 1u == H) {
+                        row[c] = make_uint2(__float_as_uint(1.0f), base + e0 + c);
+                    } else {
+                        const AliasEntry e = hr_src[hr];
+                        row[c] = make_uint2(__float_as_uint(e.p), e.a);
+                    }
+                    ++hr;
+                } else if ((lm >> (4 * j + c)) & 1u) {
+                    const float p = fminf(ratio<F32>((double)v, prm), 1.0f);
+                    row[c] = make_uint2(__float_as_uint(p), kv.idx(cx));
+                } else {
+                    row[c] = make_uint2(0u, 0u);  // past m: not stored
+                }
+                cx += cnt[4 * j + c];  // heavies packed after this element's light
+            }
+            store(j);
+        }
+    };
+    if (cached)
+        rows(LdsKeys{nullptr, s_idx}, s_row);
+    else
+        rows(MemKeys{nullptr, hidx + q0}, hrow + h0);
+}
+
+// ---- launchers -----------------------------------------------------------------
+hipError_t launch_stream_agg(const float* w, uint64_t n, const Params* prm, uint64_t* states,
+                             const StreamBufs& b, hipStream_t s) {
+    const uint32_t tiles = (uint32_t)tiles_for(n);
+    k_sp_agg<<<tiles, kSpBlock, 0, s>>>(w, (uint32_t)n, prm, states, b.slot_idx, b.slot_w);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_pack(const float* w, uint64_t n, int prec, const Params* prm,
+                              const uint64_t* states, const StreamBufs& b, AliasEntry* table,
+                              hipStream_t s, hipEvent_t ev_keys, hipEvent_t ev_rows) {
+    const uint32_t nn = (uint32_t)n;
+    const uint32_t tiles = (uint32_t)tiles_for(n);
+    u128* hkey = static_cast<u128*>(b.hkey);
+    k_sp_heavy<<<tiles, kWave, 0, s>>>(nn, prm, states, b.slot_idx, b.slot_w, hkey, b.hidx);
+    k_sp_qrange<<<(tiles + 1u + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, hkey, b.qa);
+    if (ev_keys) (void)hipEventRecord(ev_keys, s);
+    if (prec == kFP32) {
+        k_sp_rows<true><<<tiles, kSpBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx, b.qa, b.hrow,
+                                                   b.cross);
+        if (ev_rows) (void)hipEventRecord(ev_rows, s);
+        k_sp_table<true><<<tiles, kSpBlock, 0, s>>>(w, nn, prm, states, b.hidx, b.cross, b.qa,
+                                                    b.hrow, table);
+    } else {
+        k_sp_rows<false><<<tiles, kSpBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx, b.qa,
+                                                    b.hrow, b.cross);
+        if (ev_rows) (void)hipEventRecord(ev_rows, s);
+        k_sp_table<false><<<tiles, kSpBlock, 0, s>>>(w, nn, prm, states, b.hidx, b.cross, b.qa,
+                                                     b.hrow, table);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace wrsk

@@ -83,9 +83,31 @@ void Workspace::ensure_splits(uint64_t K) {
     split_cap = K;
 }
 
+void Workspace::ensure_stream() {
+    if (skey) return;
+    try {
+        skey = dev_alloc<uint64_t>(2 * max_n);
+        shidx = dev_alloc<uint32_t>(max_n);
+        shrow = dev_alloc<wrsk::AliasEntry>(max_n);
+        scross = dev_alloc<uint16_t>(max_n);
+        sqa = dev_alloc<uint32_t>(wrsk::tiles_for(max_n) + 2);
+    } catch (...) {
+        void* ptrs[] = {skey, shidx, shrow, scross, sqa};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        skey = nullptr;
+        shidx = nullptr;
+        shrow = nullptr;
+        scross = nullptr;
+        sqa = nullptr;
+        throw;
+    }
+}
+
 void Workspace::release() {
-    void* ptrs[] = {params,    red,       states, indices, pweights, prefix_hi,
-                    prefix_lo, samp_h,    samp_l, bounds,  lightpos, splits};
+    void* ptrs[] = {params,    red,    states,   indices, pweights, prefix_hi, prefix_lo,
+                    samp_h,    samp_l, bounds,   lightpos, splits,  skey,      shidx,
+                    shrow,     scross, sqa};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     params = nullptr;
@@ -101,6 +123,11 @@ void Workspace::release() {
     lightpos = nullptr;
     splits = nullptr;
     split_cap = 0;
+    skey = nullptr;
+    shidx = nullptr;
+    shrow = nullptr;
+    scross = nullptr;
+    sqa = nullptr;
 }
 
 static void require(bool cond, const char* msg) {
@@ -118,7 +145,7 @@ void require_tuning(const wrs_tuning& t) {
     require(wrsk::merge_shape_supported(merge_shape_of(t)),
             "unsupported merge shape (merge_tile 1024/2048 with merge_per 2/4/8, or 4096 with 4/8)");
     require(t.partition <= 1u, "partition must be 0 or 1");
-    require(t.direct <= WRS_PACK_SUBGROUP, "direct must be a WRS_PACK_* value");
+    require(t.direct <= WRS_PACK_STREAM, "direct must be a WRS_PACK_* value");
 }
 
 void Mean::run(hipStream_t s, const float* weights, uint64_t N, Workspace& ws, double pivot) const {
@@ -186,6 +213,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
         ls = wrsk::LightSrc{b.weights, ws.lightpos};
     const uint32_t G = wrsk::partials_for(N);
     const uint64_t words = wrsk::state_words_for(N);
+    if (form == WRS_PACK_STREAM && !dbg) ws.ensure_stream();
     if (ev) check_hip(hipEventRecord(ev[0], s), "event");
     check_hip(wrsk::launch_reduce_final(b.weights, N, (int)prec, ws.red, G, ws.states, words, 0,
                                         0.0, ws.params, s, ws.wait_polls, ws.hooks),
@@ -193,6 +221,22 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     // the finalize runs inside the reduce (its last block): an empty phase
     if (ev) check_hip(hipEventRecord(ev[1], s), "event");
     if (ev) check_hip(hipEventRecord(ev[2], s), "event");
+    if (form == WRS_PACK_STREAM && !dbg) {
+        // phases: tile aggregate, tile scan, heavy keys and ranges, heavy rows, table
+        const wrsk::StreamBufs sb{ws.indices, ws.pweights, ws.skey, ws.shidx,
+                                  ws.shrow,    ws.scross,   ws.sqa};
+        check_hip(wrsk::launch_stream_agg(b.weights, N, ws.params, ws.states, sb, s),
+                  "stream aggregate launch");
+        if (ev) check_hip(hipEventRecord(ev[3], s), "event");
+        check_hip(wrsk::launch_key_tilescan(ws.states, N, ws.params, s), "tile scan launch");
+        if (ev) check_hip(hipEventRecord(ev[4], s), "event");
+        check_hip(wrsk::launch_stream_pack(b.weights, N, (int)prec, ws.params, ws.states, sb,
+                                           reinterpret_cast<wrsk::AliasEntry*>(b.aliasTable), s,
+                                           ev ? ev[5] : nullptr, ev ? ev[6] : nullptr),
+                  "stream pack launch");
+        if (ev) check_hip(hipEventRecord(ev[7], s), "event");
+        return;
+    }
     check_hip(wrsk::launch_partition(b.weights, N, (int)prec, (int)m_cfg.tuning.partition,
                                      ws.params, ws.states, ws.indices,
                                      ws.pweights, hi, lo, ws.samp_h, ws.samp_l,

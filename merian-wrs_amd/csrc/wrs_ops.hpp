@@ -60,6 +60,14 @@ struct Workspace {
                                        // kSample-th light (PSA only)
     wrsk::SplitRec* splits = nullptr;  // [split_cap] split records of the serial pack form
     uint64_t split_cap = 0;
+    // the streamed pack form's heavy keys (16 bytes), element indices, rows and
+    // crossing offsets ([max_n] each) and the tiles' alias ranges
+    // ([tiles_for(max_n) + 1])
+    void* skey = nullptr;
+    uint32_t* shidx = nullptr;
+    wrsk::AliasEntry* shrow = nullptr;
+    uint16_t* scross = nullptr;
+    uint32_t* sqa = nullptr;
 
     static Workspace allocate(uint64_t max_n, bool partition_buffers);
     // the full prefix arrays of debug builds and of the sample statistics,
@@ -67,6 +75,8 @@ struct Workspace {
     void ensure_prefix(bool lo);
     // the split records of the serial pack form, allocated on first use
     void ensure_splits(uint64_t K);
+    // the streamed pack form's buffers, allocated on first use
+    void ensure_stream();
     void release();
 };
 

@@ -19,7 +19,8 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libwrs_amd.so")
 WRS_OK, WRS_EINVAL, WRS_ENOMEM, WRS_EHIP, WRS_ENOTSUP, WRS_EWAIT = range(6)
 PREC_FP32, PREC_FP64_CARRY, PREC_COMPENSATED, PREC_FP64 = 0, 1, 2, 3
 GEN_UNIFORM01, GEN_LOGNORMAL, GEN_ZIPF, GEN_SPIKE = 0, 1, 2, 3
-PACK_DEFAULT, PACK_DIRECT, PACK_CLASSIC, PACK_SERIAL, PACK_INLINE, PACK_SUBGROUP = 0, 1, 2, 3, 4, 5  # wrs_tuning.direct
+PACK_DEFAULT, PACK_DIRECT, PACK_CLASSIC, PACK_SERIAL, PACK_INLINE, PACK_SUBGROUP, PACK_STREAM = (
+    0, 1, 2, 3, 4, 5, 6)  # wrs_tuning.direct
 
 ALIAS_DTYPE = np.dtype([("p", np.float32), ("a", np.uint32)])
 SPLIT_DTYPE = np.dtype([("i", np.uint32), ("j", np.uint32), ("spill", np.float64)])

@@ -107,6 +107,7 @@ def lib():
                                           u32p, C.c_void_p]
     L.oro_psa_alias_table_f64.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
                                           C.c_double, f64p, u32p, C.c_void_p]
+    L.oro_psa_stream_table.argtypes = [f32p, C.c_uint32, C.c_double, C.c_int, f32p, u32p]
     L.oro_sample_alias.argtypes = [f32p, u32p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64,
                                    u32p]
     L.oro_its_sample.argtypes = [f32p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64,
@@ -321,6 +322,17 @@ def psa_alias_table_f64(w, K: int, reference_scale: bool = False, mean=None,
                                    float(dbg.mean), sp.ctypes.data, K, p, a)
     return p, a, dict(mean=dbg.mean, H=H, idx_cpu=idx, heavy_prefix=hp[:H],
                       light_prefix=lp[:n - H], splits=sp)
+
+
+def psa_stream_table(w, mean: float, f32: bool = False):
+    """The reference sweep (pack.hpp:14-83) with exact integer keys, as the
+    streamed pack form computes it: (p float32, a uint32) for the given mean."""
+    w = np.ascontiguousarray(w, np.float32)
+    n = w.size
+    p = np.empty(n, np.float32)
+    a = np.empty(n, np.uint32)
+    lib().oro_psa_stream_table(w, n, float(mean), int(bool(f32)), p, a)
+    return p, a
 
 
 # ------------------------------------------------------------ sampling ----

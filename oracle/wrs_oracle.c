@@ -580,6 +580,78 @@ void oro_psa_alias_table_f64(const float* w_in, uint32_t n, uint32_t K, int refe
     free(w);
 }
 
+/* The sequential sweep of pack.hpp:14-83 (one interval (0,0) -> (L, H-1),
+ * heavy first on ties, last heavy (1, itself) as pack.hpp:78-81) evaluated
+ * with exact integer keys: K = 53 - ilogb(mean), mean_k = mean 2^K (an
+ * integer), every light weight rounded to the nearest integer of w 2^K (exact
+ * at and above mean 2^-30), every heavy one to the nearest multiple of 2^30
+ * (exact for normal floats), X(i) = i mean_k - sum of the first i lights, Y(j) = sum of
+ * the first j heavies - j mean_k.  Heavy q is packed before light i iff
+ * Y(q+1) <= X(i) (the sweep's "residual <= mean"), with residual
+ * Y(q+1) - X(i) + mean_k rounded once to FP64.  Light p = w / mean (FP32
+ * division when f32, else w * (1/mean) in FP64), heavy p = residual / mean the
+ * same way, both rounded to float and clamped to [0, 1]. */
+typedef unsigned __int128 oro_u128;
+
+static oro_u128 qkey(float w, int K) {
+    const float y = rintf(ldexpf(w, K));
+    return (oro_u128)y;
+}
+/* a heavy weight (w > mean) times 2^K is a multiple of 2^30: rounded as w 2^(K-30) */
+static oro_u128 qkey_heavy(float w, int K) { return qkey(w, K - 30) << 30; }
+
+static float stream_ratio(double x, double mean, int f32) {
+    if (f32) return (float)((float)x / (float)mean);
+    return (float)(x * (1.0 / mean));
+}
+
+void oro_psa_stream_table(const float* w, uint32_t n, double mean, int f32, float* p,
+                          uint32_t* a) {
+    uint32_t* hid = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t* lid = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t H = 0, L = 0;
+    for (uint32_t e = 0; e < n; ++e) {
+        if ((double)w[e] > mean) hid[H++] = e;
+        else lid[L++] = e;
+    }
+    if (H == 0 || L == 0) {
+        for (uint32_t e = 0; e < n; ++e) {
+            p[e] = 1.0f;
+            a[e] = e;
+        }
+        free(hid);
+        free(lid);
+        return;
+    }
+    const int K = 53 - ilogb(mean);
+    const oro_u128 mk = (oro_u128)ldexp(mean, K);
+    oro_u128 X = 0;                      /* X(i) */
+    oro_u128 Yn = qkey_heavy(w[hid[0]], K) - mk; /* Y(q+1) */
+    uint32_t i = 0, q = 0;
+    for (;;) {
+        if (q + 1u < H && (i == L || Yn <= X)) {
+            const oro_u128 R = Yn + mk - X;
+            const float r = stream_ratio(ldexp((double)R, -K), mean, f32);
+            p[hid[q]] = fminf(fmaxf(r, 0.0f), 1.0f);
+            a[hid[q]] = hid[q + 1u];
+            ++q;
+            Yn += qkey_heavy(w[hid[q]], K) - mk;
+        } else if (i < L) {
+            const float x = w[lid[i]];
+            p[lid[i]] = fminf(stream_ratio((double)x, mean, f32), 1.0f);
+            a[lid[i]] = hid[q];
+            X += mk - qkey(x, K);
+            ++i;
+        } else {
+            break;
+        }
+    }
+    p[hid[H - 1u]] = 1.0f;
+    a[hid[H - 1u]] = hid[H - 1u];
+    free(hid);
+    free(lid);
+}
+
 /* ======================================================================== */
 /* Sampling                                                                   */
 /* ======================================================================== */

@@ -127,6 +127,13 @@ void oro_psa_alias_table_f64(const float* w, uint32_t n, uint32_t K, int referen
                              int split_rule, double mean_override, double* p, uint32_t* a,
                              oro_psa_debug* dbg);
 
+/* The same sweep with exact integer keys (the product's streamed pack form,
+ * WRS_PACK_STREAM): the mean and every weight times 2^(53 - ilogb(mean)),
+ * rounded to integers; heavy <=> w > mean.  f32 != 0: p divides in FP32 (the
+ * FP32 mode), else multiplies by the FP64 reciprocal. */
+void oro_psa_stream_table(const float* w, uint32_t n, double mean, int f32, float* p,
+                          uint32_t* a);
+
 /* ---- sampling ----------------------------------------------------------- */
 /* alias: src/host/reference/sample_alias_table.hpp:15-41 acceptance rule
  * (u2 > p -> alias) with the Philox stream of the product. */

@@ -79,7 +79,7 @@ def test_bad_tuning_is_rejected():
     L = pywrs.lib()
     plan = C.c_void_p()
     for t in (dict(merge_tile=3000), dict(merge_tile=4096, merge_per=2), dict(merge_per=3),
-              dict(partition=2), dict(direct=6)):
+              dict(partition=2), dict(direct=7)):
         cfg = pywrs._config(32, pywrs.PREC_COMPENSATED, t)
         assert L.wrs_plan_create(C.byref(cfg), 1024, C.byref(plan)) == pywrs.WRS_EINVAL, t
         msg = L.wrs_last_error()
