@@ -59,11 +59,20 @@ namespace {
 using u128 = unsigned __int128;
 
 constexpr int kWaves = kBlock / kWave;       // waves of the heavy-range search blocks
-constexpr int kSpBlock = 512;                // threads per tile of the streaming passes
-constexpr int kSpWaves = kSpBlock / kWave;
-constexpr int kGroups = kTile / 4 / kSpBlock;  // float4 groups per thread (2)
-constexpr uint32_t kRowStride = 4u * kSpBlock; // elements per row of groups (2048)
-constexpr uint32_t kKeyCache = 1024;         // heavy keys / indices staged in LDS per tile
+// Geometry of a streaming pass with B threads per tile: B / 64 waves, each
+// thread kG = 4096 / (4 B) float4 groups, rows of 4 B elements, kS (row, wave)
+// segments.
+template <int B>
+struct Geo {
+    static constexpr int kW = B / kWave;
+    static constexpr int kG = kTile / 4 / B;
+    static constexpr uint32_t kStride = 4u * B;
+    static constexpr int kS = kW * kG;
+};
+constexpr int kAggBlock = 512;    // threads per tile: aggregate pass
+constexpr int kRowsBlock = 512;   // rows pass
+constexpr int kTableBlock = 256;  // table pass
+constexpr uint32_t kKeyCache = 1024;         // alias indices staged in LDS per tile (table pass)
 constexpr uint32_t kRowCache = 512;          // heavy rows staged in LDS per tile (table pass)
 constexpr uint32_t kSlotStage = 1024;        // heavy slots staged in LDS per round (aggregate pass)
 
@@ -128,12 +137,13 @@ __device__ __forceinline__ float ratio(double x, const Params* prm) {
 
 // The 4 float4 groups of this thread (zero past m; scalar loads where w is
 // not 16-byte aligned or a group straddles m).
+template <int B>
 __device__ __forceinline__ void nat_load(const float* __restrict__ w, uint32_t base, uint32_t m,
-                                         bool aligned, float (&x)[4 * kGroups]) {
+                                         bool aligned, float (&x)[4 * Geo<B>::kG]) {
     const uint32_t t = threadIdx.x;
 #pragma unroll
-    for (int j = 0; j < kGroups; ++j) {
-        const uint32_t e0 = kRowStride * j + 4u * t;
+    for (int j = 0; j < Geo<B>::kG; ++j) {
+        const uint32_t e0 = Geo<B>::kStride * j + 4u * t;
         if (aligned && e0 + 3u < m) {
             const float4 r = ld_nt4(reinterpret_cast<const float4*>(w + base + e0));
             x[4 * j] = r.x;
@@ -148,37 +158,39 @@ __device__ __forceinline__ void nat_load(const float* __restrict__ w, uint32_t b
 }
 
 // Per-row wave scans of the thread's per-group light counts and key sums:
-// lane 63 of each wave leaves the wave totals in s_c / s_s[kSpWaves * j +
+// lane 63 of each wave leaves the wave totals in s_c / s_s[Geo<B>::kW * j +
 // wid]; excl_c / excl_s are the thread's exclusive prefixes within its wave.
+template <int B>
 struct NatScan {
-    uint32_t excl_c[kGroups];
-    uint64_t excl_s[kGroups];
+    uint32_t excl_c[Geo<B>::kG];
+    uint64_t excl_s[Geo<B>::kG];
 };
-__device__ __forceinline__ void nat_scan(const uint32_t (&lc)[kGroups],
-                                         const uint64_t (&ls)[kGroups], uint32_t* s_c,
-                                         uint64_t* s_s, NatScan& o) {
+template <int B>
+__device__ __forceinline__ void nat_scan(const uint32_t (&lc)[Geo<B>::kG],
+                                         const uint64_t (&ls)[Geo<B>::kG], uint32_t* s_c,
+                                         uint64_t* s_s, NatScan<B>& o) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
 #pragma unroll
-    for (int j = 0; j < kGroups; ++j) {
+    for (int j = 0; j < Geo<B>::kG; ++j) {
         const uint32_t ci = wave_incl_scan_u32(lc[j]);
         const uint64_t si = wave_incl_scan_u64(ls[j]);  // a wave's row: < 64 * 4 * 2^54
         if (lane == kWave - 1) {
-            s_c[kSpWaves * j + wid] = ci;
-            s_s[kSpWaves * j + wid] = si;
+            s_c[Geo<B>::kW * j + wid] = ci;
+            s_s[Geo<B>::kW * j + wid] = si;
         }
         o.excl_c[j] = ci - lc[j];
         o.excl_s[j] = si - ls[j];
     }
 }
 // Exclusive prefixes of the (row, wave) totals in element order (row-major),
-// by the first kSpWaves * kGroups lanes of the block; the caller
+// by the first Geo<B>::kW * Geo<B>::kG lanes of the block; the caller
 // synchronises before and after.
-constexpr int kSegs = kSpWaves * kGroups;
+template <int B>
 __device__ __forceinline__ void nat_prefix(const uint32_t* s_c, const uint64_t* s_s,
                                            uint32_t* s_pc, u128* s_ps) {
     const int i = threadIdx.x;
-    if (i < kSegs) {
+    if (i < Geo<B>::kS) {
         uint32_t c = 0;
         u128 sm = 0;
         for (int k = 0; k < i; ++k) {
@@ -189,9 +201,10 @@ __device__ __forceinline__ void nat_prefix(const uint32_t* s_c, const uint64_t* 
         s_ps[i] = sm;
     }
 }
+template <int B>
 __device__ __forceinline__ void nat_prefix_c(const uint32_t* s_c, uint32_t* s_pc) {
     const int i = threadIdx.x;
-    if (i < kSegs) {
+    if (i < Geo<B>::kS) {
         uint32_t c = 0;
         for (int k = 0; k < i; ++k) c += s_c[k];
         s_pc[i] = c;
@@ -222,44 +235,32 @@ __device__ __forceinline__ u128 tile_key(const uint64_t* __restrict__ states, ui
     return (u128)r0 * (u128)mk - lp;
 }
 
-// Heavy keys and indices of one tile's range [q0, q0 + kc), read from LDS
-// (ranges that fit the tile's cache) or from memory (the others).  Two types
-// rather than one with a test per access: a select between an LDS and a
-// global pointer would make every access a flat load that waits for both.
-struct LdsKeys {
-    const u128* k;
+// Heavy indices of one tile's alias range, read from LDS (ranges that fit the
+// tile's cache) or from memory (the others).  Two types rather than one with
+// a test per access: a select between an LDS and a global pointer would make
+// every access a flat load that waits for both.
+struct LdsIdx {
     const uint32_t* i;
-    __device__ __forceinline__ u128 key(uint32_t a) const { return k[a]; }
     __device__ __forceinline__ uint32_t idx(uint32_t a) const { return i[a]; }
 };
-struct MemKeys {
-    const u128* k;
+struct MemIdx {
     const uint32_t* i;
-    __device__ __forceinline__ u128 key(uint32_t a) const { return k[a]; }
     __device__ __forceinline__ uint32_t idx(uint32_t a) const { return i[a]; }
 };
-// #{a < kc : key(a) <= x}
-template <class KV>
-__device__ __forceinline__ uint32_t count_le(const KV& kv, u128 x, uint32_t kc) {
-    uint32_t a = 0, b = kc;
-    while (a < b) {
-        const uint32_t mid = (a + b) >> 1;
-        if (le128(kv.key(mid), x)) a = mid + 1u;
-        else b = mid;
-    }
-    return a;
-}
-
 }  // namespace
 
 // ---- pass 2a: tile records and heavy slots ---------------------------------
-__global__ __launch_bounds__(kSpBlock) void k_sp_agg(const float* __restrict__ w, uint32_t n,
-                                                   const Params* __restrict__ prm,
-                                                   uint64_t* __restrict__ states,
-                                                   uint32_t* __restrict__ slot_idx,
-                                                   float* __restrict__ slot_w) {
-    __shared__ uint32_t s_c[kSpWaves * kGroups];
-    __shared__ uint64_t s_l[kSpWaves], s_hl[kSpWaves], s_hh[kSpWaves];
+// Per element only the light key sum; the heavies go through LDS to their
+// slots, and their key sum is taken from the staged slots (a heavy key is a
+// multiple of 2^30, summed as key / 2^30 < 2^57, two per thread a round).
+template <int B>
+__global__ __launch_bounds__(B) void k_sp_agg(const float* __restrict__ w, uint32_t n,
+                                                     const Params* __restrict__ prm,
+                                                     uint64_t* __restrict__ states,
+                                                     uint32_t* __restrict__ slot_idx,
+                                                     float* __restrict__ slot_w) {
+    __shared__ uint32_t s_c[Geo<B>::kS], s_pc[Geo<B>::kS];
+    __shared__ uint64_t s_l[Geo<B>::kW], s_h[Geo<B>::kW];
     __shared__ uint32_t s_si[kSlotStage];
     __shared__ float s_sw[kSlotStage];
     if (prm->error) return;
@@ -270,72 +271,58 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_agg(const float* __restrict__ w
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
-    float x[4 * kGroups];
-    nat_load(w, base, m, aligned, x);
+    float x[4 * Geo<B>::kG];
+    nat_load<B>(w, base, m, aligned, x);
     const float pivot = prm->pivot;
     const int K = prm->key_k;
-    // heavy keys are multiples of 2^30 (w >= mean): summed as key / 2^30 (< 2^57)
-    bool slow = false;
+    uint32_t hm = 0, hc[Geo<B>::kG];
+    uint64_t ls = 0;
+    bool inexact = false;
 #pragma unroll
-    for (int k = 0; k < 4 * kGroups; ++k)
-        slow |= !qkey_exact(x[k], x[k] > pivot ? K - 30 : K);
-    slow = __any(slow);
-    uint32_t hc[kGroups];
-    uint32_t hm = 0;
-    uint64_t ls = 0, hs = 0;
-#pragma unroll
-    for (int j = 0; j < kGroups; ++j) {
-        const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+    for (int j = 0; j < Geo<B>::kG; ++j) {
+        const uint32_t e0 = Geo<B>::kStride * j + 4u * (uint32_t)t;
         hc[j] = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const float v = x[4 * j + c];
-            const bool heavy = v > pivot && e0 + c < m;
-            const uint64_t q = qkey(v, heavy ? K - 30 : K, slow);
-            ls += heavy ? 0ull : q;  // weights past m are zero
-            hs += heavy ? q : 0ull;
+            const bool heavy = v > pivot && e0 + c < m;  // weights past m are zero
+            ls += heavy ? 0ull : qkey(v, K, false);
+            inexact |= !heavy && !qkey_exact(v, K);
             hm |= (heavy ? 1u : 0u) << (4 * j + c);
             hc[j] += heavy ? 1u : 0u;
         }
     }
-    uint32_t ex_c[kGroups];
+    if (__any(inexact)) {  // a weight of the rounding route in this wave: sum again
+        ls = 0;
 #pragma unroll
-    for (int j = 0; j < kGroups; ++j) {
+        for (int k = 0; k < 4 * Geo<B>::kG; ++k)
+            ls += ((hm >> k) & 1u) ? 0ull : qkey(x[k], K, true);
+    }
+    uint32_t ex_c[Geo<B>::kG];
+#pragma unroll
+    for (int j = 0; j < Geo<B>::kG; ++j) {
         const uint32_t ci = wave_incl_scan_u32(hc[j]);
-        if (lane == kWave - 1) s_c[kSpWaves * j + wid] = ci;
+        if (lane == kWave - 1) s_c[Geo<B>::kW * j + wid] = ci;
         ex_c[j] = ci - hc[j];
     }
-    const uint64_t lt = wave_incl_scan_u64(ls);  // < 64 * 16 * 2^54
-    const uint64_t hl = wave_incl_scan_u64(hs & 0xffffffffull);
-    const uint64_t hh = wave_incl_scan_u64(hs >> 32);
-    if (lane == kWave - 1) {
-        s_l[wid] = lt;
-        s_hl[wid] = hl;
-        s_hh[wid] = hh;
-    }
+    const uint64_t lt = wave_incl_scan_u64(ls);  // < 64 * 8 * 2^54
+    if (lane == kWave - 1) s_l[wid] = lt;
     __syncthreads();
-    // tile-local slot of the thread's first heavy in each group
-    uint32_t gs[kGroups];
-    uint32_t run = 0;
-#pragma unroll
-    for (int j = 0; j < kGroups; ++j) {
-#pragma unroll
-        for (int q = 0; q < kSpWaves; ++q) {
-            if (q == wid) gs[j] = run + ex_c[j];
-            run += s_c[kSpWaves * j + q];
-        }
-    }
+    nat_prefix_c<B>(s_c, s_pc);
+    __syncthreads();
+    const uint32_t ctile = s_pc[Geo<B>::kS - 1] + s_c[Geo<B>::kS - 1];
     // the heavies in element order into their slots, through LDS in rounds of
     // kSlotStage so that the stores are contiguous runs
-    for (uint32_t r0 = 0; r0 < run; r0 += kSlotStage) {  // block-uniform
+    u128 hsum = 0;  // block 0's thread 0 only keeps the total
+    for (uint32_t r0 = 0; r0 < ctile; r0 += kSlotStage) {  // block-uniform
 #pragma unroll
-        for (int j = 0; j < kGroups; ++j) {
-            uint32_t slot = gs[j];
+        for (int j = 0; j < Geo<B>::kG; ++j) {
+            uint32_t slot = s_pc[Geo<B>::kW * j + wid] + ex_c[j];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 if ((hm >> (4 * j + c)) & 1u) {
                     if (slot - r0 < kSlotStage) {
-                        s_si[slot - r0] = base + kRowStride * j + 4u * (uint32_t)t + (uint32_t)c;
+                        s_si[slot - r0] = base + Geo<B>::kStride * j + 4u * (uint32_t)t + (uint32_t)c;
                         s_sw[slot - r0] = x[4 * j + c];
                     }
                     ++slot;
@@ -343,29 +330,31 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_agg(const float* __restrict__ w
             }
         }
         __syncthreads();
-        const uint32_t nr = min(kSlotStage, run - r0);
-        for (uint32_t i = (uint32_t)t; i < nr; i += kSpBlock) {
+        const uint32_t nr = min(kSlotStage, ctile - r0);
+        uint64_t hk = 0;
+        for (uint32_t i = (uint32_t)t; i < nr; i += B) {
+            const float v = s_sw[i];
             slot_idx[base + r0 + i] = s_si[i];
-            slot_w[base + r0 + i] = s_sw[i];
+            slot_w[base + r0 + i] = v;
+            hk += qkey_round(v, K - 30);
         }
+        const uint64_t hw = wave_incl_scan_u64(hk);  // < 64 * 2 * 2^57
+        if (lane == kWave - 1) s_h[wid] = hw;
         __syncthreads();
+        if (t == 0)
+            for (int q = 0; q < Geo<B>::kW; ++q) hsum += (u128)s_h[q];
     }
     if (t == 0) {
-        u128 L = 0, Hl = 0, Hh = 0;
+        u128 L = 0;
 #pragma unroll
-        for (int q = 0; q < kSpWaves; ++q) {
-            L += (u128)s_l[q];
-            Hl += (u128)s_hl[q];
-            Hh += (u128)s_hh[q];
-        }
-        const u128 Hs = (Hl + (Hh << 32)) << 30;
-        const FX h = fx_of(Hs), l = fx_of(L);
+        for (int q = 0; q < Geo<B>::kW; ++q) L += (u128)s_l[q];
+        const FX h = fx_of(hsum << 30), l = fx_of(L);
         uint64_t* rec = states + (uint64_t)tile * kCarryWords;
         rec[0] = h.lo;
         rec[1] = h.hi;
         rec[2] = l.lo;
         rec[3] = l.hi;
-        rec[4] = run;
+        rec[4] = ctile;
     }
 }
 
@@ -462,21 +451,30 @@ __global__ __launch_bounds__(kBlock) void k_sp_qrange(uint32_t n, const Params* 
 // Each heavy q the tile owns gets its row (p, heavy q+1) in hrow[q] and the
 // tile offset of the light it crosses at in cross[q] (0xffff past the last
 // light): the table pass counts those crossings instead of comparing keys.
-template <bool F32>
-__global__ __launch_bounds__(kSpBlock) void k_sp_rows(const float* __restrict__ w, uint32_t n,
-                                                      const Params* __restrict__ prm,
-                                                      const uint64_t* __restrict__ states,
-                                                      const u128* __restrict__ hkey,
-                                                      const uint32_t* __restrict__ hidx,
-                                                      const uint32_t* __restrict__ qa,
-                                                      AliasEntry* __restrict__ hrow,
-                                                      uint16_t* __restrict__ cross) {
-    __shared__ u128 s_key[kKeyCache];
-    __shared__ float s_p[kKeyCache];
-    __shared__ uint16_t s_e[kKeyCache];
-    __shared__ uint32_t s_c[kSegs], s_pc[kSegs];
-    __shared__ uint64_t s_s[kSegs];
-    __shared__ u128 s_ps[kSegs];
+//
+// The tile's key after every element, K(e) (X after the lights up to e), is
+// non-decreasing; it is kept in LDS relative to the start of its segment of
+// 256 elements (one wave's share of a row of groups), where it is below
+// 2^62, with the segments' exact start keys beside it.  Heavy q crosses at the
+// first element e with K(e) >= Y(q+1) -- a light, since K grows only at
+// lights -- found by one thread per key with a search over the segment
+// starts and one over the segment.
+template <bool F32, int B>
+__global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint32_t n,
+                                               const Params* __restrict__ prm,
+                                               const uint64_t* __restrict__ states,
+                                               const u128* __restrict__ hkey,
+                                               const uint32_t* __restrict__ hidx,
+                                               const uint32_t* __restrict__ qa,
+                                               AliasEntry* __restrict__ hrow,
+                                               uint16_t* __restrict__ cross) {
+    using G = Geo<B>;
+    constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
+    __shared__ uint64_t s_k[kTile];
+    __shared__ u128 s_xs[G::kS + 1];
+    __shared__ uint32_t s_c[G::kS], s_pc[G::kS];
+    __shared__ uint64_t s_s[G::kS];
+    __shared__ u128 s_ps[G::kS];
     if (prm->error) return;
     const uint32_t H = prm->heavy_count;
     if (H == 0 || H == n) return;
@@ -491,31 +489,19 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_rows(const float* __restrict__ 
     const int K = prm->key_k;
     const uint64_t mk = prm->mean_k;
     const float pivot = prm->pivot;
-    const u128 x0 = tile_key(states, n, tiles, tile, mk);
     const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
-    float x[4 * kGroups];
-    nat_load(w, base, m, aligned, x);
-    // the keys that cross among the tile's lights (all of them its own)
-    const uint32_t kc = q1 - q0;
-    const bool cached = kc <= kKeyCache;
-    u128 sk[kKeyCache / kSpBlock];
-    if (cached) {
-#pragma unroll
-        for (int r = 0; r < (int)(kKeyCache / kSpBlock); ++r) {
-            const uint32_t i = (uint32_t)t + kSpBlock * r;
-            if (i < kc) sk[r] = hkey[q0 + i];
-        }
-    }
+    float x[4 * G::kG];
+    nat_load<B>(w, base, m, aligned, x);
     bool slow = false;
-    uint32_t lc[kGroups];
-    uint64_t ls[kGroups];
+    uint32_t lc[G::kG];
+    uint64_t ls[G::kG];
     uint32_t lm = 0;
 #pragma unroll
-    for (int k = 0; k < 4 * kGroups; ++k) slow |= !(x[k] > pivot) && !qkey_exact(x[k], K);
+    for (int k = 0; k < 4 * G::kG; ++k) slow |= !(x[k] > pivot) && !qkey_exact(x[k], K);
     slow = __any(slow);
 #pragma unroll
-    for (int j = 0; j < kGroups; ++j) {
-        const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+    for (int j = 0; j < G::kG; ++j) {
+        const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
         lc[j] = 0;
         ls[j] = 0;
 #pragma unroll
@@ -526,79 +512,76 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_rows(const float* __restrict__ 
             ls[j] += light ? qkey(x[4 * j + c], K, slow) : 0ull;
         }
     }
-    NatScan ns;
-    nat_scan(lc, ls, s_c, s_s, ns);
-    if (cached) {
+    NatScan<B> ns;
+    nat_scan<B>(lc, ls, s_c, s_s, ns);
+    // K(e) relative to the segment start: (lights so far) mean_k - (their keys)
 #pragma unroll
-        for (int r = 0; r < (int)(kKeyCache / kSpBlock); ++r) {
-            const uint32_t i = (uint32_t)t + kSpBlock * r;
-            if (i < kc) s_key[i] = sk[r];
+    for (int j = 0; j < G::kG; ++j) {
+        uint64_t cnt = ns.excl_c[j], sum = ns.excl_s[j];
+        uint64_t kr[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if ((lm >> (4 * j + c)) & 1u) {
+                ++cnt;
+                sum += qkey(x[4 * j + c], K, slow);
+            }
+            kr[c] = cnt * mk - sum;
         }
+        uint64_t* d = &s_k[G::kStride * j + 4u * (uint32_t)t];
+        reinterpret_cast<ulonglong2*>(d)[0] = make_ulonglong2(kr[0], kr[1]);
+        reinterpret_cast<ulonglong2*>(d)[1] = make_ulonglong2(kr[2], kr[3]);
     }
     __syncthreads();
-    nat_prefix(s_c, s_s, s_pc, s_ps);
+    nat_prefix<B>(s_c, s_s, s_pc, s_ps);
     __syncthreads();
-    // the walk over the thread's lights; emit(a, p, e) records heavy q0 + a
-    auto walk = [&](const auto& kv, const auto& emit) {
-#pragma unroll
-        for (int j = 0; j < kGroups; ++j) {
-            if (((lm >> (4 * j)) & 15u) == 0) continue;
-            const uint32_t bc = s_pc[kSpWaves * j + wid] + ns.excl_c[j];
-            const u128 bs = s_ps[kSpWaves * j + wid] + (u128)ns.excl_s[j];
-            u128 xk = x0 + (u128)bc * mk - bs;  // before the group's first light
-            uint32_t a = count_le(kv, xk, kc);  // keys at or below it: earlier groups'
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if ((lm >> (4 * j + c)) & 1u) {
-                    const u128 xn = xk + (u128)(mk - qkey(x[4 * j + c], K, slow));
-                    // heavies whose key lies in (xk, xn]: their crossing is this light
-                    while (a < kc) {
-                        const u128 y = kv.key(a);
-                        if (!le128(y, xn)) break;
-                        const double r = ldexp(u128_to_double(y + (u128)mk - xn), -K);
-                        emit(a, fminf(fmaxf(ratio<F32>(r, prm), 0.0f), 1.0f),
-                             kRowStride * j + 4u * (uint32_t)t + (uint32_t)c);
-                        ++a;
-                    }
-                    xk = xn;
-                }
+    const u128 x0 = tile_key(states, n, tiles, tile, mk);
+    if (t <= G::kS) {  // segment start keys; s_xs[kS]: the key after the tile's lights
+        const uint32_t c = t < G::kS ? s_pc[t] : s_pc[G::kS - 1] + s_c[G::kS - 1];
+        const u128 sm = t < G::kS ? s_ps[t] : s_ps[G::kS - 1] + (u128)s_s[G::kS - 1];
+        s_xs[t] = x0 + (u128)c * mk - sm;
+    }
+    __syncthreads();
+    const u128 xend = s_xs[G::kS];
+    const uint32_t kc = q1 - q0;
+    for (uint32_t a = (uint32_t)t; a < kc; a += B) {
+        const u128 y = hkey[q0 + a];
+        uint32_t e;
+        u128 xa;  // X after the crossing light (X(L) past the last light)
+        if (!le128(y, xend)) {  // past the last light (the last tile only): r(q) = L
+            e = 0xffffu;
+            xa = xend;
+        } else {
+            // last segment whose start key is below y, then the first element of
+            // it whose key reaches y
+            uint32_t lo = 0, hi = G::kS - 1u;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                if (le128(y, s_xs[mid])) hi = mid - 1u;
+                else lo = mid;
             }
-        }
-        if (tile + 1u == tiles) {
-            // heavies past the last light (Y(q+1) > X(L), q < H-1): r(q) = L
-            const u128 xl = tile_key(states, n, tiles, tiles, mk);
-            for (uint32_t a = (uint32_t)t; a < kc; a += kSpBlock) {
-                const u128 y = kv.key(a);
-                if (le128(y, xl)) continue;
-                const double r = ldexp(u128_to_double(y + (u128)mk - xl), -K);
-                emit(a, fminf(fmaxf(ratio<F32>(r, prm), 0.0f), 1.0f), 0xffffu);
+            const u128 xs = s_xs[lo];
+            const uint64_t yr = (uint64_t)(y - xs);  // (0, segment span]: below 2^62
+            uint32_t b0 = lo * kSeg, b1 = b0 + kSeg - 1u;
+            while (b0 < b1) {
+                const uint32_t mid = (b0 + b1) >> 1;
+                if (s_k[mid] >= yr) b1 = mid;
+                else b0 = mid + 1u;
             }
+            e = b0;
+            xa = xs + (u128)s_k[b0];
         }
-    };
-    if (cached) {
-        // rows staged in LDS, written as contiguous runs
-        walk(LdsKeys{s_key, nullptr}, [&](uint32_t a, float p, uint32_t e) {
-            s_p[a] = p;
-            s_e[a] = (uint16_t)e;
-        });
-        __syncthreads();
-        for (uint32_t i = (uint32_t)t; i < kc; i += kSpBlock) {
-            hrow[q0 + i] = AliasEntry{s_p[i], hidx[q0 + 1u + i]};
-            cross[q0 + i] = s_e[i];
-        }
-    } else {
-        walk(MemKeys{hkey + q0, nullptr}, [&](uint32_t a, float p, uint32_t e) {
-            hrow[q0 + a] = AliasEntry{p, hidx[q0 + 1u + a]};
-            cross[q0 + a] = (uint16_t)e;
-        });
+        const double r = ldexp(u128_to_double(y + (u128)mk - xa), -K);
+        const float p = fminf(fmaxf(ratio<F32>(r, prm), 0.0f), 1.0f);
+        hrow[q0 + a] = AliasEntry{p, hidx[q0 + 1u + a]};
+        cross[q0 + a] = (uint16_t)e;
     }
 }
 
 // ---- pass 4: the table, in element order -------------------------------------
 // A light's alias is heavy q0 + (the crossings at lights before it in the
 // tile); a heavy's row comes from hrow.  No keys are compared here.
-template <bool F32>
-__global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__ w, uint32_t n,
+template <bool F32, int B>
+__global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uint32_t n,
                                                        const Params* __restrict__ prm,
                                                        const uint64_t* __restrict__ states,
                                                        const uint32_t* __restrict__ hidx,
@@ -609,7 +592,7 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__
     __shared__ uint32_t s_cnt[kTile];           // crossings at each element of the tile
     __shared__ uint32_t s_idx[kKeyCache + 1];   // hidx[q0 .. q0 + kc]
     __shared__ AliasEntry s_row[kRowCache];     // hrow[h0 ..]
-    __shared__ uint32_t s_tc[kSegs], s_th[kSegs], s_pc[kSegs], s_ph[kSegs];
+    __shared__ uint32_t s_tc[Geo<B>::kS], s_th[Geo<B>::kS], s_pc[Geo<B>::kS], s_ph[Geo<B>::kS];
     if (prm->error) return;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
@@ -623,7 +606,7 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__
     uint2 row[4];
     // rows of group j: two 16-byte stores where the table allows, else per row
     auto store = [&](int j) {
-        const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+        const uint32_t e0 = Geo<B>::kStride * j + 4u * (uint32_t)t;
         if (tab16 && e0 + 3u < m) {
             uint4* o = reinterpret_cast<uint4*>(table + base + e0);
             o[0] = make_uint4(row[0].x, row[0].y, row[1].x, row[1].y);
@@ -637,10 +620,10 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__
     };
     if (H == 0 || H == n) {  // all weights equal the mean up to its rounding: identity
 #pragma unroll
-        for (int j = 0; j < kGroups; ++j) {
+        for (int j = 0; j < Geo<B>::kG; ++j) {
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                row[c] = make_uint2(__float_as_uint(1.0f), base + kRowStride * j + 4u * t + c);
+                row[c] = make_uint2(__float_as_uint(1.0f), base + Geo<B>::kStride * j + 4u * t + c);
             store(j);
         }
         return;
@@ -652,38 +635,47 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__
     const uint32_t h0 = (uint32_t)ex[4];
     const uint32_t ct = (uint32_t)ag[4];
     const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
-    float x[4 * kGroups];
-    nat_load(w, base, m, aligned, x);
+    float x[4 * Geo<B>::kG];
+    nat_load<B>(w, base, m, aligned, x);
     const uint32_t kc = q1 - q0;
     const bool cached = kc <= kKeyCache && ct <= kRowCache;
-    constexpr int kPer = (int)(kKeyCache / kSpBlock);
+    constexpr int kPer = (int)(kKeyCache / B);
     uint32_t cr[kPer], si[kPer];
-    AliasEntry sr = AliasEntry{0.0f, 0u};
+    constexpr int kRowPer = (int)(kRowCache / B);
+    AliasEntry sr[kRowPer];
     if (cached) {
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
-            const uint32_t i = (uint32_t)t + kSpBlock * r;
+            const uint32_t i = (uint32_t)t + B * r;
             cr[r] = i < kc ? cross[q0 + i] : 0xffffu;
             si[r] = i < kc ? hidx[q0 + i] : 0u;
         }
-        if ((uint32_t)t < ct) sr = hrow[h0 + t];
+#pragma unroll
+        for (int r = 0; r < kRowPer; ++r) {
+            const uint32_t i = (uint32_t)t + B * r;
+            if (i < ct) sr[r] = hrow[h0 + i];
+        }
     }
     const uint32_t si_last = t == 0 ? hidx[q1] : 0u;  // alias of the tile's last lights
     // crossing counts: zero, then one LDS atomic per crossing
-    for (uint32_t i = (uint32_t)t; i < (uint32_t)kTile / 4u; i += kSpBlock)
+    for (uint32_t i = (uint32_t)t; i < (uint32_t)kTile / 4u; i += B)
         reinterpret_cast<uint4*>(s_cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
     if (cached) {
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
-            const uint32_t i = (uint32_t)t + kSpBlock * r;
+            const uint32_t i = (uint32_t)t + B * r;
             if (cr[r] < (uint32_t)kTile) atomicAdd(&s_cnt[cr[r]], 1u);
             if (i < kc) s_idx[i] = si[r];
         }
-        if ((uint32_t)t < ct) s_row[t] = sr;
+#pragma unroll
+        for (int r = 0; r < kRowPer; ++r) {
+            const uint32_t i = (uint32_t)t + B * r;
+            if (i < ct) s_row[i] = sr[r];
+        }
         if (t == 0) s_idx[kc] = si_last;
     } else {
-        for (uint32_t i = (uint32_t)t; i < kc; i += kSpBlock) {
+        for (uint32_t i = (uint32_t)t; i < kc; i += B) {
             const uint32_t e = cross[q0 + i];
             if (e < (uint32_t)kTile) atomicAdd(&s_cnt[e], 1u);
         }
@@ -691,11 +683,11 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__
     __syncthreads();
     // heavy and crossing counts of the thread's groups, scanned per row
     uint32_t hm = 0, lm = 0;
-    uint32_t cnt[4 * kGroups];
-    uint32_t ex_h[kGroups], ex_c[kGroups];
+    uint32_t cnt[4 * Geo<B>::kG];
+    uint32_t ex_h[Geo<B>::kG], ex_c[Geo<B>::kG];
 #pragma unroll
-    for (int j = 0; j < kGroups; ++j) {
-        const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
+    for (int j = 0; j < Geo<B>::kG; ++j) {
+        const uint32_t e0 = Geo<B>::kStride * j + 4u * (uint32_t)t;
         const uint4 cq = *reinterpret_cast<const uint4*>(&s_cnt[e0]);
         cnt[4 * j] = cq.x;
         cnt[4 * j + 1] = cq.y;
@@ -714,22 +706,22 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__
         const uint32_t hi = wave_incl_scan_u32(hc);
         const uint32_t ci = wave_incl_scan_u32(cc);
         if (lane == kWave - 1) {
-            s_th[kSpWaves * j + wid] = hi;
-            s_tc[kSpWaves * j + wid] = ci;
+            s_th[Geo<B>::kW * j + wid] = hi;
+            s_tc[Geo<B>::kW * j + wid] = ci;
         }
         ex_h[j] = hi - hc;
         ex_c[j] = ci - cc;
     }
     __syncthreads();
-    nat_prefix_c(s_th, s_ph);
-    nat_prefix_c(s_tc, s_pc);
+    nat_prefix_c<B>(s_th, s_ph);
+    nat_prefix_c<B>(s_tc, s_pc);
     __syncthreads();
     auto rows = [&](const auto& kv, const auto* hr_src) {
 #pragma unroll
-        for (int j = 0; j < kGroups; ++j) {
-            const uint32_t e0 = kRowStride * j + 4u * (uint32_t)t;
-            uint32_t hr = s_ph[kSpWaves * j + wid] + ex_h[j];  // heavies of the tile before
-            uint32_t cx = s_pc[kSpWaves * j + wid] + ex_c[j];  // crossings before the group
+        for (int j = 0; j < Geo<B>::kG; ++j) {
+            const uint32_t e0 = Geo<B>::kStride * j + 4u * (uint32_t)t;
+            uint32_t hr = s_ph[Geo<B>::kW * j + wid] + ex_h[j];  // heavies of the tile before
+            uint32_t cx = s_pc[Geo<B>::kW * j + wid] + ex_c[j];  // crossings before the group
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const float v = x[4 * j + c];
@@ -753,16 +745,16 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_table(const float* __restrict__
         }
     };
     if (cached)
-        rows(LdsKeys{nullptr, s_idx}, s_row);
+        rows(LdsIdx{s_idx}, s_row);
     else
-        rows(MemKeys{nullptr, hidx + q0}, hrow + h0);
+        rows(MemIdx{hidx + q0}, hrow + h0);
 }
 
 // ---- launchers -----------------------------------------------------------------
 hipError_t launch_stream_agg(const float* w, uint64_t n, const Params* prm, uint64_t* states,
                              const StreamBufs& b, hipStream_t s) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
-    k_sp_agg<<<tiles, kSpBlock, 0, s>>>(w, (uint32_t)n, prm, states, b.slot_idx, b.slot_w);
+    k_sp_agg<kAggBlock><<<tiles, kAggBlock, 0, s>>>(w, (uint32_t)n, prm, states, b.slot_idx, b.slot_w);
     return hipGetLastError();
 }
 
@@ -776,16 +768,16 @@ hipError_t launch_stream_pack(const float* w, uint64_t n, int prec, const Params
     k_sp_qrange<<<(tiles + 1u + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, hkey, b.qa);
     if (ev_keys) (void)hipEventRecord(ev_keys, s);
     if (prec == kFP32) {
-        k_sp_rows<true><<<tiles, kSpBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx, b.qa, b.hrow,
+        k_sp_rows<true, kRowsBlock><<<tiles, kRowsBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx, b.qa, b.hrow,
                                                    b.cross);
         if (ev_rows) (void)hipEventRecord(ev_rows, s);
-        k_sp_table<true><<<tiles, kSpBlock, 0, s>>>(w, nn, prm, states, b.hidx, b.cross, b.qa,
+        k_sp_table<true, kTableBlock><<<tiles, kTableBlock, 0, s>>>(w, nn, prm, states, b.hidx, b.cross, b.qa,
                                                     b.hrow, table);
     } else {
-        k_sp_rows<false><<<tiles, kSpBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx, b.qa,
+        k_sp_rows<false, kRowsBlock><<<tiles, kRowsBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx, b.qa,
                                                     b.hrow, b.cross);
         if (ev_rows) (void)hipEventRecord(ev_rows, s);
-        k_sp_table<false><<<tiles, kSpBlock, 0, s>>>(w, nn, prm, states, b.hidx, b.cross, b.qa,
+        k_sp_table<false, kTableBlock><<<tiles, kTableBlock, 0, s>>>(w, nn, prm, states, b.hidx, b.cross, b.qa,
                                                      b.hrow, table);
     }
     return hipGetLastError();
