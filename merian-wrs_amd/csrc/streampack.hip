@@ -69,12 +69,10 @@ struct Geo {
     static constexpr uint32_t kStride = 4u * B;
     static constexpr int kS = kW * kG;
 };
-constexpr int kAggBlock = 512;    // threads per tile: aggregate pass
 constexpr int kRowsBlock = 512;   // rows pass
 constexpr int kTableBlock = 256;  // table pass
 constexpr uint32_t kKeyCache = 1024;         // alias indices staged in LDS per tile (table pass)
 constexpr uint32_t kRowCache = 512;          // heavy rows staged in LDS per tile (table pass)
-constexpr uint32_t kSlotStage = 1024;        // heavy slots staged in LDS per round (aggregate pass)
 
 // Tile layout of the streaming passes: 512 threads per tile, thread t holds
 // the float4 groups t + 512 j (j < 2), i.e. elements 2048 j + 4 t + c (c < 4)
@@ -249,20 +247,29 @@ struct MemIdx {
 };
 }  // namespace
 
+// The light key sum of 16 weights by the rounding route (waves holding a
+// weight that is zero, subnormal or below mean 2^-30 only).
+__device__ __attribute__((noinline)) uint64_t light_keys_round(const float* v, float pivot, int K) {
+    uint64_t ls = 0;
+    for (int k = 0; k < kItems; ++k) ls += v[k] > pivot ? 0ull : qkey_round(v[k], K);
+    return ls;
+}
+
 // ---- pass 2a: tile records and heavy slots ---------------------------------
-// Per element only the light key sum; the heavies go through LDS to their
-// slots, and their key sum is taken from the staged slots (a heavy key is a
-// multiple of 2^30, summed as key / 2^30 < 2^57, two per thread a round).
-template <int B>
-__global__ __launch_bounds__(B) void k_sp_agg(const float* __restrict__ w, uint32_t n,
-                                                     const Params* __restrict__ prm,
-                                                     uint64_t* __restrict__ states,
-                                                     uint32_t* __restrict__ slot_idx,
-                                                     float* __restrict__ slot_w) {
-    __shared__ uint32_t s_c[Geo<B>::kS], s_pc[Geo<B>::kS];
-    __shared__ uint64_t s_l[Geo<B>::kW], s_h[Geo<B>::kW];
-    __shared__ uint32_t s_si[kSlotStage];
-    __shared__ float s_sw[kSlotStage];
+// 256 threads, each owning 16 consecutive elements of the tile (transposed
+// through LDS, tileload.hpp): per element only the light key is summed; the
+// thread's heavies -- consecutive slots, few per thread -- are taken one set
+// bit at a time, stored to their slots and their keys summed as key / 2^30
+// (a heavy key is a multiple of 2^30, below 2^57).
+constexpr int kPadAgg = kTile + kTile / 16;
+__global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, uint32_t n,
+                                                   const Params* __restrict__ prm,
+                                                   uint64_t* __restrict__ states,
+                                                   uint32_t* __restrict__ slot_idx,
+                                                   float* __restrict__ slot_w) {
+    __shared__ float s_buf[kPadAgg];
+    __shared__ uint32_t s_c[kWaves];
+    __shared__ uint64_t s_l[kWaves], s_hl[kWaves], s_hh[kWaves];
     if (prm->error) return;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
@@ -270,85 +277,63 @@ __global__ __launch_bounds__(B) void k_sp_agg(const float* __restrict__ w, uint3
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
-    const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
-    float x[4 * Geo<B>::kG];
-    nat_load<B>(w, base, m, aligned, x);
+    load_tile_lds<true>(w + base, m, s_buf);
+    __syncthreads();
     const float pivot = prm->pivot;
     const int K = prm->key_k;
-    uint32_t hm = 0, hc[Geo<B>::kG];
+    const uint32_t e0 = (uint32_t)(kItems * t);
+    const uint32_t nvalid = e0 >= m ? 0u : min((uint32_t)kItems, m - e0);
+    uint32_t hm = 0;
     uint64_t ls = 0;
     bool inexact = false;
 #pragma unroll
-    for (int j = 0; j < Geo<B>::kG; ++j) {
-        const uint32_t e0 = Geo<B>::kStride * j + 4u * (uint32_t)t;
-        hc[j] = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const float v = x[4 * j + c];
-            const bool heavy = v > pivot && e0 + c < m;  // weights past m are zero
-            ls += heavy ? 0ull : qkey(v, K, false);
-            inexact |= !heavy && !qkey_exact(v, K);
-            hm |= (heavy ? 1u : 0u) << (4 * j + c);
-            hc[j] += heavy ? 1u : 0u;
-        }
+    for (int k = 0; k < kItems; ++k) {
+        const float v = s_buf[17 * t + k];
+        const bool heavy = v > pivot;  // weights past m are zero: light, key 0
+        ls += heavy ? 0ull : qkey(v, K, false);
+        inexact |= !heavy && !qkey_exact(v, K);
+        hm |= (heavy ? 1u : 0u) << k;
+        asm volatile("" : "+v"(ls));  // one element at a time: few registers
     }
-    if (__any(inexact)) {  // a weight of the rounding route in this wave: sum again
-        ls = 0;
-#pragma unroll
-        for (int k = 0; k < 4 * Geo<B>::kG; ++k)
-            ls += ((hm >> k) & 1u) ? 0ull : qkey(x[k], K, true);
-    }
-    uint32_t ex_c[Geo<B>::kG];
-#pragma unroll
-    for (int j = 0; j < Geo<B>::kG; ++j) {
-        const uint32_t ci = wave_incl_scan_u32(hc[j]);
-        if (lane == kWave - 1) s_c[Geo<B>::kW * j + wid] = ci;
-        ex_c[j] = ci - hc[j];
-    }
-    const uint64_t lt = wave_incl_scan_u64(ls);  // < 64 * 8 * 2^54
-    if (lane == kWave - 1) s_l[wid] = lt;
+    hm &= (1u << nvalid) - 1u;
+    if (__any(inexact)) ls = light_keys_round(s_buf + 17 * t, pivot, K);
+    const uint32_t c = (uint32_t)__popc(hm);
+    const uint32_t ci = wave_incl_scan_u32(c);
+    if (lane == kWave - 1) s_c[wid] = ci;
     __syncthreads();
-    nat_prefix_c<B>(s_c, s_pc);
-    __syncthreads();
-    const uint32_t ctile = s_pc[Geo<B>::kS - 1] + s_c[Geo<B>::kS - 1];
-    // the heavies in element order into their slots, through LDS in rounds of
-    // kSlotStage so that the stores are contiguous runs
-    u128 hsum = 0;  // block 0's thread 0 only keeps the total
-    for (uint32_t r0 = 0; r0 < ctile; r0 += kSlotStage) {  // block-uniform
+    uint32_t slot = ci - c, ctile = 0;
 #pragma unroll
-        for (int j = 0; j < Geo<B>::kG; ++j) {
-            uint32_t slot = s_pc[Geo<B>::kW * j + wid] + ex_c[j];
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if ((hm >> (4 * j + c)) & 1u) {
-                    if (slot - r0 < kSlotStage) {
-                        s_si[slot - r0] = base + Geo<B>::kStride * j + 4u * (uint32_t)t + (uint32_t)c;
-                        s_sw[slot - r0] = x[4 * j + c];
-                    }
-                    ++slot;
-                }
-            }
-        }
-        __syncthreads();
-        const uint32_t nr = min(kSlotStage, ctile - r0);
-        uint64_t hk = 0;
-        for (uint32_t i = (uint32_t)t; i < nr; i += B) {
-            const float v = s_sw[i];
-            slot_idx[base + r0 + i] = s_si[i];
-            slot_w[base + r0 + i] = v;
-            hk += qkey_round(v, K - 30);
-        }
-        const uint64_t hw = wave_incl_scan_u64(hk);  // < 64 * 2 * 2^57
-        if (lane == kWave - 1) s_h[wid] = hw;
-        __syncthreads();
-        if (t == 0)
-            for (int q = 0; q < Geo<B>::kW; ++q) hsum += (u128)s_h[q];
+    for (int q = 0; q < kWaves; ++q) {
+        slot += q < wid ? s_c[q] : 0u;
+        ctile += s_c[q];
     }
+    uint64_t hs = 0;
+    for (uint32_t mk = hm; mk; mk &= mk - 1u) {
+        const int k = __builtin_ctz(mk);
+        const float x = s_buf[17 * t + k];
+        hs += qkey_round(x, K - 30);
+        slot_idx[base + slot] = base + e0 + (uint32_t)k;
+        slot_w[base + slot] = x;
+        ++slot;
+    }
+    const uint64_t lt = wave_incl_scan_u64(ls);  // < 64 * 16 * 2^54
+    const uint64_t hl = wave_incl_scan_u64(hs & 0xffffffffull);
+    const uint64_t hh = wave_incl_scan_u64(hs >> 32);
+    if (lane == kWave - 1) {
+        s_l[wid] = lt;
+        s_hl[wid] = hl;
+        s_hh[wid] = hh;
+    }
+    __syncthreads();
     if (t == 0) {
-        u128 L = 0;
+        u128 L = 0, Hl = 0, Hh = 0;
 #pragma unroll
-        for (int q = 0; q < Geo<B>::kW; ++q) L += (u128)s_l[q];
-        const FX h = fx_of(hsum << 30), l = fx_of(L);
+        for (int q = 0; q < kWaves; ++q) {
+            L += (u128)s_l[q];
+            Hl += (u128)s_hl[q];
+            Hh += (u128)s_hh[q];
+        }
+        const FX h = fx_of((Hl + (Hh << 32)) << 30), l = fx_of(L);
         uint64_t* rec = states + (uint64_t)tile * kCarryWords;
         rec[0] = h.lo;
         rec[1] = h.hi;
@@ -754,7 +739,7 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
 hipError_t launch_stream_agg(const float* w, uint64_t n, const Params* prm, uint64_t* states,
                              const StreamBufs& b, hipStream_t s) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
-    k_sp_agg<kAggBlock><<<tiles, kAggBlock, 0, s>>>(w, (uint32_t)n, prm, states, b.slot_idx, b.slot_w);
+    k_sp_agg<<<tiles, kBlock, 0, s>>>(w, (uint32_t)n, prm, states, b.slot_idx, b.slot_w);
     return hipGetLastError();
 }
 
