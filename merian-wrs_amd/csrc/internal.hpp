@@ -193,7 +193,10 @@ struct StreamBufs {
     AliasEntry* hrow = nullptr;
     uint16_t* cross = nullptr; // tile offset of the light a heavy's key crosses at
     uint32_t* qa = nullptr;
+    uint16_t* seg = nullptr;   // kSegWords per tile: heavies before each 256-element
+                               // segment, crossings before each segment boundary
 };
+constexpr uint32_t kSegWords = 64;  // [0, 16) heavies, [32, 49) crossings
 // the aggregate pass (after the reduce), then -- after launch_key_tilescan --
 // the heavy keys, the tiles' alias ranges, the heavies' rows and the table;
 // ev_keys / ev_rows (may be null) recorded after the heavy keys and ranges

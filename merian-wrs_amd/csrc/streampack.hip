@@ -266,7 +266,8 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
                                                    const Params* __restrict__ prm,
                                                    uint64_t* __restrict__ states,
                                                    uint32_t* __restrict__ slot_idx,
-                                                   float* __restrict__ slot_w) {
+                                                   float* __restrict__ slot_w,
+                                                   uint16_t* __restrict__ seg) {
     __shared__ float s_buf[kPadAgg];
     __shared__ uint32_t s_c[kWaves];
     __shared__ uint64_t s_l[kWaves], s_hl[kWaves], s_hh[kWaves];
@@ -307,6 +308,8 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
         slot += q < wid ? s_c[q] : 0u;
         ctile += s_c[q];
     }
+    // heavies before each segment of 256 elements (16 threads)
+    if ((t & 15) == 0) seg[(uint64_t)tile * kSegWords + (uint32_t)(t >> 4)] = (uint16_t)slot;
     uint64_t hs = 0;
     for (uint32_t mk = hm; mk; mk &= mk - 1u) {
         const int k = __builtin_ctz(mk);
@@ -452,11 +455,13 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
                                                const uint32_t* __restrict__ hidx,
                                                const uint32_t* __restrict__ qa,
                                                AliasEntry* __restrict__ hrow,
-                                               uint16_t* __restrict__ cross) {
+                                               uint16_t* __restrict__ cross,
+                                               uint16_t* __restrict__ seg) {
     using G = Geo<B>;
     constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
     __shared__ uint64_t s_k[kTile];
     __shared__ u128 s_xs[G::kS + 1];
+    __shared__ uint16_t s_e[B];
     __shared__ uint32_t s_c[G::kS], s_pc[G::kS];
     __shared__ uint64_t s_s[G::kS];
     __shared__ u128 s_ps[G::kS];
@@ -477,6 +482,14 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
     const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     float x[4 * G::kG];
     nat_load<B>(w, base, m, aligned, x);
+    const uint32_t kc = q1 - q0;
+    // the first chunk's keys and alias indices, loaded with the tile
+    u128 y0 = 0;
+    uint32_t i0 = 0;
+    if ((uint32_t)t < kc) {
+        y0 = hkey[q0 + t];
+        i0 = hidx[q0 + 1u + t];
+    }
     bool slow = false;
     uint32_t lc[G::kG];
     uint64_t ls[G::kG];
@@ -527,61 +540,86 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
     }
     __syncthreads();
     const u128 xend = s_xs[G::kS];
-    const uint32_t kc = q1 - q0;
-    for (uint32_t a = (uint32_t)t; a < kc; a += B) {
-        const u128 y = hkey[q0 + a];
-        uint32_t e;
-        u128 xa;  // X after the crossing light (X(L) past the last light)
-        if (!le128(y, xend)) {  // past the last light (the last tile only): r(q) = L
-            e = 0xffffu;
-            xa = xend;
-        } else {
-            // last segment whose start key is below y, then the first element of
-            // it whose key reaches y
-            uint32_t lo = 0, hi = G::kS - 1u;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1u) >> 1;
-                if (le128(y, s_xs[mid])) hi = mid - 1u;
-                else lo = mid;
+    uint16_t* cseg = seg + (uint64_t)tile * kSegWords + 32u;  // crossings before 256 s
+    if (t == 0) cseg[0] = 0;
+    uint32_t elast = 0;  // the last key's crossing in the previous chunk
+    for (uint32_t a0 = 0; a0 < kc; a0 += B) {  // block-uniform
+        const uint32_t a = a0 + (uint32_t)t;
+        uint32_t e = 0xffffu;
+        if (a < kc) {
+            const u128 y = a0 == 0 ? y0 : hkey[q0 + a];
+            u128 xa;  // X after the crossing light (X(L) past the last light)
+            if (!le128(y, xend)) {  // past the last light (the last tile only): r(q) = L
+                xa = xend;
+            } else {
+                // last segment whose start key is below y, then the first element
+                // of it whose key reaches y
+                uint32_t lo = 0, hi = G::kS - 1u;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1u) >> 1;
+                    if (le128(y, s_xs[mid])) hi = mid - 1u;
+                    else lo = mid;
+                }
+                const u128 xs = s_xs[lo];
+                const uint64_t yr = (uint64_t)(y - xs);  // (0, segment span]: below 2^62
+                uint32_t b0 = lo * kSeg, b1 = b0 + kSeg - 1u;
+                while (b0 < b1) {
+                    const uint32_t mid = (b0 + b1) >> 1;
+                    if (s_k[mid] >= yr) b1 = mid;
+                    else b0 = mid + 1u;
+                }
+                e = b0;
+                xa = xs + (u128)s_k[b0];
             }
-            const u128 xs = s_xs[lo];
-            const uint64_t yr = (uint64_t)(y - xs);  // (0, segment span]: below 2^62
-            uint32_t b0 = lo * kSeg, b1 = b0 + kSeg - 1u;
-            while (b0 < b1) {
-                const uint32_t mid = (b0 + b1) >> 1;
-                if (s_k[mid] >= yr) b1 = mid;
-                else b0 = mid + 1u;
-            }
-            e = b0;
-            xa = xs + (u128)s_k[b0];
+            const double r = ldexp(u128_to_double(y + (u128)mk - xa), -K);
+            const float p = fminf(fmaxf(ratio<F32>(r, prm), 0.0f), 1.0f);
+            hrow[q0 + a] = AliasEntry{p, a0 == 0 ? i0 : hidx[q0 + 1u + a]};
+            cross[q0 + a] = (uint16_t)e;
         }
-        const double r = ldexp(u128_to_double(y + (u128)mk - xa), -K);
-        const float p = fminf(fmaxf(ratio<F32>(r, prm), 0.0f), 1.0f);
-        hrow[q0 + a] = AliasEntry{p, hidx[q0 + 1u + a]};
-        cross[q0 + a] = (uint16_t)e;
+        // crossings before each segment boundary 256 s: key a is the first at or
+        // past the boundaries in (e(a-1), e(a)]; past the last key: kc
+        s_e[t] = (uint16_t)e;
+        __syncthreads();
+        const uint32_t chunk_last = s_e[B - 1];
+        if (a < kc) {
+            const int prev = t > 0 ? (int)s_e[t - 1] : (a0 > 0 ? (int)elast : -1);
+            const int slo = prev < 0 ? 1 : prev / (int)kSeg + 1;
+            const int shi = min((int)G::kS, (int)e / (int)kSeg);
+            for (int sg = slo; sg <= shi; ++sg) cseg[sg] = (uint16_t)a;
+            if (a + 1u == kc)
+                for (int sg = max(slo, (int)e / (int)kSeg + 1); sg <= (int)G::kS; ++sg)
+                    cseg[sg] = (uint16_t)kc;
+        }
+        elast = chunk_last;
+        __syncthreads();
     }
 }
 
 // ---- pass 4: the table, in element order -------------------------------------
 // A light's alias is heavy q0 + (the crossings at lights before it in the
-// tile); a heavy's row comes from hrow.  No keys are compared here.
+// tile); a heavy's row comes from hrow.  No keys are compared here: the
+// segment prefixes of the aggregate pass (heavies) and of the rows pass
+// (crossings, when the tile has any) and one wave scan give every element its
+// counts, with a single barrier for the staged indices and rows.
 template <bool F32, int B>
 __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uint32_t n,
-                                                       const Params* __restrict__ prm,
-                                                       const uint64_t* __restrict__ states,
-                                                       const uint32_t* __restrict__ hidx,
-                                                       const uint16_t* __restrict__ cross,
-                                                       const uint32_t* __restrict__ qa,
-                                                       const AliasEntry* __restrict__ hrow,
-                                                       AliasEntry* __restrict__ table) {
-    __shared__ uint32_t s_cnt[kTile];           // crossings at each element of the tile
+                                                const Params* __restrict__ prm,
+                                                const uint64_t* __restrict__ states,
+                                                const uint32_t* __restrict__ hidx,
+                                                const uint16_t* __restrict__ cross,
+                                                const uint32_t* __restrict__ qa,
+                                                const AliasEntry* __restrict__ hrow,
+                                                const uint16_t* __restrict__ seg,
+                                                AliasEntry* __restrict__ table) {
+    using G = Geo<B>;
+    constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
+    static_assert(kSeg == 4u * kWave, "a segment is one wave's share of a row");
+    __shared__ uint16_t s_cr[kKeyCache];        // cross[q0 ..]
     __shared__ uint32_t s_idx[kKeyCache + 1];   // hidx[q0 .. q0 + kc]
     __shared__ AliasEntry s_row[kRowCache];     // hrow[h0 ..]
-    __shared__ uint32_t s_tc[Geo<B>::kS], s_th[Geo<B>::kS], s_pc[Geo<B>::kS], s_ph[Geo<B>::kS];
     if (prm->error) return;
     const int t = threadIdx.x;
-    const int lane = t & (kWave - 1);
-    const int wid = t / kWave;
+    const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
@@ -591,7 +629,7 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     uint2 row[4];
     // rows of group j: two 16-byte stores where the table allows, else per row
     auto store = [&](int j) {
-        const uint32_t e0 = Geo<B>::kStride * j + 4u * (uint32_t)t;
+        const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
         if (tab16 && e0 + 3u < m) {
             uint4* o = reinterpret_cast<uint4*>(table + base + e0);
             o[0] = make_uint4(row[0].x, row[0].y, row[1].x, row[1].y);
@@ -605,10 +643,10 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     };
     if (H == 0 || H == n) {  // all weights equal the mean up to its rounding: identity
 #pragma unroll
-        for (int j = 0; j < Geo<B>::kG; ++j) {
+        for (int j = 0; j < G::kG; ++j) {
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                row[c] = make_uint2(__float_as_uint(1.0f), base + Geo<B>::kStride * j + 4u * t + c);
+                row[c] = make_uint2(__float_as_uint(1.0f), base + G::kStride * j + 4u * t + c);
             store(j);
         }
         return;
@@ -619,98 +657,81 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
     const uint32_t h0 = (uint32_t)ex[4];
     const uint32_t ct = (uint32_t)ag[4];
+    const uint16_t* hseg = seg + (uint64_t)tile * kSegWords;
+    const uint16_t* cseg = hseg + 32u;
     const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
-    float x[4 * Geo<B>::kG];
+    float x[4 * G::kG];
     nat_load<B>(w, base, m, aligned, x);
+    // the segment prefixes of this wave's groups (wave-uniform), loaded up front
+    uint32_t hpre[G::kG], clo[G::kG], chi[G::kG];
+#pragma unroll
+    for (int j = 0; j < G::kG; ++j) {
+        hpre[j] = hseg[G::kW * j + wid];
+        clo[j] = cseg[G::kW * j + wid];
+        chi[j] = cseg[G::kW * j + wid + 1];
+    }
     const uint32_t kc = q1 - q0;
     const bool cached = kc <= kKeyCache && ct <= kRowCache;
+    const bool has_seg = kc <= 0xffffu;  // the crossing prefixes fit 16 bits
     constexpr int kPer = (int)(kKeyCache / B);
-    uint32_t cr[kPer], si[kPer];
     constexpr int kRowPer = (int)(kRowCache / B);
-    AliasEntry sr[kRowPer];
     if (cached) {
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
             const uint32_t i = (uint32_t)t + B * r;
-            cr[r] = i < kc ? cross[q0 + i] : 0xffffu;
-            si[r] = i < kc ? hidx[q0 + i] : 0u;
+            if (i < kc) {
+                s_cr[i] = cross[q0 + i];
+                s_idx[i] = hidx[q0 + i];
+            }
         }
+        if (t == 0) s_idx[kc] = hidx[q1];  // alias of the tile's last lights
 #pragma unroll
         for (int r = 0; r < kRowPer; ++r) {
             const uint32_t i = (uint32_t)t + B * r;
-            if (i < ct) sr[r] = hrow[h0 + i];
-        }
-    }
-    const uint32_t si_last = t == 0 ? hidx[q1] : 0u;  // alias of the tile's last lights
-    // crossing counts: zero, then one LDS atomic per crossing
-    for (uint32_t i = (uint32_t)t; i < (uint32_t)kTile / 4u; i += B)
-        reinterpret_cast<uint4*>(s_cnt)[i] = make_uint4(0u, 0u, 0u, 0u);
-    __syncthreads();
-    if (cached) {
-#pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const uint32_t i = (uint32_t)t + B * r;
-            if (cr[r] < (uint32_t)kTile) atomicAdd(&s_cnt[cr[r]], 1u);
-            if (i < kc) s_idx[i] = si[r];
-        }
-#pragma unroll
-        for (int r = 0; r < kRowPer; ++r) {
-            const uint32_t i = (uint32_t)t + B * r;
-            if (i < ct) s_row[i] = sr[r];
-        }
-        if (t == 0) s_idx[kc] = si_last;
-    } else {
-        for (uint32_t i = (uint32_t)t; i < kc; i += B) {
-            const uint32_t e = cross[q0 + i];
-            if (e < (uint32_t)kTile) atomicAdd(&s_cnt[e], 1u);
+            if (i < ct) s_row[i] = hrow[h0 + i];
         }
     }
     __syncthreads();
-    // heavy and crossing counts of the thread's groups, scanned per row
-    uint32_t hm = 0, lm = 0;
-    uint32_t cnt[4 * Geo<B>::kG];
-    uint32_t ex_h[Geo<B>::kG], ex_c[Geo<B>::kG];
-#pragma unroll
-    for (int j = 0; j < Geo<B>::kG; ++j) {
-        const uint32_t e0 = Geo<B>::kStride * j + 4u * (uint32_t)t;
-        const uint4 cq = *reinterpret_cast<const uint4*>(&s_cnt[e0]);
-        cnt[4 * j] = cq.x;
-        cnt[4 * j + 1] = cq.y;
-        cnt[4 * j + 2] = cq.z;
-        cnt[4 * j + 3] = cq.w;
-        uint32_t hc = 0;
+    // crossings before element e: from the segment's slice of the sorted list
+    auto crossings = [&](const auto* cr, int j, uint32_t (&cx)[4]) {
+        const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
+        uint32_t lo, hi;
+        if (has_seg) {
+            lo = clo[j];
+            hi = chi[j];
+        } else {
+            lo = 0;
+            hi = kc;
+        }
+        uint32_t a = lo, b = hi;  // first crossing at or past e0
+        while (a < b) {
+            const uint32_t mid = (a + b) >> 1;
+            if (cr[mid] < e0) a = mid + 1u;
+            else b = mid;
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const bool valid = e0 + c < m;
-            const bool heavy = x[4 * j + c] > pivot;
-            hm |= (heavy && valid ? 1u : 0u) << (4 * j + c);
-            lm |= (!heavy && valid ? 1u : 0u) << (4 * j + c);
-            hc += heavy && valid ? 1u : 0u;
+            while (a < hi && cr[a] < e0 + (uint32_t)c) ++a;
+            cx[c] = a;
         }
-        const uint32_t cc = cq.x + cq.y + cq.z + cq.w;
-        const uint32_t hi = wave_incl_scan_u32(hc);
-        const uint32_t ci = wave_incl_scan_u32(cc);
-        if (lane == kWave - 1) {
-            s_th[Geo<B>::kW * j + wid] = hi;
-            s_tc[Geo<B>::kW * j + wid] = ci;
-        }
-        ex_h[j] = hi - hc;
-        ex_c[j] = ci - cc;
-    }
-    __syncthreads();
-    nat_prefix_c<B>(s_th, s_ph);
-    nat_prefix_c<B>(s_tc, s_pc);
-    __syncthreads();
-    auto rows = [&](const auto& kv, const auto* hr_src) {
+    };
+    auto rows = [&](const auto* cr, const uint32_t* ix, const AliasEntry* hr_src) {
 #pragma unroll
-        for (int j = 0; j < Geo<B>::kG; ++j) {
-            const uint32_t e0 = Geo<B>::kStride * j + 4u * (uint32_t)t;
-            uint32_t hr = s_ph[Geo<B>::kW * j + wid] + ex_h[j];  // heavies of the tile before
-            uint32_t cx = s_pc[Geo<B>::kW * j + wid] + ex_c[j];  // crossings before the group
+        for (int j = 0; j < G::kG; ++j) {
+            const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
+            uint32_t hm = 0, hc = 0;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                const float v = x[4 * j + c];
-                if ((hm >> (4 * j + c)) & 1u) {
+                const bool heavy = x[4 * j + c] > pivot && e0 + c < m;
+                hm |= (heavy ? 1u : 0u) << c;
+                hc += heavy ? 1u : 0u;
+            }
+            uint32_t hr = hpre[j] + wave_incl_scan_u32(hc) - hc;
+            uint32_t cx[4] = {0u, 0u, 0u, 0u};
+            if (kc) crossings(cr, j, cx);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((hm >> c) & 1u) {
                     if (h0 + hr + 1u == H) {
                         row[c] = make_uint2(__float_as_uint(1.0f), base + e0 + c);
                     } else {
@@ -718,28 +739,25 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
                         row[c] = make_uint2(__float_as_uint(e.p), e.a);
                     }
                     ++hr;
-                } else if ((lm >> (4 * j + c)) & 1u) {
-                    const float p = fminf(ratio<F32>((double)v, prm), 1.0f);
-                    row[c] = make_uint2(__float_as_uint(p), kv.idx(cx));
-                } else {
-                    row[c] = make_uint2(0u, 0u);  // past m: not stored
+                } else {  // a light (or past m: not stored)
+                    const float p = fminf(ratio<F32>((double)x[4 * j + c], prm), 1.0f);
+                    row[c] = make_uint2(__float_as_uint(p), ix[cx[c]]);
                 }
-                cx += cnt[4 * j + c];  // heavies packed after this element's light
             }
             store(j);
         }
     };
     if (cached)
-        rows(LdsIdx{s_idx}, s_row);
+        rows(s_cr, s_idx, s_row);
     else
-        rows(MemIdx{hidx + q0}, hrow + h0);
+        rows(cross + q0, hidx + q0, hrow + h0);
 }
 
 // ---- launchers -----------------------------------------------------------------
 hipError_t launch_stream_agg(const float* w, uint64_t n, const Params* prm, uint64_t* states,
                              const StreamBufs& b, hipStream_t s) {
     const uint32_t tiles = (uint32_t)tiles_for(n);
-    k_sp_agg<<<tiles, kBlock, 0, s>>>(w, (uint32_t)n, prm, states, b.slot_idx, b.slot_w);
+    k_sp_agg<<<tiles, kBlock, 0, s>>>(w, (uint32_t)n, prm, states, b.slot_idx, b.slot_w, b.seg);
     return hipGetLastError();
 }
 
@@ -753,17 +771,19 @@ hipError_t launch_stream_pack(const float* w, uint64_t n, int prec, const Params
     k_sp_qrange<<<(tiles + 1u + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, hkey, b.qa);
     if (ev_keys) (void)hipEventRecord(ev_keys, s);
     if (prec == kFP32) {
-        k_sp_rows<true, kRowsBlock><<<tiles, kRowsBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx, b.qa, b.hrow,
-                                                   b.cross);
+        k_sp_rows<true, kRowsBlock><<<tiles, kRowsBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx,
+                                                                 b.qa, b.hrow, b.cross, b.seg);
         if (ev_rows) (void)hipEventRecord(ev_rows, s);
-        k_sp_table<true, kTableBlock><<<tiles, kTableBlock, 0, s>>>(w, nn, prm, states, b.hidx, b.cross, b.qa,
-                                                    b.hrow, table);
+        k_sp_table<true, kTableBlock><<<tiles, kTableBlock, 0, s>>>(w, nn, prm, states, b.hidx,
+                                                                    b.cross, b.qa, b.hrow, b.seg,
+                                                                    table);
     } else {
-        k_sp_rows<false, kRowsBlock><<<tiles, kRowsBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx, b.qa,
-                                                    b.hrow, b.cross);
+        k_sp_rows<false, kRowsBlock><<<tiles, kRowsBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx,
+                                                                  b.qa, b.hrow, b.cross, b.seg);
         if (ev_rows) (void)hipEventRecord(ev_rows, s);
-        k_sp_table<false, kTableBlock><<<tiles, kTableBlock, 0, s>>>(w, nn, prm, states, b.hidx, b.cross, b.qa,
-                                                     b.hrow, table);
+        k_sp_table<false, kTableBlock><<<tiles, kTableBlock, 0, s>>>(w, nn, prm, states, b.hidx,
+                                                                     b.cross, b.qa, b.hrow, b.seg,
+                                                                     table);
     }
     return hipGetLastError();
 }

@@ -91,8 +91,9 @@ void Workspace::ensure_stream() {
         shrow = dev_alloc<wrsk::AliasEntry>(max_n);
         scross = dev_alloc<uint16_t>(max_n);
         sqa = dev_alloc<uint32_t>(wrsk::tiles_for(max_n) + 2);
+        sseg = dev_alloc<uint16_t>(wrsk::kSegWords * wrsk::tiles_for(max_n));
     } catch (...) {
-        void* ptrs[] = {skey, shidx, shrow, scross, sqa};
+        void* ptrs[] = {skey, shidx, shrow, scross, sqa, sseg};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         skey = nullptr;
@@ -100,6 +101,7 @@ void Workspace::ensure_stream() {
         shrow = nullptr;
         scross = nullptr;
         sqa = nullptr;
+        sseg = nullptr;
         throw;
     }
 }
@@ -107,7 +109,7 @@ void Workspace::ensure_stream() {
 void Workspace::release() {
     void* ptrs[] = {params,    red,    states,   indices, pweights, prefix_hi, prefix_lo,
                     samp_h,    samp_l, bounds,   lightpos, splits,  skey,      shidx,
-                    shrow,     scross, sqa};
+                    shrow,     scross, sqa,      sseg};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     params = nullptr;
@@ -128,6 +130,7 @@ void Workspace::release() {
     shrow = nullptr;
     scross = nullptr;
     sqa = nullptr;
+    sseg = nullptr;
 }
 
 static void require(bool cond, const char* msg) {
@@ -224,7 +227,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     if (form == WRS_PACK_STREAM && !dbg) {
         // phases: tile aggregate, tile scan, heavy keys and ranges, heavy rows, table
         const wrsk::StreamBufs sb{ws.indices, ws.pweights, ws.skey, ws.shidx,
-                                  ws.shrow,    ws.scross,   ws.sqa};
+                                  ws.shrow,    ws.scross,   ws.sqa, ws.sseg};
         check_hip(wrsk::launch_stream_agg(b.weights, N, ws.params, ws.states, sb, s),
                   "stream aggregate launch");
         if (ev) check_hip(hipEventRecord(ev[3], s), "event");
