@@ -270,8 +270,9 @@ def main() -> int:
     st.synchronize()
     H = int(hbuf.numpy()[0])
     p, t_ = args.precision, (tune or {})
+    stream = t_.get("direct", 0) == wrs.PACK_STREAM
     direct = t_.get("direct", 0) != 2  # WRS_PACK_CLASSIC: 2; the default is direct
-    lookback = bool(t_.get("partition"))
+    lookback = bool(t_.get("partition")) and not stream
     # bytes each kernel must move.  Classic form: the emit reads w and writes
     # index + weight of every rank, the merge pack reads them back and writes
     # the 8-byte table.  Direct form: the emit reads w and writes the heavies'
@@ -301,6 +302,23 @@ def main() -> int:
         build_kernels += [f"k_part_agg<{p}>", f"k_part_tilescan<{p}>"]
     build_kernels += [kname["partition_emit"], f"k_coarse<{0 if p == 0 else 1}, {d_}>",
                       f"k_repair_bounds_lb<{d_}>", kname["merge_pack"]]
+    if stream:
+        # the streamed form: the aggregate reads w and writes the heavies'
+        # slots (index + weight); the heavy keys read them and write a 16-byte
+        # key and an index per heavy; the rows pass reads w and writes the
+        # heavies' rows and crossing offsets; the table pass reads w, the rows,
+        # indices and crossings of its tile and writes the table
+        phase = dict(zip(("reduce", "finalize", "tile_aggregate", "tile_scan",
+                          "heavy_keys_and_ranges", "heavy_rows", "table"), phase.values()))
+        alg = {"reduce": 4 * n, "finalize": 0, "tile_aggregate": 4 * n + 8 * H, "tile_scan": 0,
+               "heavy_keys_and_ranges": 8 * H + 20 * H, "heavy_rows": 4 * n + 10 * H,
+               "table": 12 * n + 14 * H}
+        f32 = "true" if p == 0 else "false"
+        kname = {"reduce": f"k_reduce_final<{fp}>", "tile_aggregate": "k_sp_agg",
+                 "heavy_keys_and_ranges": "k_sp_heavy", "heavy_rows": f"k_sp_rows<{f32}, 512>",
+                 "table": f"k_sp_table<{f32}, 256>"}
+        build_kernels = [f"k_reduce_final<{fp}>", "k_sp_agg", "k_part_tilescan<1, true>",
+                         "k_sp_heavy", "k_sp_qrange", kname["heavy_rows"], kname["table"]]
     log("phases (ms): " + ", ".join(f"{k} {v:.4f}" for k, v in phase.items()))
     dom = max(kname, key=lambda k: phase[k])
     dom_ms = phase[dom]

@@ -112,9 +112,9 @@ std::string device_key() {
     hipDeviceProp_t prop;
     wrs::check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     std::ostringstream k;
-    // "t3": the tuning encoding (WRS_PACK_* pack forms, sweep forms included);
-    // lines of other encodings in a cache file do not match
-    k << "t4/" << prop.gcnArchName << "/cu" << prop.multiProcessorCount;
+    // "t5": the tuning encoding (WRS_PACK_* pack forms, sweep and streamed
+    // forms included); lines of other encodings in a cache file do not match
+    k << "t5/" << prop.gcnArchName << "/cu" << prop.multiProcessorCount;
     return k.str();
 }
 
@@ -199,8 +199,9 @@ wrs_tuning descend(M&& measure_raw, bool direct_rts_only) {
         }
     }
     bool sweep = false;
-    if (!direct_rts_only) {  // the reference's sweep forms (single builds only)
-        for (const uint32_t f : {WRS_PACK_SERIAL, WRS_PACK_INLINE, WRS_PACK_SUBGROUP}) {
+    if (!direct_rts_only) {  // the reference's sweep forms and the streamed form (single builds)
+        for (const uint32_t f :
+             {WRS_PACK_SERIAL, WRS_PACK_INLINE, WRS_PACK_SUBGROUP, WRS_PACK_STREAM}) {
             wrs_tuning t3 = best;
             t3.direct = f;
             const float ms3 = measure(t3);
@@ -209,7 +210,7 @@ wrs_tuning descend(M&& measure_raw, bool direct_rts_only) {
                 best = t3;
             }
         }
-        // the merge shape does not apply to them
+        // the merge shape does not apply to them (nor to the streamed form)
         sweep = best.direct >= WRS_PACK_SERIAL;
     }
     if (!sweep) {
