@@ -193,7 +193,7 @@ struct StreamBufs {
     AliasEntry* hrow = nullptr;
     uint16_t* cross = nullptr; // tile offset of the light a heavy's key crosses at
     uint32_t* qa = nullptr;
-    uint16_t* seg = nullptr;   // kSegWords per tile: heavies before each 256-element
+    uint32_t* seg = nullptr;   // kSegWords per tile: heavies before each 256-element
                                // segment, crossings before each segment boundary
 };
 constexpr uint32_t kSegWords = 64;  // [0, 16) heavies, [32, 49) crossings

@@ -100,6 +100,9 @@ __device__ __forceinline__ bool qkey_exact(float w, int K) {
     const int e = (int)(__float_as_uint(w) >> 23);
     return e != 0 && e - 150 + K >= 0;
 }
+// The same test over many weights: the smallest exponent field among them,
+// against the smallest exact one.
+__device__ __forceinline__ uint32_t qkey_emin(int K) { return (uint32_t)max(1, 150 - K); }
 __device__ __forceinline__ uint64_t qkey(float w, int K, bool wave_slow) {
     const uint32_t b = __float_as_uint(w);
     const int e = (int)(b >> 23);
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
                                                    uint64_t* __restrict__ states,
                                                    uint32_t* __restrict__ slot_idx,
                                                    float* __restrict__ slot_w,
-                                                   uint16_t* __restrict__ seg) {
+                                                   uint32_t* __restrict__ seg) {
     __shared__ float s_buf[kPadAgg];
     __shared__ uint32_t s_c[kWaves];
     __shared__ uint64_t s_l[kWaves], s_hl[kWaves], s_hh[kWaves];
@@ -284,20 +287,19 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
     const int K = prm->key_k;
     const uint32_t e0 = (uint32_t)(kItems * t);
     const uint32_t nvalid = e0 >= m ? 0u : min((uint32_t)kItems, m - e0);
-    uint32_t hm = 0;
+    uint32_t hm = 0, emn = 255u;
     uint64_t ls = 0;
-    bool inexact = false;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const float v = s_buf[17 * t + k];
         const bool heavy = v > pivot;  // weights past m are zero: light, key 0
         ls += heavy ? 0ull : qkey(v, K, false);
-        inexact |= !heavy && !qkey_exact(v, K);
+        emn = min(emn, heavy ? 255u : __float_as_uint(v) >> 23);
         hm |= (heavy ? 1u : 0u) << k;
         asm volatile("" : "+v"(ls));  // one element at a time: few registers
     }
     hm &= (1u << nvalid) - 1u;
-    if (__any(inexact)) ls = light_keys_round(s_buf + 17 * t, pivot, K);
+    if (__any(emn < qkey_emin(K))) ls = light_keys_round(s_buf + 17 * t, pivot, K);
     const uint32_t c = (uint32_t)__popc(hm);
     const uint32_t ci = wave_incl_scan_u32(c);
     if (lane == kWave - 1) s_c[wid] = ci;
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
         ctile += s_c[q];
     }
     // heavies before each segment of 256 elements (16 threads)
-    if ((t & 15) == 0) seg[(uint64_t)tile * kSegWords + (uint32_t)(t >> 4)] = (uint16_t)slot;
+    if ((t & 15) == 0) seg[(uint64_t)tile * kSegWords + (uint32_t)(t >> 4)] = slot;
     uint64_t hs = 0;
     for (uint32_t mk = hm; mk; mk &= mk - 1u) {
         const int k = __builtin_ctz(mk);
@@ -362,6 +364,18 @@ __global__ __launch_bounds__(kWave) void k_sp_heavy(uint32_t n, const Params* __
     const uint32_t base = tile * (uint32_t)kTile;
     const uint64_t* ag = states + (uint64_t)tile * kCarryWords;
     const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
+    // the first two chunks of slots are loaded with the records (the slot
+    // range [base, base + kTile) is allocated whatever the tile's count)
+    constexpr int kPre = 2;
+    float pv[kPre];
+    uint32_t pid[kPre];
+#pragma unroll
+    for (int r = 0; r < kPre; ++r) {
+        const uint32_t k = (uint32_t)(kWave * r + lane);
+        const bool in = base + k < n;
+        pv[r] = in ? slot_w[base + k] : 0.0f;
+        pid[r] = in ? slot_idx[base + k] : 0u;
+    }
     const uint32_t ct = (uint32_t)ag[4];
     if (ct == 0) return;
     const uint32_t h0 = (uint32_t)ex[4];
@@ -371,8 +385,15 @@ __global__ __launch_bounds__(kWave) void k_sp_heavy(uint32_t n, const Params* __
     for (uint32_t r = 0; r < ct; r += kWave) {  // wave-uniform
         const uint32_t k = r + (uint32_t)lane;
         const bool act = k < ct;
-        const float v = act ? slot_w[base + k] : 0.0f;
-        const uint32_t id = act ? slot_idx[base + k] : 0u;
+        float v;
+        uint32_t id;
+        if (r < (uint32_t)(kPre * kWave)) {
+            v = act ? (r == 0 ? pv[0] : pv[1]) : 0.0f;
+            id = act ? (r == 0 ? pid[0] : pid[1]) : 0u;
+        } else {
+            v = act ? slot_w[base + k] : 0.0f;
+            id = act ? slot_idx[base + k] : 0u;
+        }
         // a heavy key is a multiple of 2^30 (w >= mean): key / 2^30 < 2^57, as k_sp_agg sums it
         const u128 q = act ? (u128)qkey_round(v, K - 30) << 30 : (u128)0;  // three limbs
         const uint64_t s0 = wave_incl_scan_u64((uint64_t)(uint32_t)q);
@@ -412,14 +433,20 @@ __global__ __launch_bounds__(kBlock) void k_sp_qrange(uint32_t n, const Params* 
         if (lane == 0) qa[t] = hq;
         return;
     }
+    // the first round's probes do not depend on the tile: loaded with its record
+    const uint32_t step0 = (hq + (uint32_t)kWave - 1u) / (uint32_t)kWave;
+    const bool in0 = (uint64_t)lane * step0 < hq;
+    const u128 y0 = in0 ? hkey[(uint32_t)lane * step0] : (u128)0;
     const u128 x0 = tile_key(states, n, tiles, t, prm->mean_k);
     uint32_t lo = 0, hi = hq;  // the answer lies in [lo, hi]
+    bool first = true;
     while (lo < hi) {          // wave-uniform
         const uint32_t R = hi - lo;
         const uint32_t step = (R + (uint32_t)kWave - 1u) / (uint32_t)kWave;
         const uint32_t p = lo + (uint32_t)lane * step;
         const bool in = (uint64_t)lane * step < R;
-        const bool pr = in && hkey[p] > x0;
+        const bool pr = in && (first ? y0 : hkey[p]) > x0;
+        first = false;
         const uint64_t b = __ballot(pr);
         const uint64_t bi = __ballot(in);
         if (b) {
@@ -456,7 +483,7 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
                                                const uint32_t* __restrict__ qa,
                                                AliasEntry* __restrict__ hrow,
                                                uint16_t* __restrict__ cross,
-                                               uint16_t* __restrict__ seg) {
+                                               uint32_t* __restrict__ seg) {
     using G = Geo<B>;
     constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
     __shared__ uint64_t s_k[kTile];
@@ -490,13 +517,12 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
         y0 = hkey[q0 + t];
         i0 = hidx[q0 + 1u + t];
     }
-    bool slow = false;
     uint32_t lc[G::kG];
     uint64_t ls[G::kG];
-    uint32_t lm = 0;
+    uint32_t lm = 0, emn = 255u;
 #pragma unroll
-    for (int k = 0; k < 4 * G::kG; ++k) slow |= !(x[k] > pivot) && !qkey_exact(x[k], K);
-    slow = __any(slow);
+    for (int k = 0; k < 4 * G::kG; ++k) emn = min(emn, x[k] > pivot ? 255u : __float_as_uint(x[k]) >> 23);
+    const bool slow = __any(emn < qkey_emin(K));
 #pragma unroll
     for (int j = 0; j < G::kG; ++j) {
         const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
@@ -540,7 +566,7 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
     }
     __syncthreads();
     const u128 xend = s_xs[G::kS];
-    uint16_t* cseg = seg + (uint64_t)tile * kSegWords + 32u;  // crossings before 256 s
+    uint32_t* cseg = seg + (uint64_t)tile * kSegWords + 32u;  // crossings before 256 s
     if (t == 0) cseg[0] = 0;
     uint32_t elast = 0;  // the last key's crossing in the previous chunk
     for (uint32_t a0 = 0; a0 < kc; a0 += B) {  // block-uniform
@@ -585,10 +611,10 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
             const int prev = t > 0 ? (int)s_e[t - 1] : (a0 > 0 ? (int)elast : -1);
             const int slo = prev < 0 ? 1 : prev / (int)kSeg + 1;
             const int shi = min((int)G::kS, (int)e / (int)kSeg);
-            for (int sg = slo; sg <= shi; ++sg) cseg[sg] = (uint16_t)a;
+            for (int sg = slo; sg <= shi; ++sg) cseg[sg] = a;
             if (a + 1u == kc)
                 for (int sg = max(slo, (int)e / (int)kSeg + 1); sg <= (int)G::kS; ++sg)
-                    cseg[sg] = (uint16_t)kc;
+                    cseg[sg] = kc;
         }
         elast = chunk_last;
         __syncthreads();
@@ -609,7 +635,7 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
                                                 const uint16_t* __restrict__ cross,
                                                 const uint32_t* __restrict__ qa,
                                                 const AliasEntry* __restrict__ hrow,
-                                                const uint16_t* __restrict__ seg,
+                                                const uint32_t* __restrict__ seg,
                                                 AliasEntry* __restrict__ table) {
     using G = Geo<B>;
     constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
@@ -657,8 +683,8 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
     const uint32_t h0 = (uint32_t)ex[4];
     const uint32_t ct = (uint32_t)ag[4];
-    const uint16_t* hseg = seg + (uint64_t)tile * kSegWords;
-    const uint16_t* cseg = hseg + 32u;
+    const uint32_t* hseg = seg + (uint64_t)tile * kSegWords;
+    const uint32_t* cseg = hseg + 32u;
     const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     float x[4 * G::kG];
     nat_load<B>(w, base, m, aligned, x);
@@ -672,7 +698,6 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     }
     const uint32_t kc = q1 - q0;
     const bool cached = kc <= kKeyCache && ct <= kRowCache;
-    const bool has_seg = kc <= 0xffffu;  // the crossing prefixes fit 16 bits
     constexpr int kPer = (int)(kKeyCache / B);
     constexpr int kRowPer = (int)(kRowCache / B);
     if (cached) {
@@ -696,13 +721,8 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     auto crossings = [&](const auto* cr, int j, uint32_t (&cx)[4]) {
         const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
         uint32_t lo, hi;
-        if (has_seg) {
-            lo = clo[j];
-            hi = chi[j];
-        } else {
-            lo = 0;
-            hi = kc;
-        }
+        lo = clo[j];
+        hi = chi[j];
         uint32_t a = lo, b = hi;  // first crossing at or past e0
         while (a < b) {
             const uint32_t mid = (a + b) >> 1;

@@ -91,7 +91,7 @@ void Workspace::ensure_stream() {
         shrow = dev_alloc<wrsk::AliasEntry>(max_n);
         scross = dev_alloc<uint16_t>(max_n);
         sqa = dev_alloc<uint32_t>(wrsk::tiles_for(max_n) + 2);
-        sseg = dev_alloc<uint16_t>(wrsk::kSegWords * wrsk::tiles_for(max_n));
+        sseg = dev_alloc<uint32_t>(wrsk::kSegWords * wrsk::tiles_for(max_n));
     } catch (...) {
         void* ptrs[] = {skey, shidx, shrow, scross, sqa, sseg};
         for (void* p : ptrs)

@@ -68,7 +68,7 @@ struct Workspace {
     wrsk::AliasEntry* shrow = nullptr;
     uint16_t* scross = nullptr;
     uint32_t* sqa = nullptr;
-    uint16_t* sseg = nullptr;
+    uint32_t* sseg = nullptr;
 
     static Workspace allocate(uint64_t max_n, bool partition_buffers);
     // the full prefix arrays of debug builds and of the sample statistics,
