@@ -184,33 +184,6 @@ __device__ __forceinline__ void nat_scan(const uint32_t (&lc)[Geo<B>::kG],
         o.excl_s[j] = si - ls[j];
     }
 }
-// Exclusive prefixes of the (row, wave) totals in element order (row-major),
-// by the first Geo<B>::kW * Geo<B>::kG lanes of the block; the caller
-// synchronises before and after.
-template <int B>
-__device__ __forceinline__ void nat_prefix(const uint32_t* s_c, const uint64_t* s_s,
-                                           uint32_t* s_pc, u128* s_ps) {
-    const int i = threadIdx.x;
-    if (i < Geo<B>::kS) {
-        uint32_t c = 0;
-        u128 sm = 0;
-        for (int k = 0; k < i; ++k) {
-            c += s_c[k];
-            sm += (u128)s_s[k];
-        }
-        s_pc[i] = c;
-        s_ps[i] = sm;
-    }
-}
-template <int B>
-__device__ __forceinline__ void nat_prefix_c(const uint32_t* s_c, uint32_t* s_pc) {
-    const int i = threadIdx.x;
-    if (i < Geo<B>::kS) {
-        uint32_t c = 0;
-        for (int k = 0; k < i; ++k) c += s_c[k];
-        s_pc[i] = c;
-    }
-}
 __device__ __forceinline__ bool le128(u128 a, u128 b) {
     const uint64_t ah = (uint64_t)(a >> 64), bh = (uint64_t)(b >> 64);
     return ah < bh || (ah == bh && (uint64_t)a <= (uint64_t)b);
@@ -289,14 +262,22 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
     const uint32_t nvalid = e0 >= m ? 0u : min((uint32_t)kItems, m - e0);
     uint32_t hm = 0, emn = 255u;
     uint64_t ls = 0;
+    // four elements at a time (one LDS wait each), the key of every element
+    // computed and the lights' kept by selects
 #pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-        const float v = s_buf[17 * t + k];
-        const bool heavy = v > pivot;  // weights past m are zero: light, key 0
-        ls += heavy ? 0ull : qkey(v, K, false);
-        emn = min(emn, heavy ? 255u : __float_as_uint(v) >> 23);
-        hm |= (heavy ? 1u : 0u) << k;
-        asm volatile("" : "+v"(ls));  // one element at a time: few registers
+    for (int k0 = 0; k0 < kItems; k0 += 4) {
+        float v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[c] = s_buf[17 * t + k0 + c];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool heavy = v[c] > pivot;  // weights past m are zero: light, key 0
+            const uint64_t q = qkey(v[c], K, false);
+            ls += heavy ? 0ull : q;
+            emn = min(emn, heavy ? 255u : __float_as_uint(v[c]) >> 23);
+            hm |= (heavy ? 1u : 0u) << (k0 + c);
+        }
+        asm volatile("" : "+v"(ls));  // a batch at a time: few registers
     }
     hm &= (1u << nvalid) - 1u;
     if (__any(emn < qkey_emin(K))) ls = light_keys_round(s_buf + 17 * t, pivot, K);
@@ -348,8 +329,8 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
     }
 }
 
-// ---- pass 3a: the heavies' keys in heavy order (one wave per tile) ----------
-__global__ __launch_bounds__(kWave) void k_sp_heavy(uint32_t n, const Params* __restrict__ prm,
+// ---- pass 3a: the heavies' keys in heavy order (one wave per tile, four a block)
+__global__ __launch_bounds__(kBlock) void k_sp_heavy(uint32_t n, const Params* __restrict__ prm,
                                                     const uint64_t* __restrict__ states,
                                                     const uint32_t* __restrict__ slot_idx,
                                                     const float* __restrict__ slot_w,
@@ -360,7 +341,8 @@ __global__ __launch_bounds__(kWave) void k_sp_heavy(uint32_t n, const Params* __
     if (H == 0 || H == n) return;  // identity table
     const int lane = lane_id();
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
-    const uint32_t tile = blockIdx.x;
+    const uint32_t tile = blockIdx.x * (uint32_t)kWaves + threadIdx.x / kWave;  // one wave each
+    if (tile >= tiles) return;  // the whole wave
     const uint32_t base = tile * (uint32_t)kTile;
     const uint64_t* ag = states + (uint64_t)tile * kCarryWords;
     const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
@@ -488,10 +470,9 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
     constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
     __shared__ uint64_t s_k[kTile];
     __shared__ u128 s_xs[G::kS + 1];
-    __shared__ uint16_t s_e[B];
-    __shared__ uint32_t s_c[G::kS], s_pc[G::kS];
+    __shared__ uint32_t s_hist[G::kS + 1];
+    __shared__ uint32_t s_c[G::kS];
     __shared__ uint64_t s_s[G::kS];
-    __shared__ u128 s_ps[G::kS];
     if (prm->error) return;
     const uint32_t H = prm->heavy_count;
     if (H == 0 || H == n) return;
@@ -555,20 +536,20 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
         reinterpret_cast<ulonglong2*>(d)[0] = make_ulonglong2(kr[0], kr[1]);
         reinterpret_cast<ulonglong2*>(d)[1] = make_ulonglong2(kr[2], kr[3]);
     }
-    __syncthreads();
-    nat_prefix<B>(s_c, s_s, s_pc, s_ps);
-    __syncthreads();
+    if (t <= G::kS) s_hist[t] = 0;  // crossings per segment (G::kS: past the last light)
     const u128 x0 = tile_key(states, n, tiles, tile, mk);
+    __syncthreads();
     if (t <= G::kS) {  // segment start keys; s_xs[kS]: the key after the tile's lights
-        const uint32_t c = t < G::kS ? s_pc[t] : s_pc[G::kS - 1] + s_c[G::kS - 1];
-        const u128 sm = t < G::kS ? s_ps[t] : s_ps[G::kS - 1] + (u128)s_s[G::kS - 1];
+        uint32_t c = 0;
+        u128 sm = 0;
+        for (int k = 0; k < t; ++k) {
+            c += s_c[k];
+            sm += (u128)s_s[k];
+        }
         s_xs[t] = x0 + (u128)c * mk - sm;
     }
     __syncthreads();
     const u128 xend = s_xs[G::kS];
-    uint32_t* cseg = seg + (uint64_t)tile * kSegWords + 32u;  // crossings before 256 s
-    if (t == 0) cseg[0] = 0;
-    uint32_t elast = 0;  // the last key's crossing in the previous chunk
     for (uint32_t a0 = 0; a0 < kc; a0 += B) {  // block-uniform
         const uint32_t a = a0 + (uint32_t)t;
         uint32_t e = 0xffffu;
@@ -602,22 +583,14 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
             hrow[q0 + a] = AliasEntry{p, a0 == 0 ? i0 : hidx[q0 + 1u + a]};
             cross[q0 + a] = (uint16_t)e;
         }
-        // crossings before each segment boundary 256 s: key a is the first at or
-        // past the boundaries in (e(a-1), e(a)]; past the last key: kc
-        s_e[t] = (uint16_t)e;
-        __syncthreads();
-        const uint32_t chunk_last = s_e[B - 1];
-        if (a < kc) {
-            const int prev = t > 0 ? (int)s_e[t - 1] : (a0 > 0 ? (int)elast : -1);
-            const int slo = prev < 0 ? 1 : prev / (int)kSeg + 1;
-            const int shi = min((int)G::kS, (int)e / (int)kSeg);
-            for (int sg = slo; sg <= shi; ++sg) cseg[sg] = a;
-            if (a + 1u == kc)
-                for (int sg = max(slo, (int)e / (int)kSeg + 1); sg <= (int)G::kS; ++sg)
-                    cseg[sg] = kc;
-        }
-        elast = chunk_last;
-        __syncthreads();
+        if (a < kc) atomicAdd(&s_hist[min(e / kSeg, (uint32_t)G::kS)], 1u);
+    }
+    // crossings before each segment boundary 256 s (s <= kS)
+    __syncthreads();
+    if (t <= G::kS) {
+        uint32_t c = 0;
+        for (int k = 0; k < t; ++k) c += s_hist[k];
+        seg[(uint64_t)tile * kSegWords + 32u + (uint32_t)t] = c;
     }
 }
 
@@ -717,12 +690,28 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
         }
     }
     __syncthreads();
-    // crossings before element e: from the segment's slice of the sorted list
+    // crossings before element e: from the segment's slice [clo, chi) of the
+    // sorted list -- a short slice (every segment of a spread distribution) by
+    // one wave-uniform pass over it, a long one by a search and a walk
+    constexpr uint32_t kShortSlice = 32;
     auto crossings = [&](const auto* cr, int j, uint32_t (&cx)[4]) {
         const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
-        uint32_t lo, hi;
-        lo = clo[j];
-        hi = chi[j];
+        const uint32_t lo = clo[j], hi = chi[j];
+        if (hi - lo <= kShortSlice) {  // wave-uniform
+            uint32_t below = 0, c0 = 0, c1 = 0, c2 = 0;
+            for (uint32_t k = lo; k < hi; ++k) {
+                const uint32_t v = cr[k];
+                below += v < e0 ? 1u : 0u;
+                c0 += v == e0 ? 1u : 0u;
+                c1 += v == e0 + 1u ? 1u : 0u;
+                c2 += v == e0 + 2u ? 1u : 0u;
+            }
+            cx[0] = lo + below;
+            cx[1] = cx[0] + c0;
+            cx[2] = cx[1] + c1;
+            cx[3] = cx[2] + c2;
+            return;
+        }
         uint32_t a = lo, b = hi;  // first crossing at or past e0
         while (a < b) {
             const uint32_t mid = (a + b) >> 1;
@@ -787,7 +776,8 @@ hipError_t launch_stream_pack(const float* w, uint64_t n, int prec, const Params
     const uint32_t nn = (uint32_t)n;
     const uint32_t tiles = (uint32_t)tiles_for(n);
     u128* hkey = static_cast<u128*>(b.hkey);
-    k_sp_heavy<<<tiles, kWave, 0, s>>>(nn, prm, states, b.slot_idx, b.slot_w, hkey, b.hidx);
+    k_sp_heavy<<<(tiles + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, b.slot_idx,
+                                                                  b.slot_w, hkey, b.hidx);
     k_sp_qrange<<<(tiles + 1u + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, hkey, b.qa);
     if (ev_keys) (void)hipEventRecord(ev_keys, s);
     if (prec == kFP32) {
