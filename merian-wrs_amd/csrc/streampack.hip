@@ -74,11 +74,11 @@ constexpr int kTableBlock = 256;  // table pass
 constexpr uint32_t kKeyCache = 1024;         // alias indices staged in LDS per tile (table pass)
 constexpr uint32_t kRowCache = 512;          // heavy rows staged in LDS per tile (table pass)
 
-// Tile layout of the streaming passes: 512 threads per tile, thread t holds
-// the float4 groups t + 512 j (j < 2), i.e. elements 2048 j + 4 t + c (c < 4)
-// -- the loads and the row stores are coalesced as they are, the scans run
-// per row of groups, and two groups per thread keep the registers low enough
-// for 8 waves per SIMD.
+// Tile layout of the streaming passes: B threads per tile, thread t holds the
+// float4 groups t + B j (j < 4096 / (4 B)), i.e. elements 4 B j + 4 t + c
+// (c < 4) -- the loads and the row stores are coalesced as they are, and the
+// scans run per row of groups (the rows pass: B = 512, two groups a thread,
+// registers low enough for 8 waves per SIMD; the table pass: B = 256, four).
 
 // rint(w 2^K) for a float w >= 0 with w 2^K < 2^64: the scaling is exact in
 // FP32, rintf rounds below 2^23 (where the float's bits run out), and the
@@ -136,8 +136,8 @@ __device__ __forceinline__ float ratio(double x, const Params* prm) {
     else return (float)(x * prm->inv_mean);
 }
 
-// The 4 float4 groups of this thread (zero past m; scalar loads where w is
-// not 16-byte aligned or a group straddles m).
+// The float4 groups of this thread (zero past m; scalar loads where w is not
+// 16-byte aligned or a group straddles m).
 template <int B>
 __device__ __forceinline__ void nat_load(const float* __restrict__ w, uint32_t base, uint32_t m,
                                          bool aligned, float (&x)[4 * Geo<B>::kG]) {
@@ -209,18 +209,6 @@ __device__ __forceinline__ u128 tile_key(const uint64_t* __restrict__ states, ui
     return (u128)r0 * (u128)mk - lp;
 }
 
-// Heavy indices of one tile's alias range, read from LDS (ranges that fit the
-// tile's cache) or from memory (the others).  Two types rather than one with
-// a test per access: a select between an LDS and a global pointer would make
-// every access a flat load that waits for both.
-struct LdsIdx {
-    const uint32_t* i;
-    __device__ __forceinline__ uint32_t idx(uint32_t a) const { return i[a]; }
-};
-struct MemIdx {
-    const uint32_t* i;
-    __device__ __forceinline__ uint32_t idx(uint32_t a) const { return i[a]; }
-};
 }  // namespace
 
 // The light key sum of 16 weights by the rounding route (waves holding a
@@ -480,7 +468,6 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
     const uint32_t q0 = qa[tile], q1 = qa[tile + 1u];
     if (q0 >= q1) return;  // no crossing among this tile's lights: its weights are not read
     const int t = threadIdx.x;
-    const int wid = t / kWave;
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
@@ -596,10 +583,14 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
 
 // ---- pass 4: the table, in element order -------------------------------------
 // A light's alias is heavy q0 + (the crossings at lights before it in the
-// tile); a heavy's row comes from hrow.  No keys are compared here: the
-// segment prefixes of the aggregate pass (heavies) and of the rows pass
-// (crossings, when the tile has any) and one wave scan give every element its
-// counts, with a single barrier for the staged indices and rows.
+// tile); a heavy's row comes from hrow.  No keys are compared here.  A tile
+// whose alias range and heavy rows fit in LDS (every tile of a spread
+// distribution) stages them and counts its crossings per element into an LDS
+// histogram, so that one wave scan gives every element both its heavy rank
+// and its crossing count on top of the segment prefixes of the aggregate pass
+// (heavies) and the rows pass (crossings).  Other tiles read the indices and
+// rows from memory and count crossings in their segment's slice of the sorted
+// crossing list.
 template <bool F32, int B>
 __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uint32_t n,
                                                 const Params* __restrict__ prm,
@@ -613,7 +604,7 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     using G = Geo<B>;
     constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
     static_assert(kSeg == 4u * kWave, "a segment is one wave's share of a row");
-    __shared__ uint16_t s_cr[kKeyCache];        // cross[q0 ..]
+    __shared__ uint32_t s_hist[kTile / 2];      // crossings at each element, two 16-bit counts a word
     __shared__ uint32_t s_idx[kKeyCache + 1];   // hidx[q0 .. q0 + kc]
     __shared__ AliasEntry s_row[kRowCache];     // hrow[h0 ..]
     if (prm->error) return;
@@ -661,41 +652,118 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
     float x[4 * G::kG];
     nat_load<B>(w, base, m, aligned, x);
-    // the segment prefixes of this wave's groups (wave-uniform), loaded up front
+    const uint32_t kc = q1 - q0;
+    // the segment prefixes of this wave's groups (wave-uniform; the crossing
+    // words are written for tiles with crossings only)
     uint32_t hpre[G::kG], clo[G::kG], chi[G::kG];
 #pragma unroll
     for (int j = 0; j < G::kG; ++j) {
         hpre[j] = hseg[G::kW * j + wid];
-        clo[j] = cseg[G::kW * j + wid];
-        chi[j] = cseg[G::kW * j + wid + 1];
+        clo[j] = kc ? cseg[G::kW * j + wid] : 0u;
+        chi[j] = kc ? cseg[G::kW * j + wid + 1] : 0u;
     }
-    const uint32_t kc = q1 - q0;
     const bool cached = kc <= kKeyCache && ct <= kRowCache;
     constexpr int kPer = (int)(kKeyCache / B);
     constexpr int kRowPer = (int)(kRowCache / B);
-    if (cached) {
+    // the light p: w / mean rounded once to float
+    auto light_p = [&](float v) { return fminf(ratio<F32>((double)v, prm), 1.0f); };
+    if (cached) {  // block-uniform
+        uint32_t cv[kPer], iv[kPer];
+        AliasEntry rv[kRowPer];
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
             const uint32_t i = (uint32_t)t + B * r;
-            if (i < kc) {
-                s_cr[i] = cross[q0 + i];
-                s_idx[i] = hidx[q0 + i];
-            }
+            cv[r] = i < kc ? (uint32_t)cross[q0 + i] : 0xffffu;
+            iv[r] = i < kc ? hidx[q0 + i] : 0u;
         }
-        if (t == 0) s_idx[kc] = hidx[q1];  // alias of the tile's last lights
+        const uint32_t ilast = t == 0 ? hidx[q1] : 0u;  // alias of the tile's last lights
 #pragma unroll
         for (int r = 0; r < kRowPer; ++r) {
             const uint32_t i = (uint32_t)t + B * r;
-            if (i < ct) s_row[i] = hrow[h0 + i];
+            if (i < ct) rv[r] = hrow[h0 + i];
         }
+        if (kc) {
+            uint4* h4 = reinterpret_cast<uint4*>(s_hist);
+            for (uint32_t k = (uint32_t)t; k < kTile / 8u; k += B) h4[k] = make_uint4(0u, 0u, 0u, 0u);
+            __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t i = (uint32_t)t + B * r;
+            if (i < kc) s_idx[i] = iv[r];
+            // 0xffff: past the tile's last light (the last tile only)
+            if (cv[r] < kTile) atomicAdd(&s_hist[cv[r] >> 1], 1u << (16u * (cv[r] & 1u)));
+        }
+        if (t == 0) s_idx[kc] = ilast;
+#pragma unroll
+        for (int r = 0; r < kRowPer; ++r) {
+            const uint32_t i = (uint32_t)t + B * r;
+            if (i < ct) s_row[i] = rv[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < G::kG; ++j) {
+            const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
+            uint32_t hm = 0, hc = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const bool heavy = x[4 * j + c] > pivot && e0 + c < m;
+                hm |= (heavy ? 1u : 0u) << c;
+                hc += heavy ? 1u : 0u;
+            }
+            uint32_t n0 = 0, n1 = 0, n2 = 0, nt = 0;  // crossings at e0 .. e0 + 2, at all four
+            if (kc) {
+                const uint2 hw = *reinterpret_cast<const uint2*>(&s_hist[e0 >> 1]);
+                n0 = hw.x & 0xffffu;
+                n1 = hw.x >> 16;
+                n2 = hw.y & 0xffffu;
+                nt = n0 + n1 + n2 + (hw.y >> 16);
+            }
+            // one scan for both counts: heavies (<= 256 a wave row) in the low
+            // 12 bits, crossings (<= kKeyCache) above
+            const uint32_t v = hc | (nt << 12);
+            const uint32_t exv = wave_incl_scan_u32(v) - v;
+            uint32_t hr = hpre[j] + (exv & 0xfffu);
+            uint32_t cx = clo[j] + (exv >> 12);
+            const uint32_t nc[3] = {n0, n1, n2};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if ((hm >> c) & 1u) {
+                    if (h0 + hr + 1u == H) {
+                        row[c] = make_uint2(__float_as_uint(1.0f), base + e0 + c);
+                    } else {
+                        const AliasEntry e = s_row[hr];
+                        row[c] = make_uint2(__float_as_uint(e.p), e.a);
+                    }
+                    ++hr;
+                } else {  // a light (or past m: not stored)
+                    row[c] = make_uint2(__float_as_uint(light_p(x[4 * j + c])), s_idx[cx]);
+                }
+                if (c < 3) cx += nc[c];
+            }
+            store(j);
+        }
+        return;
     }
-    __syncthreads();
     // crossings before element e: from the segment's slice [clo, chi) of the
-    // sorted list -- a short slice (every segment of a spread distribution) by
-    // one wave-uniform pass over it, a long one by a search and a walk
+    // sorted list -- a short slice by one wave-uniform pass over it, a long one
+    // by a search and a walk
     constexpr uint32_t kShortSlice = 32;
-    auto crossings = [&](const auto* cr, int j, uint32_t (&cx)[4]) {
+    const uint16_t* cr = cross + q0;
+    const uint32_t* ix = hidx + q0;
+    const AliasEntry* hr_src = hrow + h0;
+#pragma unroll
+    for (int j = 0; j < G::kG; ++j) {
         const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
+        uint32_t hm = 0, hc = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool heavy = x[4 * j + c] > pivot && e0 + c < m;
+            hm |= (heavy ? 1u : 0u) << c;
+            hc += heavy ? 1u : 0u;
+        }
+        uint32_t hr = hpre[j] + wave_incl_scan_u32(hc) - hc;
+        uint32_t cx[4] = {0u, 0u, 0u, 0u};
         const uint32_t lo = clo[j], hi = chi[j];
         if (hi - lo <= kShortSlice) {  // wave-uniform
             uint32_t below = 0, c0 = 0, c1 = 0, c2 = 0;
@@ -710,56 +778,35 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
             cx[1] = cx[0] + c0;
             cx[2] = cx[1] + c1;
             cx[3] = cx[2] + c2;
-            return;
-        }
-        uint32_t a = lo, b = hi;  // first crossing at or past e0
-        while (a < b) {
-            const uint32_t mid = (a + b) >> 1;
-            if (cr[mid] < e0) a = mid + 1u;
-            else b = mid;
+        } else {
+            uint32_t a = lo, b = hi;  // first crossing at or past e0
+            while (a < b) {
+                const uint32_t mid = (a + b) >> 1;
+                if (cr[mid] < e0) a = mid + 1u;
+                else b = mid;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                while (a < hi && cr[a] < e0 + (uint32_t)c) ++a;
+                cx[c] = a;
+            }
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            while (a < hi && cr[a] < e0 + (uint32_t)c) ++a;
-            cx[c] = a;
-        }
-    };
-    auto rows = [&](const auto* cr, const uint32_t* ix, const AliasEntry* hr_src) {
-#pragma unroll
-        for (int j = 0; j < G::kG; ++j) {
-            const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
-            uint32_t hm = 0, hc = 0;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const bool heavy = x[4 * j + c] > pivot && e0 + c < m;
-                hm |= (heavy ? 1u : 0u) << c;
-                hc += heavy ? 1u : 0u;
-            }
-            uint32_t hr = hpre[j] + wave_incl_scan_u32(hc) - hc;
-            uint32_t cx[4] = {0u, 0u, 0u, 0u};
-            if (kc) crossings(cr, j, cx);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if ((hm >> c) & 1u) {
-                    if (h0 + hr + 1u == H) {
-                        row[c] = make_uint2(__float_as_uint(1.0f), base + e0 + c);
-                    } else {
-                        const AliasEntry e = hr_src[hr];
-                        row[c] = make_uint2(__float_as_uint(e.p), e.a);
-                    }
-                    ++hr;
-                } else {  // a light (or past m: not stored)
-                    const float p = fminf(ratio<F32>((double)x[4 * j + c], prm), 1.0f);
-                    row[c] = make_uint2(__float_as_uint(p), ix[cx[c]]);
+            if ((hm >> c) & 1u) {
+                if (h0 + hr + 1u == H) {
+                    row[c] = make_uint2(__float_as_uint(1.0f), base + e0 + c);
+                } else {
+                    const AliasEntry e = hr_src[hr];
+                    row[c] = make_uint2(__float_as_uint(e.p), e.a);
                 }
+                ++hr;
+            } else {  // a light (or past m: not stored)
+                row[c] = make_uint2(__float_as_uint(light_p(x[4 * j + c])), ix[cx[c]]);
             }
-            store(j);
         }
-    };
-    if (cached)
-        rows(s_cr, s_idx, s_row);
-    else
-        rows(cross + q0, hidx + q0, hrow + h0);
+        store(j);
+    }
 }
 
 // ---- launchers -----------------------------------------------------------------
