@@ -108,9 +108,10 @@ typedef struct {
                                those of the exact sweep (the merge forms' agree except
                                where FP64 rounding of their block offsets decides a
                                tie); its heavy probabilities round the exact residual
-                               once.  The first build of such a plan allocates 28 bytes
-                               per weight (keys, indices and rows of up to max_n - 1
-                               heavies); partition is ignored. */
+                               once.  The first build of such a plan allocates 30 bytes
+                               per weight (keys, indices, rows and crossing offsets of
+                               up to max_n - 1 heavies) and 260 bytes per tile of 4096
+                               weights; partition is ignored. */
 } wrs_tuning;
 #define WRS_PACK_DEFAULT 0u /* the direct form */
 #define WRS_PACK_DIRECT 1u

@@ -57,6 +57,10 @@ using namespace wrsdev;
 namespace {
 
 using u128 = unsigned __int128;
+template <bool V>
+struct Flag {
+    static constexpr bool value = V;
+};
 
 constexpr int kWaves = kBlock / kWave;       // waves of the heavy-range search blocks
 // Geometry of a streaming pass with B threads per tile: B / 64 waves, each
@@ -137,11 +141,23 @@ __device__ __forceinline__ float ratio(double x, const Params* prm) {
 }
 
 // The float4 groups of this thread (zero past m; scalar loads where w is not
-// 16-byte aligned or a group straddles m).
+// 16-byte aligned or a group straddles m).  A whole tile of aligned weights
+// (all but the last tile) takes a block-uniform branch to plain float4 loads.
 template <int B>
 __device__ __forceinline__ void nat_load(const float* __restrict__ w, uint32_t base, uint32_t m,
                                          bool aligned, float (&x)[4 * Geo<B>::kG]) {
     const uint32_t t = threadIdx.x;
+    if (aligned && m == (uint32_t)kTile) {
+#pragma unroll
+        for (int j = 0; j < Geo<B>::kG; ++j) {
+            const float4 r = ld_nt4(reinterpret_cast<const float4*>(w + base + Geo<B>::kStride * j + 4u * t));
+            x[4 * j] = r.x;
+            x[4 * j + 1] = r.y;
+            x[4 * j + 2] = r.z;
+            x[4 * j + 3] = r.w;
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < Geo<B>::kG; ++j) {
         const uint32_t e0 = Geo<B>::kStride * j + 4u * t;
@@ -606,7 +622,7 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     static_assert(kSeg == 4u * kWave, "a segment is one wave's share of a row");
     __shared__ uint32_t s_hist[kTile / 2];      // crossings at each element, two 16-bit counts a word
     __shared__ uint32_t s_idx[kKeyCache + 1];   // hidx[q0 .. q0 + kc]
-    __shared__ AliasEntry s_row[kRowCache];     // hrow[h0 ..]
+    __shared__ AliasEntry s_row[kRowCache + 1]; // hrow[h0 ..]
     if (prm->error) return;
     const int t = threadIdx.x;
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
@@ -615,12 +631,16 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const uint32_t H = prm->heavy_count;
+    // rows of group j: FULL (a whole tile into a 16-byte aligned table, a
+    // block-uniform choice made once per tile, so that the two paths stay
+    // apart in the code) two 16-byte stores; else the same where the group is
+    // whole and aligned, or per row
     const bool tab16 = (reinterpret_cast<uintptr_t>(table) & 15u) == 0;
+    const bool full16 = tab16 && m == (uint32_t)kTile;
     uint2 row[4];
-    // rows of group j: two 16-byte stores where the table allows, else per row
-    auto store = [&](int j) {
+    auto store = [&](int j, auto full) {
         const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
-        if (tab16 && e0 + 3u < m) {
+        if (decltype(full)::value || (tab16 && e0 + 3u < m)) {
             uint4* o = reinterpret_cast<uint4*>(table + base + e0);
             o[0] = make_uint4(row[0].x, row[0].y, row[1].x, row[1].y);
             o[1] = make_uint4(row[2].x, row[2].y, row[3].x, row[3].y);
@@ -637,7 +657,7 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
 #pragma unroll
             for (int c = 0; c < 4; ++c)
                 row[c] = make_uint2(__float_as_uint(1.0f), base + G::kStride * j + 4u * t + c);
-            store(j);
+            store(j, Flag<false>{});
         }
         return;
     }
@@ -701,48 +721,54 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
             if (i < ct) s_row[i] = rv[r];
         }
         __syncthreads();
+        auto body = [&](auto full) {
 #pragma unroll
-        for (int j = 0; j < G::kG; ++j) {
-            const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
-            uint32_t hm = 0, hc = 0;
+            for (int j = 0; j < G::kG; ++j) {
+                const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
+                uint32_t hm = 0, hc = 0;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const bool heavy = x[4 * j + c] > pivot && e0 + c < m;
-                hm |= (heavy ? 1u : 0u) << c;
-                hc += heavy ? 1u : 0u;
-            }
-            uint32_t n0 = 0, n1 = 0, n2 = 0, nt = 0;  // crossings at e0 .. e0 + 2, at all four
-            if (kc) {
-                const uint2 hw = *reinterpret_cast<const uint2*>(&s_hist[e0 >> 1]);
-                n0 = hw.x & 0xffffu;
-                n1 = hw.x >> 16;
-                n2 = hw.y & 0xffffu;
-                nt = n0 + n1 + n2 + (hw.y >> 16);
-            }
-            // one scan for both counts: heavies (<= 256 a wave row) in the low
-            // 12 bits, crossings (<= kKeyCache) above
-            const uint32_t v = hc | (nt << 12);
-            const uint32_t exv = wave_incl_scan_u32(v) - v;
-            uint32_t hr = hpre[j] + (exv & 0xfffu);
-            uint32_t cx = clo[j] + (exv >> 12);
-            const uint32_t nc[3] = {n0, n1, n2};
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if ((hm >> c) & 1u) {
-                    if (h0 + hr + 1u == H) {
-                        row[c] = make_uint2(__float_as_uint(1.0f), base + e0 + c);
-                    } else {
-                        const AliasEntry e = s_row[hr];
-                        row[c] = make_uint2(__float_as_uint(e.p), e.a);
-                    }
-                    ++hr;
-                } else {  // a light (or past m: not stored)
-                    row[c] = make_uint2(__float_as_uint(light_p(x[4 * j + c])), s_idx[cx]);
+                for (int c = 0; c < 4; ++c) {
+                    const bool heavy = x[4 * j + c] > pivot && e0 + c < m;
+                    hm |= (heavy ? 1u : 0u) << c;
+                    hc += heavy ? 1u : 0u;
                 }
-                if (c < 3) cx += nc[c];
+                uint32_t n0 = 0, n1 = 0, n2 = 0, nt = 0;  // crossings at e0 .. e0 + 2, at all four
+                if (kc) {
+                    const uint2 hw = *reinterpret_cast<const uint2*>(&s_hist[e0 >> 1]);
+                    n0 = hw.x & 0xffffu;
+                    n1 = hw.x >> 16;
+                    n2 = hw.y & 0xffffu;
+                    nt = n0 + n1 + n2 + (hw.y >> 16);
+                }
+                // one scan for both counts: heavies (<= 256 a wave row) in the low
+                // 12 bits, crossings (<= kKeyCache) above
+                const uint32_t v = hc | (nt << 12);
+                const uint32_t exv = wave_incl_scan_u32(v) - v;
+                uint32_t hr = hpre[j] + (exv & 0xfffu);
+                uint32_t cx = clo[j] + (exv >> 12);
+                const uint32_t nc[3] = {n0, n1, n2};
+                // both rows read and one selected, without branches: s_row[hr] for
+                // a light past the tile's last heavy (hr = ct) is read and unused
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const bool hv = (hm >> c) & 1u;
+                    const AliasEntry e = s_row[hr];
+                    const uint32_t ia = s_idx[cx];
+                    const float lp = light_p(x[4 * j + c]);
+                    const bool last = h0 + hr + 1u == H;  // the last heavy: (1, itself)
+                    const float p = hv ? (last ? 1.0f : e.p) : lp;
+                    const uint32_t a = hv ? (last ? base + e0 + (uint32_t)c : e.a) : ia;
+                    row[c] = make_uint2(__float_as_uint(p), a);
+                    hr += hv ? 1u : 0u;
+                    if (c < 3) cx += nc[c];
+                }
+                store(j, full);
             }
-            store(j);
-        }
+        };
+        if (full16)
+            body(Flag<true>{});
+        else
+            body(Flag<false>{});
         return;
     }
     // crossings before element e: from the segment's slice [clo, chi) of the
@@ -805,7 +831,7 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
                 row[c] = make_uint2(__float_as_uint(light_p(x[4 * j + c])), ix[cx[c]]);
             }
         }
-        store(j);
+        store(j, Flag<false>{});
     }
 }
 
