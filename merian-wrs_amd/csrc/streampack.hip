@@ -472,6 +472,7 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
                                                uint32_t* __restrict__ seg) {
     using G = Geo<B>;
     constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
+    static_assert(G::kS < kWave, "segment prefixes are scanned by one wave");
     __shared__ uint64_t s_k[kTile];
     __shared__ u128 s_xs[G::kS + 1];
     __shared__ uint32_t s_hist[G::kS + 1];
@@ -542,14 +543,15 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
     if (t <= G::kS) s_hist[t] = 0;  // crossings per segment (G::kS: past the last light)
     const u128 x0 = tile_key(states, n, tiles, tile, mk);
     __syncthreads();
-    if (t <= G::kS) {  // segment start keys; s_xs[kS]: the key after the tile's lights
-        uint32_t c = 0;
-        u128 sm = 0;
-        for (int k = 0; k < t; ++k) {
-            c += s_c[k];
-            sm += (u128)s_s[k];
-        }
-        s_xs[t] = x0 + (u128)c * mk - sm;
+    if (t < kWave) {  // wave 0: segment start keys; s_xs[kS]: the key after the tile's lights
+        // exclusive scans over lanes 0 .. kS of the segment light counts and
+        // key sums (a segment's sum is below 2^62: two 32-bit halves)
+        const uint32_t c = t < G::kS ? s_c[t] : 0u;
+        const uint64_t sv = t < G::kS ? s_s[t] : 0ull;
+        const uint32_t ce = wave_incl_scan_u32(c) - c;
+        const uint64_t sl = wave_incl_scan_u64(sv & 0xffffffffull) - (sv & 0xffffffffull);
+        const uint64_t sh = wave_incl_scan_u64(sv >> 32) - (sv >> 32);
+        if (t <= G::kS) s_xs[t] = x0 + (u128)ce * mk - ((u128)sl + ((u128)sh << 32));
     }
     __syncthreads();
     const u128 xend = s_xs[G::kS];
@@ -590,10 +592,10 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
     }
     // crossings before each segment boundary 256 s (s <= kS)
     __syncthreads();
-    if (t <= G::kS) {
-        uint32_t c = 0;
-        for (int k = 0; k < t; ++k) c += s_hist[k];
-        seg[(uint64_t)tile * kSegWords + 32u + (uint32_t)t] = c;
+    if (t < kWave) {  // wave 0: exclusive scan over lanes 0 .. kS
+        const uint32_t h = t <= G::kS ? s_hist[t] : 0u;
+        const uint32_t c = wave_incl_scan_u32(h) - h;
+        if (t <= G::kS) seg[(uint64_t)tile * kSegWords + 32u + (uint32_t)t] = c;
     }
 }
 
