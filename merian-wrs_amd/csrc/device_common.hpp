@@ -145,6 +145,11 @@ __device__ __forceinline__ float4 ld_nt4(const float4* p) {
     const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
     return make_float4(v.x, v.y, v.z, v.w);
 }
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_nt2(const float2* p) {
+    const f32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x2_t*>(p));
+    return make_float2(v.x, v.y);
+}
 __device__ __forceinline__ void st_nt4(float4* p, float4 v) {
     f32x4_t x;
     x.x = v.x;

@@ -604,11 +604,18 @@ __global__ __launch_bounds__(B) void k_sp_rows(const float* __restrict__ w, uint
 // tile); a heavy's row comes from hrow.  No keys are compared here.  A tile
 // whose alias range and heavy rows fit in LDS (every tile of a spread
 // distribution) stages them and counts its crossings per element into an LDS
-// histogram, so that one wave scan gives every element both its heavy rank
-// and its crossing count on top of the segment prefixes of the aggregate pass
+// histogram, so that wave scans give every element both its heavy rank and
+// its crossing count on top of the segment prefixes of the aggregate pass
 // (heavies) and the rows pass (crossings).  Other tiles read the indices and
 // rows from memory and count crossings in their segment's slice of the sorted
 // crossing list.
+//
+// Layout: wave w of row j owns segment s = kW j + w (elements [256 s, 256 s +
+// 256)), and its lane l the pairs of elements at 256 s + 2 l (half 0) and
+// 256 s + 128 + 2 l (half 1): a wave's 16-byte row stores then cover 1 KiB of
+// the table without gaps (a lane holding four consecutive rows writes them
+// as two 16-byte stores that each cover half of every line, measurably
+// slower), and its 8-byte weight loads 512 bytes.
 template <bool F32, int B>
 __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uint32_t n,
                                                 const Params* __restrict__ prm,
@@ -622,44 +629,47 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     using G = Geo<B>;
     constexpr uint32_t kSeg = kTile / G::kS;  // elements per segment (256)
     static_assert(kSeg == 4u * kWave, "a segment is one wave's share of a row");
+    constexpr uint32_t kHalf = kSeg / 2u;
     __shared__ uint32_t s_hist[kTile / 2];      // crossings at each element, two 16-bit counts a word
     __shared__ uint32_t s_idx[kKeyCache + 1];   // hidx[q0 .. q0 + kc]
     __shared__ AliasEntry s_row[kRowCache + 1]; // hrow[h0 ..]
     if (prm->error) return;
     const int t = threadIdx.x;
+    const uint32_t lane = (uint32_t)t & (kWave - 1u);
     const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
     const uint32_t H = prm->heavy_count;
-    // rows of group j: FULL (a whole tile into a 16-byte aligned table, a
+    // tile offset of this thread's pair (j, h)
+    auto pos = [&](int j, int h) {
+        return kSeg * (uint32_t)(G::kW * j + wid) + kHalf * (uint32_t)h + 2u * lane;
+    };
+    // the rows of a pair: FULL (a whole tile into a 16-byte aligned table, a
     // block-uniform choice made once per tile, so that the two paths stay
-    // apart in the code) two 16-byte stores; else the same where the group is
+    // apart in the code) one 16-byte store; else the same where the pair is
     // whole and aligned, or per row
     const bool tab16 = (reinterpret_cast<uintptr_t>(table) & 15u) == 0;
     const bool full16 = tab16 && m == (uint32_t)kTile;
-    uint2 row[4];
-    auto store = [&](int j, auto full) {
-        const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
-        if (decltype(full)::value || (tab16 && e0 + 3u < m)) {
-            uint4* o = reinterpret_cast<uint4*>(table + base + e0);
-            o[0] = make_uint4(row[0].x, row[0].y, row[1].x, row[1].y);
-            o[1] = make_uint4(row[2].x, row[2].y, row[3].x, row[3].y);
+    auto store = [&](uint32_t e0, uint2 r0, uint2 r1, auto full) {
+        if (decltype(full)::value || (tab16 && e0 + 1u < m)) {
+            *reinterpret_cast<uint4*>(table + base + e0) = make_uint4(r0.x, r0.y, r1.x, r1.y);
         } else {
             uint2* o = reinterpret_cast<uint2*>(table + base + e0);
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                if (e0 + c < m) o[c] = row[c];
+            if (e0 < m) o[0] = r0;
+            if (e0 + 1u < m) o[1] = r1;
         }
     };
     if (H == 0 || H == n) {  // all weights equal the mean up to its rounding: identity
 #pragma unroll
         for (int j = 0; j < G::kG; ++j) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                row[c] = make_uint2(__float_as_uint(1.0f), base + G::kStride * j + 4u * t + c);
-            store(j, Flag<false>{});
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t e0 = pos(j, h);
+                store(e0, make_uint2(__float_as_uint(1.0f), base + e0),
+                      make_uint2(__float_as_uint(1.0f), base + e0 + 1u), Flag<false>{});
+            }
         }
         return;
     }
@@ -671,11 +681,31 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     const uint32_t ct = (uint32_t)ag[4];
     const uint32_t* hseg = seg + (uint64_t)tile * kSegWords;
     const uint32_t* cseg = hseg + 32u;
-    const bool aligned = (reinterpret_cast<uintptr_t>(w) & 15u) == 0;
+    // x[4 j + 2 h + c]: element pos(j, h) + c (zero past m)
     float x[4 * G::kG];
-    nat_load<B>(w, base, m, aligned, x);
+    if ((reinterpret_cast<uintptr_t>(w) & 7u) == 0 && m == (uint32_t)kTile) {  // block-uniform
+#pragma unroll
+        for (int j = 0; j < G::kG; ++j) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float2 v = ld_nt2(reinterpret_cast<const float2*>(w + base + pos(j, h)));
+                x[4 * j + 2 * h] = v.x;
+                x[4 * j + 2 * h + 1] = v.y;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < G::kG; ++j) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t e0 = pos(j, h);
+                x[4 * j + 2 * h] = e0 < m ? w[base + e0] : 0.0f;
+                x[4 * j + 2 * h + 1] = e0 + 1u < m ? w[base + e0 + 1u] : 0.0f;
+            }
+        }
+    }
     const uint32_t kc = q1 - q0;
-    // the segment prefixes of this wave's groups (wave-uniform; the crossing
+    // the segment prefixes of this wave's segments (wave-uniform; the crossing
     // words are written for tiles with crossings only)
     uint32_t hpre[G::kG], clo[G::kG], chi[G::kG];
 #pragma unroll
@@ -684,6 +714,17 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
         clo[j] = kc ? cseg[G::kW * j + wid] : 0u;
         chi[j] = kc ? cseg[G::kW * j + wid + 1] : 0u;
     }
+    // heavy flags (bit c) and counts of the two pairs of segment j
+    auto heavies = [&](int j, uint32_t (&hm)[2], uint32_t (&hc)[2]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t e0 = pos(j, h);
+            const bool a = x[4 * j + 2 * h] > pivot && e0 < m;
+            const bool b = x[4 * j + 2 * h + 1] > pivot && e0 + 1u < m;
+            hm[h] = (a ? 1u : 0u) | (b ? 2u : 0u);
+            hc[h] = (a ? 1u : 0u) + (b ? 1u : 0u);
+        }
+    };
     const bool cached = kc <= kKeyCache && ct <= kRowCache;
     constexpr int kPer = (int)(kKeyCache / B);
     constexpr int kRowPer = (int)(kRowCache / B);
@@ -726,45 +767,48 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
         auto body = [&](auto full) {
 #pragma unroll
             for (int j = 0; j < G::kG; ++j) {
-                const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
-                uint32_t hm = 0, hc = 0;
+                uint32_t hm[2], hc[2], na[2], v[2];
+                heavies(j, hm, hc);
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const bool heavy = x[4 * j + c] > pivot && e0 + c < m;
-                    hm |= (heavy ? 1u : 0u) << c;
-                    hc += heavy ? 1u : 0u;
+                for (int h = 0; h < 2; ++h) {  // crossings at the pair's first element, at both
+                    uint32_t nt = 0;
+                    na[h] = 0;
+                    if (kc) {
+                        const uint32_t hw = s_hist[pos(j, h) >> 1];
+                        na[h] = hw & 0xffffu;
+                        nt = na[h] + (hw >> 16);
+                    }
+                    // heavies (<= 128 a half) in the low 12 bits, crossings
+                    // (<= kKeyCache) above
+                    v[h] = hc[h] | (nt << 12);
                 }
-                uint32_t n0 = 0, n1 = 0, n2 = 0, nt = 0;  // crossings at e0 .. e0 + 2, at all four
-                if (kc) {
-                    const uint2 hw = *reinterpret_cast<const uint2*>(&s_hist[e0 >> 1]);
-                    n0 = hw.x & 0xffffu;
-                    n1 = hw.x >> 16;
-                    n2 = hw.y & 0xffffu;
-                    nt = n0 + n1 + n2 + (hw.y >> 16);
-                }
-                // one scan for both counts: heavies (<= 256 a wave row) in the low
-                // 12 bits, crossings (<= kKeyCache) above
-                const uint32_t v = hc | (nt << 12);
-                const uint32_t exv = wave_incl_scan_u32(v) - v;
-                uint32_t hr = hpre[j] + (exv & 0xfffu);
-                uint32_t cx = clo[j] + (exv >> 12);
-                const uint32_t nc[3] = {n0, n1, n2};
-                // both rows read and one selected, without branches: s_row[hr] for
-                // a light past the tile's last heavy (hr = ct) is read and unused
+                const uint32_t s0 = wave_incl_scan_u32(v[0]);
+                const uint32_t s1 = wave_incl_scan_u32(v[1]);
+                const uint32_t tot0 = (uint32_t)__builtin_amdgcn_readlane((int)s0, kWave - 1);
+                const uint32_t exv[2] = {s0 - v[0], tot0 + s1 - v[1]};
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const bool hv = (hm >> c) & 1u;
-                    const AliasEntry e = s_row[hr];
-                    const uint32_t ia = s_idx[cx];
-                    const float lp = light_p(x[4 * j + c]);
-                    const bool last = h0 + hr + 1u == H;  // the last heavy: (1, itself)
-                    const float p = hv ? (last ? 1.0f : e.p) : lp;
-                    const uint32_t a = hv ? (last ? base + e0 + (uint32_t)c : e.a) : ia;
-                    row[c] = make_uint2(__float_as_uint(p), a);
-                    hr += hv ? 1u : 0u;
-                    if (c < 3) cx += nc[c];
+                for (int h = 0; h < 2; ++h) {
+                    const uint32_t e0 = pos(j, h);
+                    uint32_t hr = hpre[j] + (exv[h] & 0xfffu);
+                    uint32_t cx = clo[j] + (exv[h] >> 12);
+                    uint2 r[2];
+                    // both rows read and one selected, without branches: s_row[hr]
+                    // for a light past the tile's last heavy (hr = ct) is read and unused
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        const bool hv = (hm[h] >> c) & 1u;
+                        const AliasEntry e = s_row[hr];
+                        const uint32_t ia = s_idx[cx];
+                        const float lp = light_p(x[4 * j + 2 * h + c]);
+                        const bool last = h0 + hr + 1u == H;  // the last heavy: (1, itself)
+                        const float p = hv ? (last ? 1.0f : e.p) : lp;
+                        const uint32_t a = hv ? (last ? base + e0 + (uint32_t)c : e.a) : ia;
+                        r[c] = make_uint2(__float_as_uint(p), a);
+                        hr += hv ? 1u : 0u;
+                        if (c == 0) cx += na[h];
+                    }
+                    store(e0, r[0], r[1], full);
                 }
-                store(j, full);
             }
         };
         if (full16)
@@ -782,58 +826,65 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
     const AliasEntry* hr_src = hrow + h0;
 #pragma unroll
     for (int j = 0; j < G::kG; ++j) {
-        const uint32_t e0 = G::kStride * j + 4u * (uint32_t)t;
-        uint32_t hm = 0, hc = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const bool heavy = x[4 * j + c] > pivot && e0 + c < m;
-            hm |= (heavy ? 1u : 0u) << c;
-            hc += heavy ? 1u : 0u;
-        }
-        uint32_t hr = hpre[j] + wave_incl_scan_u32(hc) - hc;
-        uint32_t cx[4] = {0u, 0u, 0u, 0u};
+        uint32_t hm[2], hc[2];
+        heavies(j, hm, hc);
+        const uint32_t v = hc[0] | (hc[1] << 16);  // heavies of a half: <= 128
+        const uint32_t sc = wave_incl_scan_u32(v);
+        const uint32_t tot0 = (uint32_t)__builtin_amdgcn_readlane((int)sc, kWave - 1) & 0xffffu;
+        const uint32_t hx[2] = {(sc - v) & 0xffffu, tot0 + ((sc - v) >> 16)};
         const uint32_t lo = clo[j], hi = chi[j];
+        uint32_t cxa[2], cxb[2];  // crossings before the pair's first, second element
         if (hi - lo <= kShortSlice) {  // wave-uniform
-            uint32_t below = 0, c0 = 0, c1 = 0, c2 = 0;
+            const uint32_t e0 = pos(j, 0), e1 = pos(j, 1);
+            uint32_t b0 = 0, q0n = 0, b1 = 0, q1n = 0;
             for (uint32_t k = lo; k < hi; ++k) {
-                const uint32_t v = cr[k];
-                below += v < e0 ? 1u : 0u;
-                c0 += v == e0 ? 1u : 0u;
-                c1 += v == e0 + 1u ? 1u : 0u;
-                c2 += v == e0 + 2u ? 1u : 0u;
+                const uint32_t cv = cr[k];
+                b0 += cv < e0 ? 1u : 0u;
+                q0n += cv == e0 ? 1u : 0u;
+                b1 += cv < e1 ? 1u : 0u;
+                q1n += cv == e1 ? 1u : 0u;
             }
-            cx[0] = lo + below;
-            cx[1] = cx[0] + c0;
-            cx[2] = cx[1] + c1;
-            cx[3] = cx[2] + c2;
+            cxa[0] = lo + b0;
+            cxb[0] = cxa[0] + q0n;
+            cxa[1] = lo + b1;
+            cxb[1] = cxa[1] + q1n;
         } else {
-            uint32_t a = lo, b = hi;  // first crossing at or past e0
-            while (a < b) {
-                const uint32_t mid = (a + b) >> 1;
-                if (cr[mid] < e0) a = mid + 1u;
-                else b = mid;
-            }
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                while (a < hi && cr[a] < e0 + (uint32_t)c) ++a;
-                cx[c] = a;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if ((hm >> c) & 1u) {
-                if (h0 + hr + 1u == H) {
-                    row[c] = make_uint2(__float_as_uint(1.0f), base + e0 + c);
-                } else {
-                    const AliasEntry e = hr_src[hr];
-                    row[c] = make_uint2(__float_as_uint(e.p), e.a);
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t e0 = pos(j, h);
+                uint32_t a = lo, b = hi;  // first crossing at or past e0
+                while (a < b) {
+                    const uint32_t mid = (a + b) >> 1;
+                    if (cr[mid] < e0) a = mid + 1u;
+                    else b = mid;
                 }
-                ++hr;
-            } else {  // a light (or past m: not stored)
-                row[c] = make_uint2(__float_as_uint(light_p(x[4 * j + c])), ix[cx[c]]);
+                cxa[h] = a;
+                while (a < hi && cr[a] < e0 + 1u) ++a;
+                cxb[h] = a;
             }
         }
-        store(j, Flag<false>{});
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t e0 = pos(j, h);
+            uint32_t hr = hpre[j] + hx[h];
+            uint2 r[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if ((hm[h] >> c) & 1u) {
+                    if (h0 + hr + 1u == H) {
+                        r[c] = make_uint2(__float_as_uint(1.0f), base + e0 + (uint32_t)c);
+                    } else {
+                        const AliasEntry e = hr_src[hr];
+                        r[c] = make_uint2(__float_as_uint(e.p), e.a);
+                    }
+                    ++hr;
+                } else {  // a light (or past m: not stored)
+                    r[c] = make_uint2(__float_as_uint(light_p(x[4 * j + 2 * h + c])),
+                                      ix[c == 0 ? cxa[h] : cxb[h]]);
+                }
+            }
+            store(e0, r[0], r[1], Flag<false>{});
+        }
     }
 }
 

@@ -131,3 +131,24 @@ def test_stream_zipf_2e26(wrs, oracle):
     tm = tm.numpy()
     assert np.array_equal(t["a"], tm["a"])
     np.testing.assert_allclose(t["p"], tm["p"], rtol=3e-7, atol=1e-7)
+
+
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_stream_unaligned_weights(wrs, oracle, shift):
+    """Weights that start 4, 8 or 12 bytes past a 16-byte boundary take the
+    per-element loads of the streamed passes: the same table as from an
+    aligned copy, and the oracle's."""
+    n = 300007
+    case("stream unaligned weights", shift, n)
+    w = oracle.gen_zipf(4, 1.1, n)
+    dw = wrs.to_device(np.concatenate([np.full(shift, 7.0, np.float32), w,
+                                       np.full(4, 9.0, np.float32)]))
+    plan = wrs.Plan(n, tune=dict(direct=STREAM))
+    t1 = wrs.empty(n, wrs.ALIAS_DTYPE)
+    plan.psa_build(dw.offset(shift), n, t1)
+    table = wrs.empty(n, wrs.ALIAS_DTYPE)
+    t, mean = build_mean(wrs, plan, wrs.to_device(w), n, table)
+    assert same(t1.numpy(), t)
+    p, a = oracle.psa_stream_table(w, mean)
+    assert np.array_equal(t["a"], a)
+    assert np.array_equal(t["p"].view(np.uint32), p.view(np.uint32))
