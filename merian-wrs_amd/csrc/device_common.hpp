@@ -145,6 +145,15 @@ __device__ __forceinline__ float4 ld_nt4(const float4* p) {
     const f32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x4_t*>(p));
     return make_float4(v.x, v.y, v.z, v.w);
 }
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_nt_u4(uint4* p, uint4 v) {
+    u32x4_t x;
+    x.x = v.x;
+    x.y = v.y;
+    x.z = v.z;
+    x.w = v.w;
+    __builtin_nontemporal_store(x, reinterpret_cast<u32x4_t*>(p));
+}
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 ld_nt2(const float2* p) {
     const f32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const f32x2_t*>(p));
