@@ -238,9 +238,12 @@ __device__ __attribute__((noinline)) uint64_t light_keys_round(const float* v, f
 // ---- pass 2a: tile records and heavy slots ---------------------------------
 // 256 threads, each owning 16 consecutive elements of the tile (transposed
 // through LDS, tileload.hpp): per element only the light key is summed; the
-// thread's heavies -- consecutive slots, few per thread -- are taken one set
-// bit at a time, stored to their slots and their keys summed as key / 2^30
-// (a heavy key is a multiple of 2^30, below 2^57).
+// thread's heavies -- few -- are taken one set bit at a time and their keys
+// summed as key / 2^30 (a heavy weight times 2^K is an integer multiple of
+// 2^30 below 2^86: its significand shifted left, exactly).  A wave with at
+// most 64 heavies lists their tile offsets in LDS in slot order, so that its
+// lanes store the slots with one coalesced store each after the barrier
+// (others store them per thread).
 constexpr int kPadAgg = kTile + kTile / 16;
 __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, uint32_t n,
                                                    const Params* __restrict__ prm,
@@ -249,12 +252,13 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
                                                    float* __restrict__ slot_w,
                                                    uint32_t* __restrict__ seg) {
     __shared__ float s_buf[kPadAgg];
+    __shared__ uint32_t s_stage[kWaves * kWave];  // per wave: tile offsets of its heavies
     __shared__ uint32_t s_c[kWaves];
     __shared__ uint64_t s_l[kWaves], s_hl[kWaves], s_hh[kWaves];
     if (prm->error) return;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
-    const int wid = t / kWave;
+    const int wid = __builtin_amdgcn_readfirstlane(t / kWave);
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * (uint32_t)kTile;
     const uint32_t m = min((uint32_t)kTile, n - base);
@@ -287,34 +291,51 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
     if (__any(emn < qkey_emin(K))) ls = light_keys_round(s_buf + 17 * t, pivot, K);
     const uint32_t c = (uint32_t)__popc(hm);
     const uint32_t ci = wave_incl_scan_u32(c);
-    if (lane == kWave - 1) s_c[wid] = ci;
-    __syncthreads();
-    uint32_t slot = ci - c, ctile = 0;
-#pragma unroll
-    for (int q = 0; q < kWaves; ++q) {
-        slot += q < wid ? s_c[q] : 0u;
-        ctile += s_c[q];
-    }
-    // heavies before each segment of 256 elements (16 threads)
-    if ((t & 15) == 0) seg[(uint64_t)tile * kSegWords + (uint32_t)(t >> 4)] = slot;
+    const uint32_t cw = (uint32_t)__builtin_amdgcn_readlane((int)ci, kWave - 1);  // the wave's heavies
+    const bool listed = cw <= (uint32_t)kWave;  // wave-uniform
     uint64_t hs = 0;
-    for (uint32_t mk = hm; mk; mk &= mk - 1u) {
-        const int k = __builtin_ctz(mk);
-        const float x = s_buf[17 * t + k];
-        hs += qkey_round(x, K - 30);
-        slot_idx[base + slot] = base + e0 + (uint32_t)k;
-        slot_w[base + slot] = x;
-        ++slot;
+    {
+        uint32_t i = wid * kWave + ci - c;
+        for (uint32_t mk = hm; mk; mk &= mk - 1u) {
+            const int k = __builtin_ctz(mk);
+            const uint32_t b = __float_as_uint(s_buf[17 * t + k]);
+            hs += (uint64_t)((b & 0x7fffffu) | 0x800000u) << (((int)(b >> 23) - 180 + K) & 63);
+            if (listed) s_stage[i++] = e0 + (uint32_t)k;
+        }
     }
     const uint64_t lt = wave_incl_scan_u64(ls);  // < 64 * 16 * 2^54
     const uint64_t hl = wave_incl_scan_u64(hs & 0xffffffffull);
     const uint64_t hh = wave_incl_scan_u64(hs >> 32);
     if (lane == kWave - 1) {
+        s_c[wid] = ci;
         s_l[wid] = lt;
         s_hl[wid] = hl;
         s_hh[wid] = hh;
     }
     __syncthreads();
+    uint32_t slot0 = 0, ctile = 0;  // heavies of the tile before this wave, in all
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) {
+        slot0 += q < wid ? s_c[q] : 0u;
+        ctile += s_c[q];
+    }
+    // heavies before each segment of 256 elements (16 threads)
+    if ((t & 15) == 0) seg[(uint64_t)tile * kSegWords + (uint32_t)(t >> 4)] = slot0 + ci - c;
+    if (listed) {
+        if ((uint32_t)lane < cw) {
+            const uint32_t off = s_stage[wid * kWave + lane];
+            slot_idx[base + slot0 + lane] = base + off;
+            slot_w[base + slot0 + lane] = s_buf[off + (off >> 4)];
+        }
+    } else {
+        uint32_t slot = slot0 + ci - c;
+        for (uint32_t mk = hm; mk; mk &= mk - 1u) {
+            const int k = __builtin_ctz(mk);
+            slot_idx[base + slot] = base + e0 + (uint32_t)k;
+            slot_w[base + slot] = s_buf[17 * t + k];
+            ++slot;
+        }
+    }
     if (t == 0) {
         u128 L = 0, Hl = 0, Hh = 0;
 #pragma unroll
@@ -678,7 +699,9 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
         return;
     }
     const float pivot = prm->pivot;
-    const uint32_t q0 = qa[tile], q1 = qa[tile + 1u];
+    // qa is non-decreasing for valid weights (exact keys); a range that is
+    // not (NaN weights) is taken as empty rather than read out of bounds
+    const uint32_t q0 = qa[tile], q1 = max(qa[tile + 1u], q0);
     const uint64_t* ag = states + (uint64_t)tile * kCarryWords;
     const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
     const uint32_t h0 = (uint32_t)ex[4];
@@ -836,7 +859,7 @@ __global__ __launch_bounds__(B) void k_sp_table(const float* __restrict__ w, uin
         const uint32_t sc = wave_incl_scan_u32(v);
         const uint32_t tot0 = (uint32_t)__builtin_amdgcn_readlane((int)sc, kWave - 1) & 0xffffu;
         const uint32_t hx[2] = {(sc - v) & 0xffffu, tot0 + ((sc - v) >> 16)};
-        const uint32_t lo = clo[j], hi = chi[j];
+        const uint32_t lo = min(clo[j], kc), hi = max(min(chi[j], kc), lo);
         uint32_t cxa[2], cxb[2];  // crossings before the pair's first, second element
         if (hi - lo <= kShortSlice) {  // wave-uniform
             const uint32_t e0 = pos(j, 0), e1 = pos(j, 1);
