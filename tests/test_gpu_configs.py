@@ -301,7 +301,8 @@ def test_golden_tiny_cases_on_gpu(wrs, oracle):
     """The committed tiny fixtures (hand cases such as a single non-zero weight,
     all-equal weights, zero weights; seeded N <= 64) through the GPU build in
     both pack forms: heavy count, partition, alias indices exactly and p within
-    float rounding of the fixture's FP64 lower-bound table."""
+    float rounding of the fixture's FP64 lower-bound table; the streamed form
+    (exact keys) gives the same aliases on every fixture."""
     cases = json.load(open(os.path.join(GOLDEN, "psa_tiny.json")))
     for c in cases:
         w = np.array(c["weights"], np.float32)
@@ -309,7 +310,7 @@ def test_golden_tiny_cases_on_gpu(wrs, oracle):
         case("golden", c["label"], n)
         ref = c["f64"]
         dw = wrs.to_device(w)
-        for direct in (2, 1):
+        for direct in (2, 1, 6):
             plan = wrs.Plan(n, tune=dict(direct=direct))
             g = psa_debug(wrs, plan, dw, n)
             assert abs(g["mean"] - ref["mean"]) <= 1e-15 * ref["mean"], c["label"]

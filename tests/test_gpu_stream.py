@@ -11,6 +11,7 @@ pointer that is 8 but not 16-byte aligned."""
 import numpy as np
 import pytest
 
+from test_gpu_configs import implied_error
 from test_gpu_direct import KINDS, weights
 
 pytestmark = pytest.mark.gpu
@@ -152,3 +153,26 @@ def test_stream_unaligned_weights(wrs, oracle, shift):
     p, a = oracle.psa_stream_table(w, mean)
     assert np.array_equal(t["a"], a)
     assert np.array_equal(t["p"].view(np.uint32), p.view(np.uint32))
+
+
+def test_stream_spike_2e28(wrs, oracle):
+    """The 2^28 spike mixture (seed 4) in the streamed form, the tuner's choice
+    for it: bit-identical to the oracle's exact-key sweep, and an implied
+    distribution within 1e-6 relative of FP64."""
+    n = 1 << 28
+    case("stream c5", n)
+    w = wrs.empty(n, np.float32)
+    wrs.generate(wrs.GEN_SPIKE, n, 4, w)
+    wrs.synchronize()
+    wh = w.numpy()
+    plan = wrs.Plan(n, tune=dict(direct=STREAM))
+    table = wrs.empty(n, wrs.ALIAS_DTYPE)
+    t, mean = build_mean(wrs, plan, w, n, table)
+    del table, plan
+    p, a = oracle.psa_stream_table(wh, mean)
+    assert np.array_equal(t["a"], a)
+    assert np.array_equal(t["p"].view(np.uint32), p.view(np.uint32))
+    del p, a
+    mx_abs, mx_rel, zm = implied_error(t, wh)
+    print(f"  max_abs {mx_abs:.3e} max_rel {mx_rel:.3e}")
+    assert mx_rel <= 1e-6 and zm == 0.0

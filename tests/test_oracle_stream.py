@@ -15,7 +15,9 @@ It is checked against what does not share its arithmetic:
     Y(q) instead of the next heavy's Y(q+1) -- planted in a pure-Python exact
     sweep, each of which these checks reject.
 """
+import json
 import math
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -146,3 +148,19 @@ def test_planted_mistakes_are_caught(oracle, mistake):
     err = oracle.distribution_error(oracle.implied_distribution(p, a), w)
     caught.append(err["max_rel"] > 1e-6)
     assert any(caught), mistake
+
+
+def test_stream_table_matches_golden_fixtures(oracle):
+    """The committed tiny fixtures (hand cases and seeded N <= 64, FP64 tables
+    of the lower-bound sweep, tests/golden/psa_tiny.json): the exact-key sweep
+    gives the same aliases and p within float rounding."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "psa_tiny.json")
+    cases = json.load(open(path))
+    assert len(cases) >= 10
+    for c in cases:
+        w = np.array(c["weights"], np.float32)
+        ref = c["f64"]
+        p, a = oracle.psa_stream_table(w, ref["mean"])
+        assert a.tolist() == ref["a"], c["label"]
+        np.testing.assert_allclose(p.astype(np.float64), ref["p"], rtol=3e-7, atol=1e-7,
+                                   err_msg=c["label"])
