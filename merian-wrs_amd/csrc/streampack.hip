@@ -354,7 +354,13 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
     }
 }
 
-// ---- pass 3a: the heavies' keys in heavy order (one wave per tile, four a block)
+// ---- pass 3a: the heavies' keys in heavy order ------------------------------
+// One wave per two consecutive tiles (all waves fit on the device at once),
+// both tiles' first slots loaded up front.  A heavy weight times 2^K is its
+// significand shifted left, a multiple of 2^30 below 2^86 (the key k_sp_agg
+// sums): the wave scans key / 2^30, below 2^57, so that 64 of them fit one
+// 64-bit scan.
+constexpr uint32_t kHeavyTiles = 2;
 __global__ __launch_bounds__(kBlock) void k_sp_heavy(uint32_t n, const Params* __restrict__ prm,
                                                     const uint64_t* __restrict__ states,
                                                     const uint32_t* __restrict__ slot_idx,
@@ -366,61 +372,68 @@ __global__ __launch_bounds__(kBlock) void k_sp_heavy(uint32_t n, const Params* _
     if (H == 0 || H == n) return;  // identity table
     const int lane = lane_id();
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
-    const uint32_t tile = blockIdx.x * (uint32_t)kWaves + threadIdx.x / kWave;  // one wave each
-    if (tile >= tiles) return;  // the whole wave
-    const uint32_t base = tile * (uint32_t)kTile;
-    const uint64_t* ag = states + (uint64_t)tile * kCarryWords;
-    const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
-    // the first two chunks of slots are loaded with the records (the slot
-    // range [base, base + kTile) is allocated whatever the tile's count)
+    const uint32_t t0 = (blockIdx.x * (uint32_t)kWaves + threadIdx.x / kWave) * kHeavyTiles;
+    if (t0 >= tiles) return;  // the whole wave
+    // the first two chunks of each tile's slots (the slot range [base, base +
+    // kTile) is allocated whatever the tile's count)
     constexpr int kPre = 2;
-    float pv[kPre];
-    uint32_t pid[kPre];
+    float pv[kHeavyTiles][kPre];
+    uint32_t pid[kHeavyTiles][kPre];
 #pragma unroll
-    for (int r = 0; r < kPre; ++r) {
-        const uint32_t k = (uint32_t)(kWave * r + lane);
-        const bool in = base + k < n;
-        pv[r] = in ? slot_w[base + k] : 0.0f;
-        pid[r] = in ? slot_idx[base + k] : 0u;
+    for (uint32_t tt = 0; tt < kHeavyTiles; ++tt) {
+        const uint32_t base = (t0 + tt) * (uint32_t)kTile;
+#pragma unroll
+        for (int r = 0; r < kPre; ++r) {
+            const uint32_t k = (uint32_t)(kWave * r + lane);
+            const bool in = t0 + tt < tiles && base + k < n;
+            pv[tt][r] = in ? slot_w[base + k] : 0.0f;
+            pid[tt][r] = in ? slot_idx[base + k] : 0u;
+        }
     }
-    const uint32_t ct = (uint32_t)ag[4];
-    if (ct == 0) return;
-    const uint32_t h0 = (uint32_t)ex[4];
     const uint64_t mk = prm->mean_k;
     const int K = prm->key_k;
-    u128 run = u128_of(ex[0], ex[1]);  // HP(h0)
-    for (uint32_t r = 0; r < ct; r += kWave) {  // wave-uniform
-        const uint32_t k = r + (uint32_t)lane;
-        const bool act = k < ct;
-        float v;
-        uint32_t id;
-        if (r < (uint32_t)(kPre * kWave)) {
-            v = act ? (r == 0 ? pv[0] : pv[1]) : 0.0f;
-            id = act ? (r == 0 ? pid[0] : pid[1]) : 0u;
-        } else {
-            v = act ? slot_w[base + k] : 0.0f;
-            id = act ? slot_idx[base + k] : 0u;
+#pragma unroll
+    for (uint32_t tt = 0; tt < kHeavyTiles; ++tt) {
+        const uint32_t tile = t0 + tt;
+        if (tile >= tiles) break;  // the whole wave
+        const uint32_t base = tile * (uint32_t)kTile;
+        const uint64_t* ag = states + (uint64_t)tile * kCarryWords;
+        const uint64_t* ex = states + (uint64_t)kExclBase * tiles + (uint64_t)tile * kCarryWords;
+        const uint32_t ct = (uint32_t)ag[4];
+        const uint32_t h0 = (uint32_t)ex[4];
+        u128 run = u128_of(ex[0], ex[1]);  // HP(h0)
+        for (uint32_t r = 0; r < ct; r += kWave) {  // wave-uniform
+            const uint32_t k = r + (uint32_t)lane;
+            const bool act = k < ct;
+            float v;
+            uint32_t id;
+            if (r < (uint32_t)(kPre * kWave)) {
+                v = act ? (r == 0 ? pv[tt][0] : pv[tt][1]) : 0.0f;
+                id = act ? (r == 0 ? pid[tt][0] : pid[tt][1]) : 0u;
+            } else {
+                v = act ? slot_w[base + k] : 0.0f;
+                id = act ? slot_idx[base + k] : 0u;
+            }
+            const uint32_t b = __float_as_uint(v);
+            const uint64_t q = act ? (uint64_t)((b & 0x7fffffu) | 0x800000u) << (((int)(b >> 23) - 180 + K) & 63)
+                                   : 0ull;  // key / 2^30
+            const uint64_t sc = wave_incl_scan_u64(q);  // < 64 * 2^57
+            if (act) {
+                const uint32_t qi = h0 + k;
+                hkey[qi] = run + ((u128)sc << 30) - (u128)(qi + 1u) * (u128)mk;  // Y(qi + 1)
+                hidx[qi] = id;
+            }
+            run += (u128)__shfl(sc, kWave - 1, kWave) << 30;
         }
-        // a heavy key is a multiple of 2^30 (w >= mean): key / 2^30 < 2^57, as k_sp_agg sums it
-        const u128 q = act ? (u128)qkey_round(v, K - 30) << 30 : (u128)0;  // three limbs
-        const uint64_t s0 = wave_incl_scan_u64((uint64_t)(uint32_t)q);
-        const uint64_t s1 = wave_incl_scan_u64((uint64_t)(uint32_t)(q >> 32));
-        const uint64_t s2 = wave_incl_scan_u64((uint64_t)(q >> 64));
-        const u128 inc = (u128)s0 + ((u128)s1 << 32) + ((u128)s2 << 64);
-        if (act) {
-            const uint32_t qi = h0 + k;
-            hkey[qi] = run + inc - (u128)(qi + 1u) * (u128)mk;  // Y(qi + 1)
-            hidx[qi] = id;
-        }
-        const uint64_t t0 = __shfl(s0, kWave - 1, kWave), t1 = __shfl(s1, kWave - 1, kWave),
-                       t2 = __shfl(s2, kWave - 1, kWave);
-        run += (u128)t0 + ((u128)t1 << 32) + ((u128)t2 << 64);
     }
 }
 
 // ---- pass 3b: the alias range of every tile ----------------------------------
-// One wave per tile: qa[t] = first q in [0, H-1) with Y(q+1) > X(r0(t)) (H-1
-// if none) by a 64-ary search over hkey; qa[tiles] = H-1.
+// qa[t] = first q in [0, H-1) with Y(q+1) > X(r0(t)) (H-1 if none) by a 64-ary
+// search over hkey; qa[tiles] = H-1.  Each wave runs the searches of two
+// ranges, t and t + half, side by side (their probes in flight together), so
+// that all waves fit on the device at once.
+constexpr uint32_t kRangesPerWave = 2;
 __global__ __launch_bounds__(kBlock) void k_sp_qrange(uint32_t n, const Params* __restrict__ prm,
                                                       const uint64_t* __restrict__ states,
                                                       const u128* __restrict__ hkey,
@@ -428,32 +441,32 @@ __global__ __launch_bounds__(kBlock) void k_sp_qrange(uint32_t n, const Params* 
     if (prm->error) return;
     const int lane = lane_id();
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
-    const uint32_t t = blockIdx.x * (uint32_t)kWaves + threadIdx.x / kWave;
-    if (t > tiles) return;  // the whole wave
+    const uint32_t nr = tiles + 1u;  // ranges: qa[0 .. tiles]
+    const uint32_t half = (nr + 1u) / 2u;
+    const uint32_t wv = blockIdx.x * (uint32_t)kWaves + threadIdx.x / kWave;
+    if (wv >= half) return;  // the whole wave
+    const uint32_t tA = wv, tB = wv + half;  // tB < nr or absent
     const uint32_t H = prm->heavy_count;
     if (H == 0 || H == n) {
-        if (lane == 0) qa[t] = 0;
+        if (lane == 0) {
+            qa[tA] = 0;
+            if (tB < nr) qa[tB] = 0;
+        }
         return;
     }
     const uint32_t hq = H - 1u;
-    if (t == tiles) {
-        if (lane == 0) qa[t] = hq;
-        return;
-    }
-    // the first round's probes do not depend on the tile: loaded with its record
+    // the first round's probes do not depend on the tile: loaded with the records
     const uint32_t step0 = (hq + (uint32_t)kWave - 1u) / (uint32_t)kWave;
     const bool in0 = (uint64_t)lane * step0 < hq;
     const u128 y0 = in0 ? hkey[(uint32_t)lane * step0] : (u128)0;
-    const u128 x0 = tile_key(states, n, tiles, t, prm->mean_k);
-    uint32_t lo = 0, hi = hq;  // the answer lies in [lo, hi]
+    const uint64_t mk = prm->mean_k;
+    // qa[tiles] = H - 1 (an empty search); tB == nr: no second range
+    const bool sa = tA < tiles, sb = tB < tiles;
+    const u128 xA = sa ? tile_key(states, n, tiles, tA, mk) : (u128)0;
+    const u128 xB = sb ? tile_key(states, n, tiles, tB, mk) : (u128)0;
+    uint32_t loA = sa ? 0u : hq, hiA = hq, loB = sb ? 0u : hq, hiB = hq;  // answers in [lo, hi]
     bool first = true;
-    while (lo < hi) {          // wave-uniform
-        const uint32_t R = hi - lo;
-        const uint32_t step = (R + (uint32_t)kWave - 1u) / (uint32_t)kWave;
-        const uint32_t p = lo + (uint32_t)lane * step;
-        const bool in = (uint64_t)lane * step < R;
-        const bool pr = in && (first ? y0 : hkey[p]) > x0;
-        first = false;
+    auto narrow = [&](uint32_t& lo, uint32_t& hi, uint32_t step, bool pr, bool in) {
         const uint64_t b = __ballot(pr);
         const uint64_t bi = __ballot(in);
         if (b) {
@@ -465,8 +478,28 @@ __global__ __launch_bounds__(kBlock) void k_sp_qrange(uint32_t n, const Params* 
             const int last = 63 - __builtin_clzll(bi);
             lo = lo + (uint32_t)last * step + 1u;
         }
+    };
+    while (loA < hiA || loB < hiB) {  // wave-uniform
+        const bool ga = loA < hiA, gb = loB < hiB;
+        const uint32_t Ra = ga ? hiA - loA : 0u, Rb = gb ? hiB - loB : 0u;
+        const uint32_t stA = (Ra + (uint32_t)kWave - 1u) / (uint32_t)kWave;
+        const uint32_t stB = (Rb + (uint32_t)kWave - 1u) / (uint32_t)kWave;
+        const uint32_t pa = loA + (uint32_t)lane * stA, pb = loB + (uint32_t)lane * stB;
+        const bool ina = (uint64_t)lane * stA < Ra, inb = (uint64_t)lane * stB < Rb;
+        // both ranges start at [0, H-1]: the first round's probes are y0 for both
+        u128 va = y0, vb = y0;
+        if (!first) {
+            va = ina ? hkey[pa] : (u128)0;
+            vb = inb ? hkey[pb] : (u128)0;
+        }
+        first = false;
+        if (ga) narrow(loA, hiA, stA, ina && va > xA, ina);
+        if (gb) narrow(loB, hiB, stB, inb && vb > xB, inb);
     }
-    if (lane == 0) qa[t] = lo;
+    if (lane == 0) {
+        qa[tA] = loA;
+        if (tB < nr) qa[tB] = loB;
+    }
 }
 
 // ---- pass 3c: rows of the heavies that cross among a tile's lights -----------
@@ -929,9 +962,11 @@ hipError_t launch_stream_pack(const float* w, uint64_t n, int prec, const Params
     const uint32_t nn = (uint32_t)n;
     const uint32_t tiles = (uint32_t)tiles_for(n);
     u128* hkey = static_cast<u128*>(b.hkey);
-    k_sp_heavy<<<(tiles + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, b.slot_idx,
-                                                                  b.slot_w, hkey, b.hidx);
-    k_sp_qrange<<<(tiles + 1u + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, hkey, b.qa);
+    const uint32_t hwaves = (tiles + kHeavyTiles - 1u) / kHeavyTiles;
+    k_sp_heavy<<<(hwaves + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, b.slot_idx,
+                                                                   b.slot_w, hkey, b.hidx);
+    const uint32_t qwaves = (tiles + 1u + kRangesPerWave - 1u) / kRangesPerWave;
+    k_sp_qrange<<<(qwaves + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, hkey, b.qa);
     if (ev_keys) (void)hipEventRecord(ev_keys, s);
     if (prec == kFP32) {
         k_sp_rows<true, kRowsBlock><<<tiles, kRowsBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx,
