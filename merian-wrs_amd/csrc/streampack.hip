@@ -40,9 +40,9 @@
 //                 their crossing r(q) among its lights
 //   k_sp_rows     per tile: the rows of the heavies whose crossing lies among
 //                 its lights, into hrow[q] (tiles without any skip their read)
-//   k_sp_table    per tile: every row of the tile -- lights from their keys,
-//                 heavies from hrow -- written in element order, as whole
-//                 128-byte lines
+//   k_sp_table    per tile: every row of the tile -- a light's alias from the
+//                 crossings before it, its p = w/mean; heavies from hrow --
+//                 written in element order, as whole 128-byte lines
 //
 // The light p is w/mean as in the merge pack; a heavy's p is its exact
 // residual rounded once to FP64 and divided by the mean (FP64 modes) or in
