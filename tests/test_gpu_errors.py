@@ -25,7 +25,7 @@ def case(*a):
 @pytest.mark.parametrize("hook,n", [(HOOK_SKIP_PARTIAL, (1 << 20) + 7),
                                     (HOOK_SKIP_SPINE, (1 << 23) + 5),
                                     (HOOK_STALE_TOTAL, (1 << 22) + 9)])
-@pytest.mark.parametrize("direct", [1, 2])
+@pytest.mark.parametrize("direct", [1, 2, 6])  # direct, classic and streamed forms
 def test_forced_wait_failure_is_reported_and_writes_nothing(wrs, oracle, hook, n, direct):
     case("forced wait failure", hook, n, direct)
     w = oracle.gen_lognormal(5, 0.0, 2.0, n)
