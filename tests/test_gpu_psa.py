@@ -244,7 +244,7 @@ def test_table_valid_in_every_mode(wrs, oracle, kind, precision):
         assert err["max_rel"] <= 1e-6
 
 
-@pytest.mark.parametrize("direct", [2, 1])
+@pytest.mark.parametrize("direct", [2, 1, 6])  # classic, direct and streamed forms
 def test_single_build_graph_replays_and_plan_reuse(wrs, oracle, direct):
     """A captured single build replays to the eager table, with eager builds of
     other sizes on the same plan between the replays: the reduce restores its
