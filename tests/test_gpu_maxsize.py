@@ -6,7 +6,9 @@ weights c and one weight 2c at the end.  The mean c (1 + 2^-32) rounds down
 to the pivot c, so the last element is the only heavy; every light bucket gets
 p = (float)(c / mean) = 1 and alias N - 1, and the last heavy gets (1, itself):
 every row is (1.0, N - 1).  Checked on slices at both ends and in the middle,
-in both pack forms, and by alias samples (p = 1 keeps every bucket); the ITS
+in the classic, direct and streamed pack forms (the streamed one with about
+120 GB more for its keys and rows, one heavy and no crossing tile), and by
+alias samples (p = 1 keeps every bucket); the ITS
 CDF of the same weights is (i + 1) / 2^32.  Before these tests the tile count
 in the emit and scan kernels ((n + 4095) / 4096 in 32 bits), the last block
 boundary (b * merge_tile) and the last merge block's end wrapped at
@@ -32,7 +34,7 @@ def slice_rows(wrs, table, start, count):
     return out
 
 
-@pytest.mark.parametrize("direct", [2, 1])
+@pytest.mark.parametrize("direct", [2, 1, 6])  # classic, direct, streamed (30 bytes a weight more)
 def test_largest_n_closed_form_table(wrs, direct):
     case("max N", N, "direct", direct)
     w = wrs.empty(N, np.float32)
