@@ -195,6 +195,7 @@ struct StreamBufs {
     uint32_t* qa = nullptr;
     uint32_t* seg = nullptr;   // kSegWords per tile: heavies before each 256-element
                                // segment, crossings before each segment boundary
+    void* hsmp = nullptr;      // 16-byte keys of heavies 0, 64, 128, ... (the range search's index)
 };
 constexpr uint32_t kSegWords = 64;  // [0, 16) heavies, [32, 49) crossings
 // the aggregate pass (after the reduce), then -- after launch_key_tilescan --

@@ -366,7 +366,8 @@ __global__ __launch_bounds__(kBlock) void k_sp_heavy(uint32_t n, const Params* _
                                                     const uint32_t* __restrict__ slot_idx,
                                                     const float* __restrict__ slot_w,
                                                     u128* __restrict__ hkey,
-                                                    uint32_t* __restrict__ hidx) {
+                                                    uint32_t* __restrict__ hidx,
+                                                    u128* __restrict__ hsmp) {
     if (prm->error) return;
     const uint32_t H = prm->heavy_count;
     if (H == 0 || H == n) return;  // identity table
@@ -420,8 +421,10 @@ __global__ __launch_bounds__(kBlock) void k_sp_heavy(uint32_t n, const Params* _
             const uint64_t sc = wave_incl_scan_u64(q);  // < 64 * 2^57
             if (act) {
                 const uint32_t qi = h0 + k;
-                hkey[qi] = run + ((u128)sc << 30) - (u128)(qi + 1u) * (u128)mk;  // Y(qi + 1)
+                const u128 y = run + ((u128)sc << 30) - (u128)(qi + 1u) * (u128)mk;  // Y(qi + 1)
+                hkey[qi] = y;
                 hidx[qi] = id;
+                if ((qi & 63u) == 0) hsmp[qi >> 6] = y;  // every 64th key: the range search's index
             }
             run += (u128)__shfl(sc, kWave - 1, kWave) << 30;
         }
@@ -429,44 +432,46 @@ __global__ __launch_bounds__(kBlock) void k_sp_heavy(uint32_t n, const Params* _
 }
 
 // ---- pass 3b: the alias range of every tile ----------------------------------
-// qa[t] = first q in [0, H-1) with Y(q+1) > X(r0(t)) (H-1 if none) by a 64-ary
-// search over hkey; qa[tiles] = H-1.  Each wave runs the searches of two
-// ranges, t and t + half, side by side (their probes in flight together), so
-// that all waves fit on the device at once.
-constexpr uint32_t kRangesPerWave = 2;
+// One wave per tile: qa[t] = first q in [0, H-1) with Y(q+1) > X(r0(t)) (H-1
+// if none); qa[tiles] = H-1.  A 64-ary search over every 64th key (hsmp, a
+// small array the caches hold; its first round's probes are the same for
+// every tile and load with the tile's record) finds the 64 keys that hold the
+// answer, and one load of them (consecutive) and a ballot finish it.
 __global__ __launch_bounds__(kBlock) void k_sp_qrange(uint32_t n, const Params* __restrict__ prm,
                                                       const uint64_t* __restrict__ states,
                                                       const u128* __restrict__ hkey,
+                                                      const u128* __restrict__ hsmp,
                                                       uint32_t* __restrict__ qa) {
     if (prm->error) return;
     const int lane = lane_id();
     const uint32_t tiles = (uint32_t)(((uint64_t)n + kTile - 1) / kTile);
-    const uint32_t nr = tiles + 1u;  // ranges: qa[0 .. tiles]
-    const uint32_t half = (nr + 1u) / 2u;
-    const uint32_t wv = blockIdx.x * (uint32_t)kWaves + threadIdx.x / kWave;
-    if (wv >= half) return;  // the whole wave
-    const uint32_t tA = wv, tB = wv + half;  // tB < nr or absent
+    const uint32_t t = blockIdx.x * (uint32_t)kWaves + threadIdx.x / kWave;
+    if (t > tiles) return;  // the whole wave
     const uint32_t H = prm->heavy_count;
     if (H == 0 || H == n) {
-        if (lane == 0) {
-            qa[tA] = 0;
-            if (tB < nr) qa[tB] = 0;
-        }
+        if (lane == 0) qa[t] = 0;
         return;
     }
     const uint32_t hq = H - 1u;
-    // the first round's probes do not depend on the tile: loaded with the records
-    const uint32_t step0 = (hq + (uint32_t)kWave - 1u) / (uint32_t)kWave;
-    const bool in0 = (uint64_t)lane * step0 < hq;
-    const u128 y0 = in0 ? hkey[(uint32_t)lane * step0] : (u128)0;
-    const uint64_t mk = prm->mean_k;
-    // qa[tiles] = H - 1 (an empty search); tB == nr: no second range
-    const bool sa = tA < tiles, sb = tB < tiles;
-    const u128 xA = sa ? tile_key(states, n, tiles, tA, mk) : (u128)0;
-    const u128 xB = sb ? tile_key(states, n, tiles, tB, mk) : (u128)0;
-    uint32_t loA = sa ? 0u : hq, hiA = hq, loB = sb ? 0u : hq, hiB = hq;  // answers in [lo, hi]
+    if (t == tiles || hq == 0) {
+        if (lane == 0) qa[t] = hq;
+        return;
+    }
+    // samples S[i] = Y(64 i + 1), i < ns: i* = first i with S[i] > x (ns if none)
+    const uint32_t ns = (hq + 63u) / 64u;
+    const uint32_t step0 = (ns + (uint32_t)kWave - 1u) / (uint32_t)kWave;
+    const bool in0 = (uint64_t)lane * step0 < ns;
+    const u128 y0 = in0 ? hsmp[(uint32_t)lane * step0] : (u128)0;
+    const u128 x0 = tile_key(states, n, tiles, t, prm->mean_k);
+    uint32_t lo = 0, hi = ns;  // i* lies in [lo, hi]
     bool first = true;
-    auto narrow = [&](uint32_t& lo, uint32_t& hi, uint32_t step, bool pr, bool in) {
+    while (lo < hi) {          // wave-uniform
+        const uint32_t R = hi - lo;
+        const uint32_t step = (R + (uint32_t)kWave - 1u) / (uint32_t)kWave;
+        const uint32_t p = lo + (uint32_t)lane * step;
+        const bool in = (uint64_t)lane * step < R;
+        const bool pr = in && (first ? y0 : hsmp[p]) > x0;
+        first = false;
         const uint64_t b = __ballot(pr);
         const uint64_t bi = __ballot(in);
         if (b) {
@@ -478,28 +483,20 @@ __global__ __launch_bounds__(kBlock) void k_sp_qrange(uint32_t n, const Params* 
             const int last = 63 - __builtin_clzll(bi);
             lo = lo + (uint32_t)last * step + 1u;
         }
-    };
-    while (loA < hiA || loB < hiB) {  // wave-uniform
-        const bool ga = loA < hiA, gb = loB < hiB;
-        const uint32_t Ra = ga ? hiA - loA : 0u, Rb = gb ? hiB - loB : 0u;
-        const uint32_t stA = (Ra + (uint32_t)kWave - 1u) / (uint32_t)kWave;
-        const uint32_t stB = (Rb + (uint32_t)kWave - 1u) / (uint32_t)kWave;
-        const uint32_t pa = loA + (uint32_t)lane * stA, pb = loB + (uint32_t)lane * stB;
-        const bool ina = (uint64_t)lane * stA < Ra, inb = (uint64_t)lane * stB < Rb;
-        // both ranges start at [0, H-1]: the first round's probes are y0 for both
-        u128 va = y0, vb = y0;
-        if (!first) {
-            va = ina ? hkey[pa] : (u128)0;
-            vb = inb ? hkey[pb] : (u128)0;
-        }
-        first = false;
-        if (ga) narrow(loA, hiA, stA, ina && va > xA, ina);
-        if (gb) narrow(loB, hiB, stB, inb && vb > xB, inb);
     }
-    if (lane == 0) {
-        qa[tA] = loA;
-        if (tB < nr) qa[tB] = loB;
+    // S[i* - 1] <= x < S[i*]: the answer lies in [64 (i* - 1) + 1, min(64 i*, hq)];
+    // heavies q = b0 .. b0 + cnt - 1 are checked, min(64 i*, hq) if none passes
+    uint32_t ans = 0;
+    if (lo > 0) {
+        const uint32_t b0 = 64u * (lo - 1u) + 1u;
+        const uint32_t e = min(64u * lo, hq);  // the answer if no earlier key passes
+        const uint32_t cnt = e > b0 ? e - b0 : 0u;  // <= 63 keys before e
+        const bool in = (uint32_t)lane < cnt;
+        const bool pr = in && hkey[b0 + (uint32_t)lane] > x0;
+        const uint64_t b = __ballot(pr);
+        ans = b ? b0 + (uint32_t)__builtin_ctzll(b) : e;
     }
+    if (lane == 0) qa[t] = ans;
 }
 
 // ---- pass 3c: rows of the heavies that cross among a tile's lights -----------
@@ -962,11 +959,12 @@ hipError_t launch_stream_pack(const float* w, uint64_t n, int prec, const Params
     const uint32_t nn = (uint32_t)n;
     const uint32_t tiles = (uint32_t)tiles_for(n);
     u128* hkey = static_cast<u128*>(b.hkey);
+    u128* hsmp = static_cast<u128*>(b.hsmp);
     const uint32_t hwaves = (tiles + kHeavyTiles - 1u) / kHeavyTiles;
     k_sp_heavy<<<(hwaves + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, b.slot_idx,
-                                                                   b.slot_w, hkey, b.hidx);
-    const uint32_t qwaves = (tiles + 1u + kRangesPerWave - 1u) / kRangesPerWave;
-    k_sp_qrange<<<(qwaves + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, hkey, b.qa);
+                                                                   b.slot_w, hkey, b.hidx, hsmp);
+    k_sp_qrange<<<(tiles + 1u + kWaves - 1u) / kWaves, kBlock, 0, s>>>(nn, prm, states, hkey, hsmp,
+                                                                       b.qa);
     if (ev_keys) (void)hipEventRecord(ev_keys, s);
     if (prec == kFP32) {
         k_sp_rows<true, kRowsBlock><<<tiles, kRowsBlock, 0, s>>>(w, nn, prm, states, hkey, b.hidx,

@@ -92,8 +92,9 @@ void Workspace::ensure_stream() {
         scross = dev_alloc<uint16_t>(max_n);
         sqa = dev_alloc<uint32_t>(wrsk::tiles_for(max_n) + 2);
         sseg = dev_alloc<uint32_t>(wrsk::kSegWords * wrsk::tiles_for(max_n));
+        shsmp = dev_alloc<uint64_t>(2 * (max_n / 64 + 2));
     } catch (...) {
-        void* ptrs[] = {skey, shidx, shrow, scross, sqa, sseg};
+        void* ptrs[] = {skey, shidx, shrow, scross, sqa, sseg, shsmp};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         skey = nullptr;
@@ -102,6 +103,7 @@ void Workspace::ensure_stream() {
         scross = nullptr;
         sqa = nullptr;
         sseg = nullptr;
+        shsmp = nullptr;
         throw;
     }
 }
@@ -109,7 +111,7 @@ void Workspace::ensure_stream() {
 void Workspace::release() {
     void* ptrs[] = {params,    red,    states,   indices, pweights, prefix_hi, prefix_lo,
                     samp_h,    samp_l, bounds,   lightpos, splits,  skey,      shidx,
-                    shrow,     scross, sqa,      sseg};
+                    shrow,     scross, sqa,      sseg,     shsmp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     params = nullptr;
@@ -131,6 +133,7 @@ void Workspace::release() {
     scross = nullptr;
     sqa = nullptr;
     sseg = nullptr;
+    shsmp = nullptr;
 }
 
 static void require(bool cond, const char* msg) {
@@ -227,7 +230,7 @@ void PSA::run(hipStream_t s, const Buffers& b, uint64_t N, const wrs_psa_debug* 
     if (form == WRS_PACK_STREAM && !dbg) {
         // phases: tile aggregate, tile scan, heavy keys and ranges, heavy rows, table
         const wrsk::StreamBufs sb{ws.indices, ws.pweights, ws.skey, ws.shidx,
-                                  ws.shrow,    ws.scross,   ws.sqa, ws.sseg};
+                                  ws.shrow,    ws.scross,   ws.sqa, ws.sseg, ws.shsmp};
         check_hip(wrsk::launch_stream_agg(b.weights, N, ws.params, ws.states, sb, s),
                   "stream aggregate launch");
         if (ev) check_hip(hipEventRecord(ev[3], s), "event");

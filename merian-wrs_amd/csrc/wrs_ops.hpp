@@ -69,6 +69,7 @@ struct Workspace {
     uint16_t* scross = nullptr;
     uint32_t* sqa = nullptr;
     uint32_t* sseg = nullptr;
+    uint64_t* shsmp = nullptr;
 
     static Workspace allocate(uint64_t max_n, bool partition_buffers);
     // the full prefix arrays of debug builds and of the sample statistics,
