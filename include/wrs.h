@@ -110,7 +110,7 @@ typedef struct {
                                tie); its heavy probabilities round the exact residual
                                once.  The first build of such a plan allocates 30 bytes
                                per weight (keys, indices, rows and crossing offsets of
-                               up to max_n - 1 heavies) and 260 bytes per tile of 4096
+                               up to max_n - 1 heavies) and 1.3 KB per tile of 4096
                                weights; partition is ignored. */
 } wrs_tuning;
 #define WRS_PACK_DEFAULT 0u /* the direct form */
