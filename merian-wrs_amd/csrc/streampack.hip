@@ -33,7 +33,8 @@
 //   (tile scan)   k_part_tilescan<.., true> (partition.hip): the exclusive
 //                 prefix of every tile's record
 //   k_sp_heavy    per tile: Y(q+1) and the element index of each of its
-//                 heavies q, in heavy order (hkey, hidx)
+//                 heavies q, in heavy order (hkey, hidx; every 64th key also
+//                 in hsmp)
 //   k_sp_qrange   per tile t: qa[t] = #{q < H-1 : Y(q+1) <= X(r0(t))}, r0(t)
 //                 the lights before the tile -- the alias range of its lights
 //                 is [qa[t], qa[t+1]], and heavies q in [qa[t], qa[t+1]) have
@@ -359,7 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_sp_agg(const float* __restrict__ w, 
 // both tiles' first slots loaded up front.  A heavy weight times 2^K is its
 // significand shifted left, a multiple of 2^30 below 2^86 (the key k_sp_agg
 // sums): the wave scans key / 2^30, below 2^57, so that 64 of them fit one
-// 64-bit scan.
+// 64-bit scan.  Every 64th key is also written to hsmp, the first levels of
+// the range search.
 constexpr uint32_t kHeavyTiles = 2;
 __global__ __launch_bounds__(kBlock) void k_sp_heavy(uint32_t n, const Params* __restrict__ prm,
                                                     const uint64_t* __restrict__ states,
